@@ -1,0 +1,123 @@
+/*
+ * phyflow_amd.h -- C ABI of libphyflow_amd, the MI355X (gfx950) vortex-blob
+ * velocity hot path of pHyFlow.
+ *
+ * Conventions
+ *   - Every function returns PHF_OK (0) or a negative PHF_ERR_* code; the
+ *     message is available from phf_last_error() on the calling thread.
+ *   - Array arguments are plain contiguous float64 host arrays unless the
+ *     name says otherwise; the library never keeps a caller pointer after
+ *     the call returns.  Outputs are written, never accumulated into.
+ *   - Calls are thread-safe: each call borrows its own HIP stream and
+ *     device workspace.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the pHyFlow source tree).
+ */
+#ifndef PHYFLOW_AMD_H
+#define PHYFLOW_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PHF_OK 0
+#define PHF_ERR_INVALID (-1)     /* bad argument: size, NULL, non-finite input */
+#define PHF_ERR_DEVICE (-2)      /* HIP runtime error or no GPU */
+#define PHF_ERR_NOMEM (-3)       /* device allocation failed or budget exceeded */
+#define PHF_ERR_UNSUPPORTED (-4) /* option combination not provided */
+#define PHF_ERR_BUSY (-5)        /* admission refused / queue full */
+#define PHF_ERR_NOTFOUND (-6)    /* unknown handle or request id */
+
+/* ---- precision of blob interactions ------------------------------------ */
+#define PHF_PREC_F64 0  /* float64 throughout */
+#define PHF_PREC_F32C 1 /* float32 pair arithmetic, compensated per-target sums */
+
+/* ---- timed stages (HIP events on the call's stream) --------------------- */
+enum {
+    PHF_STAGE_UPLOAD = 0, /* host -> device copies */
+    PHF_STAGE_TREE,       /* bbox, Morton keys, radix sort, leaf offsets, gathers */
+    PHF_STAGE_P2M,
+    PHF_STAGE_M2M,
+    PHF_STAGE_M2L,
+    PHF_STAGE_L2L,
+    PHF_STAGE_P2P,        /* near-field (or all-pairs) sum, L2P fused */
+    PHF_STAGE_DOWNLOAD,   /* device -> host copies */
+    PHF_STAGE_REMESH,
+    PHF_STAGE_PANEL,
+    PHF_STAGE_TOTAL,
+    PHF_STAGE_COUNT
+};
+
+/* ---- library ------------------------------------------------------------- */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *phf_last_error(void);
+
+/* Selects device 0 of this process and creates the stream pool.  Called
+ * implicitly by every entry point; explicit calls are idempotent. */
+int phf_init(void);
+
+/* Device name, compute-unit count and HBM bytes of the selected GPU. */
+int phf_device_info(char *name, int64_t name_len, int32_t *num_cu,
+                    int64_t *hbm_bytes);
+
+/* Turn per-stage HIP-event timing on (1) or off (0, default). */
+int phf_set_timing(int32_t enabled);
+
+/* Per-stage milliseconds of the last timed call on this thread
+ * (PHF_STAGE_COUNT doubles; -1 for a stage the call did not run). */
+int phf_last_stage_times(double *ms);
+
+/* ---- (a) direct Gaussian Biot-Savart ------------------------------------
+ * Replaces vv2parGauss_gpu(xB,yB,wB,xT,yT,ksigmasqr,blocksize)
+ *   src/cpp/blobs/cuda/vv2parGauss_gpu/vv2parGauss_py_gpu.cu:124-206
+ * and its CPU twin vv2parGauss_C_cpu(xB,yB,wB,xT,yT,ksigmasqr)
+ *   src/cpp/blobs/c/vv2parGauss_C/vv2parGauss_C_cpu.c:89-160.
+ * u_t = inv_2pi * sum_m -dy*s, v_t = inv_2pi * sum_m dx*s,
+ * s = w_m/r2*(1-exp(-r2/ksigmasqr)), r2 = dx^2+dy^2+1e-12.
+ * No padding of nb/nt is required (the reference GPU path needed it). */
+int phf_vv2par_gauss(const double *xb, const double *yb, const double *wb,
+                     int64_t nb, const double *xt, const double *yt,
+                     int64_t nt, double ksigmasqr, int32_t prec, double *vx,
+                     double *vy);
+
+/* ---- (b) 2-D FMM with the Oseen smoother --------------------------------
+ * Replaces fmm2d_gpu / fmm2d_cpu(xB,yB,wB,xT,yT,Ndirect,xopt,cutoff,tol)
+ *   src/cpp/blobs/c++/fmm2d_py_cpu_gpu/fmm2d_py.cpp:59-186 (GPU),
+ *   :189-316 (CPU) -> fmmInteract fmm.cpp:88-304, directInteract
+ *   direct.cpp:78.
+ * Returns (qr, qi) = -sum_j w_j K(|z-z_j|)(conj(z-z_j)) exactly as the
+ * reference (caller maps to velocity (-qi/2pi, -qr/2pi), _induced.py:334).
+ * tol == 0 selects the direct sum over all pairs (direct.cpp:1409-1450).
+ * opts may be NULL (defaults: order from tol by fmm.cpp:538-546, leaf level
+ * from Ndirect by fmm.cpp:553, float64). */
+typedef struct {
+    int32_t order; /* expansion order p (coefficients 0..p); <=0: from tol */
+    int32_t leaf;  /* target mean sources per leaf box; <=0: from Ndirect */
+    int32_t prec;  /* PHF_PREC_F64 or PHF_PREC_F32C */
+    int32_t ws;    /* well-separation distance in boxes (1 or 2); <=0: 1 */
+} phf_fmm_opts;
+
+int phf_fmm2d(const double *xb, const double *yb, const double *wb,
+              int64_t nb, const double *xt, const double *yt, int64_t nt,
+              int32_t ndirect, double xopt, double cutoff, double tol,
+              const phf_fmm_opts *opts, double *qr, double *qi);
+
+/* Debug/parity view of the Morton tree the FMM builds for nb points:
+ * leaf level L, Morton keys in input order, the stable sort permutation
+ * (sorted position -> input index) and leaf offsets start[0..4^L].
+ * keys/perm hold nb entries, start holds 4^L+1 (pass NULL to skip).
+ * x0,y0,inv_s receive the binning parameters (ix = floor((x-x0)*inv_s)). */
+int phf_tree_build(const double *x, const double *y, int64_t n, int32_t L,
+                   uint32_t *keys, int32_t *perm, int32_t *start, double *x0,
+                   double *y0, double *inv_s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYFLOW_AMD_H */

@@ -1,0 +1,126 @@
+"""ctypes binding of libphyflow_amd.so (the C ABI declared in
+include/phyflow_amd.h).
+
+The library is loaded lazily on first use.  If it is missing, or if the
+GPU is not usable, calls raise RuntimeError: there is no CPU fallback.
+ctypes.CDLL releases the GIL for the duration of each call.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphyflow_amd.so")
+
+PHF_OK = 0
+PREC_F64 = 0
+PREC_F32C = 1
+
+STAGES = ("upload", "tree", "p2m", "m2m", "m2l", "l2l", "p2p", "download",
+          "remesh", "panel", "total")
+
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_dbl = ctypes.c_double
+_ptr = ctypes.c_void_p
+
+_lock = threading.Lock()
+_lib = None
+
+
+class PhfError(RuntimeError):
+    """A libphyflow_amd call returned an error status."""
+
+    def __init__(self, code, msg):
+        super().__init__("libphyflow_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+class FmmOpts(ctypes.Structure):
+    _fields_ = [("order", _i32), ("leaf", _i32), ("prec", _i32), ("ws", _i32)]
+
+
+_SIGS = {
+    "phf_last_error": ([], ctypes.c_char_p),
+    "phf_init": ([], ctypes.c_int),
+    "phf_device_info": ([ctypes.c_char_p, _i64, ctypes.POINTER(_i32), ctypes.POINTER(_i64)], ctypes.c_int),
+    "phf_set_timing": ([_i32], ctypes.c_int),
+    "phf_last_stage_times": ([_ptr], ctypes.c_int),
+    "phf_vv2par_gauss": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr, _ptr], ctypes.c_int),
+    "phf_fmm2d": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _i32, _dbl, _dbl, _dbl,
+                   ctypes.POINTER(FmmOpts), _ptr, _ptr], ctypes.c_int),
+    "phf_tree_build": ([_ptr, _ptr, _i64, _i32, _ptr, _ptr, _ptr, ctypes.POINTER(_dbl),
+                        ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)], ctypes.c_int),
+}
+
+
+def exported_symbols():
+    """Names of the C entry points this binding declares."""
+    return sorted(_SIGS)
+
+
+def lib():
+    """The loaded library (raises RuntimeError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    "libphyflow_amd.so is not built (expected at %s); run "
+                    "__graft_entry__.build() or python phyflow2.0_amd/build.py" % LIB_PATH)
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (args, res) in _SIGS.items():
+                fn = getattr(L, name, None)
+                if fn is None:
+                    continue
+                fn.argtypes = args
+                fn.restype = res
+            _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != PHF_OK:
+        msg = lib().phf_last_error()
+        raise PhfError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))
+
+
+# ---------------------------------------------------------------- helpers --
+def f64(a, name="array"):
+    """Contiguous float64 1-D view/copy of a (the reference reads raw
+    float64 buffers without checking layout; this shim enforces it)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 1:
+        a = a.ravel()
+    return a
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else ctypes.c_void_p(0)
+
+
+def device_info():
+    name = ctypes.create_string_buffer(256)
+    cu = _i32()
+    hbm = _i64()
+    call("phf_device_info", name, 256, ctypes.byref(cu), ctypes.byref(hbm))
+    return {"name": name.value.decode(), "compute_units": cu.value, "hbm_bytes": hbm.value}
+
+
+def set_timing(on):
+    call("phf_set_timing", 1 if on else 0)
+
+
+def last_stage_times():
+    ms = np.empty(len(STAGES))
+    call("phf_last_stage_times", ptr(ms))
+    return {k: float(v) for k, v in zip(STAGES, ms) if v >= 0}

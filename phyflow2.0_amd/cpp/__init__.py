@@ -1,0 +1,1 @@
+"""Mirror of the reference package level of the same name."""

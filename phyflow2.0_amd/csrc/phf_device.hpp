@@ -1,0 +1,77 @@
+// phf_device.hpp -- device-side helpers shared by the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace phf {
+
+constexpr int kWave = 64;
+
+// Pair-kernel selector.
+enum PairKind : int {
+    PAIR_GAUSS_EPS = 0,  // vv2parGauss: r2 += 1e-12, s = w/r2*(1-exp(-r2/ks))
+    PAIR_OSEEN = 1,      // fmm2d: P = w/r2 * (r2 < c2 ? -expm1(-shape r2)*scale : 1)
+};
+
+// Parameters of the pair kernel, prepared on the host in float64.
+struct PairParams {
+    double eps;       // added to r2 (Gaussian kernel only)
+    double inv_ks;    // 1/(k sigma^2) == shape of the Oseen kernel
+    double c2;        // cutoff^2 (Oseen)
+    double scale;     // -1/expm1(-shape c2) (Oseen)
+    float eps_f, nlog2e_ks_f, c2_f, scale_f;  // float32 forms
+};
+
+// Error-free transformation: (hi, lo) += x, Knuth's TwoSum.
+__device__ __forceinline__ void two_sum_add(float &hi, float &lo, float x)
+{
+    float s = hi + x;
+    float bp = s - hi;
+    float err = (hi - (s - bp)) + (x - bp);
+    hi = s;
+    lo += err;
+}
+
+// float64 reciprocal: hardware estimate refined by two Newton steps
+// (relative error below 2 ulp).
+__device__ __forceinline__ double rcp_f64(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    return y;
+}
+
+__device__ __forceinline__ float rcp_f32(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float exp2_f32(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// Interleave the low 16 bits of v onto the even bits.
+__host__ __device__ __forceinline__ uint32_t spread_bits(uint32_t v)
+{
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+__host__ __device__ __forceinline__ uint32_t compact_bits(uint32_t v)
+{
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+    v = (v | (v >> 4)) & 0x00FF00FFu;
+    v = (v | (v >> 8)) & 0x0000FFFFu;
+    return v;
+}
+
+__host__ __device__ __forceinline__ uint32_t morton2(uint32_t ix, uint32_t iy)
+{
+    return spread_bits(ix) | (spread_bits(iy) << 1);
+}
+
+}  // namespace phf

@@ -1,0 +1,392 @@
+// phf_direct.hip -- all-pairs vortex-blob velocity sums on gfx950.
+//
+// Replaces the reference's direct kernels:
+//   * vv2par_kernel (Gaussian, EPS-regularised),
+//     src/cpp/blobs/cuda/vv2parGauss_gpu/vv2parGauss_device.cu:41-162
+//   * direct_pot1_nomi_oseen_synchronized (fmm2d with tol == 0, Oseen),
+//     src/cpp/blobs/c++/fmm2d_py_cpu_gpu/cudaevalkernels.h:61-71, 451-508.
+//
+// Mapping to CDNA4: one wave64 owns TPL*64 targets held in VGPRs; all
+// lanes walk the same source sequence, so each source is fetched once per
+// wave with scalar (SMEM) loads and broadcast through SGPR operands: no LDS
+// staging and no barriers.  Long source lists are split over a second grid
+// dimension; each (target block, source chunk) writes a float64 partial and
+// a finishing pass adds the partials in chunk order, so results do not
+// depend on scheduling.
+//
+// Precision modes
+//   F64 : float64 pair arithmetic (rcp + 2 Newton steps for 1/r2).
+//   F32C: coordinates differenced in float64 and rounded once, pair
+//         arithmetic in float32 (v_exp_f32, v_rcp_f32), sums in blocks of
+//         16 terms folded into a TwoSum (hi, lo) accumulator per target.
+#include "phf_ops.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace phf {
+
+struct alignas(32) Src64 {
+    double x, y, w, pad;
+};
+struct alignas(8) Src32c {
+    double x, y;
+    float w, pad;
+};
+
+PairParams make_gauss_params(double ksigmasqr)
+{
+    PairParams p{};
+    p.eps = 1e-12;
+    p.inv_ks = 1.0 / ksigmasqr;
+    p.c2 = 0.0;
+    p.scale = 1.0;
+    p.eps_f = (float)p.eps;
+    p.nlog2e_ks_f = (float)(-1.4426950408889634074 / ksigmasqr);
+    p.c2_f = 0.0f;
+    p.scale_f = 1.0f;
+    return p;
+}
+
+PairParams make_oseen_params(double xopt, double cutoff)
+{
+    // fmm.cpp:1166-1179 (pot=1, OSEEN, INLINECOMPLEX_DIRECT, cont=true)
+    PairParams p{};
+    p.eps = 0.0;
+    p.c2 = cutoff * cutoff;
+    p.inv_ks = 1.2564312086261696770 / (xopt * xopt);
+    p.scale = -1.0 / std::expm1(-p.inv_ks * p.c2);
+    p.eps_f = 0.0f;
+    p.nlog2e_ks_f = (float)(-1.4426950408889634074 * p.inv_ks);
+    p.c2_f = (float)p.c2;
+    p.scale_f = (float)p.scale;
+    return p;
+}
+
+// ------------------------------------------------------------ packing ----
+__global__ void pack_src64(const double *__restrict__ x, const double *__restrict__ y,
+                           const double *__restrict__ w, int64_t n, Src64 *__restrict__ s)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) s[i] = Src64{x[i], y[i], w[i], 0.0};
+}
+
+__global__ void pack_src32c(const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ w, int64_t n, Src32c *__restrict__ s)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) s[i] = Src32c{x[i], y[i], (float)w[i], 0.0f};
+}
+
+// ------------------------------------------------------- pair kernels ----
+template <int KIND>
+__device__ __forceinline__ float pair_f32(float dx, float dy, float w, const PairParams &pp)
+{
+    float r2 = __builtin_fmaf(dx, dx, dy * dy);
+    if (KIND == PAIR_GAUSS_EPS) {
+        r2 += pp.eps_f;
+        float e = exp2_f32(r2 * pp.nlog2e_ks_f);
+        return (w * rcp_f32(r2)) * (1.0f - e);
+    } else {
+        float e = exp2_f32(r2 * pp.nlog2e_ks_f);
+        float om = r2 < pp.c2_f ? (1.0f - e) * pp.scale_f : 1.0f;
+        float P = (w * rcp_f32(r2)) * om;
+        return r2 > 0.0f ? P : 0.0f;
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ double pair_f64(double dx, double dy, double w, const PairParams &pp)
+{
+    double r2 = __builtin_fma(dx, dx, dy * dy);
+    if (KIND == PAIR_GAUSS_EPS) {
+        r2 += pp.eps;
+        double e = exp(-r2 * pp.inv_ks);
+        return (w * rcp_f64(r2)) * (1.0 - e);
+    } else {
+        double om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;
+        double P = (w * rcp_f64(r2)) * om;
+        return r2 > 0.0 ? P : 0.0;
+    }
+}
+
+// ------------------------------------------------------ F32C kernel ------
+template <int KIND, int TPL>
+__global__ __launch_bounds__(64) void direct_f32c(
+    const Src32c *__restrict__ src, int64_t nb, const double *__restrict__ xt,
+    const double *__restrict__ yt, int64_t nt, int64_t chunk_len, PairParams pp,
+    double2 *__restrict__ part)
+{
+    const int lane = threadIdx.x;
+    const int64_t tbase = (int64_t)blockIdx.x * (kWave * TPL);
+    double tx[TPL], ty[TPL];
+    float hu[TPL], lu[TPL], hv[TPL], lv[TPL];
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        int64_t t = tbase + lane + kWave * i;
+        tx[i] = t < nt ? xt[t] : 0.0;
+        ty[i] = t < nt ? yt[t] : 0.0;
+        hu[i] = lu[i] = hv[i] = lv[i] = 0.0f;
+    }
+    const int64_t j0 = (int64_t)blockIdx.y * chunk_len;
+    const int64_t j1 = min(nb, j0 + chunk_len);
+    int64_t j = j0;
+    for (; j + 16 <= j1; j += 16) {
+        float au[TPL], av[TPL];
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) au[i] = av[i] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const Src32c s = src[j + k];
+#pragma unroll
+            for (int i = 0; i < TPL; ++i) {
+                float dx = (float)(tx[i] - s.x);
+                float dy = (float)(ty[i] - s.y);
+                float P = pair_f32<KIND>(dx, dy, s.w, pp);
+                au[i] = __builtin_fmaf(P, dx, au[i]);
+                av[i] = __builtin_fmaf(P, dy, av[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) {
+            two_sum_add(hu[i], lu[i], au[i]);
+            two_sum_add(hv[i], lv[i], av[i]);
+        }
+    }
+    if (j < j1) {
+        float au[TPL], av[TPL];
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) au[i] = av[i] = 0.0f;
+        for (; j < j1; ++j) {
+            const Src32c s = src[j];
+#pragma unroll
+            for (int i = 0; i < TPL; ++i) {
+                float dx = (float)(tx[i] - s.x);
+                float dy = (float)(ty[i] - s.y);
+                float P = pair_f32<KIND>(dx, dy, s.w, pp);
+                au[i] = __builtin_fmaf(P, dx, au[i]);
+                av[i] = __builtin_fmaf(P, dy, av[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) {
+            two_sum_add(hu[i], lu[i], au[i]);
+            two_sum_add(hv[i], lv[i], av[i]);
+        }
+    }
+    double2 *out = part + (int64_t)blockIdx.y * nt;
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        int64_t t = tbase + lane + kWave * i;
+        if (t < nt)
+            out[t] = make_double2((double)hu[i] + (double)lu[i], (double)hv[i] + (double)lv[i]);
+    }
+}
+
+// ------------------------------------------------------- F64 kernel ------
+template <int KIND, int TPL>
+__global__ __launch_bounds__(64) void direct_f64(
+    const Src64 *__restrict__ src, int64_t nb, const double *__restrict__ xt,
+    const double *__restrict__ yt, int64_t nt, int64_t chunk_len, PairParams pp,
+    double2 *__restrict__ part)
+{
+    const int lane = threadIdx.x;
+    const int64_t tbase = (int64_t)blockIdx.x * (kWave * TPL);
+    double tx[TPL], ty[TPL], au[TPL], av[TPL];
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        int64_t t = tbase + lane + kWave * i;
+        tx[i] = t < nt ? xt[t] : 0.0;
+        ty[i] = t < nt ? yt[t] : 0.0;
+        au[i] = av[i] = 0.0;
+    }
+    const int64_t j0 = (int64_t)blockIdx.y * chunk_len;
+    const int64_t j1 = min(nb, j0 + chunk_len);
+    int64_t j = j0;
+    for (; j + 8 <= j1; j += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const Src64 s = src[j + k];
+#pragma unroll
+            for (int i = 0; i < TPL; ++i) {
+                double dx = tx[i] - s.x;
+                double dy = ty[i] - s.y;
+                double P = pair_f64<KIND>(dx, dy, s.w, pp);
+                au[i] = __builtin_fma(P, dx, au[i]);
+                av[i] = __builtin_fma(P, dy, av[i]);
+            }
+        }
+    }
+    for (; j < j1; ++j) {
+        const Src64 s = src[j];
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) {
+            double dx = tx[i] - s.x;
+            double dy = ty[i] - s.y;
+            double P = pair_f64<KIND>(dx, dy, s.w, pp);
+            au[i] = __builtin_fma(P, dx, au[i]);
+            av[i] = __builtin_fma(P, dy, av[i]);
+        }
+    }
+    double2 *out = part + (int64_t)blockIdx.y * nt;
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        int64_t t = tbase + lane + kWave * i;
+        if (t < nt) out[t] = make_double2(au[i], av[i]);
+    }
+}
+
+// Adds the chunk partials in chunk order and maps (A, B) = (sum P dx,
+// sum P dy) to the reference outputs:
+//   Gaussian: vx = inv_2pi * (-B), vy = inv_2pi * A  (vv2parGauss_C_cpu.c:150-151)
+//   Oseen:    qr = -A,  qi = B                       (direct.cpp:1482-1483)
+__global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int64_t nt,
+                              int kind, double *__restrict__ o0, double *__restrict__ o1)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double a = 0.0, b = 0.0;
+    for (int c = 0; c < nchunk; ++c) {
+        double2 p = part[(int64_t)c * nt + t];
+        a += p.x;
+        b += p.y;
+    }
+    if (kind == PAIR_GAUSS_EPS) {
+        const double inv_2pi = 0.5 / 3.141592653589793;
+        o0[t] = inv_2pi * (-b);
+        o1[t] = inv_2pi * a;
+    } else {
+        o0[t] = -a;
+        o1[t] = b;
+    }
+}
+
+// ------------------------------------------------------------- host ------
+int upload(Lane *L, const double *host, int64_t n, double **dev)
+{
+    double *d;
+    PHF_ALLOC(d, L->arena, double, (size_t)(n > 0 ? n : 1));
+    if (n > 0)
+        PHF_HIP(hipMemcpyAsync(d, host, sizeof(double) * n, hipMemcpyHostToDevice, L->stream));
+    *dev = d;
+    return PHF_OK;
+}
+
+int download(Lane *L, const double *dev, int64_t n, double *host)
+{
+    if (n > 0)
+        PHF_HIP(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost, L->stream));
+    return PHF_OK;
+}
+
+template <int KIND>
+static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *xb,
+                         const double *yb, const double *wb, int64_t nb, const double *xt,
+                         const double *yt, int64_t nt, double *o0, double *o1)
+{
+    constexpr int TPL32 = 2, TPL64 = 1;
+    const int tpl = prec == PHF_PREC_F32C ? TPL32 : TPL64;
+    const int64_t tblock = (int64_t)kWave * tpl;
+    const int64_t ngroups = (nt + tblock - 1) / tblock;
+    // Split sources so the grid has >= ~4 waves per SIMD of the chip, with
+    // chunks of at least 256 sources (a multiple of 16).
+    const int64_t want_waves = 4096;
+    int64_t nchunk = std::max<int64_t>(1, (want_waves + ngroups - 1) / ngroups);
+    nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, nb / 256));
+    nchunk = std::min<int64_t>(nchunk, 65535);
+    int64_t chunk_len = (nb + nchunk - 1) / nchunk;
+    chunk_len = (chunk_len + 15) / 16 * 16;
+    if (chunk_len == 0) chunk_len = 16;
+    nchunk = std::max<int64_t>(1, (nb + chunk_len - 1) / chunk_len);
+
+    double2 *part;
+    PHF_ALLOC(part, L->arena, double2, (size_t)nchunk * (size_t)nt);
+    dim3 grid((unsigned)ngroups, (unsigned)nchunk);
+    if (nb == 0) {
+        PHF_HIP(hipMemsetAsync(part, 0, sizeof(double2) * nt, L->stream));
+    } else if (prec == PHF_PREC_F32C) {
+        Src32c *s;
+        PHF_ALLOC(s, L->arena, Src32c, (size_t)nb);
+        pack_src32c<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
+        direct_f32c<KIND, TPL32><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
+                                                                 pp, part);
+    } else {
+        Src64 *s;
+        PHF_ALLOC(s, L->arena, Src64, (size_t)nb);
+        pack_src64<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
+        direct_f64<KIND, TPL64><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
+                                                                pp, part);
+    }
+    PHF_HIP(hipGetLastError());
+    direct_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(part, nb == 0 ? 1 : (int)nchunk, nt,
+                                                          KIND, o0, o1);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+int direct_device(Lane *L, int kind, int prec, const PairParams &pp, const double *xb,
+                  const double *yb, const double *wb, int64_t nb, const double *xt,
+                  const double *yt, int64_t nt, double *o0, double *o1)
+{
+    if (nt == 0) return PHF_OK;
+    if (kind == PAIR_GAUSS_EPS)
+        return launch_direct<PAIR_GAUSS_EPS>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
+    return launch_direct<PAIR_OSEEN>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
+}
+
+}  // namespace phf
+
+// ------------------------------------------------------------ C entry ----
+using namespace phf;
+
+extern "C" int phf_vv2par_gauss(const double *xb, const double *yb, const double *wb,
+                                int64_t nb, const double *xt, const double *yt, int64_t nt,
+                                double ksigmasqr, int32_t prec, double *vx, double *vy)
+{
+    clear_error();
+    PHF_REQUIRE(nb >= 0 && nt >= 0, "negative sizes (nb=%lld, nt=%lld)", (long long)nb,
+                (long long)nt);
+    PHF_REQUIRE(prec == PHF_PREC_F64 || prec == PHF_PREC_F32C, "unknown precision %d", prec);
+    PHF_REQUIRE(std::isfinite(ksigmasqr) && ksigmasqr > 0.0,
+                "ksigmasqr must be finite and > 0 (got %g)", ksigmasqr);
+    PHF_TRY(check_host_array(xb, nb, "xBlob"));
+    PHF_TRY(check_host_array(yb, nb, "yBlob"));
+    PHF_TRY(check_host_array(wb, nb, "wBlob"));
+    PHF_TRY(check_host_array(xt, nt, "xTarget"));
+    PHF_TRY(check_host_array(yt, nt, "yTarget"));
+    PHF_REQUIRE(nt == 0 || (vx && vy), "output arrays are NULL");
+    if (nt == 0) return PHF_OK;
+
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    StageTimer tm(L, timing_enabled());
+    bool used[PHF_STAGE_COUNT] = {};
+    used[PHF_STAGE_UPLOAD] = used[PHF_STAGE_P2P] = used[PHF_STAGE_DOWNLOAD] = true;
+    used[PHF_STAGE_TOTAL] = true;
+
+    tm.begin(PHF_STAGE_TOTAL);
+    tm.begin(PHF_STAGE_UPLOAD);
+    double *dxb, *dyb, *dwb, *dxt, *dyt, *o0, *o1;
+    PHF_TRY(upload(L, xb, nb, &dxb));
+    PHF_TRY(upload(L, yb, nb, &dyb));
+    PHF_TRY(upload(L, wb, nb, &dwb));
+    PHF_TRY(upload(L, xt, nt, &dxt));
+    PHF_TRY(upload(L, yt, nt, &dyt));
+    PHF_ALLOC(o0, L->arena, double, (size_t)nt);
+    PHF_ALLOC(o1, L->arena, double, (size_t)nt);
+    tm.end(PHF_STAGE_UPLOAD);
+    tm.begin(PHF_STAGE_P2P);
+    PairParams pp = make_gauss_params(ksigmasqr);
+    PHF_TRY(direct_device(L, PAIR_GAUSS_EPS, prec, pp, dxb, dyb, dwb, nb, dxt, dyt, nt, o0, o1));
+    tm.end(PHF_STAGE_P2P);
+    tm.begin(PHF_STAGE_DOWNLOAD);
+    PHF_TRY(download(L, o0, nt, vx));
+    PHF_TRY(download(L, o1, nt, vy));
+    tm.end(PHF_STAGE_DOWNLOAD);
+    tm.end(PHF_STAGE_TOTAL);
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    record_stage_times(L, used);
+    return PHF_OK;
+}

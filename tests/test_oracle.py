@@ -1,0 +1,264 @@
+"""Pins of the CPU oracle (oracle/phf_oracle.c) against the reference's own
+compiled code, closed forms, invariants and brute force.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from phyflow_amd import workloads as W
+
+
+def _rand_system(n, seed=0, h=None):
+    s = W.uniform_system(n, seed=seed, h=h)
+    return s["x"], s["y"], s["w"], s["sigma"]
+
+
+# ----------------------------------------------------------- A.2 Oseen ----
+def test_oseen_restatement_is_bitexact_vs_reference_direct(O):
+    if not O.ref_available():
+        pytest.skip("reference CPU FMM not built (oracle/_ref)")
+    x, y, w, sigma = _rand_system(3001, seed=3)  # odd size exercises the scalar tail too
+    xt, yt = np.random.default_rng(9).random((2, 777))
+    xopt, cutoff = W.oseen_params(sigma)
+    q = O.direct_oseen(x, y, w, xt, yt, xopt, cutoff)
+    r = O.ref_fmm2d(x, y, w, xt, yt, 35, xopt, cutoff, 0.0)
+    # the reference evaluates targets two at a time (SSE2) and the odd last
+    # target with scalar code of another association: allow 1 ulp there
+    np.testing.assert_array_equal(q[0][:-1], r[0][:-1])
+    np.testing.assert_array_equal(q[1][:-1], r[1][:-1])
+    assert O.rel_l2((q[0][-1:], q[1][-1:]), (r[0][-1:], r[1][-1:])) < 1e-15
+
+
+def test_oseen_far_pairs_are_point_vortex(O):
+    # beyond the cutoff the Oseen kernel is exactly 1/r^2 (fmm.cpp:2946-2947)
+    xopt, cutoff = 0.01, 0.05
+    qr, qi = O.direct_oseen([0.0], [0.0], [2.0], [0.3], [0.4], xopt, cutoff)
+    z = complex(0.3, 0.4)
+    assert qr[0] + 1j * qi[0] == pytest.approx(-2.0 / z, rel=1e-15)
+
+
+def test_oseen_self_term_zero(O):
+    qr, qi = O.direct_oseen([0.1], [0.2], [1.0], [0.1], [0.2], 0.01, 0.05)
+    assert qr[0] == 0.0 and qi[0] == 0.0
+
+
+# ------------------------------------------------------- A.1 Gaussian -----
+def test_gauss_single_blob_closed_form(O):
+    ks = 2 * 0.03 ** 2
+    r = np.array([1e-4, 0.01, 0.03, 0.07, 0.2, 1.3])
+    th = np.linspace(0.3, 5.0, r.size)
+    xt, yt = r * np.cos(th), r * np.sin(th)
+    vx, vy = O.vv2par_gauss([0.0], [0.0], [0.7], xt, yt, ks)
+    eps = 1e-12
+    ut = 0.7 / (2 * math.pi * r) * (1 - np.exp(-(r * r + eps) / ks)) * r * r / (r * r + eps)
+    np.testing.assert_allclose(vx, -ut * np.sin(th), rtol=1e-13, atol=0)
+    np.testing.assert_allclose(vy, ut * np.cos(th), rtol=1e-13, atol=0)
+
+
+def test_gauss_matches_reference_oseen_up_to_eps(O):
+    # same physics as the reference's compiled direct Oseen sum; the two
+    # differ only by EPS = 1e-12 in r^2 and the continuity scale (2e-14)
+    if not O.ref_available():
+        pytest.skip("reference CPU FMM not built (oracle/_ref)")
+    x, y, w, sigma = _rand_system(2048, seed=1)
+    ks = 2 * sigma * sigma
+    xopt, cutoff = W.oseen_params(sigma)
+    vx, vy = O.vv2par_gauss(x, y, w, x, y, ks)
+    qr, qi = O.ref_fmm2d(x, y, w, x, y, 35, xopt, cutoff, 0.0)
+    err = O.rel_l2((vx, vy), (-qi / (2 * math.pi), -qr / (2 * math.pi)))
+    assert err < 1e-8
+
+
+def test_gauss_linearity_selfterm_antisymmetry(O):
+    x, y, w, sigma = _rand_system(300, seed=2)
+    ks = 2 * sigma * sigma
+    a = O.vv2par_gauss(x, y, w, x, y, ks)
+    b = O.vv2par_gauss(x, y, -3.0 * w, x, y, ks)
+    np.testing.assert_allclose(b[0], -3.0 * a[0], rtol=1e-13, atol=1e-300)
+    s = O.vv2par_gauss([0.5], [0.5], [1.0], [0.5], [0.5], ks)
+    assert s[0][0] == 0.0 and s[1][0] == 0.0
+    p = O.vv2par_gauss([0.1], [0.2], [1.0], [0.13], [0.21], ks)
+    q = O.vv2par_gauss([0.13], [0.21], [1.0], [0.1], [0.2], ks)
+    assert p[0][0] == pytest.approx(-q[0][0], rel=1e-15)
+    assert p[1][0] == pytest.approx(-q[1][0], rel=1e-15)
+
+
+def test_gauss_lamb_oseen_lattice_physics(O):
+    c = W.lamb_oseen_lattice()
+    ks = c["k"] * c["sigma"] ** 2
+    vx, vy = O.vv2par_gauss(c["x"], c["y"], c["w"], c["x"], c["y"], ks)
+    ex, ey = W.lamb_oseen_velocity(c["x"], c["y"])
+    r = np.hypot(c["x"], c["y"])
+    inner = r < 0.6  # away from the truncated lattice edge
+    assert O.rel_l2((vx[inner], vy[inner]), (ex[inner], ey[inner])) < 2e-2
+
+
+# -------------------------------------------------------------- A.4 M4' ---
+def test_m4_kernel_values_and_partition_of_unity(O):
+    assert O.m4kernel_1d([0.0])[0] == 1.0
+    np.testing.assert_array_equal(O.m4kernel_1d([1.0, -1.0, 2.0, -2.0, 2.5]), 0.0)
+    u = np.random.default_rng(0).random(1000)
+    tot = sum(O.m4kernel_1d(u - k) for k in range(-2, 3))
+    np.testing.assert_allclose(tot, 1.0, rtol=0, atol=2e-15)
+    # first and second moments vanish / match (third-order interpolant)
+    m1 = sum(O.m4kernel_1d(u - k) * (u - k) for k in range(-2, 3))
+    m2 = sum(O.m4kernel_1d(u - k) * (u - k) ** 2 for k in range(-2, 3))
+    np.testing.assert_allclose(m1, 0.0, atol=2e-15)
+    np.testing.assert_allclose(m2, 0.0, atol=2e-15)
+    # diffusion term: c^2 (2 - 9 u^2 + 6|u|^3) inside, (2-|u|)^2 (1-2|u|) outside
+    c = 0.3
+    v = np.array([0.25, 1.5])
+    ref = np.array([1 - 2.5 * 0.0625 + 1.5 * 0.015625 - c * c * (2 - 9 * 0.0625 + 6 * 0.015625),
+                    0.5 * 0.25 * (1 - 1.5) - c * c * 0.25 * (1 - 3.0)])
+    np.testing.assert_allclose(O.m4kernel_1d(v, c), ref, rtol=1e-15)
+
+
+def test_regrid_conserves_moments(O):
+    rng = np.random.default_rng(4)
+    n = 5000
+    x, y = rng.uniform(-0.3, 0.4, (2, n))
+    w = rng.standard_normal(n)
+    sigma, overlap = 0.01, 1.0
+    h = sigma * overlap
+    xo, yo, wo, io, jo = O.regrid_m4(x, y, w, sigma, overlap, -1.5 * h, -1.5 * h)
+    for f, g in ((lambda a, b: 1.0, None), (lambda a, b: a, None), (lambda a, b: b, None),
+                 (lambda a, b: a * a + b * b, None)):
+        before = np.sum(w * f(x, y))
+        after = np.sum(wo * f(xo, yo))
+        assert after == pytest.approx(before, rel=1e-12, abs=1e-12 * np.sum(np.abs(w)))
+    # row-major (i, then j) order, unique cells
+    key = io * (jo.max() - jo.min() + 10) + (jo - jo.min())
+    assert np.all(np.diff(key) > 0)
+
+
+def test_regrid_stencil_covers_support_and_lattice_points_map_to_themselves(O):
+    rng = np.random.default_rng(5)
+    h, x0 = 0.02, -0.03
+    x = rng.uniform(-0.5, 0.5, 200)
+    y = rng.uniform(-0.5, 0.5, 200)
+    # lattice-aligned blobs: exactly on cell centres
+    i0 = np.array([3, -7, 11])
+    x = np.concatenate([x, x0 + (i0 + 0.5) * h])
+    y = np.concatenate([y, x0 + (i0[::-1] + 0.5) * h])
+    w = rng.standard_normal(x.size)
+    xo, yo, wo, io, jo = O.regrid_m4(x, y, w, h, 1.0, x0, x0)
+    cells = set(zip(io.tolist(), jo.tolist()))
+    for b in range(x.size):
+        tx = (x[b] - x0) / h - 0.5
+        ty = (y[b] - x0) / h - 0.5
+        need_i = [i for i in range(int(np.floor(tx)) - 3, int(np.ceil(tx)) + 4) if abs(tx - i) < 2]
+        need_j = [j for j in range(int(np.floor(ty)) - 3, int(np.ceil(ty)) + 4) if abs(ty - j) < 2]
+        for i in need_i:
+            for j in need_j:
+                assert (i, j) in cells
+    # a lone on-lattice blob keeps its circulation in its own cell
+    xs, ys, ws, iis, jjs = O.regrid_m4(x0 + np.array([4.5 * h]), x0 + np.array([-2.5 * h]),
+                                       np.array([0.75]), h, 1.0, x0, x0)
+    k = np.flatnonzero((iis == 4) & (jjs == -3))
+    assert k.size == 1 and ws[k[0]] == 0.75
+    assert np.sum(ws != 0.0) == 1 and ws.size == 16  # zero-weight cells are kept
+
+
+def test_regrid_empty(O):
+    xo, yo, wo, io, jo = O.regrid_m4(np.empty(0), np.empty(0), np.empty(0), 0.1, 1.0, 0, 0)
+    assert xo.size == 0
+
+
+# ---------------------------------------------------------- A.5 panels ----
+def _gauss_legendre_panel_velocity(xs, ys, xe, ye, gam, xt, yt, nq=4000):
+    # brute force: a constant-strength vortex sheet as point vortices.  The
+    # reference panels use Katz & Plotkin's sign (positive strength turns
+    # clockwise), the opposite of the blob kernels' counter-clockwise
+    # circulation, so each element carries circulation -gam * dl.
+    gam = -gam
+    t, wq = np.polynomial.legendre.leggauss(nq)
+    t = 0.5 * (t + 1)
+    wq = 0.5 * wq
+    L = math.hypot(xe - xs, ye - ys)
+    px = xs + t * (xe - xs)
+    py = ys + t * (ye - ys)
+    dx = xt - px
+    dy = yt - py
+    r2 = dx * dx + dy * dy
+    f = gam * L * wq / (2 * math.pi * r2)
+    return np.sum(-f * dy), np.sum(f * dx)
+
+
+def test_panel_induced_matches_point_vortex_quadrature(O):
+    xs, ys, xe, ye = np.array([0.1]), np.array([-0.2]), np.array([0.6]), np.array([0.15])
+    L = math.hypot(xe[0] - xs[0], ye[0] - ys[0])
+    cosA, sinA = (xe - xs) / L, (ye - ys) / L
+    for xt, yt in ((0.3, 0.5), (-0.4, 0.1), (1.2, -0.7), (0.35, 0.0)):
+        vx, vy = O.panel_induced([1.3], [xt], [yt], xs, ys, xe, ye, cosA, sinA)
+        qx, qy = _gauss_legendre_panel_velocity(xs[0], ys[0], xe[0], ye[0], 1.3, xt, yt)
+        assert vx[0] == pytest.approx(qx, rel=1e-9, abs=1e-12)
+        assert vy[0] == pytest.approx(qy, rel=1e-9, abs=1e-12)
+
+
+def test_panel_cylinder_closed_form_converges(O):
+    # examples/panels/ex03_convergenceRate_test.py:64-89: vx(0, 1.5) = U(1 + R^2/y^2)
+    errs = []
+    for n in (50, 100, 200, 400):
+        xp, yp = W.cylinder_panels(n, R=1.0)
+        g = W.panel_geometry(xp, yp)
+        A = O.panel_influence(g["xcp"], g["ycp"], g["xs"], g["ys"], g["xe"], g["ye"],
+                              g["cosA"], g["sinA"])
+        rhs = -(1.0 * g["cosA"] + 0.0 * g["sinA"])
+        # closed body: add the Kelvin condition (total circulation 0) by least
+        # squares as the reference's default 'prescribed' problem does
+        B = np.vstack([A, g["L"][None, :]])
+        s = np.linalg.lstsq(B, np.concatenate([rhs, [0.0]]), rcond=None)[0]
+        vx, vy = O.panel_induced(s, [0.0], [1.5], g["xs"], g["ys"], g["xe"], g["ye"], g["cosA"],
+                                 g["sinA"])
+        exact = 1.0 * (1 + 1.0 / 1.5 ** 2)
+        errs.append(abs(vx[0] + 1.0 - exact) / exact)
+    assert errs[-1] < 1e-3
+    assert errs[-1] < errs[0]
+
+
+# ------------------------------------------------------------------ LU ---
+def test_lu_matches_lapack(O):
+    rng = np.random.default_rng(6)
+    for n in (1, 7, 64, 200):
+        A = rng.standard_normal((n, n)) + n * np.eye(n) * 0.1
+        b = rng.standard_normal(n)
+        x = O.lu_solve(A, b)
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    with pytest.raises(np.linalg.LinAlgError):
+        O.lu_solve(np.zeros((3, 3)), np.ones(3))
+
+
+# --------------------------------------------------------------- tree -----
+def test_morton_binning_and_stable_sort(O):
+    rng = np.random.default_rng(7)
+    n, L = 20000, 6
+    x, y = rng.random((2, n))
+    x[:50] = x[50:100]  # duplicates exercise stability
+    y[:50] = y[50:100]
+    x0, y0 = x.min(), y.min()
+    D = max(x.max() - x0, y.max() - y0)
+    inv_s = 2.0 ** L / D
+    keys = O.morton_keys(x, y, x0, y0, inv_s, L)
+    sk, perm = O.stable_sort_keys(keys)
+    assert np.all(np.diff(sk.astype(np.int64)) >= 0)
+    assert np.array_equal(np.sort(perm), np.arange(n))
+    assert np.array_equal(sk, keys[perm])
+    same = np.flatnonzero(np.diff(sk.astype(np.int64)) == 0)
+    assert np.all(perm[same] < perm[same + 1])  # ties keep input order
+    # every point lies in the box its key names (brute-force decode)
+    def compact(v):
+        v = v & 0x55555555
+        v = (v | (v >> 1)) & 0x33333333
+        v = (v | (v >> 2)) & 0x0F0F0F0F
+        v = (v | (v >> 4)) & 0x00FF00FF
+        v = (v | (v >> 8)) & 0x0000FFFF
+        return v
+    k = keys.astype(np.int64)
+    ix, iy = compact(k), compact(k >> 1)
+    s = D / 2 ** L
+    assert np.all(x >= x0 + ix * s - 1e-12) and np.all(x <= x0 + (ix + 1) * s + 1e-12)
+    assert np.all(y >= y0 + iy * s - 1e-12) and np.all(y <= y0 + (iy + 1) * s + 1e-12)
+    start = O.leaf_offsets(sk, 4 ** L)
+    assert start[0] == 0 and start[-1] == n
+    cnt = np.bincount(sk.astype(np.int64), minlength=4 ** L)
+    assert np.array_equal(np.diff(start), cnt)
