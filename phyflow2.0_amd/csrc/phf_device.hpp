@@ -74,4 +74,42 @@ __host__ __device__ __forceinline__ uint32_t morton2(uint32_t ix, uint32_t iy)
     return spread_bits(ix) | (spread_bits(iy) << 1);
 }
 
+// ------------------------------------------------------------ pair ops --
+// P for one (target, source) pair; the caller accumulates P*dx and P*dy
+// with dx = x_target - x_source.
+//   Gaussian (vv2parGauss_C_cpu.c:144-146): P = w/(r2+eps) * (1 - exp(-(r2+eps)/ks))
+//   Oseen (direct.cpp:1428-1485):           P = w/r2 * (r2 < c2 ? -expm1(-r2/ks)*scale : 1),
+//                                            P = 0 for r2 == 0.
+// float32: r2 is clamped to >= 1e-30 so a coincident pair gives
+// 1 - exp2(-tiny) = 0 exactly and P = 0 without a select.
+template <int KIND>
+__device__ __forceinline__ float pair_f32(float dx, float dy, float w, const PairParams &pp)
+{
+    if (KIND == PAIR_GAUSS_EPS) {
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, pp.eps_f));
+        const float e = exp2_f32(r2 * pp.nlog2e_ks_f);
+        return (w * rcp_f32(r2)) * (1.0f - e);
+    } else {
+        const float r2 = __builtin_fmaxf(__builtin_fmaf(dx, dx, dy * dy), 1e-30f);
+        const float e = exp2_f32(r2 * pp.nlog2e_ks_f);
+        const float om = r2 < pp.c2_f ? __builtin_fmaf(-pp.scale_f, e, pp.scale_f) : 1.0f;
+        return (w * rcp_f32(r2)) * om;
+    }
+}
+
+template <int KIND>
+__device__ __forceinline__ double pair_f64(double dx, double dy, double w, const PairParams &pp)
+{
+    if (KIND == PAIR_GAUSS_EPS) {
+        const double r2 = __builtin_fma(dx, dx, dy * dy) + pp.eps;
+        const double e = exp(-r2 * pp.inv_ks);
+        return (w * rcp_f64(r2)) * (1.0 - e);
+    } else {
+        const double r2 = __builtin_fma(dx, dx, dy * dy);
+        const double om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;
+        const double P = (w * rcp_f64(r2)) * om;
+        return r2 > 0.0 ? P : 0.0;
+    }
+}
+
 }  // namespace phf

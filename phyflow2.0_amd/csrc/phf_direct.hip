@@ -78,38 +78,6 @@ __global__ void pack_src32c(const double *__restrict__ x, const double *__restri
     if (i < n) s[i] = Src32c{x[i], y[i], (float)w[i], 0.0f};
 }
 
-// ------------------------------------------------------- pair kernels ----
-template <int KIND>
-__device__ __forceinline__ float pair_f32(float dx, float dy, float w, const PairParams &pp)
-{
-    float r2 = __builtin_fmaf(dx, dx, dy * dy);
-    if (KIND == PAIR_GAUSS_EPS) {
-        r2 += pp.eps_f;
-        float e = exp2_f32(r2 * pp.nlog2e_ks_f);
-        return (w * rcp_f32(r2)) * (1.0f - e);
-    } else {
-        float e = exp2_f32(r2 * pp.nlog2e_ks_f);
-        float om = r2 < pp.c2_f ? (1.0f - e) * pp.scale_f : 1.0f;
-        float P = (w * rcp_f32(r2)) * om;
-        return r2 > 0.0f ? P : 0.0f;
-    }
-}
-
-template <int KIND>
-__device__ __forceinline__ double pair_f64(double dx, double dy, double w, const PairParams &pp)
-{
-    double r2 = __builtin_fma(dx, dx, dy * dy);
-    if (KIND == PAIR_GAUSS_EPS) {
-        r2 += pp.eps;
-        double e = exp(-r2 * pp.inv_ks);
-        return (w * rcp_f64(r2)) * (1.0 - e);
-    } else {
-        double om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;
-        double P = (w * rcp_f64(r2)) * om;
-        return r2 > 0.0 ? P : 0.0;
-    }
-}
-
 // ------------------------------------------------------ F32C kernel ------
 template <int KIND, int TPL>
 __global__ __launch_bounds__(64) void direct_f32c(

@@ -1,0 +1,911 @@
+// phf_fmm.hip -- 2-D FMM for the Oseen-smoothed Biot-Savart sum on gfx950.
+//
+// Replaces the reference's fmm2dInteract (fmm.cpp:88-304) and its CUDA
+// pipeline (cudaeval.cu, cudasort.cu).  Computes, for every target z,
+//     q(z) = qr + i qi = -sum_j w_j K(|z - z_j|^2) / (z - z_j)
+// with the Oseen factor K(r2) = -expm1(-shape r2) scale for r2 < cutoff^2 and
+// 1 beyond (fmm.cpp:1166-1179, cudaevalkernels.h:61-71), so the caller's
+// velocity is (-qi/2pi, -qr/2pi) exactly as for the reference.
+//
+// Tree: uniform quad-tree of depth L over the Morton-sorted points
+// (phf_tree.hip).  Far-field interactions use the standard 2-D complex
+// expansions about box centres, scaled by the box side r:
+//   multipole  A_k = sum_j w_j ((z_j - c)/r)^k                   (P2M)
+//   local      phi(z) = sum_m B_m ((z - d)/r)^m                  (L2P)
+//   M2M        A'_m = sum_{k<=m} C(m,k) 2^-k delta^(m-k) A_k      delta = (c1-c0)/R
+//   M2L        B_m = (1/r) (-1)^m u^(m+1) sum_k C(k+m,k) u^k A_k  u = r/(c-d)
+//   L2L        B'_m = sum_{j>=m} C(j,m) delta^(j-m) 2^-m B_j
+// Near set of a box: boxes within `ws` boxes in x and y, plus every box
+// whose gap is below the smoothing cutoff (the expansions represent the
+// unsmoothed 1/z kernel, which equals the Oseen kernel only beyond the
+// cutoff: the reference's THETACUTOFFCHECK, fmm.cpp:1812-1815).  The
+// interaction list of a box is the children of its parent's near set that
+// are not in its own near set.  Near pairs at the leaf level are summed
+// directly (P2P), fused with the L2P evaluation of the leaf's local
+// expansion.
+//
+// Precision: PHF_PREC_F64 runs every pass in float64.  PHF_PREC_F32C
+// stores points as float32 offsets from their leaf centre (formed in
+// float64; leaf sides and centre differences are exact in float32 because
+// the domain side carries 5 significant bits), runs P2P with compensated
+// per-target sums, P2M/M2L/L2P in float32, and M2M/L2L in float64 when the
+// order exceeds 16.
+#include "phf_fmm.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+namespace phf {
+
+template <class T> struct C2;
+template <> struct C2<float> { using type = float2; };
+template <> struct C2<double> { using type = double2; };
+
+template <class T> __device__ __forceinline__ typename C2<T>::type cmk(T r, T i)
+{
+    return typename C2<T>::type{r, i};
+}
+template <class V> __device__ __forceinline__ V cmul(V a, V b)
+{
+    return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// ------------------------------------------------------------ gathers ----
+__device__ __forceinline__ void leaf_center(uint32_t key, double x0, double y0, double s,
+                                            double &cx, double &cy)
+{
+    uint32_t ix = compact_bits(key), iy = compact_bits(key >> 1);
+    cx = x0 + ((double)ix + 0.5) * s;
+    cy = y0 + ((double)iy + 0.5) * s;
+}
+
+__global__ void gather_src_f32(const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ w, const uint32_t *__restrict__ skeys,
+                               const int32_t *__restrict__ perm, int64_t n, double x0, double y0,
+                               double s, float4 *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t j = perm[i];
+    double cx, cy;
+    leaf_center(skeys[i], x0, y0, s, cx, cy);
+    out[i] = make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
+}
+
+__global__ void gather_src_f64(const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ w, const int32_t *__restrict__ perm,
+                               int64_t n, double4 *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t j = perm[i];
+    out[i] = make_double4(x[j], y[j], w[j], 0.0);
+}
+
+__global__ void gather_tgt_f32(const double *__restrict__ x, const double *__restrict__ y,
+                               const uint32_t *__restrict__ skeys,
+                               const int32_t *__restrict__ perm, int64_t n, double x0, double y0,
+                               double s, float2 *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t j = perm[i];
+    double cx, cy;
+    leaf_center(skeys[i], x0, y0, s, cx, cy);
+    out[i] = make_float2((float)(x[j] - cx), (float)(y[j] - cy));
+}
+
+__global__ void gather_tgt_f64(const double *__restrict__ x, const double *__restrict__ y,
+                               const int32_t *__restrict__ perm, int64_t n,
+                               double2 *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t j = perm[i];
+    out[i] = make_double2(x[j], y[j]);
+}
+
+// box counts at level l from leaf offsets
+__device__ __forceinline__ int box_count(const int32_t *__restrict__ start, uint32_t b, int shift)
+{
+    return start[(int64_t)(b + 1) << shift] - start[(int64_t)b << shift];
+}
+
+// ---------------------------------------------------------------- P2M ----
+// One wave per leaf.  Lane j forms w_j z_j^k (k = 0..p) into an LDS table,
+// lane k then adds column k over the wave's points in index order.
+template <class T>
+__global__ __launch_bounds__(64) void p2m_kernel(const void *__restrict__ src_,
+                                                 const int32_t *__restrict__ start, int p1,
+                                                 double x0, double y0, double s, int Lvl,
+                                                 typename C2<T>::type *__restrict__ M)
+{
+    using V = typename C2<T>::type;
+    extern __shared__ unsigned char lds_raw[];
+    V *tab = reinterpret_cast<V *>(lds_raw);  // [64][p1]
+    const int lane = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int32_t s0 = start[b], s1 = start[b + 1];
+    V acc = cmk<T>(0, 0);
+    if (s1 > s0) {
+        double cx = 0, cy = 0;
+        if (sizeof(T) == 8) leaf_center(b, x0, y0, s, cx, cy);
+        const T inv_s = (T)(1.0 / s);
+        for (int32_t c0 = s0; c0 < s1; c0 += 64) {
+            const int32_t j = c0 + lane;
+            V z = cmk<T>(0, 0), g = cmk<T>(0, 0);
+            if (j < s1) {
+                if (sizeof(T) == 4) {
+                    float4 q = reinterpret_cast<const float4 *>(src_)[j];
+                    z = cmk<T>((T)q.x * inv_s, (T)q.y * inv_s);
+                    g = cmk<T>((T)q.z, 0);
+                } else {
+                    double4 q = reinterpret_cast<const double4 *>(src_)[j];
+                    z = cmk<T>((T)((q.x - cx) * (double)inv_s), (T)((q.y - cy) * (double)inv_s));
+                    g = cmk<T>((T)q.z, 0);
+                }
+            }
+            for (int k = 0; k < p1; ++k) {
+                tab[lane * p1 + k] = g;
+                g = cmul(g, z);
+            }
+            __syncthreads();
+            const int cnt = min(64, s1 - c0);
+            if (lane < p1)
+                for (int jj = 0; jj < cnt; ++jj) {
+                    V t = tab[jj * p1 + lane];
+                    acc.x += t.x;
+                    acc.y += t.y;
+                }
+            __syncthreads();
+        }
+    }
+    if (lane < p1) M[(int64_t)b * p1 + lane] = acc;
+}
+
+// ---------------------------------------------------------- M2M / L2L ----
+// tab[c][m][k] (c = child slot, complex, compute type TC).
+template <class T, class TC>
+__global__ void m2m_kernel(int p1, uint32_t nparent, const typename C2<T>::type *__restrict__ Mc,
+                           typename C2<T>::type *__restrict__ Mp,
+                           const typename C2<TC>::type *__restrict__ tab,
+                           const int32_t *__restrict__ start_s, int shift)
+{
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nparent * p1) return;
+    const uint32_t b = (uint32_t)(t / p1);
+    const int m = (int)(t % p1);
+    W acc = W{0, 0};
+    if (box_count(start_s, b, shift) > 0) {
+        for (int c = 0; c < 4; ++c) {
+            const V *ch = Mc + ((int64_t)(4 * b + c)) * p1;
+            const W *tr = tab + ((int64_t)c * p1 + m) * p1;
+            for (int k = 0; k <= m; ++k) {
+                V a = ch[k];
+                W tk = tr[k];
+                acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
+                acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
+            }
+        }
+    }
+    Mp[(int64_t)b * p1 + m] = V{(T)acc.x, (T)acc.y};
+}
+
+template <class T, class TC>
+__global__ void l2l_kernel(int p1, uint32_t nchild, const typename C2<T>::type *__restrict__ Bp,
+                           typename C2<T>::type *__restrict__ Bc,
+                           const typename C2<TC>::type *__restrict__ tab,
+                           const int32_t *__restrict__ start_t, int shift)
+{
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nchild * p1) return;
+    const uint32_t b = (uint32_t)(t / p1);
+    const int m = (int)(t % p1);
+    if (box_count(start_t, b, shift) == 0) return;
+    const uint32_t P = b >> 2;
+    const int c = b & 3;
+    const V *par = Bp + (int64_t)P * p1;
+    const W *tr = tab + ((int64_t)c * p1 + m) * p1;
+    W acc = W{0, 0};
+    for (int j = m; j < p1; ++j) {
+        V a = par[j];
+        W tk = tr[j];
+        acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
+        acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
+    }
+    V cur = Bc[(int64_t)b * p1 + m];
+    Bc[(int64_t)b * p1 + m] = V{(T)((TC)cur.x + acc.x), (T)((TC)cur.y + acc.y)};
+}
+
+// ---------------------------------------------------------------- M2L ----
+// One wave holds G = 64/p1 target boxes of one level; lane (g, m) owns
+// coefficient m of box g.  Per link the group forms A'_k = u^k A_k in LDS,
+// each lane contracts with its binomial row C(k+m, k) (registers), and
+// accumulates (-1)^m u^(m+1) S_m.  Boxes are visited parity-major so the
+// groups of a wave mostly share one interaction list.
+template <class T, int P1>
+__global__ __launch_bounds__(64) void m2l_kernel(
+    int l, int p1, const typename C2<T>::type *__restrict__ Ml, typename C2<T>::type *__restrict__ Bl,
+    const int32_t *__restrict__ start_s, const int32_t *__restrict__ start_t, int shift,
+    const int2 *__restrict__ il_tau, const int32_t *__restrict__ il_off /* 5 */, int il_max,
+    const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [p1][p1] */,
+    T inv_r)
+{
+    using V = typename C2<T>::type;
+    __shared__ V Ap[64];
+    const int lane = threadIdx.x;
+    const int G = 64 / p1;
+    const int g = lane / p1, m = lane - g * p1;
+    const int n = 1 << l, half = n >> 1;
+    const int64_t nbox = (int64_t)n * n;
+    const int64_t slot = (int64_t)blockIdx.x * G + g;
+    bool active = g < G && slot < nbox;
+    int bx = 0, by = 0, par = 0;
+    uint32_t box = 0;
+    if (active) {
+        const int64_t per = nbox / 4;  // boxes per parity class
+        par = (int)(slot / per);
+        const int64_t loc = slot - (int64_t)par * per;
+        bx = 2 * (int)(loc % half) + (par & 1);
+        by = 2 * (int)(loc / half) + (par >> 1);
+        box = morton2((uint32_t)bx, (uint32_t)by);
+        active = box_count(start_t, box, shift) > 0;
+    }
+    T crow[P1];
+#pragma unroll
+    for (int k = 0; k < P1; ++k) crow[k] = (k < p1 && m < p1) ? binom[m * p1 + k] : (T)0;
+    const int W2 = 2 * R + 1;
+    const int t0 = active ? il_off[par] : 0;
+    const int tn = active ? il_off[par + 1] - t0 : 0;
+    T ar = 0, ai = 0;
+    const T sgn = (m & 1) ? (T)-1 : (T)1;
+    for (int t = 0; t < il_max; ++t) {
+        bool ok = t < tn;
+        int2 o = make_int2(0, 0);
+        uint32_t sb = 0;
+        if (ok) {
+            o = il_tau[t0 + t];
+            const int sx = bx + o.x, sy = by + o.y;
+            ok = sx >= 0 && sy >= 0 && sx < n && sy < n;
+            if (ok) {
+                sb = morton2((uint32_t)sx, (uint32_t)sy);
+                ok = box_count(start_s, sb, shift) > 0;
+            }
+        }
+        const V *Uo = U + (int64_t)((o.y + R) * W2 + (o.x + R)) * (p1 + 1);
+        if (ok) {
+            V a = Ml[(int64_t)sb * p1 + m];
+            Ap[lane] = cmul(a, Uo[m]);
+        }
+        __syncthreads();
+        if (ok) {
+            T sr = 0, si = 0;
+            const V *ap = Ap + g * p1;
+#pragma unroll
+            for (int k = 0; k < P1; ++k) {
+                if (k < p1) {
+                    V v = ap[k];
+                    sr += crow[k] * v.x;
+                    si += crow[k] * v.y;
+                }
+            }
+            V um = Uo[m + 1];
+            ar += sgn * (um.x * sr - um.y * si);
+            ai += sgn * (um.x * si + um.y * sr);
+        }
+        __syncthreads();
+    }
+    if (g < G && slot < nbox && m < p1) {
+        // boxes without targets get zeros so stale data never leaks into L2L
+        Bl[(int64_t)box * p1 + m] = active ? V{ar * inv_r, ai * inv_r} : V{0, 0};
+    }
+}
+
+// ------------------------------------------------------ P2P + L2P (leaf) --
+template <int KIND>
+__global__ __launch_bounds__(64) void p2p_l2p_f32c(
+    const float4 *__restrict__ src, const int32_t *__restrict__ start_s,
+    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
+    const int32_t *__restrict__ perm_t, int Lvl, const int2 *__restrict__ near, int nnear,
+    PairParams pp, float s_f, const float2 *__restrict__ Bleaf, int p1,
+    double *__restrict__ qr, double *__restrict__ qi)
+{
+    const int lane = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int32_t t0 = start_t[b], t1 = start_t[b + 1];
+    if (t1 <= t0) return;
+    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
+    const int n = 1 << Lvl;
+    const float inv_s = 1.0f / s_f;
+    for (int32_t tc = t0; tc < t1; tc += 64) {
+        const int32_t i = tc + lane;
+        const bool valid = i < t1;
+        const float2 tp = valid ? tgt[i] : make_float2(0.f, 0.f);
+        float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
+        for (int q = 0; q < nnear; ++q) {
+            const int2 o = near[q];
+            const int sx = bx + o.x, sy = by + o.y;
+            if (sx < 0 || sy < 0 || sx >= n || sy >= n) continue;
+            const uint32_t sb = morton2((uint32_t)sx, (uint32_t)sy);
+            const int32_t j0 = start_s[sb], j1 = start_s[sb + 1];
+            if (j1 <= j0) continue;
+            const float xt = tp.x - (float)o.x * s_f;  // exact centre shift
+            const float yt = tp.y - (float)o.y * s_f;
+            int32_t j = j0;
+            for (; j + 16 <= j1; j += 16) {
+                float aa = 0.f, ab = 0.f;
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const float4 sp = src[j + k];
+                    const float dx = xt - sp.x, dy = yt - sp.y;
+                    const float P = pair_f32<KIND>(dx, dy, sp.z, pp);
+                    aa = __builtin_fmaf(P, dx, aa);
+                    ab = __builtin_fmaf(P, dy, ab);
+                }
+                two_sum_add(ha, la, aa);
+                two_sum_add(hb, lb, ab);
+            }
+            if (j < j1) {
+                float aa = 0.f, ab = 0.f;
+                for (; j < j1; ++j) {
+                    const float4 sp = src[j];
+                    const float dx = xt - sp.x, dy = yt - sp.y;
+                    const float P = pair_f32<KIND>(dx, dy, sp.z, pp);
+                    aa = __builtin_fmaf(P, dx, aa);
+                    ab = __builtin_fmaf(P, dy, ab);
+                }
+                two_sum_add(ha, la, aa);
+                two_sum_add(hb, lb, ab);
+            }
+        }
+        double fr = 0.0, fi = 0.0;
+        if (Bleaf) {
+            const float2 *Bb = Bleaf + (int64_t)b * p1;
+            const float zr = tp.x * inv_s, zi = tp.y * inv_s;
+            float pr = 0.f, pi = 0.f;
+            for (int k = p1 - 1; k >= 0; --k) {
+                const float2 c = Bb[k];
+                const float nr = pr * zr - pi * zi + c.x;
+                const float ni = pr * zi + pi * zr + c.y;
+                pr = nr;
+                pi = ni;
+            }
+            fr = pr;
+            fi = pi;
+        }
+        if (valid) {
+            const int32_t o = perm_t[i];
+            qr[o] = -(((double)ha + (double)la) + fr);
+            qi[o] = ((double)hb + (double)lb) - fi;
+        }
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void p2p_l2p_f64(
+    const double4 *__restrict__ src, const int32_t *__restrict__ start_s,
+    const double2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
+    const int32_t *__restrict__ perm_t, int Lvl, const int2 *__restrict__ near, int nnear,
+    PairParams pp, double x0, double y0, double s, const double2 *__restrict__ Bleaf, int p1,
+    double *__restrict__ qr, double *__restrict__ qi)
+{
+    const int lane = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int32_t t0 = start_t[b], t1 = start_t[b + 1];
+    if (t1 <= t0) return;
+    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
+    const int n = 1 << Lvl;
+    double cx, cy;
+    leaf_center(b, x0, y0, s, cx, cy);
+    const double inv_s = 1.0 / s;
+    for (int32_t tc = t0; tc < t1; tc += 64) {
+        const int32_t i = tc + lane;
+        const bool valid = i < t1;
+        const double2 tp = valid ? tgt[i] : make_double2(cx, cy);
+        double a = 0.0, bb = 0.0;
+        for (int q = 0; q < nnear; ++q) {
+            const int2 o = near[q];
+            const int sx = bx + o.x, sy = by + o.y;
+            if (sx < 0 || sy < 0 || sx >= n || sy >= n) continue;
+            const uint32_t sb = morton2((uint32_t)sx, (uint32_t)sy);
+            const int32_t j0 = start_s[sb], j1 = start_s[sb + 1];
+            for (int32_t j = j0; j < j1; ++j) {
+                const double4 sp = src[j];
+                const double dx = tp.x - sp.x, dy = tp.y - sp.y;
+                const double P = pair_f64<KIND>(dx, dy, sp.z, pp);
+                a = __builtin_fma(P, dx, a);
+                bb = __builtin_fma(P, dy, bb);
+            }
+        }
+        double fr = 0.0, fi = 0.0;
+        if (Bleaf) {
+            const double2 *Bv = Bleaf + (int64_t)b * p1;
+            const double zr = (tp.x - cx) * inv_s, zi = (tp.y - cy) * inv_s;
+            for (int k = p1 - 1; k >= 0; --k) {
+                const double2 c = Bv[k];
+                const double nr = fr * zr - fi * zi + c.x;
+                const double ni = fr * zi + fi * zr + c.y;
+                fr = nr;
+                fi = ni;
+            }
+        }
+        if (valid) {
+            const int32_t o = perm_t[i];
+            qr[o] = -(a + fr);
+            qi[o] = bb - fi;
+        }
+    }
+}
+
+// ------------------------------------------------------------- host ------
+namespace {
+
+double binom_d(int n, int k)
+{
+    if (k < 0 || k > n) return 0.0;
+    long double r = 1.0L;
+    for (int i = 1; i <= k; ++i) r = r * (long double)(n - k + i) / (long double)i;
+    return (double)std::llround(r) == (double)r ? (double)r : (double)r;
+}
+
+struct Lists {
+    std::vector<int2> near_leaf;
+    std::vector<std::vector<int2>> il;       // per level: concatenated parities
+    std::vector<std::vector<int32_t>> il_off;  // per level: 5 offsets
+    int R = 0;
+    int il_max_all = 0;
+};
+
+std::vector<int2> near_set(double s, double c2, int ws, int Rwin)
+{
+    std::vector<int2> out;
+    for (int dy = -Rwin; dy <= Rwin; ++dy)
+        for (int dx = -Rwin; dx <= Rwin; ++dx) {
+            double gx = std::max(std::abs(dx) - 1, 0) * s;
+            double gy = std::max(std::abs(dy) - 1, 0) * s;
+            if ((std::abs(dx) <= ws && std::abs(dy) <= ws) || gx * gx + gy * gy < c2)
+                out.push_back(make_int2(dx, dy));
+        }
+    return out;
+}
+
+int near_radius(const std::vector<int2> &v)
+{
+    int r = 0;
+    for (auto &o : v) r = std::max(r, std::max(std::abs(o.x), std::abs(o.y)));
+    return r;
+}
+
+Lists make_lists(const Domain &dom, double c2, int ws)
+{
+    Lists ls;
+    const int Lv = dom.L;
+    std::vector<std::vector<int2>> nearl(Lv + 1);
+    for (int l = 0; l <= Lv; ++l) {
+        double s = dom.D / std::ldexp(1.0, l);
+        int Rw = std::max(ws, (int)std::ceil(std::sqrt(c2) / s) + 2);
+        Rw = std::min(Rw, (1 << l));
+        nearl[l] = near_set(s, c2, ws, Rw);
+    }
+    ls.near_leaf = nearl[Lv];
+    ls.il.resize(Lv + 1);
+    ls.il_off.resize(Lv + 1);
+    for (int l = 2; l <= Lv; ++l) {
+        auto &lst = ls.il[l];
+        auto &off = ls.il_off[l];
+        off.push_back(0);
+        for (int par = 0; par < 4; ++par) {
+            const int px = par & 1, py = par >> 1;
+            std::vector<int2> cand;
+            for (auto &nu : nearl[l - 1])
+                for (int e = 0; e < 4; ++e) {
+                    int tx = 2 * nu.x + (e & 1) - px, ty = 2 * nu.y + (e >> 1) - py;
+                    bool in_near = false;
+                    for (auto &o : nearl[l])
+                        if (o.x == tx && o.y == ty) {
+                            in_near = true;
+                            break;
+                        }
+                    if (!in_near) cand.push_back(make_int2(tx, ty));
+                }
+            for (auto &c : cand) {
+                ls.R = std::max(ls.R, std::max(std::abs(c.x), std::abs(c.y)));
+                lst.push_back(c);
+            }
+            off.push_back((int32_t)lst.size());
+            ls.il_max_all = std::max(ls.il_max_all, (int)cand.size());
+        }
+    }
+    return ls;
+}
+
+// All small host tables of one call are packed into the lane's pinned
+// staging buffer and sent with a single asynchronous copy; the buffer is
+// owned by the call until its final stream synchronisation.
+struct TableBlob {
+    std::vector<char> host;
+    std::vector<std::pair<size_t, void **>> fix;
+    template <class T, class Dev>
+    void add(const std::vector<T> &v, Dev **d)
+    {
+        size_t at = (host.size() + 255) & ~size_t(255);
+        host.resize(at + std::max<size_t>(sizeof(T) * v.size(), 1));
+        if (!v.empty()) std::memcpy(host.data() + at, v.data(), sizeof(T) * v.size());
+        fix.push_back({at, reinterpret_cast<void **>(d)});
+    }
+    int send(Lane *L)
+    {
+        if (host.empty()) return PHF_OK;
+        PHF_TRY(L->pin.ensure(host.size()));
+        std::memcpy(L->pin.ptr, host.data(), host.size());
+        char *dev;
+        PHF_ALLOC(dev, L->arena, char, host.size());
+        PHF_HIP(hipMemcpyAsync(dev, L->pin.ptr, host.size(), hipMemcpyHostToDevice, L->stream));
+        for (auto &f : fix) *f.second = dev + f.first;
+        return PHF_OK;
+    }
+};
+
+}  // namespace
+
+int fmm_order_from_tol(double tol)
+{
+    // fmm.cpp:538-546 with theta = 0.5 and pot = 1
+    int p = (int)std::floor(std::log2(4.0 * tol) / std::log2(0.5) - 1.0) + 1;
+    return std::max(p, 2);
+}
+
+int fmm_level(int64_t nb, int leaf, int ndirect)
+{
+    int L;
+    if (leaf > 0) {
+        double q = (double)std::max<int64_t>(nb, 1) / leaf;
+        L = q > 1.0 ? (int)std::lround(std::log(q) / std::log(4.0)) : 0;
+    } else {
+        // fmm.cpp:553-555
+        double v = 0.5 * std::log2(5.0 / 8.0 * (double)nb / std::max(ndirect, 1));
+        L = (int)std::ceil(v);
+    }
+    return std::max(0, std::min(L, 11));
+}
+
+
+
+template <class T, int P1>
+static int launch_m2l(Lane *L, int l, int p1, const void *M, void *B, const int32_t *ss,
+                      const int32_t *st, int shift, const int2 *il, const int32_t *off, int il_max,
+                      const void *U, int R, const void *binom, double inv_r)
+{
+    using V = typename C2<T>::type;
+    const int G = 64 / p1;
+    const int64_t nbox = int64_t(1) << (2 * l);
+    const unsigned grid = (unsigned)((nbox + G - 1) / G);
+    m2l_kernel<T, P1><<<grid, 64, 0, L->stream>>>(
+        l, p1, (const V *)M, (V *)B, ss, st, shift, il, off, il_max, (const V *)U, R,
+        (const T *)binom, (T)inv_r);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+template <class T>
+static int m2l_dispatch(Lane *L, int l, int p1, const void *M, void *B, const int32_t *ss,
+                        const int32_t *st, int shift, const int2 *il, const int32_t *off,
+                        int il_max, const void *U, int R, const void *binom, double inv_r)
+{
+#define PHF_M2L(P) \
+    if (p1 <= P)   \
+        return launch_m2l<T, P>(L, l, p1, M, B, ss, st, shift, il, off, il_max, U, R, binom, inv_r);
+    PHF_M2L(9)
+    PHF_M2L(13)
+    PHF_M2L(17)
+    PHF_M2L(21)
+    PHF_M2L(25)
+    PHF_M2L(32)
+    PHF_M2L(41)
+#undef PHF_M2L
+    set_error("expansion order %d too large (max 40)", p1 - 1);
+    return PHF_ERR_UNSUPPORTED;
+}
+
+template <class T, class TC>
+static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lists &ls,
+                         const SortedPoints &S, const SortedPoints &Tg, StageTimer &tm,
+                         bool *used)
+{
+    using V = typename C2<T>::type;
+    using VC = typename C2<TC>::type;
+    const int Lv = dom.L, p1 = a.p + 1;
+    const PairParams pp = make_oseen_params(a.xopt, a.cutoff);
+
+    // sorted point data (the tree stage began in fmm_device)
+    void *src, *tgt;
+    const int32_t *perm_t = Tg.perm;
+    if (sizeof(T) == 4) {
+        float4 *s4;
+        PHF_ALLOC(s4, L->arena, float4, (size_t)std::max<int64_t>(a.nb, 1));
+        if (a.nb)
+            gather_src_f32<<<div_up(a.nb, 256), 256, 0, L->stream>>>(
+                a.xb, a.yb, a.wb, S.keys, S.perm, a.nb, dom.x0, dom.y0, dom.s, s4);
+        float2 *t2;
+        PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
+        gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(
+            a.xt, a.yt, Tg.keys, Tg.perm, a.nt, dom.x0, dom.y0, dom.s, t2);
+        src = s4;
+        tgt = t2;
+    } else {
+        double4 *s4;
+        PHF_ALLOC(s4, L->arena, double4, (size_t)std::max<int64_t>(a.nb, 1));
+        if (a.nb)
+            gather_src_f64<<<div_up(a.nb, 256), 256, 0, L->stream>>>(a.xb, a.yb, a.wb, S.perm,
+                                                                     a.nb, s4);
+        double2 *t2;
+        PHF_ALLOC(t2, L->arena, double2, (size_t)std::max<int64_t>(a.nt, 1));
+        gather_tgt_f64<<<div_up(a.nt, 256), 256, 0, L->stream>>>(a.xt, a.yt, Tg.perm, a.nt, t2);
+        src = s4;
+        tgt = t2;
+    }
+    PHF_HIP(hipGetLastError());
+    tm.end(PHF_STAGE_TREE);
+
+    // near list at the leaf level and, below, the far-field tables
+    TableBlob blob;
+    int2 *d_near = nullptr;
+    blob.add(ls.near_leaf, &d_near);
+
+    const V *Bleaf = nullptr;
+    if (Lv >= 2 && a.nb > 0) {
+        // level offsets in the coefficient arrays
+        std::vector<int64_t> off(Lv + 2, 0);
+        for (int l = 0; l <= Lv; ++l) off[l + 1] = off[l] + (int64_t(1) << (2 * l));
+        V *M, *B;
+        PHF_ALLOC(M, L->arena, V, (size_t)off[Lv + 1] * p1);
+        PHF_ALLOC(B, L->arena, V, (size_t)off[Lv + 1] * p1);
+
+        // shift tables (M2M: C(m,k) 2^-k d^(m-k); L2L: C(j,m) d^(j-m) 2^-m)
+        std::vector<VC> tm2m((size_t)4 * p1 * p1), tl2l((size_t)4 * p1 * p1);
+        for (int c = 0; c < 4; ++c) {
+            std::complex<double> d((c & 1) * 0.5 - 0.25, (c >> 1) * 0.5 - 0.25);
+            for (int m = 0; m < p1; ++m)
+                for (int k = 0; k < p1; ++k) {
+                    std::complex<double> v1 = 0, v2 = 0;
+                    if (k <= m) v1 = binom_d(m, k) * std::ldexp(1.0, -k) * std::pow(d, m - k);
+                    if (k >= m) v2 = binom_d(k, m) * std::pow(d, k - m) * std::ldexp(1.0, -m);
+                    tm2m[((size_t)c * p1 + m) * p1 + k] = VC{(TC)v1.real(), (TC)v1.imag()};
+                    tl2l[((size_t)c * p1 + m) * p1 + k] = VC{(TC)v2.real(), (TC)v2.imag()};
+                }
+        }
+        VC *d_m2m = nullptr, *d_l2l = nullptr;
+        blob.add(tm2m, &d_m2m);
+        blob.add(tl2l, &d_l2l);
+        // M2L: binomial rows C(k+m, k) and powers u^k, u = -1/offset
+        std::vector<T> bin((size_t)p1 * p1);
+        for (int m = 0; m < p1; ++m)
+            for (int k = 0; k < p1; ++k) bin[(size_t)m * p1 + k] = (T)binom_d(k + m, k);
+        const int R = std::max(ls.R, 1), W2 = 2 * R + 1;
+        std::vector<V> U((size_t)W2 * W2 * (p1 + 1));
+        for (int oy = -R; oy <= R; ++oy)
+            for (int ox = -R; ox <= R; ++ox) {
+                std::complex<double> u = 0.0;
+                if (ox || oy) u = -1.0 / std::complex<double>(ox, oy);
+                std::complex<double> pw = 1.0;
+                for (int k = 0; k <= p1; ++k) {
+                    U[((size_t)(oy + R) * W2 + (ox + R)) * (p1 + 1) + k] =
+                        V{(T)pw.real(), (T)pw.imag()};
+                    pw *= u;
+                }
+            }
+        T *d_bin = nullptr;
+        V *d_U = nullptr;
+        blob.add(bin, &d_bin);
+        blob.add(U, &d_U);
+        std::vector<int2> il_all;
+        std::vector<int32_t> il_offs;
+        std::vector<int> il_base(Lv + 1, 0), il_max(Lv + 1, 0);
+        for (int l = 2; l <= Lv; ++l) {
+            il_base[l] = (int)il_offs.size();
+            for (int q = 0; q < 5; ++q) il_offs.push_back((int32_t)il_all.size() + ls.il_off[l][q]);
+            for (int q = 0; q < 4; ++q)
+                il_max[l] = std::max(il_max[l], ls.il_off[l][q + 1] - ls.il_off[l][q]);
+            il_all.insert(il_all.end(), ls.il[l].begin(), ls.il[l].end());
+        }
+        int2 *d_il = nullptr;
+        int32_t *d_iloff = nullptr;
+        blob.add(il_all, &d_il);
+        blob.add(il_offs, &d_iloff);
+        PHF_TRY(blob.send(L));
+
+        // P2M
+        tm.begin(PHF_STAGE_P2M);
+        used[PHF_STAGE_P2M] = true;
+        const uint32_t nleaf = 1u << (2 * Lv);
+        p2m_kernel<T><<<nleaf, 64, (size_t)64 * p1 * sizeof(V), L->stream>>>(
+            src, S.start, p1, dom.x0, dom.y0, dom.s, Lv, M + off[Lv] * p1);
+        PHF_HIP(hipGetLastError());
+        tm.end(PHF_STAGE_P2M);
+        // M2M up to level 2
+        tm.begin(PHF_STAGE_M2M);
+        used[PHF_STAGE_M2M] = true;
+        for (int l = Lv - 1; l >= 2; --l) {
+            const uint32_t np = 1u << (2 * l);
+            m2m_kernel<T, TC><<<div_up((int64_t)np * p1, 256), 256, 0, L->stream>>>(
+                p1, np, M + off[l + 1] * p1, M + off[l] * p1, d_m2m, S.start, 2 * (Lv - l));
+        }
+        PHF_HIP(hipGetLastError());
+        tm.end(PHF_STAGE_M2M);
+        // M2L at every level >= 2 (writes B)
+        tm.begin(PHF_STAGE_M2L);
+        used[PHF_STAGE_M2L] = true;
+        for (int l = 2; l <= Lv; ++l) {
+            const double r = dom.D / std::ldexp(1.0, l);
+            PHF_TRY(m2l_dispatch<T>(L, l, p1, M + off[l] * p1, B + off[l] * p1, S.start,
+                                    Tg.start, 2 * (Lv - l), d_il, d_iloff + il_base[l],
+                                    il_max[l], d_U, R, d_bin, 1.0 / r));
+        }
+        tm.end(PHF_STAGE_M2L);
+        // L2L down
+        tm.begin(PHF_STAGE_L2L);
+        used[PHF_STAGE_L2L] = true;
+        for (int l = 2; l < Lv; ++l) {
+            const uint32_t nc = 1u << (2 * (l + 1));
+            l2l_kernel<T, TC><<<div_up((int64_t)nc * p1, 256), 256, 0, L->stream>>>(
+                p1, nc, B + off[l] * p1, B + off[l + 1] * p1, d_l2l, Tg.start,
+                2 * (Lv - l - 1));
+        }
+        PHF_HIP(hipGetLastError());
+        tm.end(PHF_STAGE_L2L);
+        Bleaf = B + off[Lv] * p1;
+    }
+
+    if (!d_near) PHF_TRY(blob.send(L));
+    // P2P + L2P
+    tm.begin(PHF_STAGE_P2P);
+    const uint32_t nleaf = 1u << (2 * Lv);
+    if (sizeof(T) == 4) {
+        p2p_l2p_f32c<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
+            (const float4 *)src, S.start, (const float2 *)tgt, Tg.start, perm_t, Lv, d_near,
+            (int)ls.near_leaf.size(), pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr, a.qi);
+    } else {
+        p2p_l2p_f64<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
+            (const double4 *)src, S.start, (const double2 *)tgt, Tg.start, perm_t, Lv, d_near,
+            (int)ls.near_leaf.size(), pp, dom.x0, dom.y0, dom.s, (const double2 *)Bleaf, p1,
+            a.qr, a.qi);
+    }
+    PHF_HIP(hipGetLastError());
+    tm.end(PHF_STAGE_P2P);
+    return PHF_OK;
+}
+
+int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info)
+{
+    if (a.nt == 0) return PHF_OK;
+    // 1. bounding box of sources and targets -> domain
+    tm.begin(PHF_STAGE_TREE);
+    double4 *bb;
+    PHF_ALLOC(bb, L->arena, double4, 1);
+    PHF_TRY(device_bbox(L, a.xb, a.yb, a.nb, a.same ? nullptr : a.xt, a.same ? nullptr : a.yt,
+                        a.same ? 0 : a.nt, bb));
+    double4 hb;
+    PHF_HIP(hipMemcpyAsync(&hb, bb, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    const double c2 = a.cutoff * a.cutoff;
+    int Lv = a.Lhint;
+    Domain dom = make_domain(hb.x, hb.y, hb.z, hb.w, Lv);
+    Lists ls = make_lists(dom, c2, a.ws);
+    // keep the leaf near set bounded (large cutoffs relative to the box)
+    while (Lv > 0 && near_radius(ls.near_leaf) > 6) {
+        --Lv;
+        dom = make_domain(hb.x, hb.y, hb.z, hb.w, Lv);
+        ls = make_lists(dom, c2, a.ws);
+    }
+    SortedPoints S, Tg;
+    PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
+    if (a.same)
+        Tg = S;
+    else
+        PHF_TRY(build_sorted_points(L, dom, a.xt, a.yt, a.nt, &Tg));
+    used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
+    if (info) {
+        info->L = Lv;
+        info->p = a.p;
+        info->nnear = (int)ls.near_leaf.size();
+        info->D = dom.D;
+    }
+    if (a.prec == PHF_PREC_F32C) {
+        if (a.p > 16) return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used);
+        return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used);
+    }
+    return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used);
+}
+
+}  // namespace phf
+
+// ------------------------------------------------------------ C entry ----
+using namespace phf;
+
+extern "C" int phf_fmm2d(const double *xb, const double *yb, const double *wb, int64_t nb,
+                         const double *xt, const double *yt, int64_t nt, int32_t ndirect,
+                         double xopt, double cutoff, double tol, const phf_fmm_opts *opts,
+                         double *qr, double *qi)
+{
+    clear_error();
+    PHF_REQUIRE(nb >= 0 && nt >= 0 && nb < (int64_t(1) << 31) && nt < (int64_t(1) << 31),
+                "sizes out of range (nb=%lld, nt=%lld)", (long long)nb, (long long)nt);
+    PHF_REQUIRE(std::isfinite(xopt) && xopt > 0.0, "xopt must be finite and > 0 (got %g)", xopt);
+    PHF_REQUIRE(std::isfinite(cutoff) && cutoff >= 0.0, "cutoff must be finite and >= 0 (got %g)",
+                cutoff);
+    PHF_REQUIRE(std::isfinite(tol) && tol >= 0.0, "tol must be finite and >= 0 (got %g)", tol);
+    phf_fmm_opts o{0, 0, PHF_PREC_F64, 0};
+    if (opts) o = *opts;
+    PHF_REQUIRE(o.prec == PHF_PREC_F64 || o.prec == PHF_PREC_F32C, "unknown precision %d",
+                o.prec);
+    PHF_REQUIRE(o.order <= 40, "order must be <= 40 (got %d)", o.order);
+    PHF_REQUIRE(o.ws <= 3, "ws must be <= 3 (got %d)", o.ws);
+    PHF_TRY(check_host_array(xb, nb, "xBlob"));
+    PHF_TRY(check_host_array(yb, nb, "yBlob"));
+    PHF_TRY(check_host_array(wb, nb, "wBlob"));
+    PHF_TRY(check_host_array(xt, nt, "xTarget"));
+    PHF_TRY(check_host_array(yt, nt, "yTarget"));
+    PHF_REQUIRE(nt == 0 || (qr && qi), "output arrays are NULL");
+    if (nt == 0) return PHF_OK;
+    const bool same = (xb == xt && yb == yt && nb == nt);
+
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    StageTimer tm(L, timing_enabled());
+    bool used[PHF_STAGE_COUNT] = {};
+    used[PHF_STAGE_UPLOAD] = used[PHF_STAGE_DOWNLOAD] = used[PHF_STAGE_TOTAL] = true;
+    tm.begin(PHF_STAGE_TOTAL);
+    tm.begin(PHF_STAGE_UPLOAD);
+    double *dxb, *dyb, *dwb, *dxt, *dyt, *dqr, *dqi;
+    PHF_TRY(upload(L, xb, nb, &dxb));
+    PHF_TRY(upload(L, yb, nb, &dyb));
+    PHF_TRY(upload(L, wb, nb, &dwb));
+    if (same) {
+        dxt = dxb;
+        dyt = dyb;
+    } else {
+        PHF_TRY(upload(L, xt, nt, &dxt));
+        PHF_TRY(upload(L, yt, nt, &dyt));
+    }
+    PHF_ALLOC(dqr, L->arena, double, (size_t)nt);
+    PHF_ALLOC(dqi, L->arena, double, (size_t)nt);
+    tm.end(PHF_STAGE_UPLOAD);
+    if (tol == 0.0) {
+        // direct sum over all pairs (direct.cpp:1409-1450)
+        tm.begin(PHF_STAGE_P2P);
+        used[PHF_STAGE_P2P] = true;
+        PairParams pp = make_oseen_params(xopt, cutoff);
+        PHF_TRY(direct_device(L, PAIR_OSEEN, o.prec, pp, dxb, dyb, dwb, nb, dxt, dyt, nt, dqr,
+                              dqi));
+        tm.end(PHF_STAGE_P2P);
+    } else {
+        FmmArgs a{};
+        a.xb = dxb; a.yb = dyb; a.wb = dwb; a.nb = nb;
+        a.xt = dxt; a.yt = dyt; a.nt = nt; a.same = same;
+        a.xopt = xopt; a.cutoff = cutoff;
+        a.p = o.order > 0 ? o.order : fmm_order_from_tol(tol);
+        a.ws = o.ws > 0 ? o.ws : 1;
+        a.prec = o.prec;
+        a.Lhint = fmm_level(nb, o.leaf, ndirect);
+        a.qr = dqr;
+        a.qi = dqi;
+        PHF_TRY(fmm_device(L, a, tm, used, nullptr));
+    }
+    tm.begin(PHF_STAGE_DOWNLOAD);
+    PHF_TRY(download(L, dqr, nt, qr));
+    PHF_TRY(download(L, dqi, nt, qi));
+    tm.end(PHF_STAGE_DOWNLOAD);
+    tm.end(PHF_STAGE_TOTAL);
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    record_stage_times(L, used);
+    return PHF_OK;
+}

@@ -1,0 +1,423 @@
+// phf_tree.hip -- Morton-key quad-tree build on gfx950.
+//
+// The reference bins blobs into an adaptive median-split tree
+// (fmmsort.cpp:441-785, GPU: cudasort.cu:86 partition kernels).  This
+// library uses a uniform quad-tree of depth L over a square domain:
+//
+//   bbox (exact min/max, FP64)  ->  domain x0, y0, side D (rounded up to 5
+//   significant bits so box sides and box-centre differences are exact in
+//   float32)  ->  Morton keys  ix = clamp(floor((x-x0)*inv_s), 0, 2^L-1)
+//   ->  stable LSD radix sort of (key, index), 8 bits per pass  ->  leaf
+//   offsets start[k] = lower_bound(sorted keys, k).
+//
+// Keys, permutation and offsets are integer results and match a CPU
+// stable sort bit for bit (the binning uses the same FP64 operations, with
+// no contraction possible: a subtract, a multiply, floor).
+//
+// Radix sort: per-tile digit histograms -> device-wide exclusive scan
+// (digit-major, so each tile's bucket offsets come out in order) -> stable
+// scatter.  Inside a tile, elements are ranked 256 at a time; a wave finds
+// the lanes sharing its digit with 8 ballots ("match any"), and per-wave
+// counts in LDS give the cross-wave prefix, so equal keys keep input order.
+#include "phf_tree.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace phf {
+
+// ----------------------------------------------------------------- bbox --
+constexpr int BB_THREADS = 256;
+
+__global__ __launch_bounds__(BB_THREADS) void bbox_partial(
+    const double *__restrict__ x1, const double *__restrict__ y1, int64_t n1,
+    const double *__restrict__ x2, const double *__restrict__ y2, int64_t n2,
+    double4 *__restrict__ part)
+{
+    __shared__ double4 sh[BB_THREADS];
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    const int64_t n = n1 + n2;
+    for (int64_t i = (int64_t)blockIdx.x * BB_THREADS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * BB_THREADS) {
+        double x = i < n1 ? x1[i] : x2[i - n1];
+        double y = i < n1 ? y1[i] : y2[i - n1];
+        xmin = fmin(xmin, x);
+        xmax = fmax(xmax, x);
+        ymin = fmin(ymin, y);
+        ymax = fmax(ymax, y);
+    }
+    sh[threadIdx.x] = make_double4(xmin, xmax, ymin, ymax);
+    __syncthreads();
+    for (int s = BB_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double4 a = sh[threadIdx.x], b = sh[threadIdx.x + s];
+            sh[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z),
+                                           fmax(a.w, b.w));
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(BB_THREADS) void bbox_final(const double4 *__restrict__ part,
+                                                         int np, double4 *__restrict__ out)
+{
+    __shared__ double4 sh[BB_THREADS];
+    double4 acc = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    for (int i = threadIdx.x; i < np; i += BB_THREADS) {
+        double4 b = part[i];
+        acc = make_double4(fmin(acc.x, b.x), fmax(acc.y, b.y), fmin(acc.z, b.z), fmax(acc.w, b.w));
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = BB_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double4 a = sh[threadIdx.x], b = sh[threadIdx.x + s];
+            sh[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z),
+                                           fmax(a.w, b.w));
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const double *x2,
+                const double *y2, int64_t n2, double4 *d_out)
+{
+    const int64_t n = n1 + n2;
+    int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + BB_THREADS * 8 - 1) /
+                                                                     (BB_THREADS * 8)));
+    double4 *part;
+    PHF_ALLOC(part, L->arena, double4, (size_t)nblk);
+    bbox_partial<<<nblk, BB_THREADS, 0, L->stream>>>(x1, y1, n1, x2, y2, n2, part);
+    bbox_final<<<1, BB_THREADS, 0, L->stream>>>(part, nblk, d_out);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L)
+{
+    Domain d{};
+    d.x0 = xmin;
+    d.y0 = ymin;
+    double ext = std::max(xmax - xmin, ymax - ymin);
+    if (!(ext > 0.0)) ext = 1.0;
+    int e = 0;
+    std::frexp(ext, &e);  // ext = f * 2^e, f in [0.5, 1)
+    double unit = std::ldexp(1.0, e - 5);
+    d.D = std::ceil(ext / unit) * unit;  // <= 32 units, exact
+    d.L = L;
+    d.s = d.D / std::ldexp(1.0, L);  // exact (power-of-two division)
+    d.inv_s = std::ldexp(1.0, L) / d.D;
+    return d;
+}
+
+// ----------------------------------------------------------------- keys --
+__global__ void morton_keys_kernel(const double *__restrict__ x, const double *__restrict__ y,
+                                   int64_t n, double x0, double y0, double inv_s, int L,
+                                   uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double hi = (double)((1u << L) - 1u);
+    double fx = floor((x[i] - x0) * inv_s);
+    double fy = floor((y[i] - y0) * inv_s);
+    fx = (fx >= 0.0) ? fx : 0.0;
+    fy = (fy >= 0.0) ? fy : 0.0;
+    fx = fx > hi ? hi : fx;
+    fy = fy > hi ? hi : fy;
+    keys[i] = morton2((uint32_t)fx, (uint32_t)fy);
+    vals[i] = (int32_t)i;
+}
+
+// ----------------------------------------------------------------- scan --
+// Exclusive scan of uint32 (device-wide, three passes).
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 8;
+constexpr int SC_CHUNK = SC_THREADS * SC_ITEMS;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh_wave,
+                                                         uint32_t *total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) sh_wave[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < SC_THREADS / 64; ++w) {
+        uint32_t s = sh_wave[w];
+        if (w < wave) before += s;
+        all += s;
+    }
+    __syncthreads();
+    if (total) *total = all;
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void scan_reduce(const uint32_t *__restrict__ in,
+                                                          int64_t n, uint32_t *__restrict__ sums)
+{
+    __shared__ uint32_t shw[SC_THREADS / 64];
+    const int64_t base = (int64_t)blockIdx.x * SC_CHUNK + (int64_t)threadIdx.x * SC_ITEMS;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k)
+        if (base + k < n) s += in[base + k];
+    uint32_t tot;
+    block_exclusive_scan(s, shw, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(SC_THREADS) void scan_top(uint32_t *__restrict__ sums, int64_t nb)
+{
+    __shared__ uint32_t shw[SC_THREADS / 64];
+    uint32_t carry = 0;
+    for (int64_t b0 = 0; b0 < nb; b0 += SC_THREADS) {
+        int64_t i = b0 + threadIdx.x;
+        uint32_t v = i < nb ? sums[i] : 0;
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan(v, shw, &tot);
+        if (i < nb) sums[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SC_THREADS) void scan_apply(const uint32_t *__restrict__ in,
+                                                         int64_t n,
+                                                         const uint32_t *__restrict__ sums,
+                                                         uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t shw[SC_THREADS / 64];
+    const int64_t base = (int64_t)blockIdx.x * SC_CHUNK + (int64_t)threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0;
+        s += v[k];
+    }
+    uint32_t ex = block_exclusive_scan(s, shw, nullptr) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+}
+
+int device_exclusive_scan(Lane *L, const uint32_t *in, int64_t n, uint32_t *out)
+{
+    if (n <= 0) return PHF_OK;
+    int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
+    uint32_t *sums;
+    PHF_ALLOC(sums, L->arena, uint32_t, (size_t)nb);
+    scan_reduce<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums);
+    scan_top<<<1, SC_THREADS, 0, L->stream>>>(sums, nb);
+    scan_apply<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums, out);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+// ---------------------------------------------------------- radix sort --
+constexpr int RS_THREADS = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+
+__global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint32_t *__restrict__ keys,
+                                                      int64_t n, int shift, uint32_t mask,
+                                                      uint32_t *__restrict__ hist, int nblocks)
+{
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_scatter(
+    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, int32_t *__restrict__ vout, int64_t n, int shift, int nbits,
+    const uint32_t *__restrict__ offs, int nblocks)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t mask = (1u << nbits) - 1u;
+    run[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < RS_THREADS / 64; ++w) wcnt[w][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = base + (int64_t)r * RS_THREADS + tid;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const int32_t val = valid ? vin[i] : 0;
+        const uint32_t d = (key >> shift) & mask;
+        uint64_t m = __ballot(valid);
+        for (int b = 0; b < nbits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(valid && bit);
+            m &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & lt);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; ++w) {
+            add += wcnt[w][tid];
+            wcnt[w][tid] = 0;
+        }
+        run[tid] += add;
+        __syncthreads();
+    }
+}
+
+int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
+                      int64_t n, int key_bits, uint32_t **kres, int32_t **vres)
+{
+    *kres = keys;
+    *vres = vals;
+    if (n <= 1 || key_bits <= 0) return PHF_OK;
+    const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    uint32_t *hist, *offs;
+    PHF_ALLOC(hist, L->arena, uint32_t, (size_t)256 * nblocks);
+    PHF_ALLOC(offs, L->arena, uint32_t, (size_t)256 * nblocks);
+    uint32_t *ka = keys, *kb = ktmp;
+    int32_t *va = vals, *vb = vtmp;
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        const int nbits = std::min(8, key_bits - shift);
+        const uint32_t mask = (1u << nbits) - 1u;
+        rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
+        PHF_TRY(device_exclusive_scan(L, hist, (int64_t)256 * nblocks, offs));
+        rs_scatter<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits, offs,
+                                                          nblocks);
+        PHF_HIP(hipGetLastError());
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    *kres = ka;
+    *vres = va;
+    return PHF_OK;
+}
+
+// --------------------------------------------------------- leaf offsets --
+__global__ void leaf_starts_kernel(const uint32_t *__restrict__ skeys, int64_t n, int64_t nleaf,
+                                   int32_t *__restrict__ start)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > nleaf) return;
+    int64_t lo = 0, hi = n;  // first index with key >= k
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)skeys[mid] < k)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    start[k] = (int32_t)lo;
+}
+
+int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
+                        SortedPoints *out)
+{
+    uint32_t *k0, *k1;
+    int32_t *v0, *v1;
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    PHF_ALLOC(k0, L->arena, uint32_t, m);
+    PHF_ALLOC(k1, L->arena, uint32_t, m);
+    PHF_ALLOC(v0, L->arena, int32_t, m);
+    PHF_ALLOC(v1, L->arena, int32_t, m);
+    const int64_t nleaf = int64_t(1) << (2 * dom.L);
+    int32_t *start;
+    PHF_ALLOC(start, L->arena, int32_t, (size_t)nleaf + 1);
+    if (n > 0)
+        morton_keys_kernel<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, dom.x0, dom.y0,
+                                                                  dom.inv_s, dom.L, k0, v0);
+    PHF_HIP(hipGetLastError());
+    out->keys_in = nullptr;
+    if (out->want_input_keys) {
+        PHF_ALLOC(out->keys_in, L->arena, uint32_t, m);
+        if (n > 0)
+            PHF_HIP(hipMemcpyAsync(out->keys_in, k0, sizeof(uint32_t) * n,
+                                   hipMemcpyDeviceToDevice, L->stream));
+    }
+    uint32_t *ks;
+    int32_t *vs;
+    PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, n, 2 * dom.L, &ks, &vs));
+    leaf_starts_kernel<<<div_up(nleaf + 1, 256), 256, 0, L->stream>>>(ks, n, nleaf, start);
+    PHF_HIP(hipGetLastError());
+    out->n = n;
+    out->keys = ks;
+    out->perm = vs;
+    out->start = start;
+    return PHF_OK;
+}
+
+}  // namespace phf
+
+// ------------------------------------------------------------ C entry ----
+using namespace phf;
+
+extern "C" int phf_tree_build(const double *x, const double *y, int64_t n, int32_t L,
+                              uint32_t *keys, int32_t *perm, int32_t *start, double *x0,
+                              double *y0, double *inv_s)
+{
+    clear_error();
+    PHF_REQUIRE(n >= 0 && n < (int64_t(1) << 31), "n out of range (%lld)", (long long)n);
+    PHF_REQUIRE(L >= 0 && L <= 12, "L must be in [0, 12] (got %d)", L);
+    PHF_TRY(check_host_array(x, n, "x"));
+    PHF_TRY(check_host_array(y, n, "y"));
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *Ln = g.lane;
+    PHF_TRY(Ln->arena.reset());
+    double *dx, *dy;
+    PHF_TRY(upload(Ln, x, n, &dx));
+    PHF_TRY(upload(Ln, y, n, &dy));
+    double4 *bb;
+    PHF_ALLOC(bb, Ln->arena, double4, 1);
+    double4 hb = make_double4(0, 0, 0, 0);
+    if (n > 0) {
+        PHF_TRY(device_bbox(Ln, dx, dy, n, nullptr, nullptr, 0, bb));
+        PHF_HIP(hipMemcpyAsync(&hb, bb, sizeof(double4), hipMemcpyDeviceToHost, Ln->stream));
+        PHF_HIP(hipStreamSynchronize(Ln->stream));
+    }
+    Domain dom = make_domain(hb.x, hb.y, hb.z, hb.w, L);
+    SortedPoints sp;
+    sp.want_input_keys = keys != nullptr;
+    PHF_TRY(build_sorted_points(Ln, dom, dx, dy, n, &sp));
+    if (keys && n > 0)
+        PHF_HIP(hipMemcpyAsync(keys, sp.keys_in, sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                               Ln->stream));
+    if (perm && n > 0)
+        PHF_HIP(hipMemcpyAsync(perm, sp.perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                               Ln->stream));
+    if (start)
+        PHF_HIP(hipMemcpyAsync(start, sp.start, sizeof(int32_t) * ((size_t(1) << (2 * L)) + 1),
+                               hipMemcpyDeviceToHost, Ln->stream));
+    PHF_HIP(hipStreamSynchronize(Ln->stream));
+    if (x0) *x0 = dom.x0;
+    if (y0) *y0 = dom.y0;
+    if (inv_s) *inv_s = dom.inv_s;
+    return PHF_OK;
+}

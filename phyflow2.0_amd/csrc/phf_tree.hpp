@@ -1,0 +1,44 @@
+// phf_tree.hpp -- uniform Morton quad-tree build (device).
+#pragma once
+
+#include "phf_ops.hpp"
+
+namespace phf {
+
+// Square domain [x0, x0+D) x [y0, y0+D) split into 2^L x 2^L leaf boxes.
+struct Domain {
+    double x0, y0;  // lower-left corner (bbox minimum)
+    double D;       // side, bbox extent rounded up to 5 significant bits
+    double s;       // leaf box side D / 2^L (exact)
+    double inv_s;   // 2^L / D
+    int L;
+};
+
+Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L);
+
+// Exact bounding box of (x1,y1)[n1] U (x2,y2)[n2] into d_out = {xmin, xmax,
+// ymin, ymax} (device memory).
+int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const double *x2,
+                const double *y2, int64_t n2, double4 *d_out);
+
+int device_exclusive_scan(Lane *L, const uint32_t *in, int64_t n, uint32_t *out);
+
+// Stable LSD radix sort of (keys, vals) on the low key_bits bits; ktmp/vtmp
+// are ping-pong buffers.  *kres/*vres point at whichever buffer holds the
+// result.
+int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
+                      int64_t n, int key_bits, uint32_t **kres, int32_t **vres);
+
+struct SortedPoints {
+    int64_t n = 0;
+    uint32_t *keys = nullptr;     // sorted Morton keys
+    int32_t *perm = nullptr;      // sorted position -> input index
+    int32_t *start = nullptr;     // leaf offsets, 4^L + 1 entries
+    uint32_t *keys_in = nullptr;  // keys in input order (if requested)
+    bool want_input_keys = false;
+};
+
+int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
+                        SortedPoints *out);
+
+}  // namespace phf
