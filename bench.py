@@ -1,0 +1,186 @@
+"""Benchmark of the vortex-blob velocity hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (headline): 1,048,576 blobs uniform in [0,1]^2 from seed 0,
+circulations N(0,1) h^2 with h = 1/1024, sigma = h/0.8, k = 2; velocity at
+every blob centre by the FMM (Oseen smoother, cutoff 5 xopt) at order 16,
+leaf size 64, float32 pair arithmetic with compensated sums.  The blobs are
+resident in HBM before timing starts; one step = one full velocity
+evaluation (tree build, expansions, near field), synchronised.
+
+Prints one JSON line: throughput in velocity evaluations/s, p50/p99 step
+latency, the error against a float64 direct sum on 65,536 seeded targets,
+the P2P kernel's roofline figures (HIP events on the library's stream) and
+the reference's own CPU FMM timed on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from __graft_entry__ import load_package  # noqa: E402
+
+load_package()
+
+from phyflow_amd import _native  # noqa: E402
+from phyflow_amd import workloads as W  # noqa: E402
+from phyflow_amd.cpp.blobs.kernels import gauss  # noqa: E402
+from phyflow_amd.system import BlobSystem, vel_params  # noqa: E402
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X fp32 (vector, packed) / dense matrix
+FLOP_PER_PAIR = 17
+
+
+def timed_steps(sysm, params, warmup, steps):
+    for _ in range(warmup):
+        sysm.velocity(params, fetch=False)
+    lat, p2p_ms, info = [], [], None
+    _native.lib().phf_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        sysm.velocity(params, fetch=False, info=True)
+        lat.append(time.perf_counter() - a)
+        info = sysm.last_info
+        p2p_ms.append(info["ms"].get("p2p", float("nan")))
+    _native.lib().phf_synchronize()
+    wall = time.perf_counter() - t0
+    return wall, np.array(lat), np.array(p2p_ms), info
+
+
+def rel_errors(sysm, params, s, n_check, seed):
+    """rel-L2 and max error of the evaluated velocity against the float64
+    direct sum (same Oseen kernel) on n_check seeded targets."""
+    sysm.velocity(params, fetch=False)
+    vx, vy = sysm.last_velocity()
+    sub = W.target_subset(s["x"].size, n_check, seed=seed)
+    xopt, cutoff = W.oseen_params(s["sigma"])
+    qr, qi = gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"][sub], s["y"][sub], 35, xopt, cutoff,
+                             0.0, prec=gauss.PREC_F64)
+    rx, ry = -qi / (2 * math.pi), -qr / (2 * math.pi)
+    a = np.concatenate([vx[sub], vy[sub]])
+    b = np.concatenate([rx, ry])
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b)), float(np.max(np.abs(a - b)) /
+                                                                   np.max(np.abs(b)))
+
+
+def cpu_baseline(s, budget_s=40.0):
+    """The reference's compiled CPU FMM (oracle/_ref) at its default
+    tol=1e-6, Ndirect=35 on the same workload; falls back to the oracle's
+    direct sum on a target subset if the reference build is absent."""
+    from oracle import oracle as O
+    xopt, cutoff = W.oseen_params(s["sigma"])
+    n = s["x"].size
+    if O.ref_available():
+        t = time.perf_counter()
+        O.ref_fmm2d(s["x"], s["y"], s["w"], s["x"], s["y"], 35, xopt, cutoff, 1e-6)
+        dt = time.perf_counter() - t
+        return {"value": n / dt, "unit": "evals/s", "cores": 1, "kind": "reference",
+                "sample": "full %d-blob workload, reference fmm2d CPU build (tol=1e-6, "
+                          "Ndirect=35, single-threaded as shipped), %.1f s" % (n, dt)}
+    cores = os.cpu_count() or 1
+    m = 4096
+    sub = W.target_subset(n, m, seed=0)
+    t = time.perf_counter()
+    O.direct_oseen(s["x"], s["y"], s["w"], s["x"][sub], s["y"][sub], xopt, cutoff,
+                   nthreads=cores)
+    dt = time.perf_counter() - t
+    return {"value": m / dt, "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": "%d seeded targets of the %d-blob workload, float64 direct sum" % (m, n)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--order", type=int, default=16)
+    ap.add_argument("--leaf", type=int, default=64)
+    ap.add_argument("--prec", choices=["f32c", "f64"], default="f32c")
+    ap.add_argument("--check", type=int, default=65536, help="targets for the error check (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variants", action="store_true", help="also time p in {8,16,24} x {f32c,f64}")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    s = W.uniform_system(args.n, seed=0, h=1.0 / 1024 if args.n == 1048576 else None)
+    prec = gauss.PREC_F32C if args.prec == "f32c" else gauss.PREC_F64
+    params = vel_params(s["sigma"], k=2.0, prec=prec, order=args.order, leaf=args.leaf)
+    dev = _native.device_info()
+    with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
+        wall, lat, p2p_ms, info = timed_steps(sysm, params, args.warmup, args.steps)
+        ms_step = 1e3 * wall / args.steps
+        rel, mx = rel_errors(sysm, params, s, args.check, 0) if args.check else (None, None)
+        variants = []
+        if args.variants:
+            for pr in ("f32c", "f64"):
+                for p in (8, 16, 24):
+                    vp = vel_params(s["sigma"], prec=gauss.PREC_F32C if pr == "f32c" else
+                                    gauss.PREC_F64, order=p, leaf=args.leaf)
+                    vw, vl, _, _ = timed_steps(sysm, vp, 2, 5)
+                    e = rel_errors(sysm, vp, s, 8192, 1)
+                    variants.append({"prec": pr, "order": p, "ms_per_step": 1e3 * vw / 5,
+                                     "evals_per_s": args.n / (vw / 5), "rel_l2": e[0], "max_err": e[1]})
+    t_p2p = float(np.nanmean(p2p_ms)) * 1e-3
+    pairs = info["p2p_pairs"]
+    achieved = FLOP_PER_PAIR * pairs / t_p2p / 1e12
+    out = {
+        "metric": "blob vel. evals/s @1M blobs, rel err<=1e-6 vs FP64; p50/p99 step latency",
+        "value": args.n / (wall / args.steps),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if prec == gauss.PREC_F32C else "f64",
+        "data": "synthetic",
+        "rel_l2_vs_fp64": rel,
+        "max_err_vs_fp64": mx,
+        "config": {
+            "workload": "FMM velocity of %d uniform blobs at their centres (seed 0, h=1/1024, "
+                        "sigma=h/0.8), order %d, leaf %d, Oseen cutoff 5*xopt, %s; error vs "
+                        "float64 direct sum on %d seeded targets"
+                        % (args.n, args.order, args.leaf, args.prec, args.check),
+            "n_blobs": args.n, "order": args.order, "leaf": args.leaf, "levels": info["L"],
+            "near_boxes": info["nnear"], "p2p_pairs": pairs, "m2l_links": info["m2l_links"],
+            "stage_ms": info["ms"], "device": dev["name"],
+        },
+        "roofline": {
+            "kernel": "p2p_l2p_f32c" if prec == gauss.PREC_F32C else "p2p_l2p_f64",
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS if prec == gauss.PREC_F32C else 78.6,
+            "unit": "TFLOP/s",
+            "frac": achieved / (FP32_PEAK_TFLOPS if prec == gauss.PREC_F32C else 78.6),
+            "traffic": None,
+            "flop_per_pair": FLOP_PER_PAIR,
+            "kernel_ms": t_p2p * 1e3,
+        },
+    }
+    if variants:
+        out["variants"] = variants
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(s)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

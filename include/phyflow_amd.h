@@ -65,6 +65,9 @@ int phf_init(void);
 int phf_device_info(char *name, int64_t name_len, int32_t *num_cu,
                     int64_t *hbm_bytes);
 
+/* Waits until all work the library queued on the device has finished. */
+int phf_synchronize(void);
+
 /* Turn per-stage HIP-event timing on (1) or off (0, default). */
 int phf_set_timing(int32_t enabled);
 
@@ -115,6 +118,63 @@ int phf_fmm2d(const double *xb, const double *yb, const double *wb,
 int phf_tree_build(const double *x, const double *y, int64_t n, int32_t L,
                    uint32_t *keys, int32_t *perm, int32_t *start, double *x0,
                    double *y0, double *inv_s);
+
+/* ---- resident blob systems -----------------------------------------------
+ * A phf_system keeps one blob set (x, y, w) resident in HBM with its own HIP
+ * stream and workspace, so repeated evaluations make no host copies and no
+ * device allocations.  It stands in for the state the reference's Blobs
+ * object holds on the host (src/python/blobs/__blobsClass.py:415-494,
+ * properties x, y, g at :2651-2690) and re-uploads on every GPU call
+ * (cudaeval.cu:154-342). */
+typedef struct phf_system_s phf_system;
+
+#define PHF_METHOD_FMM 0    /* blobOptions.FMM_METHOD */
+#define PHF_METHOD_DIRECT 1 /* blobOptions.DIRECT_METHOD */
+
+/* Velocity-evaluation parameters, the arguments of the reference's
+ * velocity(..., sigma, k, ..., method, Ndirect, tol, cutoff)
+ * (src/python/blobs/base/induced/_induced.py:34-36).  method FMM uses the
+ * fmm2d Oseen smoother (tol == 0: its direct sum); method DIRECT uses the
+ * vv2parGauss EPS-regularised Gaussian kernel. */
+typedef struct {
+    int32_t method;  /* PHF_METHOD_FMM or PHF_METHOD_DIRECT */
+    int32_t prec;    /* PHF_PREC_F64 or PHF_PREC_F32C */
+    int32_t order;   /* FMM order; <=0: from tol (fmm.cpp:538-546) */
+    int32_t leaf;    /* FMM mean sources per leaf; <=0: from ndirect */
+    int32_t ws;      /* FMM well-separation in boxes; <=0: 1 */
+    int32_t ndirect; /* reference Ndirect (default 35) */
+    double sigma;    /* blob core size */
+    double k;        /* core multiplier, k*sigma^2 (default 2) */
+    double tol;      /* FMM tolerance (default 1e-6); 0: direct Oseen sum */
+    double cutoff;   /* Oseen cutoff; <=0: 5*xopt (_induced.py:240) */
+} phf_vel_params;
+
+/* Work and timing of the last evaluation (stage ms are HIP-event times on
+ * the system's stream; -1 for stages not run). */
+typedef struct {
+    int32_t L, p, nnear, pad;
+    double D;
+    int64_t p2p_pairs; /* FMM: near-field pairs; direct: nb*nt */
+    int64_t m2l_links;
+    double ms[PHF_STAGE_COUNT];
+} phf_eval_info;
+
+int phf_system_create(const double *x, const double *y, const double *w, int64_t n,
+                      phf_system **out);
+int phf_system_destroy(phf_system *s);
+int phf_system_size(phf_system *s, int64_t *n);
+int phf_system_get(phf_system *s, double *x, double *y, double *w);
+int phf_system_set(phf_system *s, const double *x, const double *y, const double *w,
+                   int64_t n);
+
+/* Velocity at the blob centres, the reference's
+ * velocity(xBlob, yBlob, wBlob, sigma, k, kernel=gauss, xEval=None, ...)
+ * (_induced.py:34-166) with (vx, vy) in its convention.  vx/vy may be NULL:
+ * the result then stays on the device (phf_system_get_velocity fetches
+ * it).  info may be NULL. */
+int phf_system_velocity(phf_system *s, const phf_vel_params *p, double *vx, double *vy,
+                        phf_eval_info *info);
+int phf_system_get_velocity(phf_system *s, double *vx, double *vy);
 
 #ifdef __cplusplus
 }

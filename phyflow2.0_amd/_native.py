@@ -42,15 +42,45 @@ class FmmOpts(ctypes.Structure):
     _fields_ = [("order", _i32), ("leaf", _i32), ("prec", _i32), ("ws", _i32)]
 
 
+METHOD_FMM = 0
+METHOD_DIRECT = 1
+N_STAGES = len(STAGES)
+
+
+class VelParams(ctypes.Structure):
+    _fields_ = [("method", _i32), ("prec", _i32), ("order", _i32), ("leaf", _i32),
+                ("ws", _i32), ("ndirect", _i32), ("sigma", _dbl), ("k", _dbl),
+                ("tol", _dbl), ("cutoff", _dbl)]
+
+
+class EvalInfo(ctypes.Structure):
+    _fields_ = [("L", _i32), ("p", _i32), ("nnear", _i32), ("pad", _i32), ("D", _dbl),
+                ("p2p_pairs", _i64), ("m2l_links", _i64), ("ms", _dbl * N_STAGES)]
+
+    def as_dict(self):
+        return {"L": self.L, "p": self.p, "nnear": self.nnear, "D": self.D,
+                "p2p_pairs": self.p2p_pairs, "m2l_links": self.m2l_links,
+                "ms": {k: self.ms[i] for i, k in enumerate(STAGES) if self.ms[i] >= 0}}
+
+
 _SIGS = {
     "phf_last_error": ([], ctypes.c_char_p),
     "phf_init": ([], ctypes.c_int),
     "phf_device_info": ([ctypes.c_char_p, _i64, ctypes.POINTER(_i32), ctypes.POINTER(_i64)], ctypes.c_int),
     "phf_set_timing": ([_i32], ctypes.c_int),
+    "phf_synchronize": ([], ctypes.c_int),
     "phf_last_stage_times": ([_ptr], ctypes.c_int),
     "phf_vv2par_gauss": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr, _ptr], ctypes.c_int),
     "phf_fmm2d": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _i32, _dbl, _dbl, _dbl,
                    ctypes.POINTER(FmmOpts), _ptr, _ptr], ctypes.c_int),
+    "phf_system_create": ([_ptr, _ptr, _ptr, _i64, ctypes.POINTER(_ptr)], ctypes.c_int),
+    "phf_system_destroy": ([_ptr], ctypes.c_int),
+    "phf_system_size": ([_ptr, ctypes.POINTER(_i64)], ctypes.c_int),
+    "phf_system_get": ([_ptr, _ptr, _ptr, _ptr], ctypes.c_int),
+    "phf_system_set": ([_ptr, _ptr, _ptr, _ptr, _i64], ctypes.c_int),
+    "phf_system_velocity": ([_ptr, ctypes.POINTER(VelParams), _ptr, _ptr, ctypes.POINTER(EvalInfo)],
+                            ctypes.c_int),
+    "phf_system_get_velocity": ([_ptr, _ptr, _ptr], ctypes.c_int),
     "phf_tree_build": ([_ptr, _ptr, _i64, _i32, _ptr, _ptr, _ptr, ctypes.POINTER(_dbl),
                         ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)], ctypes.c_int),
 }

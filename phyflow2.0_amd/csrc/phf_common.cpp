@@ -244,6 +244,15 @@ int phf_device_info(char *name, int64_t name_len, int32_t *num_cu,
     return PHF_OK;
 }
 
+int phf_synchronize(void)
+{
+    phf::clear_error();
+    PHF_TRY(phf::Context::get().init());
+    PHF_HIP(hipSetDevice(phf::Context::get().device()));
+    PHF_HIP(hipDeviceSynchronize());
+    return PHF_OK;
+}
+
 int phf_set_timing(int32_t enabled)
 {
     phf::set_timing(enabled != 0);

@@ -444,6 +444,74 @@ __global__ __launch_bounds__(64) void p2p_l2p_f64(
     }
 }
 
+// ------------------------------------------------------------ counters --
+// Algorithmic work of one evaluation, for roofline accounting: P2P pairs
+// (targets of a leaf x sources of its near leaves) and M2L links.
+__global__ void count_p2p_pairs(const int32_t *__restrict__ start_s,
+                                const int32_t *__restrict__ start_t, int Lvl,
+                                const int2 *__restrict__ near, int nnear,
+                                unsigned long long *__restrict__ out)
+{
+    const int64_t nleaf = int64_t(1) << (2 * Lvl);
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nleaf) return;
+    const int64_t nt = start_t[b + 1] - start_t[b];
+    if (nt == 0) return;
+    const int bx = (int)compact_bits((uint32_t)b), by = (int)compact_bits((uint32_t)b >> 1);
+    const int n = 1 << Lvl;
+    int64_t ns = 0;
+    for (int q = 0; q < nnear; ++q) {
+        const int sx = bx + near[q].x, sy = by + near[q].y;
+        if (sx < 0 || sy < 0 || sx >= n || sy >= n) continue;
+        const uint32_t sb = morton2((uint32_t)sx, (uint32_t)sy);
+        ns += start_s[sb + 1] - start_s[sb];
+    }
+    atomicAdd(out, (unsigned long long)(nt * ns));
+}
+
+__global__ void count_m2l_links(int l, const int32_t *__restrict__ start_s,
+                                const int32_t *__restrict__ start_t, int shift,
+                                const int2 *__restrict__ il_tau,
+                                const int32_t *__restrict__ il_off,
+                                unsigned long long *__restrict__ out)
+{
+    const int n = 1 << l;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * n) return;
+    const uint32_t box = (uint32_t)i;
+    if (box_count(start_t, box, shift) == 0) return;
+    const int bx = (int)compact_bits(box), by = (int)compact_bits(box >> 1);
+    const int par = (bx & 1) | ((by & 1) << 1);
+    unsigned long long c = 0;
+    for (int t = il_off[par]; t < il_off[par + 1]; ++t) {
+        const int sx = bx + il_tau[t].x, sy = by + il_tau[t].y;
+        if (sx < 0 || sy < 0 || sx >= n || sy >= n) continue;
+        if (box_count(start_s, morton2((uint32_t)sx, (uint32_t)sy), shift) > 0) ++c;
+    }
+    if (c) atomicAdd(out, c);
+}
+
+// (qr, qi) -> velocity (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
+__global__ void q_to_velocity(const double *__restrict__ qr, const double *__restrict__ qi,
+                              int64_t n, double *__restrict__ vx, double *__restrict__ vy)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double two_pi = 2.0 * 3.141592653589793;
+    const double a = qr[i], b = qi[i];
+    vx[i] = -b / two_pi;
+    vy[i] = -a / two_pi;
+}
+
+int q_to_velocity_device(Lane *L, const double *qr, const double *qi, int64_t n, double *vx,
+                         double *vy)
+{
+    if (n <= 0) return PHF_OK;
+    q_to_velocity<<<div_up(n, 256), 256, 0, L->stream>>>(qr, qi, n, vx, vy);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
 // ------------------------------------------------------------- host ------
 namespace {
 
@@ -617,7 +685,7 @@ static int m2l_dispatch(Lane *L, int l, int p1, const void *M, void *B, const in
 template <class T, class TC>
 static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lists &ls,
                          const SortedPoints &S, const SortedPoints &Tg, StageTimer &tm,
-                         bool *used)
+                         bool *used, FmmInfo *info)
 {
     using V = typename C2<T>::type;
     using VC = typename C2<TC>::type;
@@ -720,6 +788,13 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         blob.add(il_all, &d_il);
         blob.add(il_offs, &d_iloff);
         PHF_TRY(blob.send(L));
+        if (info && info->want_counts) {
+            for (int l = 2; l <= Lv; ++l)
+                count_m2l_links<<<div_up(int64_t(1) << (2 * l), 256), 256, 0, L->stream>>>(
+                    l, S.start, Tg.start, 2 * (Lv - l), d_il, d_iloff + il_base[l],
+                    info->d_counts + 1);
+            PHF_HIP(hipGetLastError());
+        }
 
         // P2M
         tm.begin(PHF_STAGE_P2M);
@@ -764,6 +839,11 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     }
 
     if (!d_near) PHF_TRY(blob.send(L));
+    if (info && info->want_counts) {
+        count_p2p_pairs<<<div_up(int64_t(1) << (2 * Lv), 256), 256, 0, L->stream>>>(
+            S.start, Tg.start, Lv, d_near, (int)ls.near_leaf.size(), info->d_counts);
+        PHF_HIP(hipGetLastError());
+    }
     // P2P + L2P
     tm.begin(PHF_STAGE_P2P);
     const uint32_t nleaf = 1u << (2 * Lv);
@@ -816,12 +896,16 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         info->p = a.p;
         info->nnear = (int)ls.near_leaf.size();
         info->D = dom.D;
+        if (info->want_counts) {
+            PHF_ALLOC(info->d_counts, L->arena, unsigned long long, 2);
+            PHF_HIP(hipMemsetAsync(info->d_counts, 0, 2 * sizeof(unsigned long long), L->stream));
+        }
     }
     if (a.prec == PHF_PREC_F32C) {
-        if (a.p > 16) return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used);
-        return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used);
+        if (a.p > 16) return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used, info);
+        return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used, info);
     }
-    return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used);
+    return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used, info);
 }
 
 }  // namespace phf

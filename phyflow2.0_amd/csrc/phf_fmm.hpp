@@ -19,10 +19,14 @@ struct FmmArgs {
 struct FmmInfo {
     int L = 0, p = 0, nnear = 0;
     double D = 0.0;
+    bool want_counts = false;                // fill d_counts = {P2P pairs, M2L links}
+    unsigned long long *d_counts = nullptr;  // device, arena-owned
 };
 
 int fmm_order_from_tol(double tol);
 int fmm_level(int64_t nb, int leaf, int ndirect);
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info);
+int q_to_velocity_device(Lane *L, const double *qr, const double *qi, int64_t n, double *vx,
+                         double *vy);
 
 }  // namespace phf

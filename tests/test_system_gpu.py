@@ -1,0 +1,35 @@
+"""Resident blob systems: same results as the one-shot entry points."""
+import math
+
+import numpy as np
+import pytest
+
+from phyflow_amd import workloads as W
+from phyflow_amd.cpp.blobs.kernels import gauss
+from phyflow_amd.system import METHOD_DIRECT, METHOD_FMM, BlobSystem, vel_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", [gauss.PREC_F64, gauss.PREC_F32C])
+def test_system_matches_one_shot(prec, O):
+    s = W.uniform_system(40000, seed=3)
+    xopt, cutoff = W.oseen_params(s["sigma"])
+    with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
+        vx, vy = sysm.velocity(vel_params(s["sigma"], prec=prec, order=16, leaf=64), info=True)
+        info = sysm.last_info
+        assert info["p2p_pairs"] > 0 and info["L"] >= 3
+        qr, qi = gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"], s["y"], 35, xopt, cutoff, 1e-6,
+                                 prec=prec, order=16, leaf=64)
+        assert np.array_equal(vx, -qi / (2 * math.pi)) and np.array_equal(vy, -qr / (2 * math.pi))
+        d = sysm.velocity(vel_params(s["sigma"], method=METHOD_DIRECT, prec=prec))
+        ref = O.vv2par_gauss(s["x"], s["y"], s["w"], s["x"], s["y"], 2 * s["sigma"] ** 2)
+        assert O.rel_l2(d, ref) <= (1e-12 if prec == gauss.PREC_F64 else 1e-6)
+        # fetch later, and resize the system
+        sysm.velocity(vel_params(s["sigma"], prec=prec), fetch=False)
+        a = sysm.last_velocity()
+        assert a[0].size == 40000
+        sysm.set(s["x"][:100], s["y"][:100], s["w"][:100])
+        assert sysm.n == 100
+        x, y, w = sysm.get()
+        assert np.array_equal(x, s["x"][:100])
