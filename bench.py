@@ -53,6 +53,9 @@ def timed_steps(sysm, params, warmup, steps):
         p2p_ms.append(info["ms"].get("p2p", float("nan")))
     _native.lib().phf_synchronize()
     wall = time.perf_counter() - t0
+    # work counts from one extra (untimed) evaluation
+    sysm.velocity(params, fetch=False, info=True, count_work=True)
+    info = dict(info, p2p_pairs=sysm.last_info["p2p_pairs"], m2l_links=sysm.last_info["m2l_links"])
     return wall, np.array(lat), np.array(p2p_ms), info
 
 

@@ -119,6 +119,29 @@ int phf_tree_build(const double *x, const double *y, int64_t n, int32_t L,
                    uint32_t *keys, int32_t *perm, int32_t *start, double *x0,
                    double *y0, double *inv_s);
 
+/* ---- (c) M4' remeshing --------------------------------------------------
+ * m4kernel_1d(x, c) -> w replaces
+ *   src/cpp/blobs/c++/interpolation_kernels_1d/_interpolation_kernels_1d.cpp:21-93.
+ * phf_regrid_m4 replaces Regrid(xBlob, yBlob, wBlob, sigma, overlap,
+ *   xBounds, yBounds, interpKernel=0, c) of
+ *   src/python/blobs/base/regrid/_regridding.py:36-222 (x0 = xBounds[0],
+ *   y0 = yBounds[0]; h = sigma*overlap).  Output: every touched lattice cell
+ *   (zero-weight ones included) in row-major (i, then j) order with
+ *   x = i*h + h/2 + x0; integer cell indices are returned too.  Sums are
+ *   float64, deterministic, without atomics.  The result is a library-owned
+ *   blob set (its size is data-dependent). */
+int phf_m4kernel_1d(const double *x, int64_t n, double c, double *w);
+
+typedef struct phf_blobset_s phf_blobset;
+int phf_regrid_m4(const double *x, const double *y, const double *w, int64_t n, double sigma,
+                  double overlap, double x0, double y0, double c, phf_blobset **out);
+int64_t phf_blobset_size(const phf_blobset *b);
+/* Copies the blob set into caller arrays of phf_blobset_size() entries
+ * (any pointer may be NULL). */
+int phf_blobset_copy(const phf_blobset *b, double *x, double *y, double *w, int64_t *i,
+                     int64_t *j);
+void phf_blobset_free(phf_blobset *b);
+
 /* ---- resident blob systems -----------------------------------------------
  * A phf_system keeps one blob set (x, y, w) resident in HBM with its own HIP
  * stream and workspace, so repeated evaluations make no host copies and no
@@ -150,9 +173,11 @@ typedef struct {
 } phf_vel_params;
 
 /* Work and timing of the last evaluation (stage ms are HIP-event times on
- * the system's stream; -1 for stages not run). */
+ * the system's stream; -1 for stages not run).  Set flags = 1 on input to
+ * also count the FMM work (p2p_pairs, m2l_links; adds small kernels). */
+#define PHF_INFO_COUNT_WORK 1
 typedef struct {
-    int32_t L, p, nnear, pad;
+    int32_t L, p, nnear, flags;
     double D;
     int64_t p2p_pairs; /* FMM: near-field pairs; direct: nb*nt */
     int64_t m2l_links;

@@ -54,7 +54,7 @@ class VelParams(ctypes.Structure):
 
 
 class EvalInfo(ctypes.Structure):
-    _fields_ = [("L", _i32), ("p", _i32), ("nnear", _i32), ("pad", _i32), ("D", _dbl),
+    _fields_ = [("L", _i32), ("p", _i32), ("nnear", _i32), ("flags", _i32), ("D", _dbl),
                 ("p2p_pairs", _i64), ("m2l_links", _i64), ("ms", _dbl * N_STAGES)]
 
     def as_dict(self):
@@ -81,6 +81,12 @@ _SIGS = {
     "phf_system_velocity": ([_ptr, ctypes.POINTER(VelParams), _ptr, _ptr, ctypes.POINTER(EvalInfo)],
                             ctypes.c_int),
     "phf_system_get_velocity": ([_ptr, _ptr, _ptr], ctypes.c_int),
+    "phf_m4kernel_1d": ([_ptr, _i64, _dbl, _ptr], ctypes.c_int),
+    "phf_regrid_m4": ([_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, ctypes.POINTER(_ptr)],
+                      ctypes.c_int),
+    "phf_blobset_size": ([_ptr], _i64),
+    "phf_blobset_copy": ([_ptr, _ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),
+    "phf_blobset_free": ([_ptr], None),
     "phf_tree_build": ([_ptr, _ptr, _i64, _i32, _ptr, _ptr, _ptr, ctypes.POINTER(_dbl),
                         ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)], ctypes.c_int),
 }

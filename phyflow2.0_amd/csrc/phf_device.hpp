@@ -12,6 +12,10 @@ constexpr int kWave = 64;
 enum PairKind : int {
     PAIR_GAUSS_EPS = 0,  // vv2parGauss: r2 += 1e-12, s = w/r2*(1-exp(-r2/ks))
     PAIR_OSEEN = 1,      // fmm2d: P = w/r2 * (r2 < c2 ? -expm1(-shape r2)*scale : 1)
+    // Oseen in float32 when the cutoff branch is exact without the select:
+    // scale rounds to 1.0f and exp(-shape c2) < 2^-25, so 1 - e == 1.0f for
+    // every r2 >= c2 (the default cutoff 5 xopt gives exp(-31.4)).
+    PAIR_OSEEN_NOCUT = 2,
 };
 
 // Parameters of the pair kernel, prepared on the host in float64.
@@ -21,6 +25,7 @@ struct PairParams {
     double c2;        // cutoff^2 (Oseen)
     double scale;     // -1/expm1(-shape c2) (Oseen)
     float eps_f, nlog2e_ks_f, c2_f, scale_f;  // float32 forms
+    int f32_nocut;    // Oseen cutoff branch is a no-op in float32
 };
 
 // Error-free transformation: (hi, lo) += x, Knuth's TwoSum.
@@ -80,20 +85,26 @@ __host__ __device__ __forceinline__ uint32_t morton2(uint32_t ix, uint32_t iy)
 //   Gaussian (vv2parGauss_C_cpu.c:144-146): P = w/(r2+eps) * (1 - exp(-(r2+eps)/ks))
 //   Oseen (direct.cpp:1428-1485):           P = w/r2 * (r2 < c2 ? -expm1(-r2/ks)*scale : 1),
 //                                            P = 0 for r2 == 0.
-// float32: r2 is clamped to >= 1e-30 so a coincident pair gives
-// 1 - exp2(-tiny) = 0 exactly and P = 0 without a select.
+// float32: the Oseen r2 carries +1e-30 so a coincident pair gives
+// 1 - exp2(-tiny) = 0 exactly and P = 0 without a select; w (1 - e) is one
+// FMA.  Per pair: 9 VALU + 2 transcendental instructions.
 template <int KIND>
 __device__ __forceinline__ float pair_f32(float dx, float dy, float w, const PairParams &pp)
 {
     if (KIND == PAIR_GAUSS_EPS) {
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, pp.eps_f));
         const float e = exp2_f32(r2 * pp.nlog2e_ks_f);
-        return (w * rcp_f32(r2)) * (1.0f - e);
-    } else {
-        const float r2 = __builtin_fmaxf(__builtin_fmaf(dx, dx, dy * dy), 1e-30f);
+        return __builtin_fmaf(-w, e, w) * rcp_f32(r2);
+    } else if (KIND == PAIR_OSEEN_NOCUT) {
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, 1e-30f));
         const float e = exp2_f32(r2 * pp.nlog2e_ks_f);
-        const float om = r2 < pp.c2_f ? __builtin_fmaf(-pp.scale_f, e, pp.scale_f) : 1.0f;
-        return (w * rcp_f32(r2)) * om;
+        return __builtin_fmaf(-w, e, w) * rcp_f32(r2);
+    } else {
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, 1e-30f));
+        const float e = exp2_f32(r2 * pp.nlog2e_ks_f);
+        const float ws = w * pp.scale_f;
+        const float num = r2 < pp.c2_f ? __builtin_fmaf(-ws, e, ws) : w;
+        return num * rcp_f32(r2);
     }
 }
 

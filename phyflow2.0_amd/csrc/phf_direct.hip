@@ -60,6 +60,7 @@ PairParams make_oseen_params(double xopt, double cutoff)
     p.nlog2e_ks_f = (float)(-1.4426950408889634074 * p.inv_ks);
     p.c2_f = (float)p.c2;
     p.scale_f = (float)p.scale;
+    p.f32_nocut = (p.scale_f == 1.0f && p.c2 * p.inv_ks * 1.4426950408889634074 > 25.5) ? 1 : 0;
     return p;
 }
 
@@ -286,8 +287,9 @@ static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *
                                                                 pp, part);
     }
     PHF_HIP(hipGetLastError());
-    direct_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(part, nb == 0 ? 1 : (int)nchunk, nt,
-                                                          KIND, o0, o1);
+    direct_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(
+        part, nb == 0 ? 1 : (int)nchunk, nt, KIND == PAIR_GAUSS_EPS ? PAIR_GAUSS_EPS : PAIR_OSEEN,
+        o0, o1);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -299,6 +301,8 @@ int direct_device(Lane *L, int kind, int prec, const PairParams &pp, const doubl
     if (nt == 0) return PHF_OK;
     if (kind == PAIR_GAUSS_EPS)
         return launch_direct<PAIR_GAUSS_EPS>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
+    if (prec == PHF_PREC_F32C && pp.f32_nocut)
+        return launch_direct<PAIR_OSEEN_NOCUT>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
     return launch_direct<PAIR_OSEEN>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
 }
 

@@ -848,9 +848,14 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     tm.begin(PHF_STAGE_P2P);
     const uint32_t nleaf = 1u << (2 * Lv);
     if (sizeof(T) == 4) {
-        p2p_l2p_f32c<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
-            (const float4 *)src, S.start, (const float2 *)tgt, Tg.start, perm_t, Lv, d_near,
-            (int)ls.near_leaf.size(), pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr, a.qi);
+        if (pp.f32_nocut)
+            p2p_l2p_f32c<PAIR_OSEEN_NOCUT><<<nleaf, 64, 0, L->stream>>>(
+                (const float4 *)src, S.start, (const float2 *)tgt, Tg.start, perm_t, Lv, d_near,
+                (int)ls.near_leaf.size(), pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr, a.qi);
+        else
+            p2p_l2p_f32c<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
+                (const float4 *)src, S.start, (const float2 *)tgt, Tg.start, perm_t, Lv, d_near,
+                (int)ls.near_leaf.size(), pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr, a.qi);
     } else {
         p2p_l2p_f64<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
             (const double4 *)src, S.start, (const double2 *)tgt, Tg.start, perm_t, Lv, d_near,

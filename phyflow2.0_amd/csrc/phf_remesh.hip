@@ -1,0 +1,496 @@
+// phf_remesh.hip -- deterministic M4' remeshing on gfx950.
+//
+// Replaces Regrid (src/python/blobs/base/regrid/_regridding.py:36-222) and
+// m4kernel_1d (src/cpp/blobs/c++/interpolation_kernels_1d/
+// _interpolation_kernels_1d.cpp:21-93).
+//
+// Reference semantics (restated):
+//   iL = int64(ceil(((x - x0) - 2h + h/2)/h)) - 1, likewise jL     (:157-158)
+//   stencil slot s = 0..15 -> cell (iL + s%4, jL + s/4)              (:170-172)
+//   value = (M4'(ux) * M4'(uy)) * w, u = ((i + 0.5) + x0/h) - x/h    (:183-184)
+//   duplicates summed, zero-weight touched cells kept, output in
+//   row-major (i, then j) order, x_new = (i h + h/2) + x0            (:200-218)
+//
+// GPU design (no float atomics, one sort of n keys instead of 16n):
+//   1. each blob's base cell (iL, jL) -> bounds of the touched range
+//   2. stable radix sort of blobs by base-cell key (row-major in range)
+//   3. a cell is touched iff one of its 16 base cells (i-a, j-b),
+//      a, b in 0..3, holds a blob; exclusive scan of touched flags gives
+//      each output cell its row-major slot
+//   4. each touched cell sums its contributions slot-major (s = 0..15),
+//      blobs of one base cell in ascending input index: exactly the order
+//      of a stable sort of the reference's COO entries (entry e = s*n + b),
+//      so the float64 sums are reproducible run to run and match the CPU
+//      restatement.
+// Index and weight arithmetic runs in float64 with contraction disabled,
+// so cell indices and output coordinates are bit-exact to the reference
+// formulas; the cube |u|^3 is formed in double-double and rounded once (the
+// correctly rounded pow(|u|, 3.0) of the reference).
+#include "phf_remesh.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace phf {
+
+#pragma clang fp contract(off)
+
+__device__ __forceinline__ double cube_rn(double a)
+{
+    // a^3 rounded once: a*a as (p + e) exactly, then (p + e) * a.
+    const double p = a * a;
+    const double e = __builtin_fma(a, a, -p);
+    const double t = p * a;
+    const double te = __builtin_fma(p, a, -t);
+    return t + (te + e * a);
+}
+
+// M4' (_interpolation_kernels_1d.cpp:76-87), same operation order.
+__host__ __device__ __forceinline__ double m4p_dev(double x, double c, double c2)
+{
+    const double ax = fabs(x);
+    if ((ax > 1.0) && (ax < 2.0)) {
+        const double t = 2.0 - ax;
+        const double t2 = t * t;
+        return 0.5 * t2 * (1.0 - ax) - c2 * t2 * (1.0 - 2.0 * ax);
+    } else if ((ax >= 0.0) && (ax < 1.0)) {
+#ifdef __HIP_DEVICE_COMPILE__
+        const double a3 = cube_rn(ax);
+#else
+        const double a3 = std::pow(ax, 3.0);
+#endif
+        const double x2 = x * x;
+        return 1.0 - 2.5 * x2 + 1.5 * a3 - c2 * (2.0 - 9.0 * x2 + 6.0 * a3);
+    }
+    return 0.0;
+}
+
+__global__ void m4kernel_1d_kernel(const double *__restrict__ x, int64_t n, double c, double c2,
+                                   double *__restrict__ w)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = m4p_dev(x[i], c, c2);
+}
+
+// ---------------------------------------------------------- base cells --
+__device__ __forceinline__ int64_t base_index(double x, double x0, double h)
+{
+    const double h2 = 2.0 * h, hh = 0.5 * h;
+    return (int64_t)ceil((((x - x0) - h2) + hh) / h) - 1;
+}
+
+constexpr int RM_THREADS = 256;
+
+// per-block min/max of (iL, jL): longlong4 {imin, imax, jmin, jmax}
+__global__ __launch_bounds__(RM_THREADS) void base_bounds_partial(
+    const double *__restrict__ x, const double *__restrict__ y, int64_t n, double x0, double y0,
+    double h, longlong4 *__restrict__ part)
+{
+    __shared__ longlong4 sh[RM_THREADS];
+    long long imin = LLONG_MAX, imax = LLONG_MIN, jmin = LLONG_MAX, jmax = LLONG_MIN;
+    for (int64_t b = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x; b < n;
+         b += (int64_t)gridDim.x * RM_THREADS) {
+        const long long i = base_index(x[b], x0, h), j = base_index(y[b], y0, h);
+        imin = i < imin ? i : imin;
+        imax = i > imax ? i : imax;
+        jmin = j < jmin ? j : jmin;
+        jmax = j > jmax ? j : jmax;
+    }
+    sh[threadIdx.x] = make_longlong4(imin, imax, jmin, jmax);
+    __syncthreads();
+    for (int s = RM_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            longlong4 a = sh[threadIdx.x], c = sh[threadIdx.x + s];
+            sh[threadIdx.x] = make_longlong4(a.x < c.x ? a.x : c.x, a.y > c.y ? a.y : c.y,
+                                             a.z < c.z ? a.z : c.z, a.w > c.w ? a.w : c.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(RM_THREADS) void base_bounds_final(const longlong4 *__restrict__ part,
+                                                                int np, longlong4 *__restrict__ out)
+{
+    __shared__ longlong4 sh[RM_THREADS];
+    longlong4 acc = make_longlong4(LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN);
+    for (int i = threadIdx.x; i < np; i += RM_THREADS) {
+        longlong4 c = part[i];
+        acc = make_longlong4(acc.x < c.x ? acc.x : c.x, acc.y > c.y ? acc.y : c.y,
+                             acc.z < c.z ? acc.z : c.z, acc.w > c.w ? acc.w : c.w);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = RM_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            longlong4 a = sh[threadIdx.x], c = sh[threadIdx.x + s];
+            sh[threadIdx.x] = make_longlong4(a.x < c.x ? a.x : c.x, a.y > c.y ? a.y : c.y,
+                                             a.z < c.z ? a.z : c.z, a.w > c.w ? a.w : c.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// key = (iL - imin) * nj + (jL - jmin), the blob's base cell in the range
+__global__ void base_keys(const double *__restrict__ x, const double *__restrict__ y, int64_t n,
+                          double x0, double y0, double h, long long imin, long long jmin,
+                          long long nj, uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const long long i = base_index(x[b], x0, h) - imin, j = base_index(y[b], y0, h) - jmin;
+    keys[b] = (uint32_t)(i * nj + j);
+    vals[b] = (int32_t)b;
+}
+
+// sorted blob data: (x/h, y/h, w)
+__global__ void gather_scaled(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ w, const int32_t *__restrict__ perm,
+                              int64_t n, double h, double3 *__restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t b = perm[k];
+    out[k] = make_double3(x[b] / h, y[b] / h, w[b]);
+}
+
+// Cells of the output range (ni x nj, range indices offset by imin, jmin;
+// base-cell range is (ni-3) x (nj-3)).  touched = any of the 16 base cells
+// (i-a, j-b) non-empty.
+__global__ void touched_flags(const int32_t *__restrict__ cstart, long long bi, long long bj,
+                              long long ni, long long nj, uint32_t *__restrict__ flag)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ni * nj) return;
+    const long long i = c / nj, j = c - (c / nj) * nj;
+    uint32_t t = 0;
+    for (int s = 0; s < 16 && !t; ++s) {
+        const long long ii = i - (s & 3), jj = j - (s >> 2);
+        if (ii < 0 || jj < 0 || ii >= bi || jj >= bj) continue;
+        const int64_t k = ii * bj + jj;
+        t = cstart[k + 1] > cstart[k];
+    }
+    flag[c] = t;
+}
+
+__global__ void gather_sum(const int32_t *__restrict__ cstart, const double3 *__restrict__ sb,
+                           const uint32_t *__restrict__ flag, const uint32_t *__restrict__ slot,
+                           long long bi, long long bj, long long ni, long long nj,
+                           long long imin, long long jmin, double h, double x0, double y0,
+                           double x0h, double y0h, double c, double c2,
+                           double *__restrict__ xo, double *__restrict__ yo,
+                           double *__restrict__ wo, int64_t *__restrict__ io,
+                           int64_t *__restrict__ jo)
+{
+    const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cidx >= ni * nj || !flag[cidx]) return;
+    const long long i = cidx / nj, j = cidx - (cidx / nj) * nj;
+    const long long I = i + imin, J = j + jmin;  // absolute cell indices
+    const double ui = ((double)I + 0.5) + x0h;
+    const double uj = ((double)J + 0.5) + y0h;
+    double acc = 0.0;
+    bool first = true;
+    for (int s = 0; s < 16; ++s) {
+        const long long ii = i - (s & 3), jj = j - (s >> 2);
+        if (ii < 0 || jj < 0 || ii >= bi || jj >= bj) continue;
+        const int64_t k = ii * bj + jj;
+        for (int32_t q = cstart[k]; q < cstart[k + 1]; ++q) {
+            const double3 p = sb[q];
+            const double v = (m4p_dev(ui - p.x, c, c2) * m4p_dev(uj - p.y, c, c2)) * p.z;
+            acc = first ? v : acc + v;
+            first = false;
+        }
+    }
+    const uint32_t o = slot[cidx];
+    const double hh = 0.5 * h;
+    wo[o] = acc;
+    xo[o] = ((double)I * h + hh) + x0;
+    yo[o] = ((double)J * h + hh) + y0;
+    if (io) io[o] = I;
+    if (jo) jo[o] = J;
+}
+
+int remesh_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
+                  double sigma, double overlap, double x0, double y0, double c, RemeshOut *out)
+{
+    out->m = 0;
+    if (n == 0) return PHF_OK;
+    const double h = sigma * overlap;
+    // 1. bounds of base cells
+    const int nblk = (int)std::min<int64_t>(1024, (n + RM_THREADS * 8 - 1) / (RM_THREADS * 8));
+    longlong4 *part, *bnd;
+    PHF_ALLOC(part, L->arena, longlong4, (size_t)nblk);
+    PHF_ALLOC(bnd, L->arena, longlong4, 1);
+    base_bounds_partial<<<nblk, RM_THREADS, 0, L->stream>>>(x, y, n, x0, y0, h, part);
+    base_bounds_final<<<1, RM_THREADS, 0, L->stream>>>(part, nblk, bnd);
+    PHF_HIP(hipGetLastError());
+    longlong4 hb;
+    PHF_HIP(hipMemcpyAsync(&hb, bnd, sizeof(hb), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    const long long bi = hb.y - hb.x + 1, bj = hb.w - hb.z + 1;  // base-cell range
+    const long long ni = bi + 3, nj = bj + 3;                    // touched-cell range
+    if (!(bi > 0 && bj > 0) || (double)ni * (double)nj > 4.0e9 ||
+        (double)ni * (double)nj > 64.0 * (double)n + 1e6) {
+        set_error("remesh cell range %lld x %lld too large for %lld blobs", ni, nj, (long long)n);
+        return PHF_ERR_UNSUPPORTED;
+    }
+    // 2. sort blobs by base cell
+    uint32_t *k0, *k1;
+    int32_t *v0, *v1;
+    PHF_ALLOC(k0, L->arena, uint32_t, (size_t)n);
+    PHF_ALLOC(k1, L->arena, uint32_t, (size_t)n);
+    PHF_ALLOC(v0, L->arena, int32_t, (size_t)n);
+    PHF_ALLOC(v1, L->arena, int32_t, (size_t)n);
+    base_keys<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, x0, y0, h, hb.x, hb.z, bj, k0, v0);
+    PHF_HIP(hipGetLastError());
+    int key_bits = 1;
+    while (key_bits < 32 && (1ull << key_bits) < (unsigned long long)(bi * bj)) ++key_bits;
+    uint32_t *ks;
+    int32_t *vs;
+    PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, n, key_bits, &ks, &vs));
+    int32_t *cstart;
+    PHF_ALLOC(cstart, L->arena, int32_t, (size_t)(bi * bj + 1));
+    PHF_TRY(leaf_starts_device(L, ks, n, bi * bj, cstart));
+    double3 *sb;
+    PHF_ALLOC(sb, L->arena, double3, (size_t)n);
+    gather_scaled<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, w, vs, n, h, sb);
+    // 3. touched cells and their output slots
+    const int64_t ncell = ni * nj;
+    uint32_t *flag, *slot;
+    PHF_ALLOC(flag, L->arena, uint32_t, (size_t)ncell);
+    PHF_ALLOC(slot, L->arena, uint32_t, (size_t)ncell);
+    touched_flags<<<div_up(ncell, 256), 256, 0, L->stream>>>(cstart, bi, bj, ni, nj, flag);
+    PHF_HIP(hipGetLastError());
+    PHF_TRY(device_exclusive_scan(L, flag, ncell, slot));
+    uint32_t tail[2];
+    PHF_HIP(hipMemcpyAsync(&tail[0], slot + ncell - 1, 4, hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipMemcpyAsync(&tail[1], flag + ncell - 1, 4, hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    const int64_t m = (int64_t)tail[0] + tail[1];
+    // 4. outputs
+    double *xo, *yo, *wo;
+    int64_t *io = nullptr, *jo = nullptr;
+    if (out->dst_x) {
+        PHF_REQUIRE(m <= out->dst_cap, "remesh output of %lld blobs exceeds capacity %lld",
+                    (long long)m, (long long)out->dst_cap);
+        xo = out->dst_x;
+        yo = out->dst_y;
+        wo = out->dst_w;
+    } else {
+        PHF_ALLOC(xo, L->arena, double, (size_t)std::max<int64_t>(m, 1));
+        PHF_ALLOC(yo, L->arena, double, (size_t)std::max<int64_t>(m, 1));
+        PHF_ALLOC(wo, L->arena, double, (size_t)std::max<int64_t>(m, 1));
+    }
+    if (out->want_indices) {
+        PHF_ALLOC(io, L->arena, int64_t, (size_t)std::max<int64_t>(m, 1));
+        PHF_ALLOC(jo, L->arena, int64_t, (size_t)std::max<int64_t>(m, 1));
+    }
+    gather_sum<<<div_up(ncell, 256), 256, 0, L->stream>>>(
+        cstart, sb, flag, slot, bi, bj, ni, nj, hb.x, hb.z, h, x0, y0, x0 / h, y0 / h, c, c * c,
+        xo, yo, wo, io, jo);
+    PHF_HIP(hipGetLastError());
+    out->m = m;
+    out->x = xo;
+    out->y = yo;
+    out->w = wo;
+    out->i = io;
+    out->j = jo;
+    out->cells = ncell;
+    return PHF_OK;
+}
+
+int m4kernel_device(Lane *L, const double *x, int64_t n, double c, double *w)
+{
+    if (n <= 0) return PHF_OK;
+    m4kernel_1d_kernel<<<div_up(n, 256), 256, 0, L->stream>>>(x, n, c, c * c, w);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+// ------------------------------------------------------------ moments --
+// Float64 invariants sum w, sum w x, sum w y, sum w (x^2 + y^2): per-block
+// tree sums then one block over the partials, fixed order throughout.
+__global__ __launch_bounds__(RM_THREADS) void moments_partial(const double *__restrict__ x,
+                                                              const double *__restrict__ y,
+                                                              const double *__restrict__ w,
+                                                              int64_t n, double4 *__restrict__ part)
+{
+    __shared__ double4 sh[RM_THREADS];
+    double a = 0, b = 0, c = 0, d = 0;
+    for (int64_t i = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * RM_THREADS) {
+        const double wi = w[i], xi = x[i], yi = y[i];
+        a += wi;
+        b += wi * xi;
+        c += wi * yi;
+        d += wi * (xi * xi + yi * yi);
+    }
+    sh[threadIdx.x] = make_double4(a, b, c, d);
+    __syncthreads();
+    for (int s = RM_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double4 p = sh[threadIdx.x], q = sh[threadIdx.x + s];
+            sh[threadIdx.x] = make_double4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(RM_THREADS) void moments_final(const double4 *__restrict__ part,
+                                                            int np, double4 *__restrict__ out)
+{
+    __shared__ double4 sh[RM_THREADS];
+    double4 acc = make_double4(0, 0, 0, 0);
+    for (int i = threadIdx.x; i < np; i += RM_THREADS) {
+        double4 q = part[i];
+        acc = make_double4(acc.x + q.x, acc.y + q.y, acc.z + q.z, acc.w + q.w);
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = RM_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            double4 p = sh[threadIdx.x], q = sh[threadIdx.x + s];
+            sh[threadIdx.x] = make_double4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+int moments_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
+                   double4 *d_out)
+{
+    const int nblk = 256;
+    double4 *part;
+    PHF_ALLOC(part, L->arena, double4, nblk);
+    moments_partial<<<nblk, RM_THREADS, 0, L->stream>>>(x, y, w, n, part);
+    moments_final<<<1, RM_THREADS, 0, L->stream>>>(part, nblk, d_out);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+}  // namespace phf
+
+// ------------------------------------------------------------ C entry ----
+using namespace phf;
+
+struct phf_blobset_s {
+    int64_t m = 0;
+    std::vector<double> x, y, w;
+    std::vector<int64_t> i, j;
+};
+
+extern "C" {
+
+int phf_m4kernel_1d(const double *x, int64_t n, double c, double *w)
+{
+    clear_error();
+    PHF_REQUIRE(n >= 0, "negative size");
+    PHF_REQUIRE(n == 0 || (x && w), "NULL array");
+    PHF_REQUIRE(std::isfinite(c), "c must be finite");
+    if (n == 0) return PHF_OK;
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    double *dx, *dw;
+    PHF_TRY(upload(L, x, n, &dx));
+    PHF_ALLOC(dw, L->arena, double, (size_t)n);
+    PHF_TRY(m4kernel_device(L, dx, n, c, dw));
+    PHF_TRY(download(L, dw, n, w));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    return PHF_OK;
+}
+
+int phf_regrid_m4(const double *x, const double *y, const double *w, int64_t n, double sigma,
+                  double overlap, double x0, double y0, double c, phf_blobset **out)
+{
+    clear_error();
+    PHF_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    PHF_REQUIRE(n >= 0 && n < (int64_t(1) << 31), "n out of range (%lld)", (long long)n);
+    PHF_REQUIRE(std::isfinite(sigma) && sigma > 0 && std::isfinite(overlap) && overlap > 0,
+                "sigma and overlap must be finite and > 0");
+    PHF_REQUIRE(std::isfinite(x0) && std::isfinite(y0) && std::isfinite(c),
+                "bounds and c must be finite");
+    PHF_TRY(check_host_array(x, n, "xBlob"));
+    PHF_TRY(check_host_array(y, n, "yBlob"));
+    PHF_TRY(check_host_array(w, n, "wBlob"));
+    phf_blobset_s *res = new phf_blobset_s();
+    if (n == 0) {
+        *out = res;
+        return PHF_OK;
+    }
+    LaneGuard g;
+    if (g.rc != PHF_OK) {
+        delete res;
+        return g.rc;
+    }
+    Lane *L = g.lane;
+    int rc = [&]() -> int {
+        PHF_TRY(L->arena.reset());
+        StageTimer tm(L, timing_enabled());
+        bool used[PHF_STAGE_COUNT] = {};
+        used[PHF_STAGE_REMESH] = used[PHF_STAGE_TOTAL] = true;
+        tm.begin(PHF_STAGE_TOTAL);
+        double *dx, *dy, *dw;
+        PHF_TRY(upload(L, x, n, &dx));
+        PHF_TRY(upload(L, y, n, &dy));
+        PHF_TRY(upload(L, w, n, &dw));
+        tm.begin(PHF_STAGE_REMESH);
+        RemeshOut ro;
+        ro.want_indices = true;
+        PHF_TRY(remesh_device(L, dx, dy, dw, n, sigma, overlap, x0, y0, c, &ro));
+        tm.end(PHF_STAGE_REMESH);
+        res->m = ro.m;
+        res->x.resize(ro.m);
+        res->y.resize(ro.m);
+        res->w.resize(ro.m);
+        res->i.resize(ro.m);
+        res->j.resize(ro.m);
+        if (ro.m > 0) {
+            PHF_TRY(download(L, ro.x, ro.m, res->x.data()));
+            PHF_TRY(download(L, ro.y, ro.m, res->y.data()));
+            PHF_TRY(download(L, ro.w, ro.m, res->w.data()));
+            PHF_HIP(hipMemcpyAsync(res->i.data(), ro.i, sizeof(int64_t) * ro.m,
+                                   hipMemcpyDeviceToHost, L->stream));
+            PHF_HIP(hipMemcpyAsync(res->j.data(), ro.j, sizeof(int64_t) * ro.m,
+                                   hipMemcpyDeviceToHost, L->stream));
+        }
+        tm.end(PHF_STAGE_TOTAL);
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        record_stage_times(L, used);
+        return PHF_OK;
+    }();
+    if (rc != PHF_OK) {
+        delete res;
+        return rc;
+    }
+    *out = res;
+    return PHF_OK;
+}
+
+int64_t phf_blobset_size(const phf_blobset *b) { return b ? b->m : -1; }
+
+int phf_blobset_copy(const phf_blobset *b, double *x, double *y, double *w, int64_t *i,
+                     int64_t *j)
+{
+    clear_error();
+    PHF_REQUIRE(b != nullptr, "blob set is NULL");
+    if (b->m == 0) return PHF_OK;
+    if (x) std::memcpy(x, b->x.data(), sizeof(double) * b->m);
+    if (y) std::memcpy(y, b->y.data(), sizeof(double) * b->m);
+    if (w) std::memcpy(w, b->w.data(), sizeof(double) * b->m);
+    if (i) std::memcpy(i, b->i.data(), sizeof(int64_t) * b->m);
+    if (j) std::memcpy(j, b->j.data(), sizeof(int64_t) * b->m);
+    return PHF_OK;
+}
+
+void phf_blobset_free(phf_blobset *b) { delete b; }
+
+}  // extern "C"

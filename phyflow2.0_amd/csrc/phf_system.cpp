@@ -218,7 +218,8 @@ int phf_system_velocity(phf_system *h, const phf_vel_params *p, double *vx, doub
     bool used[PHF_STAGE_COUNT] = {};
     used[PHF_STAGE_TOTAL] = true;
     FmmInfo fi;
-    fi.want_counts = info != nullptr && p->method == PHF_METHOD_FMM && p->tol > 0;
+    fi.want_counts = info != nullptr && (info->flags & PHF_INFO_COUNT_WORK) &&
+                     p->method == PHF_METHOD_FMM && p->tol > 0;
     tm.begin(PHF_STAGE_TOTAL);
     PHF_TRY(system_velocity_device(L, s->x, s->y, s->w, s->n, *p, s->vx, s->vy, tm, used, &fi));
     if (vx && vy && s->n > 0) {
@@ -241,8 +242,9 @@ int phf_system_velocity(phf_system *h, const phf_vel_params *p, double *vx, doub
         info->p = fi.p;
         info->nnear = fi.nnear;
         info->D = fi.D;
-        info->p2p_pairs = p->method == PHF_METHOD_FMM && p->tol > 0 ? (int64_t)counts[0]
-                                                                     : s->n * s->n;
+        info->p2p_pairs = p->method == PHF_METHOD_FMM && p->tol > 0
+                              ? (fi.want_counts ? (int64_t)counts[0] : -1)
+                              : s->n * s->n;
         info->m2l_links = (int64_t)counts[1];
         for (int k = 0; k < PHF_STAGE_COUNT; ++k) {
             float ms = -1.0f;

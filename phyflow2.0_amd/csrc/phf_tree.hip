@@ -337,6 +337,13 @@ __global__ void leaf_starts_kernel(const uint32_t *__restrict__ skeys, int64_t n
     start[k] = (int32_t)lo;
 }
 
+int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start)
+{
+    leaf_starts_kernel<<<div_up(nkeys + 1, 256), 256, 0, L->stream>>>(skeys, n, nkeys, start);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
 int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
                         SortedPoints *out)
 {

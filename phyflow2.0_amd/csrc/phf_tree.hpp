@@ -38,6 +38,9 @@ struct SortedPoints {
     bool want_input_keys = false;
 };
 
+// start[k] = first position of sorted keys with key >= k, k = 0..nkeys.
+int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start);
+
 int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
                         SortedPoints *out);
 

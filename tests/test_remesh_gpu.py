@@ -1,0 +1,81 @@
+"""M4' remesh on the GPU vs the CPU restatement of the reference.
+
+Cell indices, cell set/order and output coordinates: bit-exact.
+Weights: float64, same summation order as a stable sort of the reference's
+COO entries; required relative difference <= 1e-15 of max|w| (the
+reference's pow(|u|,3) vs a double-double cube can differ by an ulp).
+"""
+import numpy as np
+import pytest
+
+from phyflow_amd import workloads as W
+from phyflow_amd.blobs.base.regrid import Regrid
+from phyflow_amd.cpp.blobs.remesh import m4kernel_1d
+
+pytestmark = pytest.mark.gpu
+
+
+def _cmp(x, y, w, sigma, overlap, x0, y0, c, O):
+    got = Regrid(x, y, w, sigma, overlap, [x0, 0], [y0, 0], c=c, return_indices=True)
+    ref = O.regrid_m4(x, y, w, sigma, overlap, x0, y0, c)
+    assert np.array_equal(got[3], ref[3]) and np.array_equal(got[4], ref[4])
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    scale = max(np.max(np.abs(ref[2])), 1e-300)
+    assert np.max(np.abs(got[2] - ref[2])) <= 1e-15 * scale
+    return got, ref
+
+
+def test_m4kernel_1d_matches(O):
+    x = np.concatenate([np.linspace(-2.5, 2.5, 10001), [0.0, 1.0, -1.0, 2.0, -2.0]])
+    for c in (0.0, 0.37):
+        g = m4kernel_1d(x, c)
+        r = O.m4kernel_1d(x, c)
+        assert np.max(np.abs(g - r)) <= 2e-16
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (7, 1), (5000, 2), (100000, 3)])
+def test_regrid_random(n, seed, O):
+    rng = np.random.default_rng(seed)
+    x, y = rng.uniform(-0.4, 0.6, (2, n))
+    w = rng.standard_normal(n)
+    sigma, overlap = 0.004, 1.25
+    h = sigma * overlap
+    _cmp(x, y, w, sigma, overlap, -1.5 * h, -1.5 * h, 0.0, O)
+
+
+def test_regrid_lattice_ties_and_diffusion(O):
+    # blobs exactly on cell centres and on cell edges (ceil ties)
+    h, x0 = 0.01, -0.015
+    i = np.arange(-20, 20)
+    X, Y = np.meshgrid(x0 + (i + 0.5) * h, x0 + i * h, indexing="ij")
+    x, y = X.ravel(), Y.ravel()
+    w = np.random.default_rng(8).standard_normal(x.size)
+    _cmp(x, y, w, h, 1.0, x0, x0, 0.0, O)
+    _cmp(x, y, w, h, 1.0, x0, x0, 0.21, O)  # Wee diffusion c
+
+
+def test_regrid_cylinder_annulus_conservation(O):
+    a = W.annulus_blobs(262144, sigma=0.004, seed=0)
+    sigma, overlap = 0.004, 1.25
+    h = sigma * overlap
+    got, ref = _cmp(a["x"], a["y"], a["w"], sigma, overlap, -1.5 * h, -1.5 * h, 0.0, O)
+    xo, yo, wo = got[:3]
+    for f in (lambda p, q: 1.0, lambda p, q: p, lambda p, q: q):
+        before = np.sum(a["w"] * f(a["x"], a["y"]))
+        after = np.sum(wo * f(xo, yo))
+        assert abs(after - before) <= 1e-13 * np.sum(np.abs(a["w"]))
+
+
+def test_regrid_deterministic():
+    rng = np.random.default_rng(11)
+    x, y = rng.random((2, 30000))
+    w = rng.standard_normal(30000)
+    a = Regrid(x, y, w, 0.01, 1.0, [-0.015, 0], [-0.015, 0])
+    b = Regrid(x, y, w, 0.01, 1.0, [-0.015, 0], [-0.015, 0])
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+
+
+def test_regrid_empty():
+    out = Regrid(np.empty(0), np.empty(0), np.empty(0), 0.1, 1.0, [0, 1], [0, 1])
+    assert all(v.size == 0 for v in out)

@@ -16,7 +16,8 @@ def test_system_matches_one_shot(prec, O):
     s = W.uniform_system(40000, seed=3)
     xopt, cutoff = W.oseen_params(s["sigma"])
     with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
-        vx, vy = sysm.velocity(vel_params(s["sigma"], prec=prec, order=16, leaf=64), info=True)
+        vx, vy = sysm.velocity(vel_params(s["sigma"], prec=prec, order=16, leaf=64), info=True,
+                              count_work=True)
         info = sysm.last_info
         assert info["p2p_pairs"] > 0 and info["L"] >= 3
         qr, qi = gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"], s["y"], 35, xopt, cutoff, 1e-6,
