@@ -142,6 +142,39 @@ int phf_blobset_copy(const phf_blobset *b, double *x, double *y, double *w, int6
                      int64_t *j);
 void phf_blobset_free(phf_blobset *b);
 
+/* ---- (d) vortex panels and the dense panel solve ---------------------------
+ * phf_panel_influence replaces assemble_influenceMatrix(xCP, yCP, xPS, yPS,
+ *   xPE, yPE, cosA, sinA) -> A[M,M] (row-major),
+ *   src/cpp/panels/cython/vortexPanel_Cython/_vortexPanel_Cython_cpu.pyx:81-233.
+ * phf_panel_induced_velocity replaces inducedVelocity(gamma, xEval, yEval,
+ *   xPS, yPS, xPE, yPE, cosA, sinA) -> (vx, vy), same file :243-391.
+ * phf_dense_solve replaces the solves of Panels.solve
+ *   (src/python/panels/__panelsClass.py:715-822): rows == n is the square
+ *   'fixed' system (scipy lu_factor/lu_solve, :961/:782); rows > n is the
+ *   'prescribed' least-squares system [A; L] x = [RHS; -gTotal]
+ *   (numpy.linalg.lstsq, :800-822).  B is row-major rows x n.  Float32 LU
+ *   with partial pivoting (MFMA trailing updates) + float64 iterative
+ *   refinement (normal equations with semi-normal float64 residuals when
+ *   rows > n).  info may be NULL. */
+int phf_panel_influence(const double *xCP, const double *yCP, const double *xs,
+                        const double *ys, const double *xe, const double *ye,
+                        const double *cosA, const double *sinA, int64_t m, double *A);
+int phf_panel_induced_velocity(const double *gamma, int64_t m, const double *xE,
+                               const double *yE, int64_t ne, const double *xs,
+                               const double *ys, const double *xe, const double *ye,
+                               const double *cosA, const double *sinA, double *vx, double *vy);
+
+typedef struct {
+    int32_t iterations; /* refinement sweeps */
+    int32_t converged;  /* 1: correction reached float64 resolution */
+    double lu_ms;       /* device time of the float32 factorisation */
+    double refine_ms;   /* device time of the refinement */
+    double total_ms;    /* wall time of the call */
+} phf_solve_info;
+
+int phf_dense_solve(const double *B, int64_t rows, int64_t n, const double *c, double *x,
+                    phf_solve_info *info);
+
 /* ---- resident blob systems -----------------------------------------------
  * A phf_system keeps one blob set (x, y, w) resident in HBM with its own HIP
  * stream and workspace, so repeated evaluations make no host copies and no

@@ -42,6 +42,11 @@ class FmmOpts(ctypes.Structure):
     _fields_ = [("order", _i32), ("leaf", _i32), ("prec", _i32), ("ws", _i32)]
 
 
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", _i32), ("converged", _i32), ("lu_ms", _dbl), ("refine_ms", _dbl),
+                ("total_ms", _dbl)]
+
+
 METHOD_FMM = 0
 METHOD_DIRECT = 1
 N_STAGES = len(STAGES)
@@ -87,6 +92,9 @@ _SIGS = {
     "phf_blobset_size": ([_ptr], _i64),
     "phf_blobset_copy": ([_ptr, _ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),
     "phf_blobset_free": ([_ptr], None),
+    "phf_panel_influence": ([_ptr] * 8 + [_i64, _ptr], ctypes.c_int),
+    "phf_panel_induced_velocity": ([_ptr, _i64, _ptr, _ptr, _i64] + [_ptr] * 8, ctypes.c_int),
+    "phf_dense_solve": ([_ptr, _i64, _i64, _ptr, _ptr, ctypes.POINTER(SolveInfo)], ctypes.c_int),
     "phf_tree_build": ([_ptr, _ptr, _i64, _i32, _ptr, _ptr, _ptr, ctypes.POINTER(_dbl),
                         ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)], ctypes.c_int),
 }

@@ -1,0 +1,623 @@
+// phf_panel.hip -- vortex-panel influence matrix, panel-induced velocity
+// and the dense panel solve on gfx950.
+//
+// Replaces
+//   assemble_influenceMatrix / inducedVelocity
+//     src/cpp/panels/cython/vortexPanel_Cython/_vortexPanel_Cython_cpu.pyx:81-233, 243-391
+//   Panels.solve, src/python/panels/__panelsClass.py:715-822:
+//     'fixed'      scipy lu_factor/lu_solve (LAPACK getrf/getrs)   :961, :782
+//     'prescribed' numpy.linalg.lstsq([A; L], [RHS; -gTotal])      :800-822
+//
+// Solve = LU factorisation in float32 with partial pivoting (blocked,
+// right-looking; the trailing update runs on MFMA, v_mfma_f32_16x16x4f32)
+// + iterative refinement with float64 residuals until the correction stalls
+// at float64 resolution.  Over-determined systems (the 'prescribed' problem)
+// are solved through the normal equations B^T B x = B^T c, refined with the
+// float64 semi-normal residual B^T (c - B x).
+#include "phf_panel.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+namespace phf {
+
+// ---------------------------------------------------- influence matrix ----
+// A[i*M + j]: tangential velocity at collocation point i induced by unit
+// strength on panel j (pyx:208-233, same operation order).
+__global__ void panel_influence_kernel(int64_t M, const double *__restrict__ xCP,
+                                       const double *__restrict__ yCP,
+                                       const double *__restrict__ xs, const double *__restrict__ ys,
+                                       const double *__restrict__ xe, const double *__restrict__ ye,
+                                       const double *__restrict__ cosA,
+                                       const double *__restrict__ sinA, double *__restrict__ A)
+{
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M * M) return;
+    const int64_t i = t / M, j = t - (t / M) * M;
+    const double inv_2pi = 1.0 / (2.0 * M_PI);
+    const double inv_4pi = inv_2pi / 2.0;
+    const double Lx = xe[j] - xs[j];
+    const double Ly = ye[j] - ys[j];
+    const double x2 = cosA[j] * Lx + sinA[j] * Ly;
+    const double xP = cosA[j] * (xCP[i] - xs[j]) + sinA[j] * (yCP[i] - ys[j]);
+    const double yP = -sinA[j] * (xCP[i] - xs[j]) + cosA[j] * (yCP[i] - ys[j]);
+    const double vxP = inv_2pi * (atan2(yP, (xP - x2)) - atan2(yP, xP));
+    const double vyP = -inv_4pi * log((xP * xP + yP * yP) / ((xP - x2) * (xP - x2) + yP * yP));
+    const double vx = cosA[j] * vxP - sinA[j] * vyP;
+    const double vy = sinA[j] * vxP + cosA[j] * vyP;
+    A[t] = vx * cosA[i] + vy * sinA[i];
+}
+
+// Velocity induced at evaluation points by panels of strength gamma
+// (pyx:357-391), one thread per point, panels in order.
+__global__ void panel_induced_kernel(int64_t M, const double *__restrict__ gamma, int64_t ne,
+                                     const double *__restrict__ xE, const double *__restrict__ yE,
+                                     const double *__restrict__ xs, const double *__restrict__ ys,
+                                     const double *__restrict__ xe, const double *__restrict__ ye,
+                                     const double *__restrict__ cosA,
+                                     const double *__restrict__ sinA, double *__restrict__ vx,
+                                     double *__restrict__ vy)
+{
+#pragma clang fp contract(off)
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    const double inv_2pi = 1.0 / (2.0 * M_PI);
+    const double inv_4pi = inv_2pi / 2.0;
+    double sx = 0.0, sy = 0.0;
+    for (int64_t j = 0; j < M; ++j) {
+        const double Lx = xe[j] - xs[j];
+        const double Ly = ye[j] - ys[j];
+        const double x2 = cosA[j] * Lx + sinA[j] * Ly;
+        const double xP = cosA[j] * (xE[i] - xs[j]) + sinA[j] * (yE[i] - ys[j]);
+        const double yP = -sinA[j] * (xE[i] - xs[j]) + cosA[j] * (yE[i] - ys[j]);
+        const double vxP = (gamma[j] * inv_2pi) * (atan2(yP, (xP - x2)) - atan2(yP, xP));
+        const double vyP =
+            -(gamma[j] * inv_4pi) * log((xP * xP + yP * yP) / ((xP - x2) * (xP - x2) + yP * yP));
+        sx += cosA[j] * vxP - sinA[j] * vyP;
+        sy += sinA[j] * vxP + cosA[j] * vyP;
+    }
+    vx[i] = sx;
+    vy[i] = sy;
+}
+
+// ------------------------------------------------------- matrix setup ----
+// F64 row-major B (rows x n) -> F64 column-major copy (for residuals) and,
+// for square systems, the F32 column-major working matrix.
+__global__ void to_colmajor(const double *__restrict__ B, int64_t rows, int64_t n,
+                            double *__restrict__ Bc, float *__restrict__ A32)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * n) return;
+    const int64_t j = t / rows, i = t - (t / rows) * rows;  // column-major index t = j*rows + i
+    const double v = B[i * n + j];
+    Bc[t] = v;
+    if (A32) A32[t] = (float)v;
+}
+
+// N = B^T B (n x n, column-major, F64 and F32 copies); Bc column-major rows x n.
+__global__ void normal_matrix(const double *__restrict__ Bc, int64_t rows, int64_t n,
+                              double *__restrict__ N64, float *__restrict__ N32)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * n) return;
+    const int64_t j = t / n, i = t - (t / n) * n;
+    const double *ci = Bc + i * rows, *cj = Bc + j * rows;
+    double s = 0.0;
+    for (int64_t r = 0; r < rows; ++r) s = __builtin_fma(ci[r], cj[r], s);
+    N64[t] = s;
+    N32[t] = (float)s;
+}
+
+// --------------------------------------------------------------- LU -------
+constexpr int NB = 32;  // panel width
+
+// Unblocked LU with partial pivoting of the panel A[k0:n, k0:k0+NB] (column
+// major, lda = n) in LDS.  One workgroup of 256 threads; rows <= 1024.
+__global__ __launch_bounds__(256) void lu_panel(float *__restrict__ A, int n, int k0,
+                                                int32_t *__restrict__ ipiv,
+                                                int32_t *__restrict__ info, int use_lds)
+{
+    extern __shared__ float lds_panel[];  // [NB][rows] column-major, rows = n - k0
+    __shared__ float red_v[256];
+    __shared__ int red_i[256];
+    const int rows = n - k0;
+    const int w = min(NB, n - k0);
+    const int tid = threadIdx.x;
+    // the panel is worked on in LDS when it fits, else in place (L2-resident)
+    float *P = use_lds ? lds_panel : A + (int64_t)k0 * n + k0;
+    const int64_t ld = use_lds ? rows : n;
+    if (use_lds)
+        for (int c = 0; c < w; ++c)
+            for (int r = tid; r < rows; r += 256) P[c * ld + r] = A[(int64_t)(k0 + c) * n + k0 + r];
+    __syncthreads();
+    for (int c = 0; c < w; ++c) {
+        // pivot: first row of max |P[r][c]|, r >= c
+        float best = -1.0f;
+        int bi = c;
+        for (int r = c + tid; r < rows; r += 256) {
+            const float v = fabsf(P[c * ld + r]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
+        red_v[tid] = best;
+        red_i[tid] = bi;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (tid < s) {
+                const float a = red_v[tid], b = red_v[tid + s];
+                const int ia = red_i[tid], ib = red_i[tid + s];
+                if (b > a || (b == a && ib < ia)) {
+                    red_v[tid] = b;
+                    red_i[tid] = ib;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = red_i[0];
+        const float pv = red_v[0];
+        __syncthreads();
+        if (tid == 0) {
+            ipiv[k0 + c] = k0 + p;
+            if (pv == 0.0f && info[0] == 0) info[0] = k0 + c + 1;
+        }
+        if (p != c)
+            for (int cc = tid; cc < w; cc += 256) {
+                const float t = P[cc * ld + c];
+                P[cc * ld + c] = P[cc * ld + p];
+                P[cc * ld + p] = t;
+            }
+        __syncthreads();
+        const float d = P[c * ld + c];
+        if (d != 0.0f) {
+            const float inv = 1.0f / d;
+            for (int r = c + 1 + tid; r < rows; r += 256) P[c * ld + r] *= inv;
+        }
+        __syncthreads();
+        for (int cc = c + 1; cc < w; ++cc) {
+            const float u = P[cc * ld + c];
+            for (int r = c + 1 + tid; r < rows; r += 256)
+                P[cc * ld + r] = __builtin_fmaf(-P[c * ld + r], u, P[cc * ld + r]);
+        }
+        __syncthreads();
+    }
+    if (use_lds)
+        for (int c = 0; c < w; ++c)
+            for (int r = tid; r < rows; r += 256) A[(int64_t)(k0 + c) * n + k0 + r] = P[c * ld + r];
+}
+
+// Row swaps of panel k0 applied to every column outside the panel, then
+// U12 = L11^-1 A12 for the columns right of it.  One thread per column.
+__global__ void lu_swap_trsm(float *__restrict__ A, int n, int k0,
+                             const int32_t *__restrict__ ipiv)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = min(NB, n - k0);
+    if (j >= n || (j >= k0 && j < k0 + w)) return;
+    float *col = A + (int64_t)j * n;
+    for (int c = 0; c < w; ++c) {
+        const int p = ipiv[k0 + c];
+        if (p != k0 + c) {
+            const float t = col[k0 + c];
+            col[k0 + c] = col[p];
+            col[p] = t;
+        }
+    }
+    if (j < k0 + w) return;
+    float u[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) u[c] = c < w ? col[k0 + c] : 0.0f;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+#pragma unroll
+        for (int r = c + 1; r < NB; ++r) {
+            const float l = (r < w) ? A[(int64_t)(k0 + c) * n + k0 + r] : 0.0f;
+            u[r] = __builtin_fmaf(-l, u[c], u[r]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < w) col[k0 + c] = u[c];
+}
+
+// Trailing update A22 -= L21 * U12 with MFMA: one wave per 16x16 tile of
+// A22, K = NB in steps of 4 (v_mfma_f32_16x16x4f32; A operand lane l holds
+// A[l%16][l/16], B operand B[l/16][l%16], result D[4*(l/16)+v][l%16]).
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void lu_update_mfma(float *__restrict__ A, int n, int k0)
+{
+    const int r0 = k0 + NB, c0 = k0 + NB;
+    const int m = n - r0;
+    const int tiles_r = (m + 15) / 16;
+    const int tr = blockIdx.x % tiles_r, tc = blockIdx.x / tiles_r;
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row = r0 + tr * 16, col = c0 + tc * 16;
+    floatx4 acc;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = row + 4 * lk + v, j = col + li;
+        acc[v] = (i < n && j < n) ? A[(int64_t)j * n + i] : 0.0f;
+    }
+#pragma unroll
+    for (int kk = 0; kk < NB; kk += 4) {
+        const int i = row + li, j = col + li, k = k0 + kk + lk;
+        const float a = (i < n) ? A[(int64_t)k * n + i] : 0.0f;   // L21[i][k]
+        const float b = (j < n) ? A[(int64_t)j * n + k] : 0.0f;   // U12[k][j]
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(-a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int i = row + 4 * lk + v, j = col + li;
+        if (i < n && j < n) A[(int64_t)j * n + i] = acc[v];
+    }
+}
+
+// -------------------------------------------------------- LU solve --------
+// Solves (P L U) d = s * r with the float32 factors for one right-hand
+// side (float64 in, float64 out; inner arithmetic float32, the residual is
+// pre-scaled by 1/s = 1/max|r| to keep it in float32 range).  One
+// workgroup; blocked substitution: a wave solves each NB x NB diagonal
+// block, then all threads update the rows beyond it.
+constexpr int SOLVE_T = 512;
+
+__global__ __launch_bounds__(SOLVE_T) void lu_solve_kernel(const float *__restrict__ LU, int n,
+                                                           const int32_t *__restrict__ ipiv,
+                                                           const double *__restrict__ r,
+                                                           const double *__restrict__ rmax,
+                                                           double *__restrict__ d)
+{
+    extern __shared__ float y[];  // n
+    const int tid = threadIdx.x;
+    const double s = rmax[0] > 0.0 ? rmax[0] : 1.0;
+    for (int i = tid; i < n; i += SOLVE_T) y[i] = (float)(r[i] / s);
+    __syncthreads();
+    if (tid == 0)
+        for (int k = 0; k < n; ++k) {
+            const int p = ipiv[k];
+            if (p != k) {
+                const float t = y[k];
+                y[k] = y[p];
+                y[p] = t;
+            }
+        }
+    __syncthreads();
+    // forward: L unit lower
+    for (int b0 = 0; b0 < n; b0 += NB) {
+        const int w = min(NB, n - b0);
+        if (tid < 64) {
+            for (int c = 0; c < w; ++c) {
+                const float xc = y[b0 + c];
+                if (tid > c && tid < w)
+                    y[b0 + tid] = __builtin_fmaf(-LU[(int64_t)(b0 + c) * n + b0 + tid], xc,
+                                                 y[b0 + tid]);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_s_waitcnt(0);
+            }
+        }
+        __syncthreads();
+        for (int i = b0 + w + tid; i < n; i += SOLVE_T) {
+            float acc = y[i];
+            for (int c = 0; c < w; ++c)
+                acc = __builtin_fmaf(-LU[(int64_t)(b0 + c) * n + i], y[b0 + c], acc);
+            y[i] = acc;
+        }
+        __syncthreads();
+    }
+    // backward: U upper, non-unit
+    const int nblk = (n + NB - 1) / NB;
+    for (int bk = nblk - 1; bk >= 0; --bk) {
+        const int b0 = bk * NB, w = min(NB, n - b0);
+        if (tid < 64) {
+            for (int c = w - 1; c >= 0; --c) {
+                if (tid == c) y[b0 + c] = y[b0 + c] / LU[(int64_t)(b0 + c) * n + b0 + c];
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_s_waitcnt(0);
+                const float xc = y[b0 + c];
+                if (tid < c)
+                    y[b0 + tid] = __builtin_fmaf(-LU[(int64_t)(b0 + c) * n + b0 + tid], xc,
+                                                 y[b0 + tid]);
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_s_waitcnt(0);
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < b0; i += SOLVE_T) {
+            float acc = y[i];
+            for (int c = 0; c < w; ++c)
+                acc = __builtin_fmaf(-LU[(int64_t)(b0 + c) * n + i], y[b0 + c], acc);
+            y[i] = acc;
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < n; i += SOLVE_T) d[i] = (double)y[i] * s;
+}
+
+// ------------------------------------------------- refinement kernels -----
+// r = c - B x  (B column-major rows x n), float64
+__global__ void resid_full(const double *__restrict__ Bc, int64_t rows, int64_t n,
+                           const double *__restrict__ x, const double *__restrict__ c,
+                           double *__restrict__ r)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows) return;
+    double s = c[i];
+    for (int64_t j = 0; j < n; ++j) s = __builtin_fma(-Bc[j * rows + i], x[j], s);
+    r[i] = s;
+}
+
+// g = B^T r  (float64)
+__global__ void transpose_apply(const double *__restrict__ Bc, int64_t rows, int64_t n,
+                                const double *__restrict__ r, double *__restrict__ g)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double *col = Bc + j * rows;
+    double s = 0.0;
+    for (int64_t i = 0; i < rows; ++i) s = __builtin_fma(col[i], r[i], s);
+    g[j] = s;
+}
+
+// max|v| into out[0] (one block)
+__global__ __launch_bounds__(256) void absmax_kernel(const double *__restrict__ v, int64_t n,
+                                                     double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(v[i]));
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// x += d; stats[0] = max|d|, stats[1] = max|x| (one block)
+__global__ __launch_bounds__(256) void axpy_stats(double *__restrict__ x,
+                                                  const double *__restrict__ d, int64_t n,
+                                                  double *__restrict__ stats)
+{
+    __shared__ double sd[256], sx[256];
+    double md = 0.0, mx = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double v = x[i] + d[i];
+        x[i] = v;
+        md = fmax(md, fabs(d[i]));
+        mx = fmax(mx, fabs(v));
+    }
+    sd[threadIdx.x] = md;
+    sx[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + s]);
+            sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        stats[0] = sd[0];
+        stats[1] = sx[0];
+    }
+}
+
+// ------------------------------------------------------------- host ------
+int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info)
+{
+    PHF_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), L->stream));
+    const size_t lds_max = Context::get().props().sharedMemPerBlock;
+    for (int k0 = 0; k0 < n; k0 += NB) {
+        const int rows = n - k0;
+        const size_t lds = sizeof(float) * (size_t)NB * rows;
+        const int use_lds = lds + 4096 <= lds_max ? 1 : 0;
+        lu_panel<<<1, 256, use_lds ? lds : 0, L->stream>>>(A32, n, k0, ipiv, info, use_lds);
+        lu_swap_trsm<<<div_up(n, 256), 256, 0, L->stream>>>(A32, n, k0, ipiv);
+        const int m = n - k0 - NB;
+        if (m > 0) {
+            const int t = (m + 15) / 16;
+            lu_update_mfma<<<t * t, 64, 0, L->stream>>>(A32, n, k0);
+        }
+    }
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const double *c,
+                       double *x, SolveStats *st)
+{
+    const bool normal = rows > n;
+    double *Bc, *N64 = nullptr;
+    float *A32;
+    PHF_ALLOC(Bc, L->arena, double, (size_t)(rows * n));
+    PHF_ALLOC(A32, L->arena, float, (size_t)(n * n));
+    hipEvent_t e0 = L->ev[0], e1 = L->ev[1], e2 = L->ev[2];
+    PHF_HIP(hipEventRecord(e0, L->stream));
+    to_colmajor<<<div_up(rows * n, 256), 256, 0, L->stream>>>(B, rows, n, Bc,
+                                                              normal ? nullptr : A32);
+    if (normal) {
+        PHF_ALLOC(N64, L->arena, double, (size_t)(n * n));
+        normal_matrix<<<div_up(n * n, 256), 256, 0, L->stream>>>(Bc, rows, n, N64, A32);
+    }
+    int32_t *ipiv, *info;
+    PHF_ALLOC(ipiv, L->arena, int32_t, (size_t)n);
+    PHF_ALLOC(info, L->arena, int32_t, 1);
+    PHF_TRY(lu_factor_f32(L, A32, (int)n, ipiv, info));
+    PHF_HIP(hipEventRecord(e1, L->stream));
+    // refinement
+    double *r, *g, *d, *stats, *rmax;
+    PHF_ALLOC(r, L->arena, double, (size_t)rows);
+    PHF_ALLOC(g, L->arena, double, (size_t)n);
+    PHF_ALLOC(d, L->arena, double, (size_t)n);
+    PHF_ALLOC(stats, L->arena, double, 2);
+    PHF_ALLOC(rmax, L->arena, double, 1);
+    PHF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, L->stream));
+    int it = 0;
+    bool conv = false;
+    double last_dx = INFINITY;
+    const int max_it = 60;
+    for (; it < max_it; ++it) {
+        resid_full<<<div_up(rows, 256), 256, 0, L->stream>>>(Bc, rows, n, x, c, r);
+        const double *rhs = r;
+        if (normal) {
+            transpose_apply<<<div_up(n, 256), 256, 0, L->stream>>>(Bc, rows, n, r, g);
+            rhs = g;
+        }
+        absmax_kernel<<<1, 256, 0, L->stream>>>(rhs, n, rmax);
+        lu_solve_kernel<<<1, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(A32, (int)n, ipiv,
+                                                                            rhs, rmax, d);
+        axpy_stats<<<1, 256, 0, L->stream>>>(x, d, n, stats);
+        PHF_HIP(hipGetLastError());
+        double hs[2];
+        PHF_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, L->stream));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        if (!std::isfinite(hs[0]) || !std::isfinite(hs[1])) break;
+        const double eps = 2.220446049250313e-16;
+        if (hs[0] <= 4.0 * eps * hs[1]) {  // correction at float64 resolution
+            conv = true;
+            ++it;
+            break;
+        }
+        if (it > 2 && hs[0] >= 0.5 * last_dx) {  // stalled at its limiting accuracy
+            conv = hs[0] <= 1e-10 * hs[1];
+            ++it;
+            break;
+        }
+        last_dx = hs[0];
+    }
+    PHF_HIP(hipEventRecord(e2, L->stream));
+    int32_t hinfo = 0;
+    PHF_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    float ms_lu = 0, ms_ref = 0;
+    (void)hipEventElapsedTime(&ms_lu, e0, e1);
+    (void)hipEventElapsedTime(&ms_ref, e1, e2);
+    st->iterations = it;
+    st->converged = conv && hinfo == 0;
+    st->singular_at = hinfo;
+    st->lu_ms = ms_lu;
+    st->refine_ms = ms_ref;
+    if (hinfo != 0) {
+        set_error("matrix is singular in float32 (zero pivot at column %d)", hinfo - 1);
+        return PHF_ERR_INVALID;
+    }
+    return PHF_OK;
+}
+
+}  // namespace phf
+
+// ------------------------------------------------------------ C entry ----
+using namespace phf;
+
+extern "C" {
+
+int phf_panel_influence(const double *xCP, const double *yCP, const double *xs,
+                        const double *ys, const double *xe, const double *ye,
+                        const double *cosA, const double *sinA, int64_t m, double *A)
+{
+    clear_error();
+    PHF_REQUIRE(m >= 0 && m <= 65536, "panel count out of range (%lld)", (long long)m);
+    const double *in[8] = {xCP, yCP, xs, ys, xe, ye, cosA, sinA};
+    const char *nm[8] = {"xCP", "yCP", "xPanelStart", "yPanelStart", "xPanelEnd", "yPanelEnd",
+                         "cosAlpha", "sinAlpha"};
+    for (int k = 0; k < 8; ++k) PHF_TRY(check_host_array(in[k], m, nm[k]));
+    PHF_REQUIRE(m == 0 || A, "A is NULL");
+    if (m == 0) return PHF_OK;
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    double *d[8], *dA;
+    for (int k = 0; k < 8; ++k) PHF_TRY(upload(L, in[k], m, &d[k]));
+    PHF_ALLOC(dA, L->arena, double, (size_t)(m * m));
+    StageTimer tm(L, timing_enabled());
+    bool used[PHF_STAGE_COUNT] = {};
+    used[PHF_STAGE_PANEL] = true;
+    tm.begin(PHF_STAGE_PANEL);
+    panel_influence_kernel<<<div_up(m * m, 256), 256, 0, L->stream>>>(m, d[0], d[1], d[2], d[3],
+                                                                     d[4], d[5], d[6], d[7], dA);
+    PHF_HIP(hipGetLastError());
+    tm.end(PHF_STAGE_PANEL);
+    PHF_TRY(download(L, dA, m * m, A));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    record_stage_times(L, used);
+    return PHF_OK;
+}
+
+int phf_panel_induced_velocity(const double *gamma, int64_t m, const double *xE,
+                               const double *yE, int64_t ne, const double *xs,
+                               const double *ys, const double *xe, const double *ye,
+                               const double *cosA, const double *sinA, double *vx, double *vy)
+{
+    clear_error();
+    PHF_REQUIRE(m >= 0 && ne >= 0, "negative sizes");
+    const double *in[7] = {gamma, xs, ys, xe, ye, cosA, sinA};
+    const char *nm[7] = {"gamma", "xPanelStart", "yPanelStart", "xPanelEnd", "yPanelEnd",
+                         "cosAlpha", "sinAlpha"};
+    for (int k = 0; k < 7; ++k) PHF_TRY(check_host_array(in[k], m, nm[k]));
+    PHF_TRY(check_host_array(xE, ne, "xEval"));
+    PHF_TRY(check_host_array(yE, ne, "yEval"));
+    PHF_REQUIRE(ne == 0 || (vx && vy), "output arrays are NULL");
+    if (ne == 0) return PHF_OK;
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    double *d[7], *dxE, *dyE, *dvx, *dvy;
+    for (int k = 0; k < 7; ++k) PHF_TRY(upload(L, in[k], m, &d[k]));
+    PHF_TRY(upload(L, xE, ne, &dxE));
+    PHF_TRY(upload(L, yE, ne, &dyE));
+    PHF_ALLOC(dvx, L->arena, double, (size_t)ne);
+    PHF_ALLOC(dvy, L->arena, double, (size_t)ne);
+    panel_induced_kernel<<<div_up(ne, 128), 128, 0, L->stream>>>(
+        m, d[0], ne, dxE, dyE, d[1], d[2], d[3], d[4], d[5], d[6], dvx, dvy);
+    PHF_HIP(hipGetLastError());
+    PHF_TRY(download(L, dvx, ne, vx));
+    PHF_TRY(download(L, dvy, ne, vy));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    return PHF_OK;
+}
+
+int phf_dense_solve(const double *B, int64_t rows, int64_t n, const double *c, double *x,
+                    phf_solve_info *info)
+{
+    clear_error();
+    PHF_REQUIRE(n >= 1 && n <= 4096 && rows >= n && rows <= 65536,
+                "matrix shape %lld x %lld unsupported (1 <= n <= 4096, n <= rows)",
+                (long long)rows, (long long)n);
+    PHF_TRY(check_host_array(B, rows * n, "A"));
+    PHF_TRY(check_host_array(c, rows, "b"));
+    PHF_REQUIRE(x != nullptr, "x is NULL");
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    auto t0 = std::chrono::steady_clock::now();
+    double *dB, *dc, *dx;
+    PHF_TRY(upload(L, B, rows * n, &dB));
+    PHF_TRY(upload(L, c, rows, &dc));
+    PHF_ALLOC(dx, L->arena, double, (size_t)n);
+    SolveStats st{};
+    int rc = dense_solve_device(L, dB, rows, n, dc, dx, &st);
+    if (rc == PHF_OK) {
+        PHF_TRY(download(L, dx, n, x));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (info) {
+        info->iterations = st.iterations;
+        info->converged = st.converged ? 1 : 0;
+        info->lu_ms = st.lu_ms;
+        info->refine_ms = st.refine_ms;
+        info->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+"""Drop-in for pHyFlow.panels.base.panelSolver (reference:
+src/python/panels/base/panelSolver.py) plus the dense solve of
+Panels.solve (src/python/panels/__panelsClass.py:715-822).
+
+    influenceMatrix(xCP, yCP, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd,
+                    cosAlpha, sinAlpha, hardware=0, method=1, panelKernel=1)   :31
+    inducedVelocity(strength, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd,
+                    cosAlpha, sinAlpha, xEval, yEval, hardware=0, method=1,
+                    panelKernel=1)                                            :148
+    solve(A, RHS, problemType='prescribed', L=None, gTotal=0.0)
+        -> strengths, the linear algebra of Panels.solve: 'fixed' solves
+        A s = RHS (scipy lu_factor/lu_solve, :961/:782); 'prescribed' the
+        least-squares problem [A; L] s = [RHS; -gTotal] (numpy lstsq, :822).
+    slip_rhs(vxExt, vyExt, cosAlpha, sinAlpha) -> RHS = -(v . t)              :771
+
+Only the constant-strength vortex panel kernel (panelKernel 1, 'csv', the
+reference default) and the direct method exist here, as in the reference's
+GPU-less build; ``hardware`` selects nothing: every call runs on the GPU.
+"""
+import ctypes
+
+import numpy as np
+
+from ..._native import SolveInfo, call, f64, ptr
+from ...cpp.panels.kernels import vortex
+
+__all__ = ["influenceMatrix", "inducedVelocity", "solve", "slip_rhs"]
+
+_PANEL_KERNEL_CSV = 1
+_DIRECT_METHOD = 1
+
+
+def _check(method, panelKernel):
+    if method != _DIRECT_METHOD:
+        raise NotImplementedError("FMM :: Not Implemented!")
+    if panelKernel != _PANEL_KERNEL_CSV:
+        raise NotImplementedError("only the constant-strength vortex panel kernel is provided")
+
+
+def influenceMatrix(xCP, yCP, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd, cosAlpha, sinAlpha,
+                    hardware=0, method=1, panelKernel=1):
+    _check(method, panelKernel)
+    return vortex.assemble_influenceMatrix(xCP, yCP, xPanelStart, yPanelStart, xPanelEnd,
+                                           yPanelEnd, cosAlpha, sinAlpha)
+
+
+def inducedVelocity(strength, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd, cosAlpha, sinAlpha,
+                    xEval, yEval, hardware=0, method=1, panelKernel=1):
+    _check(method, panelKernel)
+    return vortex.inducedVelocity(strength, xEval, yEval, xPanelStart, yPanelStart, xPanelEnd,
+                                  yPanelEnd, cosAlpha, sinAlpha)
+
+
+def slip_rhs(vxExt, vyExt, cosAlpha, sinAlpha):
+    """RHS of the no-slip condition for vortex panels (:771)."""
+    return (-f64(vxExt)) * f64(cosAlpha) + (-f64(vyExt)) * f64(sinAlpha)
+
+
+def dense_solve(B, c, return_info=False):
+    """x solving B x = c (square) or min ||B x - c|| (tall), float32 LU +
+    float64 iterative refinement on the GPU."""
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    c = f64(c)
+    if B.ndim != 2 or B.shape[0] != c.size or B.shape[0] < B.shape[1]:
+        raise ValueError("B must be rows x n with rows >= n and len(c) == rows")
+    x = np.empty(B.shape[1])
+    info = SolveInfo()
+    call("phf_dense_solve", ptr(B), B.shape[0], B.shape[1], ptr(c), ptr(x), ctypes.byref(info))
+    if return_info:
+        return x, {"iterations": info.iterations, "converged": bool(info.converged),
+                   "lu_ms": info.lu_ms, "refine_ms": info.refine_ms, "total_ms": info.total_ms}
+    return x
+
+
+def solve(A, RHS, problemType="prescribed", L=None, gTotal=0.0, return_info=False):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    RHS = f64(RHS)
+    if problemType == "fixed":
+        return dense_solve(A, RHS, return_info)
+    if problemType == "prescribed":
+        if L is None:
+            raise ValueError("the 'prescribed' problem needs the panel lengths L")
+        rows = np.atleast_2d(np.asarray(L, dtype=np.float64))
+        g = np.atleast_1d(np.asarray(gTotal, dtype=np.float64))
+        B = np.vstack([A, rows])
+        c = np.concatenate([RHS, -g])
+        return dense_solve(B, c, return_info)
+    raise ValueError("problemType %r not provided ('fixed' or 'prescribed')" % (problemType,))
