@@ -39,7 +39,7 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X fp32 (vector, packed) / dense matrix
 FLOP_PER_PAIR = 17
 
 
-def timed_steps(sysm, params, warmup, steps):
+def timed_steps(sysm, params, warmup, steps, counts=True):
     for _ in range(warmup):
         sysm.velocity(params, fetch=False)
     lat, p2p_ms, info = [], [], None
@@ -53,9 +53,10 @@ def timed_steps(sysm, params, warmup, steps):
         p2p_ms.append(info["ms"].get("p2p", float("nan")))
     _native.lib().phf_synchronize()
     wall = time.perf_counter() - t0
-    # work counts from one extra (untimed) evaluation
-    sysm.velocity(params, fetch=False, info=True, count_work=True)
-    info = dict(info, p2p_pairs=sysm.last_info["p2p_pairs"], m2l_links=sysm.last_info["m2l_links"])
+    if counts:  # work counts from one extra (untimed) evaluation
+        sysm.velocity(params, fetch=False, info=True, count_work=True)
+        info = dict(info, p2p_pairs=sysm.last_info["p2p_pairs"],
+                    m2l_links=sysm.last_info["m2l_links"])
     return wall, np.array(lat), np.array(p2p_ms), info
 
 
@@ -100,8 +101,84 @@ def cpu_baseline(s, budget_s=40.0):
             "sample": "%d seeded targets of the %d-blob workload, float64 direct sum" % (m, n)}
 
 
+def bench_lamb(args):
+    """Lamb-Oseen vortex on a 64x64 lattice: direct Gaussian sum (the
+    vv2parGauss kernel) at all 4,096 blob centres, float64 and F32C, vs the
+    CPU restatement of the reference direct sum timed on this host."""
+    from oracle import oracle as O
+    c = W.lamb_oseen_lattice()
+    ks = c["k"] * c["sigma"] ** 2
+    t = time.perf_counter()
+    ref = O.vv2par_gauss(c["x"], c["y"], c["w"], c["x"], c["y"], ks, nthreads=os.cpu_count() or 1)
+    t_cpu = time.perf_counter() - t
+    res = {}
+    for name, prec in (("f64", gauss.PREC_F64), ("f32c", gauss.PREC_F32C)):
+        p = vel_params(c["sigma"], method=1, prec=prec)
+        with BlobSystem(c["x"], c["y"], c["w"]) as sysm:
+            wall, lat, p2p_ms, info = timed_steps(sysm, p, args.warmup, args.steps, counts=False)
+            v = sysm.last_velocity()
+        res[name] = {"evals_per_s": c["x"].size / (wall / args.steps),
+                     "ms_per_step": 1e3 * wall / args.steps,
+                     "kernel_ms": float(np.nanmean(p2p_ms)),
+                     "rel_l2": O.rel_l2(v, ref), "max_err": O.max_rel(v, ref)}
+    n = c["x"].size
+    pairs = n * n
+    best = res["f32c"]
+    ach = FLOP_PER_PAIR * pairs / (best["kernel_ms"] * 1e-3) / 1e12
+    return {"metric": "blob vel. evals/s, Lamb-Oseen 64x64 direct sum", "value": best["evals_per_s"],
+            "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "direct Gaussian sum, 4096 lattice blobs (Lamb-Oseen)",
+                       "variants": res},
+            "roofline": {"kernel": "direct_f32c", "bound": "valu", "achieved": ach,
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
+                         "traffic": None},
+            "cpu_baseline": {"value": n / t_cpu, "unit": "evals/s", "cores": os.cpu_count() or 1,
+                             "kind": "port", "sample": "full 4096x4096 direct sum, float64, "
+                             "restatement of vv2parGauss_C_cpu (OpenMP)"}}
+
+
+def bench_cylinder(args):
+    """One coupled step of the impulsively started cylinder: 512 panels,
+    262,144 blobs in 0.5 < r < 1.5, Heun RK2 with FMM velocities (F32C,
+    order 16) + panel velocity + vInf, panel solve by float32 LU + float64
+    refinement, M4' remesh (h = 0.005) with float64 invariants."""
+    from phyflow_amd.lagrangian import PanelBody, step_rk2
+    xp, yp = W.cylinder_panels(512, R=0.5)
+    a = W.annulus_blobs(262144, sigma=0.004, seed=0)
+    t = time.perf_counter()
+    body = PanelBody(xp, yp, "prescribed", 0.0)
+    t_build = time.perf_counter() - t
+    p = vel_params(0.004, prec=gauss.PREC_F32C, order=16, leaf=64)
+    times, infos = [], []
+    for k in range(args.warmup + max(1, args.steps // 10)):
+        with BlobSystem(a["x"], a["y"], a["w"]) as sysm:
+            _native.lib().phf_synchronize()
+            t = time.perf_counter()
+            info = step_rk2(sysm, p, 0.001, vInf=(1.0, 0.0), panels=body,
+                            remesh=dict(sigma=0.004, overlap=1.25))
+            dt = time.perf_counter() - t
+        if k >= args.warmup:
+            times.append(dt)
+            infos.append(info)
+    info = infos[-1]
+    tot = float(np.sum(np.abs(a["w"])))
+    return {"metric": "cylinder step time (512 panels, 262144 blobs, RK2 + remesh)",
+            "value": 1e3 * float(np.median(times)), "unit": "ms", "n_gpus": 1,
+            "steps": len(times), "warmup": args.warmup, "ms_per_step": 1e3 * float(np.median(times)),
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "impulsively started cylinder, one coupled step",
+                       "panel_setup_ms": 1e3 * t_build, "step_info": info,
+                       "circ_rel_change": abs(info["circ_after"] - info["circ_before"]) / tot,
+                       "impulse_rel_change": max(abs(info["impulse_x_after"] - info["impulse_x_before"]),
+                                                 abs(info["impulse_y_after"] - info["impulse_y_before"])) / tot}}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["fmm1m", "lamb", "cylinder"], default="fmm1m")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
@@ -117,6 +194,12 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         return 2
+    if args.workload == "lamb":
+        print(json.dumps(bench_lamb(args)), flush=True)
+        return 0
+    if args.workload == "cylinder":
+        print(json.dumps(bench_cylinder(args)), flush=True)
+        return 0
 
     s = W.uniform_system(args.n, seed=0, h=1.0 / 1024 if args.n == 1048576 else None)
     prec = gauss.PREC_F32C if args.prec == "f32c" else gauss.PREC_F64

@@ -234,6 +234,55 @@ int phf_system_velocity(phf_system *s, const phf_vel_params *p, double *vx, doub
                         phf_eval_info *info);
 int phf_system_get_velocity(phf_system *s, double *vx, double *vy);
 
+/* ---- resident panel bodies and the coupled step -----------------------------
+ * A phf_panels object holds one panel discretisation (start/end points of m
+ * straight vortex panels), its geometry (__panelsClass.py:1036-1058: panel
+ * lengths, angles, collocation points CP = mid - 0.1 L n), its influence
+ * matrix and the float32 LU factors of A ('fixed') or of the least-squares
+ * system [A; L] ('prescribed', Kelvin row sum_j L_j s_j = -gTotal) -- the
+ * state of the reference's Panels object (__panelsClass.py:446-525). */
+typedef struct phf_panels_s phf_panels;
+#define PHF_PANELS_FIXED 0
+#define PHF_PANELS_PRESCRIBED 1
+
+int phf_panels_create(const double *xs, const double *ys, const double *xe, const double *ye,
+                      int64_t m, int32_t problem, double gTotal, phf_panels **out);
+int phf_panels_destroy(phf_panels *p);
+/* m, collocation points (m each, may be NULL) and A (m*m row-major, may be NULL). */
+int phf_panels_info(phf_panels *p, int64_t *m, double *xcp, double *ycp, double *A);
+/* Panels.solve(vxExternal, vyExternal) (__panelsClass.py:715-822) with the
+ * cached factors: strengths for RHS = -(v . t). */
+int phf_panels_solve(phf_panels *p, const double *vx, const double *vy, double *strength,
+                     int32_t *iterations);
+
+typedef struct {
+    double dt;
+    double vinf_x, vinf_y;   /* free stream */
+    int32_t panel_update;    /* 0: solve panels at stage 1 ('constant'), 1: every stage */
+    int32_t remesh;          /* 1: M4' remesh after convection */
+    double remesh_sigma, remesh_overlap, remesh_x0, remesh_y0, remesh_c;
+} phf_step_params;
+
+typedef struct {
+    int64_t n_before, n_after;
+    /* float64 invariants before/after the remesh: total circulation, linear
+     * impulse (sum w y, -sum w x) and sum w (x^2 + y^2) */
+    double circ_before, circ_after;
+    double impulse_x_before, impulse_x_after, impulse_y_before, impulse_y_after;
+    double m2_before, m2_after;
+    int32_t solve_iterations, pad;
+    double ms_convection, ms_remesh, ms_total;
+} phf_step_info;
+
+/* One time step of the resident blobs: Heun RK2 convection with blob
+ * velocities (params vp), panel velocities (panels may be NULL) and the
+ * free stream, then an optional M4' remesh.  Replaces the convection and
+ * redistribution of LagrangianSolver.evolve / Blobs.evolve
+ * (__lagrangianSolverClass.py:269-375, __blobsClass.py:705-781), whose
+ * integrator is RK4; RK2 here.  info may be NULL. */
+int phf_system_step_rk2(phf_system *s, const phf_vel_params *vp, phf_panels *panels,
+                        const phf_step_params *sp, phf_step_info *info);
+
 #ifdef __cplusplus
 }
 #endif

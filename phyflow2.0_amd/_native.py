@@ -47,6 +47,23 @@ class SolveInfo(ctypes.Structure):
                 ("total_ms", _dbl)]
 
 
+class StepParams(ctypes.Structure):
+    _fields_ = [("dt", _dbl), ("vinf_x", _dbl), ("vinf_y", _dbl), ("panel_update", _i32),
+                ("remesh", _i32), ("remesh_sigma", _dbl), ("remesh_overlap", _dbl),
+                ("remesh_x0", _dbl), ("remesh_y0", _dbl), ("remesh_c", _dbl)]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [("n_before", _i64), ("n_after", _i64), ("circ_before", _dbl), ("circ_after", _dbl),
+                ("impulse_x_before", _dbl), ("impulse_x_after", _dbl),
+                ("impulse_y_before", _dbl), ("impulse_y_after", _dbl),
+                ("m2_before", _dbl), ("m2_after", _dbl), ("solve_iterations", _i32),
+                ("pad", _i32), ("ms_convection", _dbl), ("ms_remesh", _dbl), ("ms_total", _dbl)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 METHOD_FMM = 0
 METHOD_DIRECT = 1
 N_STAGES = len(STAGES)
@@ -95,6 +112,13 @@ _SIGS = {
     "phf_panel_influence": ([_ptr] * 8 + [_i64, _ptr], ctypes.c_int),
     "phf_panel_induced_velocity": ([_ptr, _i64, _ptr, _ptr, _i64] + [_ptr] * 8, ctypes.c_int),
     "phf_dense_solve": ([_ptr, _i64, _i64, _ptr, _ptr, ctypes.POINTER(SolveInfo)], ctypes.c_int),
+    "phf_panels_create": ([_ptr, _ptr, _ptr, _ptr, _i64, _i32, _dbl, ctypes.POINTER(_ptr)],
+                          ctypes.c_int),
+    "phf_panels_destroy": ([_ptr], ctypes.c_int),
+    "phf_panels_info": ([_ptr, ctypes.POINTER(_i64), _ptr, _ptr, _ptr], ctypes.c_int),
+    "phf_panels_solve": ([_ptr, _ptr, _ptr, _ptr, ctypes.POINTER(_i32)], ctypes.c_int),
+    "phf_system_step_rk2": ([_ptr, ctypes.POINTER(VelParams), _ptr, ctypes.POINTER(StepParams),
+                             ctypes.POINTER(StepInfo)], ctypes.c_int),
     "phf_tree_build": ([_ptr, _ptr, _i64, _i32, _ptr, _ptr, _ptr, ctypes.POINTER(_dbl),
                         ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)], ctypes.c_int),
 }

@@ -429,28 +429,39 @@ int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info)
     return PHF_OK;
 }
 
-int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const double *c,
-                       double *x, SolveStats *st)
+int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f)
 {
+    f->rows = rows;
+    f->n = n;
     const bool normal = rows > n;
-    double *Bc, *N64 = nullptr;
-    float *A32;
-    PHF_ALLOC(Bc, L->arena, double, (size_t)(rows * n));
-    PHF_ALLOC(A32, L->arena, float, (size_t)(n * n));
-    hipEvent_t e0 = L->ev[0], e1 = L->ev[1], e2 = L->ev[2];
-    PHF_HIP(hipEventRecord(e0, L->stream));
-    to_colmajor<<<div_up(rows * n, 256), 256, 0, L->stream>>>(B, rows, n, Bc,
-                                                              normal ? nullptr : A32);
+    PHF_ALLOC(f->Bc, L->arena, double, (size_t)(rows * n));
+    PHF_ALLOC(f->LU, L->arena, float, (size_t)(n * n));
+    PHF_ALLOC(f->ipiv, L->arena, int32_t, (size_t)n);
+    PHF_ALLOC(f->info, L->arena, int32_t, 1);
+    to_colmajor<<<div_up(rows * n, 256), 256, 0, L->stream>>>(B, rows, n, f->Bc,
+                                                              normal ? nullptr : f->LU);
     if (normal) {
+        double *N64;
         PHF_ALLOC(N64, L->arena, double, (size_t)(n * n));
-        normal_matrix<<<div_up(n * n, 256), 256, 0, L->stream>>>(Bc, rows, n, N64, A32);
+        normal_matrix<<<div_up(n * n, 256), 256, 0, L->stream>>>(f->Bc, rows, n, N64, f->LU);
     }
-    int32_t *ipiv, *info;
-    PHF_ALLOC(ipiv, L->arena, int32_t, (size_t)n);
-    PHF_ALLOC(info, L->arena, int32_t, 1);
-    PHF_TRY(lu_factor_f32(L, A32, (int)n, ipiv, info));
-    PHF_HIP(hipEventRecord(e1, L->stream));
-    // refinement
+    PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info));
+    int32_t hinfo = 0;
+    PHF_HIP(hipMemcpyAsync(&hinfo, f->info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    f->singular_at = hinfo;
+    if (hinfo != 0) {
+        set_error("matrix is singular in float32 (zero pivot at column %d)", hinfo - 1);
+        return PHF_ERR_INVALID;
+    }
+    return PHF_OK;
+}
+
+int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double *x,
+                        SolveStats *st)
+{
+    const int64_t rows = f.rows, n = f.n;
+    const bool normal = rows > n;
     double *r, *g, *d, *stats, *rmax;
     PHF_ALLOC(r, L->arena, double, (size_t)rows);
     PHF_ALLOC(g, L->arena, double, (size_t)n);
@@ -462,15 +473,16 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
     bool conv = false;
     double last_dx = INFINITY;
     const int max_it = 60;
+    const double eps = 2.220446049250313e-16;
     for (; it < max_it; ++it) {
-        resid_full<<<div_up(rows, 256), 256, 0, L->stream>>>(Bc, rows, n, x, c, r);
+        resid_full<<<div_up(rows, 256), 256, 0, L->stream>>>(f.Bc, rows, n, x, c, r);
         const double *rhs = r;
         if (normal) {
-            transpose_apply<<<div_up(n, 256), 256, 0, L->stream>>>(Bc, rows, n, r, g);
+            transpose_apply<<<div_up(n, 256), 256, 0, L->stream>>>(f.Bc, rows, n, r, g);
             rhs = g;
         }
         absmax_kernel<<<1, 256, 0, L->stream>>>(rhs, n, rmax);
-        lu_solve_kernel<<<1, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(A32, (int)n, ipiv,
+        lu_solve_kernel<<<1, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(f.LU, (int)n, f.ipiv,
                                                                             rhs, rmax, d);
         axpy_stats<<<1, 256, 0, L->stream>>>(x, d, n, stats);
         PHF_HIP(hipGetLastError());
@@ -478,7 +490,6 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
         PHF_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, L->stream));
         PHF_HIP(hipStreamSynchronize(L->stream));
         if (!std::isfinite(hs[0]) || !std::isfinite(hs[1])) break;
-        const double eps = 2.220446049250313e-16;
         if (hs[0] <= 4.0 * eps * hs[1]) {  // correction at float64 resolution
             conv = true;
             ++it;
@@ -491,22 +502,52 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
         }
         last_dx = hs[0];
     }
+    st->iterations = it;
+    st->converged = conv;
+    return PHF_OK;
+}
+
+int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const double *c,
+                       double *x, SolveStats *st)
+{
+    hipEvent_t e0 = L->ev[0], e1 = L->ev[1], e2 = L->ev[2];
+    PHF_HIP(hipEventRecord(e0, L->stream));
+    DenseFactors f;
+    int rc = dense_factor_device(L, B, rows, n, &f);
+    st->singular_at = f.singular_at;
+    if (rc != PHF_OK) return rc;
+    PHF_HIP(hipEventRecord(e1, L->stream));
+    PHF_TRY(dense_refine_device(L, f, c, x, st));
     PHF_HIP(hipEventRecord(e2, L->stream));
-    int32_t hinfo = 0;
-    PHF_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
     float ms_lu = 0, ms_ref = 0;
     (void)hipEventElapsedTime(&ms_lu, e0, e1);
     (void)hipEventElapsedTime(&ms_ref, e1, e2);
-    st->iterations = it;
-    st->converged = conv && hinfo == 0;
-    st->singular_at = hinfo;
     st->lu_ms = ms_lu;
     st->refine_ms = ms_ref;
-    if (hinfo != 0) {
-        set_error("matrix is singular in float32 (zero pivot at column %d)", hinfo - 1);
-        return PHF_ERR_INVALID;
-    }
+    return PHF_OK;
+}
+
+int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
+                         const double *yE, const double *xs, const double *ys, const double *xe,
+                         const double *ye, const double *cosA, const double *sinA, double *vx,
+                         double *vy)
+{
+    if (ne <= 0) return PHF_OK;
+    panel_induced_kernel<<<div_up(ne, 128), 128, 0, L->stream>>>(m, gamma, ne, xE, yE, xs, ys, xe,
+                                                                 ye, cosA, sinA, vx, vy);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+int panel_influence_device(Lane *L, int64_t m, const double *xCP, const double *yCP,
+                           const double *xs, const double *ys, const double *xe, const double *ye,
+                           const double *cosA, const double *sinA, double *A)
+{
+    if (m <= 0) return PHF_OK;
+    panel_influence_kernel<<<div_up(m * m, 256), 256, 0, L->stream>>>(m, xCP, yCP, xs, ys, xe, ye,
+                                                                     cosA, sinA, A);
+    PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
 

@@ -15,9 +15,29 @@ struct SolveStats {
 // In-place float32 LU with partial pivoting of the column-major n x n A32.
 int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info);
 
+// Float32 LU factors of B (rows == n) or of B^T B (rows > n), with the
+// float64 column-major copy of B for refinement residuals (arena-owned).
+struct DenseFactors {
+    int64_t rows = 0, n = 0;
+    double *Bc = nullptr;
+    float *LU = nullptr;
+    int32_t *ipiv = nullptr, *info = nullptr;
+    int singular_at = 0;
+};
+
+int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f);
+int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double *x,
+                        SolveStats *st);
 // Solve B x = c (rows == n) or min ||B x - c||_2 (rows > n) for the
 // row-major float64 B on the device: float32 LU + float64 refinement.
 int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const double *c,
                        double *x, SolveStats *st);
+int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
+                         const double *yE, const double *xs, const double *ys, const double *xe,
+                         const double *ye, const double *cosA, const double *sinA, double *vx,
+                         double *vy);
+int panel_influence_device(Lane *L, int64_t m, const double *xCP, const double *yCP,
+                           const double *xs, const double *ys, const double *xe, const double *ye,
+                           const double *cosA, const double *sinA, double *A);
 
 }  // namespace phf

@@ -1,0 +1,395 @@
+// phf_step.hip -- resident vortex-panel bodies and one coupled time step
+// (convection by RK2 with FMM/direct blob velocities, panel velocities and
+// the free stream, then optional M4' remesh with float64 invariants).
+//
+// Reference call pattern (restated, not copied):
+//   LagrangianSolver.__coupled_convection, __lagrangianSolverClass.py:496-677:
+//   per stage the blobs' velocity at the panel collocation points plus vInf
+//   gives the slip velocity, Panels.solve() the panel strengths (stage 1
+//   only for the default 'constant' update, every stage for 'varying'),
+//   then blob velocity = blobs->blobs + panels->blobs + vInf.
+//   Blobs.redistribute (__blobsClass.py:891-897) remeshes with bounds
+//   -1.5 h.
+// The reference integrates with RK4 and has no RK2; the step here is Heun's
+// second-order scheme, x1 = x0 + dt k(x0), x = x0 + dt/2 (k(x0) + k(x1)),
+// with every stage evaluated at its own positions (the reference RK4 reuses
+// aliased positions, a defect not reproduced here).
+#include "phf_panel.hpp"
+#include "phf_remesh.hpp"
+#include "phf_system.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace phf {
+
+struct Panels {
+    std::mutex mu;
+    Lane lane;  // arena holds the persistent panel data (never reset)
+    int64_t m = 0;
+    int problem = 0;  // 0 'fixed', 1 'prescribed'
+    double gTotal = 0.0;
+    double *xs = nullptr, *ys = nullptr, *xe = nullptr, *ye = nullptr;
+    double *cosA = nullptr, *sinA = nullptr, *L = nullptr, *xcp = nullptr, *ycp = nullptr;
+    double *A = nullptr;  // influence matrix, row-major m x m
+    DenseFactors f;       // factors of A or of [A; L]
+    double *s = nullptr;  // last strengths
+    std::vector<double> hxcp, hycp;
+    int last_iterations = 0;
+};
+
+// ------------------------------------------------------------- kernels ----
+__global__ void concat2(const double *__restrict__ a, int64_t na, const double *__restrict__ b,
+                        int64_t nb, double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na)
+        out[i] = a[i];
+    else if (i < na + nb)
+        out[i] = b[i - na];
+}
+
+// RHS of the no-slip condition, -(v . t) (__panelsClass.py:771); for the
+// 'prescribed' problem the last entry is -gTotal.
+__global__ void slip_rhs_kernel(const double *__restrict__ vx, const double *__restrict__ vy,
+                                int64_t m, double vinf_x, double vinf_y,
+                                const double *__restrict__ cosA, const double *__restrict__ sinA,
+                                int prescribed, double gTotal, double *__restrict__ rhs)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) rhs[i] = (-(vx[i] + vinf_x)) * cosA[i] + (-(vy[i] + vinf_y)) * sinA[i];
+    if (prescribed && i == m) rhs[m] = -gTotal;
+}
+
+// k = v_blobs + v_panels + vInf
+__global__ void total_velocity(double *__restrict__ kx, double *__restrict__ ky,
+                               const double *__restrict__ px, const double *__restrict__ py,
+                               int64_t n, double vinf_x, double vinf_y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    kx[i] = (kx[i] + (px ? px[i] : 0.0)) + vinf_x;
+    ky[i] = (ky[i] + (py ? py[i] : 0.0)) + vinf_y;
+}
+
+__global__ void euler_stage(const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ kx, const double *__restrict__ ky,
+                            int64_t n, double dt, double *__restrict__ x1,
+                            double *__restrict__ y1)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x1[i] = x[i] + dt * kx[i];
+    y1[i] = y[i] + dt * ky[i];
+}
+
+__global__ void heun_final(double *__restrict__ x, double *__restrict__ y,
+                           const double *__restrict__ k1x, const double *__restrict__ k1y,
+                           const double *__restrict__ k2x, const double *__restrict__ k2y,
+                           int64_t n, double half_dt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = x[i] + half_dt * (k1x[i] + k2x[i]);
+    y[i] = y[i] + half_dt * (k1y[i] + k2y[i]);
+}
+
+// ------------------------------------------------------------ helpers -----
+static int panels_solve_device(Lane *L, Panels *P, const double *vx_cp, const double *vy_cp,
+                               double vinf_x, double vinf_y)
+{
+    const int64_t rows = P->f.rows;
+    double *rhs;
+    PHF_ALLOC(rhs, L->arena, double, (size_t)rows);
+    slip_rhs_kernel<<<div_up(rows, 256), 256, 0, L->stream>>>(
+        vx_cp, vy_cp, P->m, vinf_x, vinf_y, P->cosA, P->sinA, P->problem, P->gTotal, rhs);
+    PHF_HIP(hipGetLastError());
+    SolveStats st;
+    PHF_TRY(dense_refine_device(L, P->f, rhs, P->s, &st));
+    P->last_iterations = st.iterations;
+    if (!st.converged) {
+        set_error("panel solve did not converge (%d refinement sweeps)", st.iterations);
+        return PHF_ERR_INVALID;
+    }
+    return PHF_OK;
+}
+
+// Total velocity k at positions (X, Y) of the n blobs carrying w.
+static int stage_velocity(Lane *L, const double *X, const double *Y, const double *w, int64_t n,
+                          const phf_vel_params &vp, Panels *P, bool solve_panels, double vinf_x,
+                          double vinf_y, double *kx, double *ky, StageTimer &tm, bool *used)
+{
+    FmmInfo fi;
+    if (!P) {
+        PHF_TRY(velocity_device(L, X, Y, w, n, X, Y, n, true, vp, kx, ky, tm, used, &fi));
+        total_velocity<<<div_up(n, 256), 256, 0, L->stream>>>(kx, ky, nullptr, nullptr, n,
+                                                             vinf_x, vinf_y);
+        PHF_HIP(hipGetLastError());
+        return PHF_OK;
+    }
+    const int64_t m = P->m, nt = n + m;
+    double *tx, *ty, *vx, *vy, *px, *py;
+    PHF_ALLOC(tx, L->arena, double, (size_t)nt);
+    PHF_ALLOC(ty, L->arena, double, (size_t)nt);
+    PHF_ALLOC(vx, L->arena, double, (size_t)nt);
+    PHF_ALLOC(vy, L->arena, double, (size_t)nt);
+    PHF_ALLOC(px, L->arena, double, (size_t)std::max<int64_t>(n, 1));
+    PHF_ALLOC(py, L->arena, double, (size_t)std::max<int64_t>(n, 1));
+    concat2<<<div_up(nt, 256), 256, 0, L->stream>>>(X, n, P->xcp, m, tx);
+    concat2<<<div_up(nt, 256), 256, 0, L->stream>>>(Y, n, P->ycp, m, ty);
+    PHF_HIP(hipGetLastError());
+    // blobs -> blobs and blobs -> collocation points in one evaluation
+    PHF_TRY(velocity_device(L, X, Y, w, n, tx, ty, nt, false, vp, vx, vy, tm, used, &fi));
+    if (solve_panels) {
+        tm.begin(PHF_STAGE_PANEL);
+        used[PHF_STAGE_PANEL] = true;
+        PHF_TRY(panels_solve_device(L, P, vx + n, vy + n, vinf_x, vinf_y));
+        tm.end(PHF_STAGE_PANEL);
+    }
+    PHF_TRY(panel_induced_device(L, m, P->s, n, X, Y, P->xs, P->ys, P->xe, P->ye, P->cosA,
+                                 P->sinA, px, py));
+    total_velocity<<<div_up(n, 256), 256, 0, L->stream>>>(vx, vy, px, py, n, vinf_x, vinf_y);
+    PHF_HIP(hipGetLastError());
+    PHF_HIP(hipMemcpyAsync(kx, vx, sizeof(double) * n, hipMemcpyDeviceToDevice, L->stream));
+    PHF_HIP(hipMemcpyAsync(ky, vy, sizeof(double) * n, hipMemcpyDeviceToDevice, L->stream));
+    return PHF_OK;
+}
+
+static void moments_to(const double4 &m, double *circ, double *ix, double *iy, double *m2)
+{
+    // linear impulse I = (sum w y, -sum w x)
+    *circ = m.x;
+    *ix = m.z;
+    *iy = -m.y;
+    *m2 = m.w;
+}
+
+}  // namespace phf
+
+// ------------------------------------------------------------ C entry ----
+using namespace phf;
+
+struct phf_panels_s : public Panels {};
+
+extern "C" {
+
+int phf_panels_create(const double *xs, const double *ys, const double *xe, const double *ye,
+                      int64_t m, int32_t problem, double gTotal, phf_panels **out)
+{
+    clear_error();
+    PHF_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    PHF_REQUIRE(m >= 1 && m <= 4096, "panel count out of range (%lld)", (long long)m);
+    PHF_REQUIRE(problem == PHF_PANELS_FIXED || problem == PHF_PANELS_PRESCRIBED,
+                "unknown problem type %d", problem);
+    PHF_REQUIRE(std::isfinite(gTotal), "gTotal must be finite");
+    PHF_TRY(check_host_array(xs, m, "xPanelStart"));
+    PHF_TRY(check_host_array(ys, m, "yPanelStart"));
+    PHF_TRY(check_host_array(xe, m, "xPanelEnd"));
+    PHF_TRY(check_host_array(ye, m, "yPanelEnd"));
+    // geometry (__panelsClass.py:1036-1058): L = |end-start|, (cos, sin) =
+    // (end-start)/L, n = (-sin, cos), CP = mid - n L 0.1
+    std::vector<double> cs(m), sn(m), Ln(m), xc(m), yc(m);
+    for (int64_t j = 0; j < m; ++j) {
+        const double lx = xe[j] - xs[j], ly = ye[j] - ys[j];
+        Ln[j] = std::sqrt(lx * lx + ly * ly);
+        PHF_REQUIRE(Ln[j] > 0.0, "panel %lld has zero length", (long long)j);
+        cs[j] = lx / Ln[j];
+        sn[j] = ly / Ln[j];
+        const double nx = -sn[j], ny = cs[j];
+        xc[j] = 0.5 * (xe[j] + xs[j]) - nx * Ln[j] * 0.1;
+        yc[j] = 0.5 * (ye[j] + ys[j]) - ny * Ln[j] * 0.1;
+    }
+    PHF_TRY(Context::get().init());
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    phf_panels_s *P = new phf_panels_s();
+    hipError_t e = hipStreamCreateWithFlags(&P->lane.stream, hipStreamNonBlocking);
+    for (int k = 0; e == hipSuccess && k < 2 * PHF_STAGE_COUNT; ++k)
+        e = hipEventCreate(&P->lane.ev[k]);
+    if (e != hipSuccess) {
+        set_error("stream/event creation failed: %s", hipGetErrorName(e));
+        delete P;
+        return PHF_ERR_DEVICE;
+    }
+    int rc = [&]() -> int {
+        Lane *L = &P->lane;
+        P->m = m;
+        P->problem = problem;
+        P->gTotal = gTotal;
+        P->hxcp = xc;
+        P->hycp = yc;
+        PHF_TRY(upload(L, xs, m, &P->xs));
+        PHF_TRY(upload(L, ys, m, &P->ys));
+        PHF_TRY(upload(L, xe, m, &P->xe));
+        PHF_TRY(upload(L, ye, m, &P->ye));
+        PHF_TRY(upload(L, cs.data(), m, &P->cosA));
+        PHF_TRY(upload(L, sn.data(), m, &P->sinA));
+        PHF_TRY(upload(L, Ln.data(), m, &P->L));
+        PHF_TRY(upload(L, xc.data(), m, &P->xcp));
+        PHF_TRY(upload(L, yc.data(), m, &P->ycp));
+        const int64_t rows = problem == PHF_PANELS_PRESCRIBED ? m + 1 : m;
+        PHF_ALLOC(P->A, L->arena, double, (size_t)(rows * m));
+        PHF_ALLOC(P->s, L->arena, double, (size_t)m);
+        PHF_TRY(panel_influence_device(L, m, P->xcp, P->ycp, P->xs, P->ys, P->xe, P->ye, P->cosA,
+                                       P->sinA, P->A));
+        if (problem == PHF_PANELS_PRESCRIBED)  // Kelvin row: sum_j L_j s_j = -gTotal (:509-518)
+            PHF_HIP(hipMemcpyAsync(P->A + m * m, P->L, sizeof(double) * m,
+                                   hipMemcpyDeviceToDevice, L->stream));
+        PHF_TRY(dense_factor_device(L, P->A, rows, m, &P->f));
+        PHF_HIP(hipMemsetAsync(P->s, 0, sizeof(double) * m, L->stream));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        return PHF_OK;
+    }();
+    if (rc != PHF_OK) {
+        phf_panels_destroy(P);
+        return rc;
+    }
+    *out = P;
+    return PHF_OK;
+}
+
+int phf_panels_destroy(phf_panels *P)
+{
+    clear_error();
+    if (!P) return PHF_OK;
+    {
+        std::lock_guard<std::mutex> lk(P->mu);
+        if (P->lane.stream) (void)hipStreamSynchronize(P->lane.stream);
+        P->lane.arena.release();
+        P->lane.pin.release();
+        for (auto &ev : P->lane.ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (P->lane.stream) (void)hipStreamDestroy(P->lane.stream);
+    }
+    delete static_cast<phf_panels_s *>(P);
+    return PHF_OK;
+}
+
+int phf_panels_info(phf_panels *P, int64_t *m, double *xcp, double *ycp, double *A)
+{
+    clear_error();
+    PHF_REQUIRE(P != nullptr, "panels is NULL");
+    std::lock_guard<std::mutex> lk(P->mu);
+    if (m) *m = P->m;
+    if (xcp) std::memcpy(xcp, P->hxcp.data(), sizeof(double) * P->m);
+    if (ycp) std::memcpy(ycp, P->hycp.data(), sizeof(double) * P->m);
+    if (A) {
+        PHF_HIP(hipMemcpyAsync(A, P->A, sizeof(double) * P->m * P->m, hipMemcpyDeviceToHost,
+                               P->lane.stream));
+        PHF_HIP(hipStreamSynchronize(P->lane.stream));
+    }
+    return PHF_OK;
+}
+
+int phf_panels_solve(phf_panels *P, const double *vx, const double *vy, double *strength,
+                     int32_t *iterations)
+{
+    clear_error();
+    PHF_REQUIRE(P != nullptr, "panels is NULL");
+    PHF_TRY(check_host_array(vx, P->m, "vxExternal"));
+    PHF_TRY(check_host_array(vy, P->m, "vyExternal"));
+    PHF_REQUIRE(strength != nullptr, "strength is NULL");
+    std::lock_guard<std::mutex> lk(P->mu);
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    double *dvx, *dvy;
+    PHF_TRY(upload(L, vx, P->m, &dvx));
+    PHF_TRY(upload(L, vy, P->m, &dvy));
+    PHF_TRY(panels_solve_device(L, P, dvx, dvy, 0.0, 0.0));
+    PHF_HIP(hipMemcpyAsync(strength, P->s, sizeof(double) * P->m, hipMemcpyDeviceToHost,
+                           L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    if (iterations) *iterations = P->last_iterations;
+    return PHF_OK;
+}
+
+int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
+                        const phf_step_params *sp, phf_step_info *info)
+{
+    clear_error();
+    PHF_REQUIRE(h != nullptr && sp != nullptr, "NULL argument");
+    PHF_TRY(check_vel_params(vp));
+    PHF_REQUIRE(std::isfinite(sp->dt) && std::isfinite(sp->vinf_x) && std::isfinite(sp->vinf_y),
+                "dt and vInf must be finite");
+    if (sp->remesh)
+        PHF_REQUIRE(std::isfinite(sp->remesh_sigma) && sp->remesh_sigma > 0 &&
+                        std::isfinite(sp->remesh_overlap) && sp->remesh_overlap > 0 &&
+                        std::isfinite(sp->remesh_x0) && std::isfinite(sp->remesh_y0) &&
+                        std::isfinite(sp->remesh_c),
+                    "remesh parameters must be finite (sigma, overlap > 0)");
+    System *s = reinterpret_cast<System *>(h);
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lkp;
+    if (P) lkp = std::unique_lock<std::mutex>(P->mu);
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    Lane *L = &s->lane;
+    PHF_TRY(L->arena.reset());
+    auto t0 = std::chrono::steady_clock::now();
+    StageTimer tm(L, false);
+    bool used[PHF_STAGE_COUNT] = {};
+    const int64_t n = s->n;
+    phf_step_info inf{};
+    inf.n_before = n;
+    if (n > 0) {
+        double *k1x, *k1y, *k2x, *k2y, *x1, *y1;
+        PHF_ALLOC(k1x, L->arena, double, (size_t)n);
+        PHF_ALLOC(k1y, L->arena, double, (size_t)n);
+        PHF_ALLOC(k2x, L->arena, double, (size_t)n);
+        PHF_ALLOC(k2y, L->arena, double, (size_t)n);
+        PHF_ALLOC(x1, L->arena, double, (size_t)n);
+        PHF_ALLOC(y1, L->arena, double, (size_t)n);
+        PHF_TRY(stage_velocity(L, s->x, s->y, s->w, n, *vp, P, P != nullptr, sp->vinf_x,
+                               sp->vinf_y, k1x, k1y, tm, used));
+        euler_stage<<<div_up(n, 256), 256, 0, L->stream>>>(s->x, s->y, k1x, k1y, n, sp->dt, x1,
+                                                          y1);
+        PHF_TRY(stage_velocity(L, x1, y1, s->w, n, *vp, P, P != nullptr && sp->panel_update == 1,
+                               sp->vinf_x, sp->vinf_y, k2x, k2y, tm, used));
+        heun_final<<<div_up(n, 256), 256, 0, L->stream>>>(s->x, s->y, k1x, k1y, k2x, k2y, n,
+                                                         0.5 * sp->dt);
+        PHF_HIP(hipGetLastError());
+        if (P) inf.solve_iterations = P->last_iterations;
+    }
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    auto t1 = std::chrono::steady_clock::now();
+    double4 *dm;
+    PHF_ALLOC(dm, L->arena, double4, 2);
+    double4 hm[2] = {};
+    if (sp->remesh && n > 0) {
+        PHF_TRY(moments_device(L, s->x, s->y, s->w, n, dm));
+        RemeshOut ro;
+        PHF_TRY(remesh_device(L, s->x, s->y, s->w, n, sp->remesh_sigma, sp->remesh_overlap,
+                              sp->remesh_x0, sp->remesh_y0, sp->remesh_c, &ro));
+        PHF_HIP(hipMemcpyAsync(&hm[0], dm, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        PHF_TRY(sys_reserve(s, ro.m));
+        PHF_HIP(hipMemcpyAsync(s->x, ro.x, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
+        PHF_HIP(hipMemcpyAsync(s->y, ro.y, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
+        PHF_HIP(hipMemcpyAsync(s->w, ro.w, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
+        s->n = ro.m;
+        PHF_TRY(moments_device(L, s->x, s->y, s->w, s->n, dm + 1));
+        PHF_HIP(hipMemcpyAsync(&hm[1], dm + 1, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
+    } else if (n > 0) {
+        PHF_TRY(moments_device(L, s->x, s->y, s->w, n, dm));
+        PHF_HIP(hipMemcpyAsync(&hm[0], dm, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
+        hm[1] = hm[0];
+    }
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    s->has_velocity = false;
+    auto t2 = std::chrono::steady_clock::now();
+    inf.n_after = s->n;
+    moments_to(hm[0], &inf.circ_before, &inf.impulse_x_before, &inf.impulse_y_before,
+               &inf.m2_before);
+    moments_to(hm[1], &inf.circ_after, &inf.impulse_x_after, &inf.impulse_y_after,
+               &inf.m2_after);
+    inf.ms_convection = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    inf.ms_remesh = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    inf.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
+    if (info) *info = inf;
+    return PHF_OK;
+}
+
+}  // extern "C"

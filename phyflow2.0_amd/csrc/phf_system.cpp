@@ -1,22 +1,13 @@
 // phf_system.cpp -- device-resident blob systems (one stream + workspace
 // each), velocity evaluation on resident data.
-#include "phf_fmm.hpp"
+#include "phf_system.hpp"
 
 #include <cmath>
 #include <mutex>
 
 namespace phf {
 
-struct System {
-    std::mutex mu;
-    Lane lane;  // private stream, arena, pinned staging and events
-    int64_t n = 0, cap = 0;
-    double *x = nullptr, *y = nullptr, *w = nullptr;  // resident blobs
-    double *vx = nullptr, *vy = nullptr;              // last velocity
-    bool has_velocity = false;
-};
-
-static int sys_reserve(System *s, int64_t n)
+int sys_reserve(System *s, int64_t n)
 {
     if (n <= s->cap) return PHF_OK;
     double *bufs[5] = {s->x, s->y, s->w, s->vx, s->vy};
@@ -66,36 +57,39 @@ int check_vel_params(const phf_vel_params *p)
     return PHF_OK;
 }
 
-// Velocity of the resident blobs at their own centres into (vx, vy) (device).
-int system_velocity_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
-                           const phf_vel_params &p, double *vx, double *vy, StageTimer &tm,
-                           bool *used, FmmInfo *fi)
+// Velocity induced by blobs (x, y, w)[n] at targets (xt, yt)[nt] in the
+// reference's velocity() convention, into (vx, vy) (device).  same: the
+// targets are the blobs themselves.
+int velocity_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
+                    const double *xt, const double *yt, int64_t nt, bool same,
+                    const phf_vel_params &p, double *vx, double *vy, StageTimer &tm, bool *used,
+                    FmmInfo *fi)
 {
-    if (n == 0) return PHF_OK;
+    if (nt == 0) return PHF_OK;
     const double ks = p.k * p.sigma * p.sigma;
     if (p.method == PHF_METHOD_DIRECT) {
         tm.begin(PHF_STAGE_P2P);
         used[PHF_STAGE_P2P] = true;
         PairParams pp = make_gauss_params(ks);
-        PHF_TRY(direct_device(L, PAIR_GAUSS_EPS, p.prec, pp, x, y, w, n, x, y, n, vx, vy));
+        PHF_TRY(direct_device(L, PAIR_GAUSS_EPS, p.prec, pp, x, y, w, n, xt, yt, nt, vx, vy));
         tm.end(PHF_STAGE_P2P);
         return PHF_OK;
     }
     const double xopt = std::sqrt(1.2564312086261696770 * ks);  // _induced.py:236-239
     const double cutoff = p.cutoff > 0 ? p.cutoff : 5.0 * xopt;
     double *qr, *qi;
-    PHF_ALLOC(qr, L->arena, double, (size_t)n);
-    PHF_ALLOC(qi, L->arena, double, (size_t)n);
+    PHF_ALLOC(qr, L->arena, double, (size_t)nt);
+    PHF_ALLOC(qi, L->arena, double, (size_t)nt);
     if (p.tol == 0.0) {
         tm.begin(PHF_STAGE_P2P);
         used[PHF_STAGE_P2P] = true;
         PairParams pp = make_oseen_params(xopt, cutoff);
-        PHF_TRY(direct_device(L, PAIR_OSEEN, p.prec, pp, x, y, w, n, x, y, n, qr, qi));
+        PHF_TRY(direct_device(L, PAIR_OSEEN, p.prec, pp, x, y, w, n, xt, yt, nt, qr, qi));
         tm.end(PHF_STAGE_P2P);
     } else {
         FmmArgs a{};
         a.xb = x; a.yb = y; a.wb = w; a.nb = n;
-        a.xt = x; a.yt = y; a.nt = n; a.same = true;
+        a.xt = xt; a.yt = yt; a.nt = nt; a.same = same;
         a.xopt = xopt; a.cutoff = cutoff;
         a.p = p.order > 0 ? p.order : fmm_order_from_tol(p.tol);
         a.ws = p.ws > 0 ? p.ws : 1;
@@ -105,7 +99,14 @@ int system_velocity_device(Lane *L, const double *x, const double *y, const doub
         a.qi = qi;
         PHF_TRY(fmm_device(L, a, tm, used, fi));
     }
-    return q_to_velocity_device(L, qr, qi, n, vx, vy);
+    return q_to_velocity_device(L, qr, qi, nt, vx, vy);
+}
+
+int system_velocity_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
+                           const phf_vel_params &p, double *vx, double *vy, StageTimer &tm,
+                           bool *used, FmmInfo *fi)
+{
+    return velocity_device(L, x, y, w, n, x, y, n, true, p, vx, vy, tm, used, fi);
 }
 
 }  // namespace phf

@@ -1,0 +1,100 @@
+"""Resident panel bodies and the coupled vortex-blob / vortex-panel step.
+
+Counterpart of the numerical core of pHyFlow's LagrangianSolver
+(src/python/lagrangian/__lagrangianSolverClass.py:269-375, 496-677) and of
+Panels (src/python/panels/__panelsClass.py): the blob/panel coupling of one
+time step runs on the GPU with the blobs resident in HBM.
+
+    body = PanelBody(xPanel, yPanel, problemType='prescribed', gTotal=0.0)
+    info = step_rk2(system, vel_params(...), dt, vInf=(1, 0), panels=body,
+                    remesh=dict(sigma=..., overlap=...))
+
+The time integrator is Heun's RK2 (the reference has RK4 and Euler only).
+"""
+import ctypes
+
+import numpy as np
+
+from ._native import StepInfo, StepParams, call, f64, lib, ptr
+
+PROBLEM = {"fixed": 0, "prescribed": 1}
+
+__all__ = ["PanelBody", "step_rk2"]
+
+
+class PanelBody:
+    """Closed panel polygon(s) given by panel start/end points."""
+
+    def __init__(self, xPanel=None, yPanel=None, problemType="prescribed", gTotal=0.0, *,
+                 xs=None, ys=None, xe=None, ye=None):
+        if xPanel is not None:
+            # end points of consecutive panels (reference geometry dicts,
+            # examples/panels/ex01_solveNoSlip_test.py:26-38)
+            xp, yp = f64(xPanel), f64(yPanel)
+            xs, ys, xe, ye = xp[:-1], yp[:-1], xp[1:], yp[1:]
+        xs, ys, xe, ye = (f64(v).copy() for v in (xs, ys, xe, ye))
+        if problemType not in PROBLEM:
+            raise ValueError("problemType must be 'fixed' or 'prescribed'")
+        h = ctypes.c_void_p()
+        call("phf_panels_create", ptr(xs), ptr(ys), ptr(xe), ptr(ye), xs.size,
+             PROBLEM[problemType], float(gTotal), ctypes.byref(h))
+        self._h = h
+        self.m = xs.size
+        self.problemType = problemType
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().phf_panels_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def collocation_points(self):
+        x, y = np.empty(self.m), np.empty(self.m)
+        call("phf_panels_info", self._h, None, ptr(x), ptr(y), None)
+        return x, y
+
+    @property
+    def A(self):
+        A = np.empty((self.m, self.m))
+        call("phf_panels_info", self._h, None, None, None, ptr(A))
+        return A
+
+    def solve(self, vxExternal, vyExternal):
+        """Panels.solve (__panelsClass.py:715-822): strengths for the slip
+        velocity (vxExternal, vyExternal) at the collocation points."""
+        vx, vy = f64(vxExternal), f64(vyExternal)
+        s = np.empty(self.m)
+        it = ctypes.c_int32()
+        call("phf_panels_solve", self._h, ptr(vx), ptr(vy), ptr(s), ctypes.byref(it))
+        return s
+
+
+def step_rk2(system, params, dt, vInf=(0.0, 0.0), panels=None, remesh=None, panel_update="constant"):
+    """Advance a resident BlobSystem by one Heun RK2 step (+ optional M4'
+    remesh; remesh = dict(sigma=, overlap=, x0=-1.5h, y0=-1.5h, c=0)).
+    Returns the step info (invariants before/after the remesh, timings)."""
+    sp = StepParams()
+    sp.dt = float(dt)
+    sp.vinf_x, sp.vinf_y = float(vInf[0]), float(vInf[1])
+    sp.panel_update = 1 if panel_update == "varying" else 0
+    if remesh:
+        h = remesh["sigma"] * remesh.get("overlap", 1.0)
+        sp.remesh = 1
+        sp.remesh_sigma = float(remesh["sigma"])
+        sp.remesh_overlap = float(remesh.get("overlap", 1.0))
+        sp.remesh_x0 = float(remesh.get("x0", -1.5 * h))  # Blobs.redistribute bounds
+        sp.remesh_y0 = float(remesh.get("y0", -1.5 * h))
+        sp.remesh_c = float(remesh.get("c", 0.0))
+    info = StepInfo()
+    call("phf_system_step_rk2", system._h, ctypes.byref(params),
+         panels.handle if panels is not None else None, ctypes.byref(sp), ctypes.byref(info))
+    return info.as_dict()
