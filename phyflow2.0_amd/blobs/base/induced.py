@@ -1,0 +1,70 @@
+"""Drop-in for pHyFlow.blobs.base.induced.velocity (reference:
+src/python/blobs/base/induced/_induced.py:34-166), Python 3.
+
+    velocity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None,
+             yEval=None, hardware=0, blocksize=128, method=0, Ndirect=35,
+             tol=1.0e-6, cutoff=None) -> (vx, vy)
+
+Dispatch as in the reference (fullOption = 100*method + 10*hardware +
+kernel, :138):
+  method 1 (direct), kernel 1 (gauss) -> vv2parGauss with k*sigma^2 (:401, :515)
+  method 0 (FMM),    kernel 1 (gauss) -> fmm2d with xopt = sqrt(alpha k sigma^2),
+                                         cutoff = 5 xopt, mapped to
+                                         (-qi/2pi, -qr/2pi) (:236-251, :321-336)
+  FMM with the cutoff kernel raises, as the reference does (:158-163).
+Differences from the reference, all deliberate:
+  * every computation runs on the GPU (``hardware`` is accepted and selects
+    nothing; the reference's CPU and GPU variants compute the same sums);
+  * ``xEval is None`` is tested with ``is`` (the reference's ``== None`` is
+    ambiguous on modern NumPy) and no padding is needed;
+  * the algebraic cutoff kernel (kernel 0) is not provided.
+Keyword-only extensions: ``prec`` (PREC_F64 default, or PREC_F32C) and the
+FMM ``order``/``leaf``/``ws``.
+"""
+import math
+
+import numpy as np
+
+from ...cpp.blobs.kernels import gauss
+from ...cpp.blobs.kernels.gauss import PREC_F32C, PREC_F64
+
+GAUSS_KERNEL = 1   # blobOptions.GAUSS_KERNEL
+CUTOFF_KERNEL = 0  # blobOptions.CUTOFF_KERNEL
+CPU_HARDWARE = 0
+GPU_HARDWARE = 1
+FMM_METHOD = 0
+DIRECT_METHOD = 1
+ALPHA = 1.2564312086261696770  # _induced.py:236
+
+__all__ = ["velocity", "PREC_F64", "PREC_F32C"]
+
+
+def velocity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None, hardware=0,
+             blocksize=128, method=0, Ndirect=35, tol=1.0e-6, cutoff=None, *, prec=PREC_F64,
+             order=0, leaf=0, ws=0):
+    if hardware not in (CPU_HARDWARE, GPU_HARDWARE):
+        raise ValueError("hardware must be 0 (cpu) or 1 (gpu)")
+    if kernel == CUTOFF_KERNEL:
+        if method == FMM_METHOD:
+            raise Exception("FMM calculation of cutoff kernel not yet available")
+        raise NotImplementedError("the algebraic cutoff kernel is not provided by this build")
+    if kernel != GAUSS_KERNEL:
+        raise ValueError("unknown kernel %r" % (kernel,))
+    if (xEval is None) != (yEval is None):
+        raise ValueError("xEval and yEval must both be given or both be None")
+    if method == DIRECT_METHOD:
+        xe = xBlob if xEval is None else xEval
+        ye = yBlob if yEval is None else yEval
+        return gauss.vv2parGauss_gpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma, blocksize,
+                                     prec=prec)
+    if method == FMM_METHOD:
+        ksigmasqr = k * sigma * sigma
+        xopt = math.sqrt(ALPHA * ksigmasqr)
+        if cutoff is None:
+            cutoff = 5.0 * xopt
+        xe = xBlob if xEval is None else xEval
+        ye = yBlob if yEval is None else yEval
+        vy_fmm, vx_fmm = gauss.fmm2d_gpu(xBlob, yBlob, wBlob, xe, ye, Ndirect, xopt, cutoff, tol,
+                                         prec=prec, order=order, leaf=leaf, ws=ws)
+        return -vx_fmm / (2.0 * np.pi), -vy_fmm / (2.0 * np.pi)
+    raise ValueError("unknown method %r" % (method,))
