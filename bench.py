@@ -176,9 +176,160 @@ def bench_cylinder(args):
                                                  abs(info["impulse_y_after"] - info["impulse_y_before"])) / tot}}
 
 
+def bench_mix(args):
+    """256 independent uniform systems, N log-uniform in [1024, 262144]
+    (seed 0), h = 1/sqrt(N), sigma = h/0.8: batched F32C FMM (order 16,
+    leaf 64) replayed from one captured hipGraph, vs the same mix as 256
+    one-at-a-time float64 evaluations (one-shot calls with per-call copies,
+    as the reference's GPU path does, and resident systems)."""
+    from phyflow_amd.batch import BatchSystems
+    count = args.systems
+    sizes = W.uneven_sizes(count, 1024, 262144, seed=0)
+    systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
+    total = int(np.sum(sizes))
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C, order=16,
+                      leaf=64) as b:
+        for i, s in enumerate(systems):
+            b.set_system(i, s["x"], s["y"], s["w"])
+        for _ in range(args.warmup):
+            b.velocity(use_graph=True)
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            a = time.perf_counter()
+            info = b.velocity(use_graph=True)
+            lat.append(time.perf_counter() - a)
+        wall = time.perf_counter() - t0
+        dev_ms = info["ms"]
+        errs = []
+        check = list(range(0, count, max(1, count // 8)))
+        for i in check:
+            s = systems[i]
+            v = b.get_velocity(i)
+            sub = W.target_subset(s["x"].size, 1024, seed=i)
+            xopt, cutoff = W.oseen_params(s["sigma"])
+            qr, qi = gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"][sub], s["y"][sub], 35, xopt,
+                                     cutoff, 0.0, prec=gauss.PREC_F64)
+            ref = np.concatenate([-qi / (2 * math.pi), -qr / (2 * math.pi)])
+            got = np.concatenate([v[0][sub], v[1][sub]])
+            errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    batched = total / (wall / args.steps)
+    # baseline 1: one-shot float64 fmm2d per system (host buffers, per-call copies)
+    t = time.perf_counter()
+    for s in systems:
+        xopt, cutoff = W.oseen_params(s["sigma"])
+        gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"], s["y"], 35, xopt, cutoff, 1e-6,
+                        prec=gauss.PREC_F64, order=16, leaf=64)
+    t_oneshot = time.perf_counter() - t
+    # baseline 2: resident float64 systems evaluated one after another
+    resident = [BlobSystem(s["x"], s["y"], s["w"]) for s in systems]
+    ps = [vel_params(s["sigma"], prec=gauss.PREC_F64, order=16, leaf=64) for s in systems]
+    for r, p in zip(resident, ps):
+        r.velocity(p, fetch=False)
+    _native.lib().phf_synchronize()
+    t = time.perf_counter()
+    for r, p in zip(resident, ps):
+        r.velocity(p, fetch=False)
+    t_res = time.perf_counter() - t
+    for r in resident:
+        r.close()
+    return {"metric": "aggregate blob vel. evals/s, 256 uneven systems, batched F32C vs "
+                      "one-at-a-time F64",
+            "value": batched, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "%d systems, N log-uniform [1024, 262144] seed 0, total %d blobs"
+                                   % (count, total),
+                       "graph_device_ms": dev_ms,
+                       "f64_one_shot_evals_per_s": total / t_oneshot,
+                       "f64_resident_sequential_evals_per_s": total / t_res,
+                       "speedup_vs_f64_one_shot": batched / (total / t_oneshot),
+                       "speedup_vs_f64_resident_sequential": batched / (total / t_res),
+                       "rel_l2_vs_fp64_max": max(errs), "checked_systems": len(check)}}
+
+
+def bench_serve(args):
+    """Served step requests: Poisson arrivals at --rate per second for
+    --duration s over --sims resident simulations (N log-uniform in
+    [1e4, 2e6], seed 0, uniform blobs, h = 1/sqrt(N), sigma = h/0.8).  Each
+    request is one RK2 step (two velocity evaluations), an M4' remesh onto
+    the lattice of spacing h and population control (thresholds 1e-8, the
+    Blobs defaults); 80% of requests ask for 1e-6 (F32C, order 16), 20% for
+    1e-12 (float64, order 30, ws 2).  Latency = submission -> completion."""
+    from phyflow_amd.serve import Server
+    rng = np.random.default_rng(0)
+    sizes = np.floor(np.exp(rng.uniform(math.log(1e4), math.log(2e6), args.sims))).astype(int)
+    systems, params, remesh = [], [], []
+    for i, n in enumerate(sizes):
+        s = W.uniform_system(int(n), seed=2000 + i)
+        systems.append(BlobSystem(s["x"], s["y"], s["w"]))
+        params.append((vel_params(s["sigma"], prec=gauss.PREC_F32C, order=16, leaf=64),
+                       vel_params(s["sigma"], prec=gauss.PREC_F64, order=30, leaf=64, ws=2)))
+        remesh.append(dict(sigma=s["sigma"], overlap=0.8, population_control=(1e-8, 1e-8)))
+    dt = 1e-4
+    with Server(slots=args.slots) as srv:
+        ids = [srv.add(b) for b in systems]
+        for k in ids:  # warm-up: one step of each simulation, both precisions
+            srv.wait(srv.submit(k, params[k - ids[0]][0], dt, remesh=remesh[k - ids[0]]))
+            srv.wait(srv.submit(k, params[k - ids[0]][1], dt, remesh=remesh[k - ids[0]]))
+        srv.stats(reset=True)
+        t0 = time.perf_counter()
+        t_next, rids, classes, evals = 0.0, [], [], 0
+        last = t0
+        while True:
+            t_next += rng.exponential(1.0 / args.rate)
+            if t_next > args.duration:
+                break
+            now = time.perf_counter() - t0
+            if t_next > now:
+                time.sleep(t_next - now)
+            k = int(rng.integers(len(ids)))
+            hi = rng.random() < 0.2
+            rids.append(srv.submit(ids[k], params[k][1 if hi else 0], dt, remesh=remesh[k]))
+            classes.append(hi)
+            if time.perf_counter() - last > 10.0:
+                last = time.perf_counter()
+                print("serve: %.0f s, %d submitted, %d done" % (last - t0, len(rids),
+                      srv.stats()["completed"]), file=sys.stderr, flush=True)
+        res = [srv.wait(r) for r in rids]
+        st = srv.stats()
+        for i in ids:
+            srv.remove(i)
+    for b in systems:
+        b.close()
+    lat = np.array([r["latency_ms"] for r in res])
+    cls = np.array(classes)
+    evals = float(np.sum([2 * r["info"]["n_before"] for r in res]))
+    return {"metric": "served RK2 step requests: throughput and p50/p99 step latency",
+            "value": len(res) / st["span_s"], "unit": "steps/s", "n_gpus": 1, "steps": len(res),
+            "warmup": 2 * len(ids), "ms_per_step": float(np.mean(lat)),
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": "%d resident simulations, N log-uniform [1e4, 2e6] (total %d), "
+                                   "Poisson arrivals %.0f/s for %.0f s, %d slots"
+                                   % (len(sizes), int(np.sum(sizes)), args.rate, args.duration,
+                                      args.slots),
+                       "evals_per_s": evals / st["span_s"],
+                       "p50_ms_mixed": float(np.percentile(lat[~cls], 50)) if (~cls).any() else None,
+                       "p99_ms_mixed": float(np.percentile(lat[~cls], 99)) if (~cls).any() else None,
+                       "p50_ms_fp64": float(np.percentile(lat[cls], 50)) if cls.any() else None,
+                       "p99_ms_fp64": float(np.percentile(lat[cls], 99)) if cls.any() else None,
+                       "completed": st["completed"], "failed": st["failed"],
+                       "hbm_used_gb": st["hbm_used"] / 1e9}}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["fmm1m", "lamb", "cylinder"], default="fmm1m")
+    ap.add_argument("--workload", choices=["fmm1m", "lamb", "cylinder", "mix", "serve"],
+                    default="fmm1m")
+    ap.add_argument("--sims", type=int, default=16)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--rate", type=float, default=100.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=5)
@@ -199,6 +350,12 @@ def main():
         return 0
     if args.workload == "cylinder":
         print(json.dumps(bench_cylinder(args)), flush=True)
+        return 0
+    if args.workload == "mix":
+        print(json.dumps(bench_mix(args)), flush=True)
+        return 0
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
         return 0
 
     s = W.uniform_system(args.n, seed=0, h=1.0 / 1024 if args.n == 1048576 else None)

@@ -142,6 +142,13 @@ int phf_blobset_copy(const phf_blobset *b, double *x, double *y, double *w, int6
                      int64_t *j);
 void phf_blobset_free(phf_blobset *b);
 
+/* Replaces PopulationControl(xBlob, yBlob, wBlob, wThreshold,
+ * wTotalThreshold) (_regridding.py:227-330): blobs with |w| < threshold are
+ * dropped; the threshold starts at wThreshold and is divided by 10 until
+ * the dropped total |w| is below wTotalThreshold; order is kept. */
+int phf_population_control(const double *x, const double *y, const double *w, int64_t n,
+                           double wThreshold, double wTotalThreshold, phf_blobset **out);
+
 /* ---- (d) vortex panels and the dense panel solve ---------------------------
  * phf_panel_influence replaces assemble_influenceMatrix(xCP, yCP, xPS, yPS,
  *   xPE, yPE, cosA, sinA) -> A[M,M] (row-major),
@@ -261,6 +268,8 @@ typedef struct {
     int32_t panel_update;    /* 0: solve panels at stage 1 ('constant'), 1: every stage */
     int32_t remesh;          /* 1: M4' remesh after convection */
     double remesh_sigma, remesh_overlap, remesh_x0, remesh_y0, remesh_c;
+    int32_t pc_enabled, pad; /* 1: population control after the remesh */
+    double pc_w_threshold, pc_total_threshold; /* gThresholdLocal, gThresholdGlobal */
 } phf_step_params;
 
 typedef struct {
@@ -282,6 +291,81 @@ typedef struct {
  * integrator is RK4; RK2 here.  info may be NULL. */
 int phf_system_step_rk2(phf_system *s, const phf_vel_params *vp, phf_panels *panels,
                         const phf_step_params *sp, phf_step_info *info);
+
+/* ---- batched systems ---------------------------------------------------------
+ * Many independent blob systems of uneven sizes (e.g. a bluff-body
+ * parameter sweep: one Blobs object per case in the reference, each calling
+ * fmm2d on its own, fmm2d_py.cpp:59) evaluated together: every FMM phase is
+ * one launch over all systems through a segment table, and the evaluation
+ * is captured once as a hipGraph and replayed.  Velocities at the blob
+ * centres in the velocity() convention (_induced.py:334-336); each system
+ * has its own core size sigma; k, order, leaf size, well-separation and
+ * precision are shared. */
+typedef struct phf_batch_s phf_batch;
+typedef struct {
+    int32_t prec;   /* PHF_PREC_F64 or PHF_PREC_F32C */
+    int32_t order;  /* FMM order (<= 0: 16) */
+    int32_t leaf;   /* mean sources per leaf (<= 0: 64) */
+    int32_t ws;     /* well-separation (<= 0: 1) */
+    double k;       /* core multiplier (k sigma^2) */
+} phf_batch_params;
+
+typedef struct {
+    double ms;            /* device time of the evaluation (HIP events) */
+    int32_t graph;        /* 1 if replayed from the captured graph */
+    int32_t flags;        /* nonzero: a near set overflowed (results invalid) */
+    int64_t total_points;
+    int32_t levels_max, pad;
+} phf_batch_info;
+
+int phf_batch_create(const int64_t *sizes, const double *sigma, int32_t count,
+                     const phf_batch_params *bp, phf_batch **out);
+int phf_batch_destroy(phf_batch *b);
+int phf_batch_set_system(phf_batch *b, int32_t s, const double *x, const double *y,
+                         const double *w);
+/* Evaluate every system; use_graph = 1 captures the launch sequence on the
+ * first call and replays it afterwards.  info may be NULL. */
+int phf_batch_velocity(phf_batch *b, int32_t use_graph, phf_batch_info *info);
+int phf_batch_get_velocity(phf_batch *b, int32_t s, double *vx, double *vy);
+
+/* ---- serving --------------------------------------------------------------
+ * A request loop over resident simulations (no counterpart in the
+ * reference, whose callers step their Blobs synchronously).  Simulations
+ * are admitted by HBM footprint; step requests (one RK2 step + optional
+ * remesh each, phf_system_step_rk2 without panels) are queued on arrival
+ * and run by `slots` workers as slots free up, each simulation on its own
+ * HIP stream so concurrent steps overlap on the device.  Latency is
+ * measured from submission to completion. */
+typedef struct phf_server_s phf_server;
+
+typedef struct {
+    int32_t status, pad; /* 0 done, < 0 error code */
+    double latency_ms;   /* submission -> completion */
+    double queue_ms;     /* submission -> start */
+    phf_step_info info;
+} phf_request_result;
+
+typedef struct {
+    int64_t completed, failed;
+    double p50_ms, p99_ms, max_ms; /* latency percentiles of completed requests */
+    double span_s;                 /* first submission -> last completion */
+    int64_t hbm_used, hbm_budget;
+    int32_t slots, pad;
+} phf_serve_stats;
+
+/* hbm_budget_bytes <= 0: 7/8 of the device memory. */
+int phf_server_create(int32_t slots, int64_t hbm_budget_bytes, phf_server **out);
+int phf_server_destroy(phf_server *s);
+/* PHF_ERR_BUSY if the simulation's footprint does not fit the budget.  The
+ * caller keeps ownership of sys and must not use it while registered. */
+int phf_server_add_system(phf_server *s, phf_system *sys, int64_t *id);
+int phf_server_remove_system(phf_server *s, int64_t id);
+int phf_server_submit(phf_server *s, int64_t sim, const phf_vel_params *vp,
+                      const phf_step_params *sp, int64_t *req_id);
+/* timeout_s < 0: wait indefinitely.  The request is forgotten once waited. */
+int phf_server_wait(phf_server *s, int64_t req_id, double timeout_s, phf_request_result *res);
+int phf_server_drain(phf_server *s);
+int phf_server_stats(phf_server *s, int32_t reset, phf_serve_stats *st);
 
 #ifdef __cplusplus
 }

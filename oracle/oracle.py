@@ -46,6 +46,8 @@ def lib():
                                     ctypes.POINTER(ctypes.POINTER(_i64))]
         L.orc_regrid_m4.restype = _i64
         L.orc_free.argtypes = [_ptr]
+        L.orc_population_control.argtypes = [_i64, _ptr, _dbl, _dbl, _ptr, ctypes.POINTER(_dbl)]
+        L.orc_population_control.restype = _i64
         L.orc_panel_influence.argtypes = [_i64] + [_ptr] * 9
         L.orc_panel_induced.argtypes = [_i64, _ptr, _i64] + [_ptr] * 10
         L.orc_lu_factor.argtypes = [_i64, _ptr, _ptr]
@@ -136,6 +138,17 @@ def regrid_m4(x, y, w, sigma, overlap, x0, y0, c=0.0):
         out.append(np.ctypeslib.as_array(ptr, shape=(m,)).astype(dt, copy=True))
         lib().orc_free(ctypes.cast(ptr, ctypes.c_void_p))
     return tuple(out)
+
+
+def population_control(x, y, w, thr, total):
+    """_regridding.py:313-330 restated; returns (x, y, w, threshold used)."""
+    x, y, w = map(_f64, (x, y, w))
+    keep = np.empty(max(w.size, 1), np.int64)
+    t = _dbl()
+    m = lib().orc_population_control(w.size, _p(w), float(thr), float(total), _p(keep),
+                                     ctypes.byref(t))
+    k = keep[:m]
+    return x[k], y[k], w[k], t.value
 
 
 # -- A.5 ---------------------------------------------------------------------

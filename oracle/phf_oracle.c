@@ -234,6 +234,31 @@ int64_t orc_regrid_m4(int64_t n, const double *x, const double *y,
 
 void orc_free(void *p) { free(p); }
 
+/* PopulationControl, src/python/blobs/base/regrid/_regridding.py:313-330:
+ * repeat { flagged = |w| < thr; loss = sum |w[flagged]|; if loss < total:
+ * accept; else thr /= 10 }; keep the unflagged blobs in order.  Writes the
+ * kept indices into keep_idx (capacity n) and returns their count; the
+ * threshold used is returned in *thr_used.  The loss is summed in index
+ * order (numpy sums pairwise, so a loss within an ulp of the total
+ * threshold may decide differently; parity tests avoid that edge).
+ * Pinned by brute force in tests (direct evaluation of the loop).       */
+int64_t orc_population_control(int64_t n, const double *w, double thr, double total,
+                               int64_t *keep_idx, double *thr_used)
+{
+    for (;;) {
+        double loss = 0.0;
+        for (int64_t i = 0; i < n; i++)
+            if (fabs(w[i]) < thr) loss += fabs(w[i]);
+        if (loss < total || thr == 0.0) break;
+        thr /= 10.0;
+    }
+    int64_t m = 0;
+    for (int64_t i = 0; i < n; i++)
+        if (!(fabs(w[i]) < thr)) keep_idx[m++] = i;
+    *thr_used = thr;
+    return m;
+}
+
 /* ------------------------------------------------------------------------ */
 /* A.5  Constant-strength straight vortex panels.
  * Influence matrix restates src/cpp/panels/cython/vortexPanel_Cython/

@@ -50,7 +50,9 @@ class SolveInfo(ctypes.Structure):
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", _dbl), ("vinf_x", _dbl), ("vinf_y", _dbl), ("panel_update", _i32),
                 ("remesh", _i32), ("remesh_sigma", _dbl), ("remesh_overlap", _dbl),
-                ("remesh_x0", _dbl), ("remesh_y0", _dbl), ("remesh_c", _dbl)]
+                ("remesh_x0", _dbl), ("remesh_y0", _dbl), ("remesh_c", _dbl),
+                ("pc_enabled", _i32), ("pad", _i32), ("pc_w_threshold", _dbl),
+                ("pc_total_threshold", _dbl)]
 
 
 class StepInfo(ctypes.Structure):
@@ -62,6 +64,26 @@ class StepInfo(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("prec", _i32), ("order", _i32), ("leaf", _i32), ("ws", _i32), ("k", _dbl)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("ms", _dbl), ("graph", _i32), ("flags", _i32), ("total_points", _i64),
+                ("levels_max", _i32), ("pad", _i32)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("status", _i32), ("pad", _i32), ("latency_ms", _dbl), ("queue_ms", _dbl),
+                ("info", StepInfo)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", _i64), ("failed", _i64), ("p50_ms", _dbl), ("p99_ms", _dbl),
+                ("max_ms", _dbl), ("span_s", _dbl), ("hbm_used", _i64), ("hbm_budget", _i64),
+                ("slots", _i32), ("pad", _i32)]
 
 
 METHOD_FMM = 0
@@ -107,6 +129,8 @@ _SIGS = {
     "phf_regrid_m4": ([_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, ctypes.POINTER(_ptr)],
                       ctypes.c_int),
     "phf_blobset_size": ([_ptr], _i64),
+    "phf_population_control": ([_ptr, _ptr, _ptr, _i64, _dbl, _dbl, ctypes.POINTER(_ptr)],
+                               ctypes.c_int),
     "phf_blobset_copy": ([_ptr, _ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),
     "phf_blobset_free": ([_ptr], None),
     "phf_panel_influence": ([_ptr] * 8 + [_i64, _ptr], ctypes.c_int),
@@ -119,6 +143,21 @@ _SIGS = {
     "phf_panels_solve": ([_ptr, _ptr, _ptr, _ptr, ctypes.POINTER(_i32)], ctypes.c_int),
     "phf_system_step_rk2": ([_ptr, ctypes.POINTER(VelParams), _ptr, ctypes.POINTER(StepParams),
                              ctypes.POINTER(StepInfo)], ctypes.c_int),
+    "phf_batch_create": ([_ptr, _ptr, _i32, ctypes.POINTER(BatchParams), ctypes.POINTER(_ptr)],
+                         ctypes.c_int),
+    "phf_batch_destroy": ([_ptr], ctypes.c_int),
+    "phf_batch_set_system": ([_ptr, _i32, _ptr, _ptr, _ptr], ctypes.c_int),
+    "phf_batch_velocity": ([_ptr, _i32, ctypes.POINTER(BatchInfo)], ctypes.c_int),
+    "phf_batch_get_velocity": ([_ptr, _i32, _ptr, _ptr], ctypes.c_int),
+    "phf_server_create": ([_i32, _i64, ctypes.POINTER(_ptr)], ctypes.c_int),
+    "phf_server_destroy": ([_ptr], ctypes.c_int),
+    "phf_server_add_system": ([_ptr, _ptr, ctypes.POINTER(_i64)], ctypes.c_int),
+    "phf_server_remove_system": ([_ptr, _i64], ctypes.c_int),
+    "phf_server_submit": ([_ptr, _i64, ctypes.POINTER(VelParams), ctypes.POINTER(StepParams),
+                           ctypes.POINTER(_i64)], ctypes.c_int),
+    "phf_server_wait": ([_ptr, _i64, _dbl, ctypes.POINTER(RequestResult)], ctypes.c_int),
+    "phf_server_drain": ([_ptr], ctypes.c_int),
+    "phf_server_stats": ([_ptr, _i32, ctypes.POINTER(ServerStats)], ctypes.c_int),
     "phf_tree_build": ([_ptr, _ptr, _i64, _i32, _ptr, _ptr, _ptr, ctypes.POINTER(_dbl),
                         ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)], ctypes.c_int),
 }

@@ -18,7 +18,7 @@ from ..._native import call, f64, lib, ptr
 
 M4PRIME_INTERPKERNEL = 0
 
-__all__ = ["Regrid", "M4PRIME_INTERPKERNEL"]
+__all__ = ["Regrid", "PopulationControl", "M4PRIME_INTERPKERNEL"]
 
 
 def Regrid(xBlob, yBlob, wBlob, sigma, overlap, xBounds, yBounds, interpKernel=0, c=0.0, *,
@@ -42,3 +42,24 @@ def Regrid(xBlob, yBlob, wBlob, sigma, overlap, xBounds, yBounds, interpKernel=0
     finally:
         lib().phf_blobset_free(h)
     return (xo, yo, wo, io, jo) if return_indices else (xo, yo, wo)
+
+
+def PopulationControl(xBlob, yBlob, wBlob, wThreshold, wTotalThreshold):
+    """Drop-in for PopulationControl (_regridding.py:227-330): discards blobs
+    with |w| < wThreshold, dividing the threshold by 10 until the discarded
+    total |w| is below wTotalThreshold; returns the kept (x, y, w) in order."""
+    x, y, w = f64(xBlob), f64(yBlob), f64(wBlob)
+    if not (x.size == y.size == w.size):
+        raise ValueError("xBlob, yBlob, wBlob must have equal lengths")
+    if x.size == 0:
+        return x.copy(), y.copy(), w.copy()
+    h = ctypes.c_void_p()
+    call("phf_population_control", ptr(x), ptr(y), ptr(w), x.size, float(wThreshold),
+         float(wTotalThreshold), ctypes.byref(h))
+    try:
+        m = lib().phf_blobset_size(h)
+        xo, yo, wo = np.empty(m), np.empty(m), np.empty(m)
+        call("phf_blobset_copy", h, ptr(xo), ptr(yo), ptr(wo), None, None)
+    finally:
+        lib().phf_blobset_free(h)
+    return xo, yo, wo

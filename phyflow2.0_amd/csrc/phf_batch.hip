@@ -1,0 +1,997 @@
+// phf_batch.hip -- many independent blob systems of uneven sizes evaluated
+// together: one launch per FMM phase covers every system through a segment
+// table, and the whole evaluation is captured once as a hipGraph and
+// replayed (no host synchronisation inside).
+//
+// There is no batched path in the reference: every Blobs object calls the
+// FMM on its own (fmm2dInteract, fmm.cpp:88), allocating, copying and
+// freeing device memory each time (cudaeval.cu:154-342).
+//
+// Segment table
+//   points:  packed arrays; system s owns [poff[s], poff[s+1])
+//   leaves:  global leaf index = loff[s] + Morton key in system s (depth L_s)
+//   boxes:   level-major; level l of system s starts at lvl[l*(S+1)+s]
+// Each system gets its own square domain (computed on the device from its
+// bbox, exactly as the single-system path does on the host), its own leaf
+// level and its own smoothing parameters.
+//
+// Lists are not materialised: the near predicate of a box offset (dx, dy)
+// at level l,
+//   near_l = (|dx| <= ws && |dy| <= ws) || gap_x^2 + gap_y^2 < cutoff^2,
+//   gap = max(|d| - 1, 0) * s_l,
+// is evaluated in float64 by every kernel that needs it, so P2P and M2L
+// agree on near/far by construction; the interaction list at level l is
+// {tau : near_{l-1}(floor((tau + parity)/2)) && !near_l(tau)}.
+#include "phf_batch.hpp"
+#include "phf_fmm_dev.hpp"
+
+#include <mutex>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+namespace phf {
+
+constexpr int kRMax = 6;                // largest near radius (boxes)
+constexpr int kRU = 2 * kRMax + 1;      // radius of interaction-list offsets
+constexpr int kMaxLevel = 11;
+
+// largest s in [0, S) with arr[s] <= g (arr non-decreasing, arr[0] <= g)
+__device__ __forceinline__ int find_seg(const int64_t *__restrict__ arr, int S, int64_t g)
+{
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (arr[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool near_pred(int dx, int dy, int ws, double s, double c2)
+{
+    const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy;
+    if (ax <= ws && ay <= ws) return true;
+    const double gx = (double)(ax > 1 ? ax - 1 : 0) * s;
+    const double gy = (double)(ay > 1 ? ay - 1 : 0) * s;
+    return gx * gx + gy * gy < c2;
+}
+
+__device__ __forceinline__ int near_radius(int ws, double s, double c2)
+{
+    // smallest R such that every near offset has max(|dx|,|dy|) <= R
+    const double r = sqrt(c2) / s;
+    int R = (int)ceil(r) + 1;
+    return R > ws ? R : ws;
+}
+
+__device__ __forceinline__ int floordiv2(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
+
+// ------------------------------------------------------- domains, keys ----
+__global__ __launch_bounds__(256) void batch_domain(const BatchSys *__restrict__ sys,
+                                                    const double *__restrict__ x,
+                                                    const double *__restrict__ y,
+                                                    BatchDom *__restrict__ dom)
+{
+    __shared__ double4 sh[256];
+    const int s = blockIdx.x;
+    const int64_t o = sys[s].off, n = sys[s].n;
+    double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double a = x[o + i], b = y[o + i];
+        xmin = fmin(xmin, a);
+        xmax = fmax(xmax, a);
+        ymin = fmin(ymin, b);
+        ymax = fmax(ymax, b);
+    }
+    sh[threadIdx.x] = make_double4(xmin, xmax, ymin, ymax);
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            double4 a = sh[threadIdx.x], b = sh[threadIdx.x + k];
+            sh[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z),
+                                           fmax(a.w, b.w));
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double4 b = sh[0];
+        BatchDom d;
+        if (n == 0) b = make_double4(0.0, 0.0, 0.0, 0.0);
+        d.x0 = b.x;
+        d.y0 = b.z;
+        double ext = fmax(b.y - b.x, b.w - b.z);
+        if (!(ext > 0.0)) ext = 1.0;
+        int e = 0;
+        frexp(ext, &e);
+        const double unit = ldexp(1.0, e - 5);
+        d.D = ceil(ext / unit) * unit;
+        const int L = sys[s].L;
+        d.s = d.D / ldexp(1.0, L);
+        d.inv_s = ldexp(1.0, L) / d.D;
+        dom[s] = d;
+    }
+}
+
+__global__ void batch_keys(const BatchSys *__restrict__ sys, const int64_t *__restrict__ poff,
+                           int S, const BatchDom *__restrict__ dom, const double *__restrict__ x,
+                           const double *__restrict__ y, int64_t N, uint32_t *__restrict__ keys,
+                           int32_t *__restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int s = find_seg(poff, S, i);
+    const BatchDom d = dom[s];
+    const int L = sys[s].L;
+    const double hi = (double)((1u << L) - 1u);
+    double fx = floor((x[i] - d.x0) * d.inv_s);
+    double fy = floor((y[i] - d.y0) * d.inv_s);
+    fx = fx >= 0.0 ? fx : 0.0;
+    fy = fy >= 0.0 ? fy : 0.0;
+    fx = fx > hi ? hi : fx;
+    fy = fy > hi ? hi : fy;
+    keys[i] = (uint32_t)(sys[s].leafbase + morton2((uint32_t)fx, (uint32_t)fy));
+    vals[i] = (int32_t)i;
+}
+
+template <class T>
+__global__ void batch_gather(const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
+                             int S, const BatchDom *__restrict__ dom,
+                             const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ w, const uint32_t *__restrict__ skeys,
+                             const int32_t *__restrict__ perm, int64_t N, void *__restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int32_t j = perm[k];
+    if (sizeof(T) == 4) {
+        const int64_t g = skeys[k];
+        const int s = find_seg(loff, S, g);
+        const BatchDom d = dom[s];
+        double cx, cy;
+        leaf_center((uint32_t)(g - sys[s].leafbase), d.x0, d.y0, d.s, cx, cy);
+        reinterpret_cast<float4 *>(out)[k] =
+            make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
+    } else {
+        reinterpret_cast<double4 *>(out)[k] = make_double4(x[j], y[j], w[j], 0.0);
+    }
+}
+
+__global__ void batch_clear_flag(int32_t *flag) { flag[0] = 0; }
+
+// --------------------------------------------------------------- P2M ------
+template <class T>
+__global__ __launch_bounds__(64) void batch_p2m(const BatchSys *__restrict__ sys,
+                                                const int64_t *__restrict__ loff,
+                                                const int64_t *__restrict__ lvl, int S,
+                                                const BatchDom *__restrict__ dom,
+                                                const void *__restrict__ src_,
+                                                const int32_t *__restrict__ start, int p1,
+                                                typename C2<T>::type *__restrict__ M)
+{
+    using V = typename C2<T>::type;
+    extern __shared__ unsigned char lds_raw[];
+    V *tab = reinterpret_cast<V *>(lds_raw);
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    const int s = find_seg(loff, S, g);
+    const int L = sys[s].L;
+    if (L < 2) return;  // no far field in this system
+    const uint32_t b = (uint32_t)(g - sys[s].leafbase);
+    const int32_t s0 = start[g], s1 = start[g + 1];
+    V acc = cmk<T>(0, 0);
+    if (s1 > s0) {
+        const BatchDom d = dom[s];
+        double cx, cy;
+        leaf_center(b, d.x0, d.y0, d.s, cx, cy);
+        const T inv_s = (T)(1.0 / d.s);
+        for (int32_t c0 = s0; c0 < s1; c0 += 64) {
+            const int32_t j = c0 + lane;
+            V z = cmk<T>(0, 0), q = cmk<T>(0, 0);
+            if (j < s1) {
+                if (sizeof(T) == 4) {
+                    const float4 p = reinterpret_cast<const float4 *>(src_)[j];
+                    z = cmk<T>((T)p.x * inv_s, (T)p.y * inv_s);
+                    q = cmk<T>((T)p.z, 0);
+                } else {
+                    const double4 p = reinterpret_cast<const double4 *>(src_)[j];
+                    z = cmk<T>((T)((p.x - cx) * (double)inv_s), (T)((p.y - cy) * (double)inv_s));
+                    q = cmk<T>((T)p.z, 0);
+                }
+            }
+            for (int k = 0; k < p1; ++k) {
+                tab[lane * p1 + k] = q;
+                q = cmul(q, z);
+            }
+            __syncthreads();
+            const int cnt = min(64, s1 - c0);
+            if (lane < p1)
+                for (int jj = 0; jj < cnt; ++jj) {
+                    const V t = tab[jj * p1 + lane];
+                    acc.x += t.x;
+                    acc.y += t.y;
+                }
+            __syncthreads();
+        }
+    }
+    if (lane < p1) M[(lvl[L * (S + 1) + s] + b) * p1 + lane] = acc;
+}
+
+// --------------------------------------------------------- M2M / L2L ------
+template <class T, class TC>
+__global__ void batch_m2m(const BatchSys *__restrict__ sys, const int64_t *__restrict__ lvl,
+                          int S, int l, const int32_t *__restrict__ start, int p1,
+                          typename C2<T>::type *__restrict__ M,
+                          const typename C2<TC>::type *__restrict__ tab)
+{
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+    const int64_t *row = lvl + (int64_t)l * (S + 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = row[S] - row[0];
+    if (t >= nb * p1) return;
+    const int64_t gb = row[0] + t / p1;
+    const int m = (int)(t % p1);
+    const int s = find_seg(row, S, gb);
+    const int L = sys[s].L;
+    if (l >= L) return;  // level l is not a parent level of system s
+    const uint32_t b = (uint32_t)(gb - row[s]);
+    W acc = W{0, 0};
+    if (box_count(start + sys[s].leafbase, b, 2 * (L - l)) > 0) {
+        const int64_t cbase = lvl[(int64_t)(l + 1) * (S + 1) + s];
+        for (int c = 0; c < 4; ++c) {
+            const V *ch = M + (cbase + 4 * (int64_t)b + c) * p1;
+            const W *tr = tab + ((int64_t)c * p1 + m) * p1;
+            for (int k = 0; k <= m; ++k) {
+                const V a = ch[k];
+                const W tk = tr[k];
+                acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
+                acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
+            }
+        }
+    }
+    M[gb * p1 + m] = V{(T)acc.x, (T)acc.y};
+}
+
+template <class T, class TC>
+__global__ void batch_l2l(const BatchSys *__restrict__ sys, const int64_t *__restrict__ lvl,
+                          int S, int l, const int32_t *__restrict__ start, int p1,
+                          typename C2<T>::type *__restrict__ B,
+                          const typename C2<TC>::type *__restrict__ tab)
+{
+    // children at level l+1 receive their parent's local expansion
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+    const int64_t *row = lvl + (int64_t)(l + 1) * (S + 1);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nb = row[S] - row[0];
+    if (t >= nb * p1) return;
+    const int64_t gb = row[0] + t / p1;
+    const int m = (int)(t % p1);
+    const int s = find_seg(row, S, gb);
+    const int L = sys[s].L;
+    if (l < 2 || l + 1 > L) return;
+    const uint32_t b = (uint32_t)(gb - row[s]);
+    if (box_count(start + sys[s].leafbase, b, 2 * (L - l - 1)) == 0) return;
+    const V *par = B + (lvl[(int64_t)l * (S + 1) + s] + (b >> 2)) * p1;
+    const W *tr = tab + ((int64_t)(b & 3) * p1 + m) * p1;
+    W acc = W{0, 0};
+    for (int j = m; j < p1; ++j) {
+        const V a = par[j];
+        const W tk = tr[j];
+        acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
+        acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
+    }
+    const V cur = B[gb * p1 + m];
+    B[gb * p1 + m] = V{(T)((TC)cur.x + acc.x), (T)((TC)cur.y + acc.y)};
+}
+
+// --------------------------------------------------------------- M2L ------
+template <class T, int P1>
+__global__ __launch_bounds__(64) void batch_m2l(
+    const BatchSys *__restrict__ sys, const int64_t *__restrict__ lvl, int S, int l,
+    const BatchDom *__restrict__ dom, const int32_t *__restrict__ start, int p1, int ws,
+    const typename C2<T>::type *__restrict__ M, typename C2<T>::type *__restrict__ B,
+    const typename C2<T>::type *__restrict__ U, const T *__restrict__ binom,
+    int32_t *__restrict__ flag)
+{
+    using V = typename C2<T>::type;
+    __shared__ V Ap[64];
+    const int lane = threadIdx.x;
+    const int G = 64 / p1;
+    const int g = lane / p1, m = lane - g * p1;
+    const int64_t *row = lvl + (int64_t)l * (S + 1);
+    const int64_t nb = row[S] - row[0];
+    const int64_t slot = (int64_t)blockIdx.x * G + g;
+    bool active = g < G && slot < nb;
+    int s = 0, L = 0, bx = 0, by = 0, par = 0, n = 1 << l;
+    uint32_t b = 0;
+    double sl = 1.0, slp = 2.0, c2 = 0.0;
+    int Rp = 0, Rw = 0;
+    int64_t gb = 0;
+    if (active) {
+        gb = row[0] + slot;
+        s = find_seg(row, S, gb);
+        L = sys[s].L;
+        active = l <= L;
+        b = (uint32_t)(gb - row[s]);
+        bx = (int)compact_bits(b);
+        by = (int)compact_bits(b >> 1);
+        par = (bx & 1) | ((by & 1) << 1);
+        const double D = dom[s].D;
+        sl = D / ldexp(1.0, l);
+        slp = 2.0 * sl;
+        c2 = sys[s].c2;
+        Rp = near_radius(ws, slp, c2);
+        if (Rp > kRMax) {
+            if (m == 0) atomicOr(flag, 1);
+            Rp = kRMax;
+        }
+        Rw = 2 * Rp + 1;
+        if (active) active = box_count(start + sys[s].leafbase, b, 2 * (L - l)) > 0;
+    }
+    // window of candidate offsets, wave-uniform bound
+    int side = active ? 2 * Rw + 1 : 0;
+    int ncand = side * side;
+    for (int o = 32; o > 0; o >>= 1) ncand = max(ncand, __shfl_xor(ncand, o, 64));
+    T crow[P1];
+#pragma unroll
+    for (int k = 0; k < P1; ++k) crow[k] = (k < p1 && m < p1) ? binom[m * p1 + k] : (T)0;
+    const int W2 = 2 * kRU + 1;
+    const T sgn = (m & 1) ? (T)-1 : (T)1;
+    T ar = 0, ai = 0;
+    for (int t = 0; t < ncand; ++t) {
+        bool ok = active && t < side * side;
+        int tx = 0, ty = 0;
+        uint32_t sb = 0;
+        if (ok) {
+            tx = t % side - Rw;
+            ty = t / side - Rw;
+            const int sx = bx + tx, sy = by + ty;
+            ok = sx >= 0 && sy >= 0 && sx < n && sy < n;
+            if (ok) {
+                const int px = floordiv2(tx + (par & 1)), py = floordiv2(ty + (par >> 1));
+                ok = near_pred(px, py, ws, slp, c2) && !near_pred(tx, ty, ws, sl, c2);
+            }
+            if (ok) {
+                sb = morton2((uint32_t)sx, (uint32_t)sy);
+                ok = box_count(start + sys[s].leafbase, sb, 2 * (L - l)) > 0;
+            }
+        }
+        const V *Uo = U + (int64_t)((ty + kRU) * W2 + (tx + kRU)) * (p1 + 1);
+        if (ok) Ap[lane] = cmul(M[(row[s] + sb) * p1 + m], Uo[m]);
+        __syncthreads();
+        if (ok) {
+            T sr = 0, si = 0;
+            const V *ap = Ap + g * p1;
+#pragma unroll
+            for (int k = 0; k < P1; ++k)
+                if (k < p1) {
+                    const V v = ap[k];
+                    sr += crow[k] * v.x;
+                    si += crow[k] * v.y;
+                }
+            const V um = Uo[m + 1];
+            ar += sgn * (um.x * sr - um.y * si);
+            ai += sgn * (um.x * si + um.y * sr);
+        }
+        __syncthreads();
+    }
+    if (g < G && slot < nb && m < p1 && l <= L) {
+        const T inv_r = (T)(1.0 / sl);
+        B[gb * p1 + m] = active ? V{ar * inv_r, ai * inv_r} : V{0, 0};
+    }
+}
+
+// ------------------------------------------------------ P2P + L2P ---------
+template <int KIND>
+__global__ __launch_bounds__(64) void batch_p2p_f32c(
+    const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
+    const int64_t *__restrict__ lvl, int S, const BatchDom *__restrict__ dom,
+    const float4 *__restrict__ src, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ perm, int ws, const float2 *__restrict__ Bc, int p1,
+    double *__restrict__ vx, double *__restrict__ vy, int32_t *__restrict__ flag)
+{
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    const int32_t t0 = start[g], t1 = start[g + 1];
+    if (t1 <= t0) return;
+    const int s = find_seg(loff, S, g);
+    const BatchSys sy = sys[s];
+    const uint32_t b = (uint32_t)(g - sy.leafbase);
+    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
+    const int n = 1 << sy.L;
+    const double sd = dom[s].s;
+    const float s_f = (float)sd;
+    const float inv_s = 1.0f / s_f;
+    int R = near_radius(ws, sd, sy.c2);
+    if (R > kRMax) {
+        if (lane == 0) atomicOr(flag, 1);
+        R = kRMax;
+    }
+    PairParams pp{};
+    pp.nlog2e_ks_f = sy.nlog2e_ks_f;
+    pp.c2_f = sy.c2_f;
+    pp.scale_f = sy.scale_f;
+    const float2 *Bb = (sy.L >= 2 && Bc) ? Bc + (lvl[sy.L * (S + 1) + s] + b) * p1 : nullptr;
+    const double two_pi = 2.0 * 3.141592653589793;
+    for (int32_t tc = t0; tc < t1; tc += 64) {
+        const int32_t i = tc + lane;
+        const bool valid = i < t1;
+        const float4 tp = valid ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
+        for (int dy = -R; dy <= R; ++dy) {
+            for (int dx = -R; dx <= R; ++dx) {
+                const int sx = bx + dx, syy = by + dy;
+                if (sx < 0 || syy < 0 || sx >= n || syy >= n) continue;
+                if (!near_pred(dx, dy, ws, sd, sy.c2)) continue;
+                const int64_t sg = sy.leafbase + morton2((uint32_t)sx, (uint32_t)syy);
+                const int32_t j0 = start[sg], j1 = start[sg + 1];
+                if (j1 <= j0) continue;
+                const float xt = tp.x - (float)dx * s_f;
+                const float yt = tp.y - (float)dy * s_f;
+                int32_t j = j0;
+                for (; j + 16 <= j1; j += 16) {
+                    float aa = 0.f, ab = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        const float4 sp = src[j + k];
+                        const float ddx = xt - sp.x, ddy = yt - sp.y;
+                        const float P = pair_f32<KIND>(ddx, ddy, sp.z, pp);
+                        aa = __builtin_fmaf(P, ddx, aa);
+                        ab = __builtin_fmaf(P, ddy, ab);
+                    }
+                    two_sum_add(ha, la, aa);
+                    two_sum_add(hb, lb, ab);
+                }
+                if (j < j1) {
+                    float aa = 0.f, ab = 0.f;
+                    for (; j < j1; ++j) {
+                        const float4 sp = src[j];
+                        const float ddx = xt - sp.x, ddy = yt - sp.y;
+                        const float P = pair_f32<KIND>(ddx, ddy, sp.z, pp);
+                        aa = __builtin_fmaf(P, ddx, aa);
+                        ab = __builtin_fmaf(P, ddy, ab);
+                    }
+                    two_sum_add(ha, la, aa);
+                    two_sum_add(hb, lb, ab);
+                }
+            }
+        }
+        double fr = 0.0, fi = 0.0;
+        if (Bb) {
+            const float zr = tp.x * inv_s, zi = tp.y * inv_s;
+            float pr = 0.f, pi = 0.f;
+            for (int k = p1 - 1; k >= 0; --k) {
+                const float2 c = Bb[k];
+                const float nr = pr * zr - pi * zi + c.x;
+                const float ni = pr * zi + pi * zr + c.y;
+                pr = nr;
+                pi = ni;
+            }
+            fr = pr;
+            fi = pi;
+        }
+        if (valid) {
+            const int32_t o = perm[i];
+            const double qr = -(((double)ha + (double)la) + fr);
+            const double qi = ((double)hb + (double)lb) - fi;
+            vx[o] = -qi / two_pi;  // _induced.py:334-336
+            vy[o] = -qr / two_pi;
+        }
+    }
+}
+
+template <int KIND>
+__global__ __launch_bounds__(64) void batch_p2p_f64(
+    const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
+    const int64_t *__restrict__ lvl, int S, const BatchDom *__restrict__ dom,
+    const double4 *__restrict__ src, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ perm, int ws, const double2 *__restrict__ Bc, int p1,
+    double *__restrict__ vx, double *__restrict__ vy, int32_t *__restrict__ flag)
+{
+    const int lane = threadIdx.x;
+    const int64_t g = blockIdx.x;
+    const int32_t t0 = start[g], t1 = start[g + 1];
+    if (t1 <= t0) return;
+    const int s = find_seg(loff, S, g);
+    const BatchSys sy = sys[s];
+    const uint32_t b = (uint32_t)(g - sy.leafbase);
+    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
+    const int n = 1 << sy.L;
+    const BatchDom d = dom[s];
+    int R = near_radius(ws, d.s, sy.c2);
+    if (R > kRMax) {
+        if (lane == 0) atomicOr(flag, 1);
+        R = kRMax;
+    }
+    PairParams pp{};
+    pp.inv_ks = sy.inv_ks;
+    pp.c2 = sy.c2;
+    pp.scale = sy.scale;
+    double cx, cy;
+    leaf_center(b, d.x0, d.y0, d.s, cx, cy);
+    const double inv_s = 1.0 / d.s;
+    const double2 *Bb = (sy.L >= 2 && Bc) ? Bc + (lvl[sy.L * (S + 1) + s] + b) * p1 : nullptr;
+    const double two_pi = 2.0 * 3.141592653589793;
+    for (int32_t tc = t0; tc < t1; tc += 64) {
+        const int32_t i = tc + lane;
+        const bool valid = i < t1;
+        const double4 tp = valid ? src[i] : make_double4(cx, cy, 0.0, 0.0);
+        double a = 0.0, bb = 0.0;
+        for (int dy = -R; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx) {
+                const int sx = bx + dx, syy = by + dy;
+                if (sx < 0 || syy < 0 || sx >= n || syy >= n) continue;
+                if (!near_pred(dx, dy, ws, d.s, sy.c2)) continue;
+                const int64_t sg = sy.leafbase + morton2((uint32_t)sx, (uint32_t)syy);
+                for (int32_t j = start[sg]; j < start[sg + 1]; ++j) {
+                    const double4 sp = src[j];
+                    const double ddx = tp.x - sp.x, ddy = tp.y - sp.y;
+                    const double P = pair_f64<KIND>(ddx, ddy, sp.z, pp);
+                    a = __builtin_fma(P, ddx, a);
+                    bb = __builtin_fma(P, ddy, bb);
+                }
+            }
+        double fr = 0.0, fi = 0.0;
+        if (Bb) {
+            const double zr = (tp.x - cx) * inv_s, zi = (tp.y - cy) * inv_s;
+            for (int k = p1 - 1; k >= 0; --k) {
+                const double2 c = Bb[k];
+                const double nr = fr * zr - fi * zi + c.x;
+                const double ni = fr * zi + fi * zr + c.y;
+                fr = nr;
+                fi = ni;
+            }
+        }
+        if (valid) {
+            const int32_t o = perm[i];
+            const double qr = -(a + fr), qi = bb - fi;
+            vx[o] = -qi / two_pi;
+            vy[o] = -qr / two_pi;
+        }
+    }
+}
+
+// ------------------------------------------------------------- host ------
+namespace {
+
+double binom_h(int n, int k)
+{
+    if (k < 0 || k > n) return 0.0;
+    long double r = 1.0L;
+    for (int i = 1; i <= k; ++i) r = r * (long double)(n - k + i) / (long double)i;
+    return (double)r;
+}
+
+template <class T, int P1>
+int launch_bm2l(Lane *L, const BatchPlan &P, int l, int blocks, const void *M, void *B,
+                const void *U, const void *bin, int32_t *flag, const BatchSys *sys,
+                const int64_t *lvl, const BatchDom *dom, const int32_t *start)
+{
+    using V = typename C2<T>::type;
+    batch_m2l<T, P1><<<blocks, 64, 0, L->stream>>>(sys, lvl, P.S, l, dom, start, P.p + 1, P.ws,
+                                                   (const V *)M, (V *)B, (const V *)U,
+                                                   (const T *)bin, flag);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+template <class T>
+int dispatch_bm2l(Lane *L, const BatchPlan &P, int l, int blocks, const void *M, void *B,
+                  const void *U, const void *bin, int32_t *flag, const BatchSys *sys,
+                  const int64_t *lvl, const BatchDom *dom, const int32_t *start)
+{
+    const int p1 = P.p + 1;
+#define PHF_BM2L(Q) \
+    if (p1 <= Q) return launch_bm2l<T, Q>(L, P, l, blocks, M, B, U, bin, flag, sys, lvl, dom, start);
+    PHF_BM2L(9)
+    PHF_BM2L(13)
+    PHF_BM2L(17)
+    PHF_BM2L(21)
+    PHF_BM2L(25)
+    PHF_BM2L(32)
+    PHF_BM2L(41)
+#undef PHF_BM2L
+    set_error("expansion order %d too large (max 40)", P.p);
+    return PHF_ERR_UNSUPPORTED;
+}
+
+}  // namespace
+
+int batch_plan(const int64_t *sizes, const double *sigma, int S, int p, int leaf, int ws, int prec,
+               double k, BatchPlan *P)
+{
+    PHF_REQUIRE(S >= 1 && S <= 65536, "system count out of range (%d)", S);
+    PHF_REQUIRE(p >= 1 && p <= 40, "order out of range (%d)", p);
+    PHF_REQUIRE(leaf >= 1, "leaf must be >= 1");
+    P->S = S;
+    P->p = p;
+    P->ws = ws;
+    P->prec = prec;
+    P->sys.assign(S, BatchSys{});
+    P->poff.assign(S + 1, 0);
+    P->loff.assign(S + 1, 0);
+    int Lmax = 0;
+    for (int s = 0; s < S; ++s) {
+        PHF_REQUIRE(sizes[s] >= 0, "negative system size");
+        PHF_REQUIRE(std::isfinite(sigma[s]) && sigma[s] > 0, "sigma[%d] must be > 0", s);
+        const int L = fmm_level(sizes[s], leaf, 35);
+        BatchSys &b = P->sys[s];
+        b.off = P->poff[s];
+        b.n = (int32_t)sizes[s];
+        b.L = L;
+        b.leafbase = P->loff[s];
+        const double ks = k * sigma[s] * sigma[s];
+        const double xopt = std::sqrt(1.2564312086261696770 * ks);
+        const PairParams pp = make_oseen_params(xopt, 5.0 * xopt);
+        b.c2 = pp.c2;
+        b.inv_ks = pp.inv_ks;
+        b.scale = pp.scale;
+        b.c2_f = pp.c2_f;
+        b.nlog2e_ks_f = pp.nlog2e_ks_f;
+        b.scale_f = pp.scale_f;
+        b.nocut = pp.f32_nocut;
+        P->poff[s + 1] = P->poff[s] + sizes[s];
+        P->loff[s + 1] = P->loff[s] + (int64_t(1) << (2 * L));
+        Lmax = std::max(Lmax, L);
+    }
+    P->N = P->poff[S];
+    P->Lmax = Lmax;
+    P->nleaf = P->loff[S];
+    PHF_REQUIRE(P->nleaf < (int64_t(1) << 31) && P->N < (int64_t(1) << 31),
+                "batch too large (%lld points, %lld leaves)", (long long)P->N,
+                (long long)P->nleaf);
+    // level-major box bases
+    P->lvl.assign((size_t)(Lmax + 1) * (S + 1), 0);
+    int64_t base = 0;
+    for (int l = 0; l <= Lmax; ++l) {
+        for (int s = 0; s < S; ++s) {
+            P->lvl[(size_t)l * (S + 1) + s] = base;
+            if (P->sys[s].L >= l) base += int64_t(1) << (2 * l);
+        }
+        P->lvl[(size_t)l * (S + 1) + S] = base;
+    }
+    P->nbox = base;
+    P->nocut_all = true;
+    for (auto &b : P->sys) P->nocut_all = P->nocut_all && b.nocut;
+    return PHF_OK;
+}
+
+int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
+{
+    const int p1 = P.p + 1;
+    const bool f32 = P.prec == PHF_PREC_F32C;
+    const size_t N = (size_t)std::max<int64_t>(P.N, 1);
+    PHF_ALLOC(Bf->sys, L->arena, BatchSys, (size_t)P.S);
+    PHF_ALLOC(Bf->dom, L->arena, BatchDom, (size_t)P.S);
+    PHF_ALLOC(Bf->poff, L->arena, int64_t, (size_t)P.S + 1);
+    PHF_ALLOC(Bf->loff, L->arena, int64_t, (size_t)P.S + 1);
+    PHF_ALLOC(Bf->lvl, L->arena, int64_t, P.lvl.size());
+    PHF_ALLOC(Bf->k0, L->arena, uint32_t, N);
+    PHF_ALLOC(Bf->k1, L->arena, uint32_t, N);
+    PHF_ALLOC(Bf->v0, L->arena, int32_t, N);
+    PHF_ALLOC(Bf->v1, L->arena, int32_t, N);
+    PHF_ALLOC(Bf->start, L->arena, int32_t, (size_t)P.nleaf + 1);
+    PHF_ALLOC(Bf->src, L->arena, char, N * (f32 ? sizeof(float4) : sizeof(double4)));
+    const size_t csz = f32 ? sizeof(float2) : sizeof(double2);
+    PHF_ALLOC(Bf->M, L->arena, char, (size_t)std::max<int64_t>(P.nbox, 1) * p1 * csz);
+    PHF_ALLOC(Bf->B, L->arena, char, (size_t)std::max<int64_t>(P.nbox, 1) * p1 * csz);
+    PHF_ALLOC(Bf->flag, L->arena, int32_t, 1);
+    // constant tables: M2M/L2L shifts, M2L binomials and powers u^k over
+    // the interaction-offset window
+    const bool tc64 = !f32 || P.p > 16;
+    const size_t tsz = tc64 ? sizeof(double2) : sizeof(float2);
+    std::vector<char> h_m2m((size_t)4 * p1 * p1 * tsz), h_l2l((size_t)4 * p1 * p1 * tsz);
+    for (int c = 0; c < 4; ++c) {
+        std::complex<double> d((c & 1) * 0.5 - 0.25, (c >> 1) * 0.5 - 0.25);
+        for (int m = 0; m < p1; ++m)
+            for (int q = 0; q < p1; ++q) {
+                std::complex<double> v1 = 0, v2 = 0;
+                if (q <= m) v1 = binom_h(m, q) * std::ldexp(1.0, -q) * std::pow(d, m - q);
+                if (q >= m) v2 = binom_h(q, m) * std::pow(d, q - m) * std::ldexp(1.0, -m);
+                const size_t at = ((size_t)c * p1 + m) * p1 + q;
+                if (tc64) {
+                    reinterpret_cast<double2 *>(h_m2m.data())[at] = double2{v1.real(), v1.imag()};
+                    reinterpret_cast<double2 *>(h_l2l.data())[at] = double2{v2.real(), v2.imag()};
+                } else {
+                    reinterpret_cast<float2 *>(h_m2m.data())[at] =
+                        float2{(float)v1.real(), (float)v1.imag()};
+                    reinterpret_cast<float2 *>(h_l2l.data())[at] =
+                        float2{(float)v2.real(), (float)v2.imag()};
+                }
+            }
+    }
+    std::vector<char> h_bin((size_t)p1 * p1 * (f32 ? 4 : 8));
+    for (int m = 0; m < p1; ++m)
+        for (int q = 0; q < p1; ++q) {
+            const double v = binom_h(q + m, q);
+            if (f32)
+                reinterpret_cast<float *>(h_bin.data())[m * p1 + q] = (float)v;
+            else
+                reinterpret_cast<double *>(h_bin.data())[m * p1 + q] = v;
+        }
+    const int W2 = 2 * kRU + 1;
+    std::vector<char> h_U((size_t)W2 * W2 * (p1 + 1) * csz);
+    for (int oy = -kRU; oy <= kRU; ++oy)
+        for (int ox = -kRU; ox <= kRU; ++ox) {
+            std::complex<double> u = 0.0;
+            if (ox || oy) u = -1.0 / std::complex<double>(ox, oy);
+            std::complex<double> pw = 1.0;
+            for (int q = 0; q <= p1; ++q) {
+                const size_t at = ((size_t)(oy + kRU) * W2 + (ox + kRU)) * (p1 + 1) + q;
+                if (f32)
+                    reinterpret_cast<float2 *>(h_U.data())[at] =
+                        float2{(float)pw.real(), (float)pw.imag()};
+                else
+                    reinterpret_cast<double2 *>(h_U.data())[at] = double2{pw.real(), pw.imag()};
+                pw *= u;
+            }
+        }
+    PHF_ALLOC(Bf->m2m, L->arena, char, h_m2m.size());
+    PHF_ALLOC(Bf->l2l, L->arena, char, h_l2l.size());
+    PHF_ALLOC(Bf->bin, L->arena, char, h_bin.size());
+    PHF_ALLOC(Bf->U, L->arena, char, h_U.size());
+    // synchronous uploads: the plan is built once per batch
+    PHF_HIP(hipMemcpy(Bf->sys, P.sys.data(), sizeof(BatchSys) * P.S, hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->poff, P.poff.data(), sizeof(int64_t) * (P.S + 1), hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->loff, P.loff.data(), sizeof(int64_t) * (P.S + 1), hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->lvl, P.lvl.data(), sizeof(int64_t) * P.lvl.size(), hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->m2m, h_m2m.data(), h_m2m.size(), hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->l2l, h_l2l.data(), h_l2l.size(), hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->bin, h_bin.data(), h_bin.size(), hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->U, h_U.data(), h_U.size(), hipMemcpyHostToDevice));
+    // radix-sort scratch (fixed for the plan)
+    const int64_t rw = radix_ws_size(P.N);
+    PHF_ALLOC(Bf->hist, L->arena, uint32_t, (size_t)rw);
+    PHF_ALLOC(Bf->offs, L->arena, uint32_t, (size_t)rw);
+    PHF_ALLOC(Bf->sums, L->arena, uint32_t, (size_t)scan_sums_size(rw));
+    return PHF_OK;
+}
+
+template <class T, class TC>
+static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, const double *x,
+                               const double *y, const double *w, double *vx, double *vy)
+{
+    using V = typename C2<T>::type;
+    using VC = typename C2<TC>::type;
+    const int S = P.S, p1 = P.p + 1;
+    const int64_t N = P.N;
+    batch_clear_flag<<<1, 1, 0, L->stream>>>(Bf.flag);
+    batch_domain<<<S, 256, 0, L->stream>>>(Bf.sys, x, y, Bf.dom);
+    if (N > 0)
+        batch_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.poff, S, Bf.dom, x, y, N,
+                                                          Bf.k0, Bf.v0);
+    PHF_HIP(hipGetLastError());
+    int key_bits = 1;
+    while (key_bits < 32 && (int64_t(1) << key_bits) < P.nleaf) ++key_bits;
+    uint32_t *ks;
+    int32_t *vs;
+    PHF_TRY(device_radix_sort_ws(L, Bf.k0, Bf.v0, Bf.k1, Bf.v1, N, key_bits, Bf.hist, Bf.offs,
+                                 Bf.sums, &ks, &vs));
+    PHF_TRY(leaf_starts_device(L, ks, N, P.nleaf, Bf.start));
+    if (N > 0)
+        batch_gather<T><<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.loff, S, Bf.dom, x, y,
+                                                               w, ks, vs, N, Bf.src);
+    PHF_HIP(hipGetLastError());
+    V *M = reinterpret_cast<V *>(Bf.M), *B = reinterpret_cast<V *>(Bf.B);
+    const VC *m2m = reinterpret_cast<const VC *>(Bf.m2m);
+    const VC *l2l = reinterpret_cast<const VC *>(Bf.l2l);
+    if (P.Lmax >= 2) {
+        batch_p2m<T><<<(unsigned)P.nleaf, 64, (size_t)64 * p1 * sizeof(V), L->stream>>>(
+            Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, Bf.src, Bf.start, p1, M);
+        for (int l = P.Lmax - 1; l >= 2; --l) {
+            const int64_t nb = P.lvl[(size_t)l * (S + 1) + S] - P.lvl[(size_t)l * (S + 1)];
+            if (nb > 0)
+                batch_m2m<T, TC><<<div_up(nb * p1, 256), 256, 0, L->stream>>>(
+                    Bf.sys, Bf.lvl, S, l, Bf.start, p1, M, m2m);
+        }
+        PHF_HIP(hipGetLastError());
+        for (int l = 2; l <= P.Lmax; ++l) {
+            const int64_t nb = P.lvl[(size_t)l * (S + 1) + S] - P.lvl[(size_t)l * (S + 1)];
+            const int G = 64 / p1;
+            if (nb > 0)
+                PHF_TRY(dispatch_bm2l<T>(L, P, l, (int)((nb + G - 1) / G), M, B, Bf.U, Bf.bin,
+                                         Bf.flag, Bf.sys, Bf.lvl, Bf.dom, Bf.start));
+        }
+        for (int l = 2; l < P.Lmax; ++l) {
+            const int64_t nb =
+                P.lvl[(size_t)(l + 1) * (S + 1) + S] - P.lvl[(size_t)(l + 1) * (S + 1)];
+            if (nb > 0)
+                batch_l2l<T, TC><<<div_up(nb * p1, 256), 256, 0, L->stream>>>(
+                    Bf.sys, Bf.lvl, S, l, Bf.start, p1, B, l2l);
+        }
+        PHF_HIP(hipGetLastError());
+    }
+    const V *Bc = P.Lmax >= 2 ? B : nullptr;
+    if (P.nleaf > 0) {
+        if (sizeof(T) == 4) {
+            if (P.nocut_all)
+                batch_p2p_f32c<PAIR_OSEEN_NOCUT><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
+                    Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, vs,
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag);
+            else
+                batch_p2p_f32c<PAIR_OSEEN><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
+                    Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, vs,
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag);
+        } else {
+            batch_p2p_f64<PAIR_OSEEN><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
+                Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const double4 *)Bf.src, Bf.start, vs, P.ws,
+                (const double2 *)Bc, p1, vx, vy, Bf.flag);
+        }
+    }
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+int batch_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bf, const double *x, const double *y,
+                  const double *w, double *vx, double *vy)
+{
+    if (P.prec == PHF_PREC_F32C) {
+        if (P.p > 16) return batch_enqueue_typed<float, double>(L, P, Bf, x, y, w, vx, vy);
+        return batch_enqueue_typed<float, float>(L, P, Bf, x, y, w, vx, vy);
+    }
+    return batch_enqueue_typed<double, double>(L, P, Bf, x, y, w, vx, vy);
+}
+
+}  // namespace phf
+
+// ------------------------------------------------------------ C entry ----
+using namespace phf;
+
+struct phf_batch_s {
+    std::mutex mu;
+    Lane lane;
+    BatchPlan plan;
+    BatchBuffers buf;
+    double *x = nullptr, *y = nullptr, *w = nullptr, *vx = nullptr, *vy = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int phf_batch_destroy(phf_batch *b)
+{
+    clear_error();
+    if (!b) return PHF_OK;
+    {
+        std::lock_guard<std::mutex> lk(b->mu);
+        if (b->lane.stream) (void)hipStreamSynchronize(b->lane.stream);
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->lane.arena.release();
+        b->lane.pin.release();
+        for (auto &ev : b->lane.ev)
+            if (ev) (void)hipEventDestroy(ev);
+        if (b->lane.stream) (void)hipStreamDestroy(b->lane.stream);
+    }
+    delete b;
+    return PHF_OK;
+}
+
+int phf_batch_create(const int64_t *sizes, const double *sigma, int32_t count,
+                     const phf_batch_params *bp, phf_batch **out)
+{
+    clear_error();
+    PHF_REQUIRE(out && sizes && sigma && bp, "NULL argument");
+    *out = nullptr;
+    PHF_REQUIRE(bp->prec == PHF_PREC_F64 || bp->prec == PHF_PREC_F32C, "unknown precision %d",
+                bp->prec);
+    PHF_REQUIRE(std::isfinite(bp->k) && bp->k > 0, "k must be > 0");
+    PHF_TRY(Context::get().init());
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    phf_batch *b = new phf_batch();
+    int rc = [&]() -> int {
+        PHF_TRY(batch_plan(sizes, sigma, count, bp->order > 0 ? bp->order : 16,
+                           bp->leaf > 0 ? bp->leaf : 64, bp->ws > 0 ? bp->ws : 1, bp->prec, bp->k,
+                           &b->plan));
+        PHF_HIP(hipStreamCreateWithFlags(&b->lane.stream, hipStreamNonBlocking));
+        for (auto &ev : b->lane.ev) PHF_HIP(hipEventCreate(&ev));
+        Lane *L = &b->lane;
+        const size_t N = (size_t)std::max<int64_t>(b->plan.N, 1);
+        PHF_ALLOC(b->x, L->arena, double, N);
+        PHF_ALLOC(b->y, L->arena, double, N);
+        PHF_ALLOC(b->w, L->arena, double, N);
+        PHF_ALLOC(b->vx, L->arena, double, N);
+        PHF_ALLOC(b->vy, L->arena, double, N);
+        PHF_TRY(batch_alloc(L, b->plan, &b->buf));
+        return PHF_OK;
+    }();
+    if (rc != PHF_OK) {
+        phf_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PHF_OK;
+}
+
+int phf_batch_set_system(phf_batch *b, int32_t s, const double *x, const double *y,
+                         const double *w)
+{
+    clear_error();
+    PHF_REQUIRE(b != nullptr, "batch is NULL");
+    PHF_REQUIRE(s >= 0 && s < b->plan.S, "system index %d out of range", s);
+    const int64_t n = b->plan.sys[s].n, o = b->plan.sys[s].off;
+    PHF_TRY(check_host_array(x, n, "x"));
+    PHF_TRY(check_host_array(y, n, "y"));
+    PHF_TRY(check_host_array(w, n, "w"));
+    std::lock_guard<std::mutex> lk(b->mu);
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    if (n > 0) {
+        PHF_HIP(hipMemcpyAsync(b->x + o, x, sizeof(double) * n, hipMemcpyHostToDevice, b->lane.stream));
+        PHF_HIP(hipMemcpyAsync(b->y + o, y, sizeof(double) * n, hipMemcpyHostToDevice, b->lane.stream));
+        PHF_HIP(hipMemcpyAsync(b->w + o, w, sizeof(double) * n, hipMemcpyHostToDevice, b->lane.stream));
+    }
+    PHF_HIP(hipStreamSynchronize(b->lane.stream));
+    return PHF_OK;
+}
+
+int phf_batch_velocity(phf_batch *b, int32_t use_graph, phf_batch_info *info)
+{
+    clear_error();
+    PHF_REQUIRE(b != nullptr, "batch is NULL");
+    std::lock_guard<std::mutex> lk(b->mu);
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    Lane *L = &b->lane;
+    hipEvent_t e0 = L->ev[0], e1 = L->ev[1];
+    if (use_graph && !b->exec) {
+        PHF_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+        int rc = batch_enqueue(L, b->plan, b->buf, b->x, b->y, b->w, b->vx, b->vy);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(L->stream, &g);
+        if (rc != PHF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PHF_HIP(e);
+        b->graph = g;
+        PHF_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    PHF_HIP(hipEventRecord(e0, L->stream));
+    if (use_graph)
+        PHF_HIP(hipGraphLaunch(b->exec, L->stream));
+    else
+        PHF_TRY(batch_enqueue(L, b->plan, b->buf, b->x, b->y, b->w, b->vx, b->vy));
+    PHF_HIP(hipEventRecord(e1, L->stream));
+    int32_t flag = 0;
+    PHF_HIP(hipMemcpyAsync(&flag, b->buf.flag, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (info) {
+        info->ms = ms;
+        info->graph = use_graph ? 1 : 0;
+        info->flags = flag;
+        info->total_points = b->plan.N;
+        info->levels_max = b->plan.Lmax;
+    }
+    if (flag) {
+        set_error("a system's smoothing cutoff spans more than %d leaf boxes; use a larger leaf "
+                  "size", 6);
+        return PHF_ERR_UNSUPPORTED;
+    }
+    return PHF_OK;
+}
+
+int phf_batch_get_velocity(phf_batch *b, int32_t s, double *vx, double *vy)
+{
+    clear_error();
+    PHF_REQUIRE(b != nullptr, "batch is NULL");
+    PHF_REQUIRE(s >= 0 && s < b->plan.S, "system index %d out of range", s);
+    std::lock_guard<std::mutex> lk(b->mu);
+    const int64_t n = b->plan.sys[s].n, o = b->plan.sys[s].off;
+    PHF_REQUIRE(n == 0 || (vx && vy), "output arrays are NULL");
+    if (n > 0) {
+        PHF_HIP(hipMemcpyAsync(vx, b->vx + o, sizeof(double) * n, hipMemcpyDeviceToHost, b->lane.stream));
+        PHF_HIP(hipMemcpyAsync(vy, b->vy + o, sizeof(double) * n, hipMemcpyDeviceToHost, b->lane.stream));
+    }
+    PHF_HIP(hipStreamSynchronize(b->lane.stream));
+    return PHF_OK;
+}
+
+}  // extern "C"

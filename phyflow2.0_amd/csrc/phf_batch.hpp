@@ -1,0 +1,54 @@
+// phf_batch.hpp -- segment-table batched FMM over many blob systems.
+#pragma once
+
+#include <vector>
+
+#include "phf_fmm.hpp"
+
+namespace phf {
+
+// per-system description (host-built once per batch, read by the kernels)
+struct BatchSys {
+    int64_t off;       // first point in the packed arrays
+    int32_t n, L;      // points, leaf level
+    int64_t leafbase;  // first global leaf
+    double c2, inv_ks, scale;            // Oseen smoother (float64)
+    float c2_f, nlog2e_ks_f, scale_f;    // float32 forms
+    int32_t nocut;                       // float32 cutoff branch is a no-op
+};
+
+// per-system square domain, written on the device every evaluation
+struct BatchDom {
+    double x0, y0, D, s, inv_s;  // origin, side, leaf side, 2^L / D
+};
+
+struct BatchPlan {
+    int S = 0, p = 0, ws = 1, prec = 0, Lmax = 0;
+    int64_t N = 0, nleaf = 0, nbox = 0;
+    bool nocut_all = true;
+    std::vector<BatchSys> sys;
+    std::vector<int64_t> poff, loff, lvl;  // lvl: (Lmax+1) x (S+1), level-major box bases
+};
+
+struct BatchBuffers {
+    BatchSys *sys = nullptr;
+    BatchDom *dom = nullptr;
+    int64_t *poff = nullptr, *loff = nullptr, *lvl = nullptr;
+    uint32_t *k0 = nullptr, *k1 = nullptr;
+    int32_t *v0 = nullptr, *v1 = nullptr;
+    int32_t *start = nullptr;
+    char *src = nullptr, *M = nullptr, *B = nullptr;
+    char *m2m = nullptr, *l2l = nullptr, *bin = nullptr, *U = nullptr;
+    uint32_t *hist = nullptr, *offs = nullptr, *sums = nullptr;
+    int32_t *flag = nullptr;
+};
+
+int batch_plan(const int64_t *sizes, const double *sigma, int S, int p, int leaf, int ws, int prec,
+               double k, BatchPlan *P);
+int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf);
+// Enqueue one evaluation on L->stream (no allocation, no host sync: safe to
+// capture).  vx, vy: device, packed input order.
+int batch_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bf, const double *x, const double *y,
+                  const double *w, double *vx, double *vy);
+
+}  // namespace phf

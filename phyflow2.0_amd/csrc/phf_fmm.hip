@@ -31,6 +31,7 @@
 // per-target sums, P2M/M2L/L2P in float32, and M2M/L2L in float64 when the
 // order exceeds 16.
 #include "phf_fmm.hpp"
+#include "phf_fmm_dev.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -39,28 +40,6 @@
 #include <vector>
 
 namespace phf {
-
-template <class T> struct C2;
-template <> struct C2<float> { using type = float2; };
-template <> struct C2<double> { using type = double2; };
-
-template <class T> __device__ __forceinline__ typename C2<T>::type cmk(T r, T i)
-{
-    return typename C2<T>::type{r, i};
-}
-template <class V> __device__ __forceinline__ V cmul(V a, V b)
-{
-    return V{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-
-// ------------------------------------------------------------ gathers ----
-__device__ __forceinline__ void leaf_center(uint32_t key, double x0, double y0, double s,
-                                            double &cx, double &cy)
-{
-    uint32_t ix = compact_bits(key), iy = compact_bits(key >> 1);
-    cx = x0 + ((double)ix + 0.5) * s;
-    cy = y0 + ((double)iy + 0.5) * s;
-}
 
 __global__ void gather_src_f32(const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ w, const uint32_t *__restrict__ skeys,
@@ -106,12 +85,6 @@ __global__ void gather_tgt_f64(const double *__restrict__ x, const double *__res
     if (i >= n) return;
     int32_t j = perm[i];
     out[i] = make_double2(x[j], y[j]);
-}
-
-// box counts at level l from leaf offsets
-__device__ __forceinline__ int box_count(const int32_t *__restrict__ start, uint32_t b, int shift)
-{
-    return start[(int64_t)(b + 1) << shift] - start[(int64_t)b << shift];
 }
 
 // ---------------------------------------------------------------- P2M ----

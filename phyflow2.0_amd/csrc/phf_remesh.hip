@@ -374,6 +374,118 @@ int moments_device(Lane *L, const double *x, const double *y, const double *w, i
     return PHF_OK;
 }
 
+// ---------------------------------------------------- population control --
+// PopulationControl (_regridding.py:227-330): discard |w| < thr, where thr
+// starts at wThreshold and is divided by 10 until the discarded total |w|
+// falls below wTotalThreshold; kept blobs keep their order.  The losses of
+// all candidate thresholds are formed in one pass (float64, fixed order),
+// the host picks the threshold, a stable scan compacts.
+constexpr int PC_K = 32;
+
+__global__ __launch_bounds__(256) void pc_loss_partial(const double *__restrict__ w, int64_t n,
+                                                       const double *__restrict__ thr, int K,
+                                                       double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    double acc[PC_K];
+#pragma unroll
+    for (int k = 0; k < PC_K; ++k) acc[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double a = fabs(w[i]);
+#pragma unroll
+        for (int k = 0; k < PC_K; ++k)
+            if (k < K && a < thr[k]) acc[k] += a;
+    }
+    for (int k = 0; k < K; ++k) {
+        sh[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[(int64_t)blockIdx.x * K + k] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void pc_loss_final(const double *__restrict__ part, int nblk, int K,
+                              double *__restrict__ loss)
+{
+    const int k = threadIdx.x;
+    if (k >= K) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * K + k];
+    loss[k] = s;
+}
+
+__global__ void pc_flags(const double *__restrict__ w, int64_t n, double thr,
+                         uint32_t *__restrict__ keep)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keep[i] = (fabs(w[i]) < thr) ? 0u : 1u;
+}
+
+__global__ void pc_scatter(const double *__restrict__ x, const double *__restrict__ y,
+                           const double *__restrict__ w, int64_t n,
+                           const uint32_t *__restrict__ keep, const uint32_t *__restrict__ pos,
+                           double *__restrict__ xo, double *__restrict__ yo, double *__restrict__ wo)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !keep[i]) return;
+    const uint32_t o = pos[i];
+    xo[o] = x[i];
+    yo[o] = y[i];
+    wo[o] = w[i];
+}
+
+int population_control_device(Lane *L, const double *x, const double *y, const double *w,
+                              int64_t n, double wThreshold, double wTotalThreshold, double *xo,
+                              double *yo, double *wo, int64_t *m_out, double *thr_used)
+{
+    *m_out = 0;
+    if (thr_used) *thr_used = wThreshold;
+    if (n == 0) return PHF_OK;
+    // candidate thresholds by repeated division, as the reference loop does
+    std::vector<double> thr(PC_K);
+    thr[0] = wThreshold;
+    for (int k = 1; k < PC_K; ++k) thr[k] = thr[k - 1] / 10.0;
+    const int nblk = (int)std::min<int64_t>(256, (n + 255) / 256);
+    double *dthr, *part, *loss;
+    PHF_ALLOC(dthr, L->arena, double, PC_K);
+    PHF_ALLOC(part, L->arena, double, (size_t)nblk * PC_K);
+    PHF_ALLOC(loss, L->arena, double, PC_K);
+    PHF_TRY(L->pin.ensure(sizeof(double) * 2 * PC_K));
+    double *hp = reinterpret_cast<double *>(L->pin.ptr);
+    std::memcpy(hp, thr.data(), sizeof(double) * PC_K);
+    PHF_HIP(hipMemcpyAsync(dthr, hp, sizeof(double) * PC_K, hipMemcpyHostToDevice, L->stream));
+    pc_loss_partial<<<nblk, 256, 0, L->stream>>>(w, n, dthr, PC_K, part);
+    pc_loss_final<<<1, 64, 0, L->stream>>>(part, nblk, PC_K, loss);
+    PHF_HIP(hipGetLastError());
+    PHF_HIP(hipMemcpyAsync(hp + PC_K, loss, sizeof(double) * PC_K, hipMemcpyDeviceToHost,
+                           L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    double t = 0.0;  // no acceptable threshold among the candidates: keep all
+    for (int k = 0; k < PC_K; ++k)
+        if (hp[PC_K + k] < wTotalThreshold) {
+            t = thr[k];
+            break;
+        }
+    if (thr_used) *thr_used = t;
+    uint32_t *keep, *pos;
+    PHF_ALLOC(keep, L->arena, uint32_t, (size_t)n);
+    PHF_ALLOC(pos, L->arena, uint32_t, (size_t)n);
+    pc_flags<<<div_up(n, 256), 256, 0, L->stream>>>(w, n, t, keep);
+    PHF_TRY(device_exclusive_scan(L, keep, n, pos));
+    pc_scatter<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, w, n, keep, pos, xo, yo, wo);
+    PHF_HIP(hipGetLastError());
+    uint32_t tail[2];
+    PHF_HIP(hipMemcpyAsync(&tail[0], pos + n - 1, 4, hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipMemcpyAsync(&tail[1], keep + n - 1, 4, hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    *m_out = (int64_t)tail[0] + tail[1];
+    return PHF_OK;
+}
+
 }  // namespace phf
 
 // ------------------------------------------------------------ C entry ----
@@ -486,11 +598,66 @@ int phf_blobset_copy(const phf_blobset *b, double *x, double *y, double *w, int6
     if (x) std::memcpy(x, b->x.data(), sizeof(double) * b->m);
     if (y) std::memcpy(y, b->y.data(), sizeof(double) * b->m);
     if (w) std::memcpy(w, b->w.data(), sizeof(double) * b->m);
-    if (i) std::memcpy(i, b->i.data(), sizeof(int64_t) * b->m);
-    if (j) std::memcpy(j, b->j.data(), sizeof(int64_t) * b->m);
+    if (i && !b->i.empty()) std::memcpy(i, b->i.data(), sizeof(int64_t) * b->m);
+    if (j && !b->j.empty()) std::memcpy(j, b->j.data(), sizeof(int64_t) * b->m);
     return PHF_OK;
 }
 
 void phf_blobset_free(phf_blobset *b) { delete b; }
+
+int phf_population_control(const double *x, const double *y, const double *w, int64_t n,
+                           double wThreshold, double wTotalThreshold, phf_blobset **out)
+{
+    clear_error();
+    PHF_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    PHF_REQUIRE(n >= 0 && n < (int64_t(1) << 31), "n out of range (%lld)", (long long)n);
+    PHF_REQUIRE(std::isfinite(wThreshold) && wThreshold >= 0 && std::isfinite(wTotalThreshold),
+                "thresholds must be finite (wThreshold >= 0)");
+    PHF_TRY(check_host_array(x, n, "xBlob"));
+    PHF_TRY(check_host_array(y, n, "yBlob"));
+    PHF_TRY(check_host_array(w, n, "wBlob", false));
+    phf_blobset_s *res = new phf_blobset_s();
+    if (n == 0) {
+        *out = res;
+        return PHF_OK;
+    }
+    LaneGuard g;
+    if (g.rc != PHF_OK) {
+        delete res;
+        return g.rc;
+    }
+    Lane *L = g.lane;
+    int rc = [&]() -> int {
+        PHF_TRY(L->arena.reset());
+        double *dx, *dy, *dw, *xo, *yo, *wo;
+        PHF_TRY(upload(L, x, n, &dx));
+        PHF_TRY(upload(L, y, n, &dy));
+        PHF_TRY(upload(L, w, n, &dw));
+        PHF_ALLOC(xo, L->arena, double, (size_t)n);
+        PHF_ALLOC(yo, L->arena, double, (size_t)n);
+        PHF_ALLOC(wo, L->arena, double, (size_t)n);
+        int64_t m = 0;
+        PHF_TRY(population_control_device(L, dx, dy, dw, n, wThreshold, wTotalThreshold, xo, yo,
+                                          wo, &m, nullptr));
+        res->m = m;
+        res->x.resize(m);
+        res->y.resize(m);
+        res->w.resize(m);
+        if (m > 0) {
+            PHF_TRY(download(L, xo, m, res->x.data()));
+            PHF_TRY(download(L, yo, m, res->y.data()));
+            PHF_TRY(download(L, wo, m, res->w.data()));
+        }
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        return PHF_OK;
+    }();
+    if (rc != PHF_OK) {
+        delete res;
+        return rc;
+    }
+    *out = res;
+    return PHF_OK;
+}
 
 }  // extern "C"

@@ -24,4 +24,9 @@ int m4kernel_device(Lane *L, const double *x, int64_t n, double c, double *w);
 int moments_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
                    double4 *d_out);
 
+// PopulationControl: compact (x, y, w) into (xo, yo, wo) (capacity n).
+int population_control_device(Lane *L, const double *x, const double *y, const double *w,
+                              int64_t n, double wThreshold, double wTotalThreshold, double *xo,
+                              double *yo, double *wo, int64_t *m_out, double *thr_used);
+
 }  // namespace phf

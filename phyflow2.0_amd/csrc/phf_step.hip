@@ -370,6 +370,20 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
         PHF_HIP(hipMemcpyAsync(s->y, ro.y, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
         PHF_HIP(hipMemcpyAsync(s->w, ro.w, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
         s->n = ro.m;
+        if (sp->pc_enabled) {
+            // Blobs.evolve: redistribute, then populationControl (__blobsClass.py:771-781)
+            double *px, *py, *pw;
+            PHF_ALLOC(px, L->arena, double, (size_t)std::max<int64_t>(s->n, 1));
+            PHF_ALLOC(py, L->arena, double, (size_t)std::max<int64_t>(s->n, 1));
+            PHF_ALLOC(pw, L->arena, double, (size_t)std::max<int64_t>(s->n, 1));
+            int64_t m = 0;
+            PHF_TRY(population_control_device(L, s->x, s->y, s->w, s->n, sp->pc_w_threshold,
+                                              sp->pc_total_threshold, px, py, pw, &m, nullptr));
+            PHF_HIP(hipMemcpyAsync(s->x, px, sizeof(double) * m, hipMemcpyDeviceToDevice, L->stream));
+            PHF_HIP(hipMemcpyAsync(s->y, py, sizeof(double) * m, hipMemcpyDeviceToDevice, L->stream));
+            PHF_HIP(hipMemcpyAsync(s->w, pw, sizeof(double) * m, hipMemcpyDeviceToDevice, L->stream));
+            s->n = m;
+        }
         PHF_TRY(moments_device(L, s->x, s->y, s->w, s->n, dm + 1));
         PHF_HIP(hipMemcpyAsync(&hm[1], dm + 1, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
     } else if (n > 0) {

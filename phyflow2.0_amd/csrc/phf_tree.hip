@@ -210,12 +210,21 @@ __global__ __launch_bounds__(SC_THREADS) void scan_apply(const uint32_t *__restr
     }
 }
 
+int64_t scan_sums_size(int64_t n) { return (n + SC_CHUNK - 1) / SC_CHUNK + 1; }
+
 int device_exclusive_scan(Lane *L, const uint32_t *in, int64_t n, uint32_t *out)
 {
     if (n <= 0) return PHF_OK;
-    int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
     uint32_t *sums;
-    PHF_ALLOC(sums, L->arena, uint32_t, (size_t)nb);
+    PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(n));
+    return device_exclusive_scan_ws(L, in, n, out, sums);
+}
+
+int device_exclusive_scan_ws(Lane *L, const uint32_t *in, int64_t n, uint32_t *out,
+                             uint32_t *sums)
+{
+    if (n <= 0) return PHF_OK;
+    int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
     scan_reduce<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums);
     scan_top<<<1, SC_THREADS, 0, L->stream>>>(sums, nb);
     scan_apply<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums, out);
@@ -292,23 +301,42 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     }
 }
 
+int64_t radix_ws_size(int64_t n)
+{
+    const int64_t nblocks = std::max<int64_t>(1, (n + RS_TILE - 1) / RS_TILE);
+    return 256 * nblocks;
+}
+
 int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
                       int64_t n, int key_bits, uint32_t **kres, int32_t **vres)
 {
     *kres = keys;
     *vres = vals;
     if (n <= 1 || key_bits <= 0) return PHF_OK;
+    const int64_t ws = radix_ws_size(n);
+    uint32_t *hist, *offs, *sums;
+    PHF_ALLOC(hist, L->arena, uint32_t, (size_t)ws);
+    PHF_ALLOC(offs, L->arena, uint32_t, (size_t)ws);
+    PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(ws));
+    return device_radix_sort_ws(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, sums, kres,
+                                vres);
+}
+
+int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
+                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
+                         uint32_t **kres, int32_t **vres)
+{
+    *kres = keys;
+    *vres = vals;
+    if (n <= 1 || key_bits <= 0) return PHF_OK;
     const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
-    uint32_t *hist, *offs;
-    PHF_ALLOC(hist, L->arena, uint32_t, (size_t)256 * nblocks);
-    PHF_ALLOC(offs, L->arena, uint32_t, (size_t)256 * nblocks);
     uint32_t *ka = keys, *kb = ktmp;
     int32_t *va = vals, *vb = vtmp;
     for (int shift = 0; shift < key_bits; shift += 8) {
         const int nbits = std::min(8, key_bits - shift);
         const uint32_t mask = (1u << nbits) - 1u;
         rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
-        PHF_TRY(device_exclusive_scan(L, hist, (int64_t)256 * nblocks, offs));
+        PHF_TRY(device_exclusive_scan_ws(L, hist, (int64_t)256 * nblocks, offs, sums));
         rs_scatter<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits, offs,
                                                           nblocks);
         PHF_HIP(hipGetLastError());

@@ -22,6 +22,14 @@ int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const d
                 const double *y2, int64_t n2, double4 *d_out);
 
 int device_exclusive_scan(Lane *L, const uint32_t *in, int64_t n, uint32_t *out);
+// variants with caller-provided workspaces (no allocation: capturable)
+int64_t scan_sums_size(int64_t n);
+int device_exclusive_scan_ws(Lane *L, const uint32_t *in, int64_t n, uint32_t *out,
+                             uint32_t *sums);
+int64_t radix_ws_size(int64_t n);  // entries of hist and of offs
+int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
+                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
+                         uint32_t **kres, int32_t **vres);
 
 // Stable LSD radix sort of (keys, vals) on the low key_bits bits; ktmp/vtmp
 // are ping-pong buffers.  *kres/*vres point at whichever buffer holds the

@@ -94,6 +94,12 @@ def step_rk2(system, params, dt, vInf=(0.0, 0.0), panels=None, remesh=None, pane
         sp.remesh_x0 = float(remesh.get("x0", -1.5 * h))  # Blobs.redistribute bounds
         sp.remesh_y0 = float(remesh.get("y0", -1.5 * h))
         sp.remesh_c = float(remesh.get("c", 0.0))
+        if remesh.get("population_control"):
+            # Blobs defaults gThresholdLocal = gThresholdGlobal = 1e-8 (__blobsClass.py:105-106)
+            local, glob = remesh["population_control"]
+            sp.pc_enabled = 1
+            sp.pc_w_threshold = float(local)
+            sp.pc_total_threshold = float(glob)
     info = StepInfo()
     call("phf_system_step_rk2", system._h, ctypes.byref(params),
          panels.handle if panels is not None else None, ctypes.byref(sp), ctypes.byref(info))

@@ -262,3 +262,20 @@ def test_morton_binning_and_stable_sort(O):
     assert start[0] == 0 and start[-1] == n
     cnt = np.bincount(sk.astype(np.int64), minlength=4 ** L)
     assert np.array_equal(np.diff(start), cnt)
+
+
+def test_population_control_loop(O):
+    rng = np.random.default_rng(12)
+    w = rng.standard_normal(1000) * 10.0 ** rng.uniform(-12, -4, 1000)
+    x, y = rng.random((2, 1000))
+    xo, yo, wo, t = O.population_control(x, y, w, 1e-8, 1e-6)
+    # brute force of the reference loop
+    thr = 1e-8
+    while np.sum(np.abs(w[np.abs(w) < thr])) >= 1e-6:
+        thr /= 10.0
+    assert t == thr
+    keep = ~(np.abs(w) < thr)
+    assert np.array_equal(wo, w[keep]) and np.array_equal(xo, x[keep])
+    # a strict total threshold forces at least one division by 10
+    _, _, w2, t2 = O.population_control(x, y, w, 1e-5, 1e-9)
+    assert t2 < 1e-5 and np.sum(np.abs(w[np.abs(w) < t2])) < 1e-9

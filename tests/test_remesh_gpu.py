@@ -79,3 +79,16 @@ def test_regrid_deterministic():
 def test_regrid_empty():
     out = Regrid(np.empty(0), np.empty(0), np.empty(0), 0.1, 1.0, [0, 1], [0, 1])
     assert all(v.size == 0 for v in out)
+
+
+def test_population_control_matches_oracle(O):
+    from phyflow_amd.blobs.base.regrid import PopulationControl
+    rng = np.random.default_rng(13)
+    n = 200000
+    w = rng.standard_normal(n) * 10.0 ** rng.uniform(-12, -4, n)
+    x, y = rng.random((2, n))
+    for thr, tot in ((1e-8, 1e-8), (1e-6, 1e-7), (1e-10, 1.0)):
+        got = PopulationControl(x, y, w, thr, tot)
+        ref = O.population_control(x, y, w, thr, tot)
+        for a, b in zip(got, ref[:3]):
+            assert np.array_equal(a, b)
