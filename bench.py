@@ -178,12 +178,17 @@ def bench_cylinder(args):
 
 def bench_mix(args):
     """256 independent uniform systems, N log-uniform in [1024, 262144]
-    (seed 0), h = 1/sqrt(N), sigma = h/0.8: batched F32C FMM (order 16,
-    leaf 64) replayed from one captured hipGraph, vs the same mix as 256
-    one-at-a-time float64 evaluations (one-shot calls with per-call copies,
-    as the reference's GPU path does, and resident systems)."""
+    (seed 0), h = 1/sqrt(N), sigma = h/0.8.  One step of every system =
+    velocity at the blob centres (F32C FMM, order 16, leaf 64), forward-Euler
+    convection (dt 1e-3) and M4' remesh onto the lattice h (origin -1.5 h),
+    all systems in one captured hipGraph replay.  Baselines: the same mix as
+    256 one-at-a-time float64 steps (one-shot fmm2d + Regrid calls with
+    per-call copies, as the reference's GPU path does) and resident float64
+    velocity evaluations one after another."""
     from phyflow_amd.batch import BatchSystems
+    from phyflow_amd.blobs.base.regrid import Regrid
     count = args.systems
+    dt, overlap = 1e-3, 0.8
     sizes = W.uneven_sizes(count, 1024, 262144, seed=0)
     systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
     total = int(np.sum(sizes))
@@ -191,17 +196,24 @@ def bench_mix(args):
                       leaf=64) as b:
         for i, s in enumerate(systems):
             b.set_system(i, s["x"], s["y"], s["w"])
+        b.set_remesh(overlap, dt=dt)
         for _ in range(args.warmup):
-            b.velocity(use_graph=True)
+            b.step(use_graph=True)
         lat = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
             a = time.perf_counter()
-            info = b.velocity(use_graph=True)
+            info = b.step(use_graph=True)
             lat.append(time.perf_counter() - a)
         wall = time.perf_counter() - t0
-        dev_ms = info["ms"]
-        errs = []
+        dev_ms, remeshed, spilled = info["ms"], info["remeshed"], info["spilled"]
+        for _ in range(args.warmup):
+            b.velocity(use_graph=True)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            vinfo = b.velocity(use_graph=True)
+        wall_v = time.perf_counter() - t
+        errs, remesh_exact = [], True
         check = list(range(0, count, max(1, count // 8)))
         for i in check:
             s = systems[i]
@@ -213,15 +225,22 @@ def bench_mix(args):
             ref = np.concatenate([-qi / (2 * math.pi), -qr / (2 * math.pi)])
             got = np.concatenate([v[0][sub], v[1][sub]])
             errs.append(float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+            h = s["sigma"] * overlap
+            one = Regrid(s["x"] + dt * v[0], s["y"] + dt * v[1], s["w"], s["sigma"], overlap,
+                         [-1.5 * h, 0], [-1.5 * h, 0])
+            remesh_exact &= all(np.array_equal(p, q) for p, q in zip(b.get_remeshed(i), one))
     batched = total / (wall / args.steps)
-    # baseline 1: one-shot float64 fmm2d per system (host buffers, per-call copies)
+    # baseline 1: one-shot float64 step per system (host buffers, per-call copies)
     t = time.perf_counter()
     for s in systems:
         xopt, cutoff = W.oseen_params(s["sigma"])
-        gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"], s["y"], 35, xopt, cutoff, 1e-6,
-                        prec=gauss.PREC_F64, order=16, leaf=64)
+        qr, qi = gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"], s["y"], 35, xopt, cutoff, 1e-6,
+                                 prec=gauss.PREC_F64, order=16, leaf=64)
+        h = s["sigma"] * overlap
+        Regrid(s["x"] + dt * (-qi / (2 * math.pi)), s["y"] + dt * (-qr / (2 * math.pi)), s["w"],
+               s["sigma"], overlap, [-1.5 * h, 0], [-1.5 * h, 0])
     t_oneshot = time.perf_counter() - t
-    # baseline 2: resident float64 systems evaluated one after another
+    # baseline 2: resident float64 velocity, systems one after another
     resident = [BlobSystem(s["x"], s["y"], s["w"]) for s in systems]
     ps = [vel_params(s["sigma"], prec=gauss.PREC_F64, order=16, leaf=64) for s in systems]
     for r, p in zip(resident, ps):
@@ -233,8 +252,8 @@ def bench_mix(args):
     t_res = time.perf_counter() - t
     for r in resident:
         r.close()
-    return {"metric": "aggregate blob vel. evals/s, 256 uneven systems, batched F32C vs "
-                      "one-at-a-time F64",
+    return {"metric": "aggregate blob step evals/s (velocity + M4' remesh), 256 uneven systems, "
+                      "batched F32C graph vs one-at-a-time F64",
             "value": batched, "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
@@ -242,12 +261,16 @@ def bench_mix(args):
             "data": "synthetic",
             "config": {"workload": "%d systems, N log-uniform [1024, 262144] seed 0, total %d blobs"
                                    % (count, total),
-                       "graph_device_ms": dev_ms,
-                       "f64_one_shot_evals_per_s": total / t_oneshot,
-                       "f64_resident_sequential_evals_per_s": total / t_res,
+                       "graph_device_ms": dev_ms, "remeshed_blobs": remeshed,
+                       "spilled_systems": spilled,
+                       "velocity_only_evals_per_s": total / (wall_v / args.steps),
+                       "velocity_only_graph_device_ms": vinfo["ms"],
+                       "f64_one_shot_step_evals_per_s": total / t_oneshot,
+                       "f64_resident_sequential_velocity_evals_per_s": total / t_res,
                        "speedup_vs_f64_one_shot": batched / (total / t_oneshot),
-                       "speedup_vs_f64_resident_sequential": batched / (total / t_res),
-                       "rel_l2_vs_fp64_max": max(errs), "checked_systems": len(check)}}
+                       "speedup_vs_f64_resident_sequential_velocity": batched / (total / t_res),
+                       "rel_l2_vs_fp64_max": max(errs), "checked_systems": len(check),
+                       "remesh_bitexact_vs_single_system": bool(remesh_exact)}}
 
 
 def bench_serve(args):

@@ -316,7 +316,27 @@ typedef struct {
     int32_t flags;        /* nonzero: a near set overflowed (results invalid) */
     int64_t total_points;
     int32_t levels_max, pad;
+    int64_t remeshed;     /* phf_batch_step: blobs after the remesh, all systems */
+    int32_t spilled;      /* phf_batch_step: systems remeshed by the single-system
+                             path because their lattice range exceeded the
+                             reserved cells */
+    int32_t pad2;
 } phf_batch_info;
+
+/* Convection + remesh of every system after the batched velocity
+ * (phf_batch_step): x <- x + dt (vx + vinf_x) (_evolution.py:540-541), then
+ * Regrid (_regridding.py:118-218) onto the lattice of spacing
+ * h_s = sigma_s * overlap with origin (x0h h_s, y0h h_s) (Blobs.redistribute:
+ * x0h = y0h = -1.5, __blobsClass.py:891-897) and M4' parameter c. */
+typedef struct {
+    double dt;            /* 0: remesh the current positions */
+    double vinf_x, vinf_y;
+    double overlap;
+    double x0h, y0h;
+    double c;
+    double cell_factor;   /* lattice cells reserved per system:
+                             cell_factor n_s + 16 sqrt(n_s) + 256 (<= 0: 2) */
+} phf_batch_remesh_params;
 
 int phf_batch_create(const int64_t *sizes, const double *sigma, int32_t count,
                      const phf_batch_params *bp, phf_batch **out);
@@ -327,6 +347,18 @@ int phf_batch_set_system(phf_batch *b, int32_t s, const double *x, const double 
  * first call and replays it afterwards.  info may be NULL. */
 int phf_batch_velocity(phf_batch *b, int32_t use_graph, phf_batch_info *info);
 int phf_batch_get_velocity(phf_batch *b, int32_t s, double *vx, double *vy);
+/* Enable phf_batch_step with these parameters (reserves the lattice
+ * workspace; drops a previously captured step graph). */
+int phf_batch_set_remesh(phf_batch *b, const phf_batch_remesh_params *rp);
+/* Velocity of every system, then convection + remesh, as one launch
+ * sequence (use_graph = 1: captured on first use, replayed after).  The
+ * systems' own blobs are not changed; the remeshed blobs are read with
+ * phf_batch_get_remeshed. */
+int phf_batch_step(phf_batch *b, int32_t use_graph, phf_batch_info *info);
+/* Remeshed blobs of system s from the last phf_batch_step, row-major lattice
+ * order; *m receives the count; x, y, w (capacity *m) may be NULL to query
+ * the count only. */
+int phf_batch_get_remeshed(phf_batch *b, int32_t s, int64_t *m, double *x, double *y, double *w);
 
 /* ---- serving --------------------------------------------------------------
  * A request loop over resident simulations (no counterpart in the

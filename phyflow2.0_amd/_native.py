@@ -72,7 +72,13 @@ class BatchParams(ctypes.Structure):
 
 class BatchInfo(ctypes.Structure):
     _fields_ = [("ms", _dbl), ("graph", _i32), ("flags", _i32), ("total_points", _i64),
-                ("levels_max", _i32), ("pad", _i32)]
+                ("levels_max", _i32), ("pad", _i32), ("remeshed", _i64), ("spilled", _i32),
+                ("pad2", _i32)]
+
+
+class BatchRemeshParams(ctypes.Structure):
+    _fields_ = [("dt", _dbl), ("vinf_x", _dbl), ("vinf_y", _dbl), ("overlap", _dbl),
+                ("x0h", _dbl), ("y0h", _dbl), ("c", _dbl), ("cell_factor", _dbl)]
 
 
 class RequestResult(ctypes.Structure):
@@ -149,6 +155,10 @@ _SIGS = {
     "phf_batch_set_system": ([_ptr, _i32, _ptr, _ptr, _ptr], ctypes.c_int),
     "phf_batch_velocity": ([_ptr, _i32, ctypes.POINTER(BatchInfo)], ctypes.c_int),
     "phf_batch_get_velocity": ([_ptr, _i32, _ptr, _ptr], ctypes.c_int),
+    "phf_batch_set_remesh": ([_ptr, ctypes.POINTER(BatchRemeshParams)], ctypes.c_int),
+    "phf_batch_step": ([_ptr, _i32, ctypes.POINTER(BatchInfo)], ctypes.c_int),
+    "phf_batch_get_remeshed": ([_ptr, _i32, ctypes.POINTER(_i64), _ptr, _ptr, _ptr],
+                               ctypes.c_int),
     "phf_server_create": ([_i32, _i64, ctypes.POINTER(_ptr)], ctypes.c_int),
     "phf_server_destroy": ([_ptr], ctypes.c_int),
     "phf_server_add_system": ([_ptr, _ptr, ctypes.POINTER(_i64)], ctypes.c_int),

@@ -24,12 +24,15 @@
 // {tau : near_{l-1}(floor((tau + parity)/2)) && !near_l(tau)}.
 #include "phf_batch.hpp"
 #include "phf_fmm_dev.hpp"
+#include "phf_remesh.hpp"
 
+#include <map>
 #include <mutex>
 
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstring>
 #include <vector>
 
 namespace phf {
@@ -851,7 +854,24 @@ struct phf_batch_s {
     double *x = nullptr, *y = nullptr, *w = nullptr, *vx = nullptr, *vy = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<double> sigma;
+    // convection + remesh (phf_batch_step)
+    bool have_remesh = false;
+    BatchRemeshPlan rplan;
+    BatchRemeshBuffers rbuf;
+    hipGraph_t graph_s = nullptr;
+    hipGraphExec_t exec_s = nullptr;
+    std::vector<int64_t> mo;  // (offset, count) per system after the last step
+    std::map<int, std::vector<double>> spill;  // x|y|w of systems remeshed apart
 };
+
+static void drop_step_graph(phf_batch *b)
+{
+    if (b->exec_s) (void)hipGraphExecDestroy(b->exec_s);
+    if (b->graph_s) (void)hipGraphDestroy(b->graph_s);
+    b->exec_s = nullptr;
+    b->graph_s = nullptr;
+}
 
 extern "C" {
 
@@ -864,6 +884,8 @@ int phf_batch_destroy(phf_batch *b)
         if (b->lane.stream) (void)hipStreamSynchronize(b->lane.stream);
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
         if (b->graph) (void)hipGraphDestroy(b->graph);
+        drop_step_graph(b);
+        b->rbuf.arena.release();
         b->lane.arena.release();
         b->lane.pin.release();
         for (auto &ev : b->lane.ev)
@@ -886,6 +908,7 @@ int phf_batch_create(const int64_t *sizes, const double *sigma, int32_t count,
     PHF_TRY(Context::get().init());
     PHF_HIP(hipSetDevice(Context::get().device()));
     phf_batch *b = new phf_batch();
+    b->sigma.assign(sigma, sigma + std::max(count, 0));
     int rc = [&]() -> int {
         PHF_TRY(batch_plan(sizes, sigma, count, bp->order > 0 ? bp->order : 16,
                            bp->leaf > 0 ? bp->leaf : 64, bp->ws > 0 ? bp->ws : 1, bp->prec, bp->k,
@@ -990,6 +1013,162 @@ int phf_batch_get_velocity(phf_batch *b, int32_t s, double *vx, double *vy)
         PHF_HIP(hipMemcpyAsync(vx, b->vx + o, sizeof(double) * n, hipMemcpyDeviceToHost, b->lane.stream));
         PHF_HIP(hipMemcpyAsync(vy, b->vy + o, sizeof(double) * n, hipMemcpyDeviceToHost, b->lane.stream));
     }
+    PHF_HIP(hipStreamSynchronize(b->lane.stream));
+    return PHF_OK;
+}
+
+int phf_batch_set_remesh(phf_batch *b, const phf_batch_remesh_params *rp)
+{
+    clear_error();
+    PHF_REQUIRE(b && rp, "NULL argument");
+    PHF_REQUIRE(std::isfinite(rp->dt) && std::isfinite(rp->vinf_x) && std::isfinite(rp->vinf_y),
+                "dt and vinf must be finite");
+    PHF_REQUIRE(std::isfinite(rp->overlap) && rp->overlap > 0, "overlap must be > 0");
+    PHF_REQUIRE(std::isfinite(rp->x0h) && std::isfinite(rp->y0h) && std::isfinite(rp->c),
+                "lattice origin and c must be finite");
+    PHF_REQUIRE(std::isfinite(rp->cell_factor) && rp->cell_factor <= 64.0,
+                "cell_factor must be <= 64");
+    std::lock_guard<std::mutex> lk(b->mu);
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    PHF_HIP(hipStreamSynchronize(b->lane.stream));
+    BatchRemeshPlan R;
+    PHF_TRY(batch_remesh_plan(b->plan, b->sigma.data(), *rp, &R));
+    drop_step_graph(b);
+    b->have_remesh = false;
+    b->rbuf.arena.release();
+    b->rbuf = BatchRemeshBuffers();
+    PHF_TRY(batch_remesh_alloc(b->plan, R, &b->rbuf));
+    b->rplan = std::move(R);
+    b->mo.assign((size_t)2 * b->plan.S, 0);
+    b->spill.clear();
+    b->have_remesh = true;
+    return PHF_OK;
+}
+
+int phf_batch_step(phf_batch *b, int32_t use_graph, phf_batch_info *info)
+{
+    clear_error();
+    PHF_REQUIRE(b != nullptr, "batch is NULL");
+    std::lock_guard<std::mutex> lk(b->mu);
+    PHF_REQUIRE(b->have_remesh, "phf_batch_set_remesh has not been called");
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    Lane *L = &b->lane;
+    hipEvent_t e0 = L->ev[0], e1 = L->ev[1];
+    auto enqueue = [&]() -> int {
+        PHF_TRY(batch_enqueue(L, b->plan, b->buf, b->x, b->y, b->w, b->vx, b->vy));
+        return batch_remesh_enqueue(L, b->plan, b->buf, b->rplan, b->rbuf, b->x, b->y, b->w,
+                                    b->vx, b->vy);
+    };
+    if (use_graph && !b->exec_s) {
+        PHF_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(L->stream, &g);
+        if (rc != PHF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        PHF_HIP(e);
+        b->graph_s = g;
+        PHF_HIP(hipGraphInstantiate(&b->exec_s, g, nullptr, nullptr, 0));
+    }
+    PHF_HIP(hipEventRecord(e0, L->stream));
+    if (use_graph)
+        PHF_HIP(hipGraphLaunch(b->exec_s, L->stream));
+    else
+        PHF_TRY(enqueue());
+    PHF_HIP(hipEventRecord(e1, L->stream));
+    const int S = b->plan.S;
+    PHF_TRY(L->pin.ensure(sizeof(int64_t) * (2 * (size_t)S + 1)));
+    int64_t *hp = reinterpret_cast<int64_t *>(L->pin.ptr);
+    PHF_HIP(hipMemcpyAsync(hp, b->rbuf.mo, sizeof(int64_t) * 2 * S, hipMemcpyDeviceToHost,
+                           L->stream));
+    PHF_HIP(hipMemcpyAsync(hp + 2 * S, b->buf.flag, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const int32_t flag = *reinterpret_cast<int32_t *>(hp + 2 * S);
+    b->mo.assign(hp, hp + 2 * S);
+    if (flag) {
+        set_error("a system's smoothing cutoff spans more than %d leaf boxes; use a larger leaf "
+                  "size", 6);
+        return PHF_ERR_UNSUPPORTED;
+    }
+    // systems whose lattice range did not fit: single-system remesh
+    b->spill.clear();
+    int32_t spilled = 0;
+    int64_t total = 0;
+    for (int s = 0; s < S; ++s) {
+        if (b->mo[2 * s + 1] >= 0) {
+            total += b->mo[2 * s + 1];
+            continue;
+        }
+        ++spilled;
+        LaneGuard g;
+        PHF_TRY(g.rc);
+        Lane *T = g.lane;
+        PHF_TRY(T->arena.reset());
+        const BatchRm &r = b->rplan.rm[s];
+        const int64_t o = b->plan.sys[s].off, n = b->plan.sys[s].n;
+        const double *xs = b->rplan.p.dt != 0.0 ? b->rbuf.xn : b->x;
+        const double *ys = b->rplan.p.dt != 0.0 ? b->rbuf.yn : b->y;
+        RemeshOut ro;
+        PHF_TRY(remesh_device(T, xs + o, ys + o, b->w + o, n, r.h, 1.0, r.x0, r.y0,
+                              b->rplan.p.c, &ro));
+        std::vector<double> &v = b->spill[s];
+        v.resize((size_t)3 * ro.m);
+        if (ro.m > 0) {
+            PHF_HIP(hipMemcpyAsync(v.data(), ro.x, sizeof(double) * ro.m, hipMemcpyDeviceToHost,
+                                   T->stream));
+            PHF_HIP(hipMemcpyAsync(v.data() + ro.m, ro.y, sizeof(double) * ro.m,
+                                   hipMemcpyDeviceToHost, T->stream));
+            PHF_HIP(hipMemcpyAsync(v.data() + 2 * ro.m, ro.w, sizeof(double) * ro.m,
+                                   hipMemcpyDeviceToHost, T->stream));
+        }
+        PHF_HIP(hipStreamSynchronize(T->stream));
+        b->mo[2 * s + 1] = ro.m;
+        total += ro.m;
+    }
+    if (info) {
+        info->ms = ms;
+        info->graph = use_graph ? 1 : 0;
+        info->flags = flag;
+        info->total_points = b->plan.N;
+        info->levels_max = b->plan.Lmax;
+        info->remeshed = total;
+        info->spilled = spilled;
+    }
+    return PHF_OK;
+}
+
+int phf_batch_get_remeshed(phf_batch *b, int32_t s, int64_t *m, double *x, double *y, double *w)
+{
+    clear_error();
+    PHF_REQUIRE(b && m, "NULL argument");
+    PHF_REQUIRE(s >= 0 && s < b->plan.S, "system index %d out of range", s);
+    std::lock_guard<std::mutex> lk(b->mu);
+    PHF_REQUIRE(b->have_remesh && !b->mo.empty(), "no remeshed state (call phf_batch_step)");
+    const int64_t cnt = b->mo[2 * s + 1];
+    *m = cnt;
+    if (!x && !y && !w) return PHF_OK;
+    PHF_REQUIRE(cnt == 0 || (x && y && w), "output arrays are NULL");
+    if (cnt == 0) return PHF_OK;
+    auto it = b->spill.find(s);
+    if (it != b->spill.end()) {
+        std::memcpy(x, it->second.data(), sizeof(double) * cnt);
+        std::memcpy(y, it->second.data() + cnt, sizeof(double) * cnt);
+        std::memcpy(w, it->second.data() + 2 * cnt, sizeof(double) * cnt);
+        return PHF_OK;
+    }
+    const int64_t o = b->mo[2 * s];
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    PHF_HIP(hipMemcpyAsync(x, b->rbuf.xo + o, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                           b->lane.stream));
+    PHF_HIP(hipMemcpyAsync(y, b->rbuf.yo + o, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                           b->lane.stream));
+    PHF_HIP(hipMemcpyAsync(w, b->rbuf.wo + o, sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                           b->lane.stream));
     PHF_HIP(hipStreamSynchronize(b->lane.stream));
     return PHF_OK;
 }

@@ -43,6 +43,47 @@ struct BatchBuffers {
     int32_t *flag = nullptr;
 };
 
+// ---- batched convection + M4' remesh (phf_batch_remesh.hip) ----
+struct BatchRm {
+    double h, x0, y0, x0h, y0h;  // lattice spacing, origin, origin / h
+    int64_t cbase, ccap;         // cell-key range [cbase, cbase + ccap)
+};
+
+struct BatchRmDyn {  // written on the device every step
+    long long imin, jmin, bi, bj;  // base-cell range
+    int32_t ok, pad;               // 0: touched range exceeds ccap
+};
+
+struct BatchRemeshPlan {
+    phf_batch_remesh_params p{};
+    int64_t C = 0;  // total cell capacity
+    int key_bits = 1;
+    std::vector<BatchRm> rm;
+    std::vector<int64_t> cb;  // cbase per system
+};
+
+struct BatchRemeshBuffers {
+    Arena arena;
+    BatchRm *rm = nullptr;
+    BatchRmDyn *dyn = nullptr;
+    int64_t *cb = nullptr, *mo = nullptr;  // mo: (offset, count) per system
+    double *xn = nullptr, *yn = nullptr;
+    double3 *sb = nullptr;
+    int32_t *cstart = nullptr;
+    uint32_t *flag = nullptr, *slot = nullptr, *sums = nullptr;
+    double *xo = nullptr, *yo = nullptr, *wo = nullptr;
+};
+
+int64_t batch_cell_capacity(int64_t n, double cell_factor);
+int batch_remesh_plan(const BatchPlan &P, const double *sigma, const phf_batch_remesh_params &rp,
+                      BatchRemeshPlan *R);
+int batch_remesh_alloc(const BatchPlan &P, const BatchRemeshPlan &R, BatchRemeshBuffers *Bf);
+// Enqueue convection + remesh after batch_enqueue (reuses its sort buffers;
+// capturable).
+int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const BatchRemeshPlan &R,
+                         BatchRemeshBuffers &Bf, const double *x, const double *y,
+                         const double *w, const double *vx, const double *vy);
+
 int batch_plan(const int64_t *sizes, const double *sigma, int S, int p, int leaf, int ws, int prec,
                double k, BatchPlan *P);
 int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf);

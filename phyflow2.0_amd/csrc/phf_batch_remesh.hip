@@ -1,0 +1,321 @@
+// phf_batch_remesh.hip -- forward-Euler convection and M4' remesh of every
+// system of a batch in one launch sequence (capturable: no allocation, no
+// host synchronisation), after the batched velocity evaluation.
+//
+// Per system s the semantics are those of the single-system path:
+//   x <- x + dt (vx + vInf_x)                    (_evolution.py:540-541)
+//   Regrid onto the lattice h_s = sigma_s * overlap, origin (x0h, y0h) h_s
+//   (_regridding.py:118-218; Blobs.redistribute uses -1.5 h,
+//   __blobsClass.py:891-897), duplicates summed in the order of a stable
+//   sort of the reference's COO entries, zero-weight touched cells kept.
+//
+// Segment table for the lattice: system s owns the cell keys
+// [cbase_s, cbase_s + ccap_s).  Its touched-cell range (bi+3) x (bj+3),
+// computed on the device from its base-cell bounds, must fit ccap_s; a
+// system that does not fit is flagged (m_s = -1) and the host remeshes it
+// with the single-system path after the graph has run.  One stable radix
+// sort of all blobs by global base-cell key keeps systems contiguous and,
+// within a cell, blobs in ascending index; one scan of the touched flags
+// of all systems gives every output cell its packed slot, so system s's
+// output is the packed range [off_s, off_s + m_s).
+#include "phf_batch.hpp"
+#include "phf_m4_dev.hpp"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+
+namespace phf {
+
+#pragma clang fp contract(off)
+
+namespace {
+
+__device__ __forceinline__ int find_seg64(const int64_t *__restrict__ arr, int S, int64_t g)
+{
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (arr[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void brm_convect(const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ vx, const double *__restrict__ vy,
+                            int64_t N, double dt, double ux, double uy, double *__restrict__ xn,
+                            double *__restrict__ yn)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    xn[i] = x[i] + dt * (vx[i] + ux);
+    yn[i] = y[i] + dt * (vy[i] + uy);
+}
+
+// one block per system: base-cell bounds -> touched range, capacity check
+__global__ __launch_bounds__(256) void brm_bounds(const BatchSys *__restrict__ sys,
+                                                  const BatchRm *__restrict__ rm,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ y,
+                                                  BatchRmDyn *__restrict__ dyn)
+{
+    __shared__ longlong4 sh[256];
+    const int s = blockIdx.x;
+    const int64_t o = sys[s].off, n = sys[s].n;
+    const BatchRm r = rm[s];
+    long long imin = LLONG_MAX, imax = LLONG_MIN, jmin = LLONG_MAX, jmax = LLONG_MIN;
+    for (int64_t b = threadIdx.x; b < n; b += 256) {
+        const long long i = base_index(x[o + b], r.x0, r.h), j = base_index(y[o + b], r.y0, r.h);
+        imin = i < imin ? i : imin;
+        imax = i > imax ? i : imax;
+        jmin = j < jmin ? j : jmin;
+        jmax = j > jmax ? j : jmax;
+    }
+    sh[threadIdx.x] = make_longlong4(imin, imax, jmin, jmax);
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            const longlong4 a = sh[threadIdx.x], c = sh[threadIdx.x + k];
+            sh[threadIdx.x] = make_longlong4(a.x < c.x ? a.x : c.x, a.y > c.y ? a.y : c.y,
+                                             a.z < c.z ? a.z : c.z, a.w > c.w ? a.w : c.w);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        BatchRmDyn d{};
+        if (n == 0) {
+            d.ok = 1;  // nothing to remesh
+        } else {
+            const longlong4 b = sh[0];
+            d.imin = b.x;
+            d.jmin = b.z;
+            d.bi = b.y - b.x + 1;
+            d.bj = b.w - b.z + 1;
+            // (bi + 3)(bj + 3) <= ccap, checked without overflow
+            const double cells = (double)(d.bi + 3) * (double)(d.bj + 3);
+            d.ok = (d.bi > 0 && d.bj > 0 && cells <= (double)r.ccap) ? 1 : 0;
+        }
+        dyn[s] = d;
+    }
+}
+
+__global__ void brm_keys(const int64_t *__restrict__ poff, int S, const BatchRm *__restrict__ rm,
+                         const BatchRmDyn *__restrict__ dyn, const double *__restrict__ x,
+                         const double *__restrict__ y, int64_t N, uint32_t *__restrict__ keys,
+                         int32_t *__restrict__ vals)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    const int s = find_seg64(poff, S, b);
+    const BatchRm r = rm[s];
+    const BatchRmDyn d = dyn[s];
+    uint32_t key = (uint32_t)r.cbase;  // a flagged system's blobs share one key
+    if (d.ok) {
+        const long long i = base_index(x[b], r.x0, r.h) - d.imin;
+        const long long j = base_index(y[b], r.y0, r.h) - d.jmin;
+        key = (uint32_t)(r.cbase + i * d.bj + j);
+    }
+    keys[b] = key;
+    vals[b] = (int32_t)b;
+}
+
+// sorted blob data: (x/h, y/h, w)
+__global__ void brm_gather_scaled(const int64_t *__restrict__ poff, int S,
+                                  const BatchRm *__restrict__ rm, const double *__restrict__ x,
+                                  const double *__restrict__ y, const double *__restrict__ w,
+                                  const int32_t *__restrict__ perm, int64_t N,
+                                  double3 *__restrict__ out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= N) return;
+    const int32_t b = perm[k];
+    const double h = rm[find_seg64(poff, S, b)].h;
+    out[k] = make_double3(x[b] / h, y[b] / h, w[b]);
+}
+
+// touched = one of the 16 base cells (i-a, j-b), a, b in 0..3, holds a blob
+__global__ void brm_touched(const int64_t *__restrict__ cb, int S, const BatchRm *__restrict__ rm,
+                            const BatchRmDyn *__restrict__ dyn, const int32_t *__restrict__ cstart,
+                            int64_t C, uint32_t *__restrict__ flag)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const int s = find_seg64(cb, S, c);
+    const BatchRmDyn d = dyn[s];
+    const int64_t base = rm[s].cbase, loc = c - base;
+    const long long nj = d.bj + 3;
+    uint32_t t = 0;
+    if (d.ok && loc < (d.bi + 3) * nj) {
+        const long long i = loc / nj, j = loc - (loc / nj) * nj;
+        for (int q = 0; q < 16 && !t; ++q) {
+            const long long ii = i - (q & 3), jj = j - (q >> 2);
+            if (ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
+            const int64_t k = base + ii * d.bj + jj;
+            t = cstart[k + 1] > cstart[k];
+        }
+    }
+    flag[c] = t;
+}
+
+__global__ void brm_gather_sum(const int64_t *__restrict__ cb, int S,
+                               const BatchRm *__restrict__ rm, const BatchRmDyn *__restrict__ dyn,
+                               const int32_t *__restrict__ cstart, const double3 *__restrict__ sb,
+                               const uint32_t *__restrict__ flag, const uint32_t *__restrict__ slot,
+                               int64_t C, double c, double c2, double *__restrict__ xo,
+                               double *__restrict__ yo, double *__restrict__ wo)
+{
+    const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cidx >= C || !flag[cidx]) return;
+    const int s = find_seg64(cb, S, cidx);
+    const BatchRm r = rm[s];
+    const BatchRmDyn d = dyn[s];
+    const int64_t loc = cidx - r.cbase;
+    const long long nj = d.bj + 3;
+    const long long i = loc / nj, j = loc - (loc / nj) * nj;
+    const long long I = i + d.imin, J = j + d.jmin;  // absolute cell indices
+    const double ui = ((double)I + 0.5) + r.x0h;
+    const double uj = ((double)J + 0.5) + r.y0h;
+    double acc = 0.0;
+    bool first = true;
+    for (int q = 0; q < 16; ++q) {
+        const long long ii = i - (q & 3), jj = j - (q >> 2);
+        if (ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
+        const int64_t k = r.cbase + ii * d.bj + jj;
+        for (int32_t p = cstart[k]; p < cstart[k + 1]; ++p) {
+            const double3 b = sb[p];
+            const double v = (m4p_dev(ui - b.x, c, c2) * m4p_dev(uj - b.y, c, c2)) * b.z;
+            acc = first ? v : acc + v;
+            first = false;
+        }
+    }
+    const uint32_t o = slot[cidx];
+    const double hh = 0.5 * r.h;
+    wo[o] = acc;
+    xo[o] = ((double)I * r.h + hh) + r.x0;
+    yo[o] = ((double)J * r.h + hh) + r.y0;
+}
+
+// per system: packed output offset and count (-1: did not fit)
+__global__ void brm_counts(const BatchRm *__restrict__ rm, const BatchRmDyn *__restrict__ dyn,
+                           int S, const uint32_t *__restrict__ flag,
+                           const uint32_t *__restrict__ slot, int64_t C,
+                           int64_t *__restrict__ mo)
+{
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s >= S) return;
+    const int64_t total = C > 0 ? (int64_t)slot[C - 1] + flag[C - 1] : 0;
+    const int64_t b0 = rm[s].cbase, b1 = b0 + rm[s].ccap;
+    const int64_t o0 = b0 < C ? (int64_t)slot[b0] : total;
+    const int64_t o1 = b1 < C ? (int64_t)slot[b1] : total;
+    mo[2 * s] = o0;
+    mo[2 * s + 1] = dyn[s].ok ? o1 - o0 : -1;
+}
+
+}  // namespace
+
+int64_t batch_cell_capacity(int64_t n, double cell_factor)
+{
+    if (n == 0) return 0;
+    const double f = cell_factor > 0 ? cell_factor : 2.0;
+    return (int64_t)std::ceil(f * (double)n + 16.0 * std::sqrt((double)n)) + 256;
+}
+
+int batch_remesh_plan(const BatchPlan &P, const double *sigma, const phf_batch_remesh_params &rp,
+                      BatchRemeshPlan *R)
+{
+    R->p = rp;
+    R->rm.assign(P.S, BatchRm{});
+    int64_t base = 0;
+    for (int s = 0; s < P.S; ++s) {
+        BatchRm &r = R->rm[s];
+        r.h = sigma[s] * rp.overlap;
+        PHF_REQUIRE(std::isfinite(r.h) && r.h > 0, "lattice spacing of system %d is not > 0", s);
+        r.x0 = rp.x0h * r.h;
+        r.y0 = rp.y0h * r.h;
+        r.x0h = r.x0 / r.h;
+        r.y0h = r.y0 / r.h;
+        r.cbase = base;
+        r.ccap = batch_cell_capacity(P.sys[s].n, rp.cell_factor);
+        base += r.ccap;
+    }
+    R->C = base;
+    PHF_REQUIRE(R->C < (int64_t(1) << 31), "remesh cell capacity %lld too large (lower "
+                "cell_factor or split the batch)", (long long)R->C);
+    R->key_bits = 1;
+    while (R->key_bits < 32 && (int64_t(1) << R->key_bits) < R->C + 1) ++R->key_bits;
+    R->cb.resize(P.S);
+    for (int s = 0; s < P.S; ++s) R->cb[s] = R->rm[s].cbase;
+    return PHF_OK;
+}
+
+int batch_remesh_alloc(const BatchPlan &P, const BatchRemeshPlan &R, BatchRemeshBuffers *Bf)
+{
+    Arena &A = Bf->arena;
+    const size_t N = (size_t)std::max<int64_t>(P.N, 1), C = (size_t)std::max<int64_t>(R.C, 1);
+    PHF_ALLOC(Bf->rm, A, BatchRm, (size_t)P.S);
+    PHF_ALLOC(Bf->dyn, A, BatchRmDyn, (size_t)P.S);
+    PHF_ALLOC(Bf->cb, A, int64_t, (size_t)P.S);
+    PHF_ALLOC(Bf->xn, A, double, N);
+    PHF_ALLOC(Bf->yn, A, double, N);
+    PHF_ALLOC(Bf->sb, A, double3, N);
+    PHF_ALLOC(Bf->cstart, A, int32_t, C + 1);
+    PHF_ALLOC(Bf->flag, A, uint32_t, C);
+    PHF_ALLOC(Bf->slot, A, uint32_t, C);
+    PHF_ALLOC(Bf->sums, A, uint32_t, (size_t)scan_sums_size((int64_t)C));
+    PHF_ALLOC(Bf->xo, A, double, C);
+    PHF_ALLOC(Bf->yo, A, double, C);
+    PHF_ALLOC(Bf->wo, A, double, C);
+    PHF_ALLOC(Bf->mo, A, int64_t, (size_t)2 * P.S);
+    PHF_HIP(hipMemcpy(Bf->rm, R.rm.data(), sizeof(BatchRm) * P.S, hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->cb, R.cb.data(), sizeof(int64_t) * P.S, hipMemcpyHostToDevice));
+    return PHF_OK;
+}
+
+int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const BatchRemeshPlan &R,
+                         BatchRemeshBuffers &Bf, const double *x, const double *y,
+                         const double *w, const double *vx, const double *vy)
+{
+    const int S = P.S;
+    const int64_t N = P.N, C = R.C;
+    const double *xs = x, *ys = y;
+    if (R.p.dt != 0.0) {
+        if (N > 0)
+            brm_convect<<<div_up(N, 256), 256, 0, L->stream>>>(x, y, vx, vy, N, R.p.dt, R.p.vinf_x,
+                                                               R.p.vinf_y, Bf.xn, Bf.yn);
+        xs = Bf.xn;
+        ys = Bf.yn;
+    }
+    brm_bounds<<<S, 256, 0, L->stream>>>(Bv.sys, Bf.rm, xs, ys, Bf.dyn);
+    PHF_HIP(hipGetLastError());
+    uint32_t *ks = Bv.k0;
+    int32_t *vs = Bv.v0;
+    if (N > 0) {
+        brm_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, Bf.dyn, xs, ys, N,
+                                                        Bv.k0, Bv.v0);
+        PHF_HIP(hipGetLastError());
+        // the velocity pass is done with the sort buffers and workspaces
+        PHF_TRY(device_radix_sort_ws(L, Bv.k0, Bv.v0, Bv.k1, Bv.v1, N, R.key_bits, Bv.hist,
+                                     Bv.offs, Bv.sums, &ks, &vs));
+    }
+    PHF_TRY(leaf_starts_device(L, ks, N, C, Bf.cstart));
+    if (N > 0)
+        brm_gather_scaled<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, xs, ys, w, vs,
+                                                                 N, Bf.sb);
+    if (C > 0) {
+        brm_touched<<<div_up(C, 256), 256, 0, L->stream>>>(Bf.cb, S, Bf.rm, Bf.dyn, Bf.cstart, C,
+                                                           Bf.flag);
+        PHF_HIP(hipGetLastError());
+        PHF_TRY(device_exclusive_scan_ws(L, Bf.flag, C, Bf.slot, Bf.sums));
+        brm_gather_sum<<<div_up(C, 256), 256, 0, L->stream>>>(
+            Bf.cb, S, Bf.rm, Bf.dyn, Bf.cstart, Bf.sb, Bf.flag, Bf.slot, C, R.p.c, R.p.c * R.p.c,
+            Bf.xo, Bf.yo, Bf.wo);
+    }
+    brm_counts<<<div_up(S, 64), 64, 0, L->stream>>>(Bf.rm, Bf.dyn, S, Bf.flag, Bf.slot, C, Bf.mo);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+}  // namespace phf

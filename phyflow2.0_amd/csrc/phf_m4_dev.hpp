@@ -1,0 +1,53 @@
+// phf_m4_dev.hpp -- M4' weight and base-cell index shared by the single-system
+// and the batched remesh (same float64 operation order in both, contraction
+// off, so both produce bit-identical cells and weights).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+namespace phf {
+
+__device__ __forceinline__ double cube_rn(double a)
+{
+#pragma clang fp contract(off)
+    // a^3 rounded once: a*a as (p + e) exactly, then (p + e) * a.
+    const double p = a * a;
+    const double e = __builtin_fma(a, a, -p);
+    const double t = p * a;
+    const double te = __builtin_fma(p, a, -t);
+    return t + (te + e * a);
+}
+
+// M4' (_interpolation_kernels_1d.cpp:76-87), same operation order.
+__host__ __device__ __forceinline__ double m4p_dev(double x, double c, double c2)
+{
+#pragma clang fp contract(off)
+    const double ax = fabs(x);
+    if ((ax > 1.0) && (ax < 2.0)) {
+        const double t = 2.0 - ax;
+        const double t2 = t * t;
+        return 0.5 * t2 * (1.0 - ax) - c2 * t2 * (1.0 - 2.0 * ax);
+    } else if ((ax >= 0.0) && (ax < 1.0)) {
+#ifdef __HIP_DEVICE_COMPILE__
+        const double a3 = cube_rn(ax);
+#else
+        const double a3 = std::pow(ax, 3.0);
+#endif
+        const double x2 = x * x;
+        return 1.0 - 2.5 * x2 + 1.5 * a3 - c2 * (2.0 - 9.0 * x2 + 6.0 * a3);
+    }
+    return 0.0;
+}
+
+// iL = int64(ceil(((x - x0) - 2h + h/2)/h)) - 1  (_regridding.py:157-158)
+__device__ __forceinline__ int64_t base_index(double x, double x0, double h)
+{
+#pragma clang fp contract(off)
+    const double h2 = 2.0 * h, hh = 0.5 * h;
+    return (int64_t)ceil((((x - x0) - h2) + hh) / h) - 1;
+}
+
+}  // namespace phf

@@ -71,3 +71,78 @@ def test_alone_equals_in_mix_and_graph_replays():
             one.velocity(use_graph=True)
             a = one.get_velocity(0)
         assert np.array_equal(a[0], eager[k][0]) and np.array_equal(a[1], eager[k][1])
+
+
+def _single_step(s, v, dt, vinf, overlap, c=0.0):
+    """Same step through the single-system entry points: Euler convection in
+    float64 (the batched kernel's operation order), then Regrid."""
+    from phyflow_amd.blobs.base.regrid import Regrid
+    xn = s["x"] + dt * (v[0] + vinf[0])
+    yn = s["y"] + dt * (v[1] + vinf[1])
+    h = s["sigma"] * overlap
+    return Regrid(xn, yn, s["w"], s["sigma"], overlap, [-1.5 * h, 0], [-1.5 * h, 0], c=c), (xn, yn)
+
+
+@pytest.mark.parametrize("cell_factor", [2.0, 0.01])
+def test_batch_step_remesh_matches_single(cell_factor, O):
+    """Velocity + Euler step + M4' remesh of every system in one sequence:
+    each system's remeshed blobs are bit-identical to the single-system
+    Regrid of the same convected positions (cell_factor 0.01 forces every
+    non-tiny system through the single-system spill path)."""
+    sizes, systems = _mix(9, seed=7)
+    rng = np.random.default_rng(4)
+    sizes = np.concatenate([sizes, [0, 1, 3]])
+    systems.append(dict(x=np.empty(0), y=np.empty(0), w=np.empty(0), sigma=0.01))
+    for n in (1, 3):
+        x, y = rng.random((2, n))
+        systems.append(dict(x=x, y=y, w=rng.standard_normal(n) * 1e-4, sigma=0.02))
+    dt, vinf, overlap = 2e-3, (0.3, -0.2), 0.8
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C) as b:
+        for i, s in enumerate(systems):
+            b.set_system(i, s["x"], s["y"], s["w"])
+        b.set_remesh(overlap, dt=dt, vInf=vinf, cell_factor=cell_factor)
+        info = b.step(use_graph=False)
+        assert info["flags"] == 0
+        # uniform systems need ~n + 8 sqrt(n) cells; the 3-blob system spread
+        # over the unit square needs more than its reserve at any factor
+        if cell_factor < 1:
+            assert info["spilled"] >= sum(1 for n in sizes if n >= 1024)
+        else:
+            assert info["spilled"] <= 1
+        total = 0
+        for i, s in enumerate(systems):
+            v = b.get_velocity(i)
+            got = b.get_remeshed(i)
+            ref, (xn, yn) = _single_step(s, v, dt, vinf, overlap)
+            total += got[0].size
+            for a, r in zip(got, ref[:3]):
+                assert np.array_equal(a, r), i
+            if 0 < s["x"].size <= 20000 and i % 3 == 0:
+                h = s["sigma"] * overlap
+                o = O.regrid_m4(xn, yn, s["w"], s["sigma"], overlap, -1.5 * h, -1.5 * h)
+                assert np.array_equal(got[0], o[0]) and np.array_equal(got[1], o[1])
+                assert np.max(np.abs(got[2] - o[2])) <= 1e-15 * np.max(np.abs(o[2]))
+        assert total == info["remeshed"]
+
+
+def test_batch_step_graph_replays():
+    sizes, systems = _mix(6, seed=11)
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C) as b:
+        for i, s in enumerate(systems):
+            b.set_system(i, s["x"], s["y"], s["w"])
+        b.set_remesh(0.8, dt=1e-3)
+        b.step(use_graph=False)
+        eager = [b.get_remeshed(i) for i in range(len(systems))]
+        for rep in range(3):
+            info = b.step(use_graph=True)
+            assert info["graph"] and info["spilled"] == 0
+            for i in range(len(systems)):
+                g = b.get_remeshed(i)
+                assert all(np.array_equal(a, e) for a, e in zip(g, eager[i])), (rep, i)
+            b.velocity(use_graph=False)  # other eager work between replays
+        # parameters change -> new graph, still agrees with eager
+        b.set_remesh(1.0, dt=0.0)
+        g1 = b.step(use_graph=True) and [b.get_remeshed(i) for i in range(len(systems))]
+        b.step(use_graph=False)
+        for i in range(len(systems)):
+            assert all(np.array_equal(a, e) for a, e in zip(g1[i], b.get_remeshed(i)))
