@@ -344,9 +344,33 @@ def bench_serve(args):
                        "hbm_used_gb": st["hbm_used"] / 1e9}}
 
 
+def bench_copy(args):
+    """Calibration for HBM byte counters: a plain device copy of a 1 GiB
+    float32 buffer (torch's copy kernel); every launch reads and writes
+    exactly 2^30 bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    return {"metric": "device copy bandwidth (byte-counter calibration)",
+            "value": 2 * 4 * n / dt / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * dt, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "copy of 2^28 float32", "bytes_read_per_launch": 4 * n,
+                       "bytes_written_per_launch": 4 * n}}
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["fmm1m", "lamb", "cylinder", "mix", "serve"],
+    ap.add_argument("--workload", choices=["fmm1m", "lamb", "cylinder", "mix", "serve", "copy"],
                     default="fmm1m")
     ap.add_argument("--sims", type=int, default=16)
     ap.add_argument("--slots", type=int, default=8)
@@ -379,6 +403,9 @@ def main():
         return 0
     if args.workload == "serve":
         print(json.dumps(bench_serve(args)), flush=True)
+        return 0
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
         return 0
 
     s = W.uniform_system(args.n, seed=0, h=1.0 / 1024 if args.n == 1048576 else None)

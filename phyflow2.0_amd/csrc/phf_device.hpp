@@ -108,6 +108,20 @@ __device__ __forceinline__ float pair_f32(float dx, float dy, float w, const Pai
     }
 }
 
+// pair_f32 for Oseen pairs known to satisfy exp(-shape r2) <= 2^-26: then
+// w (1 - e) rounds to w in float32 (|w e| is below half an ulp of w), so
+// dropping the exponential gives bit-identical P.  7 VALU + 1 transcendental.
+template <int KIND>
+__device__ __forceinline__ float pair_f32_far(float dx, float dy, float w, const PairParams &pp)
+{
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, 1e-30f));
+    if (KIND == PAIR_OSEEN) {
+        const float num = r2 < pp.c2_f ? w * pp.scale_f : w;
+        return num * rcp_f32(r2);
+    }
+    return w * rcp_f32(r2);
+}
+
 template <int KIND>
 __device__ __forceinline__ double pair_f64(double dx, double dy, double w, const PairParams &pp)
 {

@@ -41,19 +41,6 @@
 
 namespace phf {
 
-__global__ void gather_src_f32(const double *__restrict__ x, const double *__restrict__ y,
-                               const double *__restrict__ w, const uint32_t *__restrict__ skeys,
-                               const int32_t *__restrict__ perm, int64_t n, double x0, double y0,
-                               double s, float4 *__restrict__ out)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int32_t j = perm[i];
-    double cx, cy;
-    leaf_center(skeys[i], x0, y0, s, cx, cy);
-    out[i] = make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
-}
-
 __global__ void gather_src_f64(const double *__restrict__ x, const double *__restrict__ y,
                                const double *__restrict__ w, const int32_t *__restrict__ perm,
                                int64_t n, double4 *__restrict__ out)
@@ -282,81 +269,249 @@ __global__ __launch_bounds__(64) void m2l_kernel(
 }
 
 // ------------------------------------------------------ P2P + L2P (leaf) --
-template <int KIND>
-__global__ __launch_bounds__(64) void p2p_l2p_f32c(
-    const float4 *__restrict__ src, const int32_t *__restrict__ start_s,
-    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
-    const int32_t *__restrict__ perm_t, int Lvl, const int2 *__restrict__ near, int nnear,
-    PairParams pp, float s_f, const float2 *__restrict__ Bleaf, int p1,
-    double *__restrict__ qr, double *__restrict__ qi)
+// float32 sources are stored per leaf in ranges padded to a multiple of 4
+// with zero-weight entries (exact zeros in every sum they enter), so the
+// four lane groups of a wave can split a leaf's sources without a tail.
+__global__ void padded_counts(const int32_t *__restrict__ start, int64_t nleaf,
+                              uint32_t *__restrict__ cnt)
 {
-    const int lane = threadIdx.x;
-    const uint32_t b = blockIdx.x;
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nleaf) return;
+    cnt[b] = b < nleaf ? (uint32_t)((start[b + 1] - start[b] + 3) & ~3) : 0u;
+}
+
+__global__ void gather_src_f32p(const double *__restrict__ x, const double *__restrict__ y,
+                                const double *__restrict__ w, const uint32_t *__restrict__ skeys,
+                                const int32_t *__restrict__ perm, const int32_t *__restrict__ start,
+                                const int32_t *__restrict__ sp, int64_t n, double x0, double y0,
+                                double s, float4 *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t leaf = skeys[i];
+    const int32_t j = perm[i];
+    double cx, cy;
+    leaf_center(leaf, x0, y0, s, cx, cy);
+    out[sp[leaf] + (i - start[leaf])] =
+        make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
+}
+
+__global__ void zero_pads_f32(const int32_t *__restrict__ start, const int32_t *__restrict__ sp,
+                              int64_t nleaf, float4 *__restrict__ out)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nleaf) return;
+    for (int32_t k = sp[b] + (start[b + 1] - start[b]); k < sp[b + 1]; ++k)
+        out[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// One workgroup of 4 waves per leaf.  The sources of the leaf's near leaves
+// are staged in LDS (as many near leaves, or parts of one, as fit: a
+// cursor walks the near list), then wave v takes the targets
+// [64 r + 16 v, 64 r + 16 v + 16) of round r; lane = 16 g + t, group g
+// sums staged sources g, g+4, g+8, ... for target t (chunks of 16 keep
+// ~90% of lanes busy at 64 points per leaf, against ~68% for one target per
+// lane).  Per near leaf each lane forms plain float32 partials over at most
+// 16 pairs, folded into a TwoSum (hi, lo) accumulator; the four groups are
+// combined by a fixed butterfly.  Near offsets [0, ninner) may hold pairs
+// inside the smoothing core; for the rest exp(-shape r2) <= 2^-26 for every
+// pair (checked on the host from the box gap), so their pairs skip the
+// exponential (bit-identical P).  Leaves are assigned so that the
+// workgroups one XCD receives (they are dealt round-robin over the 8 XCDs)
+// cover a contiguous range of Morton leaves and share sources in its L2.
+constexpr int P2P_STAGE = 1536;  // staged sources (float4) per batch
+
+template <int KIND, bool CORE>
+__device__ __forceinline__ void p2p_seg_f32(const float4 *__restrict__ sq, int j0, int j1,
+                                            float xt, float yt, const PairParams &pp, float &ha,
+                                            float &la, float &hb, float &lb)
+{
+    int j = j0;
+    for (; j + 64 <= j1; j += 64) {
+        float aa = 0.f, ab = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float4 q = sq[j + 4 * k];
+            const float dx = xt - q.x, dy = yt - q.y;
+            const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
+            aa = __builtin_fmaf(P, dx, aa);
+            ab = __builtin_fmaf(P, dy, ab);
+        }
+        two_sum_add(ha, la, aa);
+        two_sum_add(hb, lb, ab);
+    }
+    if (j < j1) {
+        float aa = 0.f, ab = 0.f;
+        for (; j + 16 <= j1; j += 16) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float4 q = sq[j + 4 * k];
+                const float dx = xt - q.x, dy = yt - q.y;
+                const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
+                aa = __builtin_fmaf(P, dx, aa);
+                ab = __builtin_fmaf(P, dy, ab);
+            }
+        }
+        for (; j < j1; j += 4) {
+            const float4 q = sq[j];
+            const float dx = xt - q.x, dy = yt - q.y;
+            const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
+            aa = __builtin_fmaf(P, dx, aa);
+            ab = __builtin_fmaf(P, dy, ab);
+        }
+        two_sum_add(ha, la, aa);
+        two_sum_add(hb, lb, ab);
+    }
+}
+
+__device__ __forceinline__ void two_sum_merge(float &h, float &l, float h2, float l2)
+{
+    const float s = h + h2;
+    const float bp = s - h;
+    const float err = (h - (s - bp)) + (h2 - bp);
+    h = s;
+    l = (l + l2) + err;
+}
+
+struct StageSeg {
+    int lds0, len;   // staged range [lds0, lds0 + len)
+    float sx, sy;    // centre shift of the source leaf (exact)
+    int core;        // 1: pairs may lie inside the smoothing core
+};
+
+template <int KIND>
+__global__ __launch_bounds__(256) void p2p_l2p_f32s(
+    const float4 *__restrict__ src, const int32_t *__restrict__ sp,
+    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
+    const int32_t *__restrict__ perm_t, int Lvl, uint32_t nl8, const int2 *__restrict__ near,
+    int nnear, int ninner, PairParams pp, float s_f, const float2 *__restrict__ Bleaf, int p1,
+    double *__restrict__ out_a, double *__restrict__ out_b, int velocity_out)
+{
+    __shared__ float4 st[P2P_STAGE];
+    __shared__ StageSeg seg[64];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, t = lane & 15;
+    const uint32_t b = (blockIdx.x & 7u) * nl8 + (blockIdx.x >> 3);
+    const int n = 1 << Lvl;
+    if (b >= (uint32_t)n * (uint32_t)n) return;
     const int32_t t0 = start_t[b], t1 = start_t[b + 1];
     if (t1 <= t0) return;
     const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
-    const int n = 1 << Lvl;
     const float inv_s = 1.0f / s_f;
-    for (int32_t tc = t0; tc < t1; tc += 64) {
-        const int32_t i = tc + lane;
+    for (int32_t rc = t0; rc < t1; rc += 64) {
+        const int32_t i = rc + 16 * wv + t;
         const bool valid = i < t1;
+        const bool wave_on = rc + 16 * wv < t1;
         const float2 tp = valid ? tgt[i] : make_float2(0.f, 0.f);
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        for (int q = 0; q < nnear; ++q) {
-            const int2 o = near[q];
-            const int sx = bx + o.x, sy = by + o.y;
-            if (sx < 0 || sy < 0 || sx >= n || sy >= n) continue;
-            const uint32_t sb = morton2((uint32_t)sx, (uint32_t)sy);
-            const int32_t j0 = start_s[sb], j1 = start_s[sb + 1];
-            if (j1 <= j0) continue;
-            const float xt = tp.x - (float)o.x * s_f;  // exact centre shift
-            const float yt = tp.y - (float)o.y * s_f;
-            int32_t j = j0;
-            for (; j + 16 <= j1; j += 16) {
-                float aa = 0.f, ab = 0.f;
+        // cursor over (near offset q, position jq within that leaf's range);
+        // each batch: lane l of every wave looks at near offset q + l, a wave
+        // scan of the range lengths packs as many as fit into the stage
+        int q = 0, jq = 0;
+        while (q < nnear) {
+            const int ql = q + lane;
+            int j0 = 0, len = 0;
+            float ox = 0.f, oy = 0.f;
+            if (ql < nnear) {
+                const int2 o = near[ql];
+                const int sx = bx + o.x, sy = by + o.y;
+                ox = (float)o.x * s_f;
+                oy = (float)o.y * s_f;
+                if (sx >= 0 && sy >= 0 && sx < n && sy < n) {
+                    const uint32_t sb = morton2((uint32_t)sx, (uint32_t)sy);
+                    j0 = sp[sb] + (lane == 0 ? jq : 0);
+                    len = sp[sb + 1] - j0;
+                }
+            }
+            int incl = len;
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const float4 sp = src[j + k];
-                    const float dx = xt - sp.x, dy = yt - sp.y;
-                    const float P = pair_f32<KIND>(dx, dy, sp.z, pp);
-                    aa = __builtin_fmaf(P, dx, aa);
-                    ab = __builtin_fmaf(P, dy, ab);
+            for (int d = 1; d < 64; d <<= 1) {
+                const int v = __shfl_up(incl, d);
+                if (lane >= d) incl += v;
+            }
+            const int excl = incl - len;
+            const bool in = ql < nnear && excl < P2P_STAGE;
+            const int take = in ? min(len, P2P_STAGE - excl) : 0;  // multiple of 4
+            const int count = __popcll(__ballot(in));
+            const int last_take = __shfl(take, count - 1), last_len = __shfl(len, count - 1);
+            if (last_take < last_len) {
+                jq = (count == 1 ? jq : 0) + last_take;
+                q += count - 1;
+            } else {
+                jq = 0;
+                q += count;
+            }
+            if (wv == 0 && in) {
+                seg[lane].lds0 = excl;
+                seg[lane].len = take;
+                seg[lane].sx = ox;
+                seg[lane].sy = oy;
+                seg[lane].core = ql < ninner;
+            }
+            if (in) {
+                // this lane's segment, entries wv, wv + 4, ... (8 loads in flight)
+                const float4 *gs = src + j0;
+                float4 *ls = st + excl;
+                int e = wv;
+                for (; e + 28 < take; e += 32) {
+                    float4 r[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) r[u] = gs[e + 4 * u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) ls[e + 4 * u] = r[u];
                 }
-                two_sum_add(ha, la, aa);
-                two_sum_add(hb, lb, ab);
+                for (; e < take; e += 4) ls[e] = gs[e];
             }
-            if (j < j1) {
-                float aa = 0.f, ab = 0.f;
-                for (; j < j1; ++j) {
-                    const float4 sp = src[j];
-                    const float dx = xt - sp.x, dy = yt - sp.y;
-                    const float P = pair_f32<KIND>(dx, dy, sp.z, pp);
-                    aa = __builtin_fmaf(P, dx, aa);
-                    ab = __builtin_fmaf(P, dy, ab);
+            __syncthreads();
+            if (wave_on) {
+                for (int k = 0; k < count; ++k) {
+                    const StageSeg sg = seg[k];
+                    if (sg.len == 0) continue;
+                    const float xt = tp.x - sg.sx, yt = tp.y - sg.sy;
+                    const float4 *sq = st + sg.lds0 + g;
+                    if (sg.core)
+                        p2p_seg_f32<KIND, true>(sq, 0, sg.len, xt, yt, pp, ha, la, hb, lb);
+                    else
+                        p2p_seg_f32<KIND, false>(sq, 0, sg.len, xt, yt, pp, ha, la, hb, lb);
                 }
-                two_sum_add(ha, la, aa);
-                two_sum_add(hb, lb, ab);
             }
+            __syncthreads();
         }
-        double fr = 0.0, fi = 0.0;
-        if (Bleaf) {
-            const float2 *Bb = Bleaf + (int64_t)b * p1;
-            const float zr = tp.x * inv_s, zi = tp.y * inv_s;
-            float pr = 0.f, pi = 0.f;
-            for (int k = p1 - 1; k >= 0; --k) {
-                const float2 c = Bb[k];
-                const float nr = pr * zr - pi * zi + c.x;
-                const float ni = pr * zi + pi * zr + c.y;
-                pr = nr;
-                pi = ni;
+        if (!wave_on) continue;
+        // groups (0,1) and (2,3), then the pairs: every lane of a target ends
+        // with the same value
+        two_sum_merge(ha, la, __shfl_xor(ha, 16), __shfl_xor(la, 16));
+        two_sum_merge(hb, lb, __shfl_xor(hb, 16), __shfl_xor(lb, 16));
+        two_sum_merge(ha, la, __shfl_xor(ha, 32), __shfl_xor(la, 32));
+        two_sum_merge(hb, lb, __shfl_xor(hb, 32), __shfl_xor(lb, 32));
+        if (g == 0 && valid) {
+            double fr = 0.0, fi = 0.0;
+            if (Bleaf) {
+                const float2 *Bb = Bleaf + (int64_t)b * p1;
+                const float zr = tp.x * inv_s, zi = tp.y * inv_s;
+                float pr = 0.f, pi = 0.f;
+                for (int k = p1 - 1; k >= 0; --k) {
+                    const float2 c = Bb[k];
+                    const float nr = pr * zr - pi * zi + c.x;
+                    const float ni = pr * zi + pi * zr + c.y;
+                    pr = nr;
+                    pi = ni;
+                }
+                fr = pr;
+                fi = pi;
             }
-            fr = pr;
-            fi = pi;
-        }
-        if (valid) {
             const int32_t o = perm_t[i];
-            qr[o] = -(((double)ha + (double)la) + fr);
-            qi[o] = ((double)hb + (double)lb) - fi;
+            const double qr = -(((double)ha + (double)la) + fr);
+            const double qi = ((double)hb + (double)lb) - fi;
+            if (velocity_out) {
+                // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
+                const double two_pi = 2.0 * 3.141592653589793;
+                out_a[o] = -qi / two_pi;
+                out_b[o] = -qr / two_pi;
+            } else {
+                out_a[o] = qr;
+                out_b[o] = qi;
+            }
         }
     }
 }
@@ -367,7 +522,7 @@ __global__ __launch_bounds__(64) void p2p_l2p_f64(
     const double2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
     const int32_t *__restrict__ perm_t, int Lvl, const int2 *__restrict__ near, int nnear,
     PairParams pp, double x0, double y0, double s, const double2 *__restrict__ Bleaf, int p1,
-    double *__restrict__ qr, double *__restrict__ qi)
+    double *__restrict__ qr, double *__restrict__ qi, int velocity_out)
 {
     const int lane = threadIdx.x;
     const uint32_t b = blockIdx.x;
@@ -411,8 +566,16 @@ __global__ __launch_bounds__(64) void p2p_l2p_f64(
         }
         if (valid) {
             const int32_t o = perm_t[i];
-            qr[o] = -(a + fr);
-            qi[o] = bb - fi;
+            const double vr = -(a + fr), vi = bb - fi;
+            if (velocity_out) {
+                // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
+                const double two_pi = 2.0 * 3.141592653589793;
+                qr[o] = -vi / two_pi;
+                qi[o] = -vr / two_pi;
+            } else {
+                qr[o] = vr;
+                qi[o] = vi;
+            }
         }
     }
 }
@@ -668,12 +831,21 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     // sorted point data (the tree stage began in fmm_device)
     void *src, *tgt;
     const int32_t *perm_t = Tg.perm;
+    const int64_t nleaf64 = int64_t(1) << (2 * Lv);
+    const int32_t *sp = S.start;  // source ranges per leaf (padded for float32)
     if (sizeof(T) == 4) {
+        uint32_t *cnt, *spu;
+        PHF_ALLOC(cnt, L->arena, uint32_t, (size_t)nleaf64 + 1);
+        PHF_ALLOC(spu, L->arena, uint32_t, (size_t)nleaf64 + 1);
+        padded_counts<<<div_up(nleaf64 + 1, 256), 256, 0, L->stream>>>(S.start, nleaf64, cnt);
+        PHF_TRY(device_exclusive_scan(L, cnt, nleaf64 + 1, spu));
+        sp = reinterpret_cast<const int32_t *>(spu);
         float4 *s4;
-        PHF_ALLOC(s4, L->arena, float4, (size_t)std::max<int64_t>(a.nb, 1));
+        PHF_ALLOC(s4, L->arena, float4, (size_t)(a.nb + 3 * nleaf64 + 1));
         if (a.nb)
-            gather_src_f32<<<div_up(a.nb, 256), 256, 0, L->stream>>>(
-                a.xb, a.yb, a.wb, S.keys, S.perm, a.nb, dom.x0, dom.y0, dom.s, s4);
+            gather_src_f32p<<<div_up(a.nb, 256), 256, 0, L->stream>>>(
+                a.xb, a.yb, a.wb, S.keys, S.perm, S.start, sp, a.nb, dom.x0, dom.y0, dom.s, s4);
+        zero_pads_f32<<<div_up(nleaf64, 256), 256, 0, L->stream>>>(S.start, sp, nleaf64, s4);
         float2 *t2;
         PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
         gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(
@@ -695,10 +867,30 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     PHF_HIP(hipGetLastError());
     tm.end(PHF_STAGE_TREE);
 
-    // near list at the leaf level and, below, the far-field tables
+    // near list at the leaf level, offsets that can hold pairs inside the
+    // smoothing core first: for the others the box gap g satisfies
+    // shape g^2 log2(e) >= 26 with a 2% margin for float32 rounding of the
+    // point offsets, so exp(-shape r2) <= 2^-26 for every pair
+    std::vector<int2> near_ord;
+    int ninner = 0;
+    {
+        const double shape_log2 = 1.4426950408889634074 * pp.inv_ks;
+        std::vector<int2> far;
+        for (const int2 &o : ls.near_leaf) {
+            const double gx = std::max(std::abs(o.x) - 1, 0) * dom.s * 0.99;
+            const double gy = std::max(std::abs(o.y) - 1, 0) * dom.s * 0.99;
+            if (shape_log2 * (gx * gx + gy * gy) >= 26.0)
+                far.push_back(o);
+            else
+                near_ord.push_back(o);
+        }
+        ninner = (int)near_ord.size();
+        near_ord.insert(near_ord.end(), far.begin(), far.end());
+    }
+    // near list and, below, the far-field tables
     TableBlob blob;
     int2 *d_near = nullptr;
-    blob.add(ls.near_leaf, &d_near);
+    blob.add(near_ord, &d_near);
 
     const V *Bleaf = nullptr;
     if (Lv >= 2 && a.nb > 0) {
@@ -774,7 +966,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         used[PHF_STAGE_P2M] = true;
         const uint32_t nleaf = 1u << (2 * Lv);
         p2m_kernel<T><<<nleaf, 64, (size_t)64 * p1 * sizeof(V), L->stream>>>(
-            src, S.start, p1, dom.x0, dom.y0, dom.s, Lv, M + off[Lv] * p1);
+            src, sp, p1, dom.x0, dom.y0, dom.s, Lv, M + off[Lv] * p1);
         PHF_HIP(hipGetLastError());
         tm.end(PHF_STAGE_P2M);
         // M2M up to level 2
@@ -820,20 +1012,24 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     // P2P + L2P
     tm.begin(PHF_STAGE_P2P);
     const uint32_t nleaf = 1u << (2 * Lv);
+    const int vout = a.velocity_out ? 1 : 0;
     if (sizeof(T) == 4) {
+        const uint32_t nl8 = (nleaf + 7) / 8;
         if (pp.f32_nocut)
-            p2p_l2p_f32c<PAIR_OSEEN_NOCUT><<<nleaf, 64, 0, L->stream>>>(
-                (const float4 *)src, S.start, (const float2 *)tgt, Tg.start, perm_t, Lv, d_near,
-                (int)ls.near_leaf.size(), pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr, a.qi);
+            p2p_l2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, L->stream>>>(
+                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, nl8, d_near,
+                (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr,
+                a.qi, vout);
         else
-            p2p_l2p_f32c<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
-                (const float4 *)src, S.start, (const float2 *)tgt, Tg.start, perm_t, Lv, d_near,
-                (int)ls.near_leaf.size(), pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr, a.qi);
+            p2p_l2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, L->stream>>>(
+                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, nl8, d_near,
+                (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr,
+                a.qi, vout);
     } else {
         p2p_l2p_f64<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
             (const double4 *)src, S.start, (const double2 *)tgt, Tg.start, perm_t, Lv, d_near,
-            (int)ls.near_leaf.size(), pp, dom.x0, dom.y0, dom.s, (const double2 *)Bleaf, p1,
-            a.qr, a.qi);
+            (int)near_ord.size(), pp, dom.x0, dom.y0, dom.s, (const double2 *)Bleaf, p1,
+            a.qr, a.qi, vout);
     }
     PHF_HIP(hipGetLastError());
     tm.end(PHF_STAGE_P2P);

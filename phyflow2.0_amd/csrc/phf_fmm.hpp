@@ -14,6 +14,7 @@ struct FmmArgs {
     double xopt, cutoff;         // Oseen smoother (fmm2d_py.cpp:61)
     int p, ws, prec, Lhint;      // order, well-separation, precision, leaf level
     double *qr, *qi;             // device outputs in target input order
+    bool velocity_out = false;   // qr, qi receive (vx, vy) = (-qi, -qr)/(2 pi)
 };
 
 struct FmmInfo {

@@ -95,9 +95,10 @@ int velocity_device(Lane *L, const double *x, const double *y, const double *w, 
         a.ws = p.ws > 0 ? p.ws : 1;
         a.prec = p.prec;
         a.Lhint = fmm_level(n, p.leaf, p.ndirect > 0 ? p.ndirect : 35);
-        a.qr = qr;
-        a.qi = qi;
-        PHF_TRY(fmm_device(L, a, tm, used, fi));
+        a.qr = vx;
+        a.qi = vy;
+        a.velocity_out = true;
+        return fmm_device(L, a, tm, used, fi);
     }
     return q_to_velocity_device(L, qr, qi, nt, vx, vy);
 }
