@@ -88,6 +88,16 @@ int phf_vv2par_gauss(const double *xb, const double *yb, const double *wb,
                      int64_t nt, double ksigmasqr, int32_t prec, double *vx,
                      double *vy);
 
+/* Induced vorticity of Gaussian blobs.  Replaces
+ * ww2parGauss_gpu(xB,yB,wB,xT,yT,ksigmasqr,blocksize)
+ *   src/cpp/blobs/cuda/ww2parGauss_gpu/ww2parGauss_py_gpu.cu:104-200
+ * and ww2parGauss_C_cpu(xB,yB,wB,xT,yT,ksigmasqr)
+ *   src/cpp/blobs/c/ww2parGauss_C/ww2parGauss_C_cpu.c:89-153.
+ * w_t = inv_pi * sum_m w_m exp(-(dx^2+dy^2+1e-12)/ksigmasqr) / ksigmasqr. */
+int phf_ww2par_gauss(const double *xb, const double *yb, const double *wb,
+                     int64_t nb, const double *xt, const double *yt,
+                     int64_t nt, double ksigmasqr, int32_t prec, double *w);
+
 /* ---- (b) 2-D FMM with the Oseen smoother --------------------------------
  * Replaces fmm2d_gpu / fmm2d_cpu(xB,yB,wB,xT,yT,Ndirect,xopt,cutoff,tol)
  *   src/cpp/blobs/c++/fmm2d_py_cpu_gpu/fmm2d_py.cpp:59-186 (GPU),

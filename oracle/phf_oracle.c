@@ -64,6 +64,40 @@ void orc_vv2par_gauss(int64_t nb, const double *xb, const double *yb,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Induced vorticity of Gaussian blobs.
+ * Restates src/cpp/blobs/c/ww2parGauss_C/ww2parGauss_C_cpu.c:89-153
+ * (EPS=1e-12 at :99, inv_pi=1/3.141592653589793 at :101, loop :133-144):
+ *   wk += w_m exp(-(dx*dx + dy*dy + EPS)/ks);  w_k = inv_pi*wk/ks.
+ * Pinned by: the single-blob closed form, linearity, unit mass on a fine
+ * lattice (sum w h^2 == total circulation), and the curl of this oracle's
+ * own velocity field (orc_vv2par_gauss) by central differences.            */
+void orc_ww2par_gauss(int64_t nb, const double *xb, const double *yb,
+                      const double *wb, int64_t nt, const double *xt,
+                      const double *yt, double ksigmasqr, double *w,
+                      int nthreads)
+{
+    const double EPS = 1e-12;
+    const double inv_pi = 1.0 / 3.141592653589793;
+    int64_t k;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < nt; k++) {
+        double wk = 0.0;
+        for (int64_t m = 0; m < nb; m++) {
+            double dx = xt[k] - xb[m];
+            double dy = yt[k] - yb[m];
+            double r2 = dx * dx + dy * dy + EPS;
+            wk += wb[m] * exp(-r2 / ksigmasqr);
+        }
+        w[k] = inv_pi * wk / ksigmasqr;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* A.2  Direct sum with the FMM's Oseen smoother (pot=1, cont=true).
  * Parameters restate fmm.cpp:1166-1179 (INLINECOMPLEX_DIRECT branch):
  *   c2 = cutoff^2, shape = 1.2564312086261696770/xopt^2,

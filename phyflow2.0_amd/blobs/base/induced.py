@@ -16,10 +16,20 @@ Differences from the reference, all deliberate:
   * every computation runs on the GPU (``hardware`` is accepted and selects
     nothing; the reference's CPU and GPU variants compute the same sums);
   * ``xEval is None`` is tested with ``is`` (the reference's ``== None`` is
-    ambiguous on modern NumPy) and no padding is needed;
+    an elementwise comparison on modern NumPy and raises in ``if``) and no
+    padding is needed;
   * the algebraic cutoff kernel (kernel 0) is not provided.
 Keyword-only extensions: ``prec`` (PREC_F64 default, or PREC_F32C) and the
 FMM ``order``/``leaf``/``ws``.
+
+    vorticity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None,
+              yEval=None, hardware=0, blocksize=128, method=1) -> w
+
+restates _induced.py:702-866: direct Gaussian vorticity through
+ww2parGauss with k*sigma^2 (:862, :963); the FMM method raises as in the
+reference (:798-799).  The reference's GPU branch fails when xEval is None
+(xEvalNew and nEval are never set, :945-966); here the blobs are the
+targets in that case.
 """
 import math
 
@@ -36,7 +46,7 @@ FMM_METHOD = 0
 DIRECT_METHOD = 1
 ALPHA = 1.2564312086261696770  # _induced.py:236
 
-__all__ = ["velocity", "PREC_F64", "PREC_F32C"]
+__all__ = ["velocity", "vorticity", "PREC_F64", "PREC_F32C"]
 
 
 def velocity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None, hardware=0,
@@ -68,3 +78,23 @@ def velocity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None, 
                                          prec=prec, order=order, leaf=leaf, ws=ws)
         return -vx_fmm / (2.0 * np.pi), -vy_fmm / (2.0 * np.pi)
     raise ValueError("unknown method %r" % (method,))
+
+
+def vorticity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None, hardware=0,
+              blocksize=128, method=1, *, prec=PREC_F64):
+    if hardware not in (CPU_HARDWARE, GPU_HARDWARE):
+        raise ValueError("hardware must be 0 (cpu) or 1 (gpu)")
+    if method == FMM_METHOD:
+        raise Exception("FMM method not implemented yet")
+    if method != DIRECT_METHOD:
+        raise ValueError("unknown method %r" % (method,))
+    if kernel == CUTOFF_KERNEL:
+        raise NotImplementedError("the algebraic cutoff kernel is not provided by this build")
+    if kernel != GAUSS_KERNEL:
+        raise ValueError("unknown kernel %r" % (kernel,))
+    if (xEval is None) != (yEval is None):
+        raise ValueError("xEval and yEval must both be given or both be None")
+    xe = xBlob if xEval is None else xEval
+    ye = yBlob if yEval is None else yEval
+    return gauss.ww2parGauss_gpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma, blocksize,
+                                 prec=prec)

@@ -24,6 +24,7 @@
 // {tau : near_{l-1}(floor((tau + parity)/2)) && !near_l(tau)}.
 #include "phf_batch.hpp"
 #include "phf_fmm_dev.hpp"
+#include "phf_p2p_dev.hpp"
 #include "phf_remesh.hpp"
 
 #include <map>
@@ -141,12 +142,15 @@ __global__ void batch_keys(const BatchSys *__restrict__ sys, const int64_t *__re
     vals[i] = (int32_t)i;
 }
 
+// float32: offsets from the leaf centre into the padded per-leaf ranges
+// (sp); float64: absolute positions in sorted order
 template <class T>
 __global__ void batch_gather(const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
                              int S, const BatchDom *__restrict__ dom,
                              const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ w, const uint32_t *__restrict__ skeys,
-                             const int32_t *__restrict__ perm, int64_t N, void *__restrict__ out)
+                             const int32_t *__restrict__ perm, const int32_t *__restrict__ start,
+                             const int32_t *__restrict__ sp, int64_t N, void *__restrict__ out)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
@@ -157,7 +161,7 @@ __global__ void batch_gather(const BatchSys *__restrict__ sys, const int64_t *__
         const BatchDom d = dom[s];
         double cx, cy;
         leaf_center((uint32_t)(g - sys[s].leafbase), d.x0, d.y0, d.s, cx, cy);
-        reinterpret_cast<float4 *>(out)[k] =
+        reinterpret_cast<float4 *>(out)[sp[g] + (k - start[g])] =
             make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
     } else {
         reinterpret_cast<double4 *>(out)[k] = make_double4(x[j], y[j], w[j], 0.0);
@@ -391,98 +395,91 @@ __global__ __launch_bounds__(64) void batch_m2l(
 }
 
 // ------------------------------------------------------ P2P + L2P ---------
+// Near set of a leaf of system s: the offsets of the (2R+1)^2 window that
+// pass the near predicate; offsets whose box gap g gives
+// shape g^2 log2(e) >= 26 (2% margin) skip the exponential (exact, see
+// pair_f32_far).
+struct BatchNear {
+    int R, side, ws, bx, by, n;
+    double sd, c2, shape_log2;
+    int64_t leafbase;
+    float s_f;
+    __device__ __forceinline__ bool operator()(int q, uint32_t &sb, float &ox, float &oy,
+                                               int &core) const
+    {
+        const int dx = q % side - R, dy = q / side - R;
+        ox = (float)dx * s_f;
+        oy = (float)dy * s_f;
+        const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy;
+        const double gx = (double)(ax > 1 ? ax - 1 : 0) * sd * 0.99;
+        const double gy = (double)(ay > 1 ? ay - 1 : 0) * sd * 0.99;
+        core = !(shape_log2 * (gx * gx + gy * gy) >= 26.0);
+        const int sx = bx + dx, sy = by + dy;
+        if (sx < 0 || sy < 0 || sx >= n || sy >= n) return false;
+        if (!near_pred(dx, dy, ws, sd, c2)) return false;
+        sb = (uint32_t)(leafbase + morton2((uint32_t)sx, (uint32_t)sy));
+        return true;
+    }
+};
+
+// One workgroup of 4 waves per leaf of the whole batch, sources staged in
+// LDS (phf_p2p_dev.hpp); targets are the system's own blobs, read from the
+// padded source array.  Leaves are dealt so that each XCD receives a
+// contiguous range.
 template <int KIND>
-__global__ __launch_bounds__(64) void batch_p2p_f32c(
+__global__ __launch_bounds__(256) void batch_p2p_f32s(
     const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
     const int64_t *__restrict__ lvl, int S, const BatchDom *__restrict__ dom,
     const float4 *__restrict__ src, const int32_t *__restrict__ start,
-    const int32_t *__restrict__ perm, int ws, const float2 *__restrict__ Bc, int p1,
-    double *__restrict__ vx, double *__restrict__ vy, int32_t *__restrict__ flag)
+    const int32_t *__restrict__ sp, const int32_t *__restrict__ perm, int ws,
+    const float2 *__restrict__ Bc, int p1, double *__restrict__ vx, double *__restrict__ vy,
+    int32_t *__restrict__ flag, int64_t nleaf, uint32_t nl8)
 {
-    const int lane = threadIdx.x;
-    const int64_t g = blockIdx.x;
-    const int32_t t0 = start[g], t1 = start[g + 1];
+    __shared__ float4 st[P2P_STAGE];
+    __shared__ StageSeg seg[64];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int g = lane >> 4, t = lane & 15;
+    const int64_t gl = (int64_t)(blockIdx.x & 7u) * nl8 + (blockIdx.x >> 3);
+    if (gl >= nleaf) return;
+    const int32_t t0 = start[gl], t1 = start[gl + 1];
     if (t1 <= t0) return;
-    const int s = find_seg(loff, S, g);
+    const int s = find_seg(loff, S, gl);
     const BatchSys sy = sys[s];
-    const uint32_t b = (uint32_t)(g - sy.leafbase);
-    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
-    const int n = 1 << sy.L;
+    const uint32_t b = (uint32_t)(gl - sy.leafbase);
     const double sd = dom[s].s;
     const float s_f = (float)sd;
     const float inv_s = 1.0f / s_f;
     int R = near_radius(ws, sd, sy.c2);
     if (R > kRMax) {
-        if (lane == 0) atomicOr(flag, 1);
+        if (tid == 0) atomicOr(flag, 1);
         R = kRMax;
     }
     PairParams pp{};
     pp.nlog2e_ks_f = sy.nlog2e_ks_f;
     pp.c2_f = sy.c2_f;
     pp.scale_f = sy.scale_f;
+    const BatchNear nf{R, 2 * R + 1, ws, (int)compact_bits(b), (int)compact_bits(b >> 1),
+                       1 << sy.L, sd, sy.c2, 1.4426950408889634074 * sy.inv_ks, sy.leafbase, s_f};
+    const int nnear = (2 * R + 1) * (2 * R + 1);
     const float2 *Bb = (sy.L >= 2 && Bc) ? Bc + (lvl[sy.L * (S + 1) + s] + b) * p1 : nullptr;
-    const double two_pi = 2.0 * 3.141592653589793;
-    for (int32_t tc = t0; tc < t1; tc += 64) {
-        const int32_t i = tc + lane;
+    const int32_t ts = sp[gl];
+    for (int32_t rc = t0; rc < t1; rc += 64) {
+        const int32_t i = rc + 16 * wv + t;
         const bool valid = i < t1;
-        const float4 tp = valid ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool wave_on = rc + 16 * wv < t1;
+        const float4 tq = valid ? src[ts + (i - t0)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float2 tp = make_float2(tq.x, tq.y);
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        for (int dy = -R; dy <= R; ++dy) {
-            for (int dx = -R; dx <= R; ++dx) {
-                const int sx = bx + dx, syy = by + dy;
-                if (sx < 0 || syy < 0 || sx >= n || syy >= n) continue;
-                if (!near_pred(dx, dy, ws, sd, sy.c2)) continue;
-                const int64_t sg = sy.leafbase + morton2((uint32_t)sx, (uint32_t)syy);
-                const int32_t j0 = start[sg], j1 = start[sg + 1];
-                if (j1 <= j0) continue;
-                const float xt = tp.x - (float)dx * s_f;
-                const float yt = tp.y - (float)dy * s_f;
-                int32_t j = j0;
-                for (; j + 16 <= j1; j += 16) {
-                    float aa = 0.f, ab = 0.f;
-#pragma unroll
-                    for (int k = 0; k < 16; ++k) {
-                        const float4 sp = src[j + k];
-                        const float ddx = xt - sp.x, ddy = yt - sp.y;
-                        const float P = pair_f32<KIND>(ddx, ddy, sp.z, pp);
-                        aa = __builtin_fmaf(P, ddx, aa);
-                        ab = __builtin_fmaf(P, ddy, ab);
-                    }
-                    two_sum_add(ha, la, aa);
-                    two_sum_add(hb, lb, ab);
-                }
-                if (j < j1) {
-                    float aa = 0.f, ab = 0.f;
-                    for (; j < j1; ++j) {
-                        const float4 sp = src[j];
-                        const float ddx = xt - sp.x, ddy = yt - sp.y;
-                        const float P = pair_f32<KIND>(ddx, ddy, sp.z, pp);
-                        aa = __builtin_fmaf(P, ddx, aa);
-                        ab = __builtin_fmaf(P, ddy, ab);
-                    }
-                    two_sum_add(ha, la, aa);
-                    two_sum_add(hb, lb, ab);
-                }
-            }
-        }
-        double fr = 0.0, fi = 0.0;
-        if (Bb) {
-            const float zr = tp.x * inv_s, zi = tp.y * inv_s;
-            float pr = 0.f, pi = 0.f;
-            for (int k = p1 - 1; k >= 0; --k) {
-                const float2 c = Bb[k];
-                const float nr = pr * zr - pi * zi + c.x;
-                const float ni = pr * zi + pi * zr + c.y;
-                pr = nr;
-                pi = ni;
-            }
-            fr = pr;
-            fi = pi;
-        }
-        if (valid) {
+        p2p_staged<KIND>(src, sp, st, seg, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
+        if (!wave_on) continue;
+        p2p_merge_groups(ha, la, hb, lb);
+        if (g == 0 && valid) {
+            double fr = 0.0, fi = 0.0;
+            if (Bb) l2p_f32(Bb, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
             const int32_t o = perm[i];
             const double qr = -(((double)ha + (double)la) + fr);
             const double qi = ((double)hb + (double)lb) - fi;
+            const double two_pi = 2.0 * 3.141592653589793;
             vx[o] = -qi / two_pi;  // _induced.py:334-336
             vy[o] = -qr / two_pi;
         }
@@ -680,7 +677,15 @@ int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
     PHF_ALLOC(Bf->v0, L->arena, int32_t, N);
     PHF_ALLOC(Bf->v1, L->arena, int32_t, N);
     PHF_ALLOC(Bf->start, L->arena, int32_t, (size_t)P.nleaf + 1);
-    PHF_ALLOC(Bf->src, L->arena, char, N * (f32 ? sizeof(float4) : sizeof(double4)));
+    if (f32) {
+        const size_t nl = (size_t)P.nleaf + 1;
+        PHF_ALLOC(Bf->src, L->arena, char, (N + 3 * nl) * sizeof(float4));
+        PHF_ALLOC(Bf->pcnt, L->arena, uint32_t, nl);
+        PHF_ALLOC(Bf->psp, L->arena, uint32_t, nl);
+        PHF_ALLOC(Bf->psums, L->arena, uint32_t, (size_t)scan_sums_size((int64_t)nl));
+    } else {
+        PHF_ALLOC(Bf->src, L->arena, char, N * sizeof(double4));
+    }
     const size_t csz = f32 ? sizeof(float2) : sizeof(double2);
     PHF_ALLOC(Bf->M, L->arena, char, (size_t)std::max<int64_t>(P.nbox, 1) * p1 * csz);
     PHF_ALLOC(Bf->B, L->arena, char, (size_t)std::max<int64_t>(P.nbox, 1) * p1 * csz);
@@ -777,16 +782,22 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
     PHF_TRY(device_radix_sort_ws(L, Bf.k0, Bf.v0, Bf.k1, Bf.v1, N, key_bits, Bf.hist, Bf.offs,
                                  Bf.sums, &ks, &vs));
     PHF_TRY(leaf_starts_device(L, ks, N, P.nleaf, Bf.start));
+    const int32_t *sp = Bf.start;  // source ranges per leaf (padded for float32)
+    if (sizeof(T) == 4) {
+        PHF_TRY(pad_source_ranges(L, Bf.start, P.nleaf, Bf.pcnt, Bf.psp, Bf.psums));
+        sp = reinterpret_cast<const int32_t *>(Bf.psp);
+        PHF_TRY(zero_source_pads(L, Bf.start, sp, P.nleaf, reinterpret_cast<float4 *>(Bf.src)));
+    }
     if (N > 0)
         batch_gather<T><<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.loff, S, Bf.dom, x, y,
-                                                               w, ks, vs, N, Bf.src);
+                                                               w, ks, vs, Bf.start, sp, N, Bf.src);
     PHF_HIP(hipGetLastError());
     V *M = reinterpret_cast<V *>(Bf.M), *B = reinterpret_cast<V *>(Bf.B);
     const VC *m2m = reinterpret_cast<const VC *>(Bf.m2m);
     const VC *l2l = reinterpret_cast<const VC *>(Bf.l2l);
     if (P.Lmax >= 2) {
         batch_p2m<T><<<(unsigned)P.nleaf, 64, (size_t)64 * p1 * sizeof(V), L->stream>>>(
-            Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, Bf.src, Bf.start, p1, M);
+            Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, Bf.src, sp, p1, M);
         for (int l = P.Lmax - 1; l >= 2; --l) {
             const int64_t nb = P.lvl[(size_t)l * (S + 1) + S] - P.lvl[(size_t)l * (S + 1)];
             if (nb > 0)
@@ -813,14 +824,15 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
     const V *Bc = P.Lmax >= 2 ? B : nullptr;
     if (P.nleaf > 0) {
         if (sizeof(T) == 4) {
+            const uint32_t nl8 = (uint32_t)((P.nleaf + 7) / 8);
             if (P.nocut_all)
-                batch_p2p_f32c<PAIR_OSEEN_NOCUT><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
-                    Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, vs,
-                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag);
+                batch_p2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, L->stream>>>(
+                    Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, sp, vs,
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8);
             else
-                batch_p2p_f32c<PAIR_OSEEN><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
-                    Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, vs,
-                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag);
+                batch_p2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, L->stream>>>(
+                    Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, sp, vs,
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8);
         } else {
             batch_p2p_f64<PAIR_OSEEN><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
                 Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const double4 *)Bf.src, Bf.start, vs, P.ws,

@@ -40,6 +40,7 @@ struct BatchBuffers {
     char *src = nullptr, *M = nullptr, *B = nullptr;
     char *m2m = nullptr, *l2l = nullptr, *bin = nullptr, *U = nullptr;
     uint32_t *hist = nullptr, *offs = nullptr, *sums = nullptr;
+    uint32_t *pcnt = nullptr, *psp = nullptr, *psums = nullptr;  // padded float32 ranges
     int32_t *flag = nullptr;
 };
 

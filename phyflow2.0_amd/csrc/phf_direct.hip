@@ -230,6 +230,104 @@ __global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int6
     }
 }
 
+// ------------------------------------------------ vorticity (Gaussian) ----
+// ww2parGauss_C_cpu.c:133-145:
+//   w(t) = (1/pi) sum_m w_m exp(-(|t - z_m|^2 + EPS)/ks) / ks,  EPS = 1e-12
+// Same layout as the velocity kernels with one accumulator per target.
+template <int TPL>
+__global__ __launch_bounds__(64) void vort_f32c(const Src32c *__restrict__ src, int64_t nb,
+                                                const double *__restrict__ xt,
+                                                const double *__restrict__ yt, int64_t nt,
+                                                int64_t chunk_len, PairParams pp,
+                                                double *__restrict__ part)
+{
+    const int lane = threadIdx.x;
+    const int64_t tbase = (int64_t)blockIdx.x * (kWave * TPL);
+    double tx[TPL], ty[TPL];
+    float h[TPL], l[TPL];
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        const int64_t t = tbase + lane + kWave * i;
+        tx[i] = t < nt ? xt[t] : 0.0;
+        ty[i] = t < nt ? yt[t] : 0.0;
+        h[i] = l[i] = 0.0f;
+    }
+    const int64_t j0 = (int64_t)blockIdx.y * chunk_len;
+    const int64_t j1 = min(nb, j0 + chunk_len);
+    for (int64_t j = j0; j < j1; j += 16) {
+        const int64_t je = min(j1, j + 16);
+        float a[TPL];
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) a[i] = 0.0f;
+        for (int64_t k = j; k < je; ++k) {
+            const Src32c s = src[k];
+#pragma unroll
+            for (int i = 0; i < TPL; ++i) {
+                const float dx = (float)(tx[i] - s.x), dy = (float)(ty[i] - s.y);
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, pp.eps_f));
+                a[i] = __builtin_fmaf(s.w, exp2_f32(r2 * pp.nlog2e_ks_f), a[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) two_sum_add(h[i], l[i], a[i]);
+    }
+    double *out = part + (int64_t)blockIdx.y * nt;
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        const int64_t t = tbase + lane + kWave * i;
+        if (t < nt) out[t] = (double)h[i] + (double)l[i];
+    }
+}
+
+template <int TPL>
+__global__ __launch_bounds__(64) void vort_f64(const Src64 *__restrict__ src, int64_t nb,
+                                               const double *__restrict__ xt,
+                                               const double *__restrict__ yt, int64_t nt,
+                                               int64_t chunk_len, PairParams pp,
+                                               double *__restrict__ part)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x;
+    const int64_t tbase = (int64_t)blockIdx.x * (kWave * TPL);
+    double tx[TPL], ty[TPL], a[TPL];
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        const int64_t t = tbase + lane + kWave * i;
+        tx[i] = t < nt ? xt[t] : 0.0;
+        ty[i] = t < nt ? yt[t] : 0.0;
+        a[i] = 0.0;
+    }
+    const int64_t j0 = (int64_t)blockIdx.y * chunk_len;
+    const int64_t j1 = min(nb, j0 + chunk_len);
+    for (int64_t j = j0; j < j1; ++j) {
+        const Src64 s = src[j];
+#pragma unroll
+        for (int i = 0; i < TPL; ++i) {
+            const double dx = tx[i] - s.x, dy = ty[i] - s.y;
+            const double r2 = dx * dx + dy * dy + pp.eps;
+            a[i] += s.w * exp(-r2 * pp.inv_ks);
+        }
+    }
+    double *out = part + (int64_t)blockIdx.y * nt;
+#pragma unroll
+    for (int i = 0; i < TPL; ++i) {
+        const int64_t t = tbase + lane + kWave * i;
+        if (t < nt) out[t] = a[i];
+    }
+}
+
+// chunk partials in chunk order, then inv_pi * w / ks (ww2parGauss_C_cpu.c:145)
+__global__ void vort_finish(const double *__restrict__ part, int nchunk, int64_t nt, double ks,
+                            double *__restrict__ w)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double a = 0.0;
+    for (int c = 0; c < nchunk; ++c) a += part[(int64_t)c * nt + t];
+    const double inv_pi = 1.0 / 3.141592653589793;
+    w[t] = inv_pi * a / ks;
+}
+
 // ------------------------------------------------------------- host ------
 int upload(Lane *L, const double *host, int64_t n, double **dev)
 {
@@ -290,6 +388,45 @@ static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *
     direct_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(
         part, nb == 0 ? 1 : (int)nchunk, nt, KIND == PAIR_GAUSS_EPS ? PAIR_GAUSS_EPS : PAIR_OSEEN,
         o0, o1);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, const double *yb,
+                     const double *wb, int64_t nb, const double *xt, const double *yt, int64_t nt,
+                     double *w)
+{
+    if (nt == 0) return PHF_OK;
+    const PairParams pp = make_gauss_params(ksigmasqr);
+    constexpr int TPL32 = 2, TPL64 = 1;
+    const int tpl = prec == PHF_PREC_F32C ? TPL32 : TPL64;
+    const int64_t tblock = (int64_t)kWave * tpl;
+    const int64_t ngroups = (nt + tblock - 1) / tblock;
+    int64_t nchunk = std::max<int64_t>(1, (4096 + ngroups - 1) / ngroups);
+    nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, nb / 256));
+    nchunk = std::min<int64_t>(nchunk, 65535);
+    int64_t chunk_len = (nb + nchunk - 1) / nchunk;
+    chunk_len = std::max<int64_t>(16, (chunk_len + 15) / 16 * 16);
+    nchunk = std::max<int64_t>(1, (nb + chunk_len - 1) / chunk_len);
+    double *part;
+    PHF_ALLOC(part, L->arena, double, (size_t)nchunk * (size_t)nt);
+    const dim3 grid((unsigned)ngroups, (unsigned)nchunk);
+    if (nb == 0) {
+        PHF_HIP(hipMemsetAsync(part, 0, sizeof(double) * nt, L->stream));
+    } else if (prec == PHF_PREC_F32C) {
+        Src32c *s;
+        PHF_ALLOC(s, L->arena, Src32c, (size_t)nb);
+        pack_src32c<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
+        vort_f32c<TPL32><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp, part);
+    } else {
+        Src64 *s;
+        PHF_ALLOC(s, L->arena, Src64, (size_t)nb);
+        pack_src64<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
+        vort_f64<TPL64><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp, part);
+    }
+    PHF_HIP(hipGetLastError());
+    vort_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(part, nb == 0 ? 1 : (int)nchunk, nt,
+                                                         ksigmasqr, w);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -360,5 +497,39 @@ extern "C" int phf_vv2par_gauss(const double *xb, const double *yb, const double
     tm.end(PHF_STAGE_TOTAL);
     PHF_HIP(hipStreamSynchronize(L->stream));
     record_stage_times(L, used);
+    return PHF_OK;
+}
+
+extern "C" int phf_ww2par_gauss(const double *xb, const double *yb, const double *wb,
+                                int64_t nb, const double *xt, const double *yt, int64_t nt,
+                                double ksigmasqr, int32_t prec, double *w)
+{
+    clear_error();
+    PHF_REQUIRE(nb >= 0 && nt >= 0, "negative sizes (nb=%lld, nt=%lld)", (long long)nb,
+                (long long)nt);
+    PHF_REQUIRE(prec == PHF_PREC_F64 || prec == PHF_PREC_F32C, "unknown precision %d", prec);
+    PHF_REQUIRE(std::isfinite(ksigmasqr) && ksigmasqr > 0.0,
+                "ksigmasqr must be finite and > 0 (got %g)", ksigmasqr);
+    PHF_TRY(check_host_array(xb, nb, "xBlob"));
+    PHF_TRY(check_host_array(yb, nb, "yBlob"));
+    PHF_TRY(check_host_array(wb, nb, "wBlob"));
+    PHF_TRY(check_host_array(xt, nt, "xTarget"));
+    PHF_TRY(check_host_array(yt, nt, "yTarget"));
+    PHF_REQUIRE(nt == 0 || w, "output array is NULL");
+    if (nt == 0) return PHF_OK;
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(L->arena.reset());
+    double *dxb, *dyb, *dwb, *dxt, *dyt, *o;
+    PHF_TRY(upload(L, xb, nb, &dxb));
+    PHF_TRY(upload(L, yb, nb, &dyb));
+    PHF_TRY(upload(L, wb, nb, &dwb));
+    PHF_TRY(upload(L, xt, nt, &dxt));
+    PHF_TRY(upload(L, yt, nt, &dyt));
+    PHF_ALLOC(o, L->arena, double, (size_t)nt);
+    PHF_TRY(vorticity_device(L, prec, ksigmasqr, dxb, dyb, dwb, nb, dxt, dyt, nt, o));
+    PHF_TRY(download(L, o, nt, w));
+    PHF_HIP(hipStreamSynchronize(L->stream));
     return PHF_OK;
 }

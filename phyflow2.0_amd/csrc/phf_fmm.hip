@@ -32,6 +32,7 @@
 // order exceeds 16.
 #include "phf_fmm.hpp"
 #include "phf_fmm_dev.hpp"
+#include "phf_p2p_dev.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -272,14 +273,6 @@ __global__ __launch_bounds__(64) void m2l_kernel(
 // float32 sources are stored per leaf in ranges padded to a multiple of 4
 // with zero-weight entries (exact zeros in every sum they enter), so the
 // four lane groups of a wave can split a leaf's sources without a tail.
-__global__ void padded_counts(const int32_t *__restrict__ start, int64_t nleaf,
-                              uint32_t *__restrict__ cnt)
-{
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b > nleaf) return;
-    cnt[b] = b < nleaf ? (uint32_t)((start[b + 1] - start[b] + 3) & ~3) : 0u;
-}
-
 __global__ void gather_src_f32p(const double *__restrict__ x, const double *__restrict__ y,
                                 const double *__restrict__ w, const uint32_t *__restrict__ skeys,
                                 const int32_t *__restrict__ perm, const int32_t *__restrict__ start,
@@ -296,15 +289,6 @@ __global__ void gather_src_f32p(const double *__restrict__ x, const double *__re
         make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
 }
 
-__global__ void zero_pads_f32(const int32_t *__restrict__ start, const int32_t *__restrict__ sp,
-                              int64_t nleaf, float4 *__restrict__ out)
-{
-    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nleaf) return;
-    for (int32_t k = sp[b] + (start[b + 1] - start[b]); k < sp[b + 1]; ++k)
-        out[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
 // One workgroup of 4 waves per leaf.  The sources of the leaf's near leaves
 // are staged in LDS (as many near leaves, or parts of one, as fit: a
 // cursor walks the near list), then wave v takes the targets
@@ -319,64 +303,24 @@ __global__ void zero_pads_f32(const int32_t *__restrict__ start, const int32_t *
 // exponential (bit-identical P).  Leaves are assigned so that the
 // workgroups one XCD receives (they are dealt round-robin over the 8 XCDs)
 // cover a contiguous range of Morton leaves and share sources in its L2.
-constexpr int P2P_STAGE = 1536;  // staged sources (float4) per batch
-
-template <int KIND, bool CORE>
-__device__ __forceinline__ void p2p_seg_f32(const float4 *__restrict__ sq, int j0, int j1,
-                                            float xt, float yt, const PairParams &pp, float &ha,
-                                            float &la, float &hb, float &lb)
-{
-    int j = j0;
-    for (; j + 64 <= j1; j += 64) {
-        float aa = 0.f, ab = 0.f;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const float4 q = sq[j + 4 * k];
-            const float dx = xt - q.x, dy = yt - q.y;
-            const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
-            aa = __builtin_fmaf(P, dx, aa);
-            ab = __builtin_fmaf(P, dy, ab);
-        }
-        two_sum_add(ha, la, aa);
-        two_sum_add(hb, lb, ab);
+// near set of a single-system leaf: the host-built offset list, offsets
+// [0, ninner) first
+struct NearList {
+    const int2 *near;
+    int ninner, bx, by, n;
+    float s_f;
+    __device__ __forceinline__ bool operator()(int q, uint32_t &sb, float &ox, float &oy,
+                                               int &core) const
+    {
+        const int2 o = near[q];
+        const int sx = bx + o.x, sy = by + o.y;
+        ox = (float)o.x * s_f;
+        oy = (float)o.y * s_f;
+        core = q < ninner;
+        if (sx < 0 || sy < 0 || sx >= n || sy >= n) return false;
+        sb = morton2((uint32_t)sx, (uint32_t)sy);
+        return true;
     }
-    if (j < j1) {
-        float aa = 0.f, ab = 0.f;
-        for (; j + 16 <= j1; j += 16) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float4 q = sq[j + 4 * k];
-                const float dx = xt - q.x, dy = yt - q.y;
-                const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
-                aa = __builtin_fmaf(P, dx, aa);
-                ab = __builtin_fmaf(P, dy, ab);
-            }
-        }
-        for (; j < j1; j += 4) {
-            const float4 q = sq[j];
-            const float dx = xt - q.x, dy = yt - q.y;
-            const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
-            aa = __builtin_fmaf(P, dx, aa);
-            ab = __builtin_fmaf(P, dy, ab);
-        }
-        two_sum_add(ha, la, aa);
-        two_sum_add(hb, lb, ab);
-    }
-}
-
-__device__ __forceinline__ void two_sum_merge(float &h, float &l, float h2, float l2)
-{
-    const float s = h + h2;
-    const float bp = s - h;
-    const float err = (h - (s - bp)) + (h2 - bp);
-    h = s;
-    l = (l + l2) + err;
-}
-
-struct StageSeg {
-    int lds0, len;   // staged range [lds0, lds0 + len)
-    float sx, sy;    // centre shift of the source leaf (exact)
-    int core;        // 1: pairs may lie inside the smoothing core
 };
 
 template <int KIND>
@@ -396,7 +340,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_f32s(
     if (b >= (uint32_t)n * (uint32_t)n) return;
     const int32_t t0 = start_t[b], t1 = start_t[b + 1];
     if (t1 <= t0) return;
-    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
+    const NearList nf{near, ninner, (int)compact_bits(b), (int)compact_bits(b >> 1), n, s_f};
     const float inv_s = 1.0f / s_f;
     for (int32_t rc = t0; rc < t1; rc += 64) {
         const int32_t i = rc + 16 * wv + t;
@@ -404,102 +348,12 @@ __global__ __launch_bounds__(256) void p2p_l2p_f32s(
         const bool wave_on = rc + 16 * wv < t1;
         const float2 tp = valid ? tgt[i] : make_float2(0.f, 0.f);
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        // cursor over (near offset q, position jq within that leaf's range);
-        // each batch: lane l of every wave looks at near offset q + l, a wave
-        // scan of the range lengths packs as many as fit into the stage
-        int q = 0, jq = 0;
-        while (q < nnear) {
-            const int ql = q + lane;
-            int j0 = 0, len = 0;
-            float ox = 0.f, oy = 0.f;
-            if (ql < nnear) {
-                const int2 o = near[ql];
-                const int sx = bx + o.x, sy = by + o.y;
-                ox = (float)o.x * s_f;
-                oy = (float)o.y * s_f;
-                if (sx >= 0 && sy >= 0 && sx < n && sy < n) {
-                    const uint32_t sb = morton2((uint32_t)sx, (uint32_t)sy);
-                    j0 = sp[sb] + (lane == 0 ? jq : 0);
-                    len = sp[sb + 1] - j0;
-                }
-            }
-            int incl = len;
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int v = __shfl_up(incl, d);
-                if (lane >= d) incl += v;
-            }
-            const int excl = incl - len;
-            const bool in = ql < nnear && excl < P2P_STAGE;
-            const int take = in ? min(len, P2P_STAGE - excl) : 0;  // multiple of 4
-            const int count = __popcll(__ballot(in));
-            const int last_take = __shfl(take, count - 1), last_len = __shfl(len, count - 1);
-            if (last_take < last_len) {
-                jq = (count == 1 ? jq : 0) + last_take;
-                q += count - 1;
-            } else {
-                jq = 0;
-                q += count;
-            }
-            if (wv == 0 && in) {
-                seg[lane].lds0 = excl;
-                seg[lane].len = take;
-                seg[lane].sx = ox;
-                seg[lane].sy = oy;
-                seg[lane].core = ql < ninner;
-            }
-            if (in) {
-                // this lane's segment, entries wv, wv + 4, ... (8 loads in flight)
-                const float4 *gs = src + j0;
-                float4 *ls = st + excl;
-                int e = wv;
-                for (; e + 28 < take; e += 32) {
-                    float4 r[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) r[u] = gs[e + 4 * u];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) ls[e + 4 * u] = r[u];
-                }
-                for (; e < take; e += 4) ls[e] = gs[e];
-            }
-            __syncthreads();
-            if (wave_on) {
-                for (int k = 0; k < count; ++k) {
-                    const StageSeg sg = seg[k];
-                    if (sg.len == 0) continue;
-                    const float xt = tp.x - sg.sx, yt = tp.y - sg.sy;
-                    const float4 *sq = st + sg.lds0 + g;
-                    if (sg.core)
-                        p2p_seg_f32<KIND, true>(sq, 0, sg.len, xt, yt, pp, ha, la, hb, lb);
-                    else
-                        p2p_seg_f32<KIND, false>(sq, 0, sg.len, xt, yt, pp, ha, la, hb, lb);
-                }
-            }
-            __syncthreads();
-        }
+        p2p_staged<KIND>(src, sp, st, seg, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
         if (!wave_on) continue;
-        // groups (0,1) and (2,3), then the pairs: every lane of a target ends
-        // with the same value
-        two_sum_merge(ha, la, __shfl_xor(ha, 16), __shfl_xor(la, 16));
-        two_sum_merge(hb, lb, __shfl_xor(hb, 16), __shfl_xor(lb, 16));
-        two_sum_merge(ha, la, __shfl_xor(ha, 32), __shfl_xor(la, 32));
-        two_sum_merge(hb, lb, __shfl_xor(hb, 32), __shfl_xor(lb, 32));
+        p2p_merge_groups(ha, la, hb, lb);
         if (g == 0 && valid) {
             double fr = 0.0, fi = 0.0;
-            if (Bleaf) {
-                const float2 *Bb = Bleaf + (int64_t)b * p1;
-                const float zr = tp.x * inv_s, zi = tp.y * inv_s;
-                float pr = 0.f, pi = 0.f;
-                for (int k = p1 - 1; k >= 0; --k) {
-                    const float2 c = Bb[k];
-                    const float nr = pr * zr - pi * zi + c.x;
-                    const float ni = pr * zi + pi * zr + c.y;
-                    pr = nr;
-                    pi = ni;
-                }
-                fr = pr;
-                fi = pi;
-            }
+            if (Bleaf) l2p_f32(Bleaf + (int64_t)b * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
             const int32_t o = perm_t[i];
             const double qr = -(((double)ha + (double)la) + fr);
             const double qi = ((double)hb + (double)lb) - fi;
@@ -644,6 +498,38 @@ int q_to_velocity_device(Lane *L, const double *qr, const double *qi, int64_t n,
 {
     if (n <= 0) return PHF_OK;
     q_to_velocity<<<div_up(n, 256), 256, 0, L->stream>>>(qr, qi, n, vx, vy);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+__global__ void padded_counts(const int32_t *__restrict__ start, int64_t nleaf,
+                              uint32_t *__restrict__ cnt)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nleaf) return;
+    cnt[b] = b < nleaf ? (uint32_t)((start[b + 1] - start[b] + 3) & ~3) : 0u;
+}
+
+__global__ void zero_pads_f32(const int32_t *__restrict__ start, const int32_t *__restrict__ sp,
+                              int64_t nleaf, float4 *__restrict__ out)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nleaf) return;
+    for (int32_t k = sp[b] + (start[b + 1] - start[b]); k < sp[b + 1]; ++k)
+        out[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+int pad_source_ranges(Lane *L, const int32_t *start, int64_t nleaf, uint32_t *cnt, uint32_t *sp,
+                      uint32_t *sums)
+{
+    padded_counts<<<div_up(nleaf + 1, 256), 256, 0, L->stream>>>(start, nleaf, cnt);
+    PHF_HIP(hipGetLastError());
+    return device_exclusive_scan_ws(L, cnt, nleaf + 1, sp, sums);
+}
+
+int zero_source_pads(Lane *L, const int32_t *start, const int32_t *sp, int64_t nleaf, float4 *src)
+{
+    if (nleaf > 0) zero_pads_f32<<<div_up(nleaf, 256), 256, 0, L->stream>>>(start, sp, nleaf, src);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -834,18 +720,18 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     const int64_t nleaf64 = int64_t(1) << (2 * Lv);
     const int32_t *sp = S.start;  // source ranges per leaf (padded for float32)
     if (sizeof(T) == 4) {
-        uint32_t *cnt, *spu;
+        uint32_t *cnt, *spu, *sums;
         PHF_ALLOC(cnt, L->arena, uint32_t, (size_t)nleaf64 + 1);
         PHF_ALLOC(spu, L->arena, uint32_t, (size_t)nleaf64 + 1);
-        padded_counts<<<div_up(nleaf64 + 1, 256), 256, 0, L->stream>>>(S.start, nleaf64, cnt);
-        PHF_TRY(device_exclusive_scan(L, cnt, nleaf64 + 1, spu));
+        PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(nleaf64 + 1));
+        PHF_TRY(pad_source_ranges(L, S.start, nleaf64, cnt, spu, sums));
         sp = reinterpret_cast<const int32_t *>(spu);
         float4 *s4;
         PHF_ALLOC(s4, L->arena, float4, (size_t)(a.nb + 3 * nleaf64 + 1));
         if (a.nb)
             gather_src_f32p<<<div_up(a.nb, 256), 256, 0, L->stream>>>(
                 a.xb, a.yb, a.wb, S.keys, S.perm, S.start, sp, a.nb, dom.x0, dom.y0, dom.s, s4);
-        zero_pads_f32<<<div_up(nleaf64, 256), 256, 0, L->stream>>>(S.start, sp, nleaf64, s4);
+        PHF_TRY(zero_source_pads(L, S.start, sp, nleaf64, s4));
         float2 *t2;
         PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
         gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(

@@ -27,6 +27,12 @@ struct FmmInfo {
 int fmm_order_from_tol(double tol);
 int fmm_level(int64_t nb, int leaf, int ndirect);
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info);
+// float32 source ranges per leaf padded to multiples of 4: sp[b] (nleaf + 1
+// entries) from the leaf offsets; cnt, sums: workspaces (capturable).
+int pad_source_ranges(Lane *L, const int32_t *start, int64_t nleaf, uint32_t *cnt, uint32_t *sp,
+                      uint32_t *sums);
+// zero-weight entries in the pads of a padded float32 source array
+int zero_source_pads(Lane *L, const int32_t *start, const int32_t *sp, int64_t nleaf, float4 *src);
 int q_to_velocity_device(Lane *L, const double *qr, const double *qi, int64_t n, double *vx,
                          double *vy);
 

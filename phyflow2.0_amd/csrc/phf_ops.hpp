@@ -22,5 +22,9 @@ int direct_device(Lane *L, int kind, int prec, const PairParams &pp,
                   const double *xb, const double *yb, const double *wb,
                   int64_t nb, const double *xt, const double *yt, int64_t nt,
                   double *o0, double *o1);
+// Gaussian induced vorticity (ww2parGauss) at the targets into w (device).
+int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, const double *yb,
+                     const double *wb, int64_t nb, const double *xt, const double *yt, int64_t nt,
+                     double *w);
 
 }  // namespace phf

@@ -52,35 +52,101 @@ __global__ void panel_influence_kernel(int64_t M, const double *__restrict__ xCP
 }
 
 // Velocity induced at evaluation points by panels of strength gamma
-// (pyx:357-391), one thread per point, panels in order.
-__global__ void panel_induced_kernel(int64_t M, const double *__restrict__ gamma, int64_t ne,
-                                     const double *__restrict__ xE, const double *__restrict__ yE,
-                                     const double *__restrict__ xs, const double *__restrict__ ys,
-                                     const double *__restrict__ xe, const double *__restrict__ ye,
-                                     const double *__restrict__ cosA,
-                                     const double *__restrict__ sinA, double *__restrict__ vx,
-                                     double *__restrict__ vy)
+// (pyx:357-391).  Per panel, in its frame (xP along the panel from its
+// start, yP normal; panel length x2):
+//   u = gamma/(2 pi) (atan2(yP, xP - x2) - atan2(yP, xP))
+//   v = -gamma/(4 pi) log((xP^2 + yP^2) / ((xP - x2)^2 + yP^2))
+// rotated back to the global frame.  Evaluated here as
+//   atan2(yP x2, xP (xP - x2) + yP^2)        (the angle between the two
+//                                              endpoint vectors; both
+//                                              atan2 share the sign of yP,
+//                                              so the difference lies in
+//                                              [-pi, pi] and the forms agree)
+//   log1p(x2 (2 xP - x2) / ((xP - x2)^2 + yP^2))   (num - den formed exactly)
+// which avoids the cancellation of the original forms far from the panel
+// (one atan2 and one log1p per pair instead of two atan2, a log and a
+// division).  Panel constants are formed once and staged in LDS.
+struct PanelK {
+    double xs, ys, c, s;  // start, cos, sin
+    double x2, g2, g4, pad;  // length along the panel, gamma/(2 pi), gamma/(4 pi)
+};
+
+__global__ void panel_consts(int64_t M, const double *__restrict__ gamma,
+                             const double *__restrict__ xs, const double *__restrict__ ys,
+                             const double *__restrict__ xe, const double *__restrict__ ye,
+                             const double *__restrict__ cosA, const double *__restrict__ sinA,
+                             PanelK *__restrict__ out)
 {
 #pragma clang fp contract(off)
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ne) return;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= M) return;
     const double inv_2pi = 1.0 / (2.0 * M_PI);
     const double inv_4pi = inv_2pi / 2.0;
+    const double Lx = xe[j] - xs[j];
+    const double Ly = ye[j] - ys[j];
+    PanelK k;
+    k.xs = xs[j];
+    k.ys = ys[j];
+    k.c = cosA[j];
+    k.s = sinA[j];
+    k.x2 = cosA[j] * Lx + sinA[j] * Ly;
+    k.g2 = gamma[j] * inv_2pi;
+    k.g4 = gamma[j] * inv_4pi;
+    k.pad = 0.0;
+    out[j] = k;
+}
+
+constexpr int PI_T = 256;      // evaluation points per block
+constexpr int PI_CHUNK = 256;  // panels staged per round
+
+// F32 = false: float64 throughout.  F32 = true (mixed-precision steps): the
+// panel-frame coordinates and the arguments are formed in float64, the
+// atan2 / log1p in float32, sums in float64 (about 1e-7 relative per pair).
+template <bool F32>
+__global__ __launch_bounds__(PI_T) void panel_induced_kernel(int64_t M, const PanelK *__restrict__ pk,
+                                                              int64_t ne,
+                                                              const double *__restrict__ xE,
+                                                              const double *__restrict__ yE,
+                                                              double *__restrict__ vx,
+                                                              double *__restrict__ vy)
+{
+    __shared__ PanelK sh[PI_CHUNK];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * PI_T + tid;
+    const double x = i < ne ? xE[i] : 0.0, y = i < ne ? yE[i] : 0.0;
     double sx = 0.0, sy = 0.0;
-    for (int64_t j = 0; j < M; ++j) {
-        const double Lx = xe[j] - xs[j];
-        const double Ly = ye[j] - ys[j];
-        const double x2 = cosA[j] * Lx + sinA[j] * Ly;
-        const double xP = cosA[j] * (xE[i] - xs[j]) + sinA[j] * (yE[i] - ys[j]);
-        const double yP = -sinA[j] * (xE[i] - xs[j]) + cosA[j] * (yE[i] - ys[j]);
-        const double vxP = (gamma[j] * inv_2pi) * (atan2(yP, (xP - x2)) - atan2(yP, xP));
-        const double vyP =
-            -(gamma[j] * inv_4pi) * log((xP * xP + yP * yP) / ((xP - x2) * (xP - x2) + yP * yP));
-        sx += cosA[j] * vxP - sinA[j] * vyP;
-        sy += sinA[j] * vxP + cosA[j] * vyP;
+    for (int64_t j0 = 0; j0 < M; j0 += PI_CHUNK) {
+        const int cnt = (int)min((int64_t)PI_CHUNK, M - j0);
+        __syncthreads();
+        if (tid < cnt) sh[tid] = pk[j0 + tid];
+        __syncthreads();
+        for (int k = 0; k < cnt; ++k) {
+            const PanelK p = sh[k];
+            const double dx = x - p.xs, dy = y - p.ys;
+            const double xP = p.c * dx + p.s * dy;
+            const double yP = -p.s * dx + p.c * dy;
+            const double xm = xP - p.x2;
+            const double cross = yP * p.x2, dot = xP * xm + yP * yP;
+            const double den = xm * xm + yP * yP;
+            const double num = p.x2 * (2.0 * xP - p.x2);
+            double th, lg;
+            if (F32) {
+                th = (double)atan2f((float)cross, (float)dot);
+                lg = (double)log1pf((float)num / (float)den);
+            } else {
+                th = atan2(cross, dot);
+                lg = log1p(num / den);
+            }
+            const double vxP = p.g2 * th;
+            const double vyP = -p.g4 * lg;
+            sx += p.c * vxP - p.s * vyP;
+            sy += p.s * vxP + p.c * vyP;
+        }
     }
-    vx[i] = sx;
-    vy[i] = sy;
+    if (i < ne) {
+        vx[i] = sx;
+        vy[i] = sy;
+    }
 }
 
 // ------------------------------------------------------- matrix setup ----
@@ -258,24 +324,23 @@ __global__ __launch_bounds__(64) void lu_update_mfma(float *__restrict__ A, int 
     }
 }
 
-// -------------------------------------------------------- LU solve --------
-// Solves (P L U) d = s * r with the float32 factors for one right-hand
-// side (float64 in, float64 out; inner arithmetic float32, the residual is
-// pre-scaled by 1/s = 1/max|r| to keep it in float32 range).  One
-// workgroup; blocked substitution: a wave solves each NB x NB diagonal
-// block, then all threads update the rows beyond it.
+// ------------------------------------------------- approximate inverse ----
+// X = (P L U)^-1 in float32, column k from the right-hand side e_k (one
+// workgroup per column; blocked substitution in LDS: a wave solves each
+// NB x NB diagonal block, then all threads update the rows beyond it).
+// X is stored row-major.  The refinement applies it as a matrix-vector
+// product, so every sweep is two or three memory-bound GEMVs instead of a
+// latency-bound triangular solve.
 constexpr int SOLVE_T = 512;
 
-__global__ __launch_bounds__(SOLVE_T) void lu_solve_kernel(const float *__restrict__ LU, int n,
-                                                           const int32_t *__restrict__ ipiv,
-                                                           const double *__restrict__ r,
-                                                           const double *__restrict__ rmax,
-                                                           double *__restrict__ d)
+__global__ __launch_bounds__(SOLVE_T) void lu_inverse_kernel(const float *__restrict__ LU, int n,
+                                                             const int32_t *__restrict__ ipiv,
+                                                             float *__restrict__ X)
 {
     extern __shared__ float y[];  // n
     const int tid = threadIdx.x;
-    const double s = rmax[0] > 0.0 ? rmax[0] : 1.0;
-    for (int i = tid; i < n; i += SOLVE_T) y[i] = (float)(r[i] / s);
+    const int col = blockIdx.x;
+    for (int i = tid; i < n; i += SOLVE_T) y[i] = i == col ? 1.0f : 0.0f;
     __syncthreads();
     if (tid == 0)
         for (int k = 0; k < n; ++k) {
@@ -335,55 +400,96 @@ __global__ __launch_bounds__(SOLVE_T) void lu_solve_kernel(const float *__restri
         }
         __syncthreads();
     }
-    for (int i = tid; i < n; i += SOLVE_T) d[i] = (double)y[i] * s;
+    for (int i = tid; i < n; i += SOLVE_T) X[(int64_t)i * n + col] = y[i];
 }
 
 // ------------------------------------------------- refinement kernels -----
-// r = c - B x  (B column-major rows x n), float64
-__global__ void resid_full(const double *__restrict__ Bc, int64_t rows, int64_t n,
-                           const double *__restrict__ x, const double *__restrict__ c,
-                           double *__restrict__ r)
+// Convergence state of one refinement, kept on the device so that sweeps
+// run back to back; every kernel of a sweep returns at once when done.
+struct RefineState {
+    double last_dx;
+    int32_t done, iterations, converged, pad;
+};
+
+__device__ __forceinline__ double wave_sum_f64(double v)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    double s = c[i];
-    for (int64_t j = 0; j < n; ++j) s = __builtin_fma(-Bc[j * rows + i], x[j], s);
-    r[i] = s;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;  // fixed butterfly: every lane holds the same sum
 }
 
-// g = B^T r  (float64)
-__global__ void transpose_apply(const double *__restrict__ Bc, int64_t rows, int64_t n,
-                                const double *__restrict__ r, double *__restrict__ g)
+__global__ void refine_init(double *__restrict__ x, int64_t n, RefineState *__restrict__ st)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+    if (i == 0) {
+        st->last_dx = INFINITY;
+        st->done = 0;
+        st->iterations = 0;
+        st->converged = 0;
+    }
+}
+
+// r = c - B x  (B row-major rows x n, float64), one wave per row
+__global__ __launch_bounds__(256) void resid_rows(const double *__restrict__ Br, int64_t rows,
+                                                  int64_t n, const double *__restrict__ x,
+                                                  const double *__restrict__ c,
+                                                  double *__restrict__ r,
+                                                  const RefineState *__restrict__ st)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= rows) return;
+    const double *row = Br + i * n;
+    double s = 0.0;
+    for (int64_t j = lane; j < n; j += 64) s = __builtin_fma(row[j], x[j], s);
+    s = wave_sum_f64(s);
+    if (lane == 0) r[i] = c[i] - s;
+}
+
+// g = B^T r  (B column-major rows x n, float64), one wave per column
+__global__ __launch_bounds__(256) void tapply_cols(const double *__restrict__ Bc, int64_t rows,
+                                                   int64_t n, const double *__restrict__ r,
+                                                   double *__restrict__ g,
+                                                   const RefineState *__restrict__ st)
+{
+    if (st->done) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= n) return;
     const double *col = Bc + j * rows;
     double s = 0.0;
-    for (int64_t i = 0; i < rows; ++i) s = __builtin_fma(col[i], r[i], s);
-    g[j] = s;
+    for (int64_t i = lane; i < rows; i += 64) s = __builtin_fma(col[i], r[i], s);
+    s = wave_sum_f64(s);
+    if (lane == 0) g[j] = s;
 }
 
-// max|v| into out[0] (one block)
-__global__ __launch_bounds__(256) void absmax_kernel(const double *__restrict__ v, int64_t n,
-                                                     double *__restrict__ out)
+// d = X v  (X float32 row-major n x n, products and sums in float64)
+__global__ __launch_bounds__(256) void xapply_rows(const float *__restrict__ X, int64_t n,
+                                                   const double *__restrict__ v,
+                                                   double *__restrict__ d,
+                                                   const RefineState *__restrict__ st)
 {
-    __shared__ double sh[256];
-    double m = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(v[i]));
-    sh[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[0] = sh[0];
+    if (st->done) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const float *row = X + i * n;
+    double s = 0.0;
+    for (int64_t j = lane; j < n; j += 64) s = __builtin_fma((double)row[j], v[j], s);
+    s = wave_sum_f64(s);
+    if (lane == 0) d[i] = s;
 }
 
-// x += d; stats[0] = max|d|, stats[1] = max|x| (one block)
-__global__ __launch_bounds__(256) void axpy_stats(double *__restrict__ x,
-                                                  const double *__restrict__ d, int64_t n,
-                                                  double *__restrict__ stats)
+// x += d, then the stopping rule: the correction has reached float64
+// resolution (|d| <= 4 eps |x|), or it stopped shrinking (after the third
+// sweep, |d| >= |d_prev| / 2: converged if |d| <= 1e-10 |x|), or a
+// non-finite value, or max_it sweeps.  One block.
+__global__ __launch_bounds__(256) void refine_update(double *__restrict__ x,
+                                                     const double *__restrict__ d, int64_t n,
+                                                     RefineState *__restrict__ st, int max_it)
 {
+    if (st->done) return;
     __shared__ double sd[256], sx[256];
     double md = 0.0, mx = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += 256) {
@@ -403,8 +509,24 @@ __global__ __launch_bounds__(256) void axpy_stats(double *__restrict__ x,
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        stats[0] = sd[0];
-        stats[1] = sx[0];
+        const double dmax = sd[0], xmax = sx[0];
+        const int it = st->iterations;
+        const double eps = 2.220446049250313e-16;
+        int stop = 0, conv = 0;
+        if (!isfinite(dmax) || !isfinite(xmax)) {
+            stop = 1;
+        } else if (dmax <= 4.0 * eps * xmax) {
+            stop = conv = 1;
+        } else if (it > 2 && dmax >= 0.5 * st->last_dx) {
+            stop = 1;
+            conv = dmax <= 1e-10 * xmax;
+        }
+        st->iterations = it + 1;
+        st->last_dx = dmax;
+        if (stop || it + 1 >= max_it) {
+            st->done = 1;
+            st->converged = conv;
+        }
     }
 }
 
@@ -446,6 +568,11 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
         normal_matrix<<<div_up(n * n, 256), 256, 0, L->stream>>>(f->Bc, rows, n, N64, f->LU);
     }
     PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info));
+    PHF_ALLOC(f->Xinv, L->arena, float, (size_t)(n * n));
+    lu_inverse_kernel<<<(unsigned)n, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(
+        f->LU, (int)n, f->ipiv, f->Xinv);
+    PHF_HIP(hipGetLastError());
+    f->Br = B;
     int32_t hinfo = 0;
     PHF_HIP(hipMemcpyAsync(&hinfo, f->info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
@@ -462,48 +589,33 @@ int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double 
 {
     const int64_t rows = f.rows, n = f.n;
     const bool normal = rows > n;
-    double *r, *g, *d, *stats, *rmax;
+    double *r, *g, *d;
+    RefineState *dst;
     PHF_ALLOC(r, L->arena, double, (size_t)rows);
     PHF_ALLOC(g, L->arena, double, (size_t)n);
     PHF_ALLOC(d, L->arena, double, (size_t)n);
-    PHF_ALLOC(stats, L->arena, double, 2);
-    PHF_ALLOC(rmax, L->arena, double, 1);
-    PHF_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, L->stream));
-    int it = 0;
-    bool conv = false;
-    double last_dx = INFINITY;
-    const int max_it = 60;
-    const double eps = 2.220446049250313e-16;
-    for (; it < max_it; ++it) {
-        resid_full<<<div_up(rows, 256), 256, 0, L->stream>>>(f.Bc, rows, n, x, c, r);
-        const double *rhs = r;
-        if (normal) {
-            transpose_apply<<<div_up(n, 256), 256, 0, L->stream>>>(f.Bc, rows, n, r, g);
-            rhs = g;
+    PHF_ALLOC(dst, L->arena, RefineState, 1);
+    refine_init<<<div_up(std::max<int64_t>(n, 1), 256), 256, 0, L->stream>>>(x, n, dst);
+    const int max_it = 60, burst = 6;
+    RefineState hs{};
+    for (int launched = 0; launched < max_it; launched += burst) {
+        for (int b = 0; b < burst && launched + b < max_it; ++b) {
+            resid_rows<<<div_up(rows, 4), 256, 0, L->stream>>>(f.Br, rows, n, x, c, r, dst);
+            const double *rhs = r;
+            if (normal) {
+                tapply_cols<<<div_up(n, 4), 256, 0, L->stream>>>(f.Bc, rows, n, r, g, dst);
+                rhs = g;
+            }
+            xapply_rows<<<div_up(n, 4), 256, 0, L->stream>>>(f.Xinv, n, rhs, d, dst);
+            refine_update<<<1, 256, 0, L->stream>>>(x, d, n, dst, max_it);
         }
-        absmax_kernel<<<1, 256, 0, L->stream>>>(rhs, n, rmax);
-        lu_solve_kernel<<<1, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(f.LU, (int)n, f.ipiv,
-                                                                            rhs, rmax, d);
-        axpy_stats<<<1, 256, 0, L->stream>>>(x, d, n, stats);
         PHF_HIP(hipGetLastError());
-        double hs[2];
-        PHF_HIP(hipMemcpyAsync(hs, stats, sizeof(hs), hipMemcpyDeviceToHost, L->stream));
+        PHF_HIP(hipMemcpyAsync(&hs, dst, sizeof(hs), hipMemcpyDeviceToHost, L->stream));
         PHF_HIP(hipStreamSynchronize(L->stream));
-        if (!std::isfinite(hs[0]) || !std::isfinite(hs[1])) break;
-        if (hs[0] <= 4.0 * eps * hs[1]) {  // correction at float64 resolution
-            conv = true;
-            ++it;
-            break;
-        }
-        if (it > 2 && hs[0] >= 0.5 * last_dx) {  // stalled at its limiting accuracy
-            conv = hs[0] <= 1e-10 * hs[1];
-            ++it;
-            break;
-        }
-        last_dx = hs[0];
+        if (hs.done) break;
     }
-    st->iterations = it;
-    st->converged = conv;
+    st->iterations = hs.iterations;
+    st->converged = hs.converged != 0;
     return PHF_OK;
 }
 
@@ -531,11 +643,20 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
 int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
                          const double *yE, const double *xs, const double *ys, const double *xe,
                          const double *ye, const double *cosA, const double *sinA, double *vx,
-                         double *vy)
+                         double *vy, int prec)
 {
     if (ne <= 0) return PHF_OK;
-    panel_induced_kernel<<<div_up(ne, 128), 128, 0, L->stream>>>(m, gamma, ne, xE, yE, xs, ys, xe,
-                                                                 ye, cosA, sinA, vx, vy);
+    PanelK *pk;
+    PHF_ALLOC(pk, L->arena, PanelK, (size_t)std::max<int64_t>(m, 1));
+    if (m > 0)
+        panel_consts<<<div_up(m, 256), 256, 0, L->stream>>>(m, gamma, xs, ys, xe, ye, cosA, sinA,
+                                                            pk);
+    if (prec == PHF_PREC_F32C)
+        panel_induced_kernel<true><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(m, pk, ne, xE, yE,
+                                                                             vx, vy);
+    else
+        panel_induced_kernel<false><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(m, pk, ne, xE, yE,
+                                                                              vx, vy);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -616,9 +737,8 @@ int phf_panel_induced_velocity(const double *gamma, int64_t m, const double *xE,
     PHF_TRY(upload(L, yE, ne, &dyE));
     PHF_ALLOC(dvx, L->arena, double, (size_t)ne);
     PHF_ALLOC(dvy, L->arena, double, (size_t)ne);
-    panel_induced_kernel<<<div_up(ne, 128), 128, 0, L->stream>>>(
-        m, d[0], ne, dxE, dyE, d[1], d[2], d[3], d[4], d[5], d[6], dvx, dvy);
-    PHF_HIP(hipGetLastError());
+    PHF_TRY(panel_induced_device(L, m, d[0], ne, dxE, dyE, d[1], d[2], d[3], d[4], d[5], d[6], dvx,
+                                 dvy, PHF_PREC_F64));
     PHF_TRY(download(L, dvx, ne, vx));
     PHF_TRY(download(L, dvy, ne, vy));
     PHF_HIP(hipStreamSynchronize(L->stream));

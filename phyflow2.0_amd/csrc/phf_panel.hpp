@@ -15,12 +15,15 @@ struct SolveStats {
 // In-place float32 LU with partial pivoting of the column-major n x n A32.
 int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info);
 
-// Float32 LU factors of B (rows == n) or of B^T B (rows > n), with the
-// float64 column-major copy of B for refinement residuals (arena-owned).
+// Float32 LU factors of B (rows == n) or of B^T B (rows > n) and the
+// float32 inverse formed from them, with the float64 column-major copy of B
+// for refinement residuals (arena-owned).
 struct DenseFactors {
     int64_t rows = 0, n = 0;
+    const double *Br = nullptr;  // B itself (row-major, caller-owned, outlives the factors)
     double *Bc = nullptr;
     float *LU = nullptr;
+    float *Xinv = nullptr;       // float32 (P L U)^-1, row-major
     int32_t *ipiv = nullptr, *info = nullptr;
     int singular_at = 0;
 };
@@ -35,7 +38,7 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
 int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
                          const double *yE, const double *xs, const double *ys, const double *xe,
                          const double *ye, const double *cosA, const double *sinA, double *vx,
-                         double *vy);
+                         double *vy, int prec = PHF_PREC_F64);
 int panel_influence_device(Lane *L, int64_t m, const double *xCP, const double *yCP,
                            const double *xs, const double *ys, const double *xe, const double *ye,
                            const double *cosA, const double *sinA, double *A);

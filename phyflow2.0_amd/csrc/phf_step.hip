@@ -149,7 +149,7 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
         tm.end(PHF_STAGE_PANEL);
     }
     PHF_TRY(panel_induced_device(L, m, P->s, n, X, Y, P->xs, P->ys, P->xe, P->ye, P->cosA,
-                                 P->sinA, px, py));
+                                 P->sinA, px, py, vp.prec));
     total_velocity<<<div_up(n, 256), 256, 0, L->stream>>>(vx, vy, px, py, n, vinf_x, vinf_y);
     PHF_HIP(hipGetLastError());
     PHF_HIP(hipMemcpyAsync(kx, vx, sizeof(double) * n, hipMemcpyDeviceToDevice, L->stream));

@@ -279,3 +279,41 @@ def test_population_control_loop(O):
     # a strict total threshold forces at least one division by 10
     _, _, w2, t2 = O.population_control(x, y, w, 1e-5, 1e-9)
     assert t2 < 1e-5 and np.sum(np.abs(w[np.abs(w) < t2])) < 1e-9
+
+
+def test_vorticity_closed_form_mass_and_curl(O):
+    """Gaussian vorticity: single-blob closed form, linearity, unit mass on a
+    fine lattice, and equality with the curl of the oracle's own velocity
+    field (a wrong constant, sign or core size fails one of these)."""
+    ks = 2 * 0.05 ** 2
+    r = np.array([0.0, 1e-3, 0.02, 0.05, 0.11, 0.3])
+    th = np.linspace(0.1, 6.0, r.size)
+    xt, yt = 0.2 + r * np.cos(th), -0.1 + r * np.sin(th)
+    w = O.ww2par_gauss([0.2], [-0.1], [0.7], xt, yt, ks)
+    ref = 0.7 * np.exp(-(r * r + 1e-12) / ks) / (math.pi * ks)
+    np.testing.assert_allclose(w, ref, rtol=1e-14, atol=0)
+    rng = np.random.default_rng(3)
+    xb, yb = rng.uniform(-0.3, 0.3, (2, 50))
+    g1, g2 = rng.standard_normal((2, 50))
+    a = O.ww2par_gauss(xb, yb, 2.0 * g1 - 3.0 * g2, xt, yt, ks)
+    b = 2.0 * O.ww2par_gauss(xb, yb, g1, xt, yt, ks) - 3.0 * O.ww2par_gauss(xb, yb, g2, xt, yt, ks)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * np.max(np.abs(a)))
+    # mass: a lattice of spacing h = sigma/4 over +-8 sigma integrates the
+    # Gaussian to ~exp(-pi^2 ks / h^2) accuracy
+    h = 0.05 / 4
+    g = (np.arange(-64, 65) * h)
+    X, Y = np.meshgrid(g, g)
+    mass = np.sum(O.ww2par_gauss(xb[:3], yb[:3], g1[:3], X.ravel() + 0.0, Y.ravel(), ks)) * h * h
+    # the EPS regularisation scales every blob by exp(-EPS/ks) (1 - 2e-10 here)
+    expect = np.sum(g1[:3]) * math.exp(-1e-12 / ks)
+    assert abs(mass - expect) <= 1e-13 * np.sum(np.abs(g1[:3]))
+    # curl: dv/dx - du/dy of the velocity field by central differences
+    d = 1e-5
+    px, py = np.array([0.07, -0.12, 0.31]), np.array([0.02, 0.15, -0.2])
+    u1, v1 = O.vv2par_gauss(xb, yb, g1, px + d, py, ks)
+    u2, v2 = O.vv2par_gauss(xb, yb, g1, px - d, py, ks)
+    u3, v3 = O.vv2par_gauss(xb, yb, g1, px, py + d, ks)
+    u4, v4 = O.vv2par_gauss(xb, yb, g1, px, py - d, ks)
+    curl = (v1 - v2) / (2 * d) - (u3 - u4) / (2 * d)
+    wv = O.ww2par_gauss(xb, yb, g1, px, py, ks)
+    np.testing.assert_allclose(curl, wv, rtol=1e-6, atol=1e-6 * np.max(np.abs(wv)))
