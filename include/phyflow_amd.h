@@ -142,9 +142,19 @@ int phf_tree_build(const double *x, const double *y, int64_t n, int32_t L,
  *   blob set (its size is data-dependent). */
 int phf_m4kernel_1d(const double *x, int64_t n, double c, double *w);
 
+#define PHF_REMESH_M4 0    /* Regrid with M4' (c: diffusion parameter) */
+#define PHF_REMESH_TUTTY 1 /* diffusion_tutty's weights */
+
 typedef struct phf_blobset_s phf_blobset;
 int phf_regrid_m4(const double *x, const double *y, const double *w, int64_t n, double sigma,
                   double overlap, double x0, double y0, double c, phf_blobset **out);
+/* diffusion_tutty(dt, nu, h, xBlob, yBlob, wBlob, sigma, xBounds, yBounds,
+ * overlap) of src/python/blobs/base/evolution/_evolution.py:238-445: the
+ * remesh above (h = sigma*overlap, :333; same cells, order and coordinates)
+ * with Tutty's diffusion weights, hNu = sqrt(dt*nu) (:366, :385-405). */
+int phf_diffusion_tutty(const double *x, const double *y, const double *w, int64_t n,
+                        double sigma, double overlap, double x0, double y0, double dt, double nu,
+                        phf_blobset **out);
 int64_t phf_blobset_size(const phf_blobset *b);
 /* Copies the blob set into caller arrays of phf_blobset_size() entries
  * (any pointer may be NULL). */
@@ -276,9 +286,11 @@ typedef struct {
     double dt;
     double vinf_x, vinf_y;   /* free stream */
     int32_t panel_update;    /* 0: solve panels at stage 1 ('constant'), 1: every stage */
-    int32_t remesh;          /* 1: M4' remesh after convection */
-    double remesh_sigma, remesh_overlap, remesh_x0, remesh_y0, remesh_c;
-    int32_t pc_enabled, pad; /* 1: population control after the remesh */
+    int32_t remesh;          /* 1: remesh after convection */
+    double remesh_sigma, remesh_overlap, remesh_x0, remesh_y0;
+    double remesh_c;         /* M4': the diffusion parameter c (Wee); Tutty: hNu = sqrt(nu dt) */
+    int32_t pc_enabled;      /* 1: population control after the remesh */
+    int32_t remesh_kernel;   /* PHF_REMESH_M4 or PHF_REMESH_TUTTY */
     double pc_w_threshold, pc_total_threshold; /* gThresholdLocal, gThresholdGlobal */
 } phf_step_params;
 

@@ -46,6 +46,8 @@ def lib():
                                     ctypes.POINTER(ctypes.POINTER(_dbl)), ctypes.POINTER(ctypes.POINTER(_i64)),
                                     ctypes.POINTER(ctypes.POINTER(_i64))]
         L.orc_regrid_m4.restype = _i64
+        L.orc_regrid_tutty.argtypes = L.orc_regrid_m4.argtypes
+        L.orc_regrid_tutty.restype = _i64
         L.orc_free.argtypes = [_ptr]
         L.orc_population_control.argtypes = [_i64, _ptr, _dbl, _dbl, _ptr, ctypes.POINTER(_dbl)]
         L.orc_population_control.restype = _i64
@@ -129,15 +131,25 @@ def m4kernel_1d(x, c=0.0):
 
 def regrid_m4(x, y, w, sigma, overlap, x0, y0, c=0.0):
     """_regridding.py:118-218 restated; returns (x, y, w, i, j)."""
+    return _regrid(lib().orc_regrid_m4, x, y, w, sigma, overlap, x0, y0, c)
+
+
+def regrid_tutty(x, y, w, sigma, overlap, x0, y0, hnu):
+    """diffusion_tutty, _evolution.py:325-445 restated (hnu = sqrt(nu dt));
+    returns (x, y, w, i, j)."""
+    return _regrid(lib().orc_regrid_tutty, x, y, w, sigma, overlap, x0, y0, hnu)
+
+
+def _regrid(fn, x, y, w, sigma, overlap, x0, y0, par):
     x, y, w = map(_f64, (x, y, w))
     px = ctypes.POINTER(_dbl)()
     py = ctypes.POINTER(_dbl)()
     pw = ctypes.POINTER(_dbl)()
     pi = ctypes.POINTER(_i64)()
     pj = ctypes.POINTER(_i64)()
-    m = lib().orc_regrid_m4(x.size, _p(x), _p(y), _p(w), float(sigma), float(overlap), float(x0),
-                            float(y0), float(c), ctypes.byref(px), ctypes.byref(py), ctypes.byref(pw),
-                            ctypes.byref(pi), ctypes.byref(pj))
+    m = fn(x.size, _p(x), _p(y), _p(w), float(sigma), float(overlap), float(x0), float(y0),
+           float(par), ctypes.byref(px), ctypes.byref(py), ctypes.byref(pw), ctypes.byref(pi),
+           ctypes.byref(pj))
     if m < 0:
         raise MemoryError("oracle regrid allocation failed")
     if m == 0:

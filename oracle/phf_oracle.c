@@ -204,10 +204,29 @@ static int rm_cmp(const void *a, const void *b)
 
 /* Returns the number of output cells, or -1 on allocation failure.
  * Outputs are malloc'ed and handed to the caller (free with orc_free). */
-int64_t orc_regrid_m4(int64_t n, const double *x, const double *y,
-                      const double *w, double sigma, double overlap,
-                      double x0, double y0, double c, double **xo,
-                      double **yo, double **wo, int64_t **io, int64_t **jo)
+/* Tutty's diffusion weights, src/python/blobs/base/evolution/_evolution.py:
+ * 385-405: for delta = (x - x_i)/h with x_i the node right of the base cell,
+ * a blend (1 - delta, delta) of two 3-point stencils centred on nodes i and
+ * i+1, each with mass 1, the particle's centre and variance 2 (hNu/h)^2.
+ * F[a] is the weight of node i - 1 + a.                                    */
+static void tutty_weights(double delta, double c2, double F[4])
+{
+    const double d1 = delta - 1.0;
+    const double f = (1.0 - 2.0 * c2) - delta * delta;
+    const double g = (1.0 - 2.0 * c2) - d1 * d1;
+    F[0] = (1.0 - delta) * (0.5 * ((1.0 - f) - delta));
+    F[1] = (1.0 - delta) * f + delta * (0.5 * ((1.0 - g) - d1));
+    F[2] = (1.0 - delta) * (0.5 * ((1.0 - f) + delta)) + delta * g;
+    F[3] = delta * (0.5 * ((1.0 - g) + d1));
+}
+
+/* kind 0: M4' with parameter c (Regrid); kind 1: Tutty with hnu = sqrt(nu dt)
+ * (diffusion_tutty, _evolution.py:325-445: same base cell, stencil, COO
+ * summation and output coordinates as Regrid, different weights).         */
+static int64_t regrid_core(int kind, int64_t n, const double *x, const double *y,
+                           const double *w, double sigma, double overlap,
+                           double x0, double y0, double c, double hnu, double **xo,
+                           double **yo, double **wo, int64_t **io, int64_t **jo)
 {
     *xo = *yo = *wo = NULL;
     *io = *jo = NULL;
@@ -215,21 +234,33 @@ int64_t orc_regrid_m4(int64_t n, const double *x, const double *y,
     const double h = sigma * overlap;
     const double h2 = 2.0 * h, hh = 0.5 * h;
     const double x0h = x0 / h, y0h = y0 / h;
+    const double r = hnu / h, c2 = r * r;
     rm_entry *E = (rm_entry *)malloc(sizeof(rm_entry) * 16 * (size_t)n);
     if (!E) return -1;
     for (int64_t b = 0; b < n; b++) {
         int64_t iL = (int64_t)ceil((((x[b] - x0) - h2) + hh) / h) - 1;
         int64_t jL = (int64_t)ceil((((y[b] - y0) - h2) + hh) / h) - 1;
         double xbh = x[b] / h, ybh = y[b] / h;
+        double F[4], G[4];
+        if (kind == 1) {
+            const double xc = ((double)(iL + 1) * h + hh) + x0;
+            const double yc = ((double)(jL + 1) * h + hh) + y0;
+            tutty_weights((x[b] - xc) / h, c2, F);
+            tutty_weights((y[b] - yc) / h, c2, G);
+        }
         for (int s = 0; s < 16; s++) {
             int64_t i = iL + (s % 4), j = jL + (s / 4);
-            double ux = (((double)i + 0.5) + x0h) - xbh;
-            double uy = (((double)j + 0.5) + y0h) - ybh;
             rm_entry *en = &E[(size_t)s * n + b];
             en->i = i;
             en->j = j;
             en->e = (int64_t)s * n + b;
-            en->v = (m4p(ux, c) * m4p(uy, c)) * w[b];
+            if (kind == 1) {
+                en->v = (F[s % 4] * G[s / 4]) * w[b];
+            } else {
+                double ux = (((double)i + 0.5) + x0h) - xbh;
+                double uy = (((double)j + 0.5) + y0h) - ybh;
+                en->v = (m4p(ux, c) * m4p(uy, c)) * w[b];
+            }
         }
     }
     size_t ne = 16 * (size_t)n;
@@ -264,6 +295,27 @@ int64_t orc_regrid_m4(int64_t n, const double *x, const double *y,
     free(E);
     *xo = X; *yo = Y; *wo = W; *io = I; *jo = J;
     return m;
+}
+
+int64_t orc_regrid_m4(int64_t n, const double *x, const double *y,
+                      const double *w, double sigma, double overlap,
+                      double x0, double y0, double c, double **xo,
+                      double **yo, double **wo, int64_t **io, int64_t **jo)
+{
+    return regrid_core(0, n, x, y, w, sigma, overlap, x0, y0, c, 0.0, xo, yo, wo, io, jo);
+}
+
+/* diffusion_tutty (_evolution.py:238-445) with h = sigma*overlap (:333) and
+ * hNu = sqrt(dt*nu) (:366).  Pinned by: conservation of sum w, sum w x,
+ * sum w y and growth of sum w (x^2 + y^2) by exactly 4 nu dt sum w (each
+ * axis gains variance 2 nu dt); hnu = 0 on a lattice point maps a blob to
+ * itself; the same cell set and order as Regrid.                          */
+int64_t orc_regrid_tutty(int64_t n, const double *x, const double *y,
+                         const double *w, double sigma, double overlap,
+                         double x0, double y0, double hnu, double **xo,
+                         double **yo, double **wo, int64_t **io, int64_t **jo)
+{
+    return regrid_core(1, n, x, y, w, sigma, overlap, x0, y0, 0.0, hnu, xo, yo, wo, io, jo);
 }
 
 void orc_free(void *p) { free(p); }

@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "libphyflow_amd.so")
 PHF_OK = 0
 PREC_F64 = 0
 PREC_F32C = 1
+REMESH_M4 = 0
+REMESH_TUTTY = 1
 
 STAGES = ("upload", "tree", "p2m", "m2m", "m2l", "l2l", "p2p", "download",
           "remesh", "panel", "total")
@@ -51,7 +53,7 @@ class StepParams(ctypes.Structure):
     _fields_ = [("dt", _dbl), ("vinf_x", _dbl), ("vinf_y", _dbl), ("panel_update", _i32),
                 ("remesh", _i32), ("remesh_sigma", _dbl), ("remesh_overlap", _dbl),
                 ("remesh_x0", _dbl), ("remesh_y0", _dbl), ("remesh_c", _dbl),
-                ("pc_enabled", _i32), ("pad", _i32), ("pc_w_threshold", _dbl),
+                ("pc_enabled", _i32), ("remesh_kernel", _i32), ("pc_w_threshold", _dbl),
                 ("pc_total_threshold", _dbl)]
 
 
@@ -135,6 +137,8 @@ _SIGS = {
     "phf_m4kernel_1d": ([_ptr, _i64, _dbl, _ptr], ctypes.c_int),
     "phf_regrid_m4": ([_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, ctypes.POINTER(_ptr)],
                       ctypes.c_int),
+    "phf_diffusion_tutty": ([_ptr, _ptr, _ptr, _i64, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl,
+                             ctypes.POINTER(_ptr)], ctypes.c_int),
     "phf_blobset_size": ([_ptr], _i64),
     "phf_population_control": ([_ptr, _ptr, _ptr, _i64, _dbl, _dbl, ctypes.POINTER(_ptr)],
                                ctypes.c_int),

@@ -269,6 +269,91 @@ __global__ __launch_bounds__(64) void m2l_kernel(
     }
 }
 
+// One lane per target box, 64 boxes of one parity class per wave (grid.y =
+// parity), so the interaction list, u^k and the binomial rows are uniform
+// across the wave and read as scalar operands: per link each lane forms
+// A'_k = u^k A_k and B_m += (-1)^m u^(m+1) sum_k C(k+m, k) A'_k in
+// registers (the factorisation and summation order of m2l_kernel), with no
+// LDS traffic or barriers.  Used when 6 P1 values of T fit in registers.
+template <class T, int P1>
+__global__ __launch_bounds__(64) void m2l_box_kernel(
+    int l, int p1, const typename C2<T>::type *__restrict__ Ml, typename C2<T>::type *__restrict__ Bl,
+    const int32_t *__restrict__ start_s, const int32_t *__restrict__ start_t, int shift,
+    const int2 *__restrict__ il_tau, const int32_t *__restrict__ il_off /* 5 */,
+    const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [p1][p1] */,
+    T inv_r)
+{
+    using V = typename C2<T>::type;
+    const int lane = threadIdx.x;
+    const int par = blockIdx.y;
+    const int n = 1 << l, half = n >> 1;
+    const int64_t per = ((int64_t)n * n) / 4;  // boxes per parity class
+    const int64_t loc = (int64_t)blockIdx.x * 64 + lane;
+    const bool in = loc < per;
+    int bx = 0, by = 0;
+    uint32_t box = 0;
+    bool active = false;
+    if (in) {
+        bx = 2 * (int)(loc % half) + (par & 1);
+        by = 2 * (int)(loc / half) + (par >> 1);
+        box = morton2((uint32_t)bx, (uint32_t)by);
+        active = box_count(start_t, box, shift) > 0;
+    }
+    T br[P1], bi[P1];
+#pragma unroll
+    for (int m = 0; m < P1; ++m) br[m] = bi[m] = (T)0;
+    const int W2 = 2 * R + 1;
+    const int t0 = il_off[par], tn = il_off[par + 1] - t0;
+    for (int t = 0; t < tn; ++t) {
+        const int2 o = il_tau[t0 + t];
+        const int sx = bx + o.x, sy = by + o.y;
+        bool ok = active && sx >= 0 && sy >= 0 && sx < n && sy < n;
+        uint32_t sb = 0;
+        if (ok) {
+            sb = morton2((uint32_t)sx, (uint32_t)sy);
+            ok = box_count(start_s, sb, shift) > 0;
+        }
+        if (!ok) continue;
+        const V *Uo = U + (int64_t)((o.y + R) * W2 + (o.x + R)) * (p1 + 1);
+        const V *A = Ml + (int64_t)sb * p1;
+        T apr[P1], api[P1];
+#pragma unroll
+        for (int k = 0; k < P1; ++k) {
+            apr[k] = api[k] = (T)0;
+            if (k < p1) {
+                const V a = A[k], u = Uo[k];
+                apr[k] = a.x * u.x - a.y * u.y;
+                api[k] = a.x * u.y + a.y * u.x;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < P1; ++m) {
+            if (m < p1) {
+                T sr = 0, si = 0;
+#pragma unroll
+                for (int k = 0; k < P1; ++k) {
+                    if (k < p1) {
+                        const T c = binom[m * p1 + k];
+                        sr += c * apr[k];
+                        si += c * api[k];
+                    }
+                }
+                const V um = Uo[m + 1];
+                const T sgn = (m & 1) ? (T)-1 : (T)1;
+                br[m] += sgn * (um.x * sr - um.y * si);
+                bi[m] += sgn * (um.x * si + um.y * sr);
+            }
+        }
+    }
+    if (in) {
+        V *Bb = Bl + (int64_t)box * p1;
+#pragma unroll
+        for (int m = 0; m < P1; ++m)
+            if (m < p1)  // boxes without targets get zeros (never read stale)
+                Bb[m] = active ? V{br[m] * inv_r, bi[m] * inv_r} : V{(T)0, (T)0};
+    }
+}
+
 // ------------------------------------------------------ P2P + L2P (leaf) --
 // float32 sources are stored per leaf in ranges padded to a multiple of 4
 // with zero-weight entries (exact zeros in every sum they enter), so the
@@ -346,7 +431,15 @@ __global__ __launch_bounds__(256) void p2p_l2p_f32s(
         const int32_t i = rc + 16 * wv + t;
         const bool valid = i < t1;
         const bool wave_on = rc + 16 * wv < t1;
-        const float2 tp = valid ? tgt[i] : make_float2(0.f, 0.f);
+        float2 tp = make_float2(0.f, 0.f);
+        if (valid) {
+            if (tgt) {
+                tp = tgt[i];
+            } else {  // targets are the sources: read them from the padded array
+                const float4 q = src[sp[b] + (i - t0)];
+                tp = make_float2(q.x, q.y);
+            }
+        }
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
         p2p_staged<KIND>(src, sp, st, seg, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
         if (!wave_on) continue;
@@ -674,12 +767,21 @@ static int launch_m2l(Lane *L, int l, int p1, const void *M, void *B, const int3
                       const void *U, int R, const void *binom, double inv_r)
 {
     using V = typename C2<T>::type;
-    const int G = 64 / p1;
     const int64_t nbox = int64_t(1) << (2 * l);
-    const unsigned grid = (unsigned)((nbox + G - 1) / G);
-    m2l_kernel<T, P1><<<grid, 64, 0, L->stream>>>(
-        l, p1, (const V *)M, (V *)B, ss, st, shift, il, off, il_max, (const V *)U, R,
-        (const T *)binom, (T)inv_r);
+    // lane-per-box form while its 6 P1 accumulators fit in registers
+    constexpr bool box_form = sizeof(T) == 4 ? P1 <= 25 : P1 <= 17;
+    if constexpr (box_form) {
+        const dim3 grid((unsigned)((nbox / 4 + 63) / 64), 4);
+        m2l_box_kernel<T, P1><<<grid, 64, 0, L->stream>>>(l, p1, (const V *)M, (V *)B, ss, st,
+                                                          shift, il, off, (const V *)U, R,
+                                                          (const T *)binom, (T)inv_r);
+    } else {
+        const int G = 64 / p1;
+        const unsigned grid = (unsigned)((nbox + G - 1) / G);
+        m2l_kernel<T, P1><<<grid, 64, 0, L->stream>>>(
+            l, p1, (const V *)M, (V *)B, ss, st, shift, il, off, il_max, (const V *)U, R,
+            (const T *)binom, (T)inv_r);
+    }
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -732,10 +834,12 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
             gather_src_f32p<<<div_up(a.nb, 256), 256, 0, L->stream>>>(
                 a.xb, a.yb, a.wb, S.keys, S.perm, S.start, sp, a.nb, dom.x0, dom.y0, dom.s, s4);
         PHF_TRY(zero_source_pads(L, S.start, sp, nleaf64, s4));
-        float2 *t2;
-        PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
-        gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(
-            a.xt, a.yt, Tg.keys, Tg.perm, a.nt, dom.x0, dom.y0, dom.s, t2);
+        float2 *t2 = nullptr;  // targets == sources: P2P reads the padded sources
+        if (!a.same) {
+            PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
+            gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(
+                a.xt, a.yt, Tg.keys, Tg.perm, a.nt, dom.x0, dom.y0, dom.s, t2);
+        }
         src = s4;
         tgt = t2;
     } else {

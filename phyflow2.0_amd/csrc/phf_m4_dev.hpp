@@ -50,4 +50,21 @@ __device__ __forceinline__ int64_t base_index(double x, double x0, double h)
     return (int64_t)ceil((((x - x0) - h2) + hh) / h) - 1;
 }
 
+// Tutty's diffusion weight of node i - 1 + a (a = 0..3) for a blob at
+// delta = (x - x_i)/h from the node right of its base cell, c2 = (hNu/h)^2
+// (_evolution.py:385-405, same operation order).
+__host__ __device__ __forceinline__ double tutty_weight(int a, double delta, double c2)
+{
+#pragma clang fp contract(off)
+    const double d1 = delta - 1.0;
+    const double f = (1.0 - 2.0 * c2) - delta * delta;
+    const double g = (1.0 - 2.0 * c2) - d1 * d1;
+    switch (a) {
+    case 0: return (1.0 - delta) * (0.5 * ((1.0 - f) - delta));
+    case 1: return (1.0 - delta) * f + delta * (0.5 * ((1.0 - g) - d1));
+    case 2: return (1.0 - delta) * (0.5 * ((1.0 - f) + delta)) + delta * g;
+    default: return delta * (0.5 * ((1.0 - g) + d1));
+    }
+}
+
 }  // namespace phf

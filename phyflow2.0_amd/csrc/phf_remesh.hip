@@ -179,8 +179,52 @@ __global__ void gather_sum(const int32_t *__restrict__ cstart, const double3 *__
     if (jo) jo[o] = J;
 }
 
+// Tutty's diffusion-remesh (_evolution.py:325-445): same touched cells and
+// summation order as gather_sum, weights tutty_weight(a) x tutty_weight(b)
+// for slot (a, b) of the contributing blob; sb holds unscaled (x, y, w).
+__global__ void gather_sum_tutty(const int32_t *__restrict__ cstart, const double3 *__restrict__ sb,
+                                 const uint32_t *__restrict__ flag, const uint32_t *__restrict__ slot,
+                                 long long bi, long long bj, long long ni, long long nj,
+                                 long long imin, long long jmin, double h, double x0, double y0,
+                                 double c2, double *__restrict__ xo, double *__restrict__ yo,
+                                 double *__restrict__ wo, int64_t *__restrict__ io,
+                                 int64_t *__restrict__ jo)
+{
+    const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cidx >= ni * nj || !flag[cidx]) return;
+    const long long i = cidx / nj, j = cidx - (cidx / nj) * nj;
+    const long long I = i + imin, J = j + jmin;
+    const double hh = 0.5 * h;
+    double acc = 0.0;
+    bool first = true;
+    for (int s = 0; s < 16; ++s) {
+        const int a = s & 3, b = s >> 2;
+        const long long ii = i - a, jj = j - b;
+        if (ii < 0 || jj < 0 || ii >= bi || jj >= bj) continue;
+        const int64_t k = ii * bj + jj;
+        if (cstart[k + 1] <= cstart[k]) continue;
+        // node right of the base cell (ii + imin, jj + jmin)
+        const double xc = ((double)(ii + imin + 1) * h + hh) + x0;
+        const double yc = ((double)(jj + jmin + 1) * h + hh) + y0;
+        for (int32_t q = cstart[k]; q < cstart[k + 1]; ++q) {
+            const double3 p = sb[q];
+            const double v = (tutty_weight(a, (p.x - xc) / h, c2) *
+                              tutty_weight(b, (p.y - yc) / h, c2)) * p.z;
+            acc = first ? v : acc + v;
+            first = false;
+        }
+    }
+    const uint32_t o = slot[cidx];
+    wo[o] = acc;
+    xo[o] = ((double)I * h + hh) + x0;
+    yo[o] = ((double)J * h + hh) + y0;
+    if (io) io[o] = I;
+    if (jo) jo[o] = J;
+}
+
 int remesh_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
-                  double sigma, double overlap, double x0, double y0, double c, RemeshOut *out)
+                  double sigma, double overlap, double x0, double y0, double c, RemeshOut *out,
+                  int kind)
 {
     out->m = 0;
     if (n == 0) return PHF_OK;
@@ -222,7 +266,8 @@ int remesh_device(Lane *L, const double *x, const double *y, const double *w, in
     PHF_TRY(leaf_starts_device(L, ks, n, bi * bj, cstart));
     double3 *sb;
     PHF_ALLOC(sb, L->arena, double3, (size_t)n);
-    gather_scaled<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, w, vs, n, h, sb);
+    // M4' works on x/h; Tutty on x itself
+    gather_scaled<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, w, vs, n, kind == 1 ? 1.0 : h, sb);
     // 3. touched cells and their output slots
     const int64_t ncell = ni * nj;
     uint32_t *flag, *slot;
@@ -254,9 +299,16 @@ int remesh_device(Lane *L, const double *x, const double *y, const double *w, in
         PHF_ALLOC(io, L->arena, int64_t, (size_t)std::max<int64_t>(m, 1));
         PHF_ALLOC(jo, L->arena, int64_t, (size_t)std::max<int64_t>(m, 1));
     }
-    gather_sum<<<div_up(ncell, 256), 256, 0, L->stream>>>(
-        cstart, sb, flag, slot, bi, bj, ni, nj, hb.x, hb.z, h, x0, y0, x0 / h, y0 / h, c, c * c,
-        xo, yo, wo, io, jo);
+    if (kind == 1) {
+        const double r = c / h;  // c carries hNu = sqrt(nu dt)
+        gather_sum_tutty<<<div_up(ncell, 256), 256, 0, L->stream>>>(
+            cstart, sb, flag, slot, bi, bj, ni, nj, hb.x, hb.z, h, x0, y0, r * r, xo, yo, wo, io,
+            jo);
+    } else {
+        gather_sum<<<div_up(ncell, 256), 256, 0, L->stream>>>(
+            cstart, sb, flag, slot, bi, bj, ni, nj, hb.x, hb.z, h, x0, y0, x0 / h, y0 / h, c,
+            c * c, xo, yo, wo, io, jo);
+    }
     PHF_HIP(hipGetLastError());
     out->m = m;
     out->x = xo;
@@ -484,10 +536,31 @@ int phf_m4kernel_1d(const double *x, int64_t n, double c, double *w)
     return PHF_OK;
 }
 
+static int regrid_entry(int kind, const double *x, const double *y, const double *w, int64_t n,
+                        double sigma, double overlap, double x0, double y0, double c,
+                        phf_blobset **out);
+
 int phf_regrid_m4(const double *x, const double *y, const double *w, int64_t n, double sigma,
                   double overlap, double x0, double y0, double c, phf_blobset **out)
 {
     clear_error();
+    return regrid_entry(0, x, y, w, n, sigma, overlap, x0, y0, c, out);
+}
+
+int phf_diffusion_tutty(const double *x, const double *y, const double *w, int64_t n,
+                        double sigma, double overlap, double x0, double y0, double dt, double nu,
+                        phf_blobset **out)
+{
+    clear_error();
+    PHF_REQUIRE(std::isfinite(dt) && std::isfinite(nu) && dt * nu >= 0.0,
+                "dt*nu must be finite and >= 0");
+    return regrid_entry(1, x, y, w, n, sigma, overlap, x0, y0, std::sqrt(dt * nu), out);
+}
+
+static int regrid_entry(int kind, const double *x, const double *y, const double *w, int64_t n,
+                        double sigma, double overlap, double x0, double y0, double c,
+                        phf_blobset **out)
+{
     PHF_REQUIRE(out != nullptr, "out is NULL");
     *out = nullptr;
     PHF_REQUIRE(n >= 0 && n < (int64_t(1) << 31), "n out of range (%lld)", (long long)n);
@@ -522,7 +595,7 @@ int phf_regrid_m4(const double *x, const double *y, const double *w, int64_t n, 
         tm.begin(PHF_STAGE_REMESH);
         RemeshOut ro;
         ro.want_indices = true;
-        PHF_TRY(remesh_device(L, dx, dy, dw, n, sigma, overlap, x0, y0, c, &ro));
+        PHF_TRY(remesh_device(L, dx, dy, dw, n, sigma, overlap, x0, y0, c, &ro, kind));
         tm.end(PHF_STAGE_REMESH);
         res->m = ro.m;
         res->x.resize(ro.m);

@@ -17,8 +17,11 @@ struct RemeshOut {
     int64_t cells = 0;  // cells scanned (touched-range size)
 };
 
+// kind 0: M4' with parameter c (Regrid); kind 1: Tutty's diffusion weights
+// with c = hNu = sqrt(nu dt) (diffusion_tutty).
 int remesh_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
-                  double sigma, double overlap, double x0, double y0, double c, RemeshOut *out);
+                  double sigma, double overlap, double x0, double y0, double c, RemeshOut *out,
+                  int kind = 0);
 int m4kernel_device(Lane *L, const double *x, int64_t n, double c, double *w);
 // {sum w, sum w x, sum w y, sum w (x^2+y^2)} in float64 into *d_out (device)
 int moments_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,

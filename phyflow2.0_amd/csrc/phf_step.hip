@@ -319,8 +319,11 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
         PHF_REQUIRE(std::isfinite(sp->remesh_sigma) && sp->remesh_sigma > 0 &&
                         std::isfinite(sp->remesh_overlap) && sp->remesh_overlap > 0 &&
                         std::isfinite(sp->remesh_x0) && std::isfinite(sp->remesh_y0) &&
-                        std::isfinite(sp->remesh_c),
-                    "remesh parameters must be finite (sigma, overlap > 0)");
+                        std::isfinite(sp->remesh_c) &&
+                        (sp->remesh_kernel == PHF_REMESH_M4 ||
+                         (sp->remesh_kernel == PHF_REMESH_TUTTY && sp->remesh_c >= 0)),
+                    "remesh parameters must be finite (sigma, overlap > 0; a known kernel; "
+                    "hNu >= 0 for Tutty)");
     System *s = reinterpret_cast<System *>(h);
     std::lock_guard<std::mutex> lk(s->mu);
     std::unique_lock<std::mutex> lkp;
@@ -362,7 +365,8 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
         PHF_TRY(moments_device(L, s->x, s->y, s->w, n, dm));
         RemeshOut ro;
         PHF_TRY(remesh_device(L, s->x, s->y, s->w, n, sp->remesh_sigma, sp->remesh_overlap,
-                              sp->remesh_x0, sp->remesh_y0, sp->remesh_c, &ro));
+                              sp->remesh_x0, sp->remesh_y0, sp->remesh_c, &ro,
+                              sp->remesh_kernel));
         PHF_HIP(hipMemcpyAsync(&hm[0], dm, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
         PHF_HIP(hipStreamSynchronize(L->stream));
         PHF_TRY(sys_reserve(s, ro.m));

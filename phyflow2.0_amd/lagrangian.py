@@ -80,7 +80,9 @@ class PanelBody:
 
 def step_rk2(system, params, dt, vInf=(0.0, 0.0), panels=None, remesh=None, panel_update="constant"):
     """Advance a resident BlobSystem by one Heun RK2 step (+ optional M4'
-    remesh; remesh = dict(sigma=, overlap=, x0=-1.5h, y0=-1.5h, c=0)).
+    remesh; remesh = dict(sigma=, overlap=, x0=-1.5h, y0=-1.5h, c=0) for M4' with the
+    Wee parameter c, or hnu=sqrt(nu dt) instead of c for Tutty's diffusion
+    weights; population_control=(gThresholdLocal, gThresholdGlobal) after it).
     Returns the step info (invariants before/after the remesh, timings)."""
     sp = StepParams()
     sp.dt = float(dt)
@@ -93,7 +95,11 @@ def step_rk2(system, params, dt, vInf=(0.0, 0.0), panels=None, remesh=None, pane
         sp.remesh_overlap = float(remesh.get("overlap", 1.0))
         sp.remesh_x0 = float(remesh.get("x0", -1.5 * h))  # Blobs.redistribute bounds
         sp.remesh_y0 = float(remesh.get("y0", -1.5 * h))
-        sp.remesh_c = float(remesh.get("c", 0.0))
+        if "hnu" in remesh:  # Tutty's diffusion weights, hNu = sqrt(nu dt)
+            sp.remesh_kernel = 1
+            sp.remesh_c = float(remesh["hnu"])
+        else:  # M4' with the Wee parameter c
+            sp.remesh_c = float(remesh.get("c", 0.0))
         if remesh.get("population_control"):
             # Blobs defaults gThresholdLocal = gThresholdGlobal = 1e-8 (__blobsClass.py:105-106)
             local, glob = remesh["population_control"]

@@ -28,7 +28,11 @@ def step_params(dt, vInf=(0.0, 0.0), remesh=None):
         sp.remesh_overlap = float(remesh.get("overlap", 1.0))
         sp.remesh_x0 = float(remesh.get("x0", -1.5 * h))
         sp.remesh_y0 = float(remesh.get("y0", -1.5 * h))
-        sp.remesh_c = float(remesh.get("c", 0.0))
+        if "hnu" in remesh:  # Tutty's diffusion weights, hNu = sqrt(nu dt)
+            sp.remesh_kernel = 1
+            sp.remesh_c = float(remesh["hnu"])
+        else:  # M4' with the Wee parameter c
+            sp.remesh_c = float(remesh.get("c", 0.0))
         if remesh.get("population_control"):
             # Blobs defaults gThresholdLocal = gThresholdGlobal = 1e-8 (__blobsClass.py:105-106)
             local, glob = remesh["population_control"]

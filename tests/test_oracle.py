@@ -317,3 +317,33 @@ def test_vorticity_closed_form_mass_and_curl(O):
     curl = (v1 - v2) / (2 * d) - (u3 - u4) / (2 * d)
     wv = O.ww2par_gauss(xb, yb, g1, px, py, ks)
     np.testing.assert_allclose(curl, wv, rtol=1e-6, atol=1e-6 * np.max(np.abs(wv)))
+
+
+def test_tutty_diffusion_moments(O):
+    """Tutty's diffusion-remesh: sum w, sum w x, sum w y conserved; sum w r^2
+    grows by exactly 4 nu dt sum w; with nu dt = 0 a lattice blob maps to
+    itself; the cell set and order equal Regrid's (same stencil)."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    x, y = rng.uniform(-0.5, 0.5, (2, n))
+    w = rng.standard_normal(n)
+    sigma, overlap = 0.02, 0.8
+    h = sigma * overlap
+    x0 = y0 = -1.5 * h
+    for hnu in (0.0, 0.3 * h, 0.55 * h):
+        xo, yo, wo, io, jo = O.regrid_tutty(x, y, w, sigma, overlap, x0, y0, hnu)
+        s0 = np.sum(w)
+        assert abs(np.sum(wo) - s0) <= 1e-12 * np.sum(np.abs(w))
+        assert abs(np.sum(wo * xo) - np.sum(w * x)) <= 1e-12 * np.sum(np.abs(w))
+        assert abs(np.sum(wo * yo) - np.sum(w * y)) <= 1e-12 * np.sum(np.abs(w))
+        m2 = np.sum(wo * (xo * xo + yo * yo)) - np.sum(w * (x * x + y * y))
+        assert abs(m2 - 4 * hnu * hnu * s0) <= 1e-12 * np.sum(np.abs(w)), (hnu, m2)
+        r = O.regrid_m4(x, y, w, sigma, overlap, x0, y0)
+        assert np.array_equal(io, r[3]) and np.array_equal(jo, r[4])
+    # a blob on a lattice node without diffusion stays there (weight 1 there)
+    xs = np.array([x0 + 7.5 * h])
+    ys = np.array([y0 + 3.5 * h])
+    xo, yo, wo, io, jo = O.regrid_tutty(xs, ys, [0.7], sigma, overlap, x0, y0, 0.0)
+    k = np.argmax(np.abs(wo))
+    assert io[k] == 7 and jo[k] == 3 and abs(wo[k] - 0.7) <= 1e-15
+    assert np.max(np.abs(np.delete(wo, k))) <= 1e-15

@@ -92,3 +92,45 @@ def test_population_control_matches_oracle(O):
         ref = O.population_control(x, y, w, thr, tot)
         for a, b in zip(got, ref[:3]):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,seed", [(1, 5), (5000, 6), (200000, 7)])
+def test_diffusion_tutty_matches_oracle(n, seed, O):
+    """Tutty's diffusion-remesh: cells, order, indices and coordinates
+    bit-exact; weights <= 1e-15 of max|w| (same float64 operation order)."""
+    from phyflow_amd.blobs.base.evolution import diffusion_tutty, diffusion_wee
+    rng = np.random.default_rng(seed)
+    x, y = rng.uniform(-0.4, 0.6, (2, n))
+    w = rng.standard_normal(n)
+    sigma, overlap, dt, nu = 0.01, 0.8, 0.05, 1e-4
+    h = sigma * overlap
+    x0 = y0 = -1.5 * h
+    got = diffusion_tutty(dt, nu, h, x, y, w, sigma, [x0, 0], [y0, 0], overlap,
+                          return_indices=True)
+    ref = O.regrid_tutty(x, y, w, sigma, overlap, x0, y0, np.sqrt(dt * nu))
+    for a, b in zip(got[3:], ref[3:]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.max(np.abs(got[2] - ref[2])) <= 1e-15 * max(np.max(np.abs(ref[2])), 1e-300)
+    # Wee: Regrid with c = sqrt(nu dt)/h
+    gw = diffusion_wee(dt, nu, h, x, y, w, sigma, [x0, 0], [y0, 0], overlap)
+    rw = O.regrid_m4(x, y, w, sigma, overlap, x0, y0, np.sqrt(nu * dt) / h)
+    assert np.array_equal(gw[0], rw[0]) and np.array_equal(gw[1], rw[1])
+    assert np.max(np.abs(gw[2] - rw[2])) <= 1e-15 * max(np.max(np.abs(rw[2])), 1e-300)
+
+
+def test_step_with_tutty_remesh_moments():
+    """A step with Tutty's remesh: circulation and impulse unchanged, sum
+    w r^2 up by exactly 4 nu dt sum w (float64 invariants of the step)."""
+    from phyflow_amd.lagrangian import step_rk2
+    from phyflow_amd.system import BlobSystem, vel_params
+    s = W.uniform_system(20000, seed=12)
+    hnu = 0.4 * s["sigma"] * 0.8
+    with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
+        info = step_rk2(sysm, vel_params(s["sigma"], prec=1), 1e-4,
+                        remesh=dict(sigma=s["sigma"], overlap=0.8, hnu=hnu))
+    scale = np.sum(np.abs(s["w"]))
+    assert abs(info["circ_after"] - info["circ_before"]) <= 1e-13 * scale
+    assert abs(info["impulse_x_after"] - info["impulse_x_before"]) <= 1e-13 * scale
+    d2 = info["m2_after"] - info["m2_before"]
+    assert abs(d2 - 4 * hnu * hnu * info["circ_before"]) <= 1e-12 * scale
