@@ -395,25 +395,30 @@ __global__ __launch_bounds__(64) void batch_m2l(
 }
 
 // ------------------------------------------------------ P2P + L2P ---------
-// Near set of a leaf of system s: the offsets of the (2R+1)^2 window that
-// pass the near predicate; offsets whose box gap g gives
+// Near set of a leaf of system s: the offsets of the window |d| <= R that
+// pass the near predicate, enumerated from a table of the window |d| <= kRMax
+// sorted by box gap, so that the offsets whose pairs may lie inside the
+// smoothing core come first; offsets whose box gap g gives
 // shape g^2 log2(e) >= 26 (2% margin) skip the exponential (exact, see
 // pair_f32_far).
 struct BatchNear {
-    int R, side, ws, bx, by, n;
+    const int2 *win;
+    int R, ws, bx, by, n;
     double sd, c2, shape_log2;
     int64_t leafbase;
     float s_f;
     __device__ __forceinline__ bool operator()(int q, uint32_t &sb, float &ox, float &oy,
                                                int &core) const
     {
-        const int dx = q % side - R, dy = q / side - R;
+        const int2 o = win[q];
+        const int dx = o.x, dy = o.y;
         ox = (float)dx * s_f;
         oy = (float)dy * s_f;
         const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy;
         const double gx = (double)(ax > 1 ? ax - 1 : 0) * sd * 0.99;
         const double gy = (double)(ay > 1 ? ay - 1 : 0) * sd * 0.99;
         core = !(shape_log2 * (gx * gx + gy * gy) >= 26.0);
+        if (ax > R || ay > R) return false;
         const int sx = bx + dx, sy = by + dy;
         if (sx < 0 || sy < 0 || sx >= n || sy >= n) return false;
         if (!near_pred(dx, dy, ws, sd, c2)) return false;
@@ -433,10 +438,9 @@ __global__ __launch_bounds__(256) void batch_p2p_f32s(
     const float4 *__restrict__ src, const int32_t *__restrict__ start,
     const int32_t *__restrict__ sp, const int32_t *__restrict__ perm, int ws,
     const float2 *__restrict__ Bc, int p1, double *__restrict__ vx, double *__restrict__ vy,
-    int32_t *__restrict__ flag, int64_t nleaf, uint32_t nl8)
+    int32_t *__restrict__ flag, int64_t nleaf, uint32_t nl8, const int2 *__restrict__ win)
 {
     __shared__ float4 st[P2P_STAGE];
-    __shared__ StageSeg seg[64];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, t = lane & 15;
     const int64_t gl = (int64_t)(blockIdx.x & 7u) * nl8 + (blockIdx.x >> 3);
@@ -458,9 +462,9 @@ __global__ __launch_bounds__(256) void batch_p2p_f32s(
     pp.nlog2e_ks_f = sy.nlog2e_ks_f;
     pp.c2_f = sy.c2_f;
     pp.scale_f = sy.scale_f;
-    const BatchNear nf{R, 2 * R + 1, ws, (int)compact_bits(b), (int)compact_bits(b >> 1),
-                       1 << sy.L, sd, sy.c2, 1.4426950408889634074 * sy.inv_ks, sy.leafbase, s_f};
-    const int nnear = (2 * R + 1) * (2 * R + 1);
+    const BatchNear nf{win, R, ws, (int)compact_bits(b), (int)compact_bits(b >> 1), 1 << sy.L,
+                       sd, sy.c2, 1.4426950408889634074 * sy.inv_ks, sy.leafbase, s_f};
+    const int nnear = (2 * kRMax + 1) * (2 * kRMax + 1);
     const float2 *Bb = (sy.L >= 2 && Bc) ? Bc + (lvl[sy.L * (S + 1) + s] + b) * p1 : nullptr;
     const int32_t ts = sp[gl];
     for (int32_t rc = t0; rc < t1; rc += 64) {
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(256) void batch_p2p_f32s(
         const float4 tq = valid ? src[ts + (i - t0)] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float2 tp = make_float2(tq.x, tq.y);
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        p2p_staged<KIND>(src, sp, st, seg, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
+        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
         if (!wave_on) continue;
         p2p_merge_groups(ha, la, hb, lb);
         if (g == 0 && valid) {
@@ -753,6 +757,20 @@ int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
     PHF_HIP(hipMemcpy(Bf->l2l, h_l2l.data(), h_l2l.size(), hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->bin, h_bin.data(), h_bin.size(), hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->U, h_U.data(), h_U.size(), hipMemcpyHostToDevice));
+    // near-window offsets |d| <= kRMax sorted by box gap (then raster order)
+    {
+        std::vector<int2> win;
+        for (int dy = -kRMax; dy <= kRMax; ++dy)
+            for (int dx = -kRMax; dx <= kRMax; ++dx) win.push_back(make_int2(dx, dy));
+        auto gap2 = [](const int2 &o) {
+            const int ax = std::max(std::abs(o.x) - 1, 0), ay = std::max(std::abs(o.y) - 1, 0);
+            return ax * ax + ay * ay;
+        };
+        std::stable_sort(win.begin(), win.end(),
+                         [&](const int2 &a, const int2 &b) { return gap2(a) < gap2(b); });
+        PHF_ALLOC(Bf->win, L->arena, int2, win.size());
+        PHF_HIP(hipMemcpy(Bf->win, win.data(), sizeof(int2) * win.size(), hipMemcpyHostToDevice));
+    }
     // radix-sort scratch (fixed for the plan)
     const int64_t rw = radix_ws_size(P.N);
     PHF_ALLOC(Bf->hist, L->arena, uint32_t, (size_t)rw);
@@ -828,11 +846,11 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
             if (P.nocut_all)
                 batch_p2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, L->stream>>>(
                     Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, sp, vs,
-                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8);
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8, Bf.win);
             else
                 batch_p2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, L->stream>>>(
                     Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, sp, vs,
-                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8);
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8, Bf.win);
         } else {
             batch_p2p_f64<PAIR_OSEEN><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
                 Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const double4 *)Bf.src, Bf.start, vs, P.ws,

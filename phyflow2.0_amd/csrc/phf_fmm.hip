@@ -271,21 +271,24 @@ __global__ __launch_bounds__(64) void m2l_kernel(
 
 // One lane per target box, 64 boxes of one parity class per wave (grid.y =
 // parity), so the interaction list, u^k and the binomial rows are uniform
-// across the wave and read as scalar operands: per link each lane forms
-// A'_k = u^k A_k and B_m += (-1)^m u^(m+1) sum_k C(k+m, k) A'_k in
-// registers (the factorisation and summation order of m2l_kernel), with no
-// LDS traffic or barriers.  Used when 6 P1 values of T fit in registers.
-template <class T, int P1>
+// across the wave and come in as scalar operands.  Each wave produces the
+// coefficients m in [MW z, MW z + MW) (grid.z): per link it forms
+// A'_k = u^k A_k (all k) and B_m += (-1)^m u^(m+1) sum_k C(k+m, k) A'_k in
+// registers, the factorisation and summation order of m2l_kernel, with no
+// LDS traffic or barriers; splitting m over waves gives the finest level
+// enough waves to hide the load latency.  Requires p1 == P1.
+template <class T, int P1, int MW>
 __global__ __launch_bounds__(64) void m2l_box_kernel(
-    int l, int p1, const typename C2<T>::type *__restrict__ Ml, typename C2<T>::type *__restrict__ Bl,
+    int l, const typename C2<T>::type *__restrict__ Ml, typename C2<T>::type *__restrict__ Bl,
     const int32_t *__restrict__ start_s, const int32_t *__restrict__ start_t, int shift,
     const int2 *__restrict__ il_tau, const int32_t *__restrict__ il_off /* 5 */,
-    const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [p1][p1] */,
+    const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [P1][P1] */,
     T inv_r)
 {
     using V = typename C2<T>::type;
     const int lane = threadIdx.x;
     const int par = blockIdx.y;
+    const int m0 = blockIdx.z * MW;
     const int n = 1 << l, half = n >> 1;
     const int64_t per = ((int64_t)n * n) / 4;  // boxes per parity class
     const int64_t loc = (int64_t)blockIdx.x * 64 + lane;
@@ -299,9 +302,9 @@ __global__ __launch_bounds__(64) void m2l_box_kernel(
         box = morton2((uint32_t)bx, (uint32_t)by);
         active = box_count(start_t, box, shift) > 0;
     }
-    T br[P1], bi[P1];
+    T br[MW], bi[MW];
 #pragma unroll
-    for (int m = 0; m < P1; ++m) br[m] = bi[m] = (T)0;
+    for (int q = 0; q < MW; ++q) br[q] = bi[q] = (T)0;
     const int W2 = 2 * R + 1;
     const int t0 = il_off[par], tn = il_off[par + 1] - t0;
     for (int t = 0; t < tn; ++t) {
@@ -314,43 +317,39 @@ __global__ __launch_bounds__(64) void m2l_box_kernel(
             ok = box_count(start_s, sb, shift) > 0;
         }
         if (!ok) continue;
-        const V *Uo = U + (int64_t)((o.y + R) * W2 + (o.x + R)) * (p1 + 1);
-        const V *A = Ml + (int64_t)sb * p1;
+        const V *Uo = U + (int64_t)((o.y + R) * W2 + (o.x + R)) * (P1 + 1);
+        const V *A = Ml + (int64_t)sb * P1;
         T apr[P1], api[P1];
 #pragma unroll
         for (int k = 0; k < P1; ++k) {
-            apr[k] = api[k] = (T)0;
-            if (k < p1) {
-                const V a = A[k], u = Uo[k];
-                apr[k] = a.x * u.x - a.y * u.y;
-                api[k] = a.x * u.y + a.y * u.x;
-            }
+            const V a = A[k], u = Uo[k];
+            apr[k] = a.x * u.x - a.y * u.y;
+            api[k] = a.x * u.y + a.y * u.x;
         }
 #pragma unroll
-        for (int m = 0; m < P1; ++m) {
-            if (m < p1) {
+        for (int q = 0; q < MW; ++q) {
+            const int m = m0 + q;
+            if (m < P1) {
                 T sr = 0, si = 0;
 #pragma unroll
                 for (int k = 0; k < P1; ++k) {
-                    if (k < p1) {
-                        const T c = binom[m * p1 + k];
-                        sr += c * apr[k];
-                        si += c * api[k];
-                    }
+                    const T c = binom[m * P1 + k];
+                    sr += c * apr[k];
+                    si += c * api[k];
                 }
                 const V um = Uo[m + 1];
                 const T sgn = (m & 1) ? (T)-1 : (T)1;
-                br[m] += sgn * (um.x * sr - um.y * si);
-                bi[m] += sgn * (um.x * si + um.y * sr);
+                br[q] += sgn * (um.x * sr - um.y * si);
+                bi[q] += sgn * (um.x * si + um.y * sr);
             }
         }
     }
     if (in) {
-        V *Bb = Bl + (int64_t)box * p1;
+        V *Bb = Bl + (int64_t)box * P1;
 #pragma unroll
-        for (int m = 0; m < P1; ++m)
-            if (m < p1)  // boxes without targets get zeros (never read stale)
-                Bb[m] = active ? V{br[m] * inv_r, bi[m] * inv_r} : V{(T)0, (T)0};
+        for (int q = 0; q < MW; ++q)
+            if (m0 + q < P1)  // boxes without targets get zeros (never read stale)
+                Bb[m0 + q] = active ? V{br[q] * inv_r, bi[q] * inv_r} : V{(T)0, (T)0};
     }
 }
 
@@ -417,7 +416,6 @@ __global__ __launch_bounds__(256) void p2p_l2p_f32s(
     double *__restrict__ out_a, double *__restrict__ out_b, int velocity_out)
 {
     __shared__ float4 st[P2P_STAGE];
-    __shared__ StageSeg seg[64];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, t = lane & 15;
     const uint32_t b = (blockIdx.x & 7u) * nl8 + (blockIdx.x >> 3);
@@ -441,7 +439,7 @@ __global__ __launch_bounds__(256) void p2p_l2p_f32s(
             }
         }
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        p2p_staged<KIND>(src, sp, st, seg, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
+        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
         if (!wave_on) continue;
         p2p_merge_groups(ha, la, hb, lb);
         if (g == 0 && valid) {
@@ -768,20 +766,23 @@ static int launch_m2l(Lane *L, int l, int p1, const void *M, void *B, const int3
 {
     using V = typename C2<T>::type;
     const int64_t nbox = int64_t(1) << (2 * l);
-    // lane-per-box form while its 6 P1 accumulators fit in registers
-    constexpr bool box_form = sizeof(T) == 4 ? P1 <= 25 : P1 <= 17;
-    if constexpr (box_form) {
-        const dim3 grid((unsigned)((nbox / 4 + 63) / 64), 4);
-        m2l_box_kernel<T, P1><<<grid, 64, 0, L->stream>>>(l, p1, (const V *)M, (V *)B, ss, st,
-                                                          shift, il, off, (const V *)U, R,
-                                                          (const T *)binom, (T)inv_r);
-    } else {
-        const int G = 64 / p1;
-        const unsigned grid = (unsigned)((nbox + G - 1) / G);
-        m2l_kernel<T, P1><<<grid, 64, 0, L->stream>>>(
-            l, p1, (const V *)M, (V *)B, ss, st, shift, il, off, il_max, (const V *)U, R,
-            (const T *)binom, (T)inv_r);
+    // lane-per-box form (float32, exact order 16: the bench configuration)
+    if constexpr (sizeof(T) == 4 && P1 == 17) {
+        if (p1 == P1) {
+            constexpr int MW = 4;
+            const dim3 grid((unsigned)((nbox / 4 + 63) / 64), 4, (unsigned)((P1 + MW - 1) / MW));
+            m2l_box_kernel<T, P1, MW><<<grid, 64, 0, L->stream>>>(
+                l, (const V *)M, (V *)B, ss, st, shift, il, off, (const V *)U, R,
+                (const T *)binom, (T)inv_r);
+            PHF_HIP(hipGetLastError());
+            return PHF_OK;
+        }
     }
+    const int G = 64 / p1;
+    const unsigned grid = (unsigned)((nbox + G - 1) / G);
+    m2l_kernel<T, P1><<<grid, 64, 0, L->stream>>>(
+        l, p1, (const V *)M, (V *)B, ss, st, shift, il, off, il_max, (const V *)U, R,
+        (const T *)binom, (T)inv_r);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }

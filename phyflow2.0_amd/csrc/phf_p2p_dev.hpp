@@ -68,11 +68,6 @@ __device__ __forceinline__ void two_sum_merge(float &h, float &l, float h2, floa
     l = (l + l2) + err;
 }
 
-struct StageSeg {
-    int lds0, len;   // staged range [lds0, lds0 + len)
-    float sx, sy;    // centre shift of the source leaf (exact)
-    int core;        // 1: pairs may lie inside the smoothing core
-};
 
 // every lane of a target ends with the sum of its four groups: (0,1) and
 // (2,3), then the pairs (a fixed butterfly)
@@ -103,13 +98,16 @@ __device__ __forceinline__ void l2p_f32(const float2 *__restrict__ Bb, int p1, f
 // Accumulate the near-field sum of target tp (offset from its leaf centre)
 // over every near leaf nf names: nf(q, sb, ox, oy, core) gives near index
 // q's source leaf sb, its centre shift (exact in float32) and whether its
-// pairs may lie inside the smoothing core; false: no such leaf.  Every
-// thread of the workgroup must call this (it synchronises); wave_on: the
-// wave has targets.
+// pairs may lie inside the smoothing core; false: no such leaf.  Near
+// indices with core set must come first.  Sources are staged shifted into
+// the target leaf's frame, so each batch is two contiguous ranges (core,
+// then beyond the core) summed without per-leaf setup.  Every thread of
+// the workgroup must call this (it synchronises); wave_on: the wave has
+// targets.
 template <int KIND, class NearFn>
 __device__ __forceinline__ void p2p_staged(const float4 *__restrict__ src,
                                            const int32_t *__restrict__ sp, float4 *st,
-                                           StageSeg *seg, int nnear, const NearFn &nf,
+                                           int nnear, const NearFn &nf,
                                            bool wave_on, float2 tp, const PairParams &pp,
                                            float &ha, float &la, float &hb, float &lb)
 {
@@ -135,19 +133,16 @@ __device__ __forceinline__ void p2p_staged(const float4 *__restrict__ src,
         const int take = in ? min(len, P2P_STAGE - excl) : 0;  // multiple of 4
         const int count = __popcll(__ballot(in));
         const int last_take = __shfl(take, count - 1), last_len = __shfl(len, count - 1);
+        const int used = __shfl(excl + take, count - 1);
+        // staged entries of core leaves form a prefix of the batch
+        const unsigned long long far_mask = __ballot(take > 0 && !core);
+        const int core_end = far_mask ? __shfl(excl, __ffsll((long long)far_mask) - 1) : used;
         if (last_take < last_len) {
             jq = (count == 1 ? jq : 0) + last_take;
             q += count - 1;
         } else {
             jq = 0;
             q += count;
-        }
-        if (wv == 0 && in) {
-            seg[lane].lds0 = excl;
-            seg[lane].len = take;
-            seg[lane].sx = ox;
-            seg[lane].sy = oy;
-            seg[lane].core = core;
         }
         if (in) {
             const float4 *gs = src + j0;
@@ -158,22 +153,24 @@ __device__ __forceinline__ void p2p_staged(const float4 *__restrict__ src,
 #pragma unroll
                 for (int u = 0; u < 8; ++u) r[u] = gs[e + 4 * u];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) ls[e + 4 * u] = r[u];
+                for (int u = 0; u < 8; ++u) {
+                    r[u].x += ox;
+                    r[u].y += oy;
+                    ls[e + 4 * u] = r[u];
+                }
             }
-            for (; e < take; e += 4) ls[e] = gs[e];
+            for (; e < take; e += 4) {
+                float4 r = gs[e];
+                r.x += ox;
+                r.y += oy;
+                ls[e] = r;
+            }
         }
         __syncthreads();
         if (wave_on) {
-            for (int k = 0; k < count; ++k) {
-                const StageSeg sg = seg[k];
-                if (sg.len == 0) continue;
-                const float xt = tp.x - sg.sx, yt = tp.y - sg.sy;
-                const float4 *sq = st + sg.lds0 + g;
-                if (sg.core)
-                    p2p_seg_f32<KIND, true>(sq, 0, sg.len, xt, yt, pp, ha, la, hb, lb);
-                else
-                    p2p_seg_f32<KIND, false>(sq, 0, sg.len, xt, yt, pp, ha, la, hb, lb);
-            }
+            const float4 *sq = st + g;
+            p2p_seg_f32<KIND, true>(sq, 0, core_end, tp.x, tp.y, pp, ha, la, hb, lb);
+            p2p_seg_f32<KIND, false>(sq, core_end, used, tp.x, tp.y, pp, ha, la, hb, lb);
         }
         __syncthreads();
     }
