@@ -76,6 +76,26 @@ def rel_errors(sysm, params, s, n_check, seed):
                                                                    np.max(np.abs(b)))
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fmm_1m_p2p_hbm_traffic.json")
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of the P2P kernel at the default workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter
+    scaled by what it reads for a copy of known size); None when the
+    summary is absent or was taken on another kernel or workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("kernel") != kernel or (args.n, args.order, args.leaf, args.prec) != (
+            1048576, 16, 64, "f32c"):
+        return None
+    return {"bytes_per_launch": d["bytes_per_launch"],
+            "source": "profiles/fmm_1m_p2p_hbm_traffic.json"}
+
+
 def cpu_baseline(s, budget_s=40.0):
     """The reference's compiled CPU FMM (oracle/_ref) at its default
     tol=1e-6, Ndirect=35 on the same workload; falls back to the oracle's
@@ -131,7 +151,8 @@ def bench_lamb(args):
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "direct Gaussian sum, 4096 lattice blobs (Lamb-Oseen)",
                        "variants": res},
-            "roofline": {"kernel": "direct_f32c", "bound": "valu", "achieved": ach,
+            "roofline": {"kernel": "direct_f32c", "bound": "mfma",
+                         "compute_pipe": "fp32 vector ALU (no matrix shape)", "achieved": ach,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
                          "traffic": None},
             "cpu_baseline": {"value": n / t_cpu, "unit": "evals/s", "cores": os.cpu_count() or 1,
@@ -152,24 +173,29 @@ def bench_cylinder(args):
     t_build = time.perf_counter() - t
     p = vel_params(0.004, prec=gauss.PREC_F32C, order=16, leaf=64)
     times, infos = [], []
-    for k in range(args.warmup + max(1, args.steps // 10)):
-        with BlobSystem(a["x"], a["y"], a["w"]) as sysm:
+    # one resident simulation stepped repeatedly (the first steps warm the
+    # system's device workspace); every step is a full coupled RK2 + remesh
+    with BlobSystem(a["x"], a["y"], a["w"]) as sysm:
+        for k in range(args.warmup + args.steps):
             _native.lib().phf_synchronize()
             t = time.perf_counter()
             info = step_rk2(sysm, p, 0.001, vInf=(1.0, 0.0), panels=body,
                             remesh=dict(sigma=0.004, overlap=1.25))
             dt = time.perf_counter() - t
-        if k >= args.warmup:
-            times.append(dt)
-            infos.append(info)
+            if k >= args.warmup:
+                times.append(dt)
+                infos.append(info)
     info = infos[-1]
     tot = float(np.sum(np.abs(a["w"])))
     return {"metric": "cylinder step time (512 panels, 262144 blobs, RK2 + remesh)",
             "value": 1e3 * float(np.median(times)), "unit": "ms", "n_gpus": 1,
             "steps": len(times), "warmup": args.warmup, "ms_per_step": 1e3 * float(np.median(times)),
+            "p50_ms": 1e3 * float(np.percentile(times, 50)),
+            "p99_ms": 1e3 * float(np.percentile(times, 99)),
             "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
             "data": "synthetic",
-            "config": {"workload": "impulsively started cylinder, one coupled step",
+            "config": {"workload": "impulsively started cylinder (512 panels, 262144 blobs, "
+                                   "dt 1e-3): consecutive coupled steps of one simulation",
                        "panel_setup_ms": 1e3 * t_build, "step_info": info,
                        "circ_rel_change": abs(info["circ_after"] - info["circ_before"]) / tot,
                        "impulse_rel_change": max(abs(info["impulse_x_after"] - info["impulse_x_before"]),
@@ -456,8 +482,10 @@ def main():
             "stage_ms": info["ms"], "device": dev["name"],
         },
         "roofline": {
-            "kernel": "p2p_l2p_f32c" if prec == gauss.PREC_F32C else "p2p_l2p_f64",
-            "bound": "valu",
+            "kernel": "p2p_l2p_f32s" if prec == gauss.PREC_F32C else "p2p_l2p_f64",
+            "bound": "mfma",
+            "compute_pipe": "fp32 vector ALU (the pair kernel has no matrix shape); MI355X "
+                            "fp32 peak 157.3 TFLOP/s, fp64 78.6",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS if prec == gauss.PREC_F32C else 78.6,
             "unit": "TFLOP/s",
@@ -467,6 +495,10 @@ def main():
             "kernel_ms": t_p2p * 1e3,
         },
     }
+    tr = measured_traffic(out["roofline"]["kernel"], args)
+    if tr:
+        out["roofline"]["traffic"] = tr["bytes_per_launch"]
+        out["roofline"]["traffic_source"] = tr["source"]
     if variants:
         out["variants"] = variants
     if not args.no_cpu_baseline:
