@@ -276,22 +276,42 @@ __global__ __launch_bounds__(64) void m2l_kernel(
 // A'_k = u^k A_k (all k) and B_m += (-1)^m u^(m+1) sum_k C(k+m, k) A'_k in
 // registers, the factorisation and summation order of m2l_kernel, with no
 // LDS traffic or barriers; splitting m over waves gives the finest level
-// enough waves to hide the load latency.  Requires p1 == P1.
+// enough waves to hide the load latency.  Requires p1 == P1 (float32).
+// Levels of one M2L launch: every level >= 2 runs in the same grid (they
+// are independent once the upward pass is done), so the short serial link
+// chains of the coarse levels overlap with the work of the finest.
+constexpr int kM2LMaxLev = 12;
+struct M2LLevels {
+    int nlev, lev0;
+    int64_t blk[kM2LMaxLev + 1];   // first block of each level (grid.x)
+    int64_t coff[kM2LMaxLev];      // box offset of the level in the M / B arrays
+    int il_base[kM2LMaxLev];       // index of the level's 5 parity offsets
+    int shift[kM2LMaxLev];         // 2 (L - l)
+    float inv_r[kM2LMaxLev];       // 1 / box side
+};
+
 template <class T, int P1, int MW>
 __global__ __launch_bounds__(64) void m2l_box_kernel(
-    int l, const typename C2<T>::type *__restrict__ Ml, typename C2<T>::type *__restrict__ Bl,
-    const int32_t *__restrict__ start_s, const int32_t *__restrict__ start_t, int shift,
-    const int2 *__restrict__ il_tau, const int32_t *__restrict__ il_off /* 5 */,
-    const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [P1][P1] */,
-    T inv_r)
+    M2LLevels lv, const typename C2<T>::type *__restrict__ M, typename C2<T>::type *__restrict__ B,
+    const int32_t *__restrict__ start_s, const int32_t *__restrict__ start_t,
+    const int2 *__restrict__ il_tau, const int32_t *__restrict__ il_offs,
+    const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [P1][P1] */)
 {
     using V = typename C2<T>::type;
+    int li = 0;
+    while (li + 1 < lv.nlev && (int64_t)blockIdx.x >= lv.blk[li + 1]) ++li;
+    const int l = lv.lev0 + li;
+    const typename C2<T>::type *Ml = M + lv.coff[li] * P1;
+    typename C2<T>::type *Bl = B + lv.coff[li] * P1;
+    const int32_t *il_off = il_offs + lv.il_base[li];
+    const int shift = lv.shift[li];
+    const T inv_r = (T)lv.inv_r[li];
     const int lane = threadIdx.x;
     const int par = blockIdx.y;
     const int m0 = blockIdx.z * MW;
     const int n = 1 << l, half = n >> 1;
     const int64_t per = ((int64_t)n * n) / 4;  // boxes per parity class
-    const int64_t loc = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t loc = ((int64_t)blockIdx.x - lv.blk[li]) * 64 + lane;
     const bool in = loc < per;
     int bx = 0, by = 0;
     uint32_t box = 0;
@@ -766,18 +786,6 @@ static int launch_m2l(Lane *L, int l, int p1, const void *M, void *B, const int3
 {
     using V = typename C2<T>::type;
     const int64_t nbox = int64_t(1) << (2 * l);
-    // lane-per-box form (float32, exact order 16: the bench configuration)
-    if constexpr (sizeof(T) == 4 && P1 == 17) {
-        if (p1 == P1) {
-            constexpr int MW = 4;
-            const dim3 grid((unsigned)((nbox / 4 + 63) / 64), 4, (unsigned)((P1 + MW - 1) / MW));
-            m2l_box_kernel<T, P1, MW><<<grid, 64, 0, L->stream>>>(
-                l, (const V *)M, (V *)B, ss, st, shift, il, off, (const V *)U, R,
-                (const T *)binom, (T)inv_r);
-            PHF_HIP(hipGetLastError());
-            return PHF_OK;
-        }
-    }
     const int G = 64 / p1;
     const unsigned grid = (unsigned)((nbox + G - 1) / G);
     m2l_kernel<T, P1><<<grid, 64, 0, L->stream>>>(
@@ -973,11 +981,34 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         // M2L at every level >= 2 (writes B)
         tm.begin(PHF_STAGE_M2L);
         used[PHF_STAGE_M2L] = true;
-        for (int l = 2; l <= Lv; ++l) {
-            const double r = dom.D / std::ldexp(1.0, l);
-            PHF_TRY(m2l_dispatch<T>(L, l, p1, M + off[l] * p1, B + off[l] * p1, S.start,
-                                    Tg.start, 2 * (Lv - l), d_il, d_iloff + il_base[l],
-                                    il_max[l], d_U, R, d_bin, 1.0 / r));
+        if (sizeof(T) == 4 && p1 == 17) {
+            // lane-per-box form, every level in one launch (float32, order 16)
+            constexpr int MW = 4;
+            M2LLevels lv{};
+            lv.nlev = Lv - 1;
+            lv.lev0 = 2;
+            lv.blk[0] = 0;
+            for (int l = 2; l <= Lv; ++l) {
+                const int i = l - 2;
+                const int64_t per = (int64_t(1) << (2 * l)) / 4;
+                lv.blk[i + 1] = lv.blk[i] + (per + 63) / 64;
+                lv.coff[i] = off[l];
+                lv.il_base[i] = il_base[l];
+                lv.shift[i] = 2 * (Lv - l);
+                lv.inv_r[i] = (float)(1.0 / (dom.D / std::ldexp(1.0, l)));
+            }
+            const dim3 grid((unsigned)lv.blk[lv.nlev], 4, (17 + MW - 1) / MW);
+            m2l_box_kernel<float, 17, MW><<<grid, 64, 0, L->stream>>>(
+                lv, (const float2 *)M, (float2 *)B, S.start, Tg.start, d_il, d_iloff,
+                (const float2 *)d_U, R, (const float *)d_bin);
+            PHF_HIP(hipGetLastError());
+        } else {
+            for (int l = 2; l <= Lv; ++l) {
+                const double r = dom.D / std::ldexp(1.0, l);
+                PHF_TRY(m2l_dispatch<T>(L, l, p1, M + off[l] * p1, B + off[l] * p1, S.start,
+                                        Tg.start, 2 * (Lv - l), d_il, d_iloff + il_base[l],
+                                        il_max[l], d_U, R, d_bin, 1.0 / r));
+            }
         }
         tm.end(PHF_STAGE_M2L);
         // L2L down
