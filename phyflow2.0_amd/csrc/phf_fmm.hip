@@ -185,6 +185,89 @@ __global__ void l2l_kernel(int p1, uint32_t nchild, const typename C2<T>::type *
     Bc[(int64_t)b * p1 + m] = V{(T)((TC)cur.x + acc.x), (T)((TC)cur.y + acc.y)};
 }
 
+// Same shifts with the operands staged in LDS and the inner loops unrolled
+// (exact order P1 - 1): a block of 64 lanes serves G = 64 / P1 boxes, lane
+// (g, m) one coefficient.  M2M: the 4 G children of the block's G parents
+// are consecutive boxes in Morton order, so their coefficients are one
+// contiguous, coalesced load.  L2L: group g loads its child's parent.
+// Each level then costs one load round instead of a chain of dependent
+// loads per coefficient.  Terms are added in the order of the kernels above.
+template <class T, class TC, int P1>
+__global__ __launch_bounds__(64) void m2m_lds_kernel(uint32_t nparent,
+                                                     const typename C2<T>::type *__restrict__ Mc,
+                                                     typename C2<T>::type *__restrict__ Mp,
+                                                     const typename C2<TC>::type *__restrict__ tab,
+                                                     const int32_t *__restrict__ start_s,
+                                                     int shift)
+{
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+    constexpr int G = 64 / P1;
+    __shared__ V ch[G * 4 * P1];
+    const int lane = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * G;
+    const int64_t np = min((int64_t)G, (int64_t)nparent - p0);
+    const int64_t base = 4 * p0 * P1;
+    for (int i = lane; i < G * 4 * P1; i += 64)
+        if (i < np * 4 * P1) ch[i] = Mc[base + i];
+    __syncthreads();
+    const int g = lane / P1, m = lane - (lane / P1) * P1;
+    if (g >= np) return;
+    const uint32_t b = (uint32_t)(p0 + g);
+    W acc = W{0, 0};
+    if (box_count(start_s, b, shift) > 0) {
+        for (int c = 0; c < 4; ++c) {
+            const V *chc = ch + (g * 4 + c) * P1;
+            const W *tr = tab + ((int64_t)c * P1 + m) * P1;
+#pragma unroll
+            for (int k = 0; k < P1; ++k) {
+                if (k <= m) {
+                    const V a = chc[k];
+                    const W tk = tr[k];
+                    acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
+                    acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
+                }
+            }
+        }
+    }
+    Mp[(int64_t)b * P1 + m] = V{(T)acc.x, (T)acc.y};
+}
+
+template <class T, class TC, int P1>
+__global__ __launch_bounds__(64) void l2l_lds_kernel(uint32_t nchild,
+                                                     const typename C2<T>::type *__restrict__ Bp,
+                                                     typename C2<T>::type *__restrict__ Bc,
+                                                     const typename C2<TC>::type *__restrict__ tab,
+                                                     const int32_t *__restrict__ start_t,
+                                                     int shift)
+{
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+    constexpr int G = 64 / P1;
+    __shared__ V par[G * P1];
+    const int lane = threadIdx.x;
+    const int g = lane / P1, m = lane - (lane / P1) * P1;
+    const int64_t b64 = (int64_t)blockIdx.x * G + g;
+    const bool in = g < G && b64 < (int64_t)nchild;
+    const uint32_t b = (uint32_t)b64;
+    if (in) par[g * P1 + m] = Bp[(int64_t)(b >> 2) * P1 + m];
+    __syncthreads();
+    if (!in || box_count(start_t, b, shift) == 0) return;
+    const W *tr = tab + ((int64_t)(b & 3) * P1 + m) * P1;
+    W acc = W{0, 0};
+#pragma unroll
+    for (int j = 0; j < P1; ++j) {
+        if (j >= m) {
+            const V a = par[g * P1 + j];
+            const W tk = tr[j];
+            acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
+            acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
+        }
+    }
+    const V cur = Bc[(int64_t)b * P1 + m];
+    Bc[(int64_t)b * P1 + m] = V{(T)((TC)cur.x + acc.x), (T)((TC)cur.y + acc.y)};
+}
+
 // ---------------------------------------------------------------- M2L ----
 // One wave holds G = 64/p1 target boxes of one level; lane (g, m) owns
 // coefficient m of box g.  Per link the group forms A'_k = u^k A_k in LDS,
@@ -779,6 +862,56 @@ int fmm_level(int64_t nb, int leaf, int ndirect)
 
 
 
+// M2M / L2L of one level: the LDS-staged unrolled kernels for the common
+// orders, the generic ones otherwise.
+template <class T, class TC>
+static int shift_up(Lane *L, int p1, uint32_t np, const void *Mc, void *Mp, const void *tab,
+                    const int32_t *start, int shift)
+{
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+#define PHF_M2M(P)                                                                             \
+    if (p1 == P) {                                                                             \
+        m2m_lds_kernel<T, TC, P><<<div_up(np, 64 / P), 64, 0, L->stream>>>(                    \
+            np, (const V *)Mc, (V *)Mp, (const W *)tab, start, shift);                         \
+        PHF_HIP(hipGetLastError());                                                            \
+        return PHF_OK;                                                                         \
+    }
+    PHF_M2M(9)
+    PHF_M2M(13)
+    PHF_M2M(17)
+    PHF_M2M(25)
+#undef PHF_M2M
+    m2m_kernel<T, TC><<<div_up((int64_t)np * p1, 256), 256, 0, L->stream>>>(
+        p1, np, (const V *)Mc, (V *)Mp, (const W *)tab, start, shift);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+template <class T, class TC>
+static int shift_down(Lane *L, int p1, uint32_t nc, const void *Bp, void *Bc, const void *tab,
+                      const int32_t *start, int shift)
+{
+    using V = typename C2<T>::type;
+    using W = typename C2<TC>::type;
+#define PHF_L2L(P)                                                                             \
+    if (p1 == P) {                                                                             \
+        l2l_lds_kernel<T, TC, P><<<div_up(nc, 64 / P), 64, 0, L->stream>>>(                    \
+            nc, (const V *)Bp, (V *)Bc, (const W *)tab, start, shift);                         \
+        PHF_HIP(hipGetLastError());                                                            \
+        return PHF_OK;                                                                         \
+    }
+    PHF_L2L(9)
+    PHF_L2L(13)
+    PHF_L2L(17)
+    PHF_L2L(25)
+#undef PHF_L2L
+    l2l_kernel<T, TC><<<div_up((int64_t)nc * p1, 256), 256, 0, L->stream>>>(
+        p1, nc, (const V *)Bp, (V *)Bc, (const W *)tab, start, shift);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
 template <class T, int P1>
 static int launch_m2l(Lane *L, int l, int p1, const void *M, void *B, const int32_t *ss,
                       const int32_t *st, int shift, const int2 *il, const int32_t *off, int il_max,
@@ -973,8 +1106,8 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         used[PHF_STAGE_M2M] = true;
         for (int l = Lv - 1; l >= 2; --l) {
             const uint32_t np = 1u << (2 * l);
-            m2m_kernel<T, TC><<<div_up((int64_t)np * p1, 256), 256, 0, L->stream>>>(
-                p1, np, M + off[l + 1] * p1, M + off[l] * p1, d_m2m, S.start, 2 * (Lv - l));
+            PHF_TRY((shift_up<T, TC>(L, p1, np, M + off[l + 1] * p1, M + off[l] * p1, d_m2m,
+                                     S.start, 2 * (Lv - l))));
         }
         PHF_HIP(hipGetLastError());
         tm.end(PHF_STAGE_M2M);
@@ -1016,9 +1149,8 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         used[PHF_STAGE_L2L] = true;
         for (int l = 2; l < Lv; ++l) {
             const uint32_t nc = 1u << (2 * (l + 1));
-            l2l_kernel<T, TC><<<div_up((int64_t)nc * p1, 256), 256, 0, L->stream>>>(
-                p1, nc, B + off[l] * p1, B + off[l + 1] * p1, d_l2l, Tg.start,
-                2 * (Lv - l - 1));
+            PHF_TRY((shift_down<T, TC>(L, p1, nc, B + off[l] * p1, B + off[l + 1] * p1, d_l2l,
+                                       Tg.start, 2 * (Lv - l - 1))));
         }
         PHF_HIP(hipGetLastError());
         tm.end(PHF_STAGE_L2L);
