@@ -88,6 +88,21 @@ int phf_vv2par_gauss(const double *xb, const double *yb, const double *wb,
                      int64_t nt, double ksigmasqr, int32_t prec, double *vx,
                      double *vy);
 
+/* Algebraic cutoff kernel (kernel = 0 in velocity()/vorticity()).  Replace
+ * vv2par_gpu / vv2par_C_cpu(xB,yB,wB,xT,yT,sigmasqr) and ww2par_gpu /
+ * ww2par_C_cpu (src/cpp/blobs/c/vv2par_C/vv2par_C_cpu.c:89-153,
+ * src/cpp/blobs/c/ww2par_C/ww2par_C_cpu.c:89-147, called with sigma^2 at
+ * _induced.py:585, :695):
+ *   u_t = inv_2pi sum_m -dy w_m/r2, v_t = inv_2pi sum_m dx w_m/r2,
+ *   r2 = dx^2 + dy^2 + sigmasqr;  w_t = inv_2pi sum_m w_m / sqrt(r2). */
+int phf_vv2par_cutoff(const double *xb, const double *yb, const double *wb,
+                      int64_t nb, const double *xt, const double *yt,
+                      int64_t nt, double sigmasqr, int32_t prec, double *vx,
+                      double *vy);
+int phf_ww2par_cutoff(const double *xb, const double *yb, const double *wb,
+                      int64_t nb, const double *xt, const double *yt,
+                      int64_t nt, double sigmasqr, int32_t prec, double *w);
+
 /* Induced vorticity of Gaussian blobs.  Replaces
  * ww2parGauss_gpu(xB,yB,wB,xT,yT,ksigmasqr,blocksize)
  *   src/cpp/blobs/cuda/ww2parGauss_gpu/ww2parGauss_py_gpu.cu:104-200

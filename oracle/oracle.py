@@ -38,6 +38,8 @@ def lib():
             raise RuntimeError("oracle not built: run `make -C oracle`")
         L = ctypes.CDLL(LIB_PATH)
         L.orc_vv2par_gauss.argtypes = [_i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _dbl, _ptr, _ptr, ctypes.c_int]
+        L.orc_vv2par_cutoff.argtypes = L.orc_vv2par_gauss.argtypes
+        L.orc_ww2par_cutoff.argtypes = [_i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _dbl, _ptr, ctypes.c_int]
         L.orc_ww2par_gauss.argtypes = [_i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _dbl, _ptr, ctypes.c_int]
         L.orc_direct_oseen.argtypes = [_i64, _ptr, _ptr, _ptr, _i64, _ptr, _ptr, _dbl, _dbl, _ptr, _ptr, ctypes.c_int]
         L.orc_m4kernel_1d.argtypes = [_i64, _ptr, _dbl, _ptr]
@@ -89,6 +91,25 @@ def vv2par_gauss(xb, yb, wb, xt, yt, ksigmasqr, nthreads=0):
     lib().orc_vv2par_gauss(xb.size, _p(xb), _p(yb), _p(wb), xt.size, _p(xt), _p(yt),
                            float(ksigmasqr), _p(vx), _p(vy), int(nthreads))
     return vx, vy
+
+
+def vv2par_cutoff(xb, yb, wb, xt, yt, sigmasqr, nthreads=0):
+    """vv2par_C_cpu.c:135-151 restated; returns (vx, vy)."""
+    xb, yb, wb, xt, yt = map(_f64, (xb, yb, wb, xt, yt))
+    vx = np.empty(xt.size)
+    vy = np.empty(xt.size)
+    lib().orc_vv2par_cutoff(xb.size, _p(xb), _p(yb), _p(wb), xt.size, _p(xt), _p(yt),
+                            float(sigmasqr), _p(vx), _p(vy), int(nthreads))
+    return vx, vy
+
+
+def ww2par_cutoff(xb, yb, wb, xt, yt, sigmasqr, nthreads=0):
+    """ww2par_C_cpu.c:132-144 restated; returns w."""
+    xb, yb, wb, xt, yt = map(_f64, (xb, yb, wb, xt, yt))
+    w = np.empty(xt.size)
+    lib().orc_ww2par_cutoff(xb.size, _p(xb), _p(yb), _p(wb), xt.size, _p(xt), _p(yt),
+                            float(sigmasqr), _p(w), int(nthreads))
+    return w
 
 
 def ww2par_gauss(xb, yb, wb, xt, yt, ksigmasqr, nthreads=0):

@@ -64,6 +64,66 @@ void orc_vv2par_gauss(int64_t nb, const double *xb, const double *yb,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Algebraic cutoff kernel.  Velocity restates
+ * src/cpp/blobs/c/vv2par_C/vv2par_C_cpu.c:135-151 (r2 = dx*dx + dy*dy +
+ * sigmasqr, s = w/r2, vx -= dy*s, vy += dx*s, scaled by inv_2pi =
+ * 0.5/3.141592653589793 at :100); the companion ww2par_C_cpu.c:132-144
+ * (w += w_m/sqrt(r2), scaled by inv_2pi).
+ * Pinned by: single-blob closed forms u_theta = w r/(2 pi (r^2 + s2)) and
+ * w/(2 pi sqrt(r^2 + s2)); linearity; the far field of the velocity tends
+ * to the point vortex; swap antisymmetry of the pair velocity.            */
+void orc_vv2par_cutoff(int64_t nb, const double *xb, const double *yb,
+                       const double *wb, int64_t nt, const double *xt,
+                       const double *yt, double sigmasqr, double *vx,
+                       double *vy, int nthreads)
+{
+    const double inv_2pi = 0.5 / 3.141592653589793;
+    int64_t k;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < nt; k++) {
+        double vxk = 0.0, vyk = 0.0;
+        for (int64_t m = 0; m < nb; m++) {
+            double dx = xt[k] - xb[m];
+            double dy = yt[k] - yb[m];
+            double r2 = dx * dx + dy * dy + sigmasqr;
+            double s = wb[m] / r2;
+            vxk -= dy * s;
+            vyk += dx * s;
+        }
+        vx[k] = inv_2pi * vxk;
+        vy[k] = inv_2pi * vyk;
+    }
+}
+
+void orc_ww2par_cutoff(int64_t nb, const double *xb, const double *yb,
+                       const double *wb, int64_t nt, const double *xt,
+                       const double *yt, double sigmasqr, double *w, int nthreads)
+{
+    const double inv_2pi = 0.5 / 3.141592653589793;
+    int64_t k;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < nt; k++) {
+        double wk = 0.0;
+        for (int64_t m = 0; m < nb; m++) {
+            double dx = xt[k] - xb[m];
+            double dy = yt[k] - yb[m];
+            wk += wb[m] / sqrt(dx * dx + dy * dy + sigmasqr);
+        }
+        w[k] = inv_2pi * wk;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Induced vorticity of Gaussian blobs.
  * Restates src/cpp/blobs/c/ww2parGauss_C/ww2parGauss_C_cpu.c:89-153
  * (EPS=1e-12 at :99, inv_pi=1/3.141592653589793 at :101, loop :133-144):

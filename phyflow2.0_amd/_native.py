@@ -124,6 +124,8 @@ _SIGS = {
     "phf_last_stage_times": ([_ptr], ctypes.c_int),
     "phf_vv2par_gauss": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr, _ptr], ctypes.c_int),
     "phf_ww2par_gauss": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr], ctypes.c_int),
+    "phf_vv2par_cutoff": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr, _ptr], ctypes.c_int),
+    "phf_ww2par_cutoff": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr], ctypes.c_int),
     "phf_fmm2d": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _i32, _dbl, _dbl, _dbl,
                    ctypes.POINTER(FmmOpts), _ptr, _ptr], ctypes.c_int),
     "phf_system_create": ([_ptr, _ptr, _ptr, _i64, ctypes.POINTER(_ptr)], ctypes.c_int),

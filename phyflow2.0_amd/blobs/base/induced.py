@@ -11,6 +11,7 @@ kernel, :138):
   method 0 (FMM),    kernel 1 (gauss) -> fmm2d with xopt = sqrt(alpha k sigma^2),
                                          cutoff = 5 xopt, mapped to
                                          (-qi/2pi, -qr/2pi) (:236-251, :321-336)
+  method 1 (direct), kernel 0 (cutoff) -> vv2par with sigma^2 (:585, :695)
   FMM with the cutoff kernel raises, as the reference does (:158-163).
 Differences from the reference, all deliberate:
   * every computation runs on the GPU (``hardware`` is accepted and selects
@@ -18,7 +19,6 @@ Differences from the reference, all deliberate:
   * ``xEval is None`` is tested with ``is`` (the reference's ``== None`` is
     an elementwise comparison on modern NumPy and raises in ``if``) and no
     padding is needed;
-  * the algebraic cutoff kernel (kernel 0) is not provided.
 Keyword-only extensions: ``prec`` (PREC_F64 default, or PREC_F32C) and the
 FMM ``order``/``leaf``/``ws``.
 
@@ -35,6 +35,7 @@ import math
 
 import numpy as np
 
+from ...cpp.blobs.kernels import cutoff as cutoff_kernels
 from ...cpp.blobs.kernels import gauss
 from ...cpp.blobs.kernels.gauss import PREC_F32C, PREC_F64
 
@@ -54,14 +55,17 @@ def velocity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None, 
              order=0, leaf=0, ws=0):
     if hardware not in (CPU_HARDWARE, GPU_HARDWARE):
         raise ValueError("hardware must be 0 (cpu) or 1 (gpu)")
-    if kernel == CUTOFF_KERNEL:
-        if method == FMM_METHOD:
-            raise Exception("FMM calculation of cutoff kernel not yet available")
-        raise NotImplementedError("the algebraic cutoff kernel is not provided by this build")
-    if kernel != GAUSS_KERNEL:
+    if kernel == CUTOFF_KERNEL and method == FMM_METHOD:
+        raise Exception("FMM calculation of cutoff kernel not yet available")
+    if kernel not in (GAUSS_KERNEL, CUTOFF_KERNEL):
         raise ValueError("unknown kernel %r" % (kernel,))
     if (xEval is None) != (yEval is None):
         raise ValueError("xEval and yEval must both be given or both be None")
+    if kernel == CUTOFF_KERNEL and method == DIRECT_METHOD:  # :585, :695 (sigma^2)
+        xe = xBlob if xEval is None else xEval
+        ye = yBlob if yEval is None else yEval
+        return cutoff_kernels.vv2par_gpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma, blocksize,
+                                         prec=prec)
     if method == DIRECT_METHOD:
         xe = xBlob if xEval is None else xEval
         ye = yBlob if yEval is None else yEval
@@ -88,13 +92,14 @@ def vorticity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None,
         raise Exception("FMM method not implemented yet")
     if method != DIRECT_METHOD:
         raise ValueError("unknown method %r" % (method,))
-    if kernel == CUTOFF_KERNEL:
-        raise NotImplementedError("the algebraic cutoff kernel is not provided by this build")
-    if kernel != GAUSS_KERNEL:
+    if kernel not in (GAUSS_KERNEL, CUTOFF_KERNEL):
         raise ValueError("unknown kernel %r" % (kernel,))
     if (xEval is None) != (yEval is None):
         raise ValueError("xEval and yEval must both be given or both be None")
     xe = xBlob if xEval is None else xEval
     ye = yBlob if yEval is None else yEval
+    if kernel == CUTOFF_KERNEL:  # :1023, :1120 (sigma^2)
+        return cutoff_kernels.ww2par_gpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma, blocksize,
+                                         prec=prec)
     return gauss.ww2parGauss_gpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma, blocksize,
                                  prec=prec)

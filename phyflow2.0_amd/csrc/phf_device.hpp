@@ -16,11 +16,14 @@ enum PairKind : int {
     // scale rounds to 1.0f and exp(-shape c2) < 2^-25, so 1 - e == 1.0f for
     // every r2 >= c2 (the default cutoff 5 xopt gives exp(-31.4)).
     PAIR_OSEEN_NOCUT = 2,
+    // algebraic cutoff kernel (vv2par_C_cpu.c:142-147): P = w/(r2 + sigma^2),
+    // sigma^2 carried in eps
+    PAIR_CUTOFF = 3,
 };
 
 // Parameters of the pair kernel, prepared on the host in float64.
 struct PairParams {
-    double eps;       // added to r2 (Gaussian kernel only)
+    double eps;       // added to r2 (Gaussian: EPS; cutoff: sigma^2)
     double inv_ks;    // 1/(k sigma^2) == shape of the Oseen kernel
     double c2;        // cutoff^2 (Oseen)
     double scale;     // -1/expm1(-shape c2) (Oseen)
@@ -95,6 +98,9 @@ __device__ __forceinline__ float pair_f32(float dx, float dy, float w, const Pai
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, pp.eps_f));
         const float e = exp2_f32(r2 * pp.nlog2e_ks_f);
         return __builtin_fmaf(-w, e, w) * rcp_f32(r2);
+    } else if (KIND == PAIR_CUTOFF) {
+        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, pp.eps_f));
+        return w * rcp_f32(r2);
     } else if (KIND == PAIR_OSEEN_NOCUT) {
         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, 1e-30f));
         const float e = exp2_f32(r2 * pp.nlog2e_ks_f);
@@ -129,6 +135,9 @@ __device__ __forceinline__ double pair_f64(double dx, double dy, double w, const
         const double r2 = __builtin_fma(dx, dx, dy * dy) + pp.eps;
         const double e = exp(-r2 * pp.inv_ks);
         return (w * rcp_f64(r2)) * (1.0 - e);
+    } else if (KIND == PAIR_CUTOFF) {
+        const double r2 = __builtin_fma(dx, dx, dy * dy) + pp.eps;
+        return w * rcp_f64(r2);
     } else {
         const double r2 = __builtin_fma(dx, dx, dy * dy);
         const double om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;

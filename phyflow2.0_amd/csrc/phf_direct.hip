@@ -208,6 +208,7 @@ __global__ __launch_bounds__(64) void direct_f64(
 // Adds the chunk partials in chunk order and maps (A, B) = (sum P dx,
 // sum P dy) to the reference outputs:
 //   Gaussian: vx = inv_2pi * (-B), vy = inv_2pi * A  (vv2parGauss_C_cpu.c:150-151)
+//   cutoff:   the same                               (vv2par_C_cpu.c:149-150)
 //   Oseen:    qr = -A,  qi = B                       (direct.cpp:1482-1483)
 __global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int64_t nt,
                               int kind, double *__restrict__ o0, double *__restrict__ o1)
@@ -220,7 +221,7 @@ __global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int6
         a += p.x;
         b += p.y;
     }
-    if (kind == PAIR_GAUSS_EPS) {
+    if (kind == PAIR_GAUSS_EPS || kind == PAIR_CUTOFF) {
         const double inv_2pi = 0.5 / 3.141592653589793;
         o0[t] = inv_2pi * (-b);
         o1[t] = inv_2pi * a;
@@ -234,7 +235,7 @@ __global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int6
 // ww2parGauss_C_cpu.c:133-145:
 //   w(t) = (1/pi) sum_m w_m exp(-(|t - z_m|^2 + EPS)/ks) / ks,  EPS = 1e-12
 // Same layout as the velocity kernels with one accumulator per target.
-template <int TPL>
+template <int TPL, bool CUT>
 __global__ __launch_bounds__(64) void vort_f32c(const Src32c *__restrict__ src, int64_t nb,
                                                 const double *__restrict__ xt,
                                                 const double *__restrict__ yt, int64_t nt,
@@ -265,7 +266,8 @@ __global__ __launch_bounds__(64) void vort_f32c(const Src32c *__restrict__ src, 
             for (int i = 0; i < TPL; ++i) {
                 const float dx = (float)(tx[i] - s.x), dy = (float)(ty[i] - s.y);
                 const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, pp.eps_f));
-                a[i] = __builtin_fmaf(s.w, exp2_f32(r2 * pp.nlog2e_ks_f), a[i]);
+                const float f = CUT ? __builtin_amdgcn_rsqf(r2) : exp2_f32(r2 * pp.nlog2e_ks_f);
+                a[i] = __builtin_fmaf(s.w, f, a[i]);
             }
         }
 #pragma unroll
@@ -279,7 +281,7 @@ __global__ __launch_bounds__(64) void vort_f32c(const Src32c *__restrict__ src, 
     }
 }
 
-template <int TPL>
+template <int TPL, bool CUT>
 __global__ __launch_bounds__(64) void vort_f64(const Src64 *__restrict__ src, int64_t nb,
                                                const double *__restrict__ xt,
                                                const double *__restrict__ yt, int64_t nt,
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(64) void vort_f64(const Src64 *__restrict__ src, in
         for (int i = 0; i < TPL; ++i) {
             const double dx = tx[i] - s.x, dy = ty[i] - s.y;
             const double r2 = dx * dx + dy * dy + pp.eps;
-            a[i] += s.w * exp(-r2 * pp.inv_ks);
+            a[i] += CUT ? s.w / sqrt(r2) : s.w * exp(-r2 * pp.inv_ks);
         }
     }
     double *out = part + (int64_t)blockIdx.y * nt;
@@ -316,16 +318,22 @@ __global__ __launch_bounds__(64) void vort_f64(const Src64 *__restrict__ src, in
     }
 }
 
-// chunk partials in chunk order, then inv_pi * w / ks (ww2parGauss_C_cpu.c:145)
+// chunk partials in chunk order, then inv_pi * w / ks (ww2parGauss_C_cpu.c:144)
+// or, for the cutoff kernel, inv_2pi * w (ww2par_C_cpu.c:144)
 __global__ void vort_finish(const double *__restrict__ part, int nchunk, int64_t nt, double ks,
-                            double *__restrict__ w)
+                            int cut, double *__restrict__ w)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     double a = 0.0;
     for (int c = 0; c < nchunk; ++c) a += part[(int64_t)c * nt + t];
-    const double inv_pi = 1.0 / 3.141592653589793;
-    w[t] = inv_pi * a / ks;
+    if (cut) {
+        const double inv_2pi = 0.5 / 3.141592653589793;
+        w[t] = inv_2pi * a;
+    } else {
+        const double inv_pi = 1.0 / 3.141592653589793;
+        w[t] = inv_pi * a / ks;
+    }
 }
 
 // ------------------------------------------------------------- host ------
@@ -386,18 +394,22 @@ static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *
     }
     PHF_HIP(hipGetLastError());
     direct_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(
-        part, nb == 0 ? 1 : (int)nchunk, nt, KIND == PAIR_GAUSS_EPS ? PAIR_GAUSS_EPS : PAIR_OSEEN,
-        o0, o1);
+        part, nb == 0 ? 1 : (int)nchunk, nt,
+        (KIND == PAIR_GAUSS_EPS || KIND == PAIR_CUTOFF) ? KIND : PAIR_OSEEN, o0, o1);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
 
 int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, const double *yb,
                      const double *wb, int64_t nb, const double *xt, const double *yt, int64_t nt,
-                     double *w)
+                     double *w, bool cutoff)
 {
     if (nt == 0) return PHF_OK;
-    const PairParams pp = make_gauss_params(ksigmasqr);
+    PairParams pp = make_gauss_params(ksigmasqr);
+    if (cutoff) {  // sigma^2 takes the place of EPS (ww2par_C_cpu.c:140)
+        pp.eps = ksigmasqr;
+        pp.eps_f = (float)ksigmasqr;
+    }
     constexpr int TPL32 = 2, TPL64 = 1;
     const int tpl = prec == PHF_PREC_F32C ? TPL32 : TPL64;
     const int64_t tblock = (int64_t)kWave * tpl;
@@ -417,16 +429,26 @@ int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, cons
         Src32c *s;
         PHF_ALLOC(s, L->arena, Src32c, (size_t)nb);
         pack_src32c<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
-        vort_f32c<TPL32><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp, part);
+        if (cutoff)
+            vort_f32c<TPL32, true><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp,
+                                                                  part);
+        else
+            vort_f32c<TPL32, false><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
+                                                                   pp, part);
     } else {
         Src64 *s;
         PHF_ALLOC(s, L->arena, Src64, (size_t)nb);
         pack_src64<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
-        vort_f64<TPL64><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp, part);
+        if (cutoff)
+            vort_f64<TPL64, true><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp,
+                                                                 part);
+        else
+            vort_f64<TPL64, false><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
+                                                                  pp, part);
     }
     PHF_HIP(hipGetLastError());
     vort_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(part, nb == 0 ? 1 : (int)nchunk, nt,
-                                                         ksigmasqr, w);
+                                                         ksigmasqr, cutoff ? 1 : 0, w);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -438,6 +460,8 @@ int direct_device(Lane *L, int kind, int prec, const PairParams &pp, const doubl
     if (nt == 0) return PHF_OK;
     if (kind == PAIR_GAUSS_EPS)
         return launch_direct<PAIR_GAUSS_EPS>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
+    if (kind == PAIR_CUTOFF)
+        return launch_direct<PAIR_CUTOFF>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
     if (prec == PHF_PREC_F32C && pp.f32_nocut)
         return launch_direct<PAIR_OSEEN_NOCUT>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
     return launch_direct<PAIR_OSEEN>(L, prec, pp, xb, yb, wb, nb, xt, yt, nt, o0, o1);
@@ -448,11 +472,30 @@ int direct_device(Lane *L, int kind, int prec, const PairParams &pp, const doubl
 // ------------------------------------------------------------ C entry ----
 using namespace phf;
 
+static int vv2par_entry(int kind, const double *xb, const double *yb, const double *wb,
+                        int64_t nb, const double *xt, const double *yt, int64_t nt,
+                        double ksigmasqr, int32_t prec, double *vx, double *vy);
+
 extern "C" int phf_vv2par_gauss(const double *xb, const double *yb, const double *wb,
                                 int64_t nb, const double *xt, const double *yt, int64_t nt,
                                 double ksigmasqr, int32_t prec, double *vx, double *vy)
 {
     clear_error();
+    return vv2par_entry(PAIR_GAUSS_EPS, xb, yb, wb, nb, xt, yt, nt, ksigmasqr, prec, vx, vy);
+}
+
+extern "C" int phf_vv2par_cutoff(const double *xb, const double *yb, const double *wb,
+                                 int64_t nb, const double *xt, const double *yt, int64_t nt,
+                                 double sigmasqr, int32_t prec, double *vx, double *vy)
+{
+    clear_error();
+    return vv2par_entry(PAIR_CUTOFF, xb, yb, wb, nb, xt, yt, nt, sigmasqr, prec, vx, vy);
+}
+
+static int vv2par_entry(int kind, const double *xb, const double *yb, const double *wb,
+                        int64_t nb, const double *xt, const double *yt, int64_t nt,
+                        double ksigmasqr, int32_t prec, double *vx, double *vy)
+{
     PHF_REQUIRE(nb >= 0 && nt >= 0, "negative sizes (nb=%lld, nt=%lld)", (long long)nb,
                 (long long)nt);
     PHF_REQUIRE(prec == PHF_PREC_F64 || prec == PHF_PREC_F32C, "unknown precision %d", prec);
@@ -488,7 +531,11 @@ extern "C" int phf_vv2par_gauss(const double *xb, const double *yb, const double
     tm.end(PHF_STAGE_UPLOAD);
     tm.begin(PHF_STAGE_P2P);
     PairParams pp = make_gauss_params(ksigmasqr);
-    PHF_TRY(direct_device(L, PAIR_GAUSS_EPS, prec, pp, dxb, dyb, dwb, nb, dxt, dyt, nt, o0, o1));
+    if (kind == PAIR_CUTOFF) {  // r2 = dx^2 + dy^2 + sigma^2 (vv2par_C_cpu.c:144)
+        pp.eps = ksigmasqr;
+        pp.eps_f = (float)ksigmasqr;
+    }
+    PHF_TRY(direct_device(L, kind, prec, pp, dxb, dyb, dwb, nb, dxt, dyt, nt, o0, o1));
     tm.end(PHF_STAGE_P2P);
     tm.begin(PHF_STAGE_DOWNLOAD);
     PHF_TRY(download(L, o0, nt, vx));
@@ -500,11 +547,30 @@ extern "C" int phf_vv2par_gauss(const double *xb, const double *yb, const double
     return PHF_OK;
 }
 
+static int ww2par_entry(bool cutoff, const double *xb, const double *yb, const double *wb,
+                        int64_t nb, const double *xt, const double *yt, int64_t nt,
+                        double ksigmasqr, int32_t prec, double *w);
+
 extern "C" int phf_ww2par_gauss(const double *xb, const double *yb, const double *wb,
                                 int64_t nb, const double *xt, const double *yt, int64_t nt,
                                 double ksigmasqr, int32_t prec, double *w)
 {
     clear_error();
+    return ww2par_entry(false, xb, yb, wb, nb, xt, yt, nt, ksigmasqr, prec, w);
+}
+
+extern "C" int phf_ww2par_cutoff(const double *xb, const double *yb, const double *wb,
+                                 int64_t nb, const double *xt, const double *yt, int64_t nt,
+                                 double sigmasqr, int32_t prec, double *w)
+{
+    clear_error();
+    return ww2par_entry(true, xb, yb, wb, nb, xt, yt, nt, sigmasqr, prec, w);
+}
+
+static int ww2par_entry(bool cutoff, const double *xb, const double *yb, const double *wb,
+                        int64_t nb, const double *xt, const double *yt, int64_t nt,
+                        double ksigmasqr, int32_t prec, double *w)
+{
     PHF_REQUIRE(nb >= 0 && nt >= 0, "negative sizes (nb=%lld, nt=%lld)", (long long)nb,
                 (long long)nt);
     PHF_REQUIRE(prec == PHF_PREC_F64 || prec == PHF_PREC_F32C, "unknown precision %d", prec);
@@ -528,7 +594,7 @@ extern "C" int phf_ww2par_gauss(const double *xb, const double *yb, const double
     PHF_TRY(upload(L, xt, nt, &dxt));
     PHF_TRY(upload(L, yt, nt, &dyt));
     PHF_ALLOC(o, L->arena, double, (size_t)nt);
-    PHF_TRY(vorticity_device(L, prec, ksigmasqr, dxb, dyb, dwb, nb, dxt, dyt, nt, o));
+    PHF_TRY(vorticity_device(L, prec, ksigmasqr, dxb, dyb, dwb, nb, dxt, dyt, nt, o, cutoff));
     PHF_TRY(download(L, o, nt, w));
     PHF_HIP(hipStreamSynchronize(L->stream));
     return PHF_OK;

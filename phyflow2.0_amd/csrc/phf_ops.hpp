@@ -25,6 +25,6 @@ int direct_device(Lane *L, int kind, int prec, const PairParams &pp,
 // Gaussian induced vorticity (ww2parGauss) at the targets into w (device).
 int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, const double *yb,
                      const double *wb, int64_t nb, const double *xt, const double *yt, int64_t nt,
-                     double *w);
+                     double *w, bool cutoff = false);
 
 }  // namespace phf

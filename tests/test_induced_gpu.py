@@ -29,6 +29,30 @@ def test_velocity_direct_and_fmm(O):
 def test_velocity_unsupported():
     x = np.zeros(3)
     with pytest.raises(Exception):
-        velocity(x, x, x, 0.1, kernel=0, method=0)
-    with pytest.raises(NotImplementedError):
-        velocity(x, x, x, 0.1, kernel=0, method=1)
+        velocity(x, x, x, 0.1, kernel=0, method=0)  # FMM with the cutoff kernel (:158-163)
+    with pytest.raises(ValueError):
+        velocity(x, x, x, 0.1, kernel=7, method=1)
+
+
+@pytest.mark.parametrize("n,ne", [(1, 5), (3000, 2000), (20000, 4096)])
+def test_cutoff_kernel_velocity_and_vorticity(n, ne, O):
+    """Algebraic cutoff kernel (kernel 0): velocity and ww2par sum vs the CPU
+    restatements; float64 <= 1e-12 of max, F32C rel-L2 <= 1e-6."""
+    from phyflow_amd.blobs.base.induced import vorticity
+    from phyflow_amd.cpp.blobs.kernels.cutoff import vv2par_gpu, ww2par_gpu
+    rng = np.random.default_rng(n)
+    x, y = rng.uniform(-1, 1, (2, n))
+    w = rng.standard_normal(n) * 1e-3
+    xe, ye = rng.uniform(-1.2, 1.2, (2, ne))
+    sigma = 2.0 / max(np.sqrt(n), 8)
+    ref = O.vv2par_cutoff(x, y, w, xe, ye, sigma * sigma)
+    got = velocity(x, y, w, sigma, kernel=0, method=1, xEval=xe, yEval=ye, hardware=1)
+    scale = max(np.max(np.abs(ref[0])), np.max(np.abs(ref[1])))
+    assert max(np.max(np.abs(got[0] - ref[0])), np.max(np.abs(got[1] - ref[1]))) <= 1e-12 * scale
+    g32 = vv2par_gpu(x, y, w, xe, ye, sigma * sigma, 128, prec=1)
+    assert O.rel_l2(g32, ref) <= 1e-6
+    wr = O.ww2par_cutoff(x, y, w, xe, ye, sigma * sigma)
+    wg = vorticity(x, y, w, sigma, kernel=0, xEval=xe, yEval=ye)
+    assert np.max(np.abs(wg - wr)) <= 1e-12 * np.max(np.abs(wr))
+    w32 = ww2par_gpu(x, y, w, xe, ye, sigma * sigma, prec=1)
+    assert O.rel_l2(w32, wr) <= 1e-6

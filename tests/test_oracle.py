@@ -347,3 +347,35 @@ def test_tutty_diffusion_moments(O):
     k = np.argmax(np.abs(wo))
     assert io[k] == 7 and jo[k] == 3 and abs(wo[k] - 0.7) <= 1e-15
     assert np.max(np.abs(np.delete(wo, k))) <= 1e-15
+
+
+def test_cutoff_kernel_closed_forms(O):
+    """Algebraic cutoff kernel: single-blob closed forms for the velocity and
+    the ww2par sum, linearity, far field -> point vortex, pair antisymmetry."""
+    s2 = 0.03 ** 2
+    r = np.array([0.0, 1e-3, 0.02, 0.05, 0.3, 2.0])
+    th = np.linspace(0.2, 5.5, r.size)
+    xt, yt = 0.1 + r * np.cos(th), 0.2 + r * np.sin(th)
+    vx, vy = O.vv2par_cutoff([0.1], [0.2], [0.7], xt, yt, s2)
+    ut = 0.7 * r / (2 * math.pi * (r * r + s2))
+    np.testing.assert_allclose(vx, -ut * np.sin(th), rtol=1e-13, atol=1e-16)
+    np.testing.assert_allclose(vy, ut * np.cos(th), rtol=1e-13, atol=1e-16)
+    w = O.ww2par_cutoff([0.1], [0.2], [0.7], xt, yt, s2)
+    np.testing.assert_allclose(w, 0.7 / (2 * math.pi * np.sqrt(r * r + s2)), rtol=1e-14)
+    rng = np.random.default_rng(5)
+    xb, yb = rng.uniform(-0.2, 0.2, (2, 40))
+    g1, g2 = rng.standard_normal((2, 40))
+    a = O.vv2par_cutoff(xb, yb, g1 + 2 * g2, xt, yt, s2)
+    b1, b2 = O.vv2par_cutoff(xb, yb, g1, xt, yt, s2), O.vv2par_cutoff(xb, yb, g2, xt, yt, s2)
+    np.testing.assert_allclose(a[0], b1[0] + 2 * b2[0], rtol=1e-12, atol=1e-13)
+    # far field: sum of point vortices
+    fx, fy = np.array([30.0, -41.0]), np.array([12.0, 25.0])
+    u, v = O.vv2par_cutoff(xb, yb, g1, fx, fy, s2)
+    dx, dy = fx[:, None] - xb[None], fy[:, None] - yb[None]
+    r2 = dx * dx + dy * dy
+    np.testing.assert_allclose(u, np.sum(-dy * g1 / r2, 1) / (2 * math.pi), rtol=1e-5)
+    np.testing.assert_allclose(v, np.sum(dx * g1 / r2, 1) / (2 * math.pi), rtol=1e-5)
+    # a blob's velocity at another is minus the reverse (equal strengths)
+    p = O.vv2par_cutoff([0.0], [0.0], [1.0], [0.013], [-0.021], s2)
+    q = O.vv2par_cutoff([0.013], [-0.021], [1.0], [0.0], [0.0], s2)
+    assert abs(p[0][0] + q[0][0]) <= 1e-16 and abs(p[1][0] + q[1][0]) <= 1e-16

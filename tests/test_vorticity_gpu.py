@@ -3,7 +3,7 @@ restatement of ww2parGauss_C_cpu.c.
 
 float64: <= 1e-12 of max|w|; F32C: rel-L2 <= 1e-6.  Also the vorticity()
 dispatch (targets default to the blobs, FMM and the cutoff kernel raise)
-and empty inputs."""
+and empty inputs (the cutoff kernel is covered with the velocity tests)."""
 import numpy as np
 import pytest
 
@@ -40,5 +40,5 @@ def test_vorticity_dispatch_and_edges(O):
     assert ww2parGauss_gpu(c["x"], c["y"], c["w"], e, e, 0.01).size == 0
     with pytest.raises(Exception):
         vorticity(c["x"], c["y"], c["w"], c["sigma"], method=0)
-    with pytest.raises(NotImplementedError):
-        vorticity(c["x"], c["y"], c["w"], c["sigma"], kernel=0)
+    with pytest.raises(ValueError):
+        vorticity(c["x"], c["y"], c["w"], c["sigma"], kernel=5)
