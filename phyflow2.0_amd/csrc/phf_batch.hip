@@ -474,15 +474,16 @@ __global__ __launch_bounds__(256) void batch_p2p_f32s(
         const float4 tq = valid ? src[ts + (i - t0)] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float2 tp = make_float2(tq.x, tq.y);
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
+        float lam = 1.0f;
+        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb, lam);
         if (!wave_on) continue;
         p2p_merge_groups(ha, la, hb, lb);
         if (g == 0 && valid) {
             double fr = 0.0, fi = 0.0;
             if (Bb) l2p_f32(Bb, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
             const int32_t o = perm[i];
-            const double qr = -(((double)ha + (double)la) + fr);
-            const double qi = ((double)hb + (double)lb) - fi;
+            const double qr = -(((double)ha + (double)la) * (double)lam + fr);
+            const double qi = ((double)hb + (double)lb) * (double)lam - fi;
             const double two_pi = 2.0 * 3.141592653589793;
             vx[o] = -qi / two_pi;  // _induced.py:334-336
             vy[o] = -qr / two_pi;

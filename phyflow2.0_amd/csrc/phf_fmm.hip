@@ -269,6 +269,27 @@ __global__ __launch_bounds__(64) void l2l_lds_kernel(uint32_t nchild,
 }
 
 // ---------------------------------------------------------------- M2L ----
+// Levels of one M2L launch: every level >= 2 runs in the same grid (they
+// are independent once the upward pass is done), so the short serial link
+// chains of the coarse levels overlap with the work of the finest.
+constexpr int kM2LMaxLev = 12;
+struct M2LLevels {
+    int nlev, lev0;
+    int64_t blk[kM2LMaxLev + 1];   // first block of each level (grid.x)
+    int64_t coff[kM2LMaxLev];      // box offset of the level in the M / B arrays
+    int il_base[kM2LMaxLev];       // index of the level's 5 parity offsets
+    int il_max[kM2LMaxLev];        // longest interaction list of the level
+    int shift[kM2LMaxLev];         // 2 (L - l)
+    double inv_r[kM2LMaxLev];      // 1 / box side
+};
+
+__device__ __forceinline__ int m2l_level_of(const M2LLevels &lv)
+{
+    int li = 0;
+    while (li + 1 < lv.nlev && (int64_t)blockIdx.x >= lv.blk[li + 1]) ++li;
+    return li;
+}
+
 // One wave holds G = 64/p1 target boxes of one level; lane (g, m) owns
 // coefficient m of box g.  Per link the group forms A'_k = u^k A_k in LDS,
 // each lane contracts with its binomial row C(k+m, k) (registers), and
@@ -276,20 +297,28 @@ __global__ __launch_bounds__(64) void l2l_lds_kernel(uint32_t nchild,
 // groups of a wave mostly share one interaction list.
 template <class T, int P1>
 __global__ __launch_bounds__(64) void m2l_kernel(
-    int l, int p1, const typename C2<T>::type *__restrict__ Ml, typename C2<T>::type *__restrict__ Bl,
-    const int32_t *__restrict__ start_s, const int32_t *__restrict__ start_t, int shift,
-    const int2 *__restrict__ il_tau, const int32_t *__restrict__ il_off /* 5 */, int il_max,
-    const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [p1][p1] */,
-    T inv_r)
+    M2LLevels lv, int p1, const typename C2<T>::type *__restrict__ M,
+    typename C2<T>::type *__restrict__ B, const int32_t *__restrict__ start_s,
+    const int32_t *__restrict__ start_t, const int2 *__restrict__ il_tau,
+    const int32_t *__restrict__ il_offs, const typename C2<T>::type *__restrict__ U, int R,
+    const T *__restrict__ binom /* [p1][p1] */)
 {
     using V = typename C2<T>::type;
     __shared__ V Ap[64];
+    const int li = m2l_level_of(lv);
+    const int l = lv.lev0 + li;
+    const V *Ml = M + lv.coff[li] * p1;
+    V *Bl = B + lv.coff[li] * p1;
+    const int shift = lv.shift[li];
+    const int32_t *il_off = il_offs + lv.il_base[li];
+    const int il_max = lv.il_max[li];
+    const T inv_r = (T)lv.inv_r[li];
     const int lane = threadIdx.x;
     const int G = 64 / p1;
     const int g = lane / p1, m = lane - g * p1;
     const int n = 1 << l, half = n >> 1;
     const int64_t nbox = (int64_t)n * n;
-    const int64_t slot = (int64_t)blockIdx.x * G + g;
+    const int64_t slot = ((int64_t)blockIdx.x - lv.blk[li]) * G + g;
     bool active = g < G && slot < nbox;
     int bx = 0, by = 0, par = 0;
     uint32_t box = 0;
@@ -360,18 +389,6 @@ __global__ __launch_bounds__(64) void m2l_kernel(
 // registers, the factorisation and summation order of m2l_kernel, with no
 // LDS traffic or barriers; splitting m over waves gives the finest level
 // enough waves to hide the load latency.  Requires p1 == P1 (float32).
-// Levels of one M2L launch: every level >= 2 runs in the same grid (they
-// are independent once the upward pass is done), so the short serial link
-// chains of the coarse levels overlap with the work of the finest.
-constexpr int kM2LMaxLev = 12;
-struct M2LLevels {
-    int nlev, lev0;
-    int64_t blk[kM2LMaxLev + 1];   // first block of each level (grid.x)
-    int64_t coff[kM2LMaxLev];      // box offset of the level in the M / B arrays
-    int il_base[kM2LMaxLev];       // index of the level's 5 parity offsets
-    int shift[kM2LMaxLev];         // 2 (L - l)
-    float inv_r[kM2LMaxLev];       // 1 / box side
-};
 
 template <class T, int P1, int MW>
 __global__ __launch_bounds__(64) void m2l_box_kernel(
@@ -381,8 +398,7 @@ __global__ __launch_bounds__(64) void m2l_box_kernel(
     const typename C2<T>::type *__restrict__ U, int R, const T *__restrict__ binom /* [P1][P1] */)
 {
     using V = typename C2<T>::type;
-    int li = 0;
-    while (li + 1 < lv.nlev && (int64_t)blockIdx.x >= lv.blk[li + 1]) ++li;
+    const int li = m2l_level_of(lv);
     const int l = lv.lev0 + li;
     const typename C2<T>::type *Ml = M + lv.coff[li] * P1;
     typename C2<T>::type *Bl = B + lv.coff[li] * P1;
@@ -542,15 +558,16 @@ __global__ __launch_bounds__(256) void p2p_l2p_f32s(
             }
         }
         float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb);
+        float lam = 1.0f;
+        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb, lam);
         if (!wave_on) continue;
         p2p_merge_groups(ha, la, hb, lb);
         if (g == 0 && valid) {
             double fr = 0.0, fi = 0.0;
             if (Bleaf) l2p_f32(Bleaf + (int64_t)b * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
             const int32_t o = perm_t[i];
-            const double qr = -(((double)ha + (double)la) + fr);
-            const double qi = ((double)hb + (double)lb) - fi;
+            const double qr = -(((double)ha + (double)la) * (double)lam + fr);
+            const double qi = ((double)hb + (double)lb) * (double)lam - fi;
             if (velocity_out) {
                 // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
                 const double two_pi = 2.0 * 3.141592653589793;
@@ -912,30 +929,35 @@ static int shift_down(Lane *L, int p1, uint32_t nc, const void *Bp, void *Bc, co
     return PHF_OK;
 }
 
+// M2L of every level >= 2 in one launch: the generic coefficient-per-lane
+// kernel (G = 64 / p1 boxes per wave) for any order.
 template <class T, int P1>
-static int launch_m2l(Lane *L, int l, int p1, const void *M, void *B, const int32_t *ss,
-                      const int32_t *st, int shift, const int2 *il, const int32_t *off, int il_max,
-                      const void *U, int R, const void *binom, double inv_r)
+static int launch_m2l(Lane *L, M2LLevels lv, int p1, const void *M, void *B, const int32_t *ss,
+                      const int32_t *st, const int2 *il, const int32_t *offs, const void *U, int R,
+                      const void *binom)
 {
     using V = typename C2<T>::type;
-    const int64_t nbox = int64_t(1) << (2 * l);
     const int G = 64 / p1;
-    const unsigned grid = (unsigned)((nbox + G - 1) / G);
-    m2l_kernel<T, P1><<<grid, 64, 0, L->stream>>>(
-        l, p1, (const V *)M, (V *)B, ss, st, shift, il, off, il_max, (const V *)U, R,
-        (const T *)binom, (T)inv_r);
+    int64_t nb = 0;
+    lv.blk[0] = 0;
+    for (int i = 0; i < lv.nlev; ++i) {
+        const int64_t nbox = int64_t(1) << (2 * (lv.lev0 + i));
+        lv.blk[i + 1] = lv.blk[i] + (nbox + G - 1) / G;
+    }
+    nb = lv.blk[lv.nlev];
+    m2l_kernel<T, P1><<<(unsigned)nb, 64, 0, L->stream>>>(lv, p1, (const V *)M, (V *)B, ss, st, il,
+                                                          offs, (const V *)U, R, (const T *)binom);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
 
 template <class T>
-static int m2l_dispatch(Lane *L, int l, int p1, const void *M, void *B, const int32_t *ss,
-                        const int32_t *st, int shift, const int2 *il, const int32_t *off,
-                        int il_max, const void *U, int R, const void *binom, double inv_r)
+static int m2l_dispatch(Lane *L, const M2LLevels &lv, int p1, const void *M, void *B,
+                        const int32_t *ss, const int32_t *st, const int2 *il, const int32_t *offs,
+                        const void *U, int R, const void *binom)
 {
 #define PHF_M2L(P) \
-    if (p1 <= P)   \
-        return launch_m2l<T, P>(L, l, p1, M, B, ss, st, shift, il, off, il_max, U, R, binom, inv_r);
+    if (p1 <= P) return launch_m2l<T, P>(L, lv, p1, M, B, ss, st, il, offs, U, R, binom);
     PHF_M2L(9)
     PHF_M2L(13)
     PHF_M2L(17)
@@ -1114,21 +1136,24 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         // M2L at every level >= 2 (writes B)
         tm.begin(PHF_STAGE_M2L);
         used[PHF_STAGE_M2L] = true;
+        M2LLevels lv{};
+        lv.nlev = Lv - 1;
+        lv.lev0 = 2;
+        for (int l = 2; l <= Lv; ++l) {
+            const int i = l - 2;
+            lv.coff[i] = off[l];
+            lv.il_base[i] = il_base[l];
+            lv.il_max[i] = il_max[l];
+            lv.shift[i] = 2 * (Lv - l);
+            lv.inv_r[i] = 1.0 / (dom.D / std::ldexp(1.0, l));
+        }
         if (sizeof(T) == 4 && p1 == 17) {
-            // lane-per-box form, every level in one launch (float32, order 16)
+            // lane-per-box form (float32, order 16)
             constexpr int MW = 4;
-            M2LLevels lv{};
-            lv.nlev = Lv - 1;
-            lv.lev0 = 2;
             lv.blk[0] = 0;
-            for (int l = 2; l <= Lv; ++l) {
-                const int i = l - 2;
-                const int64_t per = (int64_t(1) << (2 * l)) / 4;
+            for (int i = 0; i < lv.nlev; ++i) {
+                const int64_t per = (int64_t(1) << (2 * (i + 2))) / 4;
                 lv.blk[i + 1] = lv.blk[i] + (per + 63) / 64;
-                lv.coff[i] = off[l];
-                lv.il_base[i] = il_base[l];
-                lv.shift[i] = 2 * (Lv - l);
-                lv.inv_r[i] = (float)(1.0 / (dom.D / std::ldexp(1.0, l)));
             }
             const dim3 grid((unsigned)lv.blk[lv.nlev], 4, (17 + MW - 1) / MW);
             m2l_box_kernel<float, 17, MW><<<grid, 64, 0, L->stream>>>(
@@ -1136,12 +1161,8 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
                 (const float2 *)d_U, R, (const float *)d_bin);
             PHF_HIP(hipGetLastError());
         } else {
-            for (int l = 2; l <= Lv; ++l) {
-                const double r = dom.D / std::ldexp(1.0, l);
-                PHF_TRY(m2l_dispatch<T>(L, l, p1, M + off[l] * p1, B + off[l] * p1, S.start,
-                                        Tg.start, 2 * (Lv - l), d_il, d_iloff + il_base[l],
-                                        il_max[l], d_U, R, d_bin, 1.0 / r));
-            }
+            PHF_TRY(m2l_dispatch<T>(L, lv, p1, M, B, S.start, Tg.start, d_il, d_iloff, d_U, R,
+                                    d_bin));
         }
         tm.end(PHF_STAGE_M2L);
         // L2L down

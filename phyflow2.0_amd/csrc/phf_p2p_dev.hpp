@@ -16,10 +16,36 @@ namespace phf {
 
 constexpr int P2P_STAGE = 1536;  // staged sources (float4) per batch
 
+// Pair op in scaled coordinates x' = lam x with lam^2 = shape log2(e): the
+// smoothing factor is exp2(-r2') (the negation is a free source modifier),
+// one multiply per pair fewer than pair_f32; sums come out divided by lam.
+// CORE = false: exp(-shape r2) <= 2^-26 for the pair, so w (1 - e) rounds to
+// w and the exponential is dropped (as pair_f32_far).
+template <int KIND, bool CORE>
+__device__ __forceinline__ float pair_scaled(float dx, float dy, float w, float c2s,
+                                             float scale_f)
+{
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, 1e-30f));
+    if (CORE) {
+        const float e = exp2_f32(-r2);
+        if (KIND == PAIR_OSEEN) {
+            const float ws = w * scale_f;
+            const float num = r2 < c2s ? __builtin_fmaf(-ws, e, ws) : w;
+            return num * rcp_f32(r2);
+        }
+        return __builtin_fmaf(-w, e, w) * rcp_f32(r2);
+    }
+    if (KIND == PAIR_OSEEN) {
+        const float num = r2 < c2s ? w * scale_f : w;
+        return num * rcp_f32(r2);
+    }
+    return w * rcp_f32(r2);
+}
+
 template <int KIND, bool CORE>
 __device__ __forceinline__ void p2p_seg_f32(const float4 *__restrict__ sq, int j0, int j1,
-                                            float xt, float yt, const PairParams &pp, float &ha,
-                                            float &la, float &hb, float &lb)
+                                            float xt, float yt, float c2s, float scale_f,
+                                            float &ha, float &la, float &hb, float &lb)
 {
     int j = j0;
     for (; j + 64 <= j1; j += 64) {
@@ -28,7 +54,7 @@ __device__ __forceinline__ void p2p_seg_f32(const float4 *__restrict__ sq, int j
         for (int k = 0; k < 16; ++k) {
             const float4 q = sq[j + 4 * k];
             const float dx = xt - q.x, dy = yt - q.y;
-            const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
+            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
             aa = __builtin_fmaf(P, dx, aa);
             ab = __builtin_fmaf(P, dy, ab);
         }
@@ -42,7 +68,7 @@ __device__ __forceinline__ void p2p_seg_f32(const float4 *__restrict__ sq, int j
             for (int k = 0; k < 4; ++k) {
                 const float4 q = sq[j + 4 * k];
                 const float dx = xt - q.x, dy = yt - q.y;
-                const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
+                const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
                 aa = __builtin_fmaf(P, dx, aa);
                 ab = __builtin_fmaf(P, dy, ab);
             }
@@ -50,7 +76,7 @@ __device__ __forceinline__ void p2p_seg_f32(const float4 *__restrict__ sq, int j
         for (; j < j1; j += 4) {
             const float4 q = sq[j];
             const float dx = xt - q.x, dy = yt - q.y;
-            const float P = CORE ? pair_f32<KIND>(dx, dy, q.z, pp) : pair_f32_far<KIND>(dx, dy, q.z, pp);
+            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
             aa = __builtin_fmaf(P, dx, aa);
             ab = __builtin_fmaf(P, dy, ab);
         }
@@ -95,7 +121,8 @@ __device__ __forceinline__ void l2p_f32(const float2 *__restrict__ Bb, int p1, f
     fi = pi;
 }
 
-// Accumulate the near-field sum of target tp (offset from its leaf centre)
+// Accumulate the near-field sum of target tp (offset from its leaf centre),
+// in scaled coordinates: the true sums are lam_out times (ha + la), (hb + lb)
 // over every near leaf nf names: nf(q, sb, ox, oy, core) gives near index
 // q's source leaf sb, its centre shift (exact in float32) and whether its
 // pairs may lie inside the smoothing core; false: no such leaf.  Near
@@ -109,9 +136,15 @@ __device__ __forceinline__ void p2p_staged(const float4 *__restrict__ src,
                                            const int32_t *__restrict__ sp, float4 *st,
                                            int nnear, const NearFn &nf,
                                            bool wave_on, float2 tp, const PairParams &pp,
-                                           float &ha, float &la, float &hb, float &lb)
+                                           float &ha, float &la, float &hb, float &lb,
+                                           float &lam_out)
 {
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, g = lane >> 4;
+    // scaled coordinates (pair_scaled): sources are scaled while staged
+    const float lam = __builtin_sqrtf(-pp.nlog2e_ks_f);
+    const float c2s = pp.c2_f * (lam * lam);
+    const float txs = tp.x * lam, tys = tp.y * lam;
+    lam_out = lam;
     int q = 0, jq = 0;  // cursor: near index, position within its range
     while (q < nnear) {
         const int ql = q + lane;
@@ -154,23 +187,24 @@ __device__ __forceinline__ void p2p_staged(const float4 *__restrict__ src,
                 for (int u = 0; u < 8; ++u) r[u] = gs[e + 4 * u];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
-                    r[u].x += ox;
-                    r[u].y += oy;
+                    r[u].x = (r[u].x + ox) * lam;
+                    r[u].y = (r[u].y + oy) * lam;
                     ls[e + 4 * u] = r[u];
                 }
             }
             for (; e < take; e += 4) {
                 float4 r = gs[e];
-                r.x += ox;
-                r.y += oy;
+                r.x = (r.x + ox) * lam;
+                r.y = (r.y + oy) * lam;
                 ls[e] = r;
             }
         }
         __syncthreads();
         if (wave_on) {
             const float4 *sq = st + g;
-            p2p_seg_f32<KIND, true>(sq, 0, core_end, tp.x, tp.y, pp, ha, la, hb, lb);
-            p2p_seg_f32<KIND, false>(sq, core_end, used, tp.x, tp.y, pp, ha, la, hb, lb);
+            p2p_seg_f32<KIND, true>(sq, 0, core_end, txs, tys, c2s, pp.scale_f, ha, la, hb, lb);
+            p2p_seg_f32<KIND, false>(sq, core_end, used, txs, tys, c2s, pp.scale_f, ha, la, hb,
+                                     lb);
         }
         __syncthreads();
     }
