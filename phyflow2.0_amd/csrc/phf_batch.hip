@@ -58,11 +58,7 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ arr, int S, 
 
 __device__ __forceinline__ bool near_pred(int dx, int dy, int ws, double s, double c2)
 {
-    const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy;
-    if (ax <= ws && ay <= ws) return true;
-    const double gx = (double)(ax > 1 ? ax - 1 : 0) * s;
-    const double gy = (double)(ay > 1 ? ay - 1 : 0) * s;
-    return gx * gx + gy * gy < c2;
+    return near_offset(dx, dy, ws, s, c2);
 }
 
 __device__ __forceinline__ int near_radius(int ws, double s, double c2)
@@ -76,10 +72,13 @@ __device__ __forceinline__ int near_radius(int ws, double s, double c2)
 __device__ __forceinline__ int floordiv2(int v) { return v >= 0 ? v / 2 : -((1 - v) / 2); }
 
 // ------------------------------------------------------- domains, keys ----
+// Per-system domain: bounding box, side from choose_domain_side (the rule
+// the single-system FMM applies, so a system builds the same tree alone and
+// in the batch).
 __global__ __launch_bounds__(256) void batch_domain(const BatchSys *__restrict__ sys,
                                                     const double *__restrict__ x,
-                                                    const double *__restrict__ y,
-                                                    BatchDom *__restrict__ dom)
+                                                    const double *__restrict__ y, int ws, int p1,
+                                                    int chunk, BatchDom *__restrict__ dom)
 {
     __shared__ double4 sh[256];
     const int s = blockIdx.x;
@@ -108,13 +107,10 @@ __global__ __launch_bounds__(256) void batch_domain(const BatchSys *__restrict__
         if (n == 0) b = make_double4(0.0, 0.0, 0.0, 0.0);
         d.x0 = b.x;
         d.y0 = b.z;
-        double ext = fmax(b.y - b.x, b.w - b.z);
-        if (!(ext > 0.0)) ext = 1.0;
-        int e = 0;
-        frexp(ext, &e);
-        const double unit = ldexp(1.0, e - 5);
-        d.D = ceil(ext / unit) * unit;
-        const int L = sys[s].L;
+        const BatchSys sy = sys[s];
+        const int L = sy.L;
+        d.D = choose_domain_side(b.y - b.x, b.w - b.z, L, (double)n, (double)n, sy.cutoff, ws,
+                                 1.4426950408889634074 * sy.inv_ks, p1, chunk);
         d.s = d.D / ldexp(1.0, L);
         d.inv_s = ldexp(1.0, L) / d.D;
         dom[s] = d;
@@ -427,24 +423,18 @@ struct BatchNear {
     }
 };
 
-// One workgroup of 4 waves per leaf of the whole batch, sources staged in
-// LDS (phf_p2p_dev.hpp); targets are the system's own blobs, read from the
-// padded source array.  Leaves are dealt so that each XCD receives a
-// contiguous range.
+// P2P + L2P of leaf gl of the whole batch (all threads of the workgroup):
+// sources staged in LDS (p2p_leaf); targets are the system's own blobs,
+// read from the padded source array.
 template <int KIND>
-__global__ __launch_bounds__(256) void batch_p2p_f32s(
+__device__ __forceinline__ void batch_p2p_leaf(
     const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
     const int64_t *__restrict__ lvl, int S, const BatchDom *__restrict__ dom,
     const float4 *__restrict__ src, const int32_t *__restrict__ start,
     const int32_t *__restrict__ sp, const int32_t *__restrict__ perm, int ws,
     const float2 *__restrict__ Bc, int p1, double *__restrict__ vx, double *__restrict__ vy,
-    int32_t *__restrict__ flag, int64_t nleaf, uint32_t nl8, const int2 *__restrict__ win)
+    int32_t *__restrict__ flag, const int2 *__restrict__ win, int64_t gl, float4 *st, float4 *acc)
 {
-    __shared__ float4 st[P2P_STAGE];
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    const int g = lane >> 4, t = lane & 15;
-    const int64_t gl = (int64_t)(blockIdx.x & 7u) * nl8 + (blockIdx.x >> 3);
-    if (gl >= nleaf) return;
     const int32_t t0 = start[gl], t1 = start[gl + 1];
     if (t1 <= t0) return;
     const int s = find_seg(loff, S, gl);
@@ -455,7 +445,7 @@ __global__ __launch_bounds__(256) void batch_p2p_f32s(
     const float inv_s = 1.0f / s_f;
     int R = near_radius(ws, sd, sy.c2);
     if (R > kRMax) {
-        if (tid == 0) atomicOr(flag, 1);
+        if (threadIdx.x == 0) atomicOr(flag, 1);
         R = kRMax;
     }
     PairParams pp{};
@@ -467,28 +457,45 @@ __global__ __launch_bounds__(256) void batch_p2p_f32s(
     const int nnear = (2 * kRMax + 1) * (2 * kRMax + 1);
     const float2 *Bb = (sy.L >= 2 && Bc) ? Bc + (lvl[sy.L * (S + 1) + s] + b) * p1 : nullptr;
     const int32_t ts = sp[gl];
-    for (int32_t rc = t0; rc < t1; rc += 64) {
-        const int32_t i = rc + 16 * wv + t;
-        const bool valid = i < t1;
-        const bool wave_on = rc + 16 * wv < t1;
-        const float4 tq = valid ? src[ts + (i - t0)] : make_float4(0.f, 0.f, 0.f, 0.f);
-        const float2 tp = make_float2(tq.x, tq.y);
-        float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        float lam = 1.0f;
-        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb, lam);
-        if (!wave_on) continue;
-        p2p_merge_groups(ha, la, hb, lb);
-        if (g == 0 && valid) {
-            double fr = 0.0, fi = 0.0;
-            if (Bb) l2p_f32(Bb, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
-            const int32_t o = perm[i];
-            const double qr = -(((double)ha + (double)la) * (double)lam + fr);
-            const double qi = ((double)hb + (double)lb) * (double)lam - fi;
-            const double two_pi = 2.0 * 3.141592653589793;
-            vx[o] = -qi / two_pi;  // _induced.py:334-336
-            vy[o] = -qr / two_pi;
-        }
-    }
+    const float lam = __builtin_sqrtf(-pp.nlog2e_ks_f);
+    auto tpos = [&](int32_t i) {
+        const float4 q = src[ts + (i - t0)];
+        return make_float2(q.x, q.y);
+    };
+    auto out = [&](int32_t i, float ha, float la, float hb, float lb) {
+        double fr = 0.0, fi = 0.0;
+        const float2 tp = tpos(i);
+        if (Bb) l2p_f32(Bb, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
+        const int32_t o = perm[i];
+        const double qr = -(((double)ha + (double)la) * (double)lam + fr);
+        const double qi = ((double)hb + (double)lb) * (double)lam - fi;
+        const double two_pi = 2.0 * 3.141592653589793;
+        vx[o] = -qi / two_pi;  // _induced.py:334-336
+        vy[o] = -qr / two_pi;
+    };
+    p2p_leaf<KIND>(src, sp, st, acc, nnear, nf, t0, t1, tpos, pp, lam, out);
+}
+
+// One workgroup of 4 waves per leaf; workgroups are dealt round-robin over
+// the 8 XCDs and those of XCD k walk the Morton leaf range [split[k],
+// split[k+1]) holding an eighth of the batch's blobs (equal work per XCD,
+// empty leaves skipped).
+template <int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void batch_p2p_f32s(
+    const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
+    const int64_t *__restrict__ lvl, int S, const BatchDom *__restrict__ dom,
+    const float4 *__restrict__ src, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ sp, const int32_t *__restrict__ perm, int ws,
+    const float2 *__restrict__ Bc, int p1, double *__restrict__ vx, double *__restrict__ vy,
+    int32_t *__restrict__ flag, const int32_t *__restrict__ split, const int2 *__restrict__ win)
+{
+    __shared__ float4 st[P2P_STAGE];
+    __shared__ float4 acc[P2P_CHUNKS * 4 * 16];
+    const int32_t hi = split[(blockIdx.x & 7u) + 1];
+    for (int32_t gl = split[blockIdx.x & 7u] + (int32_t)(blockIdx.x >> 3); gl < hi;
+         gl += (int32_t)(gridDim.x >> 3))
+        batch_p2p_leaf<KIND>(sys, loff, lvl, S, dom, src, start, sp, perm, ws, Bc, p1, vx, vy,
+                             flag, win, gl, st, acc);
 }
 
 template <int KIND>
@@ -634,6 +641,7 @@ int batch_plan(const int64_t *sizes, const double *sigma, int S, int p, int leaf
         const double ks = k * sigma[s] * sigma[s];
         const double xopt = std::sqrt(1.2564312086261696770 * ks);
         const PairParams pp = make_oseen_params(xopt, 5.0 * xopt);
+        b.cutoff = 5.0 * xopt;
         b.c2 = pp.c2;
         b.inv_ks = pp.inv_ks;
         b.scale = pp.scale;
@@ -674,6 +682,7 @@ int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
     const size_t N = (size_t)std::max<int64_t>(P.N, 1);
     PHF_ALLOC(Bf->sys, L->arena, BatchSys, (size_t)P.S);
     PHF_ALLOC(Bf->dom, L->arena, BatchDom, (size_t)P.S);
+    PHF_ALLOC(Bf->split, L->arena, int32_t, 9);
     PHF_ALLOC(Bf->poff, L->arena, int64_t, (size_t)P.S + 1);
     PHF_ALLOC(Bf->loff, L->arena, int64_t, (size_t)P.S + 1);
     PHF_ALLOC(Bf->lvl, L->arena, int64_t, P.lvl.size());
@@ -789,7 +798,8 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
     const int S = P.S, p1 = P.p + 1;
     const int64_t N = P.N;
     batch_clear_flag<<<1, 1, 0, L->stream>>>(Bf.flag);
-    batch_domain<<<S, 256, 0, L->stream>>>(Bf.sys, x, y, Bf.dom);
+    batch_domain<<<S, 256, 0, L->stream>>>(Bf.sys, x, y, P.ws, P.p + 1,
+                                           P.prec == PHF_PREC_F32C ? 16 : 64, Bf.dom);
     if (N > 0)
         batch_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.poff, S, Bf.dom, x, y, N,
                                                           Bf.k0, Bf.v0);
@@ -844,14 +854,15 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
     if (P.nleaf > 0) {
         if (sizeof(T) == 4) {
             const uint32_t nl8 = (uint32_t)((P.nleaf + 7) / 8);
+            PHF_TRY(xcd_split(L, Bf.start, P.nleaf, Bf.split));
             if (P.nocut_all)
                 batch_p2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, L->stream>>>(
                     Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, sp, vs,
-                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8, Bf.win);
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, Bf.split, Bf.win);
             else
                 batch_p2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, L->stream>>>(
                     Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const float4 *)Bf.src, Bf.start, sp, vs,
-                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, P.nleaf, nl8, Bf.win);
+                    P.ws, (const float2 *)Bc, p1, vx, vy, Bf.flag, Bf.split, Bf.win);
         } else {
             batch_p2p_f64<PAIR_OSEEN><<<(unsigned)P.nleaf, 64, 0, L->stream>>>(
                 Bf.sys, Bf.loff, Bf.lvl, S, Bf.dom, (const double4 *)Bf.src, Bf.start, vs, P.ws,

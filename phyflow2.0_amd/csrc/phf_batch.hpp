@@ -12,7 +12,7 @@ struct BatchSys {
     int64_t off;       // first point in the packed arrays
     int32_t n, L;      // points, leaf level
     int64_t leafbase;  // first global leaf
-    double c2, inv_ks, scale;            // Oseen smoother (float64)
+    double cutoff, c2, inv_ks, scale;    // Oseen smoother (float64)
     float c2_f, nlog2e_ks_f, scale_f;    // float32 forms
     int32_t nocut;                       // float32 cutoff branch is a no-op
 };
@@ -33,6 +33,7 @@ struct BatchPlan {
 struct BatchBuffers {
     BatchSys *sys = nullptr;
     BatchDom *dom = nullptr;
+    int32_t *split = nullptr;  // leaf ranges of the 8 XCDs (xcd_split)
     int64_t *poff = nullptr, *loff = nullptr, *lvl = nullptr;
     uint32_t *k0 = nullptr, *k1 = nullptr;
     int32_t *v0 = nullptr, *v1 = nullptr;

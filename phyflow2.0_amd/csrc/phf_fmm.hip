@@ -492,20 +492,17 @@ __global__ void gather_src_f32p(const double *__restrict__ x, const double *__re
         make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
 }
 
-// One workgroup of 4 waves per leaf.  The sources of the leaf's near leaves
-// are staged in LDS (as many near leaves, or parts of one, as fit: a
-// cursor walks the near list), then wave v takes the targets
-// [64 r + 16 v, 64 r + 16 v + 16) of round r; lane = 16 g + t, group g
-// sums staged sources g, g+4, g+8, ... for target t (chunks of 16 keep
-// ~90% of lanes busy at 64 points per leaf, against ~68% for one target per
-// lane).  Per near leaf each lane forms plain float32 partials over at most
-// 16 pairs, folded into a TwoSum (hi, lo) accumulator; the four groups are
-// combined by a fixed butterfly.  Near offsets [0, ninner) may hold pairs
-// inside the smoothing core; for the rest exp(-shape r2) <= 2^-26 for every
-// pair (checked on the host from the box gap), so their pairs skip the
-// exponential (bit-identical P).  Leaves are assigned so that the
-// workgroups one XCD receives (they are dealt round-robin over the 8 XCDs)
-// cover a contiguous range of Morton leaves and share sources in its L2.
+// One workgroup of 4 waves per leaf (p2p_leaf): the sources of the leaf's
+// near leaves are staged in LDS, every wave sweeps a quarter of them for
+// every target, and per-wave TwoSum partials are merged in a fixed order.
+// Near offsets [0, ninner) may hold pairs inside the smoothing core; for
+// the rest exp(-shape r2) <= 2^-26 for every pair (checked on the host from
+// the box gap), so their pairs skip the exponential (bit-identical P).
+// Workgroups are dealt round-robin over the 8 XCDs; those of XCD k walk
+// the contiguous Morton leaf range [split[k], split[k+1]), which holds an
+// eighth of the targets, so the XCDs carry equal work (empty leaves of a
+// square domain around an uneven cloud cost nothing) and neighbouring
+// leaves share sources in the XCD's L2.
 // near set of a single-system leaf: the host-built offset list, offsets
 // [0, ninner) first
 struct NearList {
@@ -526,60 +523,67 @@ struct NearList {
     }
 };
 
+// P2P + L2P of target leaf b (all threads of the workgroup).
 template <int KIND>
-__global__ __launch_bounds__(256) void p2p_l2p_f32s(
+__device__ __forceinline__ void p2p_leaf_f32s(
     const float4 *__restrict__ src, const int32_t *__restrict__ sp,
     const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
-    const int32_t *__restrict__ perm_t, int Lvl, uint32_t nl8, const int2 *__restrict__ near,
-    int nnear, int ninner, PairParams pp, float s_f, const float2 *__restrict__ Bleaf, int p1,
-    double *__restrict__ out_a, double *__restrict__ out_b, int velocity_out)
+    const int32_t *__restrict__ perm_t, int Lvl, uint32_t b, const int2 *__restrict__ near,
+    int nnear, int ninner, const PairParams &pp, float s_f, const float2 *__restrict__ Bleaf,
+    int p1, double *__restrict__ out_a, double *__restrict__ out_b, int velocity_out, float4 *st,
+    float4 *acc)
 {
-    __shared__ float4 st[P2P_STAGE];
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    const int g = lane >> 4, t = lane & 15;
-    const uint32_t b = (blockIdx.x & 7u) * nl8 + (blockIdx.x >> 3);
-    const int n = 1 << Lvl;
-    if (b >= (uint32_t)n * (uint32_t)n) return;
     const int32_t t0 = start_t[b], t1 = start_t[b + 1];
     if (t1 <= t0) return;
+    const int n = 1 << Lvl;
     const NearList nf{near, ninner, (int)compact_bits(b), (int)compact_bits(b >> 1), n, s_f};
     const float inv_s = 1.0f / s_f;
-    for (int32_t rc = t0; rc < t1; rc += 64) {
-        const int32_t i = rc + 16 * wv + t;
-        const bool valid = i < t1;
-        const bool wave_on = rc + 16 * wv < t1;
-        float2 tp = make_float2(0.f, 0.f);
-        if (valid) {
-            if (tgt) {
-                tp = tgt[i];
-            } else {  // targets are the sources: read them from the padded array
-                const float4 q = src[sp[b] + (i - t0)];
-                tp = make_float2(q.x, q.y);
-            }
+    // scaled coordinates: lam^2 = shape log2(e)
+    const float lam = __builtin_sqrtf(-pp.nlog2e_ks_f);
+    const int32_t ts = sp[b];
+    auto tpos = [&](int32_t i) {
+        if (tgt) return tgt[i];
+        const float4 q = src[ts + (i - t0)];  // targets are the sources (padded array)
+        return make_float2(q.x, q.y);
+    };
+    auto out = [&](int32_t i, float ha, float la, float hb, float lb) {
+        double fr = 0.0, fi = 0.0;
+        const float2 tp = tpos(i);
+        if (Bleaf) l2p_f32(Bleaf + (int64_t)b * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
+        const int32_t o = perm_t[i];
+        const double qr = -(((double)ha + (double)la) * (double)lam + fr);
+        const double qi = ((double)hb + (double)lb) * (double)lam - fi;
+        if (velocity_out) {
+            // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
+            const double two_pi = 2.0 * 3.141592653589793;
+            out_a[o] = -qi / two_pi;
+            out_b[o] = -qr / two_pi;
+        } else {
+            out_a[o] = qr;
+            out_b[o] = qi;
         }
-        float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-        float lam = 1.0f;
-        p2p_staged<KIND>(src, sp, st, nnear, nf, wave_on, tp, pp, ha, la, hb, lb, lam);
-        if (!wave_on) continue;
-        p2p_merge_groups(ha, la, hb, lb);
-        if (g == 0 && valid) {
-            double fr = 0.0, fi = 0.0;
-            if (Bleaf) l2p_f32(Bleaf + (int64_t)b * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
-            const int32_t o = perm_t[i];
-            const double qr = -(((double)ha + (double)la) * (double)lam + fr);
-            const double qi = ((double)hb + (double)lb) * (double)lam - fi;
-            if (velocity_out) {
-                // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
-                const double two_pi = 2.0 * 3.141592653589793;
-                out_a[o] = -qi / two_pi;
-                out_b[o] = -qr / two_pi;
-            } else {
-                out_a[o] = qr;
-                out_b[o] = qi;
-            }
-        }
-    }
+    };
+    p2p_leaf<KIND>(src, sp, st, acc, nnear, nf, t0, t1, tpos, pp, lam, out);
 }
+
+template <int KIND>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p2p_l2p_f32s(
+    const float4 *__restrict__ src, const int32_t *__restrict__ sp,
+    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
+    const int32_t *__restrict__ perm_t, int Lvl, const int32_t *__restrict__ split,
+    const int2 *__restrict__ near, int nnear, int ninner, PairParams pp, float s_f,
+    const float2 *__restrict__ Bleaf, int p1, double *__restrict__ out_a,
+    double *__restrict__ out_b, int velocity_out)
+{
+    __shared__ float4 st[P2P_STAGE];
+    __shared__ float4 acc[P2P_CHUNKS * 4 * 16];
+    const int32_t hi = split[(blockIdx.x & 7u) + 1];
+    for (int32_t bl = split[blockIdx.x & 7u] + (int32_t)(blockIdx.x >> 3); bl < hi;
+         bl += (int32_t)(gridDim.x >> 3))
+        p2p_leaf_f32s<KIND>(src, sp, tgt, start_t, perm_t, Lvl, (uint32_t)bl, near, nnear, ninner,
+                            pp, s_f, Bleaf, p1, out_a, out_b, velocity_out, st, acc);
+}
+
 
 template <int KIND>
 __global__ __launch_bounds__(64) void p2p_l2p_f64(
@@ -721,6 +725,32 @@ __global__ void padded_counts(const int32_t *__restrict__ start, int64_t nleaf,
     cnt[b] = b < nleaf ? (uint32_t)((start[b + 1] - start[b] + 3) & ~3) : 0u;
 }
 
+// Leaf ranges of the 8 XCDs holding equal numbers of targets: split[k]
+// (k = 0..8) is the leaf that holds target k ceil(nt/8), nleaf when there is
+// none.  Launched with nleaf + 1 threads; every entry has one writer.
+__global__ void xcd_leaf_split(const int32_t *__restrict__ start, int64_t nleaf,
+                               int32_t *__restrict__ split)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nleaf) return;
+    const int64_t nt = start[nleaf];
+    const int64_t T = nt > 0 ? (nt + 7) / 8 : 1;
+    if (b == nleaf) {
+        for (int k = 0; k <= 8; ++k)
+            if (k == 8 || k * T >= nt) split[k] = (int32_t)nleaf;
+        return;
+    }
+    const int64_t t0 = start[b], t1 = start[b + 1];
+    for (int64_t k = (t0 + T - 1) / T; k < 8 && k * T < t1; ++k) split[k] = (int32_t)b;
+}
+
+int xcd_split(Lane *L, const int32_t *start, int64_t nleaf, int32_t *split)
+{
+    xcd_leaf_split<<<div_up(nleaf + 1, 256), 256, 0, L->stream>>>(start, nleaf, split);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
 __global__ void zero_pads_f32(const int32_t *__restrict__ start, const int32_t *__restrict__ sp,
                               int64_t nleaf, float4 *__restrict__ out)
 {
@@ -769,10 +799,7 @@ std::vector<int2> near_set(double s, double c2, int ws, int Rwin)
     std::vector<int2> out;
     for (int dy = -Rwin; dy <= Rwin; ++dy)
         for (int dx = -Rwin; dx <= Rwin; ++dx) {
-            double gx = std::max(std::abs(dx) - 1, 0) * s;
-            double gy = std::max(std::abs(dy) - 1, 0) * s;
-            if ((std::abs(dx) <= ws && std::abs(dy) <= ws) || gx * gx + gy * gy < c2)
-                out.push_back(make_int2(dx, dy));
+            if (near_offset(dx, dy, ws, s, c2)) out.push_back(make_int2(dx, dy));
         }
     return out;
 }
@@ -784,16 +811,19 @@ int near_radius(const std::vector<int2> &v)
     return r;
 }
 
+int near_window(double s, double c2, int ws, int l)
+{
+    return std::min(std::max(ws, (int)std::ceil(std::sqrt(c2) / s) + 2), 1 << l);
+}
+
 Lists make_lists(const Domain &dom, double c2, int ws)
 {
     Lists ls;
     const int Lv = dom.L;
     std::vector<std::vector<int2>> nearl(Lv + 1);
     for (int l = 0; l <= Lv; ++l) {
-        double s = dom.D / std::ldexp(1.0, l);
-        int Rw = std::max(ws, (int)std::ceil(std::sqrt(c2) / s) + 2);
-        Rw = std::min(Rw, (1 << l));
-        nearl[l] = near_set(s, c2, ws, Rw);
+        const double s = dom.D / std::ldexp(1.0, l);
+        nearl[l] = near_set(s, c2, ws, near_window(s, c2, ws, l));
     }
     ls.near_leaf = nearl[Lv];
     ls.il.resize(Lv + 1);
@@ -825,6 +855,23 @@ Lists make_lists(const Domain &dom, double c2, int ws)
         }
     }
     return ls;
+}
+
+// Leaf level L (coarsened while the leaf near set reaches beyond 6 boxes,
+// for cutoffs large against the box) and the domain side from
+// choose_domain_side.
+Domain choose_domain(const double4 &hb, int L, double nb, double nt, double cutoff, int ws,
+                     double shape_log2, int p1, int chunk)
+{
+    const double c2 = cutoff * cutoff;
+    for (;;) {
+        const double side = choose_domain_side(hb.y - hb.x, hb.w - hb.z, L, nb, nt, cutoff, ws,
+                                               shape_log2, p1, chunk);
+        const Domain d = make_domain(hb.x, hb.y, hb.z, hb.w, L, side);
+        if (L == 0 || near_radius(near_set(d.s, c2, ws, near_window(d.s, c2, ws, L))) <= 6)
+            return d;
+        --L;
+    }
 }
 
 // All small host tables of one call are packed into the lane's pinned
@@ -1030,14 +1077,8 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     {
         const double shape_log2 = 1.4426950408889634074 * pp.inv_ks;
         std::vector<int2> far;
-        for (const int2 &o : ls.near_leaf) {
-            const double gx = std::max(std::abs(o.x) - 1, 0) * dom.s * 0.99;
-            const double gy = std::max(std::abs(o.y) - 1, 0) * dom.s * 0.99;
-            if (shape_log2 * (gx * gx + gy * gy) >= 26.0)
-                far.push_back(o);
-            else
-                near_ord.push_back(o);
-        }
+        for (const int2 &o : ls.near_leaf)
+            (near_offset_core(o.x, o.y, dom.s, shape_log2) ? near_ord : far).push_back(o);
         ninner = (int)near_ord.size();
         near_ord.insert(near_ord.end(), far.begin(), far.end());
     }
@@ -1190,16 +1231,19 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     const int vout = a.velocity_out ? 1 : 0;
     if (sizeof(T) == 4) {
         const uint32_t nl8 = (nleaf + 7) / 8;
+        int32_t *split;
+        PHF_ALLOC(split, L->arena, int32_t, 9);
+        PHF_TRY(xcd_split(L, Tg.start, nleaf, split));
         if (pp.f32_nocut)
             p2p_l2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, L->stream>>>(
-                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, nl8, d_near,
-                (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr,
-                a.qi, vout);
+                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, split,
+                d_near, (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf,
+                p1, a.qr, a.qi, vout);
         else
             p2p_l2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, L->stream>>>(
-                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, nl8, d_near,
-                (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf, p1, a.qr,
-                a.qi, vout);
+                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, split,
+                d_near, (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf,
+                p1, a.qr, a.qi, vout);
     } else {
         p2p_l2p_f64<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
             (const double4 *)src, S.start, (const double2 *)tgt, Tg.start, perm_t, Lv, d_near,
@@ -1224,15 +1268,11 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
     PHF_HIP(hipMemcpyAsync(&hb, bb, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
     const double c2 = a.cutoff * a.cutoff;
-    int Lv = a.Lhint;
-    Domain dom = make_domain(hb.x, hb.y, hb.z, hb.w, Lv);
-    Lists ls = make_lists(dom, c2, a.ws);
-    // keep the leaf near set bounded (large cutoffs relative to the box)
-    while (Lv > 0 && near_radius(ls.near_leaf) > 6) {
-        --Lv;
-        dom = make_domain(hb.x, hb.y, hb.z, hb.w, Lv);
-        ls = make_lists(dom, c2, a.ws);
-    }
+    const double shape_log2 = 1.4426950408889634074 * make_oseen_params(a.xopt, a.cutoff).inv_ks;
+    const Domain dom = choose_domain(hb, a.Lhint, (double)a.nb, (double)a.nt, a.cutoff, a.ws,
+                                     shape_log2, a.p + 1, a.prec == PHF_PREC_F32C ? 16 : 64);
+    const int Lv = dom.L;
+    const Lists ls = make_lists(dom, c2, a.ws);
     SortedPoints S, Tg;
     PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
     if (a.same)

@@ -31,6 +31,9 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
 // entries) from the leaf offsets; cnt, sums: workspaces (capturable).
 int pad_source_ranges(Lane *L, const int32_t *start, int64_t nleaf, uint32_t *cnt, uint32_t *sp,
                       uint32_t *sums);
+// split[0..8]: leaf ranges [split[k], split[k+1]) of the 8 XCDs holding
+// equal numbers of points of the leaf offsets start[0..nleaf] (capturable).
+int xcd_split(Lane *L, const int32_t *start, int64_t nleaf, int32_t *split);
 // zero-weight entries in the pads of a padded float32 source array
 int zero_source_pads(Lane *L, const int32_t *start, const int32_t *sp, int64_t nleaf, float4 *src);
 int q_to_velocity_device(Lane *L, const double *qr, const double *qi, int64_t n, double *vx,

@@ -3,11 +3,9 @@
 //
 // A workgroup of 4 waves serves one target leaf.  The sources of the
 // leaf's near leaves (ranges padded to multiples of 4 with zero-weight
-// entries) are staged in LDS in batches: lane l of every wave looks at near
-// index q + l, a wave scan of the range lengths packs as many as fit (a
-// cursor splits a range that does not), and each lane copies its own range
-// with 8 loads in flight, the four waves taking every fourth entry.  Lane
-// 16 g + t then sums staged sources g, g+4, g+8, ... for target t.
+// entries) are staged in LDS in batches, shifted into the target leaf's
+// frame; every wave sweeps a quarter of each batch for every target of the
+// leaf (p2p_leaf), and the per-wave partials meet in LDS.
 #pragma once
 
 #include "phf_device.hpp"
@@ -40,49 +38,6 @@ __device__ __forceinline__ float pair_scaled(float dx, float dy, float w, float 
         return num * rcp_f32(r2);
     }
     return w * rcp_f32(r2);
-}
-
-template <int KIND, bool CORE>
-__device__ __forceinline__ void p2p_seg_f32(const float4 *__restrict__ sq, int j0, int j1,
-                                            float xt, float yt, float c2s, float scale_f,
-                                            float &ha, float &la, float &hb, float &lb)
-{
-    int j = j0;
-    for (; j + 64 <= j1; j += 64) {
-        float aa = 0.f, ab = 0.f;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const float4 q = sq[j + 4 * k];
-            const float dx = xt - q.x, dy = yt - q.y;
-            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
-            aa = __builtin_fmaf(P, dx, aa);
-            ab = __builtin_fmaf(P, dy, ab);
-        }
-        two_sum_add(ha, la, aa);
-        two_sum_add(hb, lb, ab);
-    }
-    if (j < j1) {
-        float aa = 0.f, ab = 0.f;
-        for (; j + 16 <= j1; j += 16) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float4 q = sq[j + 4 * k];
-                const float dx = xt - q.x, dy = yt - q.y;
-                const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
-                aa = __builtin_fmaf(P, dx, aa);
-                ab = __builtin_fmaf(P, dy, ab);
-            }
-        }
-        for (; j < j1; j += 4) {
-            const float4 q = sq[j];
-            const float dx = xt - q.x, dy = yt - q.y;
-            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
-            aa = __builtin_fmaf(P, dx, aa);
-            ab = __builtin_fmaf(P, dy, ab);
-        }
-        two_sum_add(ha, la, aa);
-        two_sum_add(hb, lb, ab);
-    }
 }
 
 __device__ __forceinline__ void two_sum_merge(float &h, float &l, float h2, float l2)
@@ -121,92 +76,189 @@ __device__ __forceinline__ void l2p_f32(const float2 *__restrict__ Bb, int p1, f
     fi = pi;
 }
 
-// Accumulate the near-field sum of target tp (offset from its leaf centre),
-// in scaled coordinates: the true sums are lam_out times (ha + la), (hb + lb)
-// over every near leaf nf names: nf(q, sb, ox, oy, core) gives near index
-// q's source leaf sb, its centre shift (exact in float32) and whether its
-// pairs may lie inside the smoothing core; false: no such leaf.  Near
-// indices with core set must come first.  Sources are staged shifted into
-// the target leaf's frame, so each batch is two contiguous ranges (core,
-// then beyond the core) summed without per-leaf setup.  Every thread of
-// the workgroup must call this (it synchronises); wave_on: the wave has
-// targets.
-template <int KIND, class NearFn>
-__device__ __forceinline__ void p2p_staged(const float4 *__restrict__ src,
-                                           const int32_t *__restrict__ sp, float4 *st,
-                                           int nnear, const NearFn &nf,
-                                           bool wave_on, float2 tp, const PairParams &pp,
-                                           float &ha, float &la, float &hb, float &lb,
-                                           float &lam_out)
+// Staging plan of one batch: lane l of every wave looks at near index q + l,
+// a wave scan of the range lengths packs as many as fit (the cursor (q, jq)
+// splits a range that does not); each lane copies its own range, shifted
+// into the target leaf's frame and scaled by lam, the four waves taking
+// every fourth entry.  Every wave computes the same plan.  Returns the
+// staged count; staged entries [0, core_end) come from core leaves.
+template <class NearFn>
+__device__ __forceinline__ int p2p_stage_batch(const float4 *__restrict__ src,
+                                               const int32_t *__restrict__ sp, float4 *st,
+                                               int nnear, const NearFn &nf, float lam, int &q,
+                                               int &jq, int &core_end)
 {
-    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, g = lane >> 4;
-    // scaled coordinates (pair_scaled): sources are scaled while staged
-    const float lam = __builtin_sqrtf(-pp.nlog2e_ks_f);
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int ql = q + lane;
+    int j0 = 0, len = 0, core = 0;
+    float ox = 0.f, oy = 0.f;
+    uint32_t sb = 0;
+    if (ql < nnear && nf(ql, sb, ox, oy, core)) {
+        j0 = sp[sb] + (lane == 0 ? jq : 0);
+        len = sp[sb + 1] - j0;
+    }
+    int incl = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (lane >= d) incl += v;
+    }
+    const int excl = incl - len;
+    const bool in = ql < nnear && excl < P2P_STAGE;
+    const int take = in ? min(len, P2P_STAGE - excl) : 0;  // multiple of 4
+    const int count = __popcll(__ballot(in));               // >= 1: lane 0 is in
+    const int last_take = __shfl(take, count - 1), last_len = __shfl(len, count - 1);
+    const int used = __shfl(excl + take, count - 1);
+    // staged entries of core leaves form a prefix of the batch
+    const unsigned long long far_mask = __ballot(take > 0 && !core);
+    core_end = far_mask ? __shfl(excl, __ffsll((long long)far_mask) - 1) : used;
+    if (last_take < last_len) {
+        jq = (count == 1 ? jq : 0) + last_take;
+        q += count - 1;
+    } else {
+        jq = 0;
+        q += count;
+    }
+    if (in) {
+        const float4 *gs = src + j0;
+        float4 *ls = st + excl;
+        int e = wv;
+        for (; e + 28 < take; e += 32) {
+            float4 r[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) r[u] = gs[e + 4 * u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                r[u].x = (r[u].x + ox) * lam;
+                r[u].y = (r[u].y + oy) * lam;
+                ls[e + 4 * u] = r[u];
+            }
+        }
+        for (; e < take; e += 4) {
+            float4 r = gs[e];
+            r.x = (r.x + ox) * lam;
+            r.y = (r.y + oy) * lam;
+            ls[e] = r;
+        }
+    }
+    return used;
+}
+
+// Sum over the staged entries x in [a, b) with x = o (mod 16) for the
+// target (xt, yt): plain float32 partials over at most 16 pairs folded into
+// the TwoSum accumulators.
+template <int KIND, bool CORE>
+__device__ __forceinline__ void p2p_sweep(const float4 *__restrict__ st, int a, int b, int o,
+                                          float xt, float yt, float c2s, float scale_f,
+                                          float &ha, float &la, float &hb, float &lb)
+{
+    int x = a + ((o - a) & 15);
+    for (; x + 240 < b; x += 256) {
+        float aa = 0.f, ab = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float4 q = st[x + 16 * k];
+            const float dx = xt - q.x, dy = yt - q.y;
+            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
+            aa = __builtin_fmaf(P, dx, aa);
+            ab = __builtin_fmaf(P, dy, ab);
+        }
+        two_sum_add(ha, la, aa);
+        two_sum_add(hb, lb, ab);
+    }
+    if (x < b) {
+        float aa = 0.f, ab = 0.f;
+        for (; x + 48 < b; x += 64) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const float4 q = st[x + 16 * k];
+                const float dx = xt - q.x, dy = yt - q.y;
+                const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
+                aa = __builtin_fmaf(P, dx, aa);
+                ab = __builtin_fmaf(P, dy, ab);
+            }
+        }
+        for (; x < b; x += 16) {
+            const float4 q = st[x];
+            const float dx = xt - q.x, dy = yt - q.y;
+            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
+            aa = __builtin_fmaf(P, dx, aa);
+            ab = __builtin_fmaf(P, dy, ab);
+        }
+        two_sum_add(ha, la, aa);
+        two_sum_add(hb, lb, ab);
+    }
+}
+
+constexpr int P2P_CHUNKS = 8;  // target chunks of 16 per round (128 targets)
+
+// Near-field sums of the targets [t0, t1) of one leaf, in scaled
+// coordinates x' = lam x (pair_scaled).  nf(q, sb, ox, oy, core) gives near
+// index q's source leaf sb, its centre shift (exact in float32) and whether
+// its pairs may lie inside the smoothing core (false: no such leaf); near
+// indices with core set come first.  tgt(i): target i's offset from the
+// leaf centre.  Targets go in rounds of up to 128 (chunks of 16); every
+// staged batch of near sources is swept by all four waves for every chunk,
+// lane 16 g + t of wave w taking target t of the chunk and the staged
+// entries = 4 w + g (mod 16), so the waves carry equal work whatever the
+// leaf population.  Per chunk the four groups of a wave are combined by a
+// butterfly and the wave's partial is merged into its own LDS slot
+// acc[(c * 4 + w) * 16 + t]; at the end of a round one thread per target
+// merges the four waves' slots in wave order and calls
+// out(i, hi_a, lo_a, hi_b, lo_b): the sums are lam (hi + lo).  Every thread
+// of the workgroup must call this (it synchronises).
+template <int KIND, class NearFn, class TgtFn, class OutFn>
+__device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
+                                         const int32_t *__restrict__ sp, float4 *st, float4 *acc,
+                                         int nnear, const NearFn &nf, int32_t t0, int32_t t1,
+                                         const TgtFn &tgt, const PairParams &pp, float lam,
+                                         const OutFn &out)
+{
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, g = lane >> 4, t = lane & 15;
+    const int o = 4 * wv + g;
     const float c2s = pp.c2_f * (lam * lam);
-    const float txs = tp.x * lam, tys = tp.y * lam;
-    lam_out = lam;
-    int q = 0, jq = 0;  // cursor: near index, position within its range
-    while (q < nnear) {
-        const int ql = q + lane;
-        int j0 = 0, len = 0, core = 0;
-        float ox = 0.f, oy = 0.f;
-        uint32_t sb = 0;
-        if (ql < nnear && nf(ql, sb, ox, oy, core)) {
-            j0 = sp[sb] + (lane == 0 ? jq : 0);
-            len = sp[sb + 1] - j0;
-        }
-        int incl = len;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int v = __shfl_up(incl, d);
-            if (lane >= d) incl += v;
-        }
-        const int excl = incl - len;
-        const bool in = ql < nnear && excl < P2P_STAGE;
-        const int take = in ? min(len, P2P_STAGE - excl) : 0;  // multiple of 4
-        const int count = __popcll(__ballot(in));
-        const int last_take = __shfl(take, count - 1), last_len = __shfl(len, count - 1);
-        const int used = __shfl(excl + take, count - 1);
-        // staged entries of core leaves form a prefix of the batch
-        const unsigned long long far_mask = __ballot(take > 0 && !core);
-        const int core_end = far_mask ? __shfl(excl, __ffsll((long long)far_mask) - 1) : used;
-        if (last_take < last_len) {
-            jq = (count == 1 ? jq : 0) + last_take;
-            q += count - 1;
-        } else {
-            jq = 0;
-            q += count;
-        }
-        if (in) {
-            const float4 *gs = src + j0;
-            float4 *ls = st + excl;
-            int e = wv;
-            for (; e + 28 < take; e += 32) {
-                float4 r[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) r[u] = gs[e + 4 * u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    r[u].x = (r[u].x + ox) * lam;
-                    r[u].y = (r[u].y + oy) * lam;
-                    ls[e + 4 * u] = r[u];
+    for (int32_t rc = t0; rc < t1; rc += 16 * P2P_CHUNKS) {
+        const int nch = min(P2P_CHUNKS, (int)((t1 - rc + 15) >> 4));
+        if (g == 0)
+            for (int c = 0; c < nch; ++c)
+                acc[(c * 4 + wv) * 16 + t] = make_float4(0.f, 0.f, 0.f, 0.f);
+        int q = 0, jq = 0;
+        while (q < nnear) {
+            int core_end;
+            const int used = p2p_stage_batch(src, sp, st, nnear, nf, lam, q, jq, core_end);
+            __syncthreads();
+            for (int c = 0; c < nch; ++c) {
+                const int32_t i = rc + 16 * c + t;
+                float2 tp = i < t1 ? tgt(i) : make_float2(0.f, 0.f);
+                const float xt = tp.x * lam, yt = tp.y * lam;
+                float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
+                p2p_sweep<KIND, true>(st, 0, core_end, o, xt, yt, c2s, pp.scale_f, ha, la, hb,
+                                      lb);
+                p2p_sweep<KIND, false>(st, core_end, used, o, xt, yt, c2s, pp.scale_f, ha, la,
+                                       hb, lb);
+                p2p_merge_groups(ha, la, hb, lb);
+                if (g == 0) {
+                    float4 &A = acc[(c * 4 + wv) * 16 + t];
+                    float4 v = A;
+                    two_sum_merge(v.x, v.y, ha, la);
+                    two_sum_merge(v.z, v.w, hb, lb);
+                    A = v;
                 }
             }
-            for (; e < take; e += 4) {
-                float4 r = gs[e];
-                r.x = (r.x + ox) * lam;
-                r.y = (r.y + oy) * lam;
-                ls[e] = r;
+            __syncthreads();
+        }
+        if (tid < 16 * nch && rc + tid < t1) {
+            const int c = tid >> 4, tt = tid & 15;
+            float4 v = acc[(c * 4) * 16 + tt];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) {
+                const float4 u = acc[(c * 4 + w) * 16 + tt];
+                two_sum_merge(v.x, v.y, u.x, u.y);
+                two_sum_merge(v.z, v.w, u.z, u.w);
             }
+            out(rc + tid, v.x, v.y, v.z, v.w);
         }
-        __syncthreads();
-        if (wave_on) {
-            const float4 *sq = st + g;
-            p2p_seg_f32<KIND, true>(sq, 0, core_end, txs, tys, c2s, pp.scale_f, ha, la, hb, lb);
-            p2p_seg_f32<KIND, false>(sq, core_end, used, txs, tys, c2s, pp.scale_f, ha, la, hb,
-                                     lb);
-        }
-        __syncthreads();
+        __syncthreads();  // acc is reused by the next round
     }
 }
 

@@ -20,6 +20,7 @@
 // the lanes sharing its digit with 8 ballots ("match any"), and per-wave
 // counts in LDS give the cross-wave prefix, so equal keys keep input order.
 #include "phf_tree.hpp"
+#include "phf_fmm_dev.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -95,17 +96,12 @@ int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const d
     return PHF_OK;
 }
 
-Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L)
+Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L, double side)
 {
     Domain d{};
     d.x0 = xmin;
     d.y0 = ymin;
-    double ext = std::max(xmax - xmin, ymax - ymin);
-    if (!(ext > 0.0)) ext = 1.0;
-    int e = 0;
-    std::frexp(ext, &e);  // ext = f * 2^e, f in [0.5, 1)
-    double unit = std::ldexp(1.0, e - 5);
-    d.D = std::ceil(ext / unit) * unit;  // <= 32 units, exact
+    d.D = side > 0.0 ? side : domain_side(std::max(xmax - xmin, ymax - ymin), 0.0);
     d.L = L;
     d.s = d.D / std::ldexp(1.0, L);  // exact (power-of-two division)
     d.inv_s = std::ldexp(1.0, L) / d.D;

@@ -14,7 +14,10 @@ struct Domain {
     int L;
 };
 
-Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L);
+// side > 0: the domain side to use (at least the extent, from
+// domain_side); otherwise the bounding-box side.
+Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L,
+                   double side = 0.0);
 
 // Exact bounding box of (x1,y1)[n1] U (x2,y2)[n2] into d_out = {xmin, xmax,
 // ymin, ymax} (device memory).
