@@ -48,6 +48,8 @@ enum {
     PHF_STAGE_DOWNLOAD,   /* device -> host copies */
     PHF_STAGE_REMESH,
     PHF_STAGE_PANEL,
+    PHF_STAGE_TREE_FAR,   /* tree through L2L replayed as one captured graph
+                             (resident systems); TREE..L2L are then -1 */
     PHF_STAGE_TOTAL,
     PHF_STAGE_COUNT
 };
@@ -249,8 +251,13 @@ typedef struct {
 
 /* Work and timing of the last evaluation (stage ms are HIP-event times on
  * the system's stream; -1 for stages not run).  Set flags = 1 on input to
- * also count the FMM work (p2p_pairs, m2l_links; adds small kernels). */
+ * also count the FMM work (p2p_pairs, m2l_links; adds small kernels).
+ * A resident system's FMM records its launch sequence (tree through L2L)
+ * as a HIP graph and replays it while the point count, parameters and the
+ * domain side stay the same; flags |= PHF_INFO_EAGER launches every
+ * kernel one by one instead (same results). */
 #define PHF_INFO_COUNT_WORK 1
+#define PHF_INFO_EAGER 2
 typedef struct {
     int32_t L, p, nnear, flags;
     double D;

@@ -21,7 +21,7 @@ REMESH_M4 = 0
 REMESH_TUTTY = 1
 
 STAGES = ("upload", "tree", "p2m", "m2m", "m2l", "l2l", "p2p", "download",
-          "remesh", "panel", "total")
+          "remesh", "panel", "tree_far", "total")
 
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64

@@ -78,6 +78,30 @@ void *Arena::alloc_bytes(size_t bytes)
     return p;
 }
 
+Arena::Mark Arena::mark() const
+{
+    Mark m;
+    m.nblocks = blocks.size();
+    m.cur = cur;
+    m.top = top;
+    m.used = used;
+    m.base0 = blocks.empty() ? nullptr : blocks[0].base;
+    return m;
+}
+
+bool Arena::at(const Mark &m) const
+{
+    return blocks.size() == m.nblocks && cur == m.cur && top == m.top &&
+           (blocks.empty() ? nullptr : blocks[0].base) == m.base0;
+}
+
+void Arena::jump(const Mark &from, const Mark &to)
+{
+    cur = to.cur;
+    top = to.top;
+    used += to.used - from.used;
+}
+
 void Arena::release()
 {
     for (auto &b : blocks) (void)hipFree(b.base);

@@ -82,6 +82,15 @@ struct Arena {
 
     int reset();
     void *alloc_bytes(size_t bytes);  // nullptr + error on failure
+    // allocation state: a launch sequence recorded from state m0 to m1 can
+    // be replayed (same slices) when the arena is at m0 again, by jump(m1)
+    struct Mark {
+        size_t nblocks = 0, cur = 0, top = 0, used = 0;
+        const char *base0 = nullptr;
+    };
+    Mark mark() const;
+    bool at(const Mark &m) const;
+    void jump(const Mark &from, const Mark &to);
     template <typename T>
     T *alloc(size_t count) {
         return reinterpret_cast<T *>(alloc_bytes(count * sizeof(T)));
@@ -105,6 +114,8 @@ struct Pinned {
 };
 
 // ---------------------------------------------------------------- lanes --
+struct FmmGraphCache;
+
 struct Lane {
     hipStream_t stream = nullptr;
     Arena arena;
@@ -112,6 +123,9 @@ struct Lane {
     hipEvent_t ev[2 * PHF_STAGE_COUNT] = {};
     bool busy = false;
     int index = 0;
+    // FMM launch sequences recorded as HIP graphs (calls on resident
+    // systems, whose buffers keep their addresses between calls)
+    FmmGraphCache *fmm_graphs = nullptr;
 };
 
 class Context {

@@ -38,6 +38,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 namespace phf {
@@ -54,14 +55,15 @@ __global__ void gather_src_f64(const double *__restrict__ x, const double *__res
 
 __global__ void gather_tgt_f32(const double *__restrict__ x, const double *__restrict__ y,
                                const uint32_t *__restrict__ skeys,
-                               const int32_t *__restrict__ perm, int64_t n, double x0, double y0,
-                               double s, float2 *__restrict__ out)
+                               const int32_t *__restrict__ perm, int64_t n,
+                               const double4 *__restrict__ org, double s,
+                               float2 *__restrict__ out)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int32_t j = perm[i];
     double cx, cy;
-    leaf_center(skeys[i], x0, y0, s, cx, cy);
+    leaf_center(skeys[i], org->x, org->z, s, cx, cy);
     out[i] = make_float2((float)(x[j] - cx), (float)(y[j] - cy));
 }
 
@@ -81,8 +83,8 @@ __global__ void gather_tgt_f64(const double *__restrict__ x, const double *__res
 template <class T>
 __global__ __launch_bounds__(64) void p2m_kernel(const void *__restrict__ src_,
                                                  const int32_t *__restrict__ start, int p1,
-                                                 double x0, double y0, double s, int Lvl,
-                                                 typename C2<T>::type *__restrict__ M)
+                                                 const double4 *__restrict__ org, double s,
+                                                 int Lvl, typename C2<T>::type *__restrict__ M)
 {
     using V = typename C2<T>::type;
     extern __shared__ unsigned char lds_raw[];
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(const void *__restrict__ src_,
     V acc = cmk<T>(0, 0);
     if (s1 > s0) {
         double cx = 0, cy = 0;
-        if (sizeof(T) == 8) leaf_center(b, x0, y0, s, cx, cy);
+        if (sizeof(T) == 8) leaf_center(b, org->x, org->z, s, cx, cy);
         const T inv_s = (T)(1.0 / s);
         for (int32_t c0 = s0; c0 < s1; c0 += 64) {
             const int32_t j = c0 + lane;
@@ -479,15 +481,16 @@ __global__ __launch_bounds__(64) void m2l_box_kernel(
 __global__ void gather_src_f32p(const double *__restrict__ x, const double *__restrict__ y,
                                 const double *__restrict__ w, const uint32_t *__restrict__ skeys,
                                 const int32_t *__restrict__ perm, const int32_t *__restrict__ start,
-                                const int32_t *__restrict__ sp, int64_t n, double x0, double y0,
-                                double s, float4 *__restrict__ out)
+                                const int32_t *__restrict__ sp, int64_t n,
+                                const double4 *__restrict__ org, double s,
+                                float4 *__restrict__ out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t leaf = skeys[i];
     const int32_t j = perm[i];
     double cx, cy;
-    leaf_center(leaf, x0, y0, s, cx, cy);
+    leaf_center(leaf, org->x, org->z, s, cx, cy);
     out[sp[leaf] + (i - start[leaf])] =
         make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
 }
@@ -590,8 +593,8 @@ __global__ __launch_bounds__(64) void p2p_l2p_f64(
     const double4 *__restrict__ src, const int32_t *__restrict__ start_s,
     const double2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
     const int32_t *__restrict__ perm_t, int Lvl, const int2 *__restrict__ near, int nnear,
-    PairParams pp, double x0, double y0, double s, const double2 *__restrict__ Bleaf, int p1,
-    double *__restrict__ qr, double *__restrict__ qi, int velocity_out)
+    PairParams pp, const double4 *__restrict__ org, double s, const double2 *__restrict__ Bleaf,
+    int p1, double *__restrict__ qr, double *__restrict__ qi, int velocity_out)
 {
     const int lane = threadIdx.x;
     const uint32_t b = blockIdx.x;
@@ -600,7 +603,7 @@ __global__ __launch_bounds__(64) void p2p_l2p_f64(
     const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
     const int n = 1 << Lvl;
     double cx, cy;
-    leaf_center(b, x0, y0, s, cx, cy);
+    leaf_center(b, org->x, org->z, s, cx, cy);
     const double inv_s = 1.0 / s;
     for (int32_t tc = t0; tc < t1; tc += 64) {
         const int32_t i = tc + lane;
@@ -888,14 +891,24 @@ struct TableBlob {
         if (!v.empty()) std::memcpy(host.data() + at, v.data(), sizeof(T) * v.size());
         fix.push_back({at, reinterpret_cast<void **>(d)});
     }
-    int send(Lane *L)
+    // keep != nullptr (launches being recorded into a graph): the copy
+    // reads a pinned buffer of its own, handed to *keep, so that every
+    // replay copies the same tables
+    int send(Lane *L, char **keep = nullptr)
     {
         if (host.empty()) return PHF_OK;
-        PHF_TRY(L->pin.ensure(host.size()));
-        std::memcpy(L->pin.ptr, host.data(), host.size());
+        char *h = nullptr;
+        if (keep) {
+            PHF_HIP(hipHostMalloc(&h, host.size(), hipHostMallocDefault));
+            *keep = h;
+        } else {
+            PHF_TRY(L->pin.ensure(host.size()));
+            h = L->pin.ptr;
+        }
+        std::memcpy(h, host.data(), host.size());
         char *dev;
         PHF_ALLOC(dev, L->arena, char, host.size());
-        PHF_HIP(hipMemcpyAsync(dev, L->pin.ptr, host.size(), hipMemcpyHostToDevice, L->stream));
+        PHF_HIP(hipMemcpyAsync(dev, h, host.size(), hipMemcpyHostToDevice, L->stream));
         for (auto &f : fix) *f.second = dev + f.first;
         return PHF_OK;
     }
@@ -1017,10 +1030,13 @@ static int m2l_dispatch(Lane *L, const M2LLevels &lv, int p1, const void *M, voi
     return PHF_ERR_UNSUPPORTED;
 }
 
+// launch of the near-field kernel on a stream (recorded with its arguments)
+using P2PLaunch = std::function<int(hipStream_t)>;
+
 template <class T, class TC>
 static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lists &ls,
                          const SortedPoints &S, const SortedPoints &Tg, StageTimer &tm,
-                         bool *used, FmmInfo *info)
+                         bool *used, FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep)
 {
     using V = typename C2<T>::type;
     using VC = typename C2<TC>::type;
@@ -1043,13 +1059,13 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         PHF_ALLOC(s4, L->arena, float4, (size_t)(a.nb + 3 * nleaf64 + 1));
         if (a.nb)
             gather_src_f32p<<<div_up(a.nb, 256), 256, 0, L->stream>>>(
-                a.xb, a.yb, a.wb, S.keys, S.perm, S.start, sp, a.nb, dom.x0, dom.y0, dom.s, s4);
+                a.xb, a.yb, a.wb, S.keys, S.perm, S.start, sp, a.nb, dom.org, dom.s, s4);
         PHF_TRY(zero_source_pads(L, S.start, sp, nleaf64, s4));
         float2 *t2 = nullptr;  // targets == sources: P2P reads the padded sources
         if (!a.same) {
             PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
             gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(
-                a.xt, a.yt, Tg.keys, Tg.perm, a.nt, dom.x0, dom.y0, dom.s, t2);
+                a.xt, a.yt, Tg.keys, Tg.perm, a.nt, dom.org, dom.s, t2);
         }
         src = s4;
         tgt = t2;
@@ -1147,7 +1163,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         int32_t *d_iloff = nullptr;
         blob.add(il_all, &d_il);
         blob.add(il_offs, &d_iloff);
-        PHF_TRY(blob.send(L));
+        PHF_TRY(blob.send(L, tables_keep));
         if (info && info->want_counts) {
             for (int l = 2; l <= Lv; ++l)
                 count_m2l_links<<<div_up(int64_t(1) << (2 * l), 256), 256, 0, L->stream>>>(
@@ -1161,7 +1177,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         used[PHF_STAGE_P2M] = true;
         const uint32_t nleaf = 1u << (2 * Lv);
         p2m_kernel<T><<<nleaf, 64, (size_t)64 * p1 * sizeof(V), L->stream>>>(
-            src, sp, p1, dom.x0, dom.y0, dom.s, Lv, M + off[Lv] * p1);
+            src, sp, p1, dom.org, dom.s, Lv, M + off[Lv] * p1);
         PHF_HIP(hipGetLastError());
         tm.end(PHF_STAGE_P2M);
         // M2M up to level 2
@@ -1219,47 +1235,214 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         Bleaf = B + off[Lv] * p1;
     }
 
-    if (!d_near) PHF_TRY(blob.send(L));
+    if (!d_near) PHF_TRY(blob.send(L, tables_keep));
     if (info && info->want_counts) {
         count_p2p_pairs<<<div_up(int64_t(1) << (2 * Lv), 256), 256, 0, L->stream>>>(
             S.start, Tg.start, Lv, d_near, (int)ls.near_leaf.size(), info->d_counts);
         PHF_HIP(hipGetLastError());
     }
-    // P2P + L2P
-    tm.begin(PHF_STAGE_P2P);
+    // P2P + L2P: launched here, or handed back (graph capture) to be
+    // launched after each replay
     const uint32_t nleaf = 1u << (2 * Lv);
     const int vout = a.velocity_out ? 1 : 0;
+    const int nnear = (int)near_ord.size();
+    P2PLaunch p2p;
     if (sizeof(T) == 4) {
         const uint32_t nl8 = (nleaf + 7) / 8;
         int32_t *split;
         PHF_ALLOC(split, L->arena, int32_t, 9);
         PHF_TRY(xcd_split(L, Tg.start, nleaf, split));
-        if (pp.f32_nocut)
-            p2p_l2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, L->stream>>>(
-                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, split,
-                d_near, (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf,
-                p1, a.qr, a.qi, vout);
-        else
-            p2p_l2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, L->stream>>>(
-                (const float4 *)src, sp, (const float2 *)tgt, Tg.start, perm_t, Lv, split,
-                d_near, (int)near_ord.size(), ninner, pp, (float)dom.s, (const float2 *)Bleaf,
-                p1, a.qr, a.qi, vout);
+        const float4 *s4 = (const float4 *)src;
+        const float2 *t2 = (const float2 *)tgt;
+        const float2 *Bl = (const float2 *)Bleaf;
+        const int32_t *ts = Tg.start;
+        const float s_f = (float)dom.s;
+        double *qr = a.qr, *qi = a.qi;
+        p2p = [=](hipStream_t st) {
+            if (pp.f32_nocut)
+                p2p_l2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, st>>>(
+                    s4, sp, t2, ts, perm_t, Lv, split, d_near, nnear, ninner, pp, s_f, Bl, p1, qr,
+                    qi, vout);
+            else
+                p2p_l2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(
+                    s4, sp, t2, ts, perm_t, Lv, split, d_near, nnear, ninner, pp, s_f, Bl, p1, qr,
+                    qi, vout);
+            PHF_HIP(hipGetLastError());
+            return PHF_OK;
+        };
     } else {
-        p2p_l2p_f64<PAIR_OSEEN><<<nleaf, 64, 0, L->stream>>>(
-            (const double4 *)src, S.start, (const double2 *)tgt, Tg.start, perm_t, Lv, d_near,
-            (int)near_ord.size(), pp, dom.x0, dom.y0, dom.s, (const double2 *)Bleaf, p1,
-            a.qr, a.qi, vout);
+        const double4 *s4 = (const double4 *)src;
+        const double2 *t2 = (const double2 *)tgt;
+        const double2 *Bl = (const double2 *)Bleaf;
+        const int32_t *ss = S.start, *ts = Tg.start;
+        const double4 *org = dom.org;
+        const double s = dom.s;
+        double *qr = a.qr, *qi = a.qi;
+        p2p = [=](hipStream_t st) {
+            p2p_l2p_f64<PAIR_OSEEN><<<nleaf, 64, 0, st>>>(s4, ss, t2, ts, perm_t, Lv, d_near,
+                                                         nnear, pp, org, s, Bl, p1, qr, qi, vout);
+            PHF_HIP(hipGetLastError());
+            return PHF_OK;
+        };
     }
-    PHF_HIP(hipGetLastError());
+    if (p2p_out) {
+        *p2p_out = p2p;
+        return PHF_OK;
+    }
+    tm.begin(PHF_STAGE_P2P);
+    PHF_TRY(p2p(L->stream));
     tm.end(PHF_STAGE_P2P);
+    return PHF_OK;
+}
+
+// Tree and FMM from the domain on (every launch after the bounding box);
+// p2p_out / tables_keep as run_fmm_typed.
+static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer &tm, bool *used,
+                       FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep, int *nnear)
+{
+    const double c2 = a.cutoff * a.cutoff;
+    const Lists ls = make_lists(dom, c2, a.ws);
+    *nnear = (int)ls.near_leaf.size();
+    SortedPoints S, Tg;
+    PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
+    if (a.same)
+        Tg = S;
+    else
+        PHF_TRY(build_sorted_points(L, dom, a.xt, a.yt, a.nt, &Tg));
+    if (info && info->want_counts) {
+        PHF_ALLOC(info->d_counts, L->arena, unsigned long long, 2);
+        PHF_HIP(hipMemsetAsync(info->d_counts, 0, 2 * sizeof(unsigned long long), L->stream));
+    }
+    if (a.prec == PHF_PREC_F32C) {
+        if (a.p > 16)
+            return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
+                                                tables_keep);
+        return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
+                                           tables_keep);
+    }
+    return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
+                                         tables_keep);
+}
+
+// ------------------------------------------------------ captured graphs --
+// A lane's recorded FMM launch sequences.  An entry is valid for the same
+// inputs and outputs (addresses), sizes, parameters, leaf level and
+// domain side, and the same arena state at the start (its buffers are the
+// same arena slices); the domain corner is read on the device, so moving
+// points replay the same graph while the side stays.
+struct FmmGraph {
+    FmmArgs a{};
+    int L = 0, nnear = 0;
+    double D = 0.0;
+    Arena::Mark m0, m1;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t exec = nullptr;
+    char *tables = nullptr;  // pinned copy the graph's table upload reads
+    P2PLaunch p2p;
+    uint64_t last = 0;
+};
+
+struct FmmGraphCache {
+    std::vector<FmmGraph *> v;
+    uint64_t clock = 0;
+};
+
+constexpr size_t kFmmGraphs = 4;
+
+static void drop_graph(FmmGraph *e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->g) (void)hipGraphDestroy(e->g);
+    if (e->tables) (void)hipHostFree(e->tables);
+    delete e;
+}
+
+void fmm_graphs_release(Lane *L)
+{
+    if (!L->fmm_graphs) return;
+    for (FmmGraph *e : L->fmm_graphs->v) drop_graph(e);
+    delete L->fmm_graphs;
+    L->fmm_graphs = nullptr;
+}
+
+static bool same_args(const FmmArgs &x, const FmmArgs &y)
+{
+    return x.xb == y.xb && x.yb == y.yb && x.wb == y.wb && x.nb == y.nb && x.xt == y.xt &&
+           x.yt == y.yt && x.nt == y.nt && x.same == y.same && x.xopt == y.xopt &&
+           x.cutoff == y.cutoff && x.p == y.p && x.ws == y.ws && x.prec == y.prec &&
+           x.Lhint == y.Lhint && x.qr == y.qr && x.qi == y.qi &&
+           x.velocity_out == y.velocity_out;
+}
+
+// Find or record the graph for (a, dom) at the arena's current state; the
+// arena then stands where the recorded sequence left it.
+static int fmm_graph(Lane *L, const FmmArgs &a, const Domain &dom, FmmGraph **out, bool *keep)
+{
+    if (!L->fmm_graphs) L->fmm_graphs = new FmmGraphCache;
+    FmmGraphCache &c = *L->fmm_graphs;
+    for (FmmGraph *e : c.v)
+        if (same_args(e->a, a) && e->L == dom.L && e->D == dom.D && L->arena.at(e->m0)) {
+            L->arena.jump(e->m0, e->m1);
+            e->last = ++c.clock;
+            *out = e;
+            *keep = true;
+            return PHF_OK;
+        }
+    FmmGraph *e = new FmmGraph;
+    e->a = a;
+    e->L = dom.L;
+    e->D = dom.D;
+    e->m0 = L->arena.mark();
+    StageTimer off(L, false);
+    bool used[PHF_STAGE_COUNT] = {};
+    hipError_t be = hipStreamBeginCapture(L->stream, hipStreamCaptureModeRelaxed);
+    if (be != hipSuccess) {
+        delete e;
+        PHF_HIP(be);
+    }
+    int rc = fmm_enqueue(L, a, dom, off, used, nullptr, &e->p2p, &e->tables, &e->nnear);
+    hipError_t ce = hipStreamEndCapture(L->stream, &e->g);
+    if (rc == PHF_OK && ce != hipSuccess) {
+        set_error("HIP error %s ending the FMM graph capture", hipGetErrorName(ce));
+        rc = PHF_ERR_DEVICE;
+    }
+    if (rc == PHF_OK) {
+        hipError_t ie = hipGraphInstantiate(&e->exec, e->g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            set_error("HIP error %s instantiating the FMM graph", hipGetErrorName(ie));
+            rc = PHF_ERR_DEVICE;
+        }
+    }
+    if (rc != PHF_OK) {
+        drop_graph(e);
+        return rc;
+    }
+    e->m1 = L->arena.mark();
+    e->last = ++c.clock;
+    *out = e;
+    // an arena that grew while recording is rebuilt by its next reset: the
+    // graph then serves this call only
+    *keep = e->m1.nblocks == e->m0.nblocks;
+    if (*keep) {
+        if (c.v.size() >= kFmmGraphs) {
+            auto lru = std::min_element(c.v.begin(), c.v.end(), [](FmmGraph *x, FmmGraph *y) {
+                return x->last < y->last;
+            });
+            drop_graph(*lru);
+            c.v.erase(lru);
+        }
+        c.v.push_back(e);
+    }
     return PHF_OK;
 }
 
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info)
 {
     if (a.nt == 0) return PHF_OK;
+    const bool graph = info && info->graph && !info->want_counts;
+    const int stage0 = graph ? PHF_STAGE_TREE_FAR : PHF_STAGE_TREE;
     // 1. bounding box of sources and targets -> domain
-    tm.begin(PHF_STAGE_TREE);
+    tm.begin(stage0);
     double4 *bb;
     PHF_ALLOC(bb, L->arena, double4, 1);
     PHF_TRY(device_bbox(L, a.xb, a.yb, a.nb, a.same ? nullptr : a.xt, a.same ? nullptr : a.yt,
@@ -1267,34 +1450,44 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
     double4 hb;
     PHF_HIP(hipMemcpyAsync(&hb, bb, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
-    const double c2 = a.cutoff * a.cutoff;
     const double shape_log2 = 1.4426950408889634074 * make_oseen_params(a.xopt, a.cutoff).inv_ks;
-    const Domain dom = choose_domain(hb, a.Lhint, (double)a.nb, (double)a.nt, a.cutoff, a.ws,
-                                     shape_log2, a.p + 1, a.prec == PHF_PREC_F32C ? 16 : 64);
-    const int Lv = dom.L;
-    const Lists ls = make_lists(dom, c2, a.ws);
-    SortedPoints S, Tg;
-    PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
-    if (a.same)
-        Tg = S;
-    else
-        PHF_TRY(build_sorted_points(L, dom, a.xt, a.yt, a.nt, &Tg));
-    used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
-    if (info) {
-        info->L = Lv;
-        info->p = a.p;
-        info->nnear = (int)ls.near_leaf.size();
-        info->D = dom.D;
-        if (info->want_counts) {
-            PHF_ALLOC(info->d_counts, L->arena, unsigned long long, 2);
-            PHF_HIP(hipMemsetAsync(info->d_counts, 0, 2 * sizeof(unsigned long long), L->stream));
+    Domain dom = choose_domain(hb, a.Lhint, (double)a.nb, (double)a.nt, a.cutoff, a.ws,
+                               shape_log2, a.p + 1, a.prec == PHF_PREC_F32C ? 16 : 64);
+    dom.org = bb;
+    int nnear = 0;
+    if (graph) {
+        FmmGraph *e = nullptr;
+        bool keep = true;
+        PHF_TRY(fmm_graph(L, a, dom, &e, &keep));
+        hipError_t le = hipGraphLaunch(e->exec, L->stream);
+        tm.end(stage0);
+        int rc = le == hipSuccess ? PHF_OK : PHF_ERR_DEVICE;
+        if (le != hipSuccess) set_error("HIP error %s launching the FMM graph", hipGetErrorName(le));
+        if (rc == PHF_OK) {
+            tm.begin(PHF_STAGE_P2P);
+            rc = e->p2p(L->stream);
+            tm.end(PHF_STAGE_P2P);
         }
+        nnear = e->nnear;
+        if (!keep) {
+            // the graph and its pinned tables must outlive the launches
+            const hipError_t se = hipStreamSynchronize(L->stream);
+            drop_graph(e);
+            if (rc == PHF_OK && se != hipSuccess) PHF_HIP(se);
+        }
+        PHF_TRY(rc);
+        used[PHF_STAGE_TREE_FAR] = used[PHF_STAGE_P2P] = true;
+    } else {
+        used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
+        PHF_TRY(fmm_enqueue(L, a, dom, tm, used, info, nullptr, nullptr, &nnear));
     }
-    if (a.prec == PHF_PREC_F32C) {
-        if (a.p > 16) return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used, info);
-        return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used, info);
+    if (info) {
+        info->L = dom.L;
+        info->p = a.p;
+        info->nnear = nnear;
+        info->D = dom.D;
     }
-    return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used, info);
+    return PHF_OK;
 }
 
 }  // namespace phf

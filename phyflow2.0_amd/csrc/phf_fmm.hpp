@@ -21,12 +21,18 @@ struct FmmInfo {
     int L = 0, p = 0, nnear = 0;
     double D = 0.0;
     bool want_counts = false;                // fill d_counts = {P2P pairs, M2L links}
+    bool graph = false;                      // record / replay the launches as a HIP graph
     unsigned long long *d_counts = nullptr;  // device, arena-owned
 };
 
 int fmm_order_from_tol(double tol);
 int fmm_level(int64_t nb, int leaf, int ndirect);
+// With info->graph set (and no work counts) the launch sequence after the
+// bounding box is recorded as a HIP graph in the lane's cache and replayed
+// on later calls with the same arguments, leaf level and domain side (the
+// stage then timed is PHF_STAGE_TREE_FAR).
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info);
+void fmm_graphs_release(Lane *L);
 // float32 source ranges per leaf padded to multiples of 4: sp[b] (nleaf + 1
 // entries) from the leaf offsets; cnt, sums: workspaces (capturable).
 int pad_source_ranges(Lane *L, const int32_t *start, int64_t nleaf, uint32_t *cnt, uint32_t *sp,

@@ -157,6 +157,7 @@ int phf_system_destroy(phf_system *h)
         double *bufs[5] = {s->x, s->y, s->w, s->vx, s->vy};
         for (double *b : bufs)
             if (b) (void)hipFree(b);
+        fmm_graphs_release(&s->lane);
         s->lane.arena.release();
         s->lane.pin.release();
         for (auto &ev : s->lane.ev)
@@ -222,6 +223,9 @@ int phf_system_velocity(phf_system *h, const phf_vel_params *p, double *vx, doub
     FmmInfo fi;
     fi.want_counts = info != nullptr && (info->flags & PHF_INFO_COUNT_WORK) &&
                      p->method == PHF_METHOD_FMM && p->tol > 0;
+    // resident buffers keep their addresses: the FMM launch sequence is
+    // recorded once and replayed
+    fi.graph = !(info != nullptr && (info->flags & PHF_INFO_EAGER));
     tm.begin(PHF_STAGE_TOTAL);
     PHF_TRY(system_velocity_device(L, s->x, s->y, s->w, s->n, *p, s->vx, s->vy, tm, used, &fi));
     if (vx && vy && s->n > 0) {

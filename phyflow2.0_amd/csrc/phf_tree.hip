@@ -110,11 +110,12 @@ Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L, do
 
 // ----------------------------------------------------------------- keys --
 __global__ void morton_keys_kernel(const double *__restrict__ x, const double *__restrict__ y,
-                                   int64_t n, double x0, double y0, double inv_s, int L,
+                                   int64_t n, const double4 *__restrict__ org, double inv_s, int L,
                                    uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double x0 = org->x, y0 = org->z;
     const double hi = (double)((1u << L) - 1u);
     double fx = floor((x[i] - x0) * inv_s);
     double fy = floor((y[i] - y0) * inv_s);
@@ -382,8 +383,8 @@ int build_sorted_points(Lane *L, const Domain &dom, const double *x, const doubl
     int32_t *start;
     PHF_ALLOC(start, L->arena, int32_t, (size_t)nleaf + 1);
     if (n > 0)
-        morton_keys_kernel<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, dom.x0, dom.y0,
-                                                                  dom.inv_s, dom.L, k0, v0);
+        morton_keys_kernel<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, dom.org, dom.inv_s,
+                                                                  dom.L, k0, v0);
     PHF_HIP(hipGetLastError());
     out->keys_in = nullptr;
     if (out->want_input_keys) {
@@ -434,6 +435,7 @@ extern "C" int phf_tree_build(const double *x, const double *y, int64_t n, int32
         PHF_HIP(hipStreamSynchronize(Ln->stream));
     }
     Domain dom = make_domain(hb.x, hb.y, hb.z, hb.w, L);
+    dom.org = bb;
     SortedPoints sp;
     sp.want_input_keys = keys != nullptr;
     PHF_TRY(build_sorted_points(Ln, dom, dx, dy, n, &sp));

@@ -12,6 +12,10 @@ struct Domain {
     double s;       // leaf box side D / 2^L (exact)
     double inv_s;   // 2^L / D
     int L;
+    // the same corner on the device ({xmin, xmax, ymin, ymax} from
+    // device_bbox): kernels read the origin there, so a captured launch
+    // sequence stays valid when the points move within the same side
+    const double4 *org = nullptr;
 };
 
 // side > 0: the domain side to use (at least the extent, from

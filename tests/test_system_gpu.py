@@ -34,3 +34,33 @@ def test_system_matches_one_shot(prec, O):
         assert sysm.n == 100
         x, y, w = sysm.get()
         assert np.array_equal(x, s["x"][:100])
+
+
+@pytest.mark.parametrize("prec", [gauss.PREC_F64, gauss.PREC_F32C])
+def test_graph_replay_matches_eager(prec):
+    """The recorded FMM launch sequence replays bit-identically to eager
+    launches: repeated replays with eager work between them, points moved
+    within the same domain side (replay reads the new corner), a wider cloud
+    (new recording) and a return to the first cloud (cached recording)."""
+    s = W.uniform_system(30000, seed=5)
+    p = vel_params(s["sigma"], prec=prec, order=16, leaf=64)
+
+    def same(a, b):
+        return np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+    with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
+        ref = sysm.velocity(p, eager=True)
+        for _ in range(3):
+            assert same(sysm.velocity(p), ref)
+            sysm.velocity(p, eager=True)
+        sysm.velocity(p, info=True)
+        ms = sysm.last_info["ms"]
+        assert "tree_far" in ms and "p2p" in ms and "tree" not in ms
+        sysm.set(s["x"] + 0.013, s["y"] - 0.007, s["w"])
+        ref2 = sysm.velocity(p, eager=True)
+        assert same(sysm.velocity(p), ref2) and same(sysm.velocity(p), ref2)
+        sysm.set(s["x"] * 1.7, s["y"], s["w"])
+        ref3 = sysm.velocity(p, eager=True)
+        assert same(sysm.velocity(p), ref3) and same(sysm.velocity(p), ref3)
+        sysm.set(s["x"], s["y"], s["w"])
+        assert same(sysm.velocity(p), ref)
