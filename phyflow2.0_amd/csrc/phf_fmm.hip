@@ -130,144 +130,156 @@ __global__ __launch_bounds__(64) void p2m_kernel(const void *__restrict__ src_,
 }
 
 // ---------------------------------------------------------- M2M / L2L ----
-// tab[c][m][k] (c = child slot, complex, compute type TC).
-template <class T, class TC>
-__global__ void m2m_kernel(int p1, uint32_t nparent, const typename C2<T>::type *__restrict__ Mc,
-                           typename C2<T>::type *__restrict__ Mp,
-                           const typename C2<TC>::type *__restrict__ tab,
-                           const int32_t *__restrict__ start_s, int shift)
+// Offset of the centre of descendant c (index within its ancestor's block
+// of 4^D boxes, D levels down) from the ancestor's centre, in units of the
+// ancestor's side (exact dyadic).
+template <class TC>
+__device__ __forceinline__ void descendant_offset(uint32_t c, int D, TC &dx, TC &dy)
+{
+    const uint32_t side = 1u << D;
+    const TC inv = (TC)1 / (TC)side;
+    dx = ((TC)compact_bits(c) + (TC)0.5) * inv - (TC)0.5;
+    dy = ((TC)compact_bits(c >> 1) + (TC)0.5) * inv - (TC)0.5;
+}
+
+// Upward pass over up to three levels in one launch: the multipole
+// expansion of a box at level a (from - nlev <= a < from) is the sum over
+// its 4^(from - a) descendants at level `from`, in Morton order, of
+// M_m += sum_{k <= m} C(m, k) d^(m-k) rho^k M_k (d: descendant centre from
+// the box centre in box units, rho = 2^-(from - a)), the composition of
+// the one-level shifts.  One workgroup per box: the descendants'
+// coefficients (contiguous) and the Pascal triangle are staged in LDS,
+// thread PW j + m sums coefficient m over descendants j, j + NG, ...
+// (NG = 256 / PW groups), and the group partials are added in group
+// order.  Empty boxes have zero expansions, so no box is skipped.
+struct UpLevels {
+    int nlev, lev0, from;
+    int64_t blk[4];  // first workgroup of each level
+    int64_t off[4];  // coefficient offset (boxes) of each level
+    int64_t off_from;
+};
+
+template <class T, class TC, int PW>
+__global__ __launch_bounds__(256) void m2m_multi_kernel(UpLevels u, int p1,
+                                                        typename C2<T>::type *__restrict__ M,
+                                                        const TC *__restrict__ pas)
 {
     using V = typename C2<T>::type;
     using W = typename C2<TC>::type;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nparent * p1) return;
-    const uint32_t b = (uint32_t)(t / p1);
-    const int m = (int)(t % p1);
-    W acc = W{0, 0};
-    if (box_count(start_s, b, shift) > 0) {
-        for (int c = 0; c < 4; ++c) {
-            const V *ch = Mc + ((int64_t)(4 * b + c)) * p1;
-            const W *tr = tab + ((int64_t)c * p1 + m) * p1;
-            for (int k = 0; k <= m; ++k) {
-                V a = ch[k];
-                W tk = tr[k];
-                acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
-                acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
+    constexpr int NG = 256 / PW;
+    __shared__ V ch[64 * PW];
+    __shared__ W part[NG * PW];
+    __shared__ TC pl[PW * PW];
+    int i = 0;
+    while (i + 1 < u.nlev && (int64_t)blockIdx.x >= u.blk[i + 1]) ++i;
+    const int a = u.lev0 + i, D = u.from - a;
+    const int64_t box = (int64_t)blockIdx.x - u.blk[i];
+    const int tid = threadIdx.x, nd = 1 << (2 * D);
+    const V *src = M + (u.off_from + (box << (2 * D))) * p1;
+    for (int e = tid; e < nd * p1; e += 256) ch[e] = src[e];
+    for (int e = tid; e < p1 * p1; e += 256) pl[e] = pas[e];
+    __syncthreads();
+    const int j = tid / PW, m = tid - j * PW;
+    if (j < NG && m < p1) {
+        W acc = W{0, 0};
+        const TC rho = (TC)1 / (TC)(1u << D), irho = (TC)(1u << D);
+        TC rm = (TC)1;  // rho^m
+        for (int k = 0; k < m; ++k) rm *= rho;
+        for (int c = j; c < nd; c += NG) {
+            TC dx, dy;
+            descendant_offset<TC>((uint32_t)c, D, dx, dy);
+            const V *mc = ch + c * p1;
+            // k = m .. 0: d^(m-k) gains a factor d, rho^k loses one rho
+            TC rk = rm;
+            W dp = W{1, 0};
+            for (int k = m; k >= 0; --k) {
+                const V mk = mc[k];
+                const TC f = pl[m * p1 + k] * rk;
+                const TC tx = f * dp.x, ty = f * dp.y;
+                acc.x += tx * (TC)mk.x - ty * (TC)mk.y;
+                acc.y += tx * (TC)mk.y + ty * (TC)mk.x;
+                const TC nx = dp.x * dx - dp.y * dy;
+                dp.y = dp.x * dy + dp.y * dx;
+                dp.x = nx;
+                rk *= irho;
             }
         }
+        part[j * PW + m] = acc;
     }
-    Mp[(int64_t)b * p1 + m] = V{(T)acc.x, (T)acc.y};
-}
-
-template <class T, class TC>
-__global__ void l2l_kernel(int p1, uint32_t nchild, const typename C2<T>::type *__restrict__ Bp,
-                           typename C2<T>::type *__restrict__ Bc,
-                           const typename C2<TC>::type *__restrict__ tab,
-                           const int32_t *__restrict__ start_t, int shift)
-{
-    using V = typename C2<T>::type;
-    using W = typename C2<TC>::type;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nchild * p1) return;
-    const uint32_t b = (uint32_t)(t / p1);
-    const int m = (int)(t % p1);
-    if (box_count(start_t, b, shift) == 0) return;
-    const uint32_t P = b >> 2;
-    const int c = b & 3;
-    const V *par = Bp + (int64_t)P * p1;
-    const W *tr = tab + ((int64_t)c * p1 + m) * p1;
-    W acc = W{0, 0};
-    for (int j = m; j < p1; ++j) {
-        V a = par[j];
-        W tk = tr[j];
-        acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
-        acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
-    }
-    V cur = Bc[(int64_t)b * p1 + m];
-    Bc[(int64_t)b * p1 + m] = V{(T)((TC)cur.x + acc.x), (T)((TC)cur.y + acc.y)};
-}
-
-// Same shifts with the operands staged in LDS and the inner loops unrolled
-// (exact order P1 - 1): a block of 64 lanes serves G = 64 / P1 boxes, lane
-// (g, m) one coefficient.  M2M: the 4 G children of the block's G parents
-// are consecutive boxes in Morton order, so their coefficients are one
-// contiguous, coalesced load.  L2L: group g loads its child's parent.
-// Each level then costs one load round instead of a chain of dependent
-// loads per coefficient.  Terms are added in the order of the kernels above.
-template <class T, class TC, int P1>
-__global__ __launch_bounds__(64) void m2m_lds_kernel(uint32_t nparent,
-                                                     const typename C2<T>::type *__restrict__ Mc,
-                                                     typename C2<T>::type *__restrict__ Mp,
-                                                     const typename C2<TC>::type *__restrict__ tab,
-                                                     const int32_t *__restrict__ start_s,
-                                                     int shift)
-{
-    using V = typename C2<T>::type;
-    using W = typename C2<TC>::type;
-    constexpr int G = 64 / P1;
-    __shared__ V ch[G * 4 * P1];
-    const int lane = threadIdx.x;
-    const int64_t p0 = (int64_t)blockIdx.x * G;
-    const int64_t np = min((int64_t)G, (int64_t)nparent - p0);
-    const int64_t base = 4 * p0 * P1;
-    for (int i = lane; i < G * 4 * P1; i += 64)
-        if (i < np * 4 * P1) ch[i] = Mc[base + i];
     __syncthreads();
-    const int g = lane / P1, m = lane - (lane / P1) * P1;
-    if (g >= np) return;
-    const uint32_t b = (uint32_t)(p0 + g);
-    W acc = W{0, 0};
-    if (box_count(start_s, b, shift) > 0) {
-        for (int c = 0; c < 4; ++c) {
-            const V *chc = ch + (g * 4 + c) * P1;
-            const W *tr = tab + ((int64_t)c * P1 + m) * P1;
-#pragma unroll
-            for (int k = 0; k < P1; ++k) {
-                if (k <= m) {
-                    const V a = chc[k];
-                    const W tk = tr[k];
-                    acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
-                    acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
-                }
-            }
+    if (tid < p1) {
+        W acc = part[tid];
+        const int ng = nd < NG ? nd : NG;
+        for (int g = 1; g < ng; ++g) {
+            acc.x += part[g * PW + tid].x;
+            acc.y += part[g * PW + tid].y;
         }
+        M[(u.off[i] + box) * p1 + tid] = V{(T)acc.x, (T)acc.y};
     }
-    Mp[(int64_t)b * P1 + m] = V{(T)acc.x, (T)acc.y};
 }
 
-template <class T, class TC, int P1>
-__global__ __launch_bounds__(64) void l2l_lds_kernel(uint32_t nchild,
-                                                     const typename C2<T>::type *__restrict__ Bp,
-                                                     typename C2<T>::type *__restrict__ Bc,
-                                                     const typename C2<TC>::type *__restrict__ tab,
-                                                     const int32_t *__restrict__ start_t,
-                                                     int shift)
+// Downward pass in one launch: the local expansion of leaf b gains, for
+// every ancestor level a = 2 .. Lv-1, the ancestor's own (M2L) expansion
+// shifted to the leaf, B_m += rho^m sum_{k >= m} C(k, m) d^(k-m) B_k
+// (d: leaf centre from the ancestor centre in ancestor units,
+// rho = 2^-(Lv - a)), the composition of the one-level shifts, ancestors
+// added from the coarsest.  One workgroup per leaf parent: the shared
+// ancestors' coefficients and the Pascal triangle are staged in LDS,
+// thread PW s + m serves coefficient m of sibling leaf s.
+struct DownLevels {
+    int Lv;
+    int64_t off[13];  // coefficient offset (boxes) of each level
+};
+
+template <class T, class TC, int PW>
+__global__ __launch_bounds__(256) void l2l_leaf_kernel(DownLevels dl, int p1,
+                                                       typename C2<T>::type *__restrict__ B,
+                                                       const TC *__restrict__ pas,
+                                                       const int32_t *__restrict__ start_t)
 {
     using V = typename C2<T>::type;
     using W = typename C2<TC>::type;
-    constexpr int G = 64 / P1;
-    __shared__ V par[G * P1];
-    const int lane = threadIdx.x;
-    const int g = lane / P1, m = lane - (lane / P1) * P1;
-    const int64_t b64 = (int64_t)blockIdx.x * G + g;
-    const bool in = g < G && b64 < (int64_t)nchild;
-    const uint32_t b = (uint32_t)b64;
-    if (in) par[g * P1 + m] = Bp[(int64_t)(b >> 2) * P1 + m];
-    __syncthreads();
-    if (!in || box_count(start_t, b, shift) == 0) return;
-    const W *tr = tab + ((int64_t)(b & 3) * P1 + m) * P1;
-    W acc = W{0, 0};
-#pragma unroll
-    for (int j = 0; j < P1; ++j) {
-        if (j >= m) {
-            const V a = par[g * P1 + j];
-            const W tk = tr[j];
-            acc.x += tk.x * (TC)a.x - tk.y * (TC)a.y;
-            acc.y += tk.x * (TC)a.y + tk.y * (TC)a.x;
-        }
+    __shared__ V anc[10 * PW];
+    __shared__ TC pl[PW * PW];
+    const int Lv = dl.Lv, tid = threadIdx.x;
+    const uint32_t q = blockIdx.x;  // leaf parent (level Lv - 1)
+    const int na = Lv - 2;          // ancestor levels 2 .. Lv-1
+    for (int e = tid; e < na * p1; e += 256) {
+        const int ai = e / p1, k = e - ai * p1, a = 2 + ai;
+        anc[e] = B[(dl.off[a] + (q >> (2 * (Lv - 1 - a)))) * p1 + k];
     }
-    const V cur = Bc[(int64_t)b * P1 + m];
-    Bc[(int64_t)b * P1 + m] = V{(T)((TC)cur.x + acc.x), (T)((TC)cur.y + acc.y)};
+    for (int e = tid; e < p1 * p1; e += 256) pl[e] = pas[e];
+    __syncthreads();
+    const int s = tid / PW, m = tid - s * PW;
+    if (s >= 4 || m >= p1) return;
+    const uint32_t b = 4 * q + (uint32_t)s;
+    if (start_t[b + 1] <= start_t[b]) return;
+    W acc = W{0, 0};
+    for (int ai = 0; ai < na; ++ai) {
+        const int D = Lv - (2 + ai);
+        TC dx, dy;
+        descendant_offset<TC>(b & ((1u << (2 * D)) - 1u), D, dx, dy);
+        const V *ba = anc + ai * p1;
+        W sm = W{0, 0}, dp = W{1, 0};
+        for (int k = m; k < p1; ++k) {
+            const V bk = ba[k];
+            const TC f = pl[k * p1 + m];
+            const TC tx = f * dp.x, ty = f * dp.y;
+            sm.x += tx * (TC)bk.x - ty * (TC)bk.y;
+            sm.y += tx * (TC)bk.y + ty * (TC)bk.x;
+            const TC nx = dp.x * dx - dp.y * dy;
+            dp.y = dp.x * dy + dp.y * dx;
+            dp.x = nx;
+        }
+        const TC rho = (TC)1 / (TC)(1u << D);
+        TC rm = (TC)1;
+        for (int k = 0; k < m; ++k) rm *= rho;
+        acc.x += rm * sm.x;
+        acc.y += rm * sm.y;
+    }
+    V *bl = B + (dl.off[Lv] + b) * p1;
+    const V cur = bl[m];
+    bl[m] = V{(T)((TC)cur.x + acc.x), (T)((TC)cur.y + acc.y)};
 }
 
 // ---------------------------------------------------------------- M2L ----
@@ -475,26 +487,6 @@ __global__ __launch_bounds__(64) void m2l_box_kernel(
 }
 
 // ------------------------------------------------------ P2P + L2P (leaf) --
-// float32 sources are stored per leaf in ranges padded to a multiple of 4
-// with zero-weight entries (exact zeros in every sum they enter), so the
-// four lane groups of a wave can split a leaf's sources without a tail.
-__global__ void gather_src_f32p(const double *__restrict__ x, const double *__restrict__ y,
-                                const double *__restrict__ w, const uint32_t *__restrict__ skeys,
-                                const int32_t *__restrict__ perm, const int32_t *__restrict__ start,
-                                const int32_t *__restrict__ sp, int64_t n,
-                                const double4 *__restrict__ org, double s,
-                                float4 *__restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t leaf = skeys[i];
-    const int32_t j = perm[i];
-    double cx, cy;
-    leaf_center(leaf, org->x, org->z, s, cx, cy);
-    out[sp[leaf] + (i - start[leaf])] =
-        make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
-}
-
 // One workgroup of 4 waves per leaf (p2p_leaf): the sources of the leaf's
 // near leaves are staged in LDS, every wave sweeps a quarter of them for
 // every target, and per-wave TwoSum partials are merged in a fixed order.
@@ -546,14 +538,15 @@ __device__ __forceinline__ void p2p_leaf_f32s(
     const int32_t ts = sp[b];
     auto tpos = [&](int32_t i) {
         if (tgt) return tgt[i];
-        const float4 q = src[ts + (i - t0)];  // targets are the sources (padded array)
+        const float4 q = src[ts + (i - t0)];  // targets are the sources
         return make_float2(q.x, q.y);
     };
     auto out = [&](int32_t i, float ha, float la, float hb, float lb) {
         double fr = 0.0, fi = 0.0;
         const float2 tp = tpos(i);
         if (Bleaf) l2p_f32(Bleaf + (int64_t)b * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
-        const int32_t o = perm_t[i];
+        // input index: the permutation, or the source record's index bits
+        const int32_t o = perm_t ? perm_t[i] : __float_as_int(src[ts + (i - t0)].w);
         const double qr = -(((double)ha + (double)la) * (double)lam + fr);
         const double qi = ((double)hb + (double)lb) * (double)lam - fi;
         if (velocity_out) {
@@ -891,24 +884,26 @@ struct TableBlob {
         if (!v.empty()) std::memcpy(host.data() + at, v.data(), sizeof(T) * v.size());
         fix.push_back({at, reinterpret_cast<void **>(d)});
     }
-    // keep != nullptr (launches being recorded into a graph): the copy
-    // reads a pinned buffer of its own, handed to *keep, so that every
-    // replay copies the same tables
-    int send(Lane *L, char **keep = nullptr)
+    // keep != nullptr (launches being recorded into a graph): the tables
+    // go once to a device buffer of their own handed to *keep, copied on
+    // the side stream (not recorded) and waited for, so the graph holds no
+    // copy (the tables depend only on what keys the graph)
+    int send(Lane *L, char **keep = nullptr, hipStream_t side = nullptr)
     {
         if (host.empty()) return PHF_OK;
-        char *h = nullptr;
+        char *dev;
         if (keep) {
-            PHF_HIP(hipHostMalloc(&h, host.size(), hipHostMallocDefault));
-            *keep = h;
+            PHF_HIP(hipMalloc(&dev, host.size()));
+            *keep = dev;
+            PHF_HIP(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, side));
+            PHF_HIP(hipStreamSynchronize(side));
         } else {
             PHF_TRY(L->pin.ensure(host.size()));
-            h = L->pin.ptr;
+            std::memcpy(L->pin.ptr, host.data(), host.size());
+            PHF_ALLOC(dev, L->arena, char, host.size());
+            PHF_HIP(hipMemcpyAsync(dev, L->pin.ptr, host.size(), hipMemcpyHostToDevice,
+                                   L->stream));
         }
-        std::memcpy(h, host.data(), host.size());
-        char *dev;
-        PHF_ALLOC(dev, L->arena, char, host.size());
-        PHF_HIP(hipMemcpyAsync(dev, h, host.size(), hipMemcpyHostToDevice, L->stream));
         for (auto &f : fix) *f.second = dev + f.first;
         return PHF_OK;
     }
@@ -939,54 +934,64 @@ int fmm_level(int64_t nb, int leaf, int ndirect)
 
 
 
-// M2M / L2L of one level: the LDS-staged unrolled kernels for the common
-// orders, the generic ones otherwise.
-template <class T, class TC>
-static int shift_up(Lane *L, int p1, uint32_t np, const void *Mc, void *Mp, const void *tab,
-                    const int32_t *start, int shift)
+// Multi-level M2M and leaf L2L launches: thread width PW = the smallest
+// compiled width >= p1.
+template <class T, class TC, int PW>
+static int launch_up(Lane *L, int p1, UpLevels u, void *M, const TC *pas)
 {
     using V = typename C2<T>::type;
-    using W = typename C2<TC>::type;
-#define PHF_M2M(P)                                                                             \
-    if (p1 == P) {                                                                             \
-        m2m_lds_kernel<T, TC, P><<<div_up(np, 64 / P), 64, 0, L->stream>>>(                    \
-            np, (const V *)Mc, (V *)Mp, (const W *)tab, start, shift);                         \
-        PHF_HIP(hipGetLastError());                                                            \
-        return PHF_OK;                                                                         \
-    }
-    PHF_M2M(9)
-    PHF_M2M(13)
-    PHF_M2M(17)
-    PHF_M2M(25)
-#undef PHF_M2M
-    m2m_kernel<T, TC><<<div_up((int64_t)np * p1, 256), 256, 0, L->stream>>>(
-        p1, np, (const V *)Mc, (V *)Mp, (const W *)tab, start, shift);
+    u.blk[0] = 0;
+    for (int i = 0; i < u.nlev; ++i) u.blk[i + 1] = u.blk[i] + (int64_t(1) << (2 * (u.lev0 + i)));
+    m2m_multi_kernel<T, TC, PW><<<(unsigned)u.blk[u.nlev], 256, 0, L->stream>>>(u, p1, (V *)M,
+                                                                               pas);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
 
 template <class T, class TC>
-static int shift_down(Lane *L, int p1, uint32_t nc, const void *Bp, void *Bc, const void *tab,
-                      const int32_t *start, int shift)
+static int shift_up_multi(Lane *L, int p1, const UpLevels &u, void *M, const TC *pas)
+{
+#define PHF_UP(P) \
+    if (p1 <= P) return launch_up<T, TC, P>(L, p1, u, M, pas);
+    PHF_UP(9)
+    PHF_UP(13)
+    PHF_UP(17)
+    PHF_UP(21)
+    PHF_UP(25)
+    PHF_UP(33)
+    PHF_UP(41)
+#undef PHF_UP
+    set_error("expansion order %d too large (max 40)", p1 - 1);
+    return PHF_ERR_UNSUPPORTED;
+}
+
+template <class T, class TC, int PW>
+static int launch_down(Lane *L, int p1, const DownLevels &dl, void *B, const TC *pas,
+                       const int32_t *start)
 {
     using V = typename C2<T>::type;
-    using W = typename C2<TC>::type;
-#define PHF_L2L(P)                                                                             \
-    if (p1 == P) {                                                                             \
-        l2l_lds_kernel<T, TC, P><<<div_up(nc, 64 / P), 64, 0, L->stream>>>(                    \
-            nc, (const V *)Bp, (V *)Bc, (const W *)tab, start, shift);                         \
-        PHF_HIP(hipGetLastError());                                                            \
-        return PHF_OK;                                                                         \
-    }
-    PHF_L2L(9)
-    PHF_L2L(13)
-    PHF_L2L(17)
-    PHF_L2L(25)
-#undef PHF_L2L
-    l2l_kernel<T, TC><<<div_up((int64_t)nc * p1, 256), 256, 0, L->stream>>>(
-        p1, nc, (const V *)Bp, (V *)Bc, (const W *)tab, start, shift);
+    l2l_leaf_kernel<T, TC, PW><<<(unsigned)(int64_t(1) << (2 * (dl.Lv - 1))), 256, 0,
+                                 L->stream>>>(dl, p1, (V *)B, pas, start);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
+}
+
+template <class T, class TC>
+static int shift_down_leaf(Lane *L, int p1, const DownLevels &dl, void *B, const TC *pas,
+                           const int32_t *start)
+{
+#define PHF_DOWN(P) \
+    if (p1 <= P) return launch_down<T, TC, P>(L, p1, dl, B, pas, start);
+    PHF_DOWN(9)
+    PHF_DOWN(13)
+    PHF_DOWN(17)
+    PHF_DOWN(21)
+    PHF_DOWN(25)
+    PHF_DOWN(33)
+    PHF_DOWN(41)
+#undef PHF_DOWN
+    set_error("expansion order %d too large (max 40)", p1 - 1);
+    return PHF_ERR_UNSUPPORTED;
 }
 
 // M2L of every level >= 2 in one launch: the generic coefficient-per-lane
@@ -1036,7 +1041,8 @@ using P2PLaunch = std::function<int(hipStream_t)>;
 template <class T, class TC>
 static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lists &ls,
                          const SortedPoints &S, const SortedPoints &Tg, StageTimer &tm,
-                         bool *used, FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep)
+                         bool *used, FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep,
+                         hipStream_t side)
 {
     using V = typename C2<T>::type;
     using VC = typename C2<TC>::type;
@@ -1046,22 +1052,11 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     // sorted point data (the tree stage began in fmm_device)
     void *src, *tgt;
     const int32_t *perm_t = Tg.perm;
-    const int64_t nleaf64 = int64_t(1) << (2 * Lv);
-    const int32_t *sp = S.start;  // source ranges per leaf (padded for float32)
+    const int32_t *sp = S.start;  // source ranges per leaf
     if (sizeof(T) == 4) {
-        uint32_t *cnt, *spu, *sums;
-        PHF_ALLOC(cnt, L->arena, uint32_t, (size_t)nleaf64 + 1);
-        PHF_ALLOC(spu, L->arena, uint32_t, (size_t)nleaf64 + 1);
-        PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(nleaf64 + 1));
-        PHF_TRY(pad_source_ranges(L, S.start, nleaf64, cnt, spu, sums));
-        sp = reinterpret_cast<const int32_t *>(spu);
-        float4 *s4;
-        PHF_ALLOC(s4, L->arena, float4, (size_t)(a.nb + 3 * nleaf64 + 1));
-        if (a.nb)
-            gather_src_f32p<<<div_up(a.nb, 256), 256, 0, L->stream>>>(
-                a.xb, a.yb, a.wb, S.keys, S.perm, S.start, sp, a.nb, dom.org, dom.s, s4);
-        PHF_TRY(zero_source_pads(L, S.start, sp, nleaf64, s4));
-        float2 *t2 = nullptr;  // targets == sources: P2P reads the padded sources
+        // sorted float32 records from build_sorted_payload
+        float4 *s4 = S.pay;
+        float2 *t2 = nullptr;  // targets == sources: P2P reads the source records
         if (!a.same) {
             PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
             gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(
@@ -1112,22 +1107,12 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         PHF_ALLOC(M, L->arena, V, (size_t)off[Lv + 1] * p1);
         PHF_ALLOC(B, L->arena, V, (size_t)off[Lv + 1] * p1);
 
-        // shift tables (M2M: C(m,k) 2^-k d^(m-k); L2L: C(j,m) d^(j-m) 2^-m)
-        std::vector<VC> tm2m((size_t)4 * p1 * p1), tl2l((size_t)4 * p1 * p1);
-        for (int c = 0; c < 4; ++c) {
-            std::complex<double> d((c & 1) * 0.5 - 0.25, (c >> 1) * 0.5 - 0.25);
-            for (int m = 0; m < p1; ++m)
-                for (int k = 0; k < p1; ++k) {
-                    std::complex<double> v1 = 0, v2 = 0;
-                    if (k <= m) v1 = binom_d(m, k) * std::ldexp(1.0, -k) * std::pow(d, m - k);
-                    if (k >= m) v2 = binom_d(k, m) * std::pow(d, k - m) * std::ldexp(1.0, -m);
-                    tm2m[((size_t)c * p1 + m) * p1 + k] = VC{(TC)v1.real(), (TC)v1.imag()};
-                    tl2l[((size_t)c * p1 + m) * p1 + k] = VC{(TC)v2.real(), (TC)v2.imag()};
-                }
-        }
-        VC *d_m2m = nullptr, *d_l2l = nullptr;
-        blob.add(tm2m, &d_m2m);
-        blob.add(tl2l, &d_l2l);
+        // Pascal triangle C(m, k) for the multi-level shifts
+        std::vector<TC> pas((size_t)p1 * p1);
+        for (int m = 0; m < p1; ++m)
+            for (int k = 0; k < p1; ++k) pas[(size_t)m * p1 + k] = (TC)binom_d(m, k);
+        TC *d_pas = nullptr;
+        blob.add(pas, &d_pas);
         // M2L: binomial rows C(k+m, k) and powers u^k, u = -1/offset
         std::vector<T> bin((size_t)p1 * p1);
         for (int m = 0; m < p1; ++m)
@@ -1163,7 +1148,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         int32_t *d_iloff = nullptr;
         blob.add(il_all, &d_il);
         blob.add(il_offs, &d_iloff);
-        PHF_TRY(blob.send(L, tables_keep));
+        PHF_TRY(blob.send(L, tables_keep, side));
         if (info && info->want_counts) {
             for (int l = 2; l <= Lv; ++l)
                 count_m2l_links<<<div_up(int64_t(1) << (2 * l), 256), 256, 0, L->stream>>>(
@@ -1180,15 +1165,19 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
             src, sp, p1, dom.org, dom.s, Lv, M + off[Lv] * p1);
         PHF_HIP(hipGetLastError());
         tm.end(PHF_STAGE_P2M);
-        // M2M up to level 2
+        // M2M up to level 2, three levels per launch
         tm.begin(PHF_STAGE_M2M);
         used[PHF_STAGE_M2M] = true;
-        for (int l = Lv - 1; l >= 2; --l) {
-            const uint32_t np = 1u << (2 * l);
-            PHF_TRY((shift_up<T, TC>(L, p1, np, M + off[l + 1] * p1, M + off[l] * p1, d_m2m,
-                                     S.start, 2 * (Lv - l))));
+        for (int from = Lv; from > 2;) {
+            UpLevels u{};
+            u.nlev = std::min(3, from - 2);
+            u.lev0 = from - u.nlev;
+            u.from = from;
+            u.off_from = off[from];
+            for (int i = 0; i < u.nlev; ++i) u.off[i] = off[u.lev0 + i];
+            PHF_TRY((shift_up_multi<T, TC>(L, p1, u, M, d_pas)));
+            from -= u.nlev;
         }
-        PHF_HIP(hipGetLastError());
         tm.end(PHF_STAGE_M2M);
         // M2L at every level >= 2 (writes B)
         tm.begin(PHF_STAGE_M2L);
@@ -1222,20 +1211,20 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
                                     d_bin));
         }
         tm.end(PHF_STAGE_M2L);
-        // L2L down
+        // L2L: every ancestor level shifted straight to the leaves
         tm.begin(PHF_STAGE_L2L);
         used[PHF_STAGE_L2L] = true;
-        for (int l = 2; l < Lv; ++l) {
-            const uint32_t nc = 1u << (2 * (l + 1));
-            PHF_TRY((shift_down<T, TC>(L, p1, nc, B + off[l] * p1, B + off[l + 1] * p1, d_l2l,
-                                       Tg.start, 2 * (Lv - l - 1))));
+        if (Lv > 2) {
+            DownLevels dl{};
+            dl.Lv = Lv;
+            for (int l = 0; l <= Lv; ++l) dl.off[l] = off[l];
+            PHF_TRY((shift_down_leaf<T, TC>(L, p1, dl, B, d_pas, Tg.start)));
         }
-        PHF_HIP(hipGetLastError());
         tm.end(PHF_STAGE_L2L);
         Bleaf = B + off[Lv] * p1;
     }
 
-    if (!d_near) PHF_TRY(blob.send(L, tables_keep));
+    if (!d_near) PHF_TRY(blob.send(L, tables_keep, side));
     if (info && info->want_counts) {
         count_p2p_pairs<<<div_up(int64_t(1) << (2 * Lv), 256), 256, 0, L->stream>>>(
             S.start, Tg.start, Lv, d_near, (int)ls.near_leaf.size(), info->d_counts);
@@ -1298,13 +1287,17 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
 // Tree and FMM from the domain on (every launch after the bounding box);
 // p2p_out / tables_keep as run_fmm_typed.
 static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer &tm, bool *used,
-                       FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep, int *nnear)
+                       FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep, hipStream_t side,
+                       int *nnear)
 {
     const double c2 = a.cutoff * a.cutoff;
     const Lists ls = make_lists(dom, c2, a.ws);
     *nnear = (int)ls.near_leaf.size();
     SortedPoints S, Tg;
-    PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
+    if (a.prec == PHF_PREC_F32C)
+        PHF_TRY(build_sorted_payload(L, dom, a.xb, a.yb, a.wb, a.nb, &S));
+    else
+        PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
     if (a.same)
         Tg = S;
     else
@@ -1316,12 +1309,12 @@ static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer 
     if (a.prec == PHF_PREC_F32C) {
         if (a.p > 16)
             return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
-                                                tables_keep);
+                                                tables_keep, side);
         return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
-                                           tables_keep);
+                                           tables_keep, side);
     }
     return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
-                                         tables_keep);
+                                         tables_keep, side);
 }
 
 // ------------------------------------------------------ captured graphs --
@@ -1337,7 +1330,7 @@ struct FmmGraph {
     Arena::Mark m0, m1;
     hipGraph_t g = nullptr;
     hipGraphExec_t exec = nullptr;
-    char *tables = nullptr;  // pinned copy the graph's table upload reads
+    char *tables = nullptr;  // device tables the graph's kernels read
     P2PLaunch p2p;
     uint64_t last = 0;
 };
@@ -1345,6 +1338,7 @@ struct FmmGraph {
 struct FmmGraphCache {
     std::vector<FmmGraph *> v;
     uint64_t clock = 0;
+    hipStream_t side = nullptr;  // uploads made while recording
 };
 
 constexpr size_t kFmmGraphs = 4;
@@ -1353,7 +1347,7 @@ static void drop_graph(FmmGraph *e)
 {
     if (e->exec) (void)hipGraphExecDestroy(e->exec);
     if (e->g) (void)hipGraphDestroy(e->g);
-    if (e->tables) (void)hipHostFree(e->tables);
+    if (e->tables) (void)hipFree(e->tables);
     delete e;
 }
 
@@ -1361,6 +1355,7 @@ void fmm_graphs_release(Lane *L)
 {
     if (!L->fmm_graphs) return;
     for (FmmGraph *e : L->fmm_graphs->v) drop_graph(e);
+    if (L->fmm_graphs->side) (void)hipStreamDestroy(L->fmm_graphs->side);
     delete L->fmm_graphs;
     L->fmm_graphs = nullptr;
 }
@@ -1388,6 +1383,7 @@ static int fmm_graph(Lane *L, const FmmArgs &a, const Domain &dom, FmmGraph **ou
             *keep = true;
             return PHF_OK;
         }
+    if (!c.side) PHF_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
     FmmGraph *e = new FmmGraph;
     e->a = a;
     e->L = dom.L;
@@ -1400,7 +1396,7 @@ static int fmm_graph(Lane *L, const FmmArgs &a, const Domain &dom, FmmGraph **ou
         delete e;
         PHF_HIP(be);
     }
-    int rc = fmm_enqueue(L, a, dom, off, used, nullptr, &e->p2p, &e->tables, &e->nnear);
+    int rc = fmm_enqueue(L, a, dom, off, used, nullptr, &e->p2p, &e->tables, c.side, &e->nnear);
     hipError_t ce = hipStreamEndCapture(L->stream, &e->g);
     if (rc == PHF_OK && ce != hipSuccess) {
         set_error("HIP error %s ending the FMM graph capture", hipGetErrorName(ce));
@@ -1479,7 +1475,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         used[PHF_STAGE_TREE_FAR] = used[PHF_STAGE_P2P] = true;
     } else {
         used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
-        PHF_TRY(fmm_enqueue(L, a, dom, tm, used, info, nullptr, nullptr, &nnear));
+        PHF_TRY(fmm_enqueue(L, a, dom, tm, used, info, nullptr, nullptr, nullptr, &nnear));
     }
     if (info) {
         info->L = dom.L;

@@ -109,22 +109,46 @@ Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L, do
 }
 
 // ----------------------------------------------------------------- keys --
+__device__ __forceinline__ uint32_t morton_key(double x, double y, double x0, double y0,
+                                               double inv_s, int L)
+{
+    const double hi = (double)((1u << L) - 1u);
+    double fx = floor((x - x0) * inv_s);
+    double fy = floor((y - y0) * inv_s);
+    fx = (fx >= 0.0) ? fx : 0.0;
+    fy = (fy >= 0.0) ? fy : 0.0;
+    fx = fx > hi ? hi : fx;
+    fy = fy > hi ? hi : fy;
+    return morton2((uint32_t)fx, (uint32_t)fy);
+}
+
 __global__ void morton_keys_kernel(const double *__restrict__ x, const double *__restrict__ y,
                                    int64_t n, const double4 *__restrict__ org, double inv_s, int L,
                                    uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double x0 = org->x, y0 = org->z;
-    const double hi = (double)((1u << L) - 1u);
-    double fx = floor((x[i] - x0) * inv_s);
-    double fy = floor((y[i] - y0) * inv_s);
-    fx = (fx >= 0.0) ? fx : 0.0;
-    fy = (fy >= 0.0) ? fy : 0.0;
-    fx = fx > hi ? hi : fx;
-    fy = fy > hi ? hi : fy;
-    keys[i] = morton2((uint32_t)fx, (uint32_t)fy);
+    keys[i] = morton_key(x[i], y[i], org->x, org->z, inv_s, L);
     vals[i] = (int32_t)i;
+}
+
+// Morton key and the float32 record (x - cx, y - cy, w, input index bits)
+// of each point, relative to its leaf centre (cx, cy).
+__global__ void morton_payload_kernel(const double *__restrict__ x, const double *__restrict__ y,
+                                      const double *__restrict__ w, int64_t n,
+                                      const double4 *__restrict__ org, double s, double inv_s,
+                                      int L, uint32_t *__restrict__ keys,
+                                      float4 *__restrict__ pay)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x0 = org->x, y0 = org->z, xi = x[i], yi = y[i];
+    const uint32_t key = morton_key(xi, yi, x0, y0, inv_s, L);
+    double cx, cy;
+    leaf_center(key, x0, y0, s, cx, cy);
+    keys[i] = key;
+    pay[i] = make_float4((float)(xi - cx), (float)(yi - cy), (float)w[i],
+                         __int_as_float((int32_t)i));
 }
 
 // ----------------------------------------------------------------- scan --
@@ -234,42 +258,60 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
 
+// The tile's keys (and values) are loaded up front, one load per round in
+// flight at once, before the ranking rounds.
 __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint32_t *__restrict__ keys,
                                                       int64_t n, int shift, uint32_t mask,
                                                       uint32_t *__restrict__ hist, int nblocks)
 {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
-    __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    uint32_t kr[RS_ROUNDS];
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & mask], 1u);
+        const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        kr[r] = i < n ? keys[i] : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&h[(kr[r] >> shift) & mask], 1u);
     }
     __syncthreads();
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+template <class VT>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
-    const uint32_t *__restrict__ kin, const int32_t *__restrict__ vin,
-    uint32_t *__restrict__ kout, int32_t *__restrict__ vout, int64_t n, int shift, int nbits,
-    const uint32_t *__restrict__ offs, int nblocks)
+    const uint32_t *__restrict__ kin, const VT *__restrict__ vin, uint32_t *__restrict__ kout,
+    VT *__restrict__ vout, int64_t n, int shift, int nbits, const uint32_t *__restrict__ offs,
+    int nblocks)
 {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t mask = (1u << nbits) - 1u;
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    uint32_t kr[RS_ROUNDS];
+    VT vr[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = base + (int64_t)r * RS_THREADS + tid;
+        kr[r] = i < n ? kin[i] : 0u;
+        vr[r] = i < n ? vin[i] : VT{};
+    }
     run[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
 #pragma unroll
     for (int w = 0; w < RS_THREADS / 64; ++w) wcnt[w][tid] = 0;
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
         const int64_t i = base + (int64_t)r * RS_THREADS + tid;
         const bool valid = i < n;
-        const uint32_t key = valid ? kin[i] : 0u;
-        const int32_t val = valid ? vin[i] : 0;
+        const uint32_t key = kr[r];
         const uint32_t d = (key >> shift) & mask;
         uint64_t m = __ballot(valid);
         for (int b = 0; b < nbits; ++b) {
@@ -284,7 +326,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
             uint32_t pos = run[d] + rank;
             for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
             kout[pos] = key;
-            vout[pos] = val;
+            vout[pos] = vr[r];
         }
         __syncthreads();
         uint32_t add = 0;
@@ -319,23 +361,24 @@ int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, in
                                 vres);
 }
 
-int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
-                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
-                         uint32_t **kres, int32_t **vres)
+template <class VT>
+static int radix_sort_ws(Lane *L, uint32_t *keys, VT *vals, uint32_t *ktmp, VT *vtmp, int64_t n,
+                         int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
+                         uint32_t **kres, VT **vres)
 {
     *kres = keys;
     *vres = vals;
     if (n <= 1 || key_bits <= 0) return PHF_OK;
     const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
     uint32_t *ka = keys, *kb = ktmp;
-    int32_t *va = vals, *vb = vtmp;
+    VT *va = vals, *vb = vtmp;
     for (int shift = 0; shift < key_bits; shift += 8) {
         const int nbits = std::min(8, key_bits - shift);
         const uint32_t mask = (1u << nbits) - 1u;
         rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
         PHF_TRY(device_exclusive_scan_ws(L, hist, (int64_t)256 * nblocks, offs, sums));
-        rs_scatter<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits, offs,
-                                                          nblocks);
+        rs_scatter<VT><<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits,
+                                                              offs, nblocks);
         PHF_HIP(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);
@@ -343,6 +386,14 @@ int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp,
     *kres = ka;
     *vres = va;
     return PHF_OK;
+}
+
+int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
+                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
+                         uint32_t **kres, int32_t **vres)
+{
+    return radix_sort_ws<int32_t>(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, sums, kres,
+                                  vres);
 }
 
 // --------------------------------------------------------- leaf offsets --
@@ -401,6 +452,44 @@ int build_sorted_points(Lane *L, const Domain &dom, const double *x, const doubl
     out->n = n;
     out->keys = ks;
     out->perm = vs;
+    out->start = start;
+    return PHF_OK;
+}
+
+int build_sorted_payload(Lane *L, const Domain &dom, const double *x, const double *y,
+                         const double *w, int64_t n, SortedPoints *out)
+{
+    uint32_t *k0, *k1;
+    float4 *p0, *p1;
+    const size_t m = (size_t)(n > 0 ? n : 1);
+    PHF_ALLOC(k0, L->arena, uint32_t, m);
+    PHF_ALLOC(k1, L->arena, uint32_t, m);
+    PHF_ALLOC(p0, L->arena, float4, m);
+    PHF_ALLOC(p1, L->arena, float4, m);
+    const int64_t nleaf = int64_t(1) << (2 * dom.L);
+    int32_t *start;
+    PHF_ALLOC(start, L->arena, int32_t, (size_t)nleaf + 1);
+    if (n > 0)
+        morton_payload_kernel<<<div_up(n, 256), 256, 0, L->stream>>>(
+            x, y, w, n, dom.org, dom.s, dom.inv_s, dom.L, k0, p0);
+    PHF_HIP(hipGetLastError());
+    uint32_t *ks = k0;
+    float4 *ps = p0;
+    if (n > 1) {
+        const int64_t ws = radix_ws_size(n);
+        uint32_t *hist, *offs, *sums;
+        PHF_ALLOC(hist, L->arena, uint32_t, (size_t)ws);
+        PHF_ALLOC(offs, L->arena, uint32_t, (size_t)ws);
+        PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(ws));
+        PHF_TRY(radix_sort_ws<float4>(L, k0, p0, k1, p1, n, 2 * dom.L, hist, offs, sums, &ks,
+                                      &ps));
+    }
+    leaf_starts_kernel<<<div_up(nleaf + 1, 256), 256, 0, L->stream>>>(ks, n, nleaf, start);
+    PHF_HIP(hipGetLastError());
+    out->n = n;
+    out->keys = ks;
+    out->perm = nullptr;
+    out->pay = ps;
     out->start = start;
     return PHF_OK;
 }

@@ -51,6 +51,9 @@ struct SortedPoints {
     int32_t *start = nullptr;     // leaf offsets, 4^L + 1 entries
     uint32_t *keys_in = nullptr;  // keys in input order (if requested)
     bool want_input_keys = false;
+    // build_sorted_payload: sorted float32 records (x - cx, y - cy, w,
+    // input index bits) relative to the leaf centres, perm unset
+    float4 *pay = nullptr;
 };
 
 // start[k] = first position of sorted keys with key >= k, k = 0..nkeys.
@@ -58,5 +61,9 @@ int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys,
 
 int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
                         SortedPoints *out);
+// The same order with the points' float32 records sorted along (no
+// permutation gather afterwards).
+int build_sorted_payload(Lane *L, const Domain &dom, const double *x, const double *y,
+                         const double *w, int64_t n, SortedPoints *out);
 
 }  // namespace phf
