@@ -1318,11 +1318,15 @@ static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer 
 }
 
 // ------------------------------------------------------ captured graphs --
-// A lane's recorded FMM launch sequences.  An entry is valid for the same
-// inputs and outputs (addresses), sizes, parameters, leaf level and
-// domain side, and the same arena state at the start (its buffers are the
-// same arena slices); the domain corner is read on the device, so moving
-// points replay the same graph while the side stays.
+// A lane's recorded FMM launch sequences: bounding box, tree and far field
+// (the near field is launched after each replay, timed on its own).  An
+// entry is valid for the same inputs and outputs (addresses), sizes and
+// parameters and the same arena state at the start (its buffers are the
+// same arena slices).  A call replays the most recent entry without
+// waiting for the bounding box: kernels read the domain corner on the
+// device, and the bounding box comes back with the results, so that the
+// caller can check that the points kept the recording's leaf level and
+// domain side (else it evaluates again with a fresh recording).
 struct FmmGraph {
     FmmArgs a{};
     int L = 0, nnear = 0;
@@ -1330,7 +1334,8 @@ struct FmmGraph {
     Arena::Mark m0, m1;
     hipGraph_t g = nullptr;
     hipGraphExec_t exec = nullptr;
-    char *tables = nullptr;  // device tables the graph's kernels read
+    char *tables = nullptr;     // device tables the graph's kernels read
+    double4 *bb = nullptr;      // the recorded bounding box (arena slot)
     P2PLaunch p2p;
     uint64_t last = 0;
 };
@@ -1338,7 +1343,8 @@ struct FmmGraph {
 struct FmmGraphCache {
     std::vector<FmmGraph *> v;
     uint64_t clock = 0;
-    hipStream_t side = nullptr;  // uploads made while recording
+    hipStream_t side = nullptr;    // uploads made while recording
+    double4 *hb_host = nullptr;    // pinned copy of the replayed bounding box
 };
 
 constexpr size_t kFmmGraphs = 4;
@@ -1356,6 +1362,7 @@ void fmm_graphs_release(Lane *L)
     if (!L->fmm_graphs) return;
     for (FmmGraph *e : L->fmm_graphs->v) drop_graph(e);
     if (L->fmm_graphs->side) (void)hipStreamDestroy(L->fmm_graphs->side);
+    if (L->fmm_graphs->hb_host) (void)hipHostFree(L->fmm_graphs->hb_host);
     delete L->fmm_graphs;
     L->fmm_graphs = nullptr;
 }
@@ -1369,25 +1376,16 @@ static bool same_args(const FmmArgs &x, const FmmArgs &y)
            x.velocity_out == y.velocity_out;
 }
 
-// Find or record the graph for (a, dom) at the arena's current state; the
-// arena then stands where the recorded sequence left it.
-static int fmm_graph(Lane *L, const FmmArgs &a, const Domain &dom, FmmGraph **out, bool *keep)
+// Record bounding box, tree and far field for (a, dom) from the arena's
+// current state.
+static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **out, bool *keep)
 {
-    if (!L->fmm_graphs) L->fmm_graphs = new FmmGraphCache;
     FmmGraphCache &c = *L->fmm_graphs;
-    for (FmmGraph *e : c.v)
-        if (same_args(e->a, a) && e->L == dom.L && e->D == dom.D && L->arena.at(e->m0)) {
-            L->arena.jump(e->m0, e->m1);
-            e->last = ++c.clock;
-            *out = e;
-            *keep = true;
-            return PHF_OK;
-        }
     if (!c.side) PHF_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
     FmmGraph *e = new FmmGraph;
     e->a = a;
-    e->L = dom.L;
-    e->D = dom.D;
+    e->L = dom0.L;
+    e->D = dom0.D;
     e->m0 = L->arena.mark();
     StageTimer off(L, false);
     bool used[PHF_STAGE_COUNT] = {};
@@ -1396,7 +1394,18 @@ static int fmm_graph(Lane *L, const FmmArgs &a, const Domain &dom, FmmGraph **ou
         delete e;
         PHF_HIP(be);
     }
-    int rc = fmm_enqueue(L, a, dom, off, used, nullptr, &e->p2p, &e->tables, c.side, &e->nnear);
+    int rc = PHF_OK;
+    double4 *bb = L->arena.alloc<double4>(1);
+    e->bb = bb;
+    if (!bb) rc = PHF_ERR_NOMEM;
+    if (rc == PHF_OK)
+        rc = device_bbox(L, a.xb, a.yb, a.nb, a.same ? nullptr : a.xt, a.same ? nullptr : a.yt,
+                         a.same ? 0 : a.nt, bb);
+    if (rc == PHF_OK) {
+        Domain dom = dom0;
+        dom.org = bb;
+        rc = fmm_enqueue(L, a, dom, off, used, nullptr, &e->p2p, &e->tables, c.side, &e->nnear);
+    }
     hipError_t ce = hipStreamEndCapture(L->stream, &e->g);
     if (rc == PHF_OK && ce != hipSuccess) {
         set_error("HIP error %s ending the FMM graph capture", hipGetErrorName(ce));
@@ -1420,6 +1429,15 @@ static int fmm_graph(Lane *L, const FmmArgs &a, const Domain &dom, FmmGraph **ou
     // graph then serves this call only
     *keep = e->m1.nblocks == e->m0.nblocks;
     if (*keep) {
+        // one recording per argument set and arena state: the newest
+        for (auto it = c.v.begin(); it != c.v.end();)
+            if (same_args((*it)->a, a) && (*it)->m0.base0 == e->m0.base0 &&
+                (*it)->m0.cur == e->m0.cur && (*it)->m0.top == e->m0.top) {
+                drop_graph(*it);
+                it = c.v.erase(it);
+            } else {
+                ++it;
+            }
         if (c.v.size() >= kFmmGraphs) {
             auto lru = std::min_element(c.v.begin(), c.v.end(), [](FmmGraph *x, FmmGraph *y) {
                 return x->last < y->last;
@@ -1437,24 +1455,46 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
     if (a.nt == 0) return PHF_OK;
     const bool graph = info && info->graph && !info->want_counts;
     const int stage0 = graph ? PHF_STAGE_TREE_FAR : PHF_STAGE_TREE;
-    // 1. bounding box of sources and targets -> domain
-    tm.begin(stage0);
-    double4 *bb;
-    PHF_ALLOC(bb, L->arena, double4, 1);
-    PHF_TRY(device_bbox(L, a.xb, a.yb, a.nb, a.same ? nullptr : a.xt, a.same ? nullptr : a.yt,
-                        a.same ? 0 : a.nt, bb));
-    double4 hb;
-    PHF_HIP(hipMemcpyAsync(&hb, bb, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
-    PHF_HIP(hipStreamSynchronize(L->stream));
     const double shape_log2 = 1.4426950408889634074 * make_oseen_params(a.xopt, a.cutoff).inv_ks;
-    Domain dom = choose_domain(hb, a.Lhint, (double)a.nb, (double)a.nt, a.cutoff, a.ws,
-                               shape_log2, a.p + 1, a.prec == PHF_PREC_F32C ? 16 : 64);
-    dom.org = bb;
-    int nnear = 0;
+    FmmGraph *e = nullptr;
+    bool keep = true;
     if (graph) {
-        FmmGraph *e = nullptr;
-        bool keep = true;
-        PHF_TRY(fmm_graph(L, a, dom, &e, &keep));
+        if (!L->fmm_graphs) L->fmm_graphs = new FmmGraphCache;
+        FmmGraphCache &c = *L->fmm_graphs;
+        if (!c.hb_host) PHF_HIP(hipHostMalloc(&c.hb_host, 256, hipHostMallocDefault));
+        if (!info->no_speculation)
+            for (FmmGraph *g : c.v)
+                if (same_args(g->a, a) && L->arena.at(g->m0)) e = g;
+        if (e) {
+            // replay without waiting for the bounding box
+            L->arena.jump(e->m0, e->m1);
+            e->last = ++c.clock;
+            tm.begin(stage0);
+        }
+    }
+    Domain dom{};
+    if (!e) {
+        // 1. bounding box of sources and targets -> domain
+        tm.begin(stage0);
+        const Arena::Mark m0 = L->arena.mark();
+        double4 *bb;
+        PHF_ALLOC(bb, L->arena, double4, 1);
+        PHF_TRY(device_bbox(L, a.xb, a.yb, a.nb, a.same ? nullptr : a.xt,
+                            a.same ? nullptr : a.yt, a.same ? 0 : a.nt, bb));
+        double4 hb;
+        PHF_HIP(hipMemcpyAsync(&hb, bb, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        dom = choose_domain(hb, a.Lhint, (double)a.nb, (double)a.nt, a.cutoff, a.ws, shape_log2,
+                            a.p + 1, a.prec == PHF_PREC_F32C ? 16 : 64);
+        dom.org = bb;
+        if (graph) {
+            // the recording computes the bounding box again in its own slot
+            L->arena.jump(L->arena.mark(), m0);
+            PHF_TRY(fmm_record(L, a, dom, &e, &keep));
+        }
+    }
+    int nnear = 0;
+    if (e) {
         hipError_t le = hipGraphLaunch(e->exec, L->stream);
         tm.end(stage0);
         int rc = le == hipSuccess ? PHF_OK : PHF_ERR_DEVICE;
@@ -1464,9 +1504,28 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
             rc = e->p2p(L->stream);
             tm.end(PHF_STAGE_P2P);
         }
+        if (rc == PHF_OK) {
+            double4 *hb = L->fmm_graphs->hb_host;
+            const hipError_t fe = hipMemcpyAsync(hb, e->bb, sizeof(double4),
+                                                 hipMemcpyDeviceToHost, L->stream);
+            if (fe != hipSuccess) {
+                set_error("HIP error %s reading the bounding box", hipGetErrorName(fe));
+                rc = PHF_ERR_DEVICE;
+            }
+            const int L0 = e->L;
+            const double D0 = e->D;
+            info->domain_ok = [hb, a, L0, D0, shape_log2]() {
+                const Domain d = choose_domain(*hb, a.Lhint, (double)a.nb, (double)a.nt,
+                                               a.cutoff, a.ws, shape_log2, a.p + 1,
+                                               a.prec == PHF_PREC_F32C ? 16 : 64);
+                return d.L == L0 && d.D == D0;
+            };
+        }
         nnear = e->nnear;
+        dom.L = e->L;
+        dom.D = e->D;
         if (!keep) {
-            // the graph and its pinned tables must outlive the launches
+            // the graph and its tables must outlive the launches
             const hipError_t se = hipStreamSynchronize(L->stream);
             drop_graph(e);
             if (rc == PHF_OK && se != hipSuccess) PHF_HIP(se);

@@ -3,6 +3,8 @@
 
 #include "phf_tree.hpp"
 
+#include <functional>
+
 namespace phf {
 
 struct FmmArgs {
@@ -22,15 +24,21 @@ struct FmmInfo {
     double D = 0.0;
     bool want_counts = false;                // fill d_counts = {P2P pairs, M2L links}
     bool graph = false;                      // record / replay the launches as a HIP graph
+    bool no_speculation = false;             // wait for the bounding box before replaying
+    // replayed graph: after the stream is synchronised, false when the
+    // points needed another leaf level or domain side than the recording:
+    // the results are then invalid and the call must be made again with
+    // no_speculation
+    std::function<bool()> domain_ok;
     unsigned long long *d_counts = nullptr;  // device, arena-owned
 };
 
 int fmm_order_from_tol(double tol);
 int fmm_level(int64_t nb, int leaf, int ndirect);
-// With info->graph set (and no work counts) the launch sequence after the
-// bounding box is recorded as a HIP graph in the lane's cache and replayed
-// on later calls with the same arguments, leaf level and domain side (the
-// stage then timed is PHF_STAGE_TREE_FAR).
+// With info->graph set (and no work counts) the launch sequence (bounding
+// box through L2L) is recorded as a HIP graph in the lane's cache and
+// replayed on later calls with the same arguments (the stage then timed
+// is PHF_STAGE_TREE_FAR); see FmmInfo::domain_flag.
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info);
 void fmm_graphs_release(Lane *L);
 // float32 source ranges per leaf padded to multiples of 4: sp[b] (nleaf + 1

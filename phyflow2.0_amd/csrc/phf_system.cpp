@@ -215,32 +215,42 @@ int phf_system_velocity(phf_system *h, const phf_vel_params *p, double *vx, doub
     std::lock_guard<std::mutex> lk(s->mu);
     PHF_HIP(hipSetDevice(Context::get().device()));
     Lane *L = &s->lane;
-    PHF_TRY(L->arena.reset());
     const bool timed = info != nullptr || timing_enabled();
     StageTimer tm(L, timed);
     bool used[PHF_STAGE_COUNT] = {};
-    used[PHF_STAGE_TOTAL] = true;
     FmmInfo fi;
-    fi.want_counts = info != nullptr && (info->flags & PHF_INFO_COUNT_WORK) &&
-                     p->method == PHF_METHOD_FMM && p->tol > 0;
-    // resident buffers keep their addresses: the FMM launch sequence is
-    // recorded once and replayed
-    fi.graph = !(info != nullptr && (info->flags & PHF_INFO_EAGER));
-    tm.begin(PHF_STAGE_TOTAL);
-    PHF_TRY(system_velocity_device(L, s->x, s->y, s->w, s->n, *p, s->vx, s->vy, tm, used, &fi));
-    if (vx && vy && s->n > 0) {
-        tm.begin(PHF_STAGE_DOWNLOAD);
-        used[PHF_STAGE_DOWNLOAD] = true;
-        PHF_HIP(hipMemcpyAsync(vx, s->vx, sizeof(double) * s->n, hipMemcpyDeviceToHost, L->stream));
-        PHF_HIP(hipMemcpyAsync(vy, s->vy, sizeof(double) * s->n, hipMemcpyDeviceToHost, L->stream));
-        tm.end(PHF_STAGE_DOWNLOAD);
-    }
-    tm.end(PHF_STAGE_TOTAL);
     unsigned long long counts[2] = {0, 0};
-    if (fi.want_counts && fi.d_counts)
-        PHF_HIP(hipMemcpyAsync(counts, fi.d_counts, sizeof(counts), hipMemcpyDeviceToHost,
-                               L->stream));
-    PHF_HIP(hipStreamSynchronize(L->stream));
+    // resident buffers keep their addresses: the FMM launch sequence is
+    // recorded once and replayed; a replay whose points needed another
+    // domain is made again after a fresh recording
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        PHF_TRY(L->arena.reset());
+        for (bool &u : used) u = false;
+        used[PHF_STAGE_TOTAL] = true;
+        fi = FmmInfo{};
+        fi.want_counts = info != nullptr && (info->flags & PHF_INFO_COUNT_WORK) &&
+                         p->method == PHF_METHOD_FMM && p->tol > 0;
+        fi.graph = !(info != nullptr && (info->flags & PHF_INFO_EAGER));
+        fi.no_speculation = attempt > 0;
+        tm.begin(PHF_STAGE_TOTAL);
+        PHF_TRY(system_velocity_device(L, s->x, s->y, s->w, s->n, *p, s->vx, s->vy, tm, used,
+                                       &fi));
+        if (vx && vy && s->n > 0) {
+            tm.begin(PHF_STAGE_DOWNLOAD);
+            used[PHF_STAGE_DOWNLOAD] = true;
+            PHF_HIP(hipMemcpyAsync(vx, s->vx, sizeof(double) * s->n, hipMemcpyDeviceToHost,
+                                   L->stream));
+            PHF_HIP(hipMemcpyAsync(vy, s->vy, sizeof(double) * s->n, hipMemcpyDeviceToHost,
+                                   L->stream));
+            tm.end(PHF_STAGE_DOWNLOAD);
+        }
+        tm.end(PHF_STAGE_TOTAL);
+        if (fi.want_counts && fi.d_counts)
+            PHF_HIP(hipMemcpyAsync(counts, fi.d_counts, sizeof(counts), hipMemcpyDeviceToHost,
+                                   L->stream));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        if (!fi.domain_ok || fi.domain_ok()) break;
+    }
     s->has_velocity = true;
     record_stage_times(L, used);
     if (info) {
