@@ -75,17 +75,17 @@ __device__ __forceinline__ int floordiv2(int v) { return v >= 0 ? v / 2 : -((1 -
 // Per-system domain: bounding box, side from choose_domain_side (the rule
 // the single-system FMM applies, so a system builds the same tree alone and
 // in the batch).
-__global__ __launch_bounds__(256) void batch_domain(const BatchSys *__restrict__ sys,
-                                                    const double *__restrict__ x,
-                                                    const double *__restrict__ y, int ws, int p1,
-                                                    int chunk, BatchDom *__restrict__ dom)
+// per-chunk bounding boxes {xmin, xmax, ymin, ymax}
+__global__ __launch_bounds__(256) void batch_bbox_part(const BatchChunk *__restrict__ ch,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       double4 *__restrict__ part)
 {
     __shared__ double4 sh[256];
-    const int s = blockIdx.x;
-    const int64_t o = sys[s].off, n = sys[s].n;
+    const BatchChunk c = ch[blockIdx.x];
     double xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
-    for (int64_t i = threadIdx.x; i < n; i += 256) {
-        const double a = x[o + i], b = y[o + i];
+    for (int64_t i = c.beg + threadIdx.x; i < c.end; i += 256) {
+        const double a = x[i], b = y[i];
         xmin = fmin(xmin, a);
         xmax = fmax(xmax, a);
         ymin = fmin(ymin, b);
@@ -95,19 +95,41 @@ __global__ __launch_bounds__(256) void batch_domain(const BatchSys *__restrict__
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) {
-            double4 a = sh[threadIdx.x], b = sh[threadIdx.x + k];
-            sh[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z),
-                                           fmax(a.w, b.w));
+            const double4 p = sh[threadIdx.x], q = sh[threadIdx.x + k];
+            sh[threadIdx.x] = make_double4(fmin(p.x, q.x), fmax(p.y, q.y), fmin(p.z, q.z),
+                                           fmax(p.w, q.w));
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        double4 b = sh[0];
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Per-system domain: bounding box from the chunk partials, side from
+// choose_domain_side (the rule the single-system FMM applies, so a system
+// builds the same tree alone and in the batch).
+__global__ __launch_bounds__(64) void batch_domain(const BatchSys *__restrict__ sys,
+                                                   const int32_t *__restrict__ cfirst,
+                                                   const double4 *__restrict__ part, int ws,
+                                                   int p1, int chunk, BatchDom *__restrict__ dom)
+{
+    const int s = blockIdx.x, lane = threadIdx.x;
+    double4 b = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    for (int k = cfirst[s] + lane; k < cfirst[s + 1]; k += 64) {
+        const double4 q = part[k];
+        b = make_double4(fmin(b.x, q.x), fmax(b.y, q.y), fmin(b.z, q.z), fmax(b.w, q.w));
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        const double4 q = make_double4(__shfl_xor(b.x, d), __shfl_xor(b.y, d),
+                                       __shfl_xor(b.z, d), __shfl_xor(b.w, d));
+        b = make_double4(fmin(b.x, q.x), fmax(b.y, q.y), fmin(b.z, q.z), fmax(b.w, q.w));
+    }
+    if (lane == 0) {
+        const BatchSys sy = sys[s];
+        const int64_t n = sy.n;
         BatchDom d;
         if (n == 0) b = make_double4(0.0, 0.0, 0.0, 0.0);
         d.x0 = b.x;
         d.y0 = b.z;
-        const BatchSys sy = sys[s];
         const int L = sy.L;
         d.D = choose_domain_side(b.y - b.x, b.w - b.z, L, (double)n, (double)n, sy.cutoff, ws,
                                  1.4426950408889634074 * sy.inv_ks, p1, chunk);
@@ -655,6 +677,14 @@ int batch_plan(const int64_t *sizes, const double *sigma, int S, int p, int leaf
     }
     P->N = P->poff[S];
     P->Lmax = Lmax;
+    P->chunks.clear();
+    P->cfirst.assign(S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+        P->cfirst[s] = (int32_t)P->chunks.size();
+        for (int64_t b = P->poff[s]; b < P->poff[s + 1]; b += kBatchChunk)
+            P->chunks.push_back(BatchChunk{b, std::min(b + kBatchChunk, P->poff[s + 1]), s, 0});
+    }
+    P->cfirst[S] = (int32_t)P->chunks.size();
     P->nleaf = P->loff[S];
     PHF_REQUIRE(P->nleaf < (int64_t(1) << 31) && P->N < (int64_t(1) << 31),
                 "batch too large (%lld points, %lld leaves)", (long long)P->N,
@@ -683,6 +713,9 @@ int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
     PHF_ALLOC(Bf->sys, L->arena, BatchSys, (size_t)P.S);
     PHF_ALLOC(Bf->dom, L->arena, BatchDom, (size_t)P.S);
     PHF_ALLOC(Bf->split, L->arena, int32_t, 9);
+    PHF_ALLOC(Bf->chunks, L->arena, BatchChunk, std::max<size_t>(P.chunks.size(), 1));
+    PHF_ALLOC(Bf->cfirst, L->arena, int32_t, (size_t)P.S + 1);
+    PHF_ALLOC(Bf->cpart, L->arena, char, 32 * std::max<size_t>(P.chunks.size(), 1));
     PHF_ALLOC(Bf->poff, L->arena, int64_t, (size_t)P.S + 1);
     PHF_ALLOC(Bf->loff, L->arena, int64_t, (size_t)P.S + 1);
     PHF_ALLOC(Bf->lvl, L->arena, int64_t, P.lvl.size());
@@ -763,6 +796,11 @@ int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
     PHF_HIP(hipMemcpy(Bf->poff, P.poff.data(), sizeof(int64_t) * (P.S + 1), hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->loff, P.loff.data(), sizeof(int64_t) * (P.S + 1), hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->lvl, P.lvl.data(), sizeof(int64_t) * P.lvl.size(), hipMemcpyHostToDevice));
+    if (!P.chunks.empty())
+        PHF_HIP(hipMemcpy(Bf->chunks, P.chunks.data(), sizeof(BatchChunk) * P.chunks.size(),
+                          hipMemcpyHostToDevice));
+    PHF_HIP(hipMemcpy(Bf->cfirst, P.cfirst.data(), sizeof(int32_t) * (P.S + 1),
+                      hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->m2m, h_m2m.data(), h_m2m.size(), hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->l2l, h_l2l.data(), h_l2l.size(), hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->bin, h_bin.data(), h_bin.size(), hipMemcpyHostToDevice));
@@ -798,8 +836,12 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
     const int S = P.S, p1 = P.p + 1;
     const int64_t N = P.N;
     batch_clear_flag<<<1, 1, 0, L->stream>>>(Bf.flag);
-    batch_domain<<<S, 256, 0, L->stream>>>(Bf.sys, x, y, P.ws, P.p + 1,
-                                           P.prec == PHF_PREC_F32C ? 16 : 64, Bf.dom);
+    if (!P.chunks.empty())
+        batch_bbox_part<<<(unsigned)P.chunks.size(), 256, 0, L->stream>>>(
+            Bf.chunks, x, y, reinterpret_cast<double4 *>(Bf.cpart));
+    batch_domain<<<S, 64, 0, L->stream>>>(Bf.sys, Bf.cfirst,
+                                          reinterpret_cast<const double4 *>(Bf.cpart), P.ws,
+                                          P.p + 1, P.prec == PHF_PREC_F32C ? 16 : 64, Bf.dom);
     if (N > 0)
         batch_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.poff, S, Bf.dom, x, y, N,
                                                           Bf.k0, Bf.v0);

@@ -22,12 +22,22 @@ struct BatchDom {
     double x0, y0, D, s, inv_s;  // origin, side, leaf side, 2^L / D
 };
 
+// a run of at most kBatchChunk consecutive points of one system: per-system
+// reductions (bounding boxes) go over chunks in parallel, then per system
+struct BatchChunk {
+    int64_t beg, end;
+    int32_t sys, pad;
+};
+constexpr int64_t kBatchChunk = 4096;
+
 struct BatchPlan {
     int S = 0, p = 0, ws = 1, prec = 0, Lmax = 0;
     int64_t N = 0, nleaf = 0, nbox = 0;
     bool nocut_all = true;
     std::vector<BatchSys> sys;
     std::vector<int64_t> poff, loff, lvl;  // lvl: (Lmax+1) x (S+1), level-major box bases
+    std::vector<BatchChunk> chunks;        // in system order
+    std::vector<int32_t> cfirst;           // first chunk of each system (S + 1)
 };
 
 struct BatchBuffers {
@@ -44,6 +54,9 @@ struct BatchBuffers {
     uint32_t *pcnt = nullptr, *psp = nullptr, *psums = nullptr;  // padded float32 ranges
     int2 *win = nullptr;  // near-window offsets sorted by box gap
     int32_t *flag = nullptr;
+    BatchChunk *chunks = nullptr;
+    int32_t *cfirst = nullptr;
+    char *cpart = nullptr;  // per-chunk partial reductions (32 bytes each)
 };
 
 // ---- batched convection + M4' remesh (phf_batch_remesh.hip) ----

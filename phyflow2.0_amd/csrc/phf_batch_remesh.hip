@@ -55,20 +55,19 @@ __global__ void brm_convect(const double *__restrict__ x, const double *__restri
     yn[i] = y[i] + dt * (vy[i] + uy);
 }
 
-// one block per system: base-cell bounds -> touched range, capacity check
-__global__ __launch_bounds__(256) void brm_bounds(const BatchSys *__restrict__ sys,
-                                                  const BatchRm *__restrict__ rm,
-                                                  const double *__restrict__ x,
-                                                  const double *__restrict__ y,
-                                                  BatchRmDyn *__restrict__ dyn)
+// per-chunk base-cell bounds {imin, imax, jmin, jmax}
+__global__ __launch_bounds__(256) void brm_bounds_part(const BatchChunk *__restrict__ ch,
+                                                       const BatchRm *__restrict__ rm,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       longlong4 *__restrict__ part)
 {
     __shared__ longlong4 sh[256];
-    const int s = blockIdx.x;
-    const int64_t o = sys[s].off, n = sys[s].n;
-    const BatchRm r = rm[s];
+    const BatchChunk c = ch[blockIdx.x];
+    const BatchRm r = rm[c.sys];
     long long imin = LLONG_MAX, imax = LLONG_MIN, jmin = LLONG_MAX, jmax = LLONG_MIN;
-    for (int64_t b = threadIdx.x; b < n; b += 256) {
-        const long long i = base_index(x[o + b], r.x0, r.h), j = base_index(y[o + b], r.y0, r.h);
+    for (int64_t b = c.beg + threadIdx.x; b < c.end; b += 256) {
+        const long long i = base_index(x[b], r.x0, r.h), j = base_index(y[b], r.y0, r.h);
         imin = i < imin ? i : imin;
         imax = i > imax ? i : imax;
         jmin = j < jmin ? j : jmin;
@@ -78,22 +77,51 @@ __global__ __launch_bounds__(256) void brm_bounds(const BatchSys *__restrict__ s
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if ((int)threadIdx.x < k) {
-            const longlong4 a = sh[threadIdx.x], c = sh[threadIdx.x + k];
-            sh[threadIdx.x] = make_longlong4(a.x < c.x ? a.x : c.x, a.y > c.y ? a.y : c.y,
-                                             a.z < c.z ? a.z : c.z, a.w > c.w ? a.w : c.w);
+            const longlong4 a = sh[threadIdx.x], q = sh[threadIdx.x + k];
+            sh[threadIdx.x] = make_longlong4(a.x < q.x ? a.x : q.x, a.y > q.y ? a.y : q.y,
+                                             a.z < q.z ? a.z : q.z, a.w > q.w ? a.w : q.w);
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// one wave per system: base-cell bounds from the chunk partials -> touched
+// range, capacity check
+__global__ __launch_bounds__(64) void brm_bounds(const BatchSys *__restrict__ sys,
+                                                 const BatchRm *__restrict__ rm,
+                                                 const int32_t *__restrict__ cfirst,
+                                                 const longlong4 *__restrict__ part,
+                                                 BatchRmDyn *__restrict__ dyn)
+{
+    const int s = blockIdx.x, lane = threadIdx.x;
+    long long imin = LLONG_MAX, imax = LLONG_MIN, jmin = LLONG_MAX, jmax = LLONG_MIN;
+    for (int k = cfirst[s] + lane; k < cfirst[s + 1]; k += 64) {
+        const longlong4 q = part[k];
+        imin = q.x < imin ? q.x : imin;
+        imax = q.y > imax ? q.y : imax;
+        jmin = q.z < jmin ? q.z : jmin;
+        jmax = q.w > jmax ? q.w : jmax;
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        const long long a = __shfl_xor(imin, d), b = __shfl_xor(imax, d);
+        const long long c = __shfl_xor(jmin, d), e = __shfl_xor(jmax, d);
+        imin = a < imin ? a : imin;
+        imax = b > imax ? b : imax;
+        jmin = c < jmin ? c : jmin;
+        jmax = e > jmax ? e : jmax;
+    }
+    if (lane == 0) {
+        const int64_t n = sys[s].n;
+        const BatchRm r = rm[s];
         BatchRmDyn d{};
         if (n == 0) {
             d.ok = 1;  // nothing to remesh
         } else {
-            const longlong4 b = sh[0];
-            d.imin = b.x;
-            d.jmin = b.z;
-            d.bi = b.y - b.x + 1;
-            d.bj = b.w - b.z + 1;
+            d.imin = imin;
+            d.jmin = jmin;
+            d.bi = imax - imin + 1;
+            d.bj = jmax - jmin + 1;
             // (bi + 3)(bj + 3) <= ccap, checked without overflow
             const double cells = (double)(d.bi + 3) * (double)(d.bj + 3);
             d.ok = (d.bi > 0 && d.bj > 0 && cells <= (double)r.ccap) ? 1 : 0;
@@ -288,7 +316,11 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
         xs = Bf.xn;
         ys = Bf.yn;
     }
-    brm_bounds<<<S, 256, 0, L->stream>>>(Bv.sys, Bf.rm, xs, ys, Bf.dyn);
+    if (!P.chunks.empty())
+        brm_bounds_part<<<(unsigned)P.chunks.size(), 256, 0, L->stream>>>(
+            Bv.chunks, Bf.rm, xs, ys, reinterpret_cast<longlong4 *>(Bv.cpart));
+    brm_bounds<<<S, 64, 0, L->stream>>>(Bv.sys, Bf.rm, Bv.cfirst,
+                                        reinterpret_cast<const longlong4 *>(Bv.cpart), Bf.dyn);
     PHF_HIP(hipGetLastError());
     uint32_t *ks = Bv.k0;
     int32_t *vs = Bv.v0;
