@@ -581,55 +581,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
 }
 
 
+// float64 near field + L2P: the float32 kernel's layout (p2p_leaf_f64):
+// one workgroup of 4 waves per target leaf, near sources staged in LDS,
+// XCD leaf ranges split by target count.  Absolute coordinates and
+// pair_f64's operation order.
 template <int KIND>
-__global__ __launch_bounds__(64) void p2p_l2p_f64(
+__global__ __launch_bounds__(256) void p2p_l2p_f64s(
     const double4 *__restrict__ src, const int32_t *__restrict__ start_s,
     const double2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
-    const int32_t *__restrict__ perm_t, int Lvl, const int2 *__restrict__ near, int nnear,
-    PairParams pp, const double4 *__restrict__ org, double s, const double2 *__restrict__ Bleaf,
-    int p1, double *__restrict__ qr, double *__restrict__ qi, int velocity_out)
+    const int32_t *__restrict__ perm_t, int Lvl, const int32_t *__restrict__ split,
+    const int2 *__restrict__ near, int nnear, PairParams pp, const double4 *__restrict__ org,
+    double s, const double2 *__restrict__ Bleaf, int p1, double *__restrict__ qr,
+    double *__restrict__ qi, int velocity_out)
 {
-    const int lane = threadIdx.x;
-    const uint32_t b = blockIdx.x;
-    const int32_t t0 = start_t[b], t1 = start_t[b + 1];
-    if (t1 <= t0) return;
-    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
+    __shared__ double4 st[P2P_STAGE_F64];
+    __shared__ double2 acc[P2P_CHUNKS * 4 * 16];
+    const int32_t hi = split[(blockIdx.x & 7u) + 1];
     const int n = 1 << Lvl;
-    double cx, cy;
-    leaf_center(b, org->x, org->z, s, cx, cy);
     const double inv_s = 1.0 / s;
-    for (int32_t tc = t0; tc < t1; tc += 64) {
-        const int32_t i = tc + lane;
-        const bool valid = i < t1;
-        const double2 tp = valid ? tgt[i] : make_double2(cx, cy);
-        double a = 0.0, bb = 0.0;
-        for (int q = 0; q < nnear; ++q) {
-            const int2 o = near[q];
-            const int sx = bx + o.x, sy = by + o.y;
-            if (sx < 0 || sy < 0 || sx >= n || sy >= n) continue;
-            const uint32_t sb = morton2((uint32_t)sx, (uint32_t)sy);
-            const int32_t j0 = start_s[sb], j1 = start_s[sb + 1];
-            for (int32_t j = j0; j < j1; ++j) {
-                const double4 sp = src[j];
-                const double dx = tp.x - sp.x, dy = tp.y - sp.y;
-                const double P = pair_f64<KIND>(dx, dy, sp.z, pp);
-                a = __builtin_fma(P, dx, a);
-                bb = __builtin_fma(P, dy, bb);
+    for (int32_t bl = split[blockIdx.x & 7u] + (int32_t)(blockIdx.x >> 3); bl < hi;
+         bl += (int32_t)(gridDim.x >> 3)) {
+        const uint32_t b = (uint32_t)bl;
+        const int32_t t0 = start_t[b], t1 = start_t[b + 1];
+        if (t1 <= t0) continue;
+        const NearList nf{near, nnear, (int)compact_bits(b), (int)compact_bits(b >> 1), n, 0.f};
+        double cx, cy;
+        leaf_center(b, org->x, org->z, s, cx, cy);
+        auto tpos = [&](int32_t i) { return tgt[i]; };
+        auto out = [&](int32_t i, double a, double bb) {
+            const double2 tp = tgt[i];
+            double fr = 0.0, fi = 0.0;
+            if (Bleaf) {
+                const double2 *Bv = Bleaf + (int64_t)b * p1;
+                const double zr = (tp.x - cx) * inv_s, zi = (tp.y - cy) * inv_s;
+                for (int k = p1 - 1; k >= 0; --k) {
+                    const double2 c = Bv[k];
+                    const double nr = fr * zr - fi * zi + c.x;
+                    const double ni = fr * zi + fi * zr + c.y;
+                    fr = nr;
+                    fi = ni;
+                }
             }
-        }
-        double fr = 0.0, fi = 0.0;
-        if (Bleaf) {
-            const double2 *Bv = Bleaf + (int64_t)b * p1;
-            const double zr = (tp.x - cx) * inv_s, zi = (tp.y - cy) * inv_s;
-            for (int k = p1 - 1; k >= 0; --k) {
-                const double2 c = Bv[k];
-                const double nr = fr * zr - fi * zi + c.x;
-                const double ni = fr * zi + fi * zr + c.y;
-                fr = nr;
-                fi = ni;
-            }
-        }
-        if (valid) {
             const int32_t o = perm_t[i];
             const double vr = -(a + fr), vi = bb - fi;
             if (velocity_out) {
@@ -641,7 +633,8 @@ __global__ __launch_bounds__(64) void p2p_l2p_f64(
                 qr[o] = vr;
                 qi[o] = vi;
             }
-        }
+        };
+        p2p_leaf_f64<KIND>(src, start_s, st, acc, nnear, nf, t0, t1, tpos, pp, out);
     }
 }
 
@@ -1267,9 +1260,14 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         const double4 *org = dom.org;
         const double s = dom.s;
         double *qr = a.qr, *qi = a.qi;
+        const uint32_t nl8 = (nleaf + 7) / 8;
+        int32_t *split;
+        PHF_ALLOC(split, L->arena, int32_t, 9);
+        PHF_TRY(xcd_split(L, Tg.start, nleaf, split));
         p2p = [=](hipStream_t st) {
-            p2p_l2p_f64<PAIR_OSEEN><<<nleaf, 64, 0, st>>>(s4, ss, t2, ts, perm_t, Lv, d_near,
-                                                         nnear, pp, org, s, Bl, p1, qr, qi, vout);
+            p2p_l2p_f64s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(s4, ss, t2, ts, perm_t, Lv, split,
+                                                            d_near, nnear, pp, org, s, Bl, p1,
+                                                            qr, qi, vout);
             PHF_HIP(hipGetLastError());
             return PHF_OK;
         };

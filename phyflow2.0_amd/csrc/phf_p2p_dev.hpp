@@ -77,16 +77,16 @@ __device__ __forceinline__ void l2p_f32(const float2 *__restrict__ Bb, int p1, f
 }
 
 // Staging plan of one batch: lane l of every wave looks at near index q + l,
-// a wave scan of the range lengths packs as many as fit (the cursor (q, jq)
-// splits a range that does not); each lane copies its own range, shifted
-// into the target leaf's frame and scaled by lam, the four waves taking
-// every fourth entry.  Every wave computes the same plan.  Returns the
-// staged count; staged entries [0, core_end) come from core leaves.
-template <class NearFn>
-__device__ __forceinline__ int p2p_stage_batch(const float4 *__restrict__ src,
-                                               const int32_t *__restrict__ sp, float4 *st,
-                                               int nnear, const NearFn &nf, float lam, int &q,
-                                               int &jq, int &core_end)
+// a wave scan of the range lengths packs as many as fit in STAGE entries
+// (the cursor (q, jq) splits a range that does not); each lane copies its
+// own range through xf(record, ox, oy), the four waves taking every fourth
+// entry.  Every wave computes the same plan.  Returns the staged count;
+// staged entries [0, core_end) come from core leaves.
+template <int STAGE, class R, class NearFn, class Xf>
+__device__ __forceinline__ int stage_near_batch(const R *__restrict__ src,
+                                                const int32_t *__restrict__ sp, R *st, int nnear,
+                                                const NearFn &nf, const Xf &xf, int &q, int &jq,
+                                                int &core_end)
 {
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int ql = q + lane;
@@ -104,9 +104,9 @@ __device__ __forceinline__ int p2p_stage_batch(const float4 *__restrict__ src,
         if (lane >= d) incl += v;
     }
     const int excl = incl - len;
-    const bool in = ql < nnear && excl < P2P_STAGE;
-    const int take = in ? min(len, P2P_STAGE - excl) : 0;  // multiple of 4
-    const int count = __popcll(__ballot(in));               // >= 1: lane 0 is in
+    const bool in = ql < nnear && excl < STAGE;
+    const int take = in ? min(len, STAGE - excl) : 0;
+    const int count = __popcll(__ballot(in));  // >= 1: lane 0 is in
     const int last_take = __shfl(take, count - 1), last_len = __shfl(len, count - 1);
     const int used = __shfl(excl + take, count - 1);
     // staged entries of core leaves form a prefix of the batch
@@ -120,28 +120,34 @@ __device__ __forceinline__ int p2p_stage_batch(const float4 *__restrict__ src,
         q += count;
     }
     if (in) {
-        const float4 *gs = src + j0;
-        float4 *ls = st + excl;
+        const R *gs = src + j0;
+        R *ls = st + excl;
         int e = wv;
         for (; e + 28 < take; e += 32) {
-            float4 r[8];
+            R r[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) r[u] = gs[e + 4 * u];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                r[u].x = (r[u].x + ox) * lam;
-                r[u].y = (r[u].y + oy) * lam;
-                ls[e + 4 * u] = r[u];
-            }
+            for (int u = 0; u < 8; ++u) ls[e + 4 * u] = xf(r[u], ox, oy);
         }
-        for (; e < take; e += 4) {
-            float4 r = gs[e];
-            r.x = (r.x + ox) * lam;
-            r.y = (r.y + oy) * lam;
-            ls[e] = r;
-        }
+        for (; e < take; e += 4) ls[e] = xf(gs[e], ox, oy);
     }
     return used;
+}
+
+// float32 records shifted into the target leaf's frame and scaled by lam
+template <class NearFn>
+__device__ __forceinline__ int p2p_stage_batch(const float4 *__restrict__ src,
+                                               const int32_t *__restrict__ sp, float4 *st,
+                                               int nnear, const NearFn &nf, float lam, int &q,
+                                               int &jq, int &core_end)
+{
+    auto xf = [lam](float4 r, float ox, float oy) {
+        r.x = (r.x + ox) * lam;
+        r.y = (r.y + oy) * lam;
+        return r;
+    };
+    return stage_near_batch<P2P_STAGE>(src, sp, st, nnear, nf, xf, q, jq, core_end);
 }
 
 // Sum over the staged entries x in [a, b) with x = o (mod 16) for the
@@ -257,6 +263,90 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
                 two_sum_merge(v.z, v.w, u.z, u.w);
             }
             out(rc + tid, v.x, v.y, v.z, v.w);
+        }
+        __syncthreads();  // acc is reused by the next round
+    }
+}
+
+// ---------------------------------------------------------------- float64
+constexpr int P2P_STAGE_F64 = 768;  // staged sources (double4) per batch
+
+// float64 pair sweep over staged entries x in [a, b) with x = o (mod 16),
+// absolute coordinates, pair_f64's operation order; the smoothing factor
+// of the Oseen kernel is evaluated only when some lane of the wave has a
+// pair inside the cutoff (the others take 1 exactly, as pair_f64 does).
+template <int KIND>
+__device__ __forceinline__ void p2p_sweep_f64(const double4 *__restrict__ st, int a, int b, int o,
+                                              double xt, double yt, const PairParams &pp,
+                                              double &ax, double &ay)
+{
+    for (int x = a + ((o - a) & 15); x < b; x += 16) {
+        const double4 q = st[x];
+        const double dx = xt - q.x, dy = yt - q.y;
+        double P;
+        if (KIND == PAIR_OSEEN || KIND == PAIR_OSEEN_NOCUT) {
+            const double r2 = __builtin_fma(dx, dx, dy * dy);
+            double om = 1.0;
+            if (__ballot(r2 < pp.c2)) om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;
+            const double Pv = (q.z * rcp_f64(r2)) * om;
+            P = r2 > 0.0 ? Pv : 0.0;
+        } else {
+            P = pair_f64<KIND>(dx, dy, q.z, pp);
+        }
+        ax = __builtin_fma(P, dx, ax);
+        ay = __builtin_fma(P, dy, ay);
+    }
+}
+
+// float64 near-field sums of the targets [t0, t1) of one leaf, the
+// float32 p2p_leaf's layout with float64 records and plain float64
+// partials: out(i, sum_a, sum_b).
+template <int KIND, class NearFn, class TgtFn, class OutFn>
+__device__ __forceinline__ void p2p_leaf_f64(const double4 *__restrict__ src,
+                                             const int32_t *__restrict__ sp, double4 *st,
+                                             double2 *acc, int nnear, const NearFn &nf,
+                                             int32_t t0, int32_t t1, const TgtFn &tgt,
+                                             const PairParams &pp, const OutFn &out)
+{
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, g = lane >> 4, t = lane & 15;
+    const int o = 4 * wv + g;
+    auto xf = [](double4 r, float, float) { return r; };
+    for (int32_t rc = t0; rc < t1; rc += 16 * P2P_CHUNKS) {
+        const int nch = min(P2P_CHUNKS, (int)((t1 - rc + 15) >> 4));
+        if (g == 0)
+            for (int c = 0; c < nch; ++c) acc[(c * 4 + wv) * 16 + t] = make_double2(0.0, 0.0);
+        int q = 0, jq = 0;
+        while (q < nnear) {
+            int core_end;
+            const int used = stage_near_batch<P2P_STAGE_F64>(src, sp, st, nnear, nf, xf, q, jq,
+                                                             core_end);
+            __syncthreads();
+            for (int c = 0; c < nch; ++c) {
+                const int32_t i = rc + 16 * c + t;
+                const double2 tp = i < t1 ? tgt(i) : make_double2(0.0, 0.0);
+                double ax = 0.0, ay = 0.0;
+                p2p_sweep_f64<KIND>(st, 0, used, o, tp.x, tp.y, pp, ax, ay);
+                ax += __shfl_xor(ax, 16);
+                ay += __shfl_xor(ay, 16);
+                ax += __shfl_xor(ax, 32);
+                ay += __shfl_xor(ay, 32);
+                if (g == 0) {
+                    double2 &A = acc[(c * 4 + wv) * 16 + t];
+                    A = make_double2(A.x + ax, A.y + ay);
+                }
+            }
+            __syncthreads();
+        }
+        if (tid < 16 * nch && rc + tid < t1) {
+            const int c = tid >> 4, tt = tid & 15;
+            double2 v = acc[(c * 4) * 16 + tt];
+#pragma unroll
+            for (int w = 1; w < 4; ++w) {
+                const double2 u = acc[(c * 4 + w) * 16 + tt];
+                v.x += u.x;
+                v.y += u.y;
+            }
+            out(rc + tid, v.x, v.y);
         }
         __syncthreads();  // acc is reused by the next round
     }
