@@ -15,6 +15,7 @@
 // are solved through the normal equations B^T B x = B^T c, refined with the
 // float64 semi-normal residual B^T (c - B x).
 #include "phf_panel.hpp"
+#include "phf_tree.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -640,6 +641,205 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
     return PHF_OK;
 }
 
+// ---------------------------------------------- grouped panel far field --
+// Panels are grouped 16 at a time (consecutive indices).  In complex form a
+// panel's velocity is conj(v) = i gamma e^{-i alpha} / (2 pi) log((z - z1) /
+// (z - z2)), z1/z2 its ends (the per-panel formula above, rearranged), so a
+// group's velocity beyond 3 R_g of its centre z_g (R_g: farthest panel end)
+// is the multipole series conj(v) = (1/R_g) sum_k b_k t^(k+1), t = R_g /
+// (z - z_g), with b_k = sum_p i gamma_p e^{-i alpha_p} / (2 pi) R_g
+// (zh2^(k+1) - zh1^(k+1)) / (k + 1), zh = (z_end - z_g) / R_g.  |t| <= 1/3:
+// 16 terms leave < 3e-8 of the group's velocity (float32 path), 30 terms
+// < 1e-14 (float64 path).  Nearer groups are summed panel by panel.
+// Targets are evaluated in Morton order (16-bit keys over their bounding
+// box) so that a wave's targets take the same branch for almost every
+// group; each target's sum runs over the groups in index order.
+constexpr int PG_SIZE = 16;    // panels per group
+constexpr int PG_TERMS = 30;   // series terms kept (float64; float32 uses 16)
+constexpr int PG_CHUNK = 16;   // groups staged per round (256 panels)
+
+struct PanelG {
+    double zx, zy;     // centre
+    double far2;       // (3 R_g)^2
+    double R, inv_R;   // R_g, 1 / R_g
+    int32_t first, count, pad0, pad1;
+};
+
+__global__ __launch_bounds__(64) void panel_group_coeffs(
+    int64_t M, const double *__restrict__ gamma, const double *__restrict__ xs,
+    const double *__restrict__ ys, const double *__restrict__ xe, const double *__restrict__ ye,
+    const double *__restrict__ cosA, const double *__restrict__ sinA, PanelG *__restrict__ grp,
+    double2 *__restrict__ coef)
+{
+    __shared__ double sc[4];
+    const int g = blockIdx.x, k = threadIdx.x;
+    const int64_t p0 = (int64_t)g * PG_SIZE;
+    const int cnt = (int)min((int64_t)PG_SIZE, M - p0);
+    if (k == 0) {
+        double cx = 0.0, cy = 0.0;
+        for (int q = 0; q < cnt; ++q) {
+            cx += 0.5 * (xs[p0 + q] + xe[p0 + q]);
+            cy += 0.5 * (ys[p0 + q] + ye[p0 + q]);
+        }
+        cx /= cnt;
+        cy /= cnt;
+        double r2 = 0.0;
+        for (int q = 0; q < cnt; ++q) {
+            const double a1 = xs[p0 + q] - cx, b1 = ys[p0 + q] - cy;
+            const double a2 = xe[p0 + q] - cx, b2 = ye[p0 + q] - cy;
+            r2 = fmax(r2, fmax(a1 * a1 + b1 * b1, a2 * a2 + b2 * b2));
+        }
+        const double R = sqrt(r2);
+        sc[0] = cx;
+        sc[1] = cy;
+        sc[2] = R;
+        PanelG pg;
+        pg.zx = cx;
+        pg.zy = cy;
+        pg.far2 = 9.0 * r2;
+        pg.R = R;
+        pg.inv_R = 1.0 / R;
+        pg.first = (int32_t)p0;
+        pg.count = cnt;
+        pg.pad0 = pg.pad1 = 0;
+        grp[g] = pg;
+    }
+    __syncthreads();
+    if (k >= PG_TERMS) return;
+    const double cx = sc[0], cy = sc[1], R = sc[2], iR = 1.0 / R;
+    double bx = 0.0, by = 0.0;
+    for (int q = 0; q < cnt; ++q) {
+        const int64_t p = p0 + q;
+        // zh^(k+1) for both ends
+        const double u1 = (xs[p] - cx) * iR, v1 = (ys[p] - cy) * iR;
+        const double u2 = (xe[p] - cx) * iR, v2 = (ye[p] - cy) * iR;
+        double p1r = u1, p1i = v1, p2r = u2, p2i = v2;
+        for (int e = 0; e < k; ++e) {
+            const double t1 = p1r * u1 - p1i * v1;
+            p1i = p1r * v1 + p1i * u1;
+            p1r = t1;
+            const double t2 = p2r * u2 - p2i * v2;
+            p2i = p2r * v2 + p2i * u2;
+            p2r = t2;
+        }
+        const double dr = (p2r - p1r) / (k + 1), di = (p2i - p1i) / (k + 1);
+        // f = i gamma (c - i s) / (2 pi) R = gamma R / (2 pi) (s + i c)
+        const double f = gamma[p] * R / (2.0 * M_PI);
+        const double fr = f * sinA[p], fi = f * cosA[p];
+        bx += fr * dr - fi * di;
+        by += fr * di + fi * dr;
+    }
+    coef[(int64_t)g * PG_TERMS + k] = make_double2(bx, by);
+}
+
+__global__ void target_keys16(const double *__restrict__ x, const double *__restrict__ y,
+                              int64_t n, const double4 *__restrict__ bb,
+                              uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double4 b = *bb;
+    const double ex = fmax(b.y - b.x, 1e-300), ey = fmax(b.w - b.z, 1e-300);
+    double fx = floor((x[i] - b.x) / ex * 256.0), fy = floor((y[i] - b.z) / ey * 256.0);
+    fx = fmin(fmax(fx, 0.0), 255.0);
+    fy = fmin(fmax(fy, 0.0), 255.0);
+    keys[i] = morton2((uint32_t)fx, (uint32_t)fy);
+    vals[i] = (int32_t)i;
+}
+
+template <bool F32>
+__global__ __launch_bounds__(PI_T) void panel_induced_grouped(
+    int64_t M, int64_t NG, const PanelK *__restrict__ pk, const PanelG *__restrict__ grp,
+    const double2 *__restrict__ coef, int64_t ne, const int32_t *__restrict__ order,
+    const double *__restrict__ xE, const double *__restrict__ yE, double *__restrict__ vx,
+    double *__restrict__ vy)
+{
+    constexpr int NT = F32 ? 16 : PG_TERMS;
+    __shared__ PanelK sp[PG_CHUNK * PG_SIZE];
+    __shared__ PanelG sg[PG_CHUNK];
+    __shared__ double2 sc[PG_CHUNK * NT];
+    const int tid = threadIdx.x;
+    const int64_t k = (int64_t)blockIdx.x * PI_T + tid;
+    const int64_t i = k < ne ? order[k] : 0;
+    const double x = k < ne ? xE[i] : 0.0, y = k < ne ? yE[i] : 0.0;
+    double sx = 0.0, sy = 0.0;
+    for (int64_t g0 = 0; g0 < NG; g0 += PG_CHUNK) {
+        const int ng = (int)min((int64_t)PG_CHUNK, NG - g0);
+        const int64_t p0 = g0 * PG_SIZE;
+        const int npan = (int)min((int64_t)PG_CHUNK * PG_SIZE, M - p0);
+        __syncthreads();
+        for (int e = tid; e < npan; e += PI_T) sp[e] = pk[p0 + e];
+        if (tid < ng) sg[tid] = grp[g0 + tid];
+        for (int e = tid; e < ng * NT; e += PI_T) {
+            const int gg = e / NT, kk = e - gg * NT;
+            sc[e] = coef[(g0 + gg) * PG_TERMS + kk];
+        }
+        __syncthreads();
+        for (int gg = 0; gg < ng; ++gg) {
+            const PanelG G = sg[gg];
+            const double dx = x - G.zx, dy = y - G.zy;
+            if (dx * dx + dy * dy > G.far2) {
+                // t = R / (z - z_g)
+                const double d2 = dx * dx + dy * dy;
+                const double tr = G.R * dx / d2, ti = -G.R * dy / d2;
+                const double2 *c = sc + gg * NT;
+                double wr, wi;
+                if (F32) {
+                    const float fr = (float)tr, fi = (float)ti;
+                    float ar = (float)c[NT - 1].x, ai = (float)c[NT - 1].y;
+                    for (int q = NT - 2; q >= 0; --q) {
+                        const float nr = ar * fr - ai * fi + (float)c[q].x;
+                        ai = ar * fi + ai * fr + (float)c[q].y;
+                        ar = nr;
+                    }
+                    wr = (double)(ar * fr - ai * fi);
+                    wi = (double)(ar * fi + ai * fr);
+                } else {
+                    double ar = c[NT - 1].x, ai = c[NT - 1].y;
+                    for (int q = NT - 2; q >= 0; --q) {
+                        const double nr = ar * tr - ai * ti + c[q].x;
+                        ai = ar * ti + ai * tr + c[q].y;
+                        ar = nr;
+                    }
+                    wr = ar * tr - ai * ti;
+                    wi = ar * ti + ai * tr;
+                }
+                // conj(v) = w / R
+                sx += wr * G.inv_R;
+                sy -= wi * G.inv_R;
+            } else {
+                const int q0 = G.first - (int32_t)p0;
+                for (int q = q0; q < q0 + G.count; ++q) {
+                    const PanelK p = sp[q];
+                    const double ex = x - p.xs, ey = y - p.ys;
+                    const double xP = p.c * ex + p.s * ey;
+                    const double yP = -p.s * ex + p.c * ey;
+                    const double xm = xP - p.x2;
+                    const double cross = yP * p.x2, dot = xP * xm + yP * yP;
+                    const double den = xm * xm + yP * yP;
+                    const double num = p.x2 * (2.0 * xP - p.x2);
+                    double th, lg;
+                    if (F32) {
+                        th = (double)atan2f((float)cross, (float)dot);
+                        lg = (double)log1pf((float)num / (float)den);
+                    } else {
+                        th = atan2(cross, dot);
+                        lg = log1p(num / den);
+                    }
+                    const double vxP = p.g2 * th;
+                    const double vyP = -p.g4 * lg;
+                    sx += p.c * vxP - p.s * vyP;
+                    sy += p.s * vxP + p.c * vyP;
+                }
+            }
+        }
+    }
+    if (k < ne) {
+        vx[i] = sx;
+        vy[i] = sy;
+    }
+}
+
 int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
                          const double *yE, const double *xs, const double *ys, const double *xe,
                          const double *ye, const double *cosA, const double *sinA, double *vx,
@@ -651,12 +851,44 @@ int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, co
     if (m > 0)
         panel_consts<<<div_up(m, 256), 256, 0, L->stream>>>(m, gamma, xs, ys, xe, ye, cosA, sinA,
                                                             pk);
+    if (m < 4 * PG_SIZE || ne < 4096) {
+        // few panels or targets: every pair by the per-panel formula
+        if (prec == PHF_PREC_F32C)
+            panel_induced_kernel<true><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(m, pk, ne, xE,
+                                                                                 yE, vx, vy);
+        else
+            panel_induced_kernel<false><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(m, pk, ne, xE,
+                                                                                  yE, vx, vy);
+        PHF_HIP(hipGetLastError());
+        return PHF_OK;
+    }
+    // groups and their series
+    const int64_t ng = (m + PG_SIZE - 1) / PG_SIZE;
+    PanelG *grp;
+    double2 *coef;
+    PHF_ALLOC(grp, L->arena, PanelG, (size_t)ng);
+    PHF_ALLOC(coef, L->arena, double2, (size_t)ng * PG_TERMS);
+    panel_group_coeffs<<<(unsigned)ng, 64, 0, L->stream>>>(m, gamma, xs, ys, xe, ye, cosA, sinA,
+                                                           grp, coef);
+    // targets in Morton order
+    double4 *bb;
+    PHF_ALLOC(bb, L->arena, double4, 1);
+    PHF_TRY(device_bbox(L, xE, yE, ne, nullptr, nullptr, 0, bb));
+    uint32_t *k0, *k1, *ks;
+    int32_t *v0, *v1, *vs;
+    PHF_ALLOC(k0, L->arena, uint32_t, (size_t)ne);
+    PHF_ALLOC(k1, L->arena, uint32_t, (size_t)ne);
+    PHF_ALLOC(v0, L->arena, int32_t, (size_t)ne);
+    PHF_ALLOC(v1, L->arena, int32_t, (size_t)ne);
+    target_keys16<<<div_up(ne, 256), 256, 0, L->stream>>>(xE, yE, ne, bb, k0, v0);
+    PHF_HIP(hipGetLastError());
+    PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, ne, 16, &ks, &vs));
     if (prec == PHF_PREC_F32C)
-        panel_induced_kernel<true><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(m, pk, ne, xE, yE,
-                                                                             vx, vy);
+        panel_induced_grouped<true><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(
+            m, ng, pk, grp, coef, ne, vs, xE, yE, vx, vy);
     else
-        panel_induced_kernel<false><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(m, pk, ne, xE, yE,
-                                                                              vx, vy);
+        panel_induced_grouped<false><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(
+            m, ng, pk, grp, coef, ne, vs, xE, yE, vx, vy);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
