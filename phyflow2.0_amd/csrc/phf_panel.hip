@@ -671,22 +671,34 @@ __global__ __launch_bounds__(64) void panel_group_coeffs(
     const double *__restrict__ cosA, const double *__restrict__ sinA, PanelG *__restrict__ grp,
     double2 *__restrict__ coef)
 {
-    __shared__ double sc[4];
+    __shared__ double pd[PG_SIZE][7];  // xs, ys, xe, ye, gamma, cos, sin
+    __shared__ double sc[3];
     const int g = blockIdx.x, k = threadIdx.x;
     const int64_t p0 = (int64_t)g * PG_SIZE;
     const int cnt = (int)min((int64_t)PG_SIZE, M - p0);
+    if (k < cnt) {
+        const int64_t p = p0 + k;
+        pd[k][0] = xs[p];
+        pd[k][1] = ys[p];
+        pd[k][2] = xe[p];
+        pd[k][3] = ye[p];
+        pd[k][4] = gamma[p];
+        pd[k][5] = cosA[p];
+        pd[k][6] = sinA[p];
+    }
+    __syncthreads();
     if (k == 0) {
         double cx = 0.0, cy = 0.0;
         for (int q = 0; q < cnt; ++q) {
-            cx += 0.5 * (xs[p0 + q] + xe[p0 + q]);
-            cy += 0.5 * (ys[p0 + q] + ye[p0 + q]);
+            cx += 0.5 * (pd[q][0] + pd[q][2]);
+            cy += 0.5 * (pd[q][1] + pd[q][3]);
         }
         cx /= cnt;
         cy /= cnt;
         double r2 = 0.0;
         for (int q = 0; q < cnt; ++q) {
-            const double a1 = xs[p0 + q] - cx, b1 = ys[p0 + q] - cy;
-            const double a2 = xe[p0 + q] - cx, b2 = ye[p0 + q] - cy;
+            const double a1 = pd[q][0] - cx, b1 = pd[q][1] - cy;
+            const double a2 = pd[q][2] - cx, b2 = pd[q][3] - cy;
             r2 = fmax(r2, fmax(a1 * a1 + b1 * b1, a2 * a2 + b2 * b2));
         }
         const double R = sqrt(r2);
@@ -709,10 +721,9 @@ __global__ __launch_bounds__(64) void panel_group_coeffs(
     const double cx = sc[0], cy = sc[1], R = sc[2], iR = 1.0 / R;
     double bx = 0.0, by = 0.0;
     for (int q = 0; q < cnt; ++q) {
-        const int64_t p = p0 + q;
-        // zh^(k+1) for both ends
-        const double u1 = (xs[p] - cx) * iR, v1 = (ys[p] - cy) * iR;
-        const double u2 = (xe[p] - cx) * iR, v2 = (ye[p] - cy) * iR;
+        // zh^(k+1) for both ends, zh = (end - z_g) / R_g
+        const double u1 = (pd[q][0] - cx) * iR, v1 = (pd[q][1] - cy) * iR;
+        const double u2 = (pd[q][2] - cx) * iR, v2 = (pd[q][3] - cy) * iR;
         double p1r = u1, p1i = v1, p2r = u2, p2i = v2;
         for (int e = 0; e < k; ++e) {
             const double t1 = p1r * u1 - p1i * v1;
@@ -724,8 +735,8 @@ __global__ __launch_bounds__(64) void panel_group_coeffs(
         }
         const double dr = (p2r - p1r) / (k + 1), di = (p2i - p1i) / (k + 1);
         // f = i gamma (c - i s) / (2 pi) R = gamma R / (2 pi) (s + i c)
-        const double f = gamma[p] * R / (2.0 * M_PI);
-        const double fr = f * sinA[p], fi = f * cosA[p];
+        const double f = pd[q][4] * R / (2.0 * M_PI);
+        const double fr = f * pd[q][6], fi = f * pd[q][5];
         bx += fr * dr - fi * di;
         by += fr * di + fi * dr;
     }

@@ -51,6 +51,17 @@ __global__ void concat2(const double *__restrict__ a, int64_t na, const double *
         out[i] = b[i - na];
 }
 
+// a followed by nb zeros
+__global__ void concat_zeros(const double *__restrict__ a, int64_t na, int64_t nb,
+                             double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < na)
+        out[i] = a[i];
+    else if (i < na + nb)
+        out[i] = 0.0;
+}
+
 // RHS of the no-slip condition, -(v . t) (__panelsClass.py:771); for the
 // 'prescribed' problem the last entry is -gTotal.
 __global__ void slip_rhs_kernel(const double *__restrict__ vx, const double *__restrict__ vy,
@@ -130,18 +141,22 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
         return PHF_OK;
     }
     const int64_t m = P->m, nt = n + m;
-    double *tx, *ty, *vx, *vy, *px, *py;
+    double *tx, *ty, *tw, *vx, *vy, *px, *py;
     PHF_ALLOC(tx, L->arena, double, (size_t)nt);
     PHF_ALLOC(ty, L->arena, double, (size_t)nt);
+    PHF_ALLOC(tw, L->arena, double, (size_t)nt);
     PHF_ALLOC(vx, L->arena, double, (size_t)nt);
     PHF_ALLOC(vy, L->arena, double, (size_t)nt);
     PHF_ALLOC(px, L->arena, double, (size_t)std::max<int64_t>(n, 1));
     PHF_ALLOC(py, L->arena, double, (size_t)std::max<int64_t>(n, 1));
     concat2<<<div_up(nt, 256), 256, 0, L->stream>>>(X, n, P->xcp, m, tx);
     concat2<<<div_up(nt, 256), 256, 0, L->stream>>>(Y, n, P->ycp, m, ty);
+    concat_zeros<<<div_up(nt, 256), 256, 0, L->stream>>>(w, n, m, tw);
     PHF_HIP(hipGetLastError());
-    // blobs -> blobs and blobs -> collocation points in one evaluation
-    PHF_TRY(velocity_device(L, X, Y, w, n, tx, ty, nt, false, vp, vx, vy, tm, used, &fi));
+    // blobs -> blobs and blobs -> collocation points in one evaluation: the
+    // collocation points join the sources with zero circulation (exact
+    // zeros in every sum), so targets and sources share one tree
+    PHF_TRY(velocity_device(L, tx, ty, tw, nt, tx, ty, nt, true, vp, vx, vy, tm, used, &fi));
     if (solve_panels) {
         tm.begin(PHF_STAGE_PANEL);
         used[PHF_STAGE_PANEL] = true;
