@@ -23,6 +23,7 @@
 // agree on near/far by construction; the interaction list at level l is
 // {tau : near_{l-1}(floor((tau + parity)/2)) && !near_l(tau)}.
 #include "phf_batch.hpp"
+#include "phf_fmm.hpp"
 #include "phf_fmm_dev.hpp"
 #include "phf_p2p_dev.hpp"
 #include "phf_remesh.hpp"
@@ -969,6 +970,7 @@ int phf_batch_destroy(phf_batch *b)
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
         if (b->graph) (void)hipGraphDestroy(b->graph);
         drop_step_graph(b);
+        fmm_graphs_release(&b->lane);
         b->rbuf.arena.release();
         b->lane.arena.release();
         b->lane.pin.release();

@@ -124,8 +124,12 @@ struct Lane {
     bool busy = false;
     int index = 0;
     // FMM launch sequences recorded as HIP graphs (calls on resident
-    // systems, whose buffers keep their addresses between calls)
+    // systems, whose buffers keep their addresses between calls), and the
+    // second stream (default priority) the FMM near field runs on while the
+    // far field runs on `stream`, with the events that fork and join them
     FmmGraphCache *fmm_graphs = nullptr;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 class Context {
@@ -166,6 +170,13 @@ struct StageTimer {
     }
     void end(int stage) {
         if (on) hipEventRecord(lane->ev[2 * stage + 1], lane->stream);
+    }
+    // the same on another stream of the lane
+    void begin_on(int stage, hipStream_t s) {
+        if (on) hipEventRecord(lane->ev[2 * stage], s);
+    }
+    void end_on(int stage, hipStream_t s) {
+        if (on) hipEventRecord(lane->ev[2 * stage + 1], s);
     }
 };
 

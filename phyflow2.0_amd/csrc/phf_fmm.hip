@@ -518,21 +518,19 @@ struct NearList {
     }
 };
 
-// P2P + L2P of target leaf b (all threads of the workgroup).
+// Near-field sums of target leaf b (all threads of the workgroup), written
+// in sorted target order: nsum[i] = lam (hi + lo) of both components.
 template <int KIND>
-__device__ __forceinline__ void p2p_leaf_f32s(
+__device__ __forceinline__ void p2p_near_leaf_f32s(
     const float4 *__restrict__ src, const int32_t *__restrict__ sp,
-    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
-    const int32_t *__restrict__ perm_t, int Lvl, uint32_t b, const int2 *__restrict__ near,
-    int nnear, int ninner, const PairParams &pp, float s_f, const float2 *__restrict__ Bleaf,
-    int p1, double *__restrict__ out_a, double *__restrict__ out_b, int velocity_out, float4 *st,
-    float4 *acc)
+    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t, int Lvl, uint32_t b,
+    const int2 *__restrict__ near, int nnear, int ninner, const PairParams &pp, float s_f,
+    double2 *__restrict__ nsum, float4 *st, float4 *acc)
 {
     const int32_t t0 = start_t[b], t1 = start_t[b + 1];
     if (t1 <= t0) return;
     const int n = 1 << Lvl;
     const NearList nf{near, ninner, (int)compact_bits(b), (int)compact_bits(b >> 1), n, s_f};
-    const float inv_s = 1.0f / s_f;
     // scaled coordinates: lam^2 = shape log2(e)
     const float lam = __builtin_sqrtf(-pp.nlog2e_ks_f);
     const int32_t ts = sp[b];
@@ -542,42 +540,65 @@ __device__ __forceinline__ void p2p_leaf_f32s(
         return make_float2(q.x, q.y);
     };
     auto out = [&](int32_t i, float ha, float la, float hb, float lb) {
-        double fr = 0.0, fi = 0.0;
-        const float2 tp = tpos(i);
-        if (Bleaf) l2p_f32(Bleaf + (int64_t)b * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
-        // input index: the permutation, or the source record's index bits
-        const int32_t o = perm_t ? perm_t[i] : __float_as_int(src[ts + (i - t0)].w);
-        const double qr = -(((double)ha + (double)la) * (double)lam + fr);
-        const double qi = ((double)hb + (double)lb) * (double)lam - fi;
-        if (velocity_out) {
-            // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
-            const double two_pi = 2.0 * 3.141592653589793;
-            out_a[o] = -qi / two_pi;
-            out_b[o] = -qr / two_pi;
-        } else {
-            out_a[o] = qr;
-            out_b[o] = qi;
-        }
+        nsum[i] = make_double2(((double)ha + (double)la) * (double)lam,
+                               ((double)hb + (double)lb) * (double)lam);
     };
     p2p_leaf<KIND>(src, sp, st, acc, nnear, nf, t0, t1, tpos, pp, lam, out);
 }
 
 template <int KIND>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p2p_l2p_f32s(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p2p_near_f32s(
     const float4 *__restrict__ src, const int32_t *__restrict__ sp,
-    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
-    const int32_t *__restrict__ perm_t, int Lvl, const int32_t *__restrict__ split,
-    const int2 *__restrict__ near, int nnear, int ninner, PairParams pp, float s_f,
-    const float2 *__restrict__ Bleaf, int p1, double *__restrict__ out_a,
-    double *__restrict__ out_b, int velocity_out)
+    const float2 *__restrict__ tgt, const int32_t *__restrict__ start_t, int Lvl,
+    const int32_t *__restrict__ split, const int2 *__restrict__ near, int nnear, int ninner,
+    PairParams pp, float s_f, double2 *__restrict__ nsum)
 {
     __shared__ float4 st[P2P_STAGE];
     __shared__ float4 acc[P2P_CHUNKS * 4 * 16];
     const int32_t hi = split[(blockIdx.x & 7u) + 1];
     for (int32_t bl = split[blockIdx.x & 7u] + (int32_t)(blockIdx.x >> 3); bl < hi;
          bl += (int32_t)(gridDim.x >> 3))
-        p2p_leaf_f32s<KIND>(src, sp, tgt, start_t, perm_t, Lvl, (uint32_t)bl, near, nnear, ninner,
-                            pp, s_f, Bleaf, p1, out_a, out_b, velocity_out, st, acc);
+        p2p_near_leaf_f32s<KIND>(src, sp, tgt, start_t, Lvl, (uint32_t)bl, near, nnear, ninner,
+                                 pp, s_f, nsum, st, acc);
+}
+
+// L2P and the output in input order, one thread per sorted target i of
+// leaf keys[i]: (qr, qi) = (-(near_a + far_r), near_b - far_i), far the
+// leaf's local expansion at the target (float32 Horner); targets are the
+// source records (tgt == nullptr: index bits in .w) or tgt with perm.
+__global__ void l2p_scatter_f32(const float4 *__restrict__ src, const float2 *__restrict__ tgt,
+                                const uint32_t *__restrict__ keys,
+                                const int32_t *__restrict__ perm, int64_t nt,
+                                const float2 *__restrict__ Bleaf, int p1, float inv_s,
+                                const double2 *__restrict__ nsum, double *__restrict__ out_a,
+                                double *__restrict__ out_b, int velocity_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    float2 tp;
+    int32_t o;
+    if (tgt) {
+        tp = tgt[i];
+        o = perm[i];
+    } else {
+        const float4 q = src[i];
+        tp = make_float2(q.x, q.y);
+        o = __float_as_int(q.w);
+    }
+    double fr = 0.0, fi = 0.0;
+    if (Bleaf) l2p_f32(Bleaf + (int64_t)keys[i] * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
+    const double2 ns = nsum[i];
+    const double qr = -(ns.x + fr);
+    const double qi = ns.y - fi;
+    if (velocity_out) {
+        // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
+        const double two_pi = 2.0 * 3.141592653589793;
+        out_a[o] = -qi / two_pi;
+        out_b[o] = -qr / two_pi;
+    } else {
+        out_a[o] = qr;
+        out_b[o] = qi;
+    }
 }
 
 
@@ -1028,17 +1049,58 @@ static int m2l_dispatch(Lane *L, const M2LLevels &lv, int p1, const void *M, voi
     return PHF_ERR_UNSUPPORTED;
 }
 
-// launch of the near-field kernel on a stream (recorded with its arguments)
+// a launch on a stream (recorded with its arguments)
 using P2PLaunch = std::function<int(hipStream_t)>;
+
+// How run_fmm_typed hands back its last launches.  With a split, it calls
+// mid() after the tree (capture: ends the first recording and begins the
+// second), enqueues the far field and returns the near-field launch
+// (float32: near sums in sorted order; float64: none) and the finishing
+// launch (float32: L2P and the output in input order; float64: the fused
+// near field + L2P) for the caller to issue around the far field.  Without
+// one it issues them itself (fmm_near_fork / fmm_near_join).
+struct FmmSplit {
+    std::function<int()> mid;
+    P2PLaunch near, finish;
+    char **tables_keep = nullptr;  // device tables kept for replays
+    hipStream_t side = nullptr;    // their upload stream while recording
+};
+
+static int lane_stream2(Lane *L)
+{
+    if (!L->stream2) PHF_HIP(hipStreamCreateWithFlags(&L->stream2, hipStreamNonBlocking));
+    if (!L->fork_ev) PHF_HIP(hipEventCreateWithFlags(&L->fork_ev, hipEventDisableTiming));
+    if (!L->join_ev) PHF_HIP(hipEventCreateWithFlags(&L->join_ev, hipEventDisableTiming));
+    return PHF_OK;
+}
+
+// The near field on the lane's second stream, after the work enqueued so
+// far on the lane's stream and beside what follows it there (timed as the
+// P2P stage); fmm_near_join makes the lane's stream wait for it.
+static int fmm_near_fork(Lane *L, const P2PLaunch &near, StageTimer &tm)
+{
+    PHF_TRY(lane_stream2(L));
+    PHF_HIP(hipEventRecord(L->fork_ev, L->stream));
+    PHF_HIP(hipStreamWaitEvent(L->stream2, L->fork_ev, 0));
+    tm.begin_on(PHF_STAGE_P2P, L->stream2);
+    const int rc = near(L->stream2);
+    tm.end_on(PHF_STAGE_P2P, L->stream2);
+    PHF_HIP(hipEventRecord(L->join_ev, L->stream2));
+    return rc;
+}
+
+static int fmm_near_join(Lane *L)
+{
+    PHF_HIP(hipStreamWaitEvent(L->stream, L->join_ev, 0));
+    return PHF_OK;
+}
 
 template <class T, class TC>
 static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lists &ls,
                          const SortedPoints &S, const SortedPoints &Tg, StageTimer &tm,
-                         bool *used, FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep,
-                         hipStream_t side)
+                         bool *used, FmmInfo *info, FmmSplit *spl)
 {
     using V = typename C2<T>::type;
-    using VC = typename C2<TC>::type;
     const int Lv = dom.L, p1 = a.p + 1;
     const PairParams pp = make_oseen_params(a.xopt, a.cutoff);
 
@@ -1070,7 +1132,6 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         tgt = t2;
     }
     PHF_HIP(hipGetLastError());
-    tm.end(PHF_STAGE_TREE);
 
     // near list at the leaf level, offsets that can hold pairs inside the
     // smoothing core first: for the others the box gap g satisfies
@@ -1086,31 +1147,32 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         ninner = (int)near_ord.size();
         near_ord.insert(near_ord.end(), far.begin(), far.end());
     }
-    // near list and, below, the far-field tables
+    // near list and the far-field tables, uploaded together
     TableBlob blob;
     int2 *d_near = nullptr;
     blob.add(near_ord, &d_near);
-
-    const V *Bleaf = nullptr;
-    if (Lv >= 2 && a.nb > 0) {
-        // level offsets in the coefficient arrays
-        std::vector<int64_t> off(Lv + 2, 0);
-        for (int l = 0; l <= Lv; ++l) off[l + 1] = off[l] + (int64_t(1) << (2 * l));
-        V *M, *B;
-        PHF_ALLOC(M, L->arena, V, (size_t)off[Lv + 1] * p1);
-        PHF_ALLOC(B, L->arena, V, (size_t)off[Lv + 1] * p1);
-
+    const bool far = Lv >= 2 && a.nb > 0;
+    // level offsets in the coefficient arrays
+    std::vector<int64_t> off(Lv + 2, 0);
+    for (int l = 0; l <= Lv; ++l) off[l + 1] = off[l] + (int64_t(1) << (2 * l));
+    TC *d_pas = nullptr;
+    T *d_bin = nullptr;
+    V *d_U = nullptr;
+    int2 *d_il = nullptr;
+    int32_t *d_iloff = nullptr;
+    std::vector<int> il_base(Lv + 1, 0), il_max(Lv + 1, 0);
+    const int R = std::max(ls.R, 1);
+    if (far) {
         // Pascal triangle C(m, k) for the multi-level shifts
         std::vector<TC> pas((size_t)p1 * p1);
         for (int m = 0; m < p1; ++m)
             for (int k = 0; k < p1; ++k) pas[(size_t)m * p1 + k] = (TC)binom_d(m, k);
-        TC *d_pas = nullptr;
         blob.add(pas, &d_pas);
         // M2L: binomial rows C(k+m, k) and powers u^k, u = -1/offset
         std::vector<T> bin((size_t)p1 * p1);
         for (int m = 0; m < p1; ++m)
             for (int k = 0; k < p1; ++k) bin[(size_t)m * p1 + k] = (T)binom_d(k + m, k);
-        const int R = std::max(ls.R, 1), W2 = 2 * R + 1;
+        const int W2 = 2 * R + 1;
         std::vector<V> U((size_t)W2 * W2 * (p1 + 1));
         for (int oy = -R; oy <= R; ++oy)
             for (int ox = -R; ox <= R; ++ox) {
@@ -1123,13 +1185,10 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
                     pw *= u;
                 }
             }
-        T *d_bin = nullptr;
-        V *d_U = nullptr;
         blob.add(bin, &d_bin);
         blob.add(U, &d_U);
         std::vector<int2> il_all;
         std::vector<int32_t> il_offs;
-        std::vector<int> il_base(Lv + 1, 0), il_max(Lv + 1, 0);
         for (int l = 2; l <= Lv; ++l) {
             il_base[l] = (int)il_offs.size();
             for (int q = 0; q < 5; ++q) il_offs.push_back((int32_t)il_all.size() + ls.il_off[l][q]);
@@ -1137,23 +1196,93 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
                 il_max[l] = std::max(il_max[l], ls.il_off[l][q + 1] - ls.il_off[l][q]);
             il_all.insert(il_all.end(), ls.il[l].begin(), ls.il[l].end());
         }
-        int2 *d_il = nullptr;
-        int32_t *d_iloff = nullptr;
         blob.add(il_all, &d_il);
         blob.add(il_offs, &d_iloff);
-        PHF_TRY(blob.send(L, tables_keep, side));
-        if (info && info->want_counts) {
-            for (int l = 2; l <= Lv; ++l)
-                count_m2l_links<<<div_up(int64_t(1) << (2 * l), 256), 256, 0, L->stream>>>(
-                    l, S.start, Tg.start, 2 * (Lv - l), d_il, d_iloff + il_base[l],
-                    info->d_counts + 1);
-            PHF_HIP(hipGetLastError());
-        }
+    }
+    PHF_TRY(blob.send(L, spl ? spl->tables_keep : nullptr, spl ? spl->side : nullptr));
+    V *M = nullptr, *B = nullptr;
+    if (far) {
+        PHF_ALLOC(M, L->arena, V, (size_t)off[Lv + 1] * p1);
+        PHF_ALLOC(B, L->arena, V, (size_t)off[Lv + 1] * p1);
+    }
+    const uint32_t nleaf = 1u << (2 * Lv);
+    const int nnear = (int)near_ord.size();
+    if (info && info->want_counts) {
+        for (int l = 2; far && l <= Lv; ++l)
+            count_m2l_links<<<div_up(int64_t(1) << (2 * l), 256), 256, 0, L->stream>>>(
+                l, S.start, Tg.start, 2 * (Lv - l), d_il, d_iloff + il_base[l],
+                info->d_counts + 1);
+        count_p2p_pairs<<<div_up((int64_t)nleaf, 256), 256, 0, L->stream>>>(
+            S.start, Tg.start, Lv, d_near, nnear, info->d_counts);
+        PHF_HIP(hipGetLastError());
+    }
 
+    // near field and finish: float32 sums the near field in sorted order
+    // (beside the far field) and then adds the leaf expansions and writes
+    // the outputs in input order; float64 does both in one kernel after
+    // the far field
+    const int vout = a.velocity_out ? 1 : 0;
+    const uint32_t nl8 = (nleaf + 7) / 8;
+    int32_t *split;
+    PHF_ALLOC(split, L->arena, int32_t, 9);
+    PHF_TRY(xcd_split(L, Tg.start, nleaf, split));
+    const int32_t *ts = Tg.start;
+    double *qr = a.qr, *qi = a.qi;
+    P2PLaunch near, finish;
+    if (sizeof(T) == 4) {
+        const float4 *s4 = (const float4 *)src;
+        const float2 *t2 = (const float2 *)tgt;
+        const float2 *Bl = far ? (const float2 *)B + off[Lv] * p1 : nullptr;
+        const uint32_t *tkeys = Tg.keys;
+        const float s_f = (float)dom.s, inv_s = 1.0f / s_f;
+        const int64_t nt = a.nt;
+        double2 *nsum;
+        PHF_ALLOC(nsum, L->arena, double2, (size_t)std::max<int64_t>(a.nt, 1));
+        near = [=](hipStream_t st) {
+            if (pp.f32_nocut)
+                p2p_near_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, st>>>(
+                    s4, sp, t2, ts, Lv, split, d_near, nnear, ninner, pp, s_f, nsum);
+            else
+                p2p_near_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(
+                    s4, sp, t2, ts, Lv, split, d_near, nnear, ninner, pp, s_f, nsum);
+            PHF_HIP(hipGetLastError());
+            return PHF_OK;
+        };
+        finish = [=](hipStream_t st) {
+            l2p_scatter_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2, tkeys, perm_t, nt, Bl, p1,
+                                                             inv_s, nsum, qr, qi, vout);
+            PHF_HIP(hipGetLastError());
+            return PHF_OK;
+        };
+    } else {
+        const double4 *s4 = (const double4 *)src;
+        const double2 *t2 = (const double2 *)tgt;
+        const double2 *Bl = far ? (const double2 *)B + off[Lv] * p1 : nullptr;
+        const int32_t *ss = S.start;
+        const double4 *org = dom.org;
+        const double s = dom.s;
+        finish = [=](hipStream_t st) {
+            p2p_l2p_f64s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(s4, ss, t2, ts, perm_t, Lv, split,
+                                                            d_near, nnear, pp, org, s, Bl, p1,
+                                                            qr, qi, vout);
+            PHF_HIP(hipGetLastError());
+            return PHF_OK;
+        };
+    }
+    tm.end(PHF_STAGE_TREE);
+    int rc = PHF_OK;
+    if (spl)
+        rc = spl->mid();
+    else if (near)
+        rc = fmm_near_fork(L, near, tm);
+    PHF_TRY(rc);
+
+    // far field (float32: beside the near field); a failure still joins
+    auto far_field = [&]() -> int {
+        if (!far) return PHF_OK;
         // P2M
         tm.begin(PHF_STAGE_P2M);
         used[PHF_STAGE_P2M] = true;
-        const uint32_t nleaf = 1u << (2 * Lv);
         p2m_kernel<T><<<nleaf, 64, (size_t)64 * p1 * sizeof(V), L->stream>>>(
             src, sp, p1, dom.org, dom.s, Lv, M + off[Lv] * p1);
         PHF_HIP(hipGetLastError());
@@ -1214,79 +1343,31 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
             PHF_TRY((shift_down_leaf<T, TC>(L, p1, dl, B, d_pas, Tg.start)));
         }
         tm.end(PHF_STAGE_L2L);
-        Bleaf = B + off[Lv] * p1;
-    }
-
-    if (!d_near) PHF_TRY(blob.send(L, tables_keep, side));
-    if (info && info->want_counts) {
-        count_p2p_pairs<<<div_up(int64_t(1) << (2 * Lv), 256), 256, 0, L->stream>>>(
-            S.start, Tg.start, Lv, d_near, (int)ls.near_leaf.size(), info->d_counts);
-        PHF_HIP(hipGetLastError());
-    }
-    // P2P + L2P: launched here, or handed back (graph capture) to be
-    // launched after each replay
-    const uint32_t nleaf = 1u << (2 * Lv);
-    const int vout = a.velocity_out ? 1 : 0;
-    const int nnear = (int)near_ord.size();
-    P2PLaunch p2p;
-    if (sizeof(T) == 4) {
-        const uint32_t nl8 = (nleaf + 7) / 8;
-        int32_t *split;
-        PHF_ALLOC(split, L->arena, int32_t, 9);
-        PHF_TRY(xcd_split(L, Tg.start, nleaf, split));
-        const float4 *s4 = (const float4 *)src;
-        const float2 *t2 = (const float2 *)tgt;
-        const float2 *Bl = (const float2 *)Bleaf;
-        const int32_t *ts = Tg.start;
-        const float s_f = (float)dom.s;
-        double *qr = a.qr, *qi = a.qi;
-        p2p = [=](hipStream_t st) {
-            if (pp.f32_nocut)
-                p2p_l2p_f32s<PAIR_OSEEN_NOCUT><<<8 * nl8, 256, 0, st>>>(
-                    s4, sp, t2, ts, perm_t, Lv, split, d_near, nnear, ninner, pp, s_f, Bl, p1, qr,
-                    qi, vout);
-            else
-                p2p_l2p_f32s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(
-                    s4, sp, t2, ts, perm_t, Lv, split, d_near, nnear, ninner, pp, s_f, Bl, p1, qr,
-                    qi, vout);
-            PHF_HIP(hipGetLastError());
-            return PHF_OK;
-        };
-    } else {
-        const double4 *s4 = (const double4 *)src;
-        const double2 *t2 = (const double2 *)tgt;
-        const double2 *Bl = (const double2 *)Bleaf;
-        const int32_t *ss = S.start, *ts = Tg.start;
-        const double4 *org = dom.org;
-        const double s = dom.s;
-        double *qr = a.qr, *qi = a.qi;
-        const uint32_t nl8 = (nleaf + 7) / 8;
-        int32_t *split;
-        PHF_ALLOC(split, L->arena, int32_t, 9);
-        PHF_TRY(xcd_split(L, Tg.start, nleaf, split));
-        p2p = [=](hipStream_t st) {
-            p2p_l2p_f64s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(s4, ss, t2, ts, perm_t, Lv, split,
-                                                            d_near, nnear, pp, org, s, Bl, p1,
-                                                            qr, qi, vout);
-            PHF_HIP(hipGetLastError());
-            return PHF_OK;
-        };
-    }
-    if (p2p_out) {
-        *p2p_out = p2p;
         return PHF_OK;
+    };
+    rc = far_field();
+    if (spl) {
+        spl->near = near;
+        spl->finish = finish;
+        return rc;
     }
+    if (near) {
+        const int jr = fmm_near_join(L);
+        PHF_TRY(rc);
+        PHF_TRY(jr);
+        return finish(L->stream);
+    }
+    PHF_TRY(rc);
     tm.begin(PHF_STAGE_P2P);
-    PHF_TRY(p2p(L->stream));
+    rc = finish(L->stream);
     tm.end(PHF_STAGE_P2P);
-    return PHF_OK;
+    return rc;
 }
 
 // Tree and FMM from the domain on (every launch after the bounding box);
-// p2p_out / tables_keep as run_fmm_typed.
+// spl as run_fmm_typed.
 static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer &tm, bool *used,
-                       FmmInfo *info, P2PLaunch *p2p_out, char **tables_keep, hipStream_t side,
-                       int *nnear)
+                       FmmInfo *info, FmmSplit *spl, int *nnear)
 {
     const double c2 = a.cutoff * a.cutoff;
     const Lists ls = make_lists(dom, c2, a.ws);
@@ -1306,35 +1387,34 @@ static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer 
     }
     if (a.prec == PHF_PREC_F32C) {
         if (a.p > 16)
-            return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
-                                                tables_keep, side);
-        return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
-                                           tables_keep, side);
+            return run_fmm_typed<float, double>(L, a, dom, ls, S, Tg, tm, used, info, spl);
+        return run_fmm_typed<float, float>(L, a, dom, ls, S, Tg, tm, used, info, spl);
     }
-    return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used, info, p2p_out,
-                                         tables_keep, side);
+    return run_fmm_typed<double, double>(L, a, dom, ls, S, Tg, tm, used, info, spl);
 }
 
 // ------------------------------------------------------ captured graphs --
-// A lane's recorded FMM launch sequences: bounding box, tree and far field
-// (the near field is launched after each replay, timed on its own).  An
-// entry is valid for the same inputs and outputs (addresses), sizes and
-// parameters and the same arena state at the start (its buffers are the
-// same arena slices).  A call replays the most recent entry without
-// waiting for the bounding box: kernels read the domain corner on the
-// device, and the bounding box comes back with the results, so that the
-// caller can check that the points kept the recording's leaf level and
-// domain side (else it evaluates again with a fresh recording).
+// A lane's recorded FMM launch sequences, in two graphs: bounding box and
+// tree, then the far field; the near field (float32) is launched between
+// the two on the lane's second stream and the finishing kernel after both,
+// eagerly, so that the near field is timed on its own.  An entry is valid
+// for the same inputs and outputs (addresses), sizes and parameters and
+// the same arena state at the start (its buffers are the same arena
+// slices).  A call replays the most recent entry without waiting for the
+// bounding box: kernels read the domain corner on the device, and the
+// bounding box comes back with the results, so that the caller can check
+// that the points kept the recording's leaf level and domain side (else it
+// evaluates again with a fresh recording).
 struct FmmGraph {
     FmmArgs a{};
     int L = 0, nnear = 0;
     double D = 0.0;
     Arena::Mark m0, m1;
-    hipGraph_t g = nullptr;
-    hipGraphExec_t exec = nullptr;
-    char *tables = nullptr;     // device tables the graph's kernels read
+    hipGraph_t g[2] = {};
+    hipGraphExec_t exec[2] = {};
+    char *tables = nullptr;     // device tables the graphs' kernels read
     double4 *bb = nullptr;      // the recorded bounding box (arena slot)
-    P2PLaunch p2p;
+    P2PLaunch near, finish;
     uint64_t last = 0;
 };
 
@@ -1349,14 +1429,24 @@ constexpr size_t kFmmGraphs = 4;
 
 static void drop_graph(FmmGraph *e)
 {
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->g) (void)hipGraphDestroy(e->g);
+    for (int k = 0; k < 2; ++k) {
+        if (e->exec[k]) (void)hipGraphExecDestroy(e->exec[k]);
+        if (e->g[k]) (void)hipGraphDestroy(e->g[k]);
+    }
     if (e->tables) (void)hipFree(e->tables);
     delete e;
 }
 
 void fmm_graphs_release(Lane *L)
 {
+    if (L->stream2) {
+        (void)hipStreamSynchronize(L->stream2);
+        (void)hipStreamDestroy(L->stream2);
+        L->stream2 = nullptr;
+    }
+    if (L->fork_ev) (void)hipEventDestroy(L->fork_ev);
+    if (L->join_ev) (void)hipEventDestroy(L->join_ev);
+    L->fork_ev = L->join_ev = nullptr;
     if (!L->fmm_graphs) return;
     for (FmmGraph *e : L->fmm_graphs->v) drop_graph(e);
     if (L->fmm_graphs->side) (void)hipStreamDestroy(L->fmm_graphs->side);
@@ -1374,8 +1464,8 @@ static bool same_args(const FmmArgs &x, const FmmArgs &y)
            x.velocity_out == y.velocity_out;
 }
 
-// Record bounding box, tree and far field for (a, dom) from the arena's
-// current state.
+// Record bounding box and tree, then the far field, for (a, dom) from the
+// arena's current state.
 static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **out, bool *keep)
 {
     FmmGraphCache &c = *L->fmm_graphs;
@@ -1392,6 +1482,7 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
         delete e;
         PHF_HIP(be);
     }
+    int part = 0;  // the graph being recorded
     int rc = PHF_OK;
     double4 *bb = L->arena.alloc<double4>(1);
     e->bb = bb;
@@ -1402,15 +1493,33 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
     if (rc == PHF_OK) {
         Domain dom = dom0;
         dom.org = bb;
-        rc = fmm_enqueue(L, a, dom, off, used, nullptr, &e->p2p, &e->tables, c.side, &e->nnear);
+        FmmSplit spl;
+        spl.tables_keep = &e->tables;
+        spl.side = c.side;
+        spl.mid = [&]() -> int {
+            PHF_HIP(hipStreamEndCapture(L->stream, &e->g[0]));
+            part = 1;
+            PHF_HIP(hipStreamBeginCapture(L->stream, hipStreamCaptureModeRelaxed));
+            part = 2;
+            return PHF_OK;
+        };
+        rc = fmm_enqueue(L, a, dom, off, used, nullptr, &spl, &e->nnear);
+        e->near = spl.near;
+        e->finish = spl.finish;
+        if (rc == PHF_OK && part != 2) {
+            set_error("FMM recording did not reach the far field");
+            rc = PHF_ERR_DEVICE;
+        }
     }
-    hipError_t ce = hipStreamEndCapture(L->stream, &e->g);
-    if (rc == PHF_OK && ce != hipSuccess) {
-        set_error("HIP error %s ending the FMM graph capture", hipGetErrorName(ce));
-        rc = PHF_ERR_DEVICE;
+    if (part != 1) {
+        hipError_t ce = hipStreamEndCapture(L->stream, &e->g[part == 0 ? 0 : 1]);
+        if (rc == PHF_OK && ce != hipSuccess) {
+            set_error("HIP error %s ending the FMM graph capture", hipGetErrorName(ce));
+            rc = PHF_ERR_DEVICE;
+        }
     }
-    if (rc == PHF_OK) {
-        hipError_t ie = hipGraphInstantiate(&e->exec, e->g, nullptr, nullptr, 0);
+    for (int k = 0; k < 2 && rc == PHF_OK; ++k) {
+        hipError_t ie = hipGraphInstantiate(&e->exec[k], e->g[k], nullptr, nullptr, 0);
         if (ie != hipSuccess) {
             set_error("HIP error %s instantiating the FMM graph", hipGetErrorName(ie));
             rc = PHF_ERR_DEVICE;
@@ -1446,6 +1555,36 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
         c.v.push_back(e);
     }
     return PHF_OK;
+}
+
+// Launch a recorded entry: tree graph, near field beside the far-field
+// graph, then the finishing kernel.
+static int fmm_replay(Lane *L, FmmGraph *e, StageTimer &tm, int stage0)
+{
+    hipError_t le = hipGraphLaunch(e->exec[0], L->stream);
+    if (le != hipSuccess) {
+        set_error("HIP error %s launching the FMM tree graph", hipGetErrorName(le));
+        return PHF_ERR_DEVICE;
+    }
+    int rc = e->near ? fmm_near_fork(L, e->near, tm) : PHF_OK;
+    PHF_TRY(rc);
+    le = hipGraphLaunch(e->exec[1], L->stream);
+    tm.end(stage0);
+    if (le != hipSuccess) {
+        set_error("HIP error %s launching the FMM far-field graph", hipGetErrorName(le));
+        rc = PHF_ERR_DEVICE;
+    }
+    if (e->near) {
+        const int jr = fmm_near_join(L);
+        PHF_TRY(rc);
+        PHF_TRY(jr);
+        return e->finish(L->stream);
+    }
+    PHF_TRY(rc);
+    tm.begin(PHF_STAGE_P2P);
+    rc = e->finish(L->stream);
+    tm.end(PHF_STAGE_P2P);
+    return rc;
 }
 
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info)
@@ -1493,15 +1632,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
     }
     int nnear = 0;
     if (e) {
-        hipError_t le = hipGraphLaunch(e->exec, L->stream);
-        tm.end(stage0);
-        int rc = le == hipSuccess ? PHF_OK : PHF_ERR_DEVICE;
-        if (le != hipSuccess) set_error("HIP error %s launching the FMM graph", hipGetErrorName(le));
-        if (rc == PHF_OK) {
-            tm.begin(PHF_STAGE_P2P);
-            rc = e->p2p(L->stream);
-            tm.end(PHF_STAGE_P2P);
-        }
+        int rc = fmm_replay(L, e, tm, stage0);
         if (rc == PHF_OK) {
             double4 *hb = L->fmm_graphs->hb_host;
             const hipError_t fe = hipMemcpyAsync(hb, e->bb, sizeof(double4),
@@ -1523,7 +1654,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         dom.L = e->L;
         dom.D = e->D;
         if (!keep) {
-            // the graph and its tables must outlive the launches
+            // the graphs and their tables must outlive the launches
             const hipError_t se = hipStreamSynchronize(L->stream);
             drop_graph(e);
             if (rc == PHF_OK && se != hipSuccess) PHF_HIP(se);
@@ -1532,7 +1663,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         used[PHF_STAGE_TREE_FAR] = used[PHF_STAGE_P2P] = true;
     } else {
         used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
-        PHF_TRY(fmm_enqueue(L, a, dom, tm, used, info, nullptr, nullptr, nullptr, &nnear));
+        PHF_TRY(fmm_enqueue(L, a, dom, tm, used, info, nullptr, &nnear));
     }
     if (info) {
         info->L = dom.L;

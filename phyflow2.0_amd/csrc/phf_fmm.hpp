@@ -36,10 +36,13 @@ struct FmmInfo {
 int fmm_order_from_tol(double tol);
 int fmm_level(int64_t nb, int leaf, int ndirect);
 // With info->graph set (and no work counts) the launch sequence (bounding
-// box through L2L) is recorded as a HIP graph in the lane's cache and
+// box through L2L) is recorded as HIP graphs in the lane's cache and
 // replayed on later calls with the same arguments (the stage then timed
-// is PHF_STAGE_TREE_FAR); see FmmInfo::domain_flag.
+// is PHF_STAGE_TREE_FAR); see FmmInfo::domain_ok.  The float32 near field
+// runs on the lane's second stream beside the far field; the lane's stream
+// waits for it before the call's last kernel.
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info);
+// the lane's recorded graphs, second stream and its events
 void fmm_graphs_release(Lane *L);
 // float32 source ranges per leaf padded to multiples of 4: sp[b] (nleaf + 1
 // entries) from the leaf offsets; cnt, sums: workspaces (capturable).

@@ -272,6 +272,7 @@ int phf_panels_destroy(phf_panels *P)
     {
         std::lock_guard<std::mutex> lk(P->mu);
         if (P->lane.stream) (void)hipStreamSynchronize(P->lane.stream);
+        fmm_graphs_release(&P->lane);
         P->lane.arena.release();
         P->lane.pin.release();
         for (auto &ev : P->lane.ev)
