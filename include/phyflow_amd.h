@@ -36,7 +36,9 @@ extern "C" {
 #define PHF_PREC_F64 0  /* float64 throughout */
 #define PHF_PREC_F32C 1 /* float32 pair arithmetic, compensated per-target sums */
 
-/* ---- timed stages (HIP events on the call's stream) --------------------- */
+/* ---- timed stages (HIP events on the call's streams) -------------------- */
+/* The float32 FMM runs its far field (P2M..L2L) on a second stream beside
+ * the near field, so those stages overlap P2P in time. */
 enum {
     PHF_STAGE_UPLOAD = 0, /* host -> device copies */
     PHF_STAGE_TREE,       /* bbox, Morton keys, radix sort, leaf offsets, gathers */
@@ -44,12 +46,13 @@ enum {
     PHF_STAGE_M2M,
     PHF_STAGE_M2L,
     PHF_STAGE_L2L,
-    PHF_STAGE_P2P,        /* near-field (or all-pairs) sum, L2P fused */
+    PHF_STAGE_P2P,        /* near-field (or all-pairs) sum; float64 FMM: L2P fused */
     PHF_STAGE_DOWNLOAD,   /* device -> host copies */
     PHF_STAGE_REMESH,
     PHF_STAGE_PANEL,
-    PHF_STAGE_TREE_FAR,   /* tree through L2L replayed as one captured graph
-                             (resident systems); TREE..L2L are then -1 */
+    PHF_STAGE_TREE_FAR,   /* tree through L2L replayed as captured graphs
+                             (resident systems): TREE is then the tree graph
+                             alone and P2M..L2L are -1 */
     PHF_STAGE_TOTAL,
     PHF_STAGE_COUNT
 };

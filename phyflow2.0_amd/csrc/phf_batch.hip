@@ -824,7 +824,6 @@ int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
     const int64_t rw = radix_ws_size(P.N);
     PHF_ALLOC(Bf->hist, L->arena, uint32_t, (size_t)rw);
     PHF_ALLOC(Bf->offs, L->arena, uint32_t, (size_t)rw);
-    PHF_ALLOC(Bf->sums, L->arena, uint32_t, (size_t)scan_sums_size(rw));
     return PHF_OK;
 }
 
@@ -852,7 +851,7 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
     uint32_t *ks;
     int32_t *vs;
     PHF_TRY(device_radix_sort_ws(L, Bf.k0, Bf.v0, Bf.k1, Bf.v1, N, key_bits, Bf.hist, Bf.offs,
-                                 Bf.sums, &ks, &vs));
+                                 &ks, &vs));
     PHF_TRY(leaf_starts_device(L, ks, N, P.nleaf, Bf.start));
     const int32_t *sp = Bf.start;  // source ranges per leaf (padded for float32)
     if (sizeof(T) == 4) {

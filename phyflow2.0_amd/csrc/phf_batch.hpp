@@ -50,7 +50,7 @@ struct BatchBuffers {
     int32_t *start = nullptr;
     char *src = nullptr, *M = nullptr, *B = nullptr;
     char *m2m = nullptr, *l2l = nullptr, *bin = nullptr, *U = nullptr;
-    uint32_t *hist = nullptr, *offs = nullptr, *sums = nullptr;
+    uint32_t *hist = nullptr, *offs = nullptr;
     uint32_t *pcnt = nullptr, *psp = nullptr, *psums = nullptr;  // padded float32 ranges
     int2 *win = nullptr;  // near-window offsets sorted by box gap
     int32_t *flag = nullptr;

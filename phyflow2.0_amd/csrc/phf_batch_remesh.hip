@@ -330,7 +330,7 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
         PHF_HIP(hipGetLastError());
         // the velocity pass is done with the sort buffers and workspaces
         PHF_TRY(device_radix_sort_ws(L, Bv.k0, Bv.v0, Bv.k1, Bv.v1, N, R.key_bits, Bv.hist,
-                                     Bv.offs, Bv.sums, &ks, &vs));
+                                     Bv.offs, &ks, &vs));
     }
     PHF_TRY(leaf_starts_device(L, ks, N, C, Bf.cstart));
     if (N > 0)

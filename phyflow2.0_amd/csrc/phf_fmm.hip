@@ -1057,8 +1057,11 @@ using P2PLaunch = std::function<int(hipStream_t)>;
 // second), enqueues the far field and returns the near-field launch
 // (float32: near sums in sorted order; float64: none) and the finishing
 // launch (float32: L2P and the output in input order; float64: the fused
-// near field + L2P) for the caller to issue around the far field.  Without
-// one it issues them itself (fmm_near_fork / fmm_near_join).
+// near field + L2P) for the caller to issue after the far field.  Without
+// one it issues them itself: the float32 far field on the lane's second
+// stream beside the near field (fmm_far_fork / fmm_far_done / fmm_far_join),
+// so that the near field follows the tree on the lane's stream with no
+// cross-stream wait in between.
 struct FmmSplit {
     std::function<int()> mid;
     P2PLaunch near, finish;
@@ -1074,25 +1077,36 @@ static int lane_stream2(Lane *L)
     return PHF_OK;
 }
 
-// The near field on the lane's second stream, after the work enqueued so
-// far on the lane's stream and beside what follows it there (timed as the
-// P2P stage); fmm_near_join makes the lane's stream wait for it.
-static int fmm_near_fork(Lane *L, const P2PLaunch &near, StageTimer &tm)
+// Work enqueued on the lane's second stream after fmm_far_fork runs after
+// what was enqueued so far on the lane's stream; fmm_far_done marks its
+// end, and fmm_far_join makes the lane's stream wait for that mark.
+static int fmm_far_fork(Lane *L)
 {
     PHF_TRY(lane_stream2(L));
     PHF_HIP(hipEventRecord(L->fork_ev, L->stream));
     PHF_HIP(hipStreamWaitEvent(L->stream2, L->fork_ev, 0));
-    tm.begin_on(PHF_STAGE_P2P, L->stream2);
-    const int rc = near(L->stream2);
-    tm.end_on(PHF_STAGE_P2P, L->stream2);
-    PHF_HIP(hipEventRecord(L->join_ev, L->stream2));
-    return rc;
+    return PHF_OK;
 }
 
-static int fmm_near_join(Lane *L)
+static int fmm_far_done(Lane *L)
+{
+    PHF_HIP(hipEventRecord(L->join_ev, L->stream2));
+    return PHF_OK;
+}
+
+static int fmm_far_join(Lane *L)
 {
     PHF_HIP(hipStreamWaitEvent(L->stream, L->join_ev, 0));
     return PHF_OK;
+}
+
+// near field on the lane's stream, timed as the P2P stage
+static int fmm_near(Lane *L, const P2PLaunch &near, StageTimer &tm)
+{
+    tm.begin(PHF_STAGE_P2P);
+    const int rc = near(L->stream);
+    tm.end(PHF_STAGE_P2P);
+    return rc;
 }
 
 template <class T, class TC>
@@ -1223,9 +1237,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     // the far field
     const int vout = a.velocity_out ? 1 : 0;
     const uint32_t nl8 = (nleaf + 7) / 8;
-    int32_t *split;
-    PHF_ALLOC(split, L->arena, int32_t, 9);
-    PHF_TRY(xcd_split(L, Tg.start, nleaf, split));
+    const int32_t *split = Tg.split;  // XCD leaf ranges of the targets
     const int32_t *ts = Tg.start;
     double *qr = a.qr, *qi = a.qi;
     P2PLaunch near, finish;
@@ -1270,14 +1282,22 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         };
     }
     tm.end(PHF_STAGE_TREE);
+    // float32 without a split: the far field goes to the second stream
+    const bool fork = !spl && near;
     int rc = PHF_OK;
-    if (spl)
+    if (spl) {
         rc = spl->mid();
-    else if (near)
-        rc = fmm_near_fork(L, near, tm);
+    } else if (fork) {
+        // the near field is enqueued first, so that it starts as soon as
+        // the tree is done
+        rc = fmm_far_fork(L);
+        if (rc == PHF_OK) rc = fmm_near(L, near, tm);
+        if (rc != PHF_OK) (void)fmm_far_done(L);  // the fork is left balanced
+    }
     PHF_TRY(rc);
 
-    // far field (float32: beside the near field); a failure still joins
+    // far field (on the lane's stream, or the second stream while the two
+    // are swapped); a failure still restores and joins them
     auto far_field = [&]() -> int {
         if (!far) return PHF_OK;
         // P2M
@@ -1345,14 +1365,17 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         tm.end(PHF_STAGE_L2L);
         return PHF_OK;
     };
+    if (fork) std::swap(L->stream, L->stream2);
     rc = far_field();
+    if (fork) std::swap(L->stream, L->stream2);
     if (spl) {
         spl->near = near;
         spl->finish = finish;
         return rc;
     }
-    if (near) {
-        const int jr = fmm_near_join(L);
+    if (fork) {
+        const int dr = fmm_far_done(L);
+        const int jr = dr == PHF_OK ? fmm_far_join(L) : dr;
         PHF_TRY(rc);
         PHF_TRY(jr);
         return finish(L->stream);
@@ -1373,14 +1396,21 @@ static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer 
     const Lists ls = make_lists(dom, c2, a.ws);
     *nnear = (int)ls.near_leaf.size();
     SortedPoints S, Tg;
+    S.want_split = a.same;
     if (a.prec == PHF_PREC_F32C)
         PHF_TRY(build_sorted_payload(L, dom, a.xb, a.yb, a.wb, a.nb, &S));
     else
         PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
-    if (a.same)
+    if (a.same) {
         Tg = S;
-    else
-        PHF_TRY(build_sorted_points(L, dom, a.xt, a.yt, a.nt, &Tg));
+    } else {
+        // the source build has completed the bounding box at dom.org
+        Domain dt = dom;
+        dt.parts = nullptr;
+        dt.nparts = 0;
+        Tg.want_split = true;
+        PHF_TRY(build_sorted_points(L, dt, a.xt, a.yt, a.nt, &Tg));
+    }
     if (info && info->want_counts) {
         PHF_ALLOC(info->d_counts, L->arena, unsigned long long, 2);
         PHF_HIP(hipMemsetAsync(info->d_counts, 0, 2 * sizeof(unsigned long long), L->stream));
@@ -1395,9 +1425,11 @@ static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer 
 
 // ------------------------------------------------------ captured graphs --
 // A lane's recorded FMM launch sequences, in two graphs: bounding box and
-// tree, then the far field; the near field (float32) is launched between
-// the two on the lane's second stream and the finishing kernel after both,
-// eagerly, so that the near field is timed on its own.  An entry is valid
+// tree, then the far field.  A replay launches the tree graph on the lane's
+// stream and then, for float32, the far-field graph on the lane's second
+// stream beside the near field, which follows the tree on the lane's stream
+// (launched eagerly, so that it is timed on its own), and the finishing
+// kernel after both.  An entry is valid
 // for the same inputs and outputs (addresses), sizes and parameters and
 // the same arena state at the start (its buffers are the same arena
 // slices).  A call replays the most recent entry without waiting for the
@@ -1487,12 +1519,20 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
     double4 *bb = L->arena.alloc<double4>(1);
     e->bb = bb;
     if (!bb) rc = PHF_ERR_NOMEM;
-    if (rc == PHF_OK)
+    // with sources, the bounding box is completed by the source tree build
+    double4 *parts = nullptr;
+    int np = 0;
+    if (rc == PHF_OK && a.nb > 0)
+        rc = device_bbox_parts(L, a.xb, a.yb, a.nb, a.same ? nullptr : a.xt,
+                               a.same ? nullptr : a.yt, a.same ? 0 : a.nt, &parts, &np);
+    else if (rc == PHF_OK)
         rc = device_bbox(L, a.xb, a.yb, a.nb, a.same ? nullptr : a.xt, a.same ? nullptr : a.yt,
                          a.same ? 0 : a.nt, bb);
     if (rc == PHF_OK) {
         Domain dom = dom0;
         dom.org = bb;
+        dom.parts = parts;
+        dom.nparts = np;
         FmmSplit spl;
         spl.tables_keep = &e->tables;
         spl.side = c.side;
@@ -1558,27 +1598,39 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
 }
 
 // Launch a recorded entry: tree graph, near field beside the far-field
-// graph, then the finishing kernel.
+// graph (float32), then the finishing kernel.
 static int fmm_replay(Lane *L, FmmGraph *e, StageTimer &tm, int stage0)
 {
     hipError_t le = hipGraphLaunch(e->exec[0], L->stream);
+    tm.end(PHF_STAGE_TREE);
     if (le != hipSuccess) {
         set_error("HIP error %s launching the FMM tree graph", hipGetErrorName(le));
         return PHF_ERR_DEVICE;
     }
-    int rc = e->near ? fmm_near_fork(L, e->near, tm) : PHF_OK;
-    PHF_TRY(rc);
+    int rc = PHF_OK;
+    if (e->near) {
+        // near field first (it starts as soon as the tree is done), then the
+        // far-field graph beside it
+        PHF_TRY(fmm_far_fork(L));
+        const int nr = fmm_near(L, e->near, tm);
+        le = hipGraphLaunch(e->exec[1], L->stream2);
+        tm.end_on(stage0, L->stream2);
+        const int dr = fmm_far_done(L);
+        if (le != hipSuccess) {
+            set_error("HIP error %s launching the FMM far-field graph", hipGetErrorName(le));
+            rc = PHF_ERR_DEVICE;
+        }
+        const int jr = dr == PHF_OK ? fmm_far_join(L) : dr;
+        PHF_TRY(nr);
+        PHF_TRY(rc);
+        PHF_TRY(jr);
+        return e->finish(L->stream);
+    }
     le = hipGraphLaunch(e->exec[1], L->stream);
     tm.end(stage0);
     if (le != hipSuccess) {
         set_error("HIP error %s launching the FMM far-field graph", hipGetErrorName(le));
         rc = PHF_ERR_DEVICE;
-    }
-    if (e->near) {
-        const int jr = fmm_near_join(L);
-        PHF_TRY(rc);
-        PHF_TRY(jr);
-        return e->finish(L->stream);
     }
     PHF_TRY(rc);
     tm.begin(PHF_STAGE_P2P);
@@ -1632,6 +1684,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
     }
     int nnear = 0;
     if (e) {
+        tm.begin(PHF_STAGE_TREE);  // the tree graph
         int rc = fmm_replay(L, e, tm, stage0);
         if (rc == PHF_OK) {
             double4 *hb = L->fmm_graphs->hb_host;
@@ -1660,7 +1713,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
             if (rc == PHF_OK && se != hipSuccess) PHF_HIP(se);
         }
         PHF_TRY(rc);
-        used[PHF_STAGE_TREE_FAR] = used[PHF_STAGE_P2P] = true;
+        used[PHF_STAGE_TREE_FAR] = used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
     } else {
         used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
         PHF_TRY(fmm_enqueue(L, a, dom, tm, used, info, nullptr, &nnear));

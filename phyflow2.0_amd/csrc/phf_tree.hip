@@ -14,9 +14,10 @@
 // stable sort bit for bit (the binning uses the same FP64 operations, with
 // no contraction possible: a subtract, a multiply, floor).
 //
-// Radix sort: per-tile digit histograms -> device-wide exclusive scan
-// (digit-major, so each tile's bucket offsets come out in order) -> stable
-// scatter.  Inside a tile, elements are ranked 256 at a time; a wave finds
+// Radix sort: per-tile digit histograms -> per-digit scan over the tiles
+// (one workgroup per digit; the digit totals are scanned again by every
+// tile of the scatter) -> stable scatter.  The first pass's histograms come
+// from the kernel that forms the keys.  Inside a tile, elements are ranked 256 at a time; a wave finds
 // the lanes sharing its digit with 8 ballots ("match any"), and per-wave
 // counts in LDS give the cross-wave prefix, so equal keys keep input order.
 #include "phf_tree.hpp"
@@ -29,6 +30,29 @@ namespace phf {
 
 // ----------------------------------------------------------------- bbox --
 constexpr int BB_THREADS = 256;
+
+__device__ __forceinline__ double4 bbox_combine(const double4 &a, const double4 &b)
+{
+    return make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z), fmax(a.w, b.w));
+}
+
+// {xmin, xmax, ymin, ymax} of the np partial boxes, in every thread (all
+// threads of the BB_THREADS-wide workgroup call it)
+__device__ __forceinline__ double4 bbox_reduce_block(const double4 *__restrict__ part, int np,
+                                                     double4 *sh)
+{
+    double4 acc = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+    for (int i = threadIdx.x; i < np; i += BB_THREADS) acc = bbox_combine(acc, part[i]);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = BB_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] = bbox_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    const double4 r = sh[0];
+    __syncthreads();
+    return r;
+}
 
 __global__ __launch_bounds__(BB_THREADS) void bbox_partial(
     const double *__restrict__ x1, const double *__restrict__ y1, int64_t n1,
@@ -64,26 +88,12 @@ __global__ __launch_bounds__(BB_THREADS) void bbox_final(const double4 *__restri
                                                          int np, double4 *__restrict__ out)
 {
     __shared__ double4 sh[BB_THREADS];
-    double4 acc = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
-    for (int i = threadIdx.x; i < np; i += BB_THREADS) {
-        double4 b = part[i];
-        acc = make_double4(fmin(acc.x, b.x), fmax(acc.y, b.y), fmin(acc.z, b.z), fmax(acc.w, b.w));
-    }
-    sh[threadIdx.x] = acc;
-    __syncthreads();
-    for (int s = BB_THREADS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            double4 a = sh[threadIdx.x], b = sh[threadIdx.x + s];
-            sh[threadIdx.x] = make_double4(fmin(a.x, b.x), fmax(a.y, b.y), fmin(a.z, b.z),
-                                           fmax(a.w, b.w));
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = sh[0];
+    const double4 r = bbox_reduce_block(part, np, sh);
+    if (threadIdx.x == 0) *out = r;
 }
 
-int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const double *x2,
-                const double *y2, int64_t n2, double4 *d_out)
+int device_bbox_parts(Lane *L, const double *x1, const double *y1, int64_t n1, const double *x2,
+                      const double *y2, int64_t n2, double4 **parts, int *np)
 {
     const int64_t n = n1 + n2;
     int nblk = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + BB_THREADS * 8 - 1) /
@@ -91,6 +101,18 @@ int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const d
     double4 *part;
     PHF_ALLOC(part, L->arena, double4, (size_t)nblk);
     bbox_partial<<<nblk, BB_THREADS, 0, L->stream>>>(x1, y1, n1, x2, y2, n2, part);
+    PHF_HIP(hipGetLastError());
+    *parts = part;
+    *np = nblk;
+    return PHF_OK;
+}
+
+int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const double *x2,
+                const double *y2, int64_t n2, double4 *d_out)
+{
+    double4 *part;
+    int nblk;
+    PHF_TRY(device_bbox_parts(L, x1, y1, n1, x2, y2, n2, &part, &nblk));
     bbox_final<<<1, BB_THREADS, 0, L->stream>>>(part, nblk, d_out);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
@@ -120,35 +142,6 @@ __device__ __forceinline__ uint32_t morton_key(double x, double y, double x0, do
     fx = fx > hi ? hi : fx;
     fy = fy > hi ? hi : fy;
     return morton2((uint32_t)fx, (uint32_t)fy);
-}
-
-__global__ void morton_keys_kernel(const double *__restrict__ x, const double *__restrict__ y,
-                                   int64_t n, const double4 *__restrict__ org, double inv_s, int L,
-                                   uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = morton_key(x[i], y[i], org->x, org->z, inv_s, L);
-    vals[i] = (int32_t)i;
-}
-
-// Morton key and the float32 record (x - cx, y - cy, w, input index bits)
-// of each point, relative to its leaf centre (cx, cy).
-__global__ void morton_payload_kernel(const double *__restrict__ x, const double *__restrict__ y,
-                                      const double *__restrict__ w, int64_t n,
-                                      const double4 *__restrict__ org, double s, double inv_s,
-                                      int L, uint32_t *__restrict__ keys,
-                                      float4 *__restrict__ pay)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double x0 = org->x, y0 = org->z, xi = x[i], yi = y[i];
-    const uint32_t key = morton_key(xi, yi, x0, y0, inv_s, L);
-    double cx, cy;
-    leaf_center(key, x0, y0, s, cx, cy);
-    keys[i] = key;
-    pay[i] = make_float4((float)(xi - cx), (float)(yi - cy), (float)w[i],
-                         __int_as_float((int32_t)i));
 }
 
 // ----------------------------------------------------------------- scan --
@@ -257,6 +250,64 @@ int device_exclusive_scan_ws(Lane *L, const uint32_t *in, int64_t n, uint32_t *o
 constexpr int RS_THREADS = 256;
 constexpr int RS_ROUNDS = 16;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
+static_assert(RS_THREADS == BB_THREADS && RS_THREADS == SC_THREADS, "one workgroup width");
+
+// Corner of the Morton grid: the device bounding box, or, with parts set,
+// the reduction of the np partial boxes of bbox_partial, which workgroup 0
+// also stores at *org for the kernels that follow.  All threads call it.
+__device__ __forceinline__ void tile_origin(const double4 *org, const double4 *__restrict__ parts,
+                                            int np, double &x0, double &y0)
+{
+    if (!parts) {
+        x0 = org->x;
+        y0 = org->z;
+        return;
+    }
+    __shared__ double4 sh[BB_THREADS];
+    const double4 bb = bbox_reduce_block(parts, np, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *const_cast<double4 *>(org) = bb;
+    x0 = bb.x;
+    y0 = bb.z;
+}
+
+// Morton keys of the workgroup's tile of RS_TILE points and either the
+// input index (pay == nullptr) or the float32 record (x - cx, y - cy, w,
+// input index bits) relative to the point's leaf centre (cx, cy); with
+// hist set, the tile's histogram of the first radix digit (key & mask) in
+// the layout of rs_hist.
+__global__ __launch_bounds__(RS_THREADS) void morton_tiles(
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
+    int64_t n, const double4 *org, const double4 *__restrict__ parts, int np, double s,
+    double inv_s, int L, uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
+    float4 *__restrict__ pay, uint32_t *__restrict__ hist, int nblocks, uint32_t mask)
+{
+    __shared__ uint32_t h[256];
+    double x0, y0;
+    tile_origin(org, parts, np, x0, y0);
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        if (i < n) {
+            const double xi = x[i], yi = y[i];
+            const uint32_t key = morton_key(xi, yi, x0, y0, inv_s, L);
+            keys[i] = key;
+            if (pay) {
+                double cx, cy;
+                leaf_center(key, x0, y0, s, cx, cy);
+                pay[i] = make_float4((float)(xi - cx), (float)(yi - cy), (float)w[i],
+                                     __int_as_float((int32_t)i));
+            } else {
+                vals[i] = (int32_t)i;
+            }
+            if (hist) atomicAdd(&h[key & mask], 1u);
+        }
+    }
+    __syncthreads();
+    if (hist) hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
 
 // The tile's keys (and values) are loaded up front, one load per round in
 // flight at once, before the ranking rounds.
@@ -283,14 +334,37 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint32_t *__restrict
     hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Row d of the tile histograms (workgroup d): offs[d][b] = tiles before b,
+// tot[d] = all tiles.
+__global__ __launch_bounds__(RS_THREADS) void rs_rowscan(const uint32_t *__restrict__ hist,
+                                                         int nblocks, uint32_t *__restrict__ offs,
+                                                         uint32_t *__restrict__ tot)
+{
+    __shared__ uint32_t shw[RS_THREADS / 64];
+    const int64_t row = (int64_t)blockIdx.x * nblocks;
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nblocks; b0 += RS_THREADS) {
+        const int b = b0 + (int)threadIdx.x;
+        const uint32_t v = b < nblocks ? hist[row + b] : 0u;
+        uint32_t t;
+        const uint32_t ex = block_exclusive_scan(v, shw, &t);
+        if (b < nblocks) offs[row + b] = carry + ex;
+        carry += t;
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+// Stable scatter of the tile: digit d's run starts at the digit base (the
+// exclusive scan of tot) plus offs[d][tile].
 template <class VT>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     const uint32_t *__restrict__ kin, const VT *__restrict__ vin, uint32_t *__restrict__ kout,
     VT *__restrict__ vout, int64_t n, int shift, int nbits, const uint32_t *__restrict__ offs,
-    int nblocks)
+    const uint32_t *__restrict__ tot, int nblocks)
 {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    __shared__ uint32_t shw[RS_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t mask = (1u << nbits) - 1u;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -302,7 +376,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
         kr[r] = i < n ? kin[i] : 0u;
         vr[r] = i < n ? vin[i] : VT{};
     }
-    run[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+    const bool digit = tid < (1 << nbits);
+    const uint32_t dbase = block_exclusive_scan(digit ? tot[tid] : 0u, shw, nullptr);
+    run[tid] = dbase + (digit ? offs[(int64_t)tid * nblocks + blockIdx.x] : 0u);
 #pragma unroll
     for (int w = 0; w < RS_THREADS / 64; ++w) wcnt[w][tid] = 0;
     __syncthreads();
@@ -343,8 +419,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
 int64_t radix_ws_size(int64_t n)
 {
     const int64_t nblocks = std::max<int64_t>(1, (n + RS_TILE - 1) / RS_TILE);
-    return 256 * nblocks;
+    return 256 * nblocks + 256;  // tile histograms, then the digit totals
 }
+
+static int radix_tiles(int64_t n) { return (int)std::max<int64_t>(1, (n + RS_TILE - 1) / RS_TILE); }
 
 int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
                       int64_t n, int key_bits, uint32_t **kres, int32_t **vres)
@@ -353,32 +431,33 @@ int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, in
     *vres = vals;
     if (n <= 1 || key_bits <= 0) return PHF_OK;
     const int64_t ws = radix_ws_size(n);
-    uint32_t *hist, *offs, *sums;
+    uint32_t *hist, *offs;
     PHF_ALLOC(hist, L->arena, uint32_t, (size_t)ws);
     PHF_ALLOC(offs, L->arena, uint32_t, (size_t)ws);
-    PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(ws));
-    return device_radix_sort_ws(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, sums, kres,
-                                vres);
+    return device_radix_sort_ws(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, kres, vres);
 }
 
+// hist0: the first pass's tile histograms are already in hist (morton_tiles)
 template <class VT>
 static int radix_sort_ws(Lane *L, uint32_t *keys, VT *vals, uint32_t *ktmp, VT *vtmp, int64_t n,
-                         int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
-                         uint32_t **kres, VT **vres)
+                         int key_bits, uint32_t *hist, uint32_t *offs, uint32_t **kres,
+                         VT **vres, bool hist0 = false)
 {
     *kres = keys;
     *vres = vals;
     if (n <= 1 || key_bits <= 0) return PHF_OK;
-    const int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    const int nblocks = radix_tiles(n);
+    uint32_t *tot = hist + (int64_t)256 * nblocks;
     uint32_t *ka = keys, *kb = ktmp;
     VT *va = vals, *vb = vtmp;
     for (int shift = 0; shift < key_bits; shift += 8) {
         const int nbits = std::min(8, key_bits - shift);
         const uint32_t mask = (1u << nbits) - 1u;
-        rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
-        PHF_TRY(device_exclusive_scan_ws(L, hist, (int64_t)256 * nblocks, offs, sums));
+        if (shift > 0 || !hist0)
+            rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
+        rs_rowscan<<<1u << nbits, RS_THREADS, 0, L->stream>>>(hist, nblocks, offs, tot);
         rs_scatter<VT><<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits,
-                                                              offs, nblocks);
+                                                              offs, tot, nblocks);
         PHF_HIP(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);
@@ -389,18 +468,25 @@ static int radix_sort_ws(Lane *L, uint32_t *keys, VT *vals, uint32_t *ktmp, VT *
 }
 
 int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
-                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
+                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs,
                          uint32_t **kres, int32_t **vres)
 {
-    return radix_sort_ws<int32_t>(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, sums, kres,
-                                  vres);
+    return radix_sort_ws<int32_t>(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, kres, vres);
 }
 
 // --------------------------------------------------------- leaf offsets --
+// start[k] = first sorted position with key >= k (k = 0..nleaf); with split
+// set, also the leaf ranges of the 8 XCDs holding equal numbers of points:
+// split[j] (j = 0..8) is the leaf of sorted point j ceil(n/8), nleaf when
+// there is none.
 __global__ void leaf_starts_kernel(const uint32_t *__restrict__ skeys, int64_t n, int64_t nleaf,
-                                   int32_t *__restrict__ start)
+                                   int32_t *__restrict__ start, int32_t *__restrict__ split)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (split && k <= 8) {
+        const int64_t T = n > 0 ? (n + 7) / 8 : 1;
+        split[k] = (k < 8 && k * T < n) ? (int32_t)skeys[k * T] : (int32_t)nleaf;
+    }
     if (k > nleaf) return;
     int64_t lo = 0, hi = n;  // first index with key >= k
     while (lo < hi) {
@@ -413,29 +499,48 @@ __global__ void leaf_starts_kernel(const uint32_t *__restrict__ skeys, int64_t n
     start[k] = (int32_t)lo;
 }
 
-int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start)
+int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start,
+                       int32_t *split)
 {
-    leaf_starts_kernel<<<div_up(nkeys + 1, 256), 256, 0, L->stream>>>(skeys, n, nkeys, start);
+    leaf_starts_kernel<<<div_up(std::max<int64_t>(nkeys + 1, 9), 256), 256, 0, L->stream>>>(
+        skeys, n, nkeys, start, split);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
 
-int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
-                        SortedPoints *out)
+// Keys and values (index or float32 record) in sorted order, leaf offsets
+// and, if asked for, the XCD leaf ranges.
+template <class VT>
+static int build_sorted(Lane *L, const Domain &dom, const double *x, const double *y,
+                        const double *w, int64_t n, SortedPoints *out)
 {
+    constexpr bool PAY = sizeof(VT) == sizeof(float4);
     uint32_t *k0, *k1;
-    int32_t *v0, *v1;
+    VT *v0, *v1;
     const size_t m = (size_t)(n > 0 ? n : 1);
     PHF_ALLOC(k0, L->arena, uint32_t, m);
     PHF_ALLOC(k1, L->arena, uint32_t, m);
-    PHF_ALLOC(v0, L->arena, int32_t, m);
-    PHF_ALLOC(v1, L->arena, int32_t, m);
+    PHF_ALLOC(v0, L->arena, VT, m);
+    PHF_ALLOC(v1, L->arena, VT, m);
     const int64_t nleaf = int64_t(1) << (2 * dom.L);
     int32_t *start;
     PHF_ALLOC(start, L->arena, int32_t, (size_t)nleaf + 1);
+    out->split = nullptr;
+    if (out->want_split) PHF_ALLOC(out->split, L->arena, int32_t, 9);
+    const int key_bits = 2 * dom.L;
+    const bool sort = n > 1 && key_bits > 0;
+    uint32_t *hist = nullptr, *offs = nullptr;
+    if (sort) {
+        const int64_t ws = radix_ws_size(n);
+        PHF_ALLOC(hist, L->arena, uint32_t, (size_t)ws);
+        PHF_ALLOC(offs, L->arena, uint32_t, (size_t)ws);
+    }
+    const int nblocks = radix_tiles(n);
     if (n > 0)
-        morton_keys_kernel<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, dom.org, dom.inv_s,
-                                                                  dom.L, k0, v0);
+        morton_tiles<<<nblocks, RS_THREADS, 0, L->stream>>>(
+            x, y, w, n, dom.org, dom.parts, dom.nparts, dom.s, dom.inv_s, dom.L, k0,
+            PAY ? nullptr : (int32_t *)v0, PAY ? (float4 *)v0 : nullptr, hist, nblocks,
+            (1u << std::min(8, key_bits)) - 1u);
     PHF_HIP(hipGetLastError());
     out->keys_in = nullptr;
     if (out->want_input_keys) {
@@ -444,54 +549,33 @@ int build_sorted_points(Lane *L, const Domain &dom, const double *x, const doubl
             PHF_HIP(hipMemcpyAsync(out->keys_in, k0, sizeof(uint32_t) * n,
                                    hipMemcpyDeviceToDevice, L->stream));
     }
-    uint32_t *ks;
-    int32_t *vs;
-    PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, n, 2 * dom.L, &ks, &vs));
-    leaf_starts_kernel<<<div_up(nleaf + 1, 256), 256, 0, L->stream>>>(ks, n, nleaf, start);
-    PHF_HIP(hipGetLastError());
+    uint32_t *ks = k0;
+    VT *vs = v0;
+    if (sort) PHF_TRY(radix_sort_ws<VT>(L, k0, v0, k1, v1, n, key_bits, hist, offs, &ks, &vs, true));
+    PHF_TRY(leaf_starts_device(L, ks, n, nleaf, start, out->split));
     out->n = n;
     out->keys = ks;
-    out->perm = vs;
     out->start = start;
+    if (PAY) {
+        out->perm = nullptr;
+        out->pay = (float4 *)vs;
+    } else {
+        out->perm = (int32_t *)vs;
+        out->pay = nullptr;
+    }
     return PHF_OK;
+}
+
+int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
+                        SortedPoints *out)
+{
+    return build_sorted<int32_t>(L, dom, x, y, nullptr, n, out);
 }
 
 int build_sorted_payload(Lane *L, const Domain &dom, const double *x, const double *y,
                          const double *w, int64_t n, SortedPoints *out)
 {
-    uint32_t *k0, *k1;
-    float4 *p0, *p1;
-    const size_t m = (size_t)(n > 0 ? n : 1);
-    PHF_ALLOC(k0, L->arena, uint32_t, m);
-    PHF_ALLOC(k1, L->arena, uint32_t, m);
-    PHF_ALLOC(p0, L->arena, float4, m);
-    PHF_ALLOC(p1, L->arena, float4, m);
-    const int64_t nleaf = int64_t(1) << (2 * dom.L);
-    int32_t *start;
-    PHF_ALLOC(start, L->arena, int32_t, (size_t)nleaf + 1);
-    if (n > 0)
-        morton_payload_kernel<<<div_up(n, 256), 256, 0, L->stream>>>(
-            x, y, w, n, dom.org, dom.s, dom.inv_s, dom.L, k0, p0);
-    PHF_HIP(hipGetLastError());
-    uint32_t *ks = k0;
-    float4 *ps = p0;
-    if (n > 1) {
-        const int64_t ws = radix_ws_size(n);
-        uint32_t *hist, *offs, *sums;
-        PHF_ALLOC(hist, L->arena, uint32_t, (size_t)ws);
-        PHF_ALLOC(offs, L->arena, uint32_t, (size_t)ws);
-        PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(ws));
-        PHF_TRY(radix_sort_ws<float4>(L, k0, p0, k1, p1, n, 2 * dom.L, hist, offs, sums, &ks,
-                                      &ps));
-    }
-    leaf_starts_kernel<<<div_up(nleaf + 1, 256), 256, 0, L->stream>>>(ks, n, nleaf, start);
-    PHF_HIP(hipGetLastError());
-    out->n = n;
-    out->keys = ks;
-    out->perm = nullptr;
-    out->pay = ps;
-    out->start = start;
-    return PHF_OK;
+    return build_sorted<float4>(L, dom, x, y, w, n, out);
 }
 
 }  // namespace phf

@@ -16,6 +16,10 @@ struct Domain {
     // device_bbox): kernels read the origin there, so a captured launch
     // sequence stays valid when the points move within the same side
     const double4 *org = nullptr;
+    // optional: the partial boxes of device_bbox_parts, reduced by the
+    // first tree build that gets them, which then stores the box at *org
+    const double4 *parts = nullptr;
+    int nparts = 0;
 };
 
 // side > 0: the domain side to use (at least the extent, from
@@ -27,6 +31,9 @@ Domain make_domain(double xmin, double xmax, double ymin, double ymax, int L,
 // ymin, ymax} (device memory).
 int device_bbox(Lane *L, const double *x1, const double *y1, int64_t n1, const double *x2,
                 const double *y2, int64_t n2, double4 *d_out);
+// its first kernel only: *np partial boxes at *parts (arena)
+int device_bbox_parts(Lane *L, const double *x1, const double *y1, int64_t n1, const double *x2,
+                      const double *y2, int64_t n2, double4 **parts, int *np);
 
 int device_exclusive_scan(Lane *L, const uint32_t *in, int64_t n, uint32_t *out);
 // variants with caller-provided workspaces (no allocation: capturable)
@@ -35,7 +42,7 @@ int device_exclusive_scan_ws(Lane *L, const uint32_t *in, int64_t n, uint32_t *o
                              uint32_t *sums);
 int64_t radix_ws_size(int64_t n);  // entries of hist and of offs
 int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
-                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs, uint32_t *sums,
+                         int64_t n, int key_bits, uint32_t *hist, uint32_t *offs,
                          uint32_t **kres, int32_t **vres);
 
 // Stable LSD radix sort of (keys, vals) on the low key_bits bits; ktmp/vtmp
@@ -54,10 +61,16 @@ struct SortedPoints {
     // build_sorted_payload: sorted float32 records (x - cx, y - cy, w,
     // input index bits) relative to the leaf centres, perm unset
     float4 *pay = nullptr;
+    // leaf ranges [split[k], split[k+1]) of the 8 XCDs holding equal
+    // numbers of points (9 entries; if requested)
+    int32_t *split = nullptr;
+    bool want_split = false;
 };
 
-// start[k] = first position of sorted keys with key >= k, k = 0..nkeys.
-int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start);
+// start[k] = first position of sorted keys with key >= k, k = 0..nkeys;
+// split (nullable): the XCD leaf ranges as SortedPoints::split.
+int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start,
+                       int32_t *split = nullptr);
 
 int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
                         SortedPoints *out);
