@@ -47,12 +47,14 @@ def timed_steps(sysm, params, warmup, steps, counts=True):
     t0 = time.perf_counter()
     for _ in range(steps):
         a = time.perf_counter()
-        sysm.velocity(params, fetch=False, info=True)
+        sysm.velocity(params, fetch=False, p2p_timing_only=True)
         lat.append(time.perf_counter() - a)
-        info = sysm.last_info
-        p2p_ms.append(info["ms"].get("p2p", float("nan")))
+        p2p_ms.append(sysm.last_info["ms"].get("p2p", float("nan")))
     _native.lib().phf_synchronize()
     wall = time.perf_counter() - t0
+    # stage timings from one more (untimed) evaluation
+    sysm.velocity(params, fetch=False, info=True)
+    info = sysm.last_info
     if counts:  # work counts from one extra (untimed) evaluation
         sysm.velocity(params, fetch=False, info=True, count_work=True)
         info = dict(info, p2p_pairs=sysm.last_info["p2p_pairs"],
@@ -404,8 +406,10 @@ def main():
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 100 for fmm1m, 30 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps first (default 20 for fmm1m, 5 otherwise)")
     ap.add_argument("--n", type=int, default=1048576)
     ap.add_argument("--order", type=int, default=16)
     ap.add_argument("--leaf", type=int, default=64)
@@ -414,6 +418,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variants", action="store_true", help="also time p in {8,16,24} x {f32c,f64}")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 100 if args.workload == "fmm1m" else 30
+    if args.warmup is None:
+        args.warmup = 20 if args.workload == "fmm1m" else 5
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)

@@ -258,9 +258,12 @@ typedef struct {
  * A resident system's FMM records its launch sequence (tree through L2L)
  * as a HIP graph and replays it while the point count, parameters and the
  * domain side stay the same; flags |= PHF_INFO_EAGER launches every
- * kernel one by one instead (same results). */
+ * kernel one by one instead (same results).  flags |= PHF_INFO_TIME_P2P
+ * times only the near-field (P2P) stage, with two events on the stream
+ * (the other stages report -1). */
 #define PHF_INFO_COUNT_WORK 1
 #define PHF_INFO_EAGER 2
+#define PHF_INFO_TIME_P2P 4
 typedef struct {
     int32_t L, p, nnear, flags;
     double D;

@@ -160,23 +160,26 @@ struct LaneGuard {
     }
 };
 
-// Per-stage timing with HIP events on the lane's stream.
+// Per-stage timing with HIP events on the lane's stream; mask selects the
+// stages that are timed (bit s for stage s).
 struct StageTimer {
     Lane *lane;
     bool on;
+    uint32_t mask = ~0u;
     StageTimer(Lane *l, bool enabled) : lane(l), on(enabled) {}
+    bool timed(int stage) const { return on && ((mask >> stage) & 1u); }
     void begin(int stage) {
-        if (on) hipEventRecord(lane->ev[2 * stage], lane->stream);
+        if (timed(stage)) hipEventRecord(lane->ev[2 * stage], lane->stream);
     }
     void end(int stage) {
-        if (on) hipEventRecord(lane->ev[2 * stage + 1], lane->stream);
+        if (timed(stage)) hipEventRecord(lane->ev[2 * stage + 1], lane->stream);
     }
     // the same on another stream of the lane
     void begin_on(int stage, hipStream_t s) {
-        if (on) hipEventRecord(lane->ev[2 * stage], s);
+        if (timed(stage)) hipEventRecord(lane->ev[2 * stage], s);
     }
     void end_on(int stage, hipStream_t s) {
-        if (on) hipEventRecord(lane->ev[2 * stage + 1], s);
+        if (timed(stage)) hipEventRecord(lane->ev[2 * stage + 1], s);
     }
 };
 

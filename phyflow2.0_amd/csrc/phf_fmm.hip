@@ -1446,6 +1446,7 @@ struct FmmGraph {
     hipGraphExec_t exec[2] = {};
     char *tables = nullptr;     // device tables the graphs' kernels read
     double4 *bb = nullptr;      // the recorded bounding box (arena slot)
+    bool bb_host = false;       // the graph also writes it to the pinned copy
     P2PLaunch near, finish;
     uint64_t last = 0;
 };
@@ -1533,6 +1534,9 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
         dom.org = bb;
         dom.parts = parts;
         dom.nparts = np;
+        // the source tree build also writes the box to the pinned copy
+        dom.org_host = parts ? c.hb_host : nullptr;
+        e->bb_host = parts != nullptr;
         FmmSplit spl;
         spl.tables_keep = &e->tables;
         spl.side = c.side;
@@ -1650,7 +1654,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
     if (graph) {
         if (!L->fmm_graphs) L->fmm_graphs = new FmmGraphCache;
         FmmGraphCache &c = *L->fmm_graphs;
-        if (!c.hb_host) PHF_HIP(hipHostMalloc(&c.hb_host, 256, hipHostMallocDefault));
+        if (!c.hb_host) PHF_HIP(hipHostMalloc(&c.hb_host, 256, hipHostMallocCoherent));
         if (!info->no_speculation)
             for (FmmGraph *g : c.v)
                 if (same_args(g->a, a) && L->arena.at(g->m0)) e = g;
@@ -1688,8 +1692,9 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         int rc = fmm_replay(L, e, tm, stage0);
         if (rc == PHF_OK) {
             double4 *hb = L->fmm_graphs->hb_host;
-            const hipError_t fe = hipMemcpyAsync(hb, e->bb, sizeof(double4),
-                                                 hipMemcpyDeviceToHost, L->stream);
+            const hipError_t fe = e->bb_host ? hipSuccess
+                                             : hipMemcpyAsync(hb, e->bb, sizeof(double4),
+                                                              hipMemcpyDeviceToHost, L->stream);
             if (fe != hipSuccess) {
                 set_error("HIP error %s reading the bounding box", hipGetErrorName(fe));
                 rc = PHF_ERR_DEVICE;

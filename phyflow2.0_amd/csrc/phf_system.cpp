@@ -217,6 +217,8 @@ int phf_system_velocity(phf_system *h, const phf_vel_params *p, double *vx, doub
     Lane *L = &s->lane;
     const bool timed = info != nullptr || timing_enabled();
     StageTimer tm(L, timed);
+    if (info != nullptr && (info->flags & PHF_INFO_TIME_P2P))
+        tm.mask = 1u << PHF_STAGE_P2P;
     bool used[PHF_STAGE_COUNT] = {};
     FmmInfo fi;
     unsigned long long counts[2] = {0, 0};
@@ -252,6 +254,7 @@ int phf_system_velocity(phf_system *h, const phf_vel_params *p, double *vx, doub
         if (!fi.domain_ok || fi.domain_ok()) break;
     }
     s->has_velocity = true;
+    for (int k = 0; k < PHF_STAGE_COUNT; ++k) used[k] = used[k] && tm.timed(k);
     record_stage_times(L, used);
     if (info) {
         info->L = fi.L;

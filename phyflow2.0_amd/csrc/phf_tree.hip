@@ -37,15 +37,16 @@ __device__ __forceinline__ double4 bbox_combine(const double4 &a, const double4 
 }
 
 // {xmin, xmax, ymin, ymax} of the np partial boxes, in every thread (all
-// threads of the BB_THREADS-wide workgroup call it)
+// NT threads of the workgroup call it; sh holds NT entries)
+template <int NT>
 __device__ __forceinline__ double4 bbox_reduce_block(const double4 *__restrict__ part, int np,
                                                      double4 *sh)
 {
     double4 acc = make_double4(INFINITY, -INFINITY, INFINITY, -INFINITY);
-    for (int i = threadIdx.x; i < np; i += BB_THREADS) acc = bbox_combine(acc, part[i]);
+    for (int i = threadIdx.x; i < np; i += NT) acc = bbox_combine(acc, part[i]);
     sh[threadIdx.x] = acc;
     __syncthreads();
-    for (int s = BB_THREADS / 2; s > 0; s >>= 1) {
+    for (int s = NT / 2; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) sh[threadIdx.x] = bbox_combine(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(BB_THREADS) void bbox_final(const double4 *__restri
                                                          int np, double4 *__restrict__ out)
 {
     __shared__ double4 sh[BB_THREADS];
-    const double4 r = bbox_reduce_block(part, np, sh);
+    const double4 r = bbox_reduce_block<BB_THREADS>(part, np, sh);
     if (threadIdx.x == 0) *out = r;
 }
 
@@ -150,6 +151,9 @@ constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 8;
 constexpr int SC_CHUNK = SC_THREADS * SC_ITEMS;
 
+// exclusive scan over the NT threads of the workgroup (all call it;
+// sh_wave holds NT / 64 entries)
+template <int NT = SC_THREADS>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *sh_wave,
                                                          uint32_t *total)
 {
@@ -163,7 +167,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
     if (lane == 63) sh_wave[wave] = inc;
     __syncthreads();
     uint32_t before = 0, all = 0;
-    for (int w = 0; w < SC_THREADS / 64; ++w) {
+    for (int w = 0; w < NT / 64; ++w) {
         uint32_t s = sh_wave[w];
         if (w < wave) before += s;
         all += s;
@@ -247,25 +251,32 @@ int device_exclusive_scan_ws(Lane *L, const uint32_t *in, int64_t n, uint32_t *o
 }
 
 // ---------------------------------------------------------- radix sort --
-constexpr int RS_THREADS = 256;
-constexpr int RS_ROUNDS = 16;
+// One workgroup of RS_THREADS (16 waves) per tile of RS_TILE keys, taken in
+// RS_ROUNDS rounds of RS_THREADS consecutive keys.
+constexpr int RS_THREADS = 1024;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_ROUNDS = 4;
 constexpr int RS_TILE = RS_THREADS * RS_ROUNDS;
-static_assert(RS_THREADS == BB_THREADS && RS_THREADS == SC_THREADS, "one workgroup width");
+constexpr int RS_SCAN = 256;  // workgroup of rs_rowscan
 
 // Corner of the Morton grid: the device bounding box, or, with parts set,
 // the reduction of the np partial boxes of bbox_partial, which workgroup 0
-// also stores at *org for the kernels that follow.  All threads call it.
+// also stores at *org for the kernels that follow (and at *org_host if
+// set).  All threads call it.
 __device__ __forceinline__ void tile_origin(const double4 *org, const double4 *__restrict__ parts,
-                                            int np, double &x0, double &y0)
+                                            int np, double4 *org_host, double &x0, double &y0)
 {
     if (!parts) {
         x0 = org->x;
         y0 = org->z;
         return;
     }
-    __shared__ double4 sh[BB_THREADS];
-    const double4 bb = bbox_reduce_block(parts, np, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *const_cast<double4 *>(org) = bb;
+    __shared__ double4 sh[RS_THREADS];
+    const double4 bb = bbox_reduce_block<RS_THREADS>(parts, np, sh);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *const_cast<double4 *>(org) = bb;
+        if (org_host) *org_host = bb;
+    }
     x0 = bb.x;
     y0 = bb.z;
 }
@@ -277,19 +288,21 @@ __device__ __forceinline__ void tile_origin(const double4 *org, const double4 *_
 // the layout of rs_hist.
 __global__ __launch_bounds__(RS_THREADS) void morton_tiles(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
-    int64_t n, const double4 *org, const double4 *__restrict__ parts, int np, double s,
-    double inv_s, int L, uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
-    float4 *__restrict__ pay, uint32_t *__restrict__ hist, int nblocks, uint32_t mask)
+    int64_t n, const double4 *org, const double4 *__restrict__ parts, int np,
+    double4 *org_host, double s, double inv_s, int L, uint32_t *__restrict__ keys,
+    int32_t *__restrict__ vals, float4 *__restrict__ pay, uint32_t *__restrict__ hist,
+    int nblocks, uint32_t mask)
 {
     __shared__ uint32_t h[256];
+    const int tid = threadIdx.x;
     double x0, y0;
-    tile_origin(org, parts, np, x0, y0);
-    h[threadIdx.x] = 0;
+    tile_origin(org, parts, np, org_host, x0, y0);
+    if (tid < 256) h[tid] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-#pragma unroll 4
+#pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        const int64_t i = base + (int64_t)r * RS_THREADS + tid;
         if (i < n) {
             const double xi = x[i], yi = y[i];
             const uint32_t key = morton_key(xi, yi, x0, y0, inv_s, L);
@@ -306,48 +319,49 @@ __global__ __launch_bounds__(RS_THREADS) void morton_tiles(
         }
     }
     __syncthreads();
-    if (hist) hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    if (hist && tid < 256) hist[(int64_t)tid * nblocks + blockIdx.x] = h[tid];
 }
 
-// The tile's keys (and values) are loaded up front, one load per round in
-// flight at once, before the ranking rounds.
+// Tile histograms of the digit (key >> shift) & mask: hist[d][tile]
+// (digit-major, nblocks tiles per row).
 __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint32_t *__restrict__ keys,
                                                       int64_t n, int shift, uint32_t mask,
                                                       uint32_t *__restrict__ hist, int nblocks)
 {
     __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    const int tid = threadIdx.x;
+    if (tid < 256) h[tid] = 0;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     uint32_t kr[RS_ROUNDS];
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        const int64_t i = base + (int64_t)r * RS_THREADS + tid;
         kr[r] = i < n ? keys[i] : 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < RS_ROUNDS; ++r) {
-        const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+        const int64_t i = base + (int64_t)r * RS_THREADS + tid;
         if (i < n) atomicAdd(&h[(kr[r] >> shift) & mask], 1u);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    if (tid < 256) hist[(int64_t)tid * nblocks + blockIdx.x] = h[tid];
 }
 
 // Row d of the tile histograms (workgroup d): offs[d][b] = tiles before b,
 // tot[d] = all tiles.
-__global__ __launch_bounds__(RS_THREADS) void rs_rowscan(const uint32_t *__restrict__ hist,
-                                                         int nblocks, uint32_t *__restrict__ offs,
-                                                         uint32_t *__restrict__ tot)
+__global__ __launch_bounds__(RS_SCAN) void rs_rowscan(const uint32_t *__restrict__ hist,
+                                                      int nblocks, uint32_t *__restrict__ offs,
+                                                      uint32_t *__restrict__ tot)
 {
-    __shared__ uint32_t shw[RS_THREADS / 64];
+    __shared__ uint32_t shw[RS_SCAN / 64];
     const int64_t row = (int64_t)blockIdx.x * nblocks;
     uint32_t carry = 0;
-    for (int b0 = 0; b0 < nblocks; b0 += RS_THREADS) {
+    for (int b0 = 0; b0 < nblocks; b0 += RS_SCAN) {
         const int b = b0 + (int)threadIdx.x;
         const uint32_t v = b < nblocks ? hist[row + b] : 0u;
         uint32_t t;
-        const uint32_t ex = block_exclusive_scan(v, shw, &t);
+        const uint32_t ex = block_exclusive_scan<RS_SCAN>(v, shw, &t);
         if (b < nblocks) offs[row + b] = carry + ex;
         carry += t;
     }
@@ -355,7 +369,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_rowscan(const uint32_t *__restr
 }
 
 // Stable scatter of the tile: digit d's run starts at the digit base (the
-// exclusive scan of tot) plus offs[d][tile].
+// exclusive scan of tot) plus offs[d][tile].  Per round, each wave ranks
+// its keys among the lanes with the same digit (nbits ballots) and counts
+// them per digit; one thread per digit turns the 16 waves' counts into
+// positions, so equal keys keep their input order.
 template <class VT>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     const uint32_t *__restrict__ kin, const VT *__restrict__ vin, uint32_t *__restrict__ kout,
@@ -363,8 +380,9 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     const uint32_t *__restrict__ tot, int nblocks)
 {
     __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
-    __shared__ uint32_t shw[RS_THREADS / 64];
+    __shared__ uint32_t cnt[RS_WAVES][256];
+    __shared__ uint32_t pre[RS_WAVES][256];
+    __shared__ uint32_t shw[RS_WAVES];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t mask = (1u << nbits) - 1u;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -376,11 +394,11 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
         kr[r] = i < n ? kin[i] : 0u;
         vr[r] = i < n ? vin[i] : VT{};
     }
-    const bool digit = tid < (1 << nbits);
-    const uint32_t dbase = block_exclusive_scan(digit ? tot[tid] : 0u, shw, nullptr);
-    run[tid] = dbase + (digit ? offs[(int64_t)tid * nblocks + blockIdx.x] : 0u);
+    const bool digit = tid < (1 << nbits);  // tid < 256
+    const uint32_t dbase = block_exclusive_scan<RS_THREADS>(digit ? tot[tid] : 0u, shw, nullptr);
+    if (tid < 256) run[tid] = dbase + (digit ? offs[(int64_t)tid * nblocks + blockIdx.x] : 0u);
 #pragma unroll
-    for (int w = 0; w < RS_THREADS / 64; ++w) wcnt[w][tid] = 0;
+    for (int k = 0; k < RS_WAVES * 256 / RS_THREADS; ++k) (&cnt[0][0])[tid + k * RS_THREADS] = 0;
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
@@ -396,23 +414,28 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
             m &= bit ? bal : ~bal;
         }
         const uint32_t rank = (uint32_t)__popcll(m & lt);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+        if (valid && rank == 0) cnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (tid < 256) {
+            // positions of each wave's first key of digit tid, in wave order
+            uint32_t c[RS_WAVES];
+#pragma unroll
+            for (int w = 0; w < RS_WAVES; ++w) c[w] = cnt[w][tid];
+            uint32_t s = run[tid];
+#pragma unroll
+            for (int w = 0; w < RS_WAVES; ++w) {
+                pre[w][tid] = s;
+                cnt[w][tid] = 0;
+                s += c[w];
+            }
+            run[tid] = s;
+        }
         __syncthreads();
         if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += wcnt[w][d];
+            const uint32_t pos = pre[wave][d] + rank;
             kout[pos] = key;
             vout[pos] = vr[r];
         }
-        __syncthreads();
-        uint32_t add = 0;
-#pragma unroll
-        for (int w = 0; w < RS_THREADS / 64; ++w) {
-            add += wcnt[w][tid];
-            wcnt[w][tid] = 0;
-        }
-        run[tid] += add;
-        __syncthreads();
     }
 }
 
@@ -455,7 +478,7 @@ static int radix_sort_ws(Lane *L, uint32_t *keys, VT *vals, uint32_t *ktmp, VT *
         const uint32_t mask = (1u << nbits) - 1u;
         if (shift > 0 || !hist0)
             rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
-        rs_rowscan<<<1u << nbits, RS_THREADS, 0, L->stream>>>(hist, nblocks, offs, tot);
+        rs_rowscan<<<1u << nbits, RS_SCAN, 0, L->stream>>>(hist, nblocks, offs, tot);
         rs_scatter<VT><<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits,
                                                               offs, tot, nblocks);
         PHF_HIP(hipGetLastError());
@@ -538,7 +561,7 @@ static int build_sorted(Lane *L, const Domain &dom, const double *x, const doubl
     const int nblocks = radix_tiles(n);
     if (n > 0)
         morton_tiles<<<nblocks, RS_THREADS, 0, L->stream>>>(
-            x, y, w, n, dom.org, dom.parts, dom.nparts, dom.s, dom.inv_s, dom.L, k0,
+            x, y, w, n, dom.org, dom.parts, dom.nparts, dom.org_host, dom.s, dom.inv_s, dom.L, k0,
             PAY ? nullptr : (int32_t *)v0, PAY ? (float4 *)v0 : nullptr, hist, nblocks,
             (1u << std::min(8, key_bits)) - 1u);
     PHF_HIP(hipGetLastError());

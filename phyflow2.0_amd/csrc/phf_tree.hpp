@@ -20,6 +20,8 @@ struct Domain {
     // first tree build that gets them, which then stores the box at *org
     const double4 *parts = nullptr;
     int nparts = 0;
+    // with parts: also a host-coherent pinned copy of the completed box
+    double4 *org_host = nullptr;
 };
 
 // side > 0: the domain side to use (at least the extent, from

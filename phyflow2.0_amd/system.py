@@ -66,20 +66,23 @@ class BlobSystem:
             raise ValueError("x, y, w must have equal lengths")
         call("phf_system_set", self._h, ptr(x), ptr(y), ptr(w), x.size)
 
-    def velocity(self, params, fetch=True, info=False, count_work=False, eager=False):
+    def velocity(self, params, fetch=True, info=False, count_work=False, eager=False,
+                 p2p_timing_only=False):
         """Velocity at the blob centres; with fetch=False the result stays
         on the device (fetch later with last_velocity()).  info=True records
         stage timings (and, with count_work, the FMM pair/link counts) in
-        last_info.  The FMM launch sequence is replayed as a captured HIP
-        graph while n, the parameters and the domain side are unchanged;
-        eager=True launches every kernel one by one (same results)."""
+        last_info; p2p_timing_only=True times only the near-field stage (two
+        events).  The FMM launch sequence is replayed as a
+        captured HIP graph while n, the parameters and the domain side are
+        unchanged; eager=True launches every kernel one by one (same
+        results)."""
         n = self.n
         vx = np.empty(n) if fetch else None
         vy = np.empty(n) if fetch else None
-        info = info or eager
+        info = info or eager or p2p_timing_only
         ei = EvalInfo() if info else None
         if ei is not None:
-            ei.flags = (1 if count_work else 0) | (2 if eager else 0)
+            ei.flags = (1 if count_work else 0) | (2 if eager else 0) | (4 if p2p_timing_only else 0)
         call("phf_system_velocity", self._h, ctypes.byref(params),
              ptr(vx) if fetch else None, ptr(vy) if fetch else None,
              ctypes.byref(ei) if info else None)
