@@ -169,17 +169,17 @@ struct StageTimer {
     StageTimer(Lane *l, bool enabled) : lane(l), on(enabled) {}
     bool timed(int stage) const { return on && ((mask >> stage) & 1u); }
     void begin(int stage) {
-        if (timed(stage)) hipEventRecord(lane->ev[2 * stage], lane->stream);
+        if (timed(stage)) (void)hipEventRecord(lane->ev[2 * stage], lane->stream);
     }
     void end(int stage) {
-        if (timed(stage)) hipEventRecord(lane->ev[2 * stage + 1], lane->stream);
+        if (timed(stage)) (void)hipEventRecord(lane->ev[2 * stage + 1], lane->stream);
     }
     // the same on another stream of the lane
     void begin_on(int stage, hipStream_t s) {
-        if (timed(stage)) hipEventRecord(lane->ev[2 * stage], s);
+        if (timed(stage)) (void)hipEventRecord(lane->ev[2 * stage], s);
     }
     void end_on(int stage, hipStream_t s) {
-        if (timed(stage)) hipEventRecord(lane->ev[2 * stage + 1], s);
+        if (timed(stage)) (void)hipEventRecord(lane->ev[2 * stage + 1], s);
     }
 };
 

@@ -54,7 +54,7 @@ __global__ void gather_src_f64(const double *__restrict__ x, const double *__res
 }
 
 __global__ void gather_tgt_f32(const double *__restrict__ x, const double *__restrict__ y,
-                               const uint32_t *__restrict__ skeys,
+                               const uint32_t *__restrict__ skeys, int fsh,
                                const int32_t *__restrict__ perm, int64_t n,
                                const double4 *__restrict__ org, double s,
                                float2 *__restrict__ out)
@@ -63,7 +63,7 @@ __global__ void gather_tgt_f32(const double *__restrict__ x, const double *__res
     if (i >= n) return;
     int32_t j = perm[i];
     double cx, cy;
-    leaf_center(skeys[i], org->x, org->z, s, cx, cy);
+    leaf_center(skeys[i] >> fsh, org->x, org->z, s, cx, cy);
     out[i] = make_float2((float)(x[j] - cx), (float)(y[j] - cy));
 }
 
@@ -563,11 +563,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
 }
 
 // L2P and the output in input order, one thread per sorted target i of
-// leaf keys[i]: (qr, qi) = (-(near_a + far_r), near_b - far_i), far the
+// leaf keys[i] >> fsh: (qr, qi) = (-(near_a + far_r), near_b - far_i), far the
 // leaf's local expansion at the target (float32 Horner); targets are the
 // source records (tgt == nullptr: index bits in .w) or tgt with perm.
 __global__ void l2p_scatter_f32(const float4 *__restrict__ src, const float2 *__restrict__ tgt,
-                                const uint32_t *__restrict__ keys,
+                                const uint32_t *__restrict__ keys, int fsh,
                                 const int32_t *__restrict__ perm, int64_t nt,
                                 const float2 *__restrict__ Bleaf, int p1, float inv_s,
                                 const double2 *__restrict__ nsum, double *__restrict__ out_a,
@@ -586,7 +586,8 @@ __global__ void l2p_scatter_f32(const float4 *__restrict__ src, const float2 *__
         o = __float_as_int(q.w);
     }
     double fr = 0.0, fi = 0.0;
-    if (Bleaf) l2p_f32(Bleaf + (int64_t)keys[i] * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
+    if (Bleaf)
+        l2p_f32(Bleaf + (int64_t)(keys[i] >> fsh) * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
     const double2 ns = nsum[i];
     const double qr = -(ns.x + fr);
     const double qi = ns.y - fi;
@@ -1129,7 +1130,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         if (!a.same) {
             PHF_ALLOC(t2, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
             gather_tgt_f32<<<div_up(a.nt, 256), 256, 0, L->stream>>>(
-                a.xt, a.yt, Tg.keys, Tg.perm, a.nt, dom.org, dom.s, t2);
+                a.xt, a.yt, Tg.keys, 2 * Tg.fine, Tg.perm, a.nt, dom.org, dom.s, t2);
         }
         src = s4;
         tgt = t2;
@@ -1246,6 +1247,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         const float2 *t2 = (const float2 *)tgt;
         const float2 *Bl = far ? (const float2 *)B + off[Lv] * p1 : nullptr;
         const uint32_t *tkeys = Tg.keys;
+        const int fsh = 2 * Tg.fine;
         const float s_f = (float)dom.s, inv_s = 1.0f / s_f;
         const int64_t nt = a.nt;
         double2 *nsum;
@@ -1261,8 +1263,8 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
             return PHF_OK;
         };
         finish = [=](hipStream_t st) {
-            l2p_scatter_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2, tkeys, perm_t, nt, Bl, p1,
-                                                             inv_s, nsum, qr, qi, vout);
+            l2p_scatter_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2, tkeys, fsh, perm_t, nt, Bl,
+                                                             p1, inv_s, nsum, qr, qi, vout);
             PHF_HIP(hipGetLastError());
             return PHF_OK;
         };
@@ -1389,12 +1391,22 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
 
 // Tree and FMM from the domain on (every launch after the bounding box);
 // spl as run_fmm_typed.
-static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom, StageTimer &tm, bool *used,
-                       FmmInfo *info, FmmSplit *spl, int *nnear)
+static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom_in, StageTimer &tm,
+                       bool *used, FmmInfo *info, FmmSplit *spl, int *nnear)
 {
     const double c2 = a.cutoff * a.cutoff;
-    const Lists ls = make_lists(dom, c2, a.ws);
+    const Lists ls = make_lists(dom_in, c2, a.ws);
     *nnear = (int)ls.near_leaf.size();
+    // points in Morton order inside the leaves too, with the key bits the
+    // radix passes leave free: targets that share a wave, and sources taken
+    // together, sit close, so that more pair batches skip the smoothing
+    // exponential
+    const Domain dom = [&] {
+        Domain d = dom_in;
+        const int kb = 2 * d.L, spare = (kb + 7) / 8 * 8 - kb;
+        d.fine = std::min(3, spare / 2);
+        return d;
+    }();
     SortedPoints S, Tg;
     S.want_split = a.same;
     if (a.prec == PHF_PREC_F32C)

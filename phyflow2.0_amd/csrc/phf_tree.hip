@@ -281,7 +281,8 @@ __device__ __forceinline__ void tile_origin(const double4 *org, const double4 *_
     y0 = bb.z;
 }
 
-// Morton keys of the workgroup's tile of RS_TILE points and either the
+// Morton keys of the workgroup's tile of RS_TILE points (L + fine bits per
+// axis, inv_sf = 2^fine inv_s; the leaf is key >> 2 fine) and either the
 // input index (pay == nullptr) or the float32 record (x - cx, y - cy, w,
 // input index bits) relative to the point's leaf centre (cx, cy); with
 // hist set, the tile's histogram of the first radix digit (key & mask) in
@@ -289,7 +290,7 @@ __device__ __forceinline__ void tile_origin(const double4 *org, const double4 *_
 __global__ __launch_bounds__(RS_THREADS) void morton_tiles(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
     int64_t n, const double4 *org, const double4 *__restrict__ parts, int np,
-    double4 *org_host, double s, double inv_s, int L, uint32_t *__restrict__ keys,
+    double4 *org_host, double s, double inv_sf, int L, int fine, uint32_t *__restrict__ keys,
     int32_t *__restrict__ vals, float4 *__restrict__ pay, uint32_t *__restrict__ hist,
     int nblocks, uint32_t mask)
 {
@@ -305,11 +306,11 @@ __global__ __launch_bounds__(RS_THREADS) void morton_tiles(
         const int64_t i = base + (int64_t)r * RS_THREADS + tid;
         if (i < n) {
             const double xi = x[i], yi = y[i];
-            const uint32_t key = morton_key(xi, yi, x0, y0, inv_s, L);
+            const uint32_t key = morton_key(xi, yi, x0, y0, inv_sf, L + fine);
             keys[i] = key;
             if (pay) {
                 double cx, cy;
-                leaf_center(key, x0, y0, s, cx, cy);
+                leaf_center(key >> (2 * fine), x0, y0, s, cx, cy);
                 pay[i] = make_float4((float)(xi - cx), (float)(yi - cy), (float)w[i],
                                      __int_as_float((int32_t)i));
             } else {
@@ -498,23 +499,24 @@ int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp,
 }
 
 // --------------------------------------------------------- leaf offsets --
-// start[k] = first sorted position with key >= k (k = 0..nleaf); with split
-// set, also the leaf ranges of the 8 XCDs holding equal numbers of points:
-// split[j] (j = 0..8) is the leaf of sorted point j ceil(n/8), nleaf when
-// there is none.
+// start[k] = first position of sorted keys with leaf (key >> fsh) >= k
+// (k = 0..nleaf); with split set, also the leaf ranges of the 8 XCDs
+// holding equal numbers of points: split[j] (j = 0..8) is the leaf of
+// sorted point j ceil(n/8), nleaf when there is none.
 __global__ void leaf_starts_kernel(const uint32_t *__restrict__ skeys, int64_t n, int64_t nleaf,
-                                   int32_t *__restrict__ start, int32_t *__restrict__ split)
+                                   int fsh, int32_t *__restrict__ start,
+                                   int32_t *__restrict__ split)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (split && k <= 8) {
         const int64_t T = n > 0 ? (n + 7) / 8 : 1;
-        split[k] = (k < 8 && k * T < n) ? (int32_t)skeys[k * T] : (int32_t)nleaf;
+        split[k] = (k < 8 && k * T < n) ? (int32_t)(skeys[k * T] >> fsh) : (int32_t)nleaf;
     }
     if (k > nleaf) return;
-    int64_t lo = 0, hi = n;  // first index with key >= k
+    int64_t lo = 0, hi = n;  // first index with leaf >= k
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)skeys[mid] < k)
+        if ((int64_t)(skeys[mid] >> fsh) < k)
             lo = mid + 1;
         else
             hi = mid;
@@ -523,10 +525,10 @@ __global__ void leaf_starts_kernel(const uint32_t *__restrict__ skeys, int64_t n
 }
 
 int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start,
-                       int32_t *split)
+                       int32_t *split, int fine)
 {
     leaf_starts_kernel<<<div_up(std::max<int64_t>(nkeys + 1, 9), 256), 256, 0, L->stream>>>(
-        skeys, n, nkeys, start, split);
+        skeys, n, nkeys, 2 * fine, start, split);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -550,7 +552,7 @@ static int build_sorted(Lane *L, const Domain &dom, const double *x, const doubl
     PHF_ALLOC(start, L->arena, int32_t, (size_t)nleaf + 1);
     out->split = nullptr;
     if (out->want_split) PHF_ALLOC(out->split, L->arena, int32_t, 9);
-    const int key_bits = 2 * dom.L;
+    const int key_bits = 2 * (dom.L + dom.fine);
     const bool sort = n > 1 && key_bits > 0;
     uint32_t *hist = nullptr, *offs = nullptr;
     if (sort) {
@@ -561,7 +563,8 @@ static int build_sorted(Lane *L, const Domain &dom, const double *x, const doubl
     const int nblocks = radix_tiles(n);
     if (n > 0)
         morton_tiles<<<nblocks, RS_THREADS, 0, L->stream>>>(
-            x, y, w, n, dom.org, dom.parts, dom.nparts, dom.org_host, dom.s, dom.inv_s, dom.L, k0,
+            x, y, w, n, dom.org, dom.parts, dom.nparts, dom.org_host, dom.s,
+            std::ldexp(dom.inv_s, dom.fine), dom.L, dom.fine, k0,
             PAY ? nullptr : (int32_t *)v0, PAY ? (float4 *)v0 : nullptr, hist, nblocks,
             (1u << std::min(8, key_bits)) - 1u);
     PHF_HIP(hipGetLastError());
@@ -575,7 +578,8 @@ static int build_sorted(Lane *L, const Domain &dom, const double *x, const doubl
     uint32_t *ks = k0;
     VT *vs = v0;
     if (sort) PHF_TRY(radix_sort_ws<VT>(L, k0, v0, k1, v1, n, key_bits, hist, offs, &ks, &vs, true));
-    PHF_TRY(leaf_starts_device(L, ks, n, nleaf, start, out->split));
+    PHF_TRY(leaf_starts_device(L, ks, n, nleaf, start, out->split, dom.fine));
+    out->fine = dom.fine;
     out->n = n;
     out->keys = ks;
     out->start = start;

@@ -12,6 +12,9 @@ struct Domain {
     double s;       // leaf box side D / 2^L (exact)
     double inv_s;   // 2^L / D
     int L;
+    // sort keys carry 2 * fine more bits below the leaf index, the Morton
+    // order of a 2^fine x 2^fine grid inside each leaf (leaf = key >> 2 fine)
+    int fine = 0;
     // the same corner on the device ({xmin, xmax, ymin, ymax} from
     // device_bbox): kernels read the origin there, so a captured launch
     // sequence stays valid when the points move within the same side
@@ -67,12 +70,13 @@ struct SortedPoints {
     // numbers of points (9 entries; if requested)
     int32_t *split = nullptr;
     bool want_split = false;
+    int fine = 0;  // keys hold 2 * fine bits below the leaf index
 };
 
-// start[k] = first position of sorted keys with key >= k, k = 0..nkeys;
-// split (nullable): the XCD leaf ranges as SortedPoints::split.
+// start[k] = first position of sorted keys with (key >> 2 fine) >= k,
+// k = 0..nkeys; split (nullable): the XCD leaf ranges as SortedPoints::split.
 int leaf_starts_device(Lane *L, const uint32_t *skeys, int64_t n, int64_t nkeys, int32_t *start,
-                       int32_t *split = nullptr);
+                       int32_t *split = nullptr, int fine = 0);
 
 int build_sorted_points(Lane *L, const Domain &dom, const double *x, const double *y, int64_t n,
                         SortedPoints *out);
