@@ -490,7 +490,7 @@ def main():
             "stage_ms": info["ms"], "device": dev["name"],
         },
         "roofline": {
-            "kernel": "p2p_near_f32s" if prec == gauss.PREC_F32C else "p2p_l2p_f64s",
+            "kernel": "p2p_near_f32s" if prec == gauss.PREC_F32C else "p2p_near_f64s",
             "bound": "mfma",
             "compute_pipe": "fp32 vector ALU (the pair kernel has no matrix shape); MI355X "
                             "fp32 peak 157.3 TFLOP/s, fp64 78.6",

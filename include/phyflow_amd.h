@@ -37,8 +37,8 @@ extern "C" {
 #define PHF_PREC_F32C 1 /* float32 pair arithmetic, compensated per-target sums */
 
 /* ---- timed stages (HIP events on the call's streams) -------------------- */
-/* The float32 FMM runs its far field (P2M..L2L) on a second stream beside
- * the near field, so those stages overlap P2P in time. */
+/* The FMM runs its far field (P2M..L2L) on a second stream beside the near
+ * field, so those stages overlap P2P in time. */
 enum {
     PHF_STAGE_UPLOAD = 0, /* host -> device copies */
     PHF_STAGE_TREE,       /* bbox, Morton keys, radix sort, leaf offsets, gathers */
@@ -46,7 +46,7 @@ enum {
     PHF_STAGE_M2M,
     PHF_STAGE_M2L,
     PHF_STAGE_L2L,
-    PHF_STAGE_P2P,        /* near-field (or all-pairs) sum; float64 FMM: L2P fused */
+    PHF_STAGE_P2P,        /* near-field (or all-pairs) sum */
     PHF_STAGE_DOWNLOAD,   /* device -> host copies */
     PHF_STAGE_REMESH,
     PHF_STAGE_PANEL,

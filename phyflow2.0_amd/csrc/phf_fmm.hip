@@ -603,60 +603,74 @@ __global__ void l2p_scatter_f32(const float4 *__restrict__ src, const float2 *__
 }
 
 
-// float64 near field + L2P: the float32 kernel's layout (p2p_leaf_f64):
-// one workgroup of 4 waves per target leaf, near sources staged in LDS,
-// XCD leaf ranges split by target count.  Absolute coordinates and
+// float64 near-field sums, one workgroup (4 waves) per target leaf over
+// the XCD leaf ranges split by target count, written in sorted target
+// order: nsum[i] = (a, b) of target i.  Absolute coordinates and
 // pair_f64's operation order.
 template <int KIND>
-__global__ __launch_bounds__(256) void p2p_l2p_f64s(
+__global__ __launch_bounds__(256) void p2p_near_f64s(
     const double4 *__restrict__ src, const int32_t *__restrict__ start_s,
-    const double2 *__restrict__ tgt, const int32_t *__restrict__ start_t,
-    const int32_t *__restrict__ perm_t, int Lvl, const int32_t *__restrict__ split,
-    const int2 *__restrict__ near, int nnear, PairParams pp, const double4 *__restrict__ org,
-    double s, const double2 *__restrict__ Bleaf, int p1, double *__restrict__ qr,
-    double *__restrict__ qi, int velocity_out)
+    const double2 *__restrict__ tgt, const int32_t *__restrict__ start_t, int Lvl,
+    const int32_t *__restrict__ split, const int2 *__restrict__ near, int nnear, PairParams pp,
+    double2 *__restrict__ nsum)
 {
     __shared__ double4 st[P2P_STAGE_F64];
     __shared__ double2 acc[P2P_CHUNKS * 4 * 16];
     const int32_t hi = split[(blockIdx.x & 7u) + 1];
     const int n = 1 << Lvl;
-    const double inv_s = 1.0 / s;
     for (int32_t bl = split[blockIdx.x & 7u] + (int32_t)(blockIdx.x >> 3); bl < hi;
          bl += (int32_t)(gridDim.x >> 3)) {
         const uint32_t b = (uint32_t)bl;
         const int32_t t0 = start_t[b], t1 = start_t[b + 1];
         if (t1 <= t0) continue;
         const NearList nf{near, nnear, (int)compact_bits(b), (int)compact_bits(b >> 1), n, 0.f};
+        auto tpos = [&](int32_t i) { return tgt[i]; };
+        auto out = [&](int32_t i, double a, double bb) { nsum[i] = make_double2(a, bb); };
+        p2p_leaf_f64<KIND>(src, start_s, st, acc, nnear, nf, t0, t1, tpos, pp, out);
+    }
+}
+
+// float64 L2P and the output in input order, one thread per sorted target
+// i of leaf keys[i] >> fsh: (qr, qi) = (-(a + far_r), b - far_i) with the
+// leaf's local expansion at (t - c)/s (Horner).
+__global__ void l2p_scatter_f64(const double2 *__restrict__ tgt,
+                                const uint32_t *__restrict__ keys, int fsh,
+                                const int32_t *__restrict__ perm, int64_t nt,
+                                const double4 *__restrict__ org, double s,
+                                const double2 *__restrict__ Bleaf, int p1,
+                                const double2 *__restrict__ nsum, double *__restrict__ qr,
+                                double *__restrict__ qi, int velocity_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const double2 tp = tgt[i];
+    double fr = 0.0, fi = 0.0;
+    if (Bleaf) {
+        const uint32_t b = keys[i] >> fsh;
         double cx, cy;
         leaf_center(b, org->x, org->z, s, cx, cy);
-        auto tpos = [&](int32_t i) { return tgt[i]; };
-        auto out = [&](int32_t i, double a, double bb) {
-            const double2 tp = tgt[i];
-            double fr = 0.0, fi = 0.0;
-            if (Bleaf) {
-                const double2 *Bv = Bleaf + (int64_t)b * p1;
-                const double zr = (tp.x - cx) * inv_s, zi = (tp.y - cy) * inv_s;
-                for (int k = p1 - 1; k >= 0; --k) {
-                    const double2 c = Bv[k];
-                    const double nr = fr * zr - fi * zi + c.x;
-                    const double ni = fr * zi + fi * zr + c.y;
-                    fr = nr;
-                    fi = ni;
-                }
-            }
-            const int32_t o = perm_t[i];
-            const double vr = -(a + fr), vi = bb - fi;
-            if (velocity_out) {
-                // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
-                const double two_pi = 2.0 * 3.141592653589793;
-                qr[o] = -vi / two_pi;
-                qi[o] = -vr / two_pi;
-            } else {
-                qr[o] = vr;
-                qi[o] = vi;
-            }
-        };
-        p2p_leaf_f64<KIND>(src, start_s, st, acc, nnear, nf, t0, t1, tpos, pp, out);
+        const double inv_s = 1.0 / s;
+        const double2 *Bv = Bleaf + (int64_t)b * p1;
+        const double zr = (tp.x - cx) * inv_s, zi = (tp.y - cy) * inv_s;
+        for (int k = p1 - 1; k >= 0; --k) {
+            const double2 c = Bv[k];
+            const double nr = fr * zr - fi * zi + c.x;
+            const double ni = fr * zi + fi * zr + c.y;
+            fr = nr;
+            fi = ni;
+        }
+    }
+    const double2 ns = nsum[i];
+    const int32_t o = perm[i];
+    const double vr = -(ns.x + fr), vi = ns.y - fi;
+    if (velocity_out) {
+        // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
+        const double two_pi = 2.0 * 3.141592653589793;
+        qr[o] = -vi / two_pi;
+        qi[o] = -vr / two_pi;
+    } else {
+        qr[o] = vr;
+        qi[o] = vi;
     }
 }
 
@@ -1055,14 +1069,13 @@ using P2PLaunch = std::function<int(hipStream_t)>;
 
 // How run_fmm_typed hands back its last launches.  With a split, it calls
 // mid() after the tree (capture: ends the first recording and begins the
-// second), enqueues the far field and returns the near-field launch
-// (float32: near sums in sorted order; float64: none) and the finishing
-// launch (float32: L2P and the output in input order; float64: the fused
-// near field + L2P) for the caller to issue after the far field.  Without
-// one it issues them itself: the float32 far field on the lane's second
-// stream beside the near field (fmm_far_fork / fmm_far_done / fmm_far_join),
-// so that the near field follows the tree on the lane's stream with no
-// cross-stream wait in between.
+// second), enqueues the far field and returns the near-field launch (sums
+// in sorted target order) and the finishing launch (L2P and the output in
+// input order) for the caller to issue around the far field.  Without one
+// it issues them itself: the far field on the lane's second stream beside
+// the near field (fmm_far_fork / fmm_far_done / fmm_far_join), so that the
+// near field follows the tree on the lane's stream with no cross-stream
+// wait in between.  An empty near launch runs everything in order.
 struct FmmSplit {
     std::function<int()> mid;
     P2PLaunch near, finish;
@@ -1232,10 +1245,9 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         PHF_HIP(hipGetLastError());
     }
 
-    // near field and finish: float32 sums the near field in sorted order
-    // (beside the far field) and then adds the leaf expansions and writes
-    // the outputs in input order; float64 does both in one kernel after
-    // the far field
+    // near field and finish: the near field is summed in sorted target
+    // order (beside the far field), then the leaf expansions are added and
+    // the outputs written in input order
     const int vout = a.velocity_out ? 1 : 0;
     const uint32_t nl8 = (nleaf + 7) / 8;
     const int32_t *split = Tg.split;  // XCD leaf ranges of the targets
@@ -1275,16 +1287,26 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
         const int32_t *ss = S.start;
         const double4 *org = dom.org;
         const double s = dom.s;
+        const uint32_t *tkeys = Tg.keys;
+        const int fsh = 2 * Tg.fine;
+        const int64_t nt = a.nt;
+        double2 *nsum;
+        PHF_ALLOC(nsum, L->arena, double2, (size_t)std::max<int64_t>(a.nt, 1));
+        near = [=](hipStream_t st) {
+            p2p_near_f64s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(s4, ss, t2, ts, Lv, split, d_near,
+                                                             nnear, pp, nsum);
+            PHF_HIP(hipGetLastError());
+            return PHF_OK;
+        };
         finish = [=](hipStream_t st) {
-            p2p_l2p_f64s<PAIR_OSEEN><<<8 * nl8, 256, 0, st>>>(s4, ss, t2, ts, perm_t, Lv, split,
-                                                            d_near, nnear, pp, org, s, Bl, p1,
-                                                            qr, qi, vout);
+            l2p_scatter_f64<<<div_up(nt, 256), 256, 0, st>>>(t2, tkeys, fsh, perm_t, nt, org, s,
+                                                             Bl, p1, nsum, qr, qi, vout);
             PHF_HIP(hipGetLastError());
             return PHF_OK;
         };
     }
     tm.end(PHF_STAGE_TREE);
-    // float32 without a split: the far field goes to the second stream
+    // without a split: the far field goes to the second stream
     const bool fork = !spl && near;
     int rc = PHF_OK;
     if (spl) {
@@ -1614,7 +1636,7 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
 }
 
 // Launch a recorded entry: tree graph, near field beside the far-field
-// graph (float32), then the finishing kernel.
+// graph, then the finishing kernel.
 static int fmm_replay(Lane *L, FmmGraph *e, StageTimer &tm, int stage0)
 {
     hipError_t le = hipGraphLaunch(e->exec[0], L->stream);

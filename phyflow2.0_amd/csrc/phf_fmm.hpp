@@ -38,9 +38,9 @@ int fmm_level(int64_t nb, int leaf, int ndirect);
 // With info->graph set (and no work counts) the launch sequence (bounding
 // box through L2L) is recorded as HIP graphs in the lane's cache and
 // replayed on later calls with the same arguments (the stage then timed
-// is PHF_STAGE_TREE_FAR); see FmmInfo::domain_ok.  The float32 near field
-// runs on the lane's second stream beside the far field; the lane's stream
-// waits for it before the call's last kernel.
+// is PHF_STAGE_TREE_FAR); see FmmInfo::domain_ok.  The near field runs on
+// the lane's stream while the far field runs on the lane's second stream;
+// the lane's stream waits for the far field before the call's last kernel.
 int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *info);
 // the lane's recorded graphs, second stream and its events
 void fmm_graphs_release(Lane *L);
