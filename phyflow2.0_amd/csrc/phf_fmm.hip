@@ -1488,7 +1488,6 @@ struct FmmGraph {
 struct FmmGraphCache {
     std::vector<FmmGraph *> v;
     uint64_t clock = 0;
-    hipStream_t side = nullptr;    // uploads made while recording
     double4 *hb_host = nullptr;    // pinned copy of the replayed bounding box
 };
 
@@ -1516,7 +1515,6 @@ void fmm_graphs_release(Lane *L)
     L->fork_ev = L->join_ev = nullptr;
     if (!L->fmm_graphs) return;
     for (FmmGraph *e : L->fmm_graphs->v) drop_graph(e);
-    if (L->fmm_graphs->side) (void)hipStreamDestroy(L->fmm_graphs->side);
     if (L->fmm_graphs->hb_host) (void)hipHostFree(L->fmm_graphs->hb_host);
     delete L->fmm_graphs;
     L->fmm_graphs = nullptr;
@@ -1536,7 +1534,9 @@ static bool same_args(const FmmArgs &x, const FmmArgs &y)
 static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **out, bool *keep)
 {
     FmmGraphCache &c = *L->fmm_graphs;
-    if (!c.side) PHF_HIP(hipStreamCreateWithFlags(&c.side, hipStreamNonBlocking));
+    // table uploads made while recording go through the lane's second
+    // stream, idle then (it carries the far field of launches only)
+    PHF_TRY(lane_stream2(L));
     FmmGraph *e = new FmmGraph;
     e->a = a;
     e->L = dom0.L;
@@ -1573,7 +1573,7 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
         e->bb_host = parts != nullptr;
         FmmSplit spl;
         spl.tables_keep = &e->tables;
-        spl.side = c.side;
+        spl.side = L->stream2;
         spl.mid = [&]() -> int {
             PHF_HIP(hipStreamEndCapture(L->stream, &e->g[0]));
             part = 1;
