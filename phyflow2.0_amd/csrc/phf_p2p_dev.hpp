@@ -150,49 +150,70 @@ __device__ __forceinline__ int p2p_stage_batch(const float4 *__restrict__ src,
     return stage_near_batch<P2P_STAGE>(src, sp, st, nnear, nf, xf, q, jq, core_end);
 }
 
-// Sum over the staged entries x in [a, b) with x = o (mod 16) for the
-// target (xt, yt): plain float32 partials over at most 16 pairs folded into
-// the TwoSum accumulators.
-template <int KIND, bool CORE>
+// Sum over the staged entries x in [a, b) with x = o (mod 16) for NT
+// targets (xt[u], yt[u]) at once (each staged record read once for all of
+// them): per target, plain float32 partials over at most 16 pairs folded
+// into the TwoSum accumulators (the same order for any NT).
+template <int KIND, bool CORE, int NT>
 __device__ __forceinline__ void p2p_sweep(const float4 *__restrict__ st, int a, int b, int o,
-                                          float xt, float yt, float c2s, float scale_f,
-                                          float &ha, float &la, float &hb, float &lb)
+                                          const float *xt, const float *yt, float c2s,
+                                          float scale_f, float *ha, float *la, float *hb,
+                                          float *lb)
 {
     int x = a + ((o - a) & 15);
     for (; x + 240 < b; x += 256) {
-        float aa = 0.f, ab = 0.f;
+        float aa[NT], ab[NT];
+#pragma unroll
+        for (int u = 0; u < NT; ++u) aa[u] = ab[u] = 0.f;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             const float4 q = st[x + 16 * k];
-            const float dx = xt - q.x, dy = yt - q.y;
-            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
-            aa = __builtin_fmaf(P, dx, aa);
-            ab = __builtin_fmaf(P, dy, ab);
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const float dx = xt[u] - q.x, dy = yt[u] - q.y;
+                const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
+                aa[u] = __builtin_fmaf(P, dx, aa[u]);
+                ab[u] = __builtin_fmaf(P, dy, ab[u]);
+            }
         }
-        two_sum_add(ha, la, aa);
-        two_sum_add(hb, lb, ab);
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            two_sum_add(ha[u], la[u], aa[u]);
+            two_sum_add(hb[u], lb[u], ab[u]);
+        }
     }
     if (x < b) {
-        float aa = 0.f, ab = 0.f;
+        float aa[NT], ab[NT];
+#pragma unroll
+        for (int u = 0; u < NT; ++u) aa[u] = ab[u] = 0.f;
         for (; x + 48 < b; x += 64) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const float4 q = st[x + 16 * k];
-                const float dx = xt - q.x, dy = yt - q.y;
-                const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
-                aa = __builtin_fmaf(P, dx, aa);
-                ab = __builtin_fmaf(P, dy, ab);
+#pragma unroll
+                for (int u = 0; u < NT; ++u) {
+                    const float dx = xt[u] - q.x, dy = yt[u] - q.y;
+                    const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
+                    aa[u] = __builtin_fmaf(P, dx, aa[u]);
+                    ab[u] = __builtin_fmaf(P, dy, ab[u]);
+                }
             }
         }
         for (; x < b; x += 16) {
             const float4 q = st[x];
-            const float dx = xt - q.x, dy = yt - q.y;
-            const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
-            aa = __builtin_fmaf(P, dx, aa);
-            ab = __builtin_fmaf(P, dy, ab);
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const float dx = xt[u] - q.x, dy = yt[u] - q.y;
+                const float P = pair_scaled<KIND, CORE>(dx, dy, q.z, c2s, scale_f);
+                aa[u] = __builtin_fmaf(P, dx, aa[u]);
+                ab[u] = __builtin_fmaf(P, dy, ab[u]);
+            }
         }
-        two_sum_add(ha, la, aa);
-        two_sum_add(hb, lb, ab);
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            two_sum_add(ha[u], la[u], aa[u]);
+            two_sum_add(hb[u], lb[u], ab[u]);
+        }
     }
 }
 
@@ -233,22 +254,37 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
             int core_end;
             const int used = p2p_stage_batch(src, sp, st, nnear, nf, lam, q, jq, core_end);
             __syncthreads();
-            for (int c = 0; c < nch; ++c) {
-                const int32_t i = rc + 16 * c + t;
-                float2 tp = i < t1 ? tgt(i) : make_float2(0.f, 0.f);
-                const float xt = tp.x * lam, yt = tp.y * lam;
-                float ha = 0.f, la = 0.f, hb = 0.f, lb = 0.f;
-                p2p_sweep<KIND, true>(st, 0, core_end, o, xt, yt, c2s, pp.scale_f, ha, la, hb,
-                                      lb);
-                p2p_sweep<KIND, false>(st, core_end, used, o, xt, yt, c2s, pp.scale_f, ha, la,
-                                       hb, lb);
-                p2p_merge_groups(ha, la, hb, lb);
-                if (g == 0) {
-                    float4 &A = acc[(c * 4 + wv) * 16 + t];
-                    float4 v = A;
-                    two_sum_merge(v.x, v.y, ha, la);
-                    two_sum_merge(v.z, v.w, hb, lb);
-                    A = v;
+            // chunks two at a time (each staged record read once for both)
+            for (int c = 0; c < nch; c += 2) {
+                float xt[2], yt[2], ha[2] = {}, la[2] = {}, hb[2] = {}, lb[2] = {};
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int32_t i = rc + 16 * (c + u) + t;
+                    const float2 tp = i < t1 ? tgt(i) : make_float2(0.f, 0.f);
+                    xt[u] = tp.x * lam;
+                    yt[u] = tp.y * lam;
+                }
+                const int nu = min(2, nch - c);
+                if (nu == 2) {
+                    p2p_sweep<KIND, true, 2>(st, 0, core_end, o, xt, yt, c2s, pp.scale_f, ha, la,
+                                             hb, lb);
+                    p2p_sweep<KIND, false, 2>(st, core_end, used, o, xt, yt, c2s, pp.scale_f, ha,
+                                              la, hb, lb);
+                } else {
+                    p2p_sweep<KIND, true, 1>(st, 0, core_end, o, xt, yt, c2s, pp.scale_f, ha, la,
+                                             hb, lb);
+                    p2p_sweep<KIND, false, 1>(st, core_end, used, o, xt, yt, c2s, pp.scale_f, ha,
+                                              la, hb, lb);
+                }
+                for (int u = 0; u < nu; ++u) {
+                    p2p_merge_groups(ha[u], la[u], hb[u], lb[u]);
+                    if (g == 0) {
+                        float4 &A = acc[((c + u) * 4 + wv) * 16 + t];
+                        float4 v = A;
+                        two_sum_merge(v.x, v.y, ha[u], la[u]);
+                        two_sum_merge(v.z, v.w, hb[u], lb[u]);
+                        A = v;
+                    }
                 }
             }
             __syncthreads();
