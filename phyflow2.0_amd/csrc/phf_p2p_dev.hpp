@@ -307,30 +307,36 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
 // ---------------------------------------------------------------- float64
 constexpr int P2P_STAGE_F64 = 768;  // staged sources (double4) per batch
 
-// float64 pair sweep over staged entries x in [a, b) with x = o (mod 16),
-// absolute coordinates, pair_f64's operation order; the smoothing factor
-// of the Oseen kernel is evaluated only when some lane of the wave has a
-// pair inside the cutoff (the others take 1 exactly, as pair_f64 does).
-template <int KIND>
+// float64 pair sweep over staged entries x in [a, b) with x = o (mod 16)
+// for NT targets at once (each staged record read once for all of them),
+// absolute coordinates, pair_f64's operation order; per target, the
+// smoothing factor of the Oseen kernel is evaluated only when some lane of
+// the wave has a pair inside the cutoff (the others take 1 exactly, as
+// pair_f64 does).
+template <int KIND, int NT>
 __device__ __forceinline__ void p2p_sweep_f64(const double4 *__restrict__ st, int a, int b, int o,
-                                              double xt, double yt, const PairParams &pp,
-                                              double &ax, double &ay)
+                                              const double *xt, const double *yt,
+                                              const PairParams &pp, double *ax, double *ay)
 {
     for (int x = a + ((o - a) & 15); x < b; x += 16) {
         const double4 q = st[x];
-        const double dx = xt - q.x, dy = yt - q.y;
-        double P;
-        if (KIND == PAIR_OSEEN || KIND == PAIR_OSEEN_NOCUT) {
-            const double r2 = __builtin_fma(dx, dx, dy * dy);
-            double om = 1.0;
-            if (__ballot(r2 < pp.c2)) om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;
-            const double Pv = (q.z * rcp_f64(r2)) * om;
-            P = r2 > 0.0 ? Pv : 0.0;
-        } else {
-            P = pair_f64<KIND>(dx, dy, q.z, pp);
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            const double dx = xt[u] - q.x, dy = yt[u] - q.y;
+            double P;
+            if (KIND == PAIR_OSEEN || KIND == PAIR_OSEEN_NOCUT) {
+                const double r2 = __builtin_fma(dx, dx, dy * dy);
+                double om = 1.0;
+                if (__ballot(r2 < pp.c2))
+                    om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;
+                const double Pv = (q.z * rcp_f64(r2)) * om;
+                P = r2 > 0.0 ? Pv : 0.0;
+            } else {
+                P = pair_f64<KIND>(dx, dy, q.z, pp);
+            }
+            ax[u] = __builtin_fma(P, dx, ax[u]);
+            ay[u] = __builtin_fma(P, dy, ay[u]);
         }
-        ax = __builtin_fma(P, dx, ax);
-        ay = __builtin_fma(P, dy, ay);
     }
 }
 
@@ -357,18 +363,30 @@ __device__ __forceinline__ void p2p_leaf_f64(const double4 *__restrict__ src,
             const int used = stage_near_batch<P2P_STAGE_F64>(src, sp, st, nnear, nf, xf, q, jq,
                                                              core_end);
             __syncthreads();
-            for (int c = 0; c < nch; ++c) {
-                const int32_t i = rc + 16 * c + t;
-                const double2 tp = i < t1 ? tgt(i) : make_double2(0.0, 0.0);
-                double ax = 0.0, ay = 0.0;
-                p2p_sweep_f64<KIND>(st, 0, used, o, tp.x, tp.y, pp, ax, ay);
-                ax += __shfl_xor(ax, 16);
-                ay += __shfl_xor(ay, 16);
-                ax += __shfl_xor(ax, 32);
-                ay += __shfl_xor(ay, 32);
-                if (g == 0) {
-                    double2 &A = acc[(c * 4 + wv) * 16 + t];
-                    A = make_double2(A.x + ax, A.y + ay);
+            // chunks two at a time (each staged record read once for both)
+            for (int c = 0; c < nch; c += 2) {
+                double xt[2], yt[2], ax[2] = {}, ay[2] = {};
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int32_t i = rc + 16 * (c + u) + t;
+                    const double2 tp = i < t1 ? tgt(i) : make_double2(0.0, 0.0);
+                    xt[u] = tp.x;
+                    yt[u] = tp.y;
+                }
+                const int nu = min(2, nch - c);
+                if (nu == 2)
+                    p2p_sweep_f64<KIND, 2>(st, 0, used, o, xt, yt, pp, ax, ay);
+                else
+                    p2p_sweep_f64<KIND, 1>(st, 0, used, o, xt, yt, pp, ax, ay);
+                for (int u = 0; u < nu; ++u) {
+                    ax[u] += __shfl_xor(ax[u], 16);
+                    ay[u] += __shfl_xor(ay[u], 16);
+                    ax[u] += __shfl_xor(ax[u], 32);
+                    ay[u] += __shfl_xor(ay[u], 32);
+                    if (g == 0) {
+                        double2 &A = acc[((c + u) * 4 + wv) * 16 + t];
+                        A = make_double2(A.x + ax[u], A.y + ay[u]);
+                    }
                 }
             }
             __syncthreads();
