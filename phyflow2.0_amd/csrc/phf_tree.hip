@@ -373,20 +373,26 @@ __global__ __launch_bounds__(RS_SCAN) void rs_rowscan(const uint32_t *__restrict
 // exclusive scan of tot) plus offs[d][tile].  Per round, each wave ranks
 // its keys among the lanes with the same digit (nbits ballots) and counts
 // them per digit; one thread per digit turns the 16 waves' counts into
-// positions, so equal keys keep their input order.
+// positions in the tile's own sorted order (its digit starts from
+// hist[d][tile]), so equal keys keep their input order.  The tile is
+// sorted in LDS first and then written out in that order, consecutive
+// threads to consecutive places of each digit's run.
 template <class VT>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     const uint32_t *__restrict__ kin, const VT *__restrict__ vin, uint32_t *__restrict__ kout,
-    VT *__restrict__ vout, int64_t n, int shift, int nbits, const uint32_t *__restrict__ offs,
-    const uint32_t *__restrict__ tot, int nblocks)
+    VT *__restrict__ vout, int64_t n, int shift, int nbits, const uint32_t *__restrict__ hist,
+    const uint32_t *__restrict__ offs, const uint32_t *__restrict__ tot, int nblocks)
 {
-    __shared__ uint32_t run[256];
+    __shared__ uint32_t lrun[256], lstart[256], gbase[256];
     __shared__ uint32_t cnt[RS_WAVES][256];
     __shared__ uint32_t pre[RS_WAVES][256];
     __shared__ uint32_t shw[RS_WAVES];
+    __shared__ uint32_t sk[RS_TILE];
+    __shared__ VT sv[RS_TILE];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t mask = (1u << nbits) - 1u;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int tile_n = (int)min((int64_t)RS_TILE, n - base);
     uint32_t kr[RS_ROUNDS];
     VT vr[RS_ROUNDS];
 #pragma unroll
@@ -396,8 +402,13 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
         vr[r] = i < n ? vin[i] : VT{};
     }
     const bool digit = tid < (1 << nbits);  // tid < 256
+    const int64_t hrow = (int64_t)tid * nblocks + blockIdx.x;
     const uint32_t dbase = block_exclusive_scan<RS_THREADS>(digit ? tot[tid] : 0u, shw, nullptr);
-    if (tid < 256) run[tid] = dbase + (digit ? offs[(int64_t)tid * nblocks + blockIdx.x] : 0u);
+    const uint32_t lbase = block_exclusive_scan<RS_THREADS>(digit ? hist[hrow] : 0u, shw, nullptr);
+    if (tid < 256) {
+        gbase[tid] = dbase + (digit ? offs[hrow] : 0u);
+        lstart[tid] = lrun[tid] = lbase;
+    }
 #pragma unroll
     for (int k = 0; k < RS_WAVES * 256 / RS_THREADS; ++k) (&cnt[0][0])[tid + k * RS_THREADS] = 0;
     __syncthreads();
@@ -418,25 +429,33 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
         if (valid && rank == 0) cnt[wave][d] = (uint32_t)__popcll(m);
         __syncthreads();
         if (tid < 256) {
-            // positions of each wave's first key of digit tid, in wave order
+            // tile positions of each wave's first key of digit tid, in wave order
             uint32_t c[RS_WAVES];
 #pragma unroll
             for (int w = 0; w < RS_WAVES; ++w) c[w] = cnt[w][tid];
-            uint32_t s = run[tid];
+            uint32_t s = lrun[tid];
 #pragma unroll
             for (int w = 0; w < RS_WAVES; ++w) {
                 pre[w][tid] = s;
                 cnt[w][tid] = 0;
                 s += c[w];
             }
-            run[tid] = s;
+            lrun[tid] = s;
         }
         __syncthreads();
         if (valid) {
-            const uint32_t pos = pre[wave][d] + rank;
-            kout[pos] = key;
-            vout[pos] = vr[r];
+            const uint32_t lp = pre[wave][d] + rank;
+            sk[lp] = key;
+            sv[lp] = vr[r];
         }
+    }
+    __syncthreads();
+    for (int j = tid; j < tile_n; j += RS_THREADS) {
+        const uint32_t key = sk[j];
+        const uint32_t d = (key >> shift) & mask;
+        const uint32_t pos = gbase[d] + ((uint32_t)j - lstart[d]);
+        kout[pos] = key;
+        vout[pos] = sv[j];
     }
 }
 
@@ -481,7 +500,7 @@ static int radix_sort_ws(Lane *L, uint32_t *keys, VT *vals, uint32_t *ktmp, VT *
             rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
         rs_rowscan<<<1u << nbits, RS_SCAN, 0, L->stream>>>(hist, nblocks, offs, tot);
         rs_scatter<VT><<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits,
-                                                              offs, tot, nblocks);
+                                                              hist, offs, tot, nblocks);
         PHF_HIP(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);
