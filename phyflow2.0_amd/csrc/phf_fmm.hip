@@ -562,32 +562,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
                                  pp, s_f, nsum, st, acc);
 }
 
-// L2P and the output in input order, one thread per sorted target i of
-// leaf keys[i] >> fsh: (qr, qi) = (-(near_a + far_r), near_b - far_i), far the
-// leaf's local expansion at the target (float32 Horner); targets are the
-// source records (tgt == nullptr: index bits in .w) or tgt with perm.
-__global__ void l2p_scatter_f32(const float4 *__restrict__ src, const float2 *__restrict__ tgt,
-                                const uint32_t *__restrict__ keys, int fsh,
-                                const int32_t *__restrict__ perm, int64_t nt,
-                                const float2 *__restrict__ Bleaf, int p1, float inv_s,
+// L2P on the far side (the far field's last kernel, beside the near
+// field): ffar[i] = the local expansion of leaf keys[i] >> fsh at sorted
+// target i (float32 Horner); targets are the source records (tgt null) or
+// tgt.
+__global__ void l2p_far_f32(const float4 *__restrict__ src, const float2 *__restrict__ tgt,
+                            const uint32_t *__restrict__ keys, int fsh, int64_t nt,
+                            const float2 *__restrict__ Bleaf, int p1, float inv_s,
+                            float2 *__restrict__ ffar)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    float2 tp;
+    if (tgt) {
+        tp = tgt[i];
+    } else {
+        const float4 q = src[i];
+        tp = make_float2(q.x, q.y);
+    }
+    double fr, fi;
+    l2p_f32(Bleaf + (int64_t)(keys[i] >> fsh) * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
+    ffar[i] = make_float2((float)fr, (float)fi);
+}
+
+// The output in input order, one thread per sorted target i: (qr, qi) =
+// (-(near_a + far_r), near_b - far_i) with the far part ffar from
+// l2p_far_f32 (none when ffar is null); the input index is perm[i]
+// (tgt given) or the source record's index bits.
+__global__ void out_scatter_f32(const float4 *__restrict__ src, const int32_t *__restrict__ perm,
+                                int64_t nt, const float2 *__restrict__ ffar,
                                 const double2 *__restrict__ nsum, double *__restrict__ out_a,
                                 double *__restrict__ out_b, int velocity_out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
-    float2 tp;
-    int32_t o;
-    if (tgt) {
-        tp = tgt[i];
-        o = perm[i];
-    } else {
-        const float4 q = src[i];
-        tp = make_float2(q.x, q.y);
-        o = __float_as_int(q.w);
-    }
+    const int32_t o = perm ? perm[i] : __float_as_int(src[i].w);
     double fr = 0.0, fi = 0.0;
-    if (Bleaf)
-        l2p_f32(Bleaf + (int64_t)(keys[i] >> fsh) * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
+    if (ffar) {
+        const float2 f = ffar[i];
+        fr = f.x;
+        fi = f.y;
+    }
     const double2 ns = nsum[i];
     const double qr = -(ns.x + fr);
     const double qi = ns.y - fi;
@@ -601,7 +616,6 @@ __global__ void l2p_scatter_f32(const float4 *__restrict__ src, const float2 *__
         out_b[o] = qi;
     }
 }
-
 
 // float64 near-field sums, one workgroup (4 waves) per target leaf over
 // the XCD leaf ranges split by target count, written in sorted target
@@ -1253,7 +1267,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
     const int32_t *split = Tg.split;  // XCD leaf ranges of the targets
     const int32_t *ts = Tg.start;
     double *qr = a.qr, *qi = a.qi;
-    P2PLaunch near, finish;
+    P2PLaunch near, finish, far_tail;
     if (sizeof(T) == 4) {
         const float4 *s4 = (const float4 *)src;
         const float2 *t2 = (const float2 *)tgt;
@@ -1274,9 +1288,21 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
             PHF_HIP(hipGetLastError());
             return PHF_OK;
         };
+        // the far side evaluates the leaf expansions at the targets (after
+        // L2L, beside the near field); the finish only adds and scatters
+        float2 *ffar = nullptr;
+        if (Bl) {
+            PHF_ALLOC(ffar, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
+            far_tail = [=](hipStream_t st) {
+                l2p_far_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2, tkeys, fsh, nt, Bl, p1,
+                                                             inv_s, ffar);
+                PHF_HIP(hipGetLastError());
+                return PHF_OK;
+            };
+        }
         finish = [=](hipStream_t st) {
-            l2p_scatter_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2, tkeys, fsh, perm_t, nt, Bl,
-                                                             p1, inv_s, nsum, qr, qi, vout);
+            out_scatter_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2 ? perm_t : nullptr, nt, ffar,
+                                                             nsum, qr, qi, vout);
             PHF_HIP(hipGetLastError());
             return PHF_OK;
         };
@@ -1386,6 +1412,7 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
             for (int l = 0; l <= Lv; ++l) dl.off[l] = off[l];
             PHF_TRY((shift_down_leaf<T, TC>(L, p1, dl, B, d_pas, Tg.start)));
         }
+        if (far_tail) PHF_TRY(far_tail(L->stream));
         tm.end(PHF_STAGE_L2L);
         return PHF_OK;
     };
