@@ -563,46 +563,69 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void p
 }
 
 // L2P on the far side (the far field's last kernel, beside the near
-// field): ffar[i] = the local expansion of leaf keys[i] >> fsh at sorted
-// target i (float32 Horner); targets are the source records (tgt null) or
-// tgt.
+// field), one thread per sorted target i with input index o: ffar[o] = the
+// local expansion of leaf keys[i] >> fsh at the target (float32 Horner)
+// and inv[o] = i, both in input order; targets are the source records (tgt
+// null: index bits in .w) or tgt with perm.
 __global__ void l2p_far_f32(const float4 *__restrict__ src, const float2 *__restrict__ tgt,
-                            const uint32_t *__restrict__ keys, int fsh, int64_t nt,
-                            const float2 *__restrict__ Bleaf, int p1, float inv_s,
-                            float2 *__restrict__ ffar)
+                            const int32_t *__restrict__ perm, const uint32_t *__restrict__ keys,
+                            int fsh, int64_t nt, const float2 *__restrict__ Bleaf, int p1,
+                            float inv_s, float2 *__restrict__ ffar, int32_t *__restrict__ inv)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     float2 tp;
+    int32_t o;
     if (tgt) {
         tp = tgt[i];
+        o = perm[i];
     } else {
         const float4 q = src[i];
         tp = make_float2(q.x, q.y);
+        o = __float_as_int(q.w);
     }
     double fr, fi;
     l2p_f32(Bleaf + (int64_t)(keys[i] >> fsh) * p1, p1, tp.x * inv_s, tp.y * inv_s, fr, fi);
-    ffar[i] = make_float2((float)fr, (float)fi);
+    ffar[o] = make_float2((float)fr, (float)fi);
+    inv[o] = (int32_t)i;
 }
 
-// The output in input order, one thread per sorted target i: (qr, qi) =
-// (-(near_a + far_r), near_b - far_i) with the far part ffar from
-// l2p_far_f32 (none when ffar is null); the input index is perm[i]
-// (tgt given) or the source record's index bits.
+// The outputs in input order with the far part: one thread per input index
+// o (coalesced writes), its near sums read from sorted position inv[o].
+__global__ void out_gather_f32(const int32_t *__restrict__ inv, int64_t nt,
+                               const float2 *__restrict__ ffar,
+                               const double2 *__restrict__ nsum, double *__restrict__ out_a,
+                               double *__restrict__ out_b, int velocity_out)
+{
+    const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nt) return;
+    const double2 ns = nsum[inv[o]];
+    const float2 f = ffar[o];
+    const double qr = -(ns.x + (double)f.x);
+    const double qi = ns.y - (double)f.y;
+    if (velocity_out) {
+        // (vx, vy) = (-qi/(2 pi), -qr/(2 pi)) (_induced.py:334-336)
+        const double two_pi = 2.0 * 3.141592653589793;
+        out_a[o] = -qi / two_pi;
+        out_b[o] = -qr / two_pi;
+    } else {
+        out_a[o] = qr;
+        out_b[o] = qi;
+    }
+}
+
+// The outputs without a far field, one thread per sorted target i: (qr,
+// qi) = (-near_a, near_b) to input index perm[i] (perm given) or the source
+// record's index bits.
 __global__ void out_scatter_f32(const float4 *__restrict__ src, const int32_t *__restrict__ perm,
-                                int64_t nt, const float2 *__restrict__ ffar,
-                                const double2 *__restrict__ nsum, double *__restrict__ out_a,
-                                double *__restrict__ out_b, int velocity_out)
+                                int64_t nt, const double2 *__restrict__ nsum,
+                                double *__restrict__ out_a, double *__restrict__ out_b,
+                                int velocity_out)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nt) return;
     const int32_t o = perm ? perm[i] : __float_as_int(src[i].w);
-    double fr = 0.0, fi = 0.0;
-    if (ffar) {
-        const float2 f = ffar[i];
-        fr = f.x;
-        fi = f.y;
-    }
+    const double fr = 0.0, fi = 0.0;
     const double2 ns = nsum[i];
     const double qr = -(ns.x + fr);
     const double qi = ns.y - fi;
@@ -1289,23 +1312,35 @@ static int run_fmm_typed(Lane *L, const FmmArgs &a, const Domain &dom, const Lis
             return PHF_OK;
         };
         // the far side evaluates the leaf expansions at the targets (after
-        // L2L, beside the near field); the finish only adds and scatters
-        float2 *ffar = nullptr;
+        // L2L, beside the near field) into input order, with the inverse
+        // permutation; the finish then reads the near sums through it and
+        // writes the outputs in input order (no scattered writes after the
+        // near field)
         if (Bl) {
+            float2 *ffar;
+            int32_t *inv;
             PHF_ALLOC(ffar, L->arena, float2, (size_t)std::max<int64_t>(a.nt, 1));
+            PHF_ALLOC(inv, L->arena, int32_t, (size_t)std::max<int64_t>(a.nt, 1));
             far_tail = [=](hipStream_t st) {
-                l2p_far_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2, tkeys, fsh, nt, Bl, p1,
-                                                             inv_s, ffar);
+                l2p_far_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2, perm_t, tkeys, fsh, nt, Bl,
+                                                             p1, inv_s, ffar, inv);
+                PHF_HIP(hipGetLastError());
+                return PHF_OK;
+            };
+            finish = [=](hipStream_t st) {
+                out_gather_f32<<<div_up(nt, 256), 256, 0, st>>>(inv, nt, ffar, nsum, qr, qi,
+                                                                vout);
+                PHF_HIP(hipGetLastError());
+                return PHF_OK;
+            };
+        } else {
+            finish = [=](hipStream_t st) {
+                out_scatter_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2 ? perm_t : nullptr, nt,
+                                                                 nsum, qr, qi, vout);
                 PHF_HIP(hipGetLastError());
                 return PHF_OK;
             };
         }
-        finish = [=](hipStream_t st) {
-            out_scatter_f32<<<div_up(nt, 256), 256, 0, st>>>(s4, t2 ? perm_t : nullptr, nt, ffar,
-                                                             nsum, qr, qi, vout);
-            PHF_HIP(hipGetLastError());
-            return PHF_OK;
-        };
     } else {
         const double4 *s4 = (const double4 *)src;
         const double2 *t2 = (const double2 *)tgt;
