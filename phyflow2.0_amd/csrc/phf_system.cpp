@@ -7,15 +7,19 @@
 
 namespace phf {
 
+// Capacity for n blobs (contents not kept).  A system that outgrows its
+// buffers (a remesh that adds blobs) gets half again as many, so that a
+// slowly growing simulation does not reallocate every step.
 int sys_reserve(System *s, int64_t n)
 {
     if (n <= s->cap) return PHF_OK;
     double *bufs[5] = {s->x, s->y, s->w, s->vx, s->vy};
+    const bool grow = s->cap > 0;
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     s->x = s->y = s->w = s->vx = s->vy = nullptr;
+    const int64_t c = std::max<int64_t>(grow ? std::max(n, s->cap + s->cap / 2) : n, 1);
     s->cap = 0;
-    int64_t c = std::max<int64_t>(n, 1);
     PHF_HIP(hipMalloc(&s->x, sizeof(double) * c));
     PHF_HIP(hipMalloc(&s->y, sizeof(double) * c));
     PHF_HIP(hipMalloc(&s->w, sizeof(double) * c));
