@@ -41,27 +41,6 @@ struct Panels {
 };
 
 // ------------------------------------------------------------- kernels ----
-__global__ void concat2(const double *__restrict__ a, int64_t na, const double *__restrict__ b,
-                        int64_t nb, double *__restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < na)
-        out[i] = a[i];
-    else if (i < na + nb)
-        out[i] = b[i - na];
-}
-
-// a followed by nb zeros
-__global__ void concat_zeros(const double *__restrict__ a, int64_t na, int64_t nb,
-                             double *__restrict__ out)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < na)
-        out[i] = a[i];
-    else if (i < na + nb)
-        out[i] = 0.0;
-}
-
 // RHS of the no-slip condition, -(v . t) (__panelsClass.py:771); for the
 // 'prescribed' problem the last entry is -gTotal.
 __global__ void slip_rhs_kernel(const double *__restrict__ vx, const double *__restrict__ vy,
@@ -83,6 +62,38 @@ __global__ void total_velocity(double *__restrict__ kx, double *__restrict__ ky,
     if (i >= n) return;
     kx[i] = (kx[i] + (px ? px[i] : 0.0)) + vinf_x;
     ky[i] = (ky[i] + (py ? py[i] : 0.0)) + vinf_y;
+}
+
+// (kx, ky) = (vx + px + vinf_x, vy + py + vinf_y) for the first n entries
+__global__ void total_velocity_to(const double *__restrict__ vx, const double *__restrict__ vy,
+                                  const double *__restrict__ px, const double *__restrict__ py,
+                                  int64_t n, double vinf_x, double vinf_y,
+                                  double *__restrict__ kx, double *__restrict__ ky)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    kx[i] = (vx[i] + px[i]) + vinf_x;
+    ky[i] = (vy[i] + py[i]) + vinf_y;
+}
+
+// FMM points of a coupled stage: the n blobs (x, y, w), then the m
+// collocation points with zero circulation
+__global__ void concat_points(const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ w, int64_t n,
+                              const double *__restrict__ xc, const double *__restrict__ yc,
+                              int64_t m, double *__restrict__ tx, double *__restrict__ ty,
+                              double *__restrict__ tw)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        tx[i] = x[i];
+        ty[i] = y[i];
+        tw[i] = w[i];
+    } else if (i < n + m) {
+        tx[i] = xc[i - n];
+        ty[i] = yc[i - n];
+        tw[i] = 0.0;
+    }
 }
 
 __global__ void euler_stage(const double *__restrict__ x, const double *__restrict__ y,
@@ -149,9 +160,8 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
     PHF_ALLOC(vy, L->arena, double, (size_t)nt);
     PHF_ALLOC(px, L->arena, double, (size_t)std::max<int64_t>(n, 1));
     PHF_ALLOC(py, L->arena, double, (size_t)std::max<int64_t>(n, 1));
-    concat2<<<div_up(nt, 256), 256, 0, L->stream>>>(X, n, P->xcp, m, tx);
-    concat2<<<div_up(nt, 256), 256, 0, L->stream>>>(Y, n, P->ycp, m, ty);
-    concat_zeros<<<div_up(nt, 256), 256, 0, L->stream>>>(w, n, m, tw);
+    concat_points<<<div_up(nt, 256), 256, 0, L->stream>>>(X, Y, w, n, P->xcp, P->ycp, m, tx, ty,
+                                                          tw);
     PHF_HIP(hipGetLastError());
     // blobs -> blobs and blobs -> collocation points in one evaluation: the
     // collocation points join the sources with zero circulation (exact
@@ -165,10 +175,9 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
     }
     PHF_TRY(panel_induced_device(L, m, P->s, n, X, Y, P->xs, P->ys, P->xe, P->ye, P->cosA,
                                  P->sinA, px, py, vp.prec));
-    total_velocity<<<div_up(n, 256), 256, 0, L->stream>>>(vx, vy, px, py, n, vinf_x, vinf_y);
+    total_velocity_to<<<div_up(n, 256), 256, 0, L->stream>>>(vx, vy, px, py, n, vinf_x, vinf_y,
+                                                             kx, ky);
     PHF_HIP(hipGetLastError());
-    PHF_HIP(hipMemcpyAsync(kx, vx, sizeof(double) * n, hipMemcpyDeviceToDevice, L->stream));
-    PHF_HIP(hipMemcpyAsync(ky, vy, sizeof(double) * n, hipMemcpyDeviceToDevice, L->stream));
     return PHF_OK;
 }
 
