@@ -418,10 +418,12 @@ __device__ __forceinline__ double wave_sum_f64(double v)
     return v;  // fixed butterfly: every lane holds the same sum
 }
 
-__global__ void refine_init(double *__restrict__ x, int64_t n, RefineState *__restrict__ st)
+// warm: x holds a starting guess (kept); otherwise x = 0
+__global__ void refine_init(double *__restrict__ x, int64_t n, int warm,
+                            RefineState *__restrict__ st)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = 0.0;
+    if (i < n && !warm) x[i] = 0.0;
     if (i == 0) {
         st->last_dx = INFINITY;
         st->done = 0;
@@ -586,7 +588,7 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
 }
 
 int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double *x,
-                        SolveStats *st)
+                        SolveStats *st, bool warm, int first_burst)
 {
     const int64_t rows = f.rows, n = f.n;
     const bool normal = rows > n;
@@ -596,10 +598,12 @@ int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double 
     PHF_ALLOC(g, L->arena, double, (size_t)n);
     PHF_ALLOC(d, L->arena, double, (size_t)n);
     PHF_ALLOC(dst, L->arena, RefineState, 1);
-    refine_init<<<div_up(std::max<int64_t>(n, 1), 256), 256, 0, L->stream>>>(x, n, dst);
-    const int max_it = 60, burst = 6;
+    refine_init<<<div_up(std::max<int64_t>(n, 1), 256), 256, 0, L->stream>>>(x, n, warm ? 1 : 0,
+                                                                          dst);
+    const int max_it = 60;
     RefineState hs{};
-    for (int launched = 0; launched < max_it; launched += burst) {
+    for (int launched = 0, burst = std::max(1, first_burst); launched < max_it;
+         launched += burst, burst = 6) {
         for (int b = 0; b < burst && launched + b < max_it; ++b) {
             resid_rows<<<div_up(rows, 4), 256, 0, L->stream>>>(f.Br, rows, n, x, c, r, dst);
             const double *rhs = r;

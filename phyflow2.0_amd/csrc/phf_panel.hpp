@@ -29,8 +29,10 @@ struct DenseFactors {
 };
 
 int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f);
+// Refinement sweeps from x = 0, or from the x given (warm); sweeps go in
+// bursts between host checks, the first of first_burst sweeps.
 int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double *x,
-                        SolveStats *st);
+                        SolveStats *st, bool warm = false, int first_burst = 6);
 // Solve B x = c (rows == n) or min ||B x - c||_2 (rows > n) for the
 // row-major float64 B on the device: float32 LU + float64 refinement.
 int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const double *c,

@@ -38,6 +38,7 @@ struct Panels {
     double *s = nullptr;  // last strengths
     std::vector<double> hxcp, hycp;
     int last_iterations = 0;
+    bool have_solution = false;  // s holds a converged solution
 };
 
 // ------------------------------------------------------------- kernels ----
@@ -128,9 +129,16 @@ static int panels_solve_device(Lane *L, Panels *P, const double *vx_cp, const do
     slip_rhs_kernel<<<div_up(rows, 256), 256, 0, L->stream>>>(
         vx_cp, vy_cp, P->m, vinf_x, vinf_y, P->cosA, P->sinA, P->problem, P->gTotal, rhs);
     PHF_HIP(hipGetLastError());
+    // start from the last strengths (the slip condition changes little from
+    // one stage to the next) with a first burst of one sweep more than the
+    // last solve took
     SolveStats st;
-    PHF_TRY(dense_refine_device(L, P->f, rhs, P->s, &st));
+    const bool warm = P->have_solution;
+    P->have_solution = false;
+    PHF_TRY(dense_refine_device(L, P->f, rhs, P->s, &st, warm,
+                                warm ? std::min(6, P->last_iterations + 1) : 6));
     P->last_iterations = st.iterations;
+    P->have_solution = st.converged;
     if (!st.converged) {
         set_error("panel solve did not converge (%d refinement sweeps)", st.iterations);
         return PHF_ERR_INVALID;
