@@ -177,7 +177,8 @@ __global__ void brm_touched(const int64_t *__restrict__ cb, int S, const BatchRm
     const long long nj = d.bj + 3;
     uint32_t t = 0;
     if (d.ok && loc < (d.bi + 3) * nj) {
-        const long long i = loc / nj, j = loc - (loc / nj) * nj;
+        long long i, j;
+        divmod_cells(loc, nj, i, j);
         for (int q = 0; q < 16 && !t; ++q) {
             const long long ii = i - (q & 3), jj = j - (q >> 2);
             if (ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
@@ -202,7 +203,8 @@ __global__ void brm_gather_sum(const int64_t *__restrict__ cb, int S,
     const BatchRmDyn d = dyn[s];
     const int64_t loc = cidx - r.cbase;
     const long long nj = d.bj + 3;
-    const long long i = loc / nj, j = loc - (loc / nj) * nj;
+    long long i, j;
+    divmod_cells(loc, nj, i, j);
     const long long I = i + d.imin, J = j + d.jmin;  // absolute cell indices
     const double ui = ((double)I + 0.5) + r.x0h;
     const double uj = ((double)J + 0.5) + r.y0h;

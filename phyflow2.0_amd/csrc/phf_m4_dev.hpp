@@ -67,4 +67,18 @@ __host__ __device__ __forceinline__ double tutty_weight(int a, double delta, dou
     }
 }
 
+// (q, r) = (a / b, a % b) for 0 <= a, 0 < b, in 32-bit arithmetic when
+// both fit (the per-system cell grids do), else 64-bit
+__device__ __forceinline__ void divmod_cells(long long a, long long b, long long &q, long long &r)
+{
+    if (((unsigned long long)a | (unsigned long long)b) < (1ull << 32)) {
+        const uint32_t q32 = (uint32_t)a / (uint32_t)b;
+        q = q32;
+        r = (long long)((uint32_t)a - q32 * (uint32_t)b);
+    } else {
+        q = a / b;
+        r = a - q * b;
+    }
+}
+
 }  // namespace phf

@@ -131,7 +131,8 @@ __global__ void touched_flags(const int32_t *__restrict__ cstart, long long bi, 
 {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ni * nj) return;
-    const long long i = c / nj, j = c - (c / nj) * nj;
+    long long i, j;
+    divmod_cells(c, nj, i, j);
     uint32_t t = 0;
     for (int s = 0; s < 16 && !t; ++s) {
         const long long ii = i - (s & 3), jj = j - (s >> 2);
@@ -153,7 +154,8 @@ __global__ void gather_sum(const int32_t *__restrict__ cstart, const double3 *__
 {
     const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cidx >= ni * nj || !flag[cidx]) return;
-    const long long i = cidx / nj, j = cidx - (cidx / nj) * nj;
+    long long i, j;
+    divmod_cells(cidx, nj, i, j);
     const long long I = i + imin, J = j + jmin;  // absolute cell indices
     const double ui = ((double)I + 0.5) + x0h;
     const double uj = ((double)J + 0.5) + y0h;
@@ -192,7 +194,8 @@ __global__ void gather_sum_tutty(const int32_t *__restrict__ cstart, const doubl
 {
     const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cidx >= ni * nj || !flag[cidx]) return;
-    const long long i = cidx / nj, j = cidx - (cidx / nj) * nj;
+    long long i, j;
+    divmod_cells(cidx, nj, i, j);
     const long long I = i + imin, J = j + jmin;
     const double hh = 0.5 * h;
     double acc = 0.0;
