@@ -128,6 +128,38 @@ __device__ __forceinline__ float pair_f32_far(float dx, float dy, float w, const
     return w * rcp_f32(r2);
 }
 
+// expm1(x) for x <= 0 in float64, within 1.5 ulp (checked against a
+// long-double expm1 over [-50, 0] including subnormal x), with no branch:
+// Cody-Waite reduction x = k ln2 + r, |r| <= ln2/2, with ln2 = LN2_HI +
+// LN2_LO and LN2_HI carrying 32 significant bits so that k LN2_HI is exact;
+// expm1(r) by its Taylor series through r^13 (truncation below 2e-17
+// relative at |r| = ln2/2); expm1(x) = 2^k expm1(r) + (2^k - 1), whose
+// second term is exact for k >= -53.  x is clamped at -40, where expm1
+// rounds to -1.
+__device__ __forceinline__ double expm1_neg_f64(double x)
+{
+    x = __builtin_fmax(x, -40.0);
+    const double k = __builtin_rint(x * 1.4426950408889634);
+    double r = __builtin_fma(-k, 0x1.62e42ff000000p-1, x);
+    r = __builtin_fma(-k, -0x1.718432a1b0e26p-35, r);
+    // q = sum_{j=0..12} r^j / (j+1)!
+    double q = 1.6059043836821613e-10;
+    q = __builtin_fma(q, r, 2.08767569878681e-09);
+    q = __builtin_fma(q, r, 2.505210838544172e-08);
+    q = __builtin_fma(q, r, 2.7557319223985888e-07);
+    q = __builtin_fma(q, r, 2.7557319223985893e-06);
+    q = __builtin_fma(q, r, 2.4801587301587302e-05);
+    q = __builtin_fma(q, r, 1.9841269841269841e-04);
+    q = __builtin_fma(q, r, 1.3888888888888889e-03);
+    q = __builtin_fma(q, r, 8.3333333333333332e-03);
+    q = __builtin_fma(q, r, 4.1666666666666664e-02);
+    q = __builtin_fma(q, r, 1.6666666666666666e-01);
+    q = __builtin_fma(q, r, 0.5);
+    q = __builtin_fma(q, r, 1.0);
+    const double t = __builtin_amdgcn_ldexp(1.0, (int)k);
+    return __builtin_fma(t, r * q, t - 1.0);
+}
+
 template <int KIND>
 __device__ __forceinline__ double pair_f64(double dx, double dy, double w, const PairParams &pp)
 {
@@ -139,8 +171,11 @@ __device__ __forceinline__ double pair_f64(double dx, double dy, double w, const
         const double r2 = __builtin_fma(dx, dx, dy * dy) + pp.eps;
         return w * rcp_f64(r2);
     } else {
+        // scale (1 - e) rises through 1 exactly at r2 = c2 (scale =
+        // 1/(1 - exp(-shape c2))), so the cutoff branch is a min; with
+        // cutoff 0 the scale is +inf and every r2 > 0 gives 1
         const double r2 = __builtin_fma(dx, dx, dy * dy);
-        const double om = r2 < pp.c2 ? -expm1(-pp.inv_ks * r2) * pp.scale : 1.0;
+        const double om = __builtin_fmin(-expm1_neg_f64(-pp.inv_ks * r2) * pp.scale, 1.0);
         const double P = (w * rcp_f64(r2)) * om;
         return r2 > 0.0 ? P : 0.0;
     }

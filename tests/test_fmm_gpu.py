@@ -32,6 +32,23 @@ def test_tol0_is_direct(prec, tol, O):
     assert O.rel_l2(got, ref) <= tol
 
 
+@pytest.mark.parametrize("prec,tol", [(F64, 1e-12), (F32C, 1e-6)])
+def test_small_cutoff(prec, tol, O):
+    """cutoff = xopt: the smoothing scale 1/(1 - exp(-alpha)) ~ 1.4 differs
+    from 1 in float32 and pairs lie on both sides of the cutoff, so the
+    kernel's cutoff branch decides (direct path and FMM near field)."""
+    x, y, w, xopt, _ = _sys(6000, 11)
+    cutoff = xopt
+    ref = O.direct_oseen(x, y, w, x, y, xopt, cutoff)
+    got = gauss.fmm2d_gpu(x, y, w, x, y, 35, xopt, cutoff, 0.0, prec=prec)
+    assert O.rel_l2(got, ref) <= tol
+    got = gauss.fmm2d_gpu(x, y, w, x, y, 35, xopt, cutoff, 1e-6, prec=prec, order=20, leaf=32)
+    assert O.rel_l2(got, ref) <= 1e-6
+    # the cutoff matters at this size: the default cutoff gives another answer
+    far = O.direct_oseen(x, y, w, x, y, xopt, 5.0 * xopt)
+    assert O.rel_l2(far, ref) > 1e-3
+
+
 @pytest.mark.parametrize("prec,order,ws,leaf,tol", [
     (F64, 16, 1, 64, 1e-6), (F64, 30, 2, 32, 1e-12), (F32C, 16, 1, 64, 1e-6),
     (F32C, 24, 1, 16, 1e-6), (F32C, 8, 1, 64, 1e-4), (F64, 16, 2, 16, 1e-9)])
