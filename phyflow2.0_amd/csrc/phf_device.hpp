@@ -164,9 +164,10 @@ template <int KIND>
 __device__ __forceinline__ double pair_f64(double dx, double dy, double w, const PairParams &pp)
 {
     if (KIND == PAIR_GAUSS_EPS) {
+        // 1 - exp(-r2/ks) as -expm1 (agrees with the subtraction to an ulp
+        // of 1, without its cancellation at small r2)
         const double r2 = __builtin_fma(dx, dx, dy * dy) + pp.eps;
-        const double e = exp(-r2 * pp.inv_ks);
-        return (w * rcp_f64(r2)) * (1.0 - e);
+        return (w * rcp_f64(r2)) * -expm1_neg_f64(-r2 * pp.inv_ks);
     } else if (KIND == PAIR_CUTOFF) {
         const double r2 = __builtin_fma(dx, dx, dy * dy) + pp.eps;
         return w * rcp_f64(r2);

@@ -215,9 +215,22 @@ __global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int6
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
+    // eight partials loaded at a time (independent loads in flight), added
+    // one by one in chunk order
     double a = 0.0, b = 0.0;
-    for (int c = 0; c < nchunk; ++c) {
-        double2 p = part[(int64_t)c * nt + t];
+    int c = 0;
+    for (; c + 8 <= nchunk; c += 8) {
+        double2 p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(c + u) * nt + t];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a += p[u].x;
+            b += p[u].y;
+        }
+    }
+    for (; c < nchunk; ++c) {
+        const double2 p = part[(int64_t)c * nt + t];
         a += p.x;
         b += p.y;
     }
@@ -326,7 +339,15 @@ __global__ void vort_finish(const double *__restrict__ part, int nchunk, int64_t
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     double a = 0.0;
-    for (int c = 0; c < nchunk; ++c) a += part[(int64_t)c * nt + t];
+    int c = 0;
+    for (; c + 8 <= nchunk; c += 8) {
+        double p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(c + u) * nt + t];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a += p[u];
+    }
+    for (; c < nchunk; ++c) a += part[(int64_t)c * nt + t];
     if (cut) {
         const double inv_2pi = 0.5 / 3.141592653589793;
         w[t] = inv_2pi * a;
@@ -354,6 +375,25 @@ int download(Lane *L, const double *dev, int64_t n, double *host)
     return PHF_OK;
 }
 
+// Source chunks of a direct sum over nt targets in groups of 64 * tpl:
+// enough chunks that the grid holds ~8 waves per SIMD of the chip, each of
+// at least 64 sources (a multiple of 16); one chunk once the target groups
+// alone fill the chip.
+static void direct_split(int64_t nt, int tpl, int64_t nb, int64_t *nchunk_out,
+                         int64_t *chunk_len_out)
+{
+    const int64_t tblock = (int64_t)kWave * tpl;
+    const int64_t ngroups = (nt + tblock - 1) / tblock;
+    const int64_t want_waves = 8192;
+    int64_t nchunk = std::max<int64_t>(1, (want_waves + ngroups - 1) / ngroups);
+    nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, nb / 64));
+    nchunk = std::min<int64_t>(nchunk, 65535);
+    int64_t chunk_len = (nb + nchunk - 1) / nchunk;
+    chunk_len = std::max<int64_t>(16, (chunk_len + 15) / 16 * 16);
+    *nchunk_out = std::max<int64_t>(1, (nb + chunk_len - 1) / chunk_len);
+    *chunk_len_out = chunk_len;
+}
+
 template <int KIND>
 static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *xb,
                          const double *yb, const double *wb, int64_t nb, const double *xt,
@@ -363,16 +403,8 @@ static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *
     const int tpl = prec == PHF_PREC_F32C ? TPL32 : TPL64;
     const int64_t tblock = (int64_t)kWave * tpl;
     const int64_t ngroups = (nt + tblock - 1) / tblock;
-    // Split sources so the grid has >= ~4 waves per SIMD of the chip, with
-    // chunks of at least 256 sources (a multiple of 16).
-    const int64_t want_waves = 4096;
-    int64_t nchunk = std::max<int64_t>(1, (want_waves + ngroups - 1) / ngroups);
-    nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, nb / 256));
-    nchunk = std::min<int64_t>(nchunk, 65535);
-    int64_t chunk_len = (nb + nchunk - 1) / nchunk;
-    chunk_len = (chunk_len + 15) / 16 * 16;
-    if (chunk_len == 0) chunk_len = 16;
-    nchunk = std::max<int64_t>(1, (nb + chunk_len - 1) / chunk_len);
+    int64_t nchunk, chunk_len;
+    direct_split(nt, tpl, nb, &nchunk, &chunk_len);
 
     double2 *part;
     PHF_ALLOC(part, L->arena, double2, (size_t)nchunk * (size_t)nt);
@@ -414,12 +446,8 @@ int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, cons
     const int tpl = prec == PHF_PREC_F32C ? TPL32 : TPL64;
     const int64_t tblock = (int64_t)kWave * tpl;
     const int64_t ngroups = (nt + tblock - 1) / tblock;
-    int64_t nchunk = std::max<int64_t>(1, (4096 + ngroups - 1) / ngroups);
-    nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, nb / 256));
-    nchunk = std::min<int64_t>(nchunk, 65535);
-    int64_t chunk_len = (nb + nchunk - 1) / nchunk;
-    chunk_len = std::max<int64_t>(16, (chunk_len + 15) / 16 * 16);
-    nchunk = std::max<int64_t>(1, (nb + chunk_len - 1) / chunk_len);
+    int64_t nchunk, chunk_len;
+    direct_split(nt, tpl, nb, &nchunk, &chunk_len);
     double *part;
     PHF_ALLOC(part, L->arena, double, (size_t)nchunk * (size_t)nt);
     const dim3 grid((unsigned)ngroups, (unsigned)nchunk);

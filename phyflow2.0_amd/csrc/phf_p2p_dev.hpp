@@ -309,7 +309,7 @@ constexpr int P2P_STAGE_F64 = 768;  // staged sources (double4) per batch
 
 // float64 pair sweep over staged entries x in [a, b) with x = o (mod 16)
 // for NT targets at once (each staged record read once for all of them),
-// absolute coordinates, pair_f64 for every pair (branch-free).
+// absolute coordinates, pair_f64's operation order.
 template <int KIND, int NT>
 __device__ __forceinline__ void p2p_sweep_f64(const double4 *__restrict__ st, int a, int b, int o,
                                               const double *xt, const double *yt,
@@ -320,7 +320,19 @@ __device__ __forceinline__ void p2p_sweep_f64(const double4 *__restrict__ st, in
 #pragma unroll
         for (int u = 0; u < NT; ++u) {
             const double dx = xt[u] - q.x, dy = yt[u] - q.y;
-            const double P = pair_f64<KIND>(dx, dy, q.z, pp);
+            double P;
+            if (KIND == PAIR_OSEEN || KIND == PAIR_OSEEN_NOCUT) {
+                // pair_f64's Oseen form; the smoothing factor is skipped
+                // (1 exactly) when no lane of the wave is inside the cutoff
+                const double r2 = __builtin_fma(dx, dx, dy * dy);
+                double om = 1.0;
+                if (__ballot(r2 < pp.c2))
+                    om = __builtin_fmin(-expm1_neg_f64(-pp.inv_ks * r2) * pp.scale, 1.0);
+                const double Pv = (q.z * rcp_f64(r2)) * om;
+                P = r2 > 0.0 ? Pv : 0.0;
+            } else {
+                P = pair_f64<KIND>(dx, dy, q.z, pp);
+            }
             ax[u] = __builtin_fma(P, dx, ax[u]);
             ay[u] = __builtin_fma(P, dy, ay[u]);
         }
