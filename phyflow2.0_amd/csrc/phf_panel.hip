@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace phf {
@@ -770,9 +771,12 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
     double *__restrict__ vy)
 {
     constexpr int NT = F32 ? 16 : PG_TERMS;
+    // series coefficients in the precision of the sum (float32: converted
+    // once per chunk, not per term and target)
+    using CT = typename std::conditional<F32, float2, double2>::type;
     __shared__ PanelK sp[PG_CHUNK * PG_SIZE];
     __shared__ PanelG sg[PG_CHUNK];
-    __shared__ double2 sc[PG_CHUNK * NT];
+    __shared__ CT sc[PG_CHUNK * NT];
     const int tid = threadIdx.x;
     const int64_t k = (int64_t)blockIdx.x * PI_T + tid;
     const int64_t i = k < ne ? order[k] : 0;
@@ -787,29 +791,37 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
         if (tid < ng) sg[tid] = grp[g0 + tid];
         for (int e = tid; e < ng * NT; e += PI_T) {
             const int gg = e / NT, kk = e - gg * NT;
-            sc[e] = coef[(g0 + gg) * PG_TERMS + kk];
+            const double2 c = coef[(g0 + gg) * PG_TERMS + kk];
+            if constexpr (F32)
+                sc[e] = make_float2((float)c.x, (float)c.y);
+            else
+                sc[e] = c;
         }
         __syncthreads();
         for (int gg = 0; gg < ng; ++gg) {
             const PanelG G = sg[gg];
             const double dx = x - G.zx, dy = y - G.zy;
             if (dx * dx + dy * dy > G.far2) {
-                // t = R / (z - z_g)
-                const double d2 = dx * dx + dy * dy;
-                const double tr = G.R * dx / d2, ti = -G.R * dy / d2;
-                const double2 *c = sc + gg * NT;
+                const CT *c = sc + gg * NT;
                 double wr, wi;
-                if (F32) {
-                    const float fr = (float)tr, fi = (float)ti;
-                    float ar = (float)c[NT - 1].x, ai = (float)c[NT - 1].y;
+                if constexpr (F32) {
+                    // t = R / (z - z_g) in float32, Horner in four FMAs a term
+                    const float fx = (float)dx, fy = (float)dy;
+                    const float sc_t = (float)G.R * rcp_f32(__builtin_fmaf(fx, fx, fy * fy));
+                    const float fr = fx * sc_t, fi = -fy * sc_t;
+                    float ar = c[NT - 1].x, ai = c[NT - 1].y;
+#pragma unroll
                     for (int q = NT - 2; q >= 0; --q) {
-                        const float nr = ar * fr - ai * fi + (float)c[q].x;
-                        ai = ar * fi + ai * fr + (float)c[q].y;
+                        const float nr = __builtin_fmaf(ar, fr, __builtin_fmaf(-ai, fi, c[q].x));
+                        ai = __builtin_fmaf(ar, fi, __builtin_fmaf(ai, fr, c[q].y));
                         ar = nr;
                     }
-                    wr = (double)(ar * fr - ai * fi);
-                    wi = (double)(ar * fi + ai * fr);
+                    wr = (double)__builtin_fmaf(ar, fr, -ai * fi);
+                    wi = (double)__builtin_fmaf(ar, fi, ai * fr);
                 } else {
+                    // t = R / (z - z_g)
+                    const double d2 = dx * dx + dy * dy;
+                    const double tr = G.R * dx / d2, ti = -G.R * dy / d2;
                     double ar = c[NT - 1].x, ai = c[NT - 1].y;
                     for (int q = NT - 2; q >= 0; --q) {
                         const double nr = ar * tr - ai * ti + c[q].x;

@@ -3,7 +3,8 @@
 Pins: Heun's second order (error ratio ~4 per halving of dt on a vortex
 pair whose exact motion is known), free-stream translation, one coupled
 blob/panel step against a CPU restatement (oracle direct Oseen sum, float64
-least squares, oracle panel velocities) to 1e-11, and exact conservation of
+least squares, oracle panel velocities) to 1e-11 in float64 and to the 1e-6
+mixed contract in F32C with the grouped panel series, and exact conservation of
 circulation/impulse by the remesh (float64, <= 1e-13 relative)."""
 import math
 
@@ -53,18 +54,10 @@ def _cpu_stage(O, x, y, w, sigma, g, s_panel, vinf):
     return vx + px + vinf[0], vy + py + vinf[1]
 
 
-def test_coupled_step_matches_cpu_restatement(O):
-    m, n, sigma, dt, vinf = 64, 2000, 0.02, 0.01, (1.0, 0.0)
-    xp, yp = W.cylinder_panels(m, R=0.5)
-    g = W.panel_geometry(xp, yp)
-    a = W.annulus_blobs(n, gamma_std=1e-2, sigma=sigma, seed=1)
-    body = PanelBody(xp, yp, "prescribed", 0.0)
-    xc, yc = body.collocation_points()
-    np.testing.assert_allclose(xc, g["xcp"], rtol=0, atol=1e-15)
-    with BlobSystem(a["x"], a["y"], a["w"]) as s:
-        info = step_rk2(s, vel_params(sigma, tol=0.0), dt, vInf=vinf, panels=body)
-        xg, yg, _ = s.get()
-    # CPU restatement: slip at CPs, least squares with the Kelvin row, Heun
+def _cpu_step(O, a, g, sigma, dt, vinf):
+    """CPU restatement of one coupled step: slip at the collocation points,
+    least squares with the Kelvin row, Heun with oracle blob and panel
+    velocities."""
     xopt, cutoff = W.oseen_params(sigma)
     A = O.panel_influence(g["xcp"], g["ycp"], g["xs"], g["ys"], g["xe"], g["ye"], g["cosA"],
                           g["sinA"])
@@ -76,11 +69,44 @@ def test_coupled_step_matches_cpu_restatement(O):
     k1 = _cpu_stage(O, a["x"], a["y"], a["w"], sigma, g, sp, vinf)
     x1, y1 = a["x"] + dt * k1[0], a["y"] + dt * k1[1]
     k2 = _cpu_stage(O, x1, y1, a["w"], sigma, g, sp, vinf)
-    xr = a["x"] + 0.5 * dt * (k1[0] + k2[0])
-    yr = a["y"] + 0.5 * dt * (k1[1] + k2[1])
+    return a["x"] + 0.5 * dt * (k1[0] + k2[0]), a["y"] + 0.5 * dt * (k1[1] + k2[1])
+
+
+def test_coupled_step_matches_cpu_restatement(O):
+    m, n, sigma, dt, vinf = 64, 2000, 0.02, 0.01, (1.0, 0.0)
+    xp, yp = W.cylinder_panels(m, R=0.5)
+    g = W.panel_geometry(xp, yp)
+    a = W.annulus_blobs(n, gamma_std=1e-2, sigma=sigma, seed=1)
+    body = PanelBody(xp, yp, "prescribed", 0.0)
+    xc, yc = body.collocation_points()
+    np.testing.assert_allclose(xc, g["xcp"], rtol=0, atol=1e-15)
+    with BlobSystem(a["x"], a["y"], a["w"]) as s:
+        info = step_rk2(s, vel_params(sigma, tol=0.0), dt, vInf=vinf, panels=body)
+        xg, yg, _ = s.get()
+    xr, yr = _cpu_step(O, a, g, sigma, dt, vinf)
     dx = np.max(np.abs(np.concatenate([xg - xr, yg - yr])))
     disp = np.max(np.abs(np.concatenate([xr - a["x"], yr - a["y"]])))
     assert dx <= 1e-11 * disp, (dx, disp, info)
+
+
+def test_coupled_step_mixed_precision_grouped_panels(O):
+    """Mixed precision (FMM order 16 in F32C, float32 panel series and
+    transcendentals) with enough blobs for the grouped panel path (512
+    panels in groups of 16): the displacement beyond the free stream agrees
+    with the float64 CPU restatement to the 1e-6 relative L2 of the mixed
+    contract."""
+    m, n, sigma, dt, vinf = 512, 20000, 0.01, 0.01, (1.0, 0.0)
+    xp, yp = W.cylinder_panels(m, R=0.5)
+    g = W.panel_geometry(xp, yp)
+    a = W.annulus_blobs(n, gamma_std=1e-2, sigma=sigma, seed=4)
+    body = PanelBody(xp, yp, "prescribed", 0.0)
+    with BlobSystem(a["x"], a["y"], a["w"]) as s:
+        step_rk2(s, vel_params(sigma, prec=1, order=16), dt, vInf=vinf, panels=body)
+        xg, yg, _ = s.get()
+    xr, yr = _cpu_step(O, a, g, sigma, dt, vinf)
+    ref = (xr - a["x"] - vinf[0] * dt, yr - a["y"] - vinf[1] * dt)
+    got = (xg - a["x"] - vinf[0] * dt, yg - a["y"] - vinf[1] * dt)
+    assert O.rel_l2(got, ref) <= 1e-6
 
 
 def test_remesh_conserves_invariants():
