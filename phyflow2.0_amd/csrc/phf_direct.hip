@@ -82,7 +82,8 @@ __global__ void pack_src32c(const double *__restrict__ x, const double *__restri
 // ------------------------------------------------------ F32C kernel ------
 template <int KIND, int TPL>
 __global__ __launch_bounds__(64) void direct_f32c(
-    const Src32c *__restrict__ src, int64_t nb, const double *__restrict__ xt,
+    const double *__restrict__ xb, const double *__restrict__ yb, const double *__restrict__ wb,
+    int64_t nb, const double *__restrict__ xt,
     const double *__restrict__ yt, int64_t nt, int64_t chunk_len, PairParams pp,
     double2 *__restrict__ part)
 {
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(64) void direct_f32c(
         for (int i = 0; i < TPL; ++i) au[i] = av[i] = 0.0f;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const Src32c s = src[j + k];
+            const Src32c s{xb[j + k], yb[j + k], (float)wb[j + k], 0.0f};
 #pragma unroll
             for (int i = 0; i < TPL; ++i) {
                 float dx = (float)(tx[i] - s.x);
@@ -127,7 +128,7 @@ __global__ __launch_bounds__(64) void direct_f32c(
 #pragma unroll
         for (int i = 0; i < TPL; ++i) au[i] = av[i] = 0.0f;
         for (; j < j1; ++j) {
-            const Src32c s = src[j];
+            const Src32c s{xb[j], yb[j], (float)wb[j], 0.0f};
 #pragma unroll
             for (int i = 0; i < TPL; ++i) {
                 float dx = (float)(tx[i] - s.x);
@@ -155,7 +156,8 @@ __global__ __launch_bounds__(64) void direct_f32c(
 // ------------------------------------------------------- F64 kernel ------
 template <int KIND, int TPL>
 __global__ __launch_bounds__(64) void direct_f64(
-    const Src64 *__restrict__ src, int64_t nb, const double *__restrict__ xt,
+    const double *__restrict__ xb, const double *__restrict__ yb, const double *__restrict__ wb,
+    int64_t nb, const double *__restrict__ xt,
     const double *__restrict__ yt, int64_t nt, int64_t chunk_len, PairParams pp,
     double2 *__restrict__ part)
 {
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(64) void direct_f64(
     for (; j + 8 <= j1; j += 8) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const Src64 s = src[j + k];
+            const Src64 s{xb[j + k], yb[j + k], wb[j + k], 0.0};
 #pragma unroll
             for (int i = 0; i < TPL; ++i) {
                 double dx = tx[i] - s.x;
@@ -187,7 +189,7 @@ __global__ __launch_bounds__(64) void direct_f64(
         }
     }
     for (; j < j1; ++j) {
-        const Src64 s = src[j];
+        const Src64 s{xb[j], yb[j], wb[j], 0.0};
 #pragma unroll
         for (int i = 0; i < TPL; ++i) {
             double dx = tx[i] - s.x;
@@ -412,17 +414,12 @@ static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *
     if (nb == 0) {
         PHF_HIP(hipMemsetAsync(part, 0, sizeof(double2) * nt, L->stream));
     } else if (prec == PHF_PREC_F32C) {
-        Src32c *s;
-        PHF_ALLOC(s, L->arena, Src32c, (size_t)nb);
-        pack_src32c<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
-        direct_f32c<KIND, TPL32><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
-                                                                 pp, part);
+        // sources read in place: every lane of a wave reads the same one
+        direct_f32c<KIND, TPL32><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                                 chunk_len, pp, part);
     } else {
-        Src64 *s;
-        PHF_ALLOC(s, L->arena, Src64, (size_t)nb);
-        pack_src64<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
-        direct_f64<KIND, TPL64><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
-                                                                pp, part);
+        direct_f64<KIND, TPL64><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                                chunk_len, pp, part);
     }
     PHF_HIP(hipGetLastError());
     direct_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(
