@@ -64,21 +64,6 @@ PairParams make_oseen_params(double xopt, double cutoff)
     return p;
 }
 
-// ------------------------------------------------------------ packing ----
-__global__ void pack_src64(const double *__restrict__ x, const double *__restrict__ y,
-                           const double *__restrict__ w, int64_t n, Src64 *__restrict__ s)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) s[i] = Src64{x[i], y[i], w[i], 0.0};
-}
-
-__global__ void pack_src32c(const double *__restrict__ x, const double *__restrict__ y,
-                            const double *__restrict__ w, int64_t n, Src32c *__restrict__ s)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) s[i] = Src32c{x[i], y[i], (float)w[i], 0.0f};
-}
-
 // ------------------------------------------------------ F32C kernel ------
 template <int KIND, int TPL>
 __global__ __launch_bounds__(64) void direct_f32c(
@@ -251,7 +236,9 @@ __global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int6
 //   w(t) = (1/pi) sum_m w_m exp(-(|t - z_m|^2 + EPS)/ks) / ks,  EPS = 1e-12
 // Same layout as the velocity kernels with one accumulator per target.
 template <int TPL, bool CUT>
-__global__ __launch_bounds__(64) void vort_f32c(const Src32c *__restrict__ src, int64_t nb,
+__global__ __launch_bounds__(64) void vort_f32c(const double *__restrict__ xb,
+                                                const double *__restrict__ yb,
+                                                const double *__restrict__ wb, int64_t nb,
                                                 const double *__restrict__ xt,
                                                 const double *__restrict__ yt, int64_t nt,
                                                 int64_t chunk_len, PairParams pp,
@@ -276,7 +263,7 @@ __global__ __launch_bounds__(64) void vort_f32c(const Src32c *__restrict__ src, 
 #pragma unroll
         for (int i = 0; i < TPL; ++i) a[i] = 0.0f;
         for (int64_t k = j; k < je; ++k) {
-            const Src32c s = src[k];
+            const Src32c s{xb[k], yb[k], (float)wb[k], 0.0f};
 #pragma unroll
             for (int i = 0; i < TPL; ++i) {
                 const float dx = (float)(tx[i] - s.x), dy = (float)(ty[i] - s.y);
@@ -297,7 +284,9 @@ __global__ __launch_bounds__(64) void vort_f32c(const Src32c *__restrict__ src, 
 }
 
 template <int TPL, bool CUT>
-__global__ __launch_bounds__(64) void vort_f64(const Src64 *__restrict__ src, int64_t nb,
+__global__ __launch_bounds__(64) void vort_f64(const double *__restrict__ xb,
+                                               const double *__restrict__ yb,
+                                               const double *__restrict__ wb, int64_t nb,
                                                const double *__restrict__ xt,
                                                const double *__restrict__ yt, int64_t nt,
                                                int64_t chunk_len, PairParams pp,
@@ -317,7 +306,7 @@ __global__ __launch_bounds__(64) void vort_f64(const Src64 *__restrict__ src, in
     const int64_t j0 = (int64_t)blockIdx.y * chunk_len;
     const int64_t j1 = min(nb, j0 + chunk_len);
     for (int64_t j = j0; j < j1; ++j) {
-        const Src64 s = src[j];
+        const Src64 s{xb[j], yb[j], wb[j], 0.0};
 #pragma unroll
         for (int i = 0; i < TPL; ++i) {
             const double dx = tx[i] - s.x, dy = ty[i] - s.y;
@@ -451,25 +440,19 @@ int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, cons
     if (nb == 0) {
         PHF_HIP(hipMemsetAsync(part, 0, sizeof(double) * nt, L->stream));
     } else if (prec == PHF_PREC_F32C) {
-        Src32c *s;
-        PHF_ALLOC(s, L->arena, Src32c, (size_t)nb);
-        pack_src32c<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
         if (cutoff)
-            vort_f32c<TPL32, true><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp,
-                                                                  part);
+            vort_f32c<TPL32, true><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                                  chunk_len, pp, part);
         else
-            vort_f32c<TPL32, false><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
-                                                                   pp, part);
+            vort_f32c<TPL32, false><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                                   chunk_len, pp, part);
     } else {
-        Src64 *s;
-        PHF_ALLOC(s, L->arena, Src64, (size_t)nb);
-        pack_src64<<<div_up(nb, 256), 256, 0, L->stream>>>(xb, yb, wb, nb, s);
         if (cutoff)
-            vort_f64<TPL64, true><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len, pp,
-                                                                 part);
+            vort_f64<TPL64, true><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                                 chunk_len, pp, part);
         else
-            vort_f64<TPL64, false><<<grid, kWave, 0, L->stream>>>(s, nb, xt, yt, nt, chunk_len,
-                                                                  pp, part);
+            vort_f64<TPL64, false><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                                  chunk_len, pp, part);
     }
     PHF_HIP(hipGetLastError());
     vort_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(part, nb == 0 ? 1 : (int)nchunk, nt,
