@@ -26,6 +26,7 @@
 
 namespace phf {
 
+// one source as a kernel uses it, read in place from the x, y, w arrays
 struct alignas(32) Src64 {
     double x, y, w, pad;
 };
