@@ -748,6 +748,119 @@ __global__ __launch_bounds__(64) void panel_group_coeffs(
     coef[(int64_t)g * PG_TERMS + k] = make_double2(bx, by);
 }
 
+// panel_group_coeffs without the strengths: the group headers, and per
+// panel p of the group and term k the factor geo[p * PG_TERMS + k] =
+// R_g / (2 pi) (s_p + i c_p) (zh2^(k+1) - zh1^(k+1)) / (k + 1), so that
+// b_k = sum_p gamma_p geo[p][k] (panel_group_apply).
+__global__ __launch_bounds__(64) void panel_group_geom(
+    int64_t M, const double *__restrict__ xs, const double *__restrict__ ys,
+    const double *__restrict__ xe, const double *__restrict__ ye,
+    const double *__restrict__ cosA, const double *__restrict__ sinA, PanelG *__restrict__ grp,
+    double2 *__restrict__ geo)
+{
+    __shared__ double pd[PG_SIZE][6];  // xs, ys, xe, ye, cos, sin
+    __shared__ double sc[3];
+    const int g = blockIdx.x, k = threadIdx.x;
+    const int64_t p0 = (int64_t)g * PG_SIZE;
+    const int cnt = (int)min((int64_t)PG_SIZE, M - p0);
+    if (k < cnt) {
+        const int64_t p = p0 + k;
+        pd[k][0] = xs[p];
+        pd[k][1] = ys[p];
+        pd[k][2] = xe[p];
+        pd[k][3] = ye[p];
+        pd[k][4] = cosA[p];
+        pd[k][5] = sinA[p];
+    }
+    __syncthreads();
+    if (k == 0) {
+        double cx = 0.0, cy = 0.0;
+        for (int q = 0; q < cnt; ++q) {
+            cx += 0.5 * (pd[q][0] + pd[q][2]);
+            cy += 0.5 * (pd[q][1] + pd[q][3]);
+        }
+        cx /= cnt;
+        cy /= cnt;
+        double r2 = 0.0;
+        for (int q = 0; q < cnt; ++q) {
+            const double a1 = pd[q][0] - cx, b1 = pd[q][1] - cy;
+            const double a2 = pd[q][2] - cx, b2 = pd[q][3] - cy;
+            r2 = fmax(r2, fmax(a1 * a1 + b1 * b1, a2 * a2 + b2 * b2));
+        }
+        const double R = sqrt(r2);
+        sc[0] = cx;
+        sc[1] = cy;
+        sc[2] = R;
+        PanelG pg;
+        pg.zx = cx;
+        pg.zy = cy;
+        pg.far2 = 9.0 * r2;
+        pg.R = R;
+        pg.inv_R = 1.0 / R;
+        pg.first = (int32_t)p0;
+        pg.count = cnt;
+        pg.pad0 = pg.pad1 = 0;
+        grp[g] = pg;
+    }
+    __syncthreads();
+    if (k >= PG_TERMS) return;
+    const double cx = sc[0], cy = sc[1], R = sc[2], iR = 1.0 / R;
+    for (int q = 0; q < cnt; ++q) {
+        const double u1 = (pd[q][0] - cx) * iR, v1 = (pd[q][1] - cy) * iR;
+        const double u2 = (pd[q][2] - cx) * iR, v2 = (pd[q][3] - cy) * iR;
+        double p1r = u1, p1i = v1, p2r = u2, p2i = v2;
+        for (int e = 0; e < k; ++e) {
+            const double t1 = p1r * u1 - p1i * v1;
+            p1i = p1r * v1 + p1i * u1;
+            p1r = t1;
+            const double t2 = p2r * u2 - p2i * v2;
+            p2i = p2r * v2 + p2i * u2;
+            p2r = t2;
+        }
+        const double dr = (p2r - p1r) / (k + 1), di = (p2i - p1i) / (k + 1);
+        const double f = R / (2.0 * M_PI);
+        const double fr = f * pd[q][5], fi = f * pd[q][4];
+        geo[(p0 + q) * PG_TERMS + k] = make_double2(fr * dr - fi * di, fr * di + fi * dr);
+    }
+}
+
+// b_k of every group from the strengths: one thread per (group, term),
+// the group's panels in index order.
+__global__ void panel_group_apply(int64_t M, int64_t NG, const double *__restrict__ gamma,
+                                  const double2 *__restrict__ geo, double2 *__restrict__ coef)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NG * PG_TERMS) return;
+    const int64_t g = e / PG_TERMS, k = e - g * PG_TERMS;
+    const int64_t p0 = g * PG_SIZE, p1 = min(M, p0 + PG_SIZE);
+    double bx = 0.0, by = 0.0;
+    for (int64_t p = p0; p < p1; ++p) {
+        const double2 f = geo[p * PG_TERMS + k];
+        bx += gamma[p] * f.x;
+        by += gamma[p] * f.y;
+    }
+    coef[e] = make_double2(bx, by);
+}
+
+int panel_series_geom_device(Lane *L, int64_t m, const double *xs, const double *ys,
+                             const double *xe, const double *ye, const double *cosA,
+                             const double *sinA, PanelSeriesGeom *out)
+{
+    *out = PanelSeriesGeom{};
+    if (m < 4 * PG_SIZE) return PHF_OK;  // the per-panel path serves such bodies
+    const int64_t ng = (m + PG_SIZE - 1) / PG_SIZE;
+    PanelG *grp;
+    double2 *geo;
+    PHF_ALLOC(grp, L->arena, PanelG, (size_t)ng);
+    PHF_ALLOC(geo, L->arena, double2, (size_t)m * PG_TERMS);
+    panel_group_geom<<<(unsigned)ng, 64, 0, L->stream>>>(m, xs, ys, xe, ye, cosA, sinA, grp, geo);
+    PHF_HIP(hipGetLastError());
+    out->grp = grp;
+    out->geo = geo;
+    out->m = m;
+    return PHF_OK;
+}
+
 __global__ void target_keys16(const double *__restrict__ x, const double *__restrict__ y,
                               int64_t n, const double4 *__restrict__ bb,
                               uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
@@ -870,7 +983,7 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
 int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
                          const double *yE, const double *xs, const double *ys, const double *xe,
                          const double *ye, const double *cosA, const double *sinA, double *vx,
-                         double *vy, int prec)
+                         double *vy, int prec, const PanelSeriesGeom *sg)
 {
     if (ne <= 0) return PHF_OK;
     PanelK *pk;
@@ -893,10 +1006,17 @@ int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, co
     const int64_t ng = (m + PG_SIZE - 1) / PG_SIZE;
     PanelG *grp;
     double2 *coef;
-    PHF_ALLOC(grp, L->arena, PanelG, (size_t)ng);
     PHF_ALLOC(coef, L->arena, double2, (size_t)ng * PG_TERMS);
-    panel_group_coeffs<<<(unsigned)ng, 64, 0, L->stream>>>(m, gamma, xs, ys, xe, ye, cosA, sinA,
-                                                           grp, coef);
+    if (sg && sg->grp && sg->m == m) {
+        // fixed body: strengths times the precomputed factors
+        grp = static_cast<PanelG *>(sg->grp);
+        panel_group_apply<<<div_up(ng * PG_TERMS, 256), 256, 0, L->stream>>>(m, ng, gamma,
+                                                                             sg->geo, coef);
+    } else {
+        PHF_ALLOC(grp, L->arena, PanelG, (size_t)ng);
+        panel_group_coeffs<<<(unsigned)ng, 64, 0, L->stream>>>(m, gamma, xs, ys, xe, ye, cosA,
+                                                               sinA, grp, coef);
+    }
     // targets in Morton order
     double4 *bb;
     PHF_ALLOC(bb, L->arena, double4, 1);

@@ -37,10 +37,30 @@ int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double 
 // row-major float64 B on the device: float32 LU + float64 refinement.
 int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const double *c,
                        double *x, SolveStats *st);
+// The strength-independent part of the grouped panel velocity of a fixed
+// body: group centres and radii, and per panel and series term the factor
+// that times the panel's strength gives its share of the group's
+// multipole coefficient.  Device buffers from the arena of the lane that
+// computed them (panel_series_geom_device); grp == nullptr: not computed
+// (the body has too few panels for the grouped path).
+struct PanelSeriesGeom {
+    void *grp = nullptr;
+    double2 *geo = nullptr;
+    int64_t m = 0;
+};
+int panel_series_geom_device(Lane *L, int64_t m, const double *xs, const double *ys,
+                             const double *xe, const double *ye, const double *cosA,
+                             const double *sinA, PanelSeriesGeom *out);
+
+// Velocity induced at (xE, yE) by the m vortex panels of strengths gamma.
+// sg (optional): the body's PanelSeriesGeom, so the grouped path forms its
+// series coefficients as strengths times factors instead of from the
+// geometry.
 int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
                          const double *yE, const double *xs, const double *ys, const double *xe,
                          const double *ye, const double *cosA, const double *sinA, double *vx,
-                         double *vy, int prec = PHF_PREC_F64);
+                         double *vy, int prec = PHF_PREC_F64,
+                         const PanelSeriesGeom *sg = nullptr);
 int panel_influence_device(Lane *L, int64_t m, const double *xCP, const double *yCP,
                            const double *xs, const double *ys, const double *xe, const double *ye,
                            const double *cosA, const double *sinA, double *A);

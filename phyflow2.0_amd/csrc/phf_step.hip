@@ -36,6 +36,7 @@ struct Panels {
     double *A = nullptr;  // influence matrix, row-major m x m
     DenseFactors f;       // factors of A or of [A; L]
     double *s = nullptr;  // last strengths
+    PanelSeriesGeom sg;   // grouped-velocity factors of this body
     std::vector<double> hxcp, hycp;
     int last_iterations = 0;
     bool have_solution = false;  // s holds a converged solution
@@ -182,7 +183,7 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
         tm.end(PHF_STAGE_PANEL);
     }
     PHF_TRY(panel_induced_device(L, m, P->s, n, X, Y, P->xs, P->ys, P->xe, P->ye, P->cosA,
-                                 P->sinA, px, py, vp.prec));
+                                 P->sinA, px, py, vp.prec, &P->sg));
     total_velocity_to<<<div_up(n, 256), 256, 0, L->stream>>>(vx, vy, px, py, n, vinf_x, vinf_y,
                                                              kx, ky);
     PHF_HIP(hipGetLastError());
@@ -270,6 +271,8 @@ int phf_panels_create(const double *xs, const double *ys, const double *xe, cons
             PHF_HIP(hipMemcpyAsync(P->A + m * m, P->L, sizeof(double) * m,
                                    hipMemcpyDeviceToDevice, L->stream));
         PHF_TRY(dense_factor_device(L, P->A, rows, m, &P->f));
+        PHF_TRY(panel_series_geom_device(L, m, P->xs, P->ys, P->xe, P->ye, P->cosA, P->sinA,
+                                         &P->sg));
         PHF_HIP(hipMemsetAsync(P->s, 0, sizeof(double) * m, L->stream));
         PHF_HIP(hipStreamSynchronize(L->stream));
         return PHF_OK;
