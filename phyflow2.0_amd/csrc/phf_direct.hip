@@ -11,7 +11,7 @@
 // wave with scalar (SMEM) loads and broadcast through SGPR operands: no LDS
 // staging and no barriers.  Long source lists are split over a second grid
 // dimension; each (target block, source chunk) writes a float64 partial and
-// a finishing pass adds the partials in chunk order, so results do not
+// a finishing pass adds the partials in a fixed order, so results do not
 // depend on scheduling.
 //
 // Precision modes
@@ -193,35 +193,50 @@ __global__ __launch_bounds__(64) void direct_f64(
     }
 }
 
-// Adds the chunk partials in chunk order and maps (A, B) = (sum P dx,
-// sum P dy) to the reference outputs:
+// Adds the chunk partials and maps (A, B) = (sum P dx, sum P dy) to the
+// reference outputs:
 //   Gaussian: vx = inv_2pi * (-B), vy = inv_2pi * A  (vv2parGauss_C_cpu.c:150-151)
 //   cutoff:   the same                               (vv2par_C_cpu.c:149-150)
 //   Oseen:    qr = -A,  qi = B                       (direct.cpp:1482-1483)
-__global__ void direct_finish(const double2 *__restrict__ part, int nchunk, int64_t nt,
-                              int kind, double *__restrict__ o0, double *__restrict__ o1)
+// A workgroup of 4 waves serves 64 targets: wave w adds the w-th quarter of
+// the chunks in chunk order (coalesced rows, eight loads in flight), and
+// the four quarter sums are added in wave order, so the result is fixed
+// for a given chunking.
+constexpr int FIN_T = 64;  // targets per finishing workgroup
+__global__ __launch_bounds__(256) void direct_finish(const double2 *__restrict__ part,
+                                                     int nchunk, int64_t nt, int kind,
+                                                     double *__restrict__ o0,
+                                                     double *__restrict__ o1)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    // eight partials loaded at a time (independent loads in flight), added
-    // one by one in chunk order
+    __shared__ double2 red[4][FIN_T];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * FIN_T + l;
+    const int per = (nchunk + 3) / 4;
+    const int c1 = min(nchunk, (w + 1) * per);
     double a = 0.0, b = 0.0;
-    int c = 0;
-    for (; c + 8 <= nchunk; c += 8) {
-        double2 p[8];
+    if (t < nt) {
+        int c = w * per;
+        for (; c + 8 <= c1; c += 8) {
+            double2 p[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(c + u) * nt + t];
+            for (int u = 0; u < 8; ++u) p[u] = part[(int64_t)(c + u) * nt + t];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            a += p[u].x;
-            b += p[u].y;
+            for (int u = 0; u < 8; ++u) {
+                a += p[u].x;
+                b += p[u].y;
+            }
+        }
+        for (; c < c1; ++c) {
+            const double2 p = part[(int64_t)c * nt + t];
+            a += p.x;
+            b += p.y;
         }
     }
-    for (; c < nchunk; ++c) {
-        const double2 p = part[(int64_t)c * nt + t];
-        a += p.x;
-        b += p.y;
-    }
+    red[w][l] = make_double2(a, b);
+    __syncthreads();
+    if (w != 0 || t >= nt) return;
+    a = ((red[0][l].x + red[1][l].x) + red[2][l].x) + red[3][l].x;
+    b = ((red[0][l].y + red[1][l].y) + red[2][l].y) + red[3][l].y;
     if (kind == PAIR_GAUSS_EPS || kind == PAIR_CUTOFF) {
         const double inv_2pi = 0.5 / 3.141592653589793;
         o0[t] = inv_2pi * (-b);
@@ -412,7 +427,7 @@ static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *
                                                                 chunk_len, pp, part);
     }
     PHF_HIP(hipGetLastError());
-    direct_finish<<<div_up(nt, 256), 256, 0, L->stream>>>(
+    direct_finish<<<div_up(nt, FIN_T), 256, 0, L->stream>>>(
         part, nb == 0 ? 1 : (int)nchunk, nt,
         (KIND == PAIR_GAUSS_EPS || KIND == PAIR_CUTOFF) ? KIND : PAIR_OSEEN, o0, o1);
     PHF_HIP(hipGetLastError());
