@@ -13,9 +13,13 @@ kernel, :138):
                                          (-qi/2pi, -qr/2pi) (:236-251, :321-336)
   method 1 (direct), kernel 0 (cutoff) -> vv2par with sigma^2 (:585, :695)
   FMM with the cutoff kernel raises, as the reference does (:158-163).
+Each branch calls the same attribute path as the reference,
+kernels.gauss.fmm2d_cpu / fmm2d_gpu / vv2parGauss_C_cpu / vv2parGauss_gpu
+and kernels.cutoff.vv2par_C_cpu / vv2par_gpu, chosen by ``hardware``.
 Differences from the reference, all deliberate:
-  * every computation runs on the GPU (``hardware`` is accepted and selects
-    nothing; the reference's CPU and GPU variants compute the same sums);
+  * every computation runs on the GPU (the *_cpu names are bound to the same
+    GPU computation; the reference's CPU and GPU variants compute the same
+    sums);
   * ``xEval is None`` is tested with ``is`` (the reference's ``== None`` is
     an elementwise comparison on modern NumPy and raises in ``if``) and no
     padding is needed;
@@ -35,8 +39,7 @@ import math
 
 import numpy as np
 
-from ...cpp.blobs.kernels import cutoff as cutoff_kernels
-from ...cpp.blobs.kernels import gauss
+from ...cpp.blobs import kernels
 from ...cpp.blobs.kernels.gauss import PREC_F32C, PREC_F64
 
 GAUSS_KERNEL = 1   # blobOptions.GAUSS_KERNEL
@@ -64,13 +67,19 @@ def velocity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None, 
     if kernel == CUTOFF_KERNEL and method == DIRECT_METHOD:  # :585, :695 (sigma^2)
         xe = xBlob if xEval is None else xEval
         ye = yBlob if yEval is None else yEval
-        return cutoff_kernels.vv2par_gpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma, blocksize,
+        if hardware == CPU_HARDWARE:
+            return kernels.cutoff.vv2par_C_cpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma,
+                                               prec=prec)
+        return kernels.cutoff.vv2par_gpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma, blocksize,
                                          prec=prec)
     if method == DIRECT_METHOD:
         xe = xBlob if xEval is None else xEval
         ye = yBlob if yEval is None else yEval
-        return gauss.vv2parGauss_gpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma, blocksize,
-                                     prec=prec)
+        if hardware == CPU_HARDWARE:
+            return kernels.gauss.vv2parGauss_C_cpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma,
+                                                   prec=prec)
+        return kernels.gauss.vv2parGauss_gpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma,
+                                             blocksize, prec=prec)
     if method == FMM_METHOD:
         ksigmasqr = k * sigma * sigma
         xopt = math.sqrt(ALPHA * ksigmasqr)
@@ -78,8 +87,9 @@ def velocity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None, 
             cutoff = 5.0 * xopt
         xe = xBlob if xEval is None else xEval
         ye = yBlob if yEval is None else yEval
-        vy_fmm, vx_fmm = gauss.fmm2d_gpu(xBlob, yBlob, wBlob, xe, ye, Ndirect, xopt, cutoff, tol,
-                                         prec=prec, order=order, leaf=leaf, ws=ws)
+        fmm2d = kernels.gauss.fmm2d_cpu if hardware == CPU_HARDWARE else kernels.gauss.fmm2d_gpu
+        vy_fmm, vx_fmm = fmm2d(xBlob, yBlob, wBlob, xe, ye, Ndirect, xopt, cutoff, tol, prec=prec,
+                               order=order, leaf=leaf, ws=ws)
         return -vx_fmm / (2.0 * np.pi), -vy_fmm / (2.0 * np.pi)
     raise ValueError("unknown method %r" % (method,))
 
@@ -99,7 +109,13 @@ def vorticity(xBlob, yBlob, wBlob, sigma, k=2, kernel=1, xEval=None, yEval=None,
     xe = xBlob if xEval is None else xEval
     ye = yBlob if yEval is None else yEval
     if kernel == CUTOFF_KERNEL:  # :1023, :1120 (sigma^2)
-        return cutoff_kernels.ww2par_gpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma, blocksize,
+        if hardware == CPU_HARDWARE:
+            return kernels.cutoff.ww2par_C_cpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma,
+                                               prec=prec)
+        return kernels.cutoff.ww2par_gpu(xBlob, yBlob, wBlob, xe, ye, sigma * sigma, blocksize,
                                          prec=prec)
-    return gauss.ww2parGauss_gpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma, blocksize,
-                                 prec=prec)
+    if hardware == CPU_HARDWARE:  # :862, :963
+        return kernels.gauss.ww2parGauss_C_cpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma,
+                                               prec=prec)
+    return kernels.gauss.ww2parGauss_gpu(xBlob, yBlob, wBlob, xe, ye, k * sigma * sigma,
+                                         blocksize, prec=prec)

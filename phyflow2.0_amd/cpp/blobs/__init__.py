@@ -1,1 +1,5 @@
-"""Mirror of the reference package level of the same name."""
+"""Mirror of pHyFlow.cpp.blobs (reference src/python/cpp/blobs/__init__.py),
+which imports its kernels and remesh modules."""
+from . import kernels, remesh
+
+__all__ = ["kernels", "remesh"]

@@ -1,1 +1,5 @@
-"""Mirror of the reference package level of the same name."""
+"""Mirror of pHyFlow.cpp.panels (reference src/python/cpp/panels/__init__.py),
+which imports its kernels module."""
+from . import kernels
+
+__all__ = ["kernels"]

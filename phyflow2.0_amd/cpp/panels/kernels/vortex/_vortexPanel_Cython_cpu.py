@@ -10,7 +10,7 @@ Both run in float64 on the GPU.
 """
 import numpy as np
 
-from ...._native import call, f64, ptr
+from ....._native import call, f64, ptr
 
 __all__ = ["assemble_influenceMatrix", "inducedVelocity"]
 

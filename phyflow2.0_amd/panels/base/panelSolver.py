@@ -22,7 +22,7 @@ import ctypes
 import numpy as np
 
 from ..._native import SolveInfo, call, f64, ptr
-from ...cpp.panels.kernels import vortex
+from ...cpp.panels import kernels
 
 __all__ = ["influenceMatrix", "inducedVelocity", "solve", "slip_rhs"]
 
@@ -40,15 +40,15 @@ def _check(method, panelKernel):
 def influenceMatrix(xCP, yCP, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd, cosAlpha, sinAlpha,
                     hardware=0, method=1, panelKernel=1):
     _check(method, panelKernel)
-    return vortex.assemble_influenceMatrix(xCP, yCP, xPanelStart, yPanelStart, xPanelEnd,
-                                           yPanelEnd, cosAlpha, sinAlpha)
+    return kernels.vortex._vortexPanel_Cython_cpu.assemble_influenceMatrix(  # :130
+        xCP, yCP, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd, cosAlpha, sinAlpha)
 
 
 def inducedVelocity(strength, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd, cosAlpha, sinAlpha,
                     xEval, yEval, hardware=0, method=1, panelKernel=1):
     _check(method, panelKernel)
-    return vortex.inducedVelocity(strength, xEval, yEval, xPanelStart, yPanelStart, xPanelEnd,
-                                  yPanelEnd, cosAlpha, sinAlpha)
+    return kernels.vortex._vortexPanel_Cython_cpu.inducedVelocity(  # :249
+        strength, xEval, yEval, xPanelStart, yPanelStart, xPanelEnd, yPanelEnd, cosAlpha, sinAlpha)
 
 
 def slip_rhs(vxExt, vyExt, cosAlpha, sinAlpha):

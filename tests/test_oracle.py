@@ -105,12 +105,56 @@ def test_m4_kernel_values_and_partition_of_unity(O):
     m2 = sum(O.m4kernel_1d(u - k) * (u - k) ** 2 for k in range(-2, 3))
     np.testing.assert_allclose(m1, 0.0, atol=2e-15)
     np.testing.assert_allclose(m2, 0.0, atol=2e-15)
-    # diffusion term: c^2 (2 - 9 u^2 + 6|u|^3) inside, (2-|u|)^2 (1-2|u|) outside
+
+
+def test_m4_wee_diffusion_moments(O):
+    """Wee's diffusion term (_interpolation_kernels_1d.cpp:78,82 with
+    c = sqrt(nu dt)/h, _evolution.py:228): for every offset u the weights
+    W_c(u - k) still sum to 1 with zero first moment, and their second
+    moment is exactly 2 c^2 -- one step of the heat equation, whose variance
+    grows by 2 nu dt per axis.  A dropped c-term gives 0, a wrong sign -2c^2,
+    a wrong coefficient breaks the mass or first moment."""
+    u = np.concatenate([np.random.default_rng(1).random(2000)[1:], [0.5, 0.999, 1e-9]])
+    assert np.all((u > 0) & (u < 1))
+    for c in (0.05, 0.2, 0.3, 0.45):
+        W = [(k, O.m4kernel_1d(u - k, c)) for k in range(-2, 3)]
+        m0 = sum(w for _, w in W)
+        m1 = sum(w * (u - k) for k, w in W)
+        m2 = sum(w * (u - k) ** 2 for k, w in W)
+        np.testing.assert_allclose(m0, 1.0, rtol=0, atol=4e-15)
+        np.testing.assert_allclose(m1, 0.0, rtol=0, atol=4e-15)
+        np.testing.assert_allclose(m2, 2 * c * c, rtol=0, atol=4e-15)
+    # the c-term is even in u and vanishes at |u| >= 2
+    v = np.linspace(0, 2.5, 501)
+    np.testing.assert_array_equal(O.m4kernel_1d(v, 0.3), O.m4kernel_1d(-v, 0.3))
+    np.testing.assert_array_equal(O.m4kernel_1d(v[v >= 2.0], 0.3), 0.0)
+    # At |u| = 1 exactly the reference's strict tests (|x| > 1, |x| < 1) fall
+    # through to weight 0, where both pieces of the kernel equal c^2: a blob
+    # on a lattice line loses 2 c^2 of its weight along that axis.  Kept as the
+    # reference has it (c = 0, plain M4', is unaffected: both pieces vanish).
     c = 0.3
-    v = np.array([0.25, 1.5])
-    ref = np.array([1 - 2.5 * 0.0625 + 1.5 * 0.015625 - c * c * (2 - 9 * 0.0625 + 6 * 0.015625),
-                    0.5 * 0.25 * (1 - 1.5) - c * c * 0.25 * (1 - 3.0)])
-    np.testing.assert_allclose(O.m4kernel_1d(v, c), ref, rtol=1e-15)
+    np.testing.assert_array_equal(O.m4kernel_1d([1.0, -1.0], c), 0.0)
+    np.testing.assert_allclose(O.m4kernel_1d([1.0 - 1e-12, 1.0 + 1e-12], c), c * c, rtol=1e-9)
+    on = sum(O.m4kernel_1d(np.array([0.0 - k]), c)[0] for k in range(-2, 3))
+    assert abs(on - (1.0 - 2 * c * c)) <= 1e-15
+
+
+def test_regrid_wee_moments(O):
+    """Regrid with Wee's c: sum w, sum w x, sum w y conserved, and
+    sum w (x^2 + y^2) up by exactly 4 c^2 h^2 sum w (2 c^2 h^2 per axis)."""
+    rng = np.random.default_rng(9)
+    n = 5000
+    x, y = rng.uniform(-0.3, 0.4, (2, n))
+    w = rng.standard_normal(n)
+    sigma, overlap, c = 0.01, 0.8, 0.31
+    h = sigma * overlap
+    xo, yo, wo, io, jo = O.regrid_m4(x, y, w, sigma, overlap, -1.5 * h, -1.5 * h, c)
+    scale = np.sum(np.abs(w))
+    assert abs(np.sum(wo) - np.sum(w)) <= 1e-13 * scale
+    assert abs(np.sum(wo * xo) - np.sum(w * x)) <= 1e-13 * scale
+    assert abs(np.sum(wo * yo) - np.sum(w * y)) <= 1e-13 * scale
+    d2 = np.sum(wo * (xo * xo + yo * yo)) - np.sum(w * (x * x + y * y))
+    assert abs(d2 - 4 * c * c * h * h * np.sum(w)) <= 1e-13 * scale
 
 
 def test_regrid_conserves_moments(O):

@@ -134,3 +134,24 @@ def test_step_with_tutty_remesh_moments():
     assert abs(info["impulse_x_after"] - info["impulse_x_before"]) <= 1e-13 * scale
     d2 = info["m2_after"] - info["m2_before"]
     assert abs(d2 - 4 * hnu * hnu * info["circ_before"]) <= 1e-12 * scale
+
+
+def test_step_with_wee_remesh_moments():
+    """A step with Wee's M4' (c = sqrt(nu dt)/h): circulation and impulse
+    unchanged, sum w r^2 up by exactly 4 c^2 h^2 sum w (float64 invariants
+    of the step)."""
+    from phyflow_amd.lagrangian import step_rk2
+    from phyflow_amd.system import BlobSystem, vel_params
+    s = W.uniform_system(20000, seed=14)
+    overlap = 0.8
+    h = s["sigma"] * overlap
+    c = 0.37
+    with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
+        info = step_rk2(sysm, vel_params(s["sigma"], prec=1), 1e-4,
+                        remesh=dict(sigma=s["sigma"], overlap=overlap, c=c))
+    scale = np.sum(np.abs(s["w"]))
+    assert abs(info["circ_after"] - info["circ_before"]) <= 1e-13 * scale
+    assert abs(info["impulse_x_after"] - info["impulse_x_before"]) <= 1e-13 * scale
+    assert abs(info["impulse_y_after"] - info["impulse_y_before"]) <= 1e-13 * scale
+    d2 = info["m2_after"] - info["m2_before"]
+    assert abs(d2 - 4 * c * c * h * h * info["circ_before"]) <= 1e-12 * scale
