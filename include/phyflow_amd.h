@@ -93,6 +93,29 @@ int phf_vv2par_gauss(const double *xb, const double *yb, const double *wb,
                      int64_t nt, double ksigmasqr, int32_t prec, double *vx,
                      double *vy);
 
+/* Allocation-free variant of phf_vv2par_gauss.  It has no counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same sum and outputs, but every array is DEVICE memory of
+ * the library's GPU (each must lie inside one device allocation; a host
+ * pointer is rejected with PHF_ERR_INVALID), and the call makes no device
+ * or host allocation: the partial sums live in a file-scope device array of
+ * 2^21 double2 (32 MiB) per GPU, held until the process ends.
+ * Limit: nt <= 2,097,152 targets (larger nt is rejected with
+ * PHF_ERR_INVALID before any launch).
+ * stream: a hipStream_t of the caller; the work is enqueued there and the
+ * call returns without waiting.  NULL: the library's own stream, and the
+ * call returns when the results are written.
+ * Every call on one GPU shares that GPU's scratch array, so calls that
+ * overlap on one GPU (different streams or threads without ordering) race;
+ * calls ordered on one stream do not.  phf_init() sets the array up; the
+ * first call does it if phf_init() was not called. */
+int phf_vv2par_gauss_noalloc(const double *xb, const double *yb,
+                             const double *wb, int64_t nb, const double *xt,
+                             const double *yt, int64_t nt, double ksigmasqr,
+                             int32_t prec, double *vx, double *vy,
+                             void *stream);
+
 /* Algebraic cutoff kernel (kernel = 0 in velocity()/vorticity()).  Replace
  * vv2par_gpu / vv2par_C_cpu(xB,yB,wB,xT,yT,sigmasqr) and ww2par_gpu /
  * ww2par_C_cpu (src/cpp/blobs/c/vv2par_C/vv2par_C_cpu.c:89-153,

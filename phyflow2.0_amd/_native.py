@@ -123,6 +123,8 @@ _SIGS = {
     "phf_synchronize": ([], ctypes.c_int),
     "phf_last_stage_times": ([_ptr], ctypes.c_int),
     "phf_vv2par_gauss": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr, _ptr], ctypes.c_int),
+    "phf_vv2par_gauss_noalloc": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr, _ptr,
+                                  _ptr], ctypes.c_int),
     "phf_ww2par_gauss": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr], ctypes.c_int),
     "phf_vv2par_cutoff": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr, _ptr], ctypes.c_int),
     "phf_ww2par_cutoff": ([_ptr, _ptr, _ptr, _i64, _ptr, _ptr, _i64, _dbl, _i32, _ptr], ctypes.c_int),

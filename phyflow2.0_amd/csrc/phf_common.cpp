@@ -156,6 +156,8 @@ int Context::init()
     PHF_HIP(hipSetDevice(0));
     PHF_HIP(hipGetDeviceProperties(&props_, 0));
     dev_ = 0;
+    // device address of the allocation-free direct sum's scratch array
+    PHF_TRY(direct_scratch_setup(dev_));
     for (int i = 0; i < kLanes; ++i) {
         Lane *l = new Lane();
         l->index = i;

@@ -191,6 +191,10 @@ void reset_stage_times();
 bool timing_enabled();
 void set_timing(bool on);
 
+// Looks up (once per device, with that device current) the device address
+// of the file-scope scratch array used by phf_vv2par_gauss_noalloc.
+int direct_scratch_setup(int dev);
+
 // ---------------------------------------------------------- validation ---
 int check_host_array(const double *p, int64_t n, const char *name,
                      bool finite = true);

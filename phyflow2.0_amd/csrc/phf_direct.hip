@@ -385,20 +385,62 @@ int download(Lane *L, const double *dev, int64_t n, double *host)
 // Source chunks of a direct sum over nt targets in groups of 64 * tpl:
 // enough chunks that the grid holds ~8 waves per SIMD of the chip, each of
 // at least 64 sources (a multiple of 16); one chunk once the target groups
-// alone fill the chip.
+// alone fill the chip.  max_chunks caps the count (the partial sums take
+// nchunk * nt double2).
 static void direct_split(int64_t nt, int tpl, int64_t nb, int64_t *nchunk_out,
-                         int64_t *chunk_len_out)
+                         int64_t *chunk_len_out, int64_t max_chunks = 65535)
 {
     const int64_t tblock = (int64_t)kWave * tpl;
     const int64_t ngroups = (nt + tblock - 1) / tblock;
     const int64_t want_waves = 8192;
     int64_t nchunk = std::max<int64_t>(1, (want_waves + ngroups - 1) / ngroups);
     nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, nb / 64));
-    nchunk = std::min<int64_t>(nchunk, 65535);
+    nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, std::min<int64_t>(max_chunks, 65535)));
     int64_t chunk_len = (nb + nchunk - 1) / nchunk;
     chunk_len = std::max<int64_t>(16, (chunk_len + 15) / 16 * 16);
     *nchunk_out = std::max<int64_t>(1, (nb + chunk_len - 1) / chunk_len);
     *chunk_len_out = chunk_len;
+}
+
+// Zero partial row for an empty source set (a kernel rather than a memset,
+// so the sequence holds only kernel launches).
+__global__ __launch_bounds__(256) void zero_part(double2 *__restrict__ part, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) part[i] = make_double2(0.0, 0.0);
+}
+
+constexpr int DIRECT_TPL32 = 2, DIRECT_TPL64 = 1;
+
+// Enqueue the pair kernel and the finishing kernel on stream st with the
+// partial sums in part (nchunk * nt double2; nchunk, chunk_len from
+// direct_split).
+template <int KIND>
+static int enqueue_direct(hipStream_t st, int prec, const PairParams &pp, const double *xb,
+                          const double *yb, const double *wb, int64_t nb, const double *xt,
+                          const double *yt, int64_t nt, int64_t nchunk, int64_t chunk_len,
+                          double2 *part, double *o0, double *o1)
+{
+    const int tpl = prec == PHF_PREC_F32C ? DIRECT_TPL32 : DIRECT_TPL64;
+    const int64_t tblock = (int64_t)kWave * tpl;
+    const int64_t ngroups = (nt + tblock - 1) / tblock;
+    dim3 grid((unsigned)ngroups, (unsigned)nchunk);
+    if (nb == 0) {
+        zero_part<<<div_up(nt, 256), 256, 0, st>>>(part, nt);
+    } else if (prec == PHF_PREC_F32C) {
+        // sources read in place: every lane of a wave reads the same one
+        direct_f32c<KIND, DIRECT_TPL32><<<grid, kWave, 0, st>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                                chunk_len, pp, part);
+    } else {
+        direct_f64<KIND, DIRECT_TPL64><<<grid, kWave, 0, st>>>(xb, yb, wb, nb, xt, yt, nt,
+                                                               chunk_len, pp, part);
+    }
+    PHF_HIP(hipGetLastError());
+    direct_finish<<<div_up(nt, FIN_T), 256, 0, st>>>(
+        part, nb == 0 ? 1 : (int)nchunk, nt,
+        (KIND == PAIR_GAUSS_EPS || KIND == PAIR_CUTOFF) ? KIND : PAIR_OSEEN, o0, o1);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
 }
 
 template <int KIND>
@@ -406,32 +448,13 @@ static int launch_direct(Lane *L, int prec, const PairParams &pp, const double *
                          const double *yb, const double *wb, int64_t nb, const double *xt,
                          const double *yt, int64_t nt, double *o0, double *o1)
 {
-    constexpr int TPL32 = 2, TPL64 = 1;
-    const int tpl = prec == PHF_PREC_F32C ? TPL32 : TPL64;
-    const int64_t tblock = (int64_t)kWave * tpl;
-    const int64_t ngroups = (nt + tblock - 1) / tblock;
+    const int tpl = prec == PHF_PREC_F32C ? DIRECT_TPL32 : DIRECT_TPL64;
     int64_t nchunk, chunk_len;
     direct_split(nt, tpl, nb, &nchunk, &chunk_len);
-
     double2 *part;
     PHF_ALLOC(part, L->arena, double2, (size_t)nchunk * (size_t)nt);
-    dim3 grid((unsigned)ngroups, (unsigned)nchunk);
-    if (nb == 0) {
-        PHF_HIP(hipMemsetAsync(part, 0, sizeof(double2) * nt, L->stream));
-    } else if (prec == PHF_PREC_F32C) {
-        // sources read in place: every lane of a wave reads the same one
-        direct_f32c<KIND, TPL32><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
-                                                                 chunk_len, pp, part);
-    } else {
-        direct_f64<KIND, TPL64><<<grid, kWave, 0, L->stream>>>(xb, yb, wb, nb, xt, yt, nt,
-                                                                chunk_len, pp, part);
-    }
-    PHF_HIP(hipGetLastError());
-    direct_finish<<<div_up(nt, FIN_T), 256, 0, L->stream>>>(
-        part, nb == 0 ? 1 : (int)nchunk, nt,
-        (KIND == PAIR_GAUSS_EPS || KIND == PAIR_CUTOFF) ? KIND : PAIR_OSEEN, o0, o1);
-    PHF_HIP(hipGetLastError());
-    return PHF_OK;
+    return enqueue_direct<KIND>(L->stream, prec, pp, xb, yb, wb, nb, xt, yt, nt, nchunk,
+                                chunk_len, part, o0, o1);
 }
 
 int vorticity_device(Lane *L, int prec, double ksigmasqr, const double *xb, const double *yb,
@@ -621,5 +644,112 @@ static int ww2par_entry(bool cutoff, const double *xb, const double *yb, const d
     PHF_TRY(vorticity_device(L, prec, ksigmasqr, dxb, dyb, dwb, nb, dxt, dyt, nt, o, cutoff));
     PHF_TRY(download(L, o, nt, w));
     PHF_HIP(hipStreamSynchronize(L->stream));
+    return PHF_OK;
+}
+
+// ------------------------------------------- allocation-free direct sum ---
+// phf_vv2par_gauss_noalloc keeps its partial sums in one file-scope device
+// array instead of a workspace from hipMalloc.  Each GPU the library runs
+// on holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: direct_scratch_setup (called by the library's
+// init with the device current) asks the runtime for the device address
+// and every launch takes that pointer.
+namespace phf {
+
+constexpr int64_t kScratchPart = int64_t(1) << 21;  // double2 entries: 32 MiB
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) double2 scratch[kScratchPart];
+static double2 *g_scratch[kScratchDevices] = {};
+
+int direct_scratch_setup(int dev)
+{
+    PHF_REQUIRE(dev >= 0 && dev < kScratchDevices, "device %d out of range", dev);
+    if (g_scratch[dev]) return PHF_OK;
+    void *p = nullptr;
+    PHF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PHF_REQUIRE(p != nullptr, "no device address for the direct-sum scratch array");
+    g_scratch[dev] = reinterpret_cast<double2 *>(p);
+    return PHF_OK;
+}
+
+// A kernel argument must lie inside one device (or managed) allocation of
+// the current device, so a host pointer fails here instead of faulting the
+// GPU.
+static int check_device_range(const void *p, int64_t n, const char *name)
+{
+    if (n == 0) return PHF_OK;
+    PHF_REQUIRE(p != nullptr, "%s is NULL but has %lld entries", name, (long long)n);
+    hipPointerAttribute_t at{};
+    hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("%s is not device memory (%s)", name, hipGetErrorName(e));
+        return PHF_ERR_INVALID;
+    }
+    PHF_REQUIRE(at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged,
+                "%s is not device memory (memory type %d)", name, (int)at.type);
+    hipDeviceptr_t base = nullptr;
+    size_t bytes = 0;
+    e = hipMemGetAddressRange(&base, &bytes, const_cast<void *>(p));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("%s: allocation range unknown (%s)", name, hipGetErrorName(e));
+        return PHF_ERR_INVALID;
+    }
+    const char *b = reinterpret_cast<const char *>(base);
+    const char *q = reinterpret_cast<const char *>(p);
+    PHF_REQUIRE(q >= b && (uint64_t)(q - b) + (uint64_t)n * sizeof(double) <= bytes,
+                "%s: %lld doubles run past the end of its allocation", name, (long long)n);
+    return PHF_OK;
+}
+
+}  // namespace phf
+
+extern "C" int phf_vv2par_gauss_noalloc(const double *xb, const double *yb, const double *wb,
+                                        int64_t nb, const double *xt, const double *yt,
+                                        int64_t nt, double ksigmasqr, int32_t prec, double *vx,
+                                        double *vy, void *stream)
+{
+    clear_error();
+    PHF_REQUIRE(nb >= 0 && nt >= 0, "negative sizes (nb=%lld, nt=%lld)", (long long)nb,
+                (long long)nt);
+    PHF_REQUIRE(nt <= kScratchPart,
+                "nt=%lld exceeds the %lld targets the allocation-free scratch holds",
+                (long long)nt, (long long)kScratchPart);
+    PHF_REQUIRE(nb < (int64_t(1) << 40), "nb out of range (%lld)", (long long)nb);
+    PHF_REQUIRE(prec == PHF_PREC_F64 || prec == PHF_PREC_F32C, "unknown precision %d", prec);
+    PHF_REQUIRE(std::isfinite(ksigmasqr) && ksigmasqr > 0.0,
+                "ksigmasqr must be finite and > 0 (got %g)", ksigmasqr);
+    if (nt == 0) return PHF_OK;
+    PHF_TRY(Context::get().init());
+    const int dev = Context::get().device();
+    PHF_HIP(hipSetDevice(dev));
+    PHF_REQUIRE(dev >= 0 && dev < kScratchDevices && g_scratch[dev] != nullptr,
+                "the direct-sum scratch array of device %d is not set up", dev);
+    PHF_TRY(check_device_range(xb, nb, "xBlob"));
+    PHF_TRY(check_device_range(yb, nb, "yBlob"));
+    PHF_TRY(check_device_range(wb, nb, "wBlob"));
+    PHF_TRY(check_device_range(xt, nt, "xTarget"));
+    PHF_TRY(check_device_range(yt, nt, "yTarget"));
+    PHF_TRY(check_device_range(vx, nt, "vx"));
+    PHF_TRY(check_device_range(vy, nt, "vy"));
+
+    const int tpl = prec == PHF_PREC_F32C ? DIRECT_TPL32 : DIRECT_TPL64;
+    int64_t nchunk, chunk_len;
+    direct_split(nt, tpl, nb, &nchunk, &chunk_len, kScratchPart / nt);
+    PHF_REQUIRE(nchunk * nt <= kScratchPart, "partial sums (%lld x %lld) exceed the scratch",
+                (long long)nchunk, (long long)nt);
+    const PairParams pp = make_gauss_params(ksigmasqr);
+    if (stream) {
+        // enqueue on the caller's stream; the caller synchronises
+        return enqueue_direct<PAIR_GAUSS_EPS>(reinterpret_cast<hipStream_t>(stream), prec, pp,
+                                              xb, yb, wb, nb, xt, yt, nt, nchunk, chunk_len,
+                                              g_scratch[dev], vx, vy);
+    }
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    PHF_TRY(enqueue_direct<PAIR_GAUSS_EPS>(g.lane->stream, prec, pp, xb, yb, wb, nb, xt, yt, nt,
+                                           nchunk, chunk_len, g_scratch[dev], vx, vy));
+    PHF_HIP(hipStreamSynchronize(g.lane->stream));
     return PHF_OK;
 }
