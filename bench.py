@@ -36,6 +36,8 @@ from phyflow_amd.cpp.blobs.kernels import gauss  # noqa: E402
 from phyflow_amd.system import BlobSystem, vel_params  # noqa: E402
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X fp32 (vector, packed) / dense matrix
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
 FLOP_PER_PAIR = 17
 
 
@@ -62,20 +64,54 @@ def timed_steps(sysm, params, warmup, steps, counts=True):
     return wall, np.array(lat), np.array(p2p_ms), info
 
 
-def rel_errors(sysm, params, s, n_check, seed):
-    """rel-L2 and max error of the evaluated velocity against the float64
-    direct sum (same Oseen kernel) on n_check seeded targets."""
+class OracleReference:
+    """float64 velocity of the CPU oracle (restated direct Oseen sum,
+    direct.cpp:1409-1485, OpenMP on every host core) at n_check seeded
+    targets; computed once and reused by every variant."""
+
+    def __init__(self, s, n_check, seed):
+        from oracle import oracle as O
+        self.sub = W.target_subset(s["x"].size, n_check, seed=seed)
+        xopt, cutoff = W.oseen_params(s["sigma"])
+        t = time.perf_counter()
+        qr, qi = O.direct_oseen(s["x"], s["y"], s["w"], s["x"][self.sub], s["y"][self.sub],
+                                xopt, cutoff, nthreads=host_cores())
+        self.seconds = time.perf_counter() - t
+        self.ref = np.concatenate([-qi / (2 * math.pi), -qr / (2 * math.pi)])
+
+    def errors(self, vx, vy):
+        a = np.concatenate([vx[self.sub], vy[self.sub]])
+        b = self.ref
+        return (float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+
+
+def rel_errors(sysm, params, oracle_ref):
+    """rel-L2 and max error of the evaluated velocity against the oracle."""
     sysm.velocity(params, fetch=False)
     vx, vy = sysm.last_velocity()
-    sub = W.target_subset(s["x"].size, n_check, seed=seed)
-    xopt, cutoff = W.oseen_params(s["sigma"])
-    qr, qi = gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"][sub], s["y"][sub], 35, xopt, cutoff,
-                             0.0, prec=gauss.PREC_F64)
-    rx, ry = -qi / (2 * math.pi), -qr / (2 * math.pi)
-    a = np.concatenate([vx[sub], vy[sub]])
-    b = np.concatenate([rx, ry])
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b)), float(np.max(np.abs(a - b)) /
-                                                                   np.max(np.abs(b)))
+    return oracle_ref.errors(vx, vy)
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def host_cpu():
+    """Model name and usable core count of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cores_usable": host_cores(), "cores_online": os.cpu_count()}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fmm_1m_p2p_hbm_traffic.json")
@@ -111,8 +147,9 @@ def cpu_baseline(s, budget_s=40.0):
         dt = time.perf_counter() - t
         return {"value": n / dt, "unit": "evals/s", "cores": 1, "kind": "reference",
                 "sample": "full %d-blob workload, reference fmm2d CPU build (tol=1e-6, "
-                          "Ndirect=35, single-threaded as shipped), %.1f s" % (n, dt)}
-    cores = os.cpu_count() or 1
+                          "Ndirect=35, single-threaded as shipped), %.1f s" % (n, dt),
+                "host_cpu": host_cpu()}
+    cores = host_cores()
     m = 4096
     sub = W.target_subset(n, m, seed=0)
     t = time.perf_counter()
@@ -120,7 +157,8 @@ def cpu_baseline(s, budget_s=40.0):
                    nthreads=cores)
     dt = time.perf_counter() - t
     return {"value": m / dt, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": "%d seeded targets of the %d-blob workload, float64 direct sum" % (m, n)}
+            "sample": "%d seeded targets of the %d-blob workload, float64 direct sum" % (m, n),
+            "host_cpu": host_cpu()}
 
 
 def bench_lamb(args):
@@ -131,7 +169,7 @@ def bench_lamb(args):
     c = W.lamb_oseen_lattice()
     ks = c["k"] * c["sigma"] ** 2
     t = time.perf_counter()
-    ref = O.vv2par_gauss(c["x"], c["y"], c["w"], c["x"], c["y"], ks, nthreads=os.cpu_count() or 1)
+    ref = O.vv2par_gauss(c["x"], c["y"], c["w"], c["x"], c["y"], ks, nthreads=host_cores())
     t_cpu = time.perf_counter() - t
     res = {}
     for name, prec in (("f64", gauss.PREC_F64), ("f32c", gauss.PREC_F32C)):
@@ -153,13 +191,14 @@ def bench_lamb(args):
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "direct Gaussian sum, 4096 lattice blobs (Lamb-Oseen)",
                        "variants": res},
-            "roofline": {"kernel": "direct_f32c", "bound": "mfma",
+            "roofline": {"kernel": "direct_f32c", "bound": "valu",
                          "compute_pipe": "fp32 vector ALU (no matrix shape)", "achieved": ach,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
                          "traffic": None},
-            "cpu_baseline": {"value": n / t_cpu, "unit": "evals/s", "cores": os.cpu_count() or 1,
+            "cpu_baseline": {"value": n / t_cpu, "unit": "evals/s", "cores": host_cores(),
                              "kind": "port", "sample": "full 4096x4096 direct sum, float64, "
-                             "restatement of vv2parGauss_C_cpu (OpenMP)"}}
+                             "restatement of vv2parGauss_C_cpu (OpenMP)",
+                             "host_cpu": host_cpu()}}
 
 
 def bench_cylinder(args):
@@ -202,6 +241,101 @@ def bench_cylinder(args):
                        "circ_rel_change": abs(info["circ_after"] - info["circ_before"]) / tot,
                        "impulse_rel_change": max(abs(info["impulse_x_after"] - info["impulse_x_before"]),
                                                  abs(info["impulse_y_after"] - info["impulse_y_before"])) / tot}}
+
+
+def mix_leg_child(timeout_s=400):
+    """mix_leg in a child process (its own device context and memory), so
+    the headline line is printed whatever happens there."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "mixleg"]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after %d s" % timeout_s}
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": "exit %d: %s" % (r.returncode, r.stderr[-400:])}
+    return json.loads(lines[-1])
+
+
+def mix_leg(args, count=256, steps=10):
+    """The 256-system mix in compact form (N log-uniform in [1024, 262144],
+    seed 0, uniform blobs, h = 1/sqrt(N), sigma = h/0.8): batched F32C
+    velocity (order 16, leaf 64, one captured graph per evaluation) and the
+    batched step (velocity + Euler + M4' remesh), against one-at-a-time
+    float64 velocity of the same systems (order 16, leaf 64): one-shot
+    fmm2d calls with host arrays (per-call copies, as the reference's GPU
+    path) and resident systems evaluated one after another.  Eight systems
+    are checked against the CPU oracle on 1,024 seeded targets each."""
+    from oracle import oracle as O
+    from phyflow_amd.batch import BatchSystems
+    t_all = time.perf_counter()
+    sizes = W.uneven_sizes(count, 1024, 262144, seed=0)
+    systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
+    total = int(np.sum(sizes))
+    errs = []
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C, order=16,
+                      leaf=64) as b:
+        for i, s in enumerate(systems):
+            b.set_system(i, s["x"], s["y"], s["w"])
+        b.set_remesh(0.8, dt=1e-3)
+        for _ in range(2):
+            b.velocity(use_graph=True)
+        t = time.perf_counter()
+        for _ in range(steps):
+            b.velocity(use_graph=True)
+        wall_v = time.perf_counter() - t
+        for i in range(0, count, count // 8):
+            s = systems[i]
+            v = b.get_velocity(i)
+            sub = W.target_subset(s["x"].size, 1024, seed=i)
+            xopt, cutoff = W.oseen_params(s["sigma"])
+            qr, qi = O.direct_oseen(s["x"], s["y"], s["w"], s["x"][sub], s["y"][sub], xopt,
+                                    cutoff, nthreads=host_cores())
+            errs.append(O.rel_l2((v[0][sub], v[1][sub]),
+                                 (-qi / (2 * math.pi), -qr / (2 * math.pi))))
+        for _ in range(2):
+            b.step(use_graph=True)
+        t = time.perf_counter()
+        for _ in range(steps):
+            b.step(use_graph=True)
+        wall_s = time.perf_counter() - t
+    batched_v = total / (wall_v / steps)
+    # one-at-a-time float64: one-shot calls with host arrays
+    p64 = dict(prec=gauss.PREC_F64, order=16, leaf=64)
+    s0 = systems[0]
+    x0, c0 = W.oseen_params(s0["sigma"])
+    gauss.fmm2d_gpu(s0["x"], s0["y"], s0["w"], s0["x"], s0["y"], 35, x0, c0, 1e-6, **p64)
+    t = time.perf_counter()
+    for s in systems:
+        xopt, cutoff = W.oseen_params(s["sigma"])
+        gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"], s["y"], 35, xopt, cutoff, 1e-6, **p64)
+    t_oneshot = time.perf_counter() - t
+    # one-at-a-time float64: resident systems, one evaluation after another
+    resident = [BlobSystem(s["x"], s["y"], s["w"]) for s in systems]
+    try:
+        ps = [vel_params(s["sigma"], **p64) for s in systems]
+        for r, p in zip(resident, ps):
+            r.velocity(p, fetch=False)
+        _native.lib().phf_synchronize()
+        t = time.perf_counter()
+        for r, p in zip(resident, ps):
+            r.velocity(p, fetch=False)
+        t_res = time.perf_counter() - t
+    finally:
+        for r in resident:
+            r.close()
+    return {"workload": "%d systems, N log-uniform [1024, 262144] seed 0, total %d blobs"
+                        % (count, total),
+            "batched_f32c_velocity_evals_per_s": batched_v,
+            "batched_f32c_step_evals_per_s": total / (wall_s / steps),
+            "batched_steps_timed": steps,
+            "f64_one_shot_sequential_evals_per_s": total / t_oneshot,
+            "f64_resident_sequential_evals_per_s": total / t_res,
+            "speedup_batched_velocity_vs_f64_one_shot": batched_v / (total / t_oneshot),
+            "speedup_batched_velocity_vs_f64_resident": batched_v / (total / t_res),
+            "rel_l2_vs_oracle_max": max(errs), "checked_systems": len(errs),
+            "leg_seconds": time.perf_counter() - t_all}
 
 
 def bench_mix(args):
@@ -398,7 +532,8 @@ def bench_copy(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["fmm1m", "lamb", "cylinder", "mix", "serve", "copy"],
+    ap.add_argument("--workload", choices=["fmm1m", "lamb", "cylinder", "mix", "serve", "copy",
+                                           "mixleg"],
                     default="fmm1m")
     ap.add_argument("--sims", type=int, default=16)
     ap.add_argument("--slots", type=int, default=8)
@@ -416,6 +551,8 @@ def main():
     ap.add_argument("--prec", choices=["f32c", "f64"], default="f32c")
     ap.add_argument("--check", type=int, default=65536, help="targets for the error check (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mix", action="store_true",
+                    help="skip the 256-system batched vs one-at-a-time float64 leg")
     ap.add_argument("--variants", action="store_true", help="also time p in {8,16,24} x {f32c,f64}")
     args = ap.parse_args()
     if args.steps is None:
@@ -441,23 +578,37 @@ def main():
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)), flush=True)
         return 0
+    if args.workload == "mixleg":
+        print(json.dumps(mix_leg(args)), flush=True)
+        return 0
 
     s = W.uniform_system(args.n, seed=0, h=1.0 / 1024 if args.n == 1048576 else None)
     prec = gauss.PREC_F32C if args.prec == "f32c" else gauss.PREC_F64
     params = vel_params(s["sigma"], k=2.0, prec=prec, order=args.order, leaf=args.leaf)
     dev = _native.device_info()
+    oref = OracleReference(s, args.check, 0) if args.check else None
     with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
         wall, lat, p2p_ms, info = timed_steps(sysm, params, args.warmup, args.steps)
         ms_step = 1e3 * wall / args.steps
-        rel, mx = rel_errors(sysm, params, s, args.check, 0) if args.check else (None, None)
+        rel, mx = rel_errors(sysm, params, oref) if oref else (None, None)
+        # per-stage event times of one eager evaluation (every kernel
+        # launched on its own; the far field still runs beside the near field)
+        sysm.velocity(params, fetch=False, info=True, eager=True)
+        eager_ms = sysm.last_info["ms"]
+        # the same configuration in float64
+        p64 = vel_params(s["sigma"], k=2.0, prec=gauss.PREC_F64, order=args.order, leaf=args.leaf)
+        w64, lat64, _, _ = timed_steps(sysm, p64, 3, 10, counts=False)
+        e64 = rel_errors(sysm, p64, oref) if oref else (None, None)
+        f64_same = {"prec": "f64", "order": args.order, "ms_per_step": 1e2 * w64,
+                    "evals_per_s": args.n / (w64 / 10), "rel_l2": e64[0], "max_err": e64[1]}
         variants = []
         if args.variants:
             for pr in ("f32c", "f64"):
                 for p in (8, 16, 24):
                     vp = vel_params(s["sigma"], prec=gauss.PREC_F32C if pr == "f32c" else
                                     gauss.PREC_F64, order=p, leaf=args.leaf)
-                    vw, vl, _, _ = timed_steps(sysm, vp, 2, 5)
-                    e = rel_errors(sysm, vp, s, 8192, 1)
+                    vw, vl, _, _ = timed_steps(sysm, vp, 2, 5, counts=False)
+                    e = rel_errors(sysm, vp, oref) if oref else (None, None)
                     variants.append({"prec": pr, "order": p, "ms_per_step": 1e3 * vw / 5,
                                      "evals_per_s": args.n / (vw / 5), "rel_l2": e[0], "max_err": e[1]})
     # the one-shot entry point with host arrays (copies over PCIe both ways
@@ -473,6 +624,12 @@ def main():
     t_p2p = float(np.nanmean(p2p_ms)) * 1e-3
     pairs = info["p2p_pairs"]
     achieved = FLOP_PER_PAIR * pairs / t_p2p / 1e12
+    peak = FP32_PEAK_TFLOPS if prec == gauss.PREC_F32C else FP64_PEAK_TFLOPS
+    s_bytes = 4 if prec == gauss.PREC_F32C else 8
+    b_tree = args.n * (6 * s_bytes + 4)  # read x, y, w; write permuted x, y, w + int32 index
+    t_tree = info["ms"].get("tree", float("nan")) * 1e-3
+    m2l_flop = 8 * (args.order + 1) ** 2 * info["m2l_links"]
+    t_m2l = eager_ms.get("m2l", float("nan")) * 1e-3
     out = {
         "metric": "blob vel. evals/s @1M blobs, rel err<=1e-6 vs FP64; p50/p99 step latency",
         "value": args.n / (wall / args.steps),
@@ -490,6 +647,9 @@ def main():
         "data": "synthetic",
         "rel_l2_vs_fp64": rel,
         "max_err_vs_fp64": mx,
+        "error_reference": ("CPU oracle: float64 direct Oseen sum (restatement of "
+                            "direct.cpp:1409-1485) at %d seeded targets, %d host threads, %.1f s"
+                            % (args.check, host_cores(), oref.seconds)) if oref else None,
         "config": {
             "workload": "FMM velocity of %d uniform blobs at their centres (seed 0, h=1/1024, "
                         "sigma=h/0.8), order %d, leaf %d, Oseen cutoff 5*xopt, %s; error vs "
@@ -497,21 +657,36 @@ def main():
                         % (args.n, args.order, args.leaf, args.prec, args.check),
             "n_blobs": args.n, "order": args.order, "leaf": args.leaf, "levels": info["L"],
             "near_boxes": info["nnear"], "p2p_pairs": pairs, "m2l_links": info["m2l_links"],
-            "stage_ms": info["ms"], "device": dev["name"],
+            "stage_ms": info["ms"], "eager_stage_ms": eager_ms, "device": dev["name"],
             "pcie_inclusive_evals_per_s": pcie_rate,
+            "f64_same_config": f64_same,
         },
         "roofline": {
             "kernel": "p2p_near_f32s" if prec == gauss.PREC_F32C else "p2p_near_f64s",
-            "bound": "mfma",
-            "compute_pipe": "fp32 vector ALU (the pair kernel has no matrix shape); MI355X "
-                            "fp32 peak 157.3 TFLOP/s, fp64 78.6",
+            "bound": "valu",
+            "compute_pipe": ("fp32" if prec == gauss.PREC_F32C else "fp64") +
+                            " vector ALU (the pair kernel has no matrix shape); MI355X vector "
+                            "peak fp32 157.3 TFLOP/s, fp64 78.6",
             "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS if prec == gauss.PREC_F32C else 78.6,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": achieved / (FP32_PEAK_TFLOPS if prec == gauss.PREC_F32C else 78.6),
+            "frac": achieved / peak,
             "traffic": None,
             "flop_per_pair": FLOP_PER_PAIR,
             "kernel_ms": t_p2p * 1e3,
+        },
+        "roofline_other": {
+            "tree": {"kernels": "tree graph (bbox, keys, 2 radix passes, leaf offsets)",
+                     "bound": "hbm", "algorithmic_bytes": b_tree,
+                     "bytes_rule": "N*(6s+4), s=%d" % s_bytes, "ms": t_tree * 1e3,
+                     "achieved": b_tree / t_tree / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": b_tree / t_tree / 1e9 / HBM_PEAK_GBS},
+            "m2l": {"kernel": "m2l_box_kernel" if prec == gauss.PREC_F32C else "m2l_kernel",
+                    "bound": "valu (no MFMA shape is used)", "flop": m2l_flop,
+                    "flop_rule": "8(p+1)^2 per far link", "ms": t_m2l * 1e3,
+                    "note": "eager event time, overlapping the near field",
+                    "achieved": m2l_flop / t_m2l / 1e12, "peak": peak, "unit": "TFLOP/s",
+                    "frac": m2l_flop / t_m2l / 1e12 / peak},
         },
     }
     tr = measured_traffic(out["roofline"]["kernel"], args)
@@ -520,6 +695,8 @@ def main():
         out["roofline"]["traffic_source"] = tr["source"]
     if variants:
         out["variants"] = variants
+    if not args.no_mix:
+        out["config4_mix"] = mix_leg_child()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)

@@ -1170,6 +1170,7 @@ int phf_batch_step(phf_batch *b, int32_t use_graph, phf_batch_info *info)
                            L->stream));
     PHF_HIP(hipMemcpyAsync(hp + 2 * S, b->buf.flag, sizeof(int32_t), hipMemcpyDeviceToHost,
                            L->stream));
+    PHF_TRY(L->pin.mark(L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
@@ -1193,7 +1194,7 @@ int phf_batch_step(phf_batch *b, int32_t use_graph, phf_batch_info *info)
         LaneGuard g;
         PHF_TRY(g.rc);
         Lane *T = g.lane;
-        PHF_TRY(T->arena.reset());
+        PHF_TRY(reset_arena(T));
         const BatchRm &r = b->rplan.rm[s];
         const int64_t o = b->plan.sys[s].off, n = b->plan.sys[s].n;
         const double *xs = b->rplan.p.dt != 0.0 ? b->rbuf.xn : b->x;

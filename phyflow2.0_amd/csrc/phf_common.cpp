@@ -119,19 +119,50 @@ size_t Arena::capacity() const
 
 int Pinned::ensure(size_t bytes)
 {
+    if (pending) {
+        PHF_HIP(hipEventSynchronize(last));
+        pending = false;
+    }
     if (bytes <= cap) return PHF_OK;
-    release();
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    cap = 0;
     size_t want = (bytes + (size_t(1) << 20) - 1) & ~((size_t(1) << 20) - 1);
     PHF_HIP(hipHostMalloc(&ptr, want, hipHostMallocDefault));
     cap = want;
     return PHF_OK;
 }
 
+int Pinned::mark(hipStream_t s)
+{
+    if (!last) PHF_HIP(hipEventCreateWithFlags(&last, hipEventDisableTiming));
+    PHF_HIP(hipEventRecord(last, s));
+    pending = true;
+    return PHF_OK;
+}
+
 void Pinned::release()
 {
+    if (pending) (void)hipEventSynchronize(last);
+    pending = false;
     if (ptr) (void)hipHostFree(ptr);
+    if (last) (void)hipEventDestroy(last);
     ptr = nullptr;
+    last = nullptr;
     cap = 0;
+}
+
+int lane_quiesce(Lane *L)
+{
+    if (L->stream) PHF_HIP(hipStreamSynchronize(L->stream));
+    if (L->stream2) PHF_HIP(hipStreamSynchronize(L->stream2));
+    return PHF_OK;
+}
+
+int reset_arena(Lane *L)
+{
+    if (L->arena.reset_frees()) PHF_TRY(lane_quiesce(L));
+    return L->arena.reset();
 }
 
 // --------------------------------------------------------------- context --

@@ -81,6 +81,7 @@ struct Arena {
     size_t peak = 0;  // largest pass so far
 
     int reset();
+    bool reset_frees() const { return blocks.size() > 1; }  // reset() would free blocks
     void *alloc_bytes(size_t bytes);  // nullptr + error on failure
     // allocation state: a launch sequence recorded from state m0 to m1 can
     // be replayed (same slices) when the arena is at m0 again, by jump(m1)
@@ -105,11 +106,18 @@ struct Arena {
         if (!(ptr)) return PHF_ERR_NOMEM;                                    \
     } while (0)
 
-// Pinned host staging buffer (grown on demand).
+// Pinned host staging buffer (grown on demand).  Copies that read or write
+// it are asynchronous, so the host neither writes into it nor frees it
+// while one may still be pending: mark() records an event after the copies
+// enqueued on a stream, and ensure() (before every host use) and release()
+// wait for that event first.
 struct Pinned {
     char *ptr = nullptr;
     size_t cap = 0;
+    hipEvent_t last = nullptr;
+    bool pending = false;
     int ensure(size_t bytes);
+    int mark(hipStream_t s);
     void release();
 };
 
@@ -131,6 +139,13 @@ struct Lane {
     hipStream_t stream2 = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
+
+// Waits for all work enqueued on the lane's streams (its stream and, once
+// created, its second stream).
+int lane_quiesce(Lane *L);
+// Arena::reset for a lane: when the reset frees device blocks, the lane's
+// streams are waited for first, so no enqueued kernel still uses them.
+int reset_arena(Lane *L);
 
 class Context {
   public:

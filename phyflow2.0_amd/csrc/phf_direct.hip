@@ -559,7 +559,7 @@ static int vv2par_entry(int kind, const double *xb, const double *yb, const doub
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     StageTimer tm(L, timing_enabled());
     bool used[PHF_STAGE_COUNT] = {};
     used[PHF_STAGE_UPLOAD] = used[PHF_STAGE_P2P] = used[PHF_STAGE_DOWNLOAD] = true;
@@ -633,7 +633,7 @@ static int ww2par_entry(bool cutoff, const double *xb, const double *yb, const d
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     double *dxb, *dyb, *dwb, *dxt, *dyt, *o;
     PHF_TRY(upload(L, xb, nb, &dxb));
     PHF_TRY(upload(L, yb, nb, &dyb));

@@ -969,6 +969,7 @@ struct TableBlob {
             PHF_ALLOC(dev, L->arena, char, host.size());
             PHF_HIP(hipMemcpyAsync(dev, L->pin.ptr, host.size(), hipMemcpyHostToDevice,
                                    L->stream));
+            PHF_TRY(L->pin.mark(L->stream));
         }
         for (auto &f : fix) *f.second = dev + f.first;
         return PHF_OK;
@@ -1555,8 +1556,11 @@ struct FmmGraphCache {
 
 constexpr size_t kFmmGraphs = 4;
 
-static void drop_graph(FmmGraph *e)
+// Destroys a recording and frees its tables; the lane's streams are waited
+// for first, so no launch of the recording still reads them.
+static void drop_graph(Lane *L, FmmGraph *e)
 {
+    (void)lane_quiesce(L);
     for (int k = 0; k < 2; ++k) {
         if (e->exec[k]) (void)hipGraphExecDestroy(e->exec[k]);
         if (e->g[k]) (void)hipGraphDestroy(e->g[k]);
@@ -1567,6 +1571,7 @@ static void drop_graph(FmmGraph *e)
 
 void fmm_graphs_release(Lane *L)
 {
+    (void)lane_quiesce(L);
     if (L->stream2) {
         (void)hipStreamSynchronize(L->stream2);
         (void)hipStreamDestroy(L->stream2);
@@ -1576,7 +1581,7 @@ void fmm_graphs_release(Lane *L)
     if (L->join_ev) (void)hipEventDestroy(L->join_ev);
     L->fork_ev = L->join_ev = nullptr;
     if (!L->fmm_graphs) return;
-    for (FmmGraph *e : L->fmm_graphs->v) drop_graph(e);
+    for (FmmGraph *e : L->fmm_graphs->v) drop_graph(L, e);
     if (L->fmm_graphs->hb_host) (void)hipHostFree(L->fmm_graphs->hb_host);
     delete L->fmm_graphs;
     L->fmm_graphs = nullptr;
@@ -1666,7 +1671,7 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
         }
     }
     if (rc != PHF_OK) {
-        drop_graph(e);
+        drop_graph(L, e);
         return rc;
     }
     e->m1 = L->arena.mark();
@@ -1680,7 +1685,7 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
         for (auto it = c.v.begin(); it != c.v.end();)
             if (same_args((*it)->a, a) && (*it)->m0.base0 == e->m0.base0 &&
                 (*it)->m0.cur == e->m0.cur && (*it)->m0.top == e->m0.top) {
-                drop_graph(*it);
+                drop_graph(L, *it);
                 it = c.v.erase(it);
             } else {
                 ++it;
@@ -1689,7 +1694,7 @@ static int fmm_record(Lane *L, const FmmArgs &a, const Domain &dom0, FmmGraph **
             auto lru = std::min_element(c.v.begin(), c.v.end(), [](FmmGraph *x, FmmGraph *y) {
                 return x->last < y->last;
             });
-            drop_graph(*lru);
+            drop_graph(L, *lru);
             c.v.erase(lru);
         }
         c.v.push_back(e);
@@ -1809,9 +1814,9 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         dom.D = e->D;
         if (!keep) {
             // the graphs and their tables must outlive the launches
-            const hipError_t se = hipStreamSynchronize(L->stream);
-            drop_graph(e);
-            if (rc == PHF_OK && se != hipSuccess) PHF_HIP(se);
+            const int qr = lane_quiesce(L);
+            drop_graph(L, e);
+            if (rc == PHF_OK) PHF_TRY(qr);
         }
         PHF_TRY(rc);
         used[PHF_STAGE_TREE_FAR] = used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
@@ -1863,7 +1868,7 @@ extern "C" int phf_fmm2d(const double *xb, const double *yb, const double *wb, i
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     StageTimer tm(L, timing_enabled());
     bool used[PHF_STAGE_COUNT] = {};
     used[PHF_STAGE_UPLOAD] = used[PHF_STAGE_DOWNLOAD] = used[PHF_STAGE_TOTAL] = true;

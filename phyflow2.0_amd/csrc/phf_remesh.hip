@@ -483,6 +483,7 @@ int population_control_device(Lane *L, const double *x, const double *y, const d
     PHF_HIP(hipGetLastError());
     PHF_HIP(hipMemcpyAsync(hp + PC_K, loss, sizeof(double) * PC_K, hipMemcpyDeviceToHost,
                            L->stream));
+    PHF_TRY(L->pin.mark(L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
     double t = 0.0;  // no acceptable threshold among the candidates: keep all
     for (int k = 0; k < PC_K; ++k)
@@ -529,7 +530,7 @@ int phf_m4kernel_1d(const double *x, int64_t n, double c, double *w)
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     double *dx, *dw;
     PHF_TRY(upload(L, x, n, &dx));
     PHF_ALLOC(dw, L->arena, double, (size_t)n);
@@ -586,7 +587,7 @@ static int regrid_entry(int kind, const double *x, const double *y, const double
     }
     Lane *L = g.lane;
     int rc = [&]() -> int {
-        PHF_TRY(L->arena.reset());
+        PHF_TRY(reset_arena(L));
         StageTimer tm(L, timing_enabled());
         bool used[PHF_STAGE_COUNT] = {};
         used[PHF_STAGE_REMESH] = used[PHF_STAGE_TOTAL] = true;
@@ -670,7 +671,7 @@ int phf_population_control(const double *x, const double *y, const double *w, in
     }
     Lane *L = g.lane;
     int rc = [&]() -> int {
-        PHF_TRY(L->arena.reset());
+        PHF_TRY(reset_arena(L));
         double *dx, *dy, *dw, *xo, *yo, *wo;
         PHF_TRY(upload(L, x, n, &dx));
         PHF_TRY(upload(L, y, n, &dy));

@@ -331,7 +331,7 @@ int phf_panels_solve(phf_panels *P, const double *vx, const double *vy, double *
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     double *dvx, *dvy;
     PHF_TRY(upload(L, vx, P->m, &dvx));
     PHF_TRY(upload(L, vy, P->m, &dvy));
@@ -366,7 +366,7 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
     if (P) lkp = std::unique_lock<std::mutex>(P->mu);
     PHF_HIP(hipSetDevice(Context::get().device()));
     Lane *L = &s->lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     auto t0 = std::chrono::steady_clock::now();
     StageTimer tm(L, false);
     bool used[PHF_STAGE_COUNT] = {};

@@ -13,6 +13,8 @@ namespace phf {
 int sys_reserve(System *s, int64_t n)
 {
     if (n <= s->cap) return PHF_OK;
+    // work enqueued on the system's streams may still read the old buffers
+    PHF_TRY(lane_quiesce(&s->lane));
     double *bufs[5] = {s->x, s->y, s->w, s->vx, s->vy};
     const bool grow = s->cap > 0;
     for (double *b : bufs)
@@ -230,7 +232,7 @@ int phf_system_velocity(phf_system *h, const phf_vel_params *p, double *vx, doub
     // recorded once and replayed; a replay whose points needed another
     // domain is made again after a fresh recording
     for (int attempt = 0; attempt < 2; ++attempt) {
-        PHF_TRY(L->arena.reset());
+        PHF_TRY(reset_arena(L));
         for (bool &u : used) u = false;
         used[PHF_STAGE_TOTAL] = true;
         fi = FmmInfo{};

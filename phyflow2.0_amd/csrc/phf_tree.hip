@@ -641,7 +641,7 @@ extern "C" int phf_tree_build(const double *x, const double *y, int64_t n, int32
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *Ln = g.lane;
-    PHF_TRY(Ln->arena.reset());
+    PHF_TRY(reset_arena(Ln));
     double *dx, *dy;
     PHF_TRY(upload(Ln, x, n, &dx));
     PHF_TRY(upload(Ln, y, n, &dy));

@@ -1,0 +1,192 @@
+"""The benchmark configurations at their stated sizes, against the CPU
+oracle (not against another GPU path).
+
+  config 2  1,048,576 uniform blobs (seed 0, h = 1/1024, sigma = h/0.8),
+            FMM at orders 8, 16 and 24 with leaf 64, float64 and F32C:
+            velocity at a seeded 4,096-target subset (seed 1) against the
+            oracle's float64 direct Oseen sum (direct.cpp:1409-1485).
+  config 3  512-panel cylinder + 262,144 annulus blobs: one coupled RK2
+            step (F32C FMM order 16) against the CPU restatement of the step
+            (oracle blob and panel velocities, float64 least squares) at a
+            4,096-blob subset, and the float64 invariants of the step with
+            the M4' remesh.
+  config 4  256 uneven systems batched (F32C, one graph): eight of them
+            against the oracle direct sum, remesh bit-exact against the
+            oracle remesh of the same convected positions.
+  config 5  one served request of each accuracy class (F32C order 16;
+            float64 order 30, ws 2) on a 10,000-blob simulation: RK2 + M4'
+            remesh + population control against the CPU restatement.
+
+Tolerances: the mixed path 1e-6 relative L2 at order >= 16; float64 per
+order (the truncation error of the expansion: 1e-4 at p = 8, 1e-7 at
+p = 16, 1e-10 at p = 24, 1e-12 at p = 30 with ws = 2)."""
+import math
+
+import numpy as np
+import pytest
+
+from phyflow_amd import workloads as W
+from phyflow_amd._native import PREC_F32C, PREC_F64
+from phyflow_amd.lagrangian import PanelBody, step_rk2
+from phyflow_amd.system import BlobSystem, vel_params
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+TWO_PI = 2.0 * math.pi
+
+
+def _velocity_oracle(O, s, xt, yt):
+    xopt, cutoff = W.oseen_params(s["sigma"])
+    qr, qi = O.direct_oseen(s["x"], s["y"], s["w"], xt, yt, xopt, cutoff)
+    return -qi / TWO_PI, -qr / TWO_PI
+
+
+# ------------------------------------------------------------- config 2 ---
+@pytest.fixture(scope="module")
+def config2(O):
+    s = W.uniform_system(1048576, seed=0, h=1.0 / 1024)
+    sub = W.target_subset(s["x"].size, 4096, seed=0)  # drawn from stream seed + 1
+    ref = _velocity_oracle(O, s, s["x"][sub], s["y"][sub])
+    with BlobSystem(s["x"], s["y"], s["w"]) as sysm:
+        yield s, sub, ref, sysm
+
+
+@pytest.mark.parametrize("prec,order,l2,mx", [
+    (PREC_F32C, 8, 1e-4, 1e-3),
+    (PREC_F32C, 16, 1e-6, 1e-5),
+    (PREC_F32C, 24, 1e-6, 1e-5),
+    (PREC_F64, 8, 1e-4, 1e-3),
+    (PREC_F64, 16, 1e-7, 1e-6),
+    (PREC_F64, 24, 1e-10, 1e-9),
+])
+def test_config2_fmm_1m_blobs_vs_oracle(config2, O, prec, order, l2, mx):
+    s, sub, ref, sysm = config2
+    vx, vy = sysm.velocity(vel_params(s["sigma"], prec=prec, order=order, leaf=64))
+    got = (vx[sub], vy[sub])
+    e2, em = O.rel_l2(got, ref), O.max_rel(got, ref)
+    print("config2 prec=%d p=%d rel_l2=%.3e max=%.3e" % (prec, order, e2, em), flush=True)
+    assert e2 <= l2 and em <= mx
+
+
+# ------------------------------------------------------------- config 3 ---
+def _panel_strengths_cpu(O, a, g, sigma, vinf):
+    """'prescribed' solve (__panelsClass.py:800-822): least squares of
+    [A; L] s = [-(v . t); 0] with the slip velocity of blobs + free stream
+    at the collocation points."""
+    A = O.panel_influence(g["xcp"], g["ycp"], g["xs"], g["ys"], g["xe"], g["ye"], g["cosA"],
+                          g["sinA"])
+    vx, vy = _velocity_oracle(O, dict(a, sigma=sigma), g["xcp"], g["ycp"])
+    rhs = -(vx + vinf[0]) * g["cosA"] - (vy + vinf[1]) * g["sinA"]
+    return np.linalg.lstsq(np.vstack([A, g["L"][None, :]]), np.concatenate([rhs, [0.0]]),
+                           rcond=None)[0]
+
+
+def _stage_cpu(O, src, sigma, g, sp, vinf, xt, yt):
+    vx, vy = _velocity_oracle(O, dict(src, sigma=sigma), xt, yt)
+    px, py = O.panel_induced(sp, xt, yt, g["xs"], g["ys"], g["xe"], g["ye"], g["cosA"], g["sinA"])
+    return vx + px + vinf[0], vy + py + vinf[1]
+
+
+def test_config3_cylinder_step_full_size(O):
+    sigma, dt, vinf = 0.004, 1e-3, (1.0, 0.0)
+    xp, yp = W.cylinder_panels(512, R=0.5)
+    g = W.panel_geometry(xp, yp)
+    a = W.annulus_blobs(262144, sigma=sigma, seed=0)
+    body = PanelBody(xp, yp, "prescribed", 0.0)
+    p = vel_params(sigma, prec=PREC_F32C, order=16, leaf=64)
+    with BlobSystem(a["x"], a["y"], a["w"]) as s:
+        step_rk2(s, p, dt, vInf=vinf, panels=body)
+        xg, yg, _ = s.get()
+    # CPU restatement: stage 1 at every blob, stage 2 at the subset
+    sp = _panel_strengths_cpu(O, a, g, sigma, vinf)
+    k1 = _stage_cpu(O, a, sigma, g, sp, vinf, a["x"], a["y"])
+    moved = dict(x=a["x"] + dt * k1[0], y=a["y"] + dt * k1[1], w=a["w"])
+    sub = W.target_subset(a["x"].size, 4096, seed=3)
+    k2 = _stage_cpu(O, moved, sigma, g, sp, vinf, moved["x"][sub], moved["y"][sub])
+    xr = a["x"][sub] + 0.5 * dt * (k1[0][sub] + k2[0])
+    yr = a["y"][sub] + 0.5 * dt * (k1[1][sub] + k2[1])
+    ref = (xr - a["x"][sub] - vinf[0] * dt, yr - a["y"][sub] - vinf[1] * dt)
+    got = (xg[sub] - a["x"][sub] - vinf[0] * dt, yg[sub] - a["y"][sub] - vinf[1] * dt)
+    e = O.rel_l2(got, ref)
+    print("config3 displacement rel_l2=%.3e" % e, flush=True)
+    assert e <= 1e-6
+    # the same step with the M4' remesh onto h = 0.005: float64 invariants
+    with BlobSystem(a["x"], a["y"], a["w"]) as s:
+        info = step_rk2(s, p, dt, vInf=vinf, panels=body, remesh=dict(sigma=sigma, overlap=1.25))
+    tot = float(np.sum(np.abs(a["w"])))
+    assert abs(info["circ_after"] - info["circ_before"]) <= 1e-13 * tot
+    assert abs(info["impulse_x_after"] - info["impulse_x_before"]) <= 1e-13 * 1.5 * tot
+    assert abs(info["impulse_y_after"] - info["impulse_y_before"]) <= 1e-13 * 1.5 * tot
+    body.close()
+
+
+# ------------------------------------------------------------- config 4 ---
+def test_config4_mix_256_systems(O):
+    from phyflow_amd.batch import BatchSystems
+    sizes = W.uneven_sizes(256, 1024, 262144, seed=0)
+    systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
+    dt, overlap = 1e-3, 0.8
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C, order=16,
+                      leaf=64) as b:
+        for i, s in enumerate(systems):
+            b.set_system(i, s["x"], s["y"], s["w"])
+        b.set_remesh(overlap, dt=dt)
+        b.step(use_graph=True)
+        for i in range(0, 256, 32):
+            s = systems[i]
+            v = b.get_velocity(i)
+            sub = W.target_subset(s["x"].size, 1024, seed=i)
+            ref = _velocity_oracle(O, s, s["x"][sub], s["y"][sub])
+            e = O.rel_l2((v[0][sub], v[1][sub]), ref)
+            assert e <= 1e-6, (i, e)
+            h = s["sigma"] * overlap
+            o = O.regrid_m4(s["x"] + dt * v[0], s["y"] + dt * v[1], s["w"], s["sigma"], overlap,
+                            -1.5 * h, -1.5 * h)
+            xr, yr, wr = b.get_remeshed(i)
+            assert np.array_equal(xr, o[0]) and np.array_equal(yr, o[1]), i
+            assert O.rel_l2(wr, o[2]) <= 1e-15, i
+
+
+# ------------------------------------------------------------- config 5 ---
+def _served_step_cpu(O, s, dt, overlap, pc):
+    k1 = _velocity_oracle(O, s, s["x"], s["y"])
+    moved = dict(x=s["x"] + dt * k1[0], y=s["y"] + dt * k1[1], w=s["w"], sigma=s["sigma"])
+    k2 = _velocity_oracle(O, moved, moved["x"], moved["y"])
+    xn = s["x"] + 0.5 * dt * (k1[0] + k2[0])
+    yn = s["y"] + 0.5 * dt * (k1[1] + k2[1])
+    h = s["sigma"] * overlap
+    xo, yo, wo, _, _ = O.regrid_m4(xn, yn, s["w"], s["sigma"], overlap, -1.5 * h, -1.5 * h)
+    xo, yo, wo, _ = O.population_control(xo, yo, wo, pc[0], pc[1])
+    return xo, yo, wo
+
+
+@pytest.mark.parametrize("cls", ["f32c_p16", "f64_p30_ws2"])
+def test_config5_served_request_vs_cpu(O, cls):
+    from phyflow_amd.serve import Server
+    s = W.uniform_system(10000, seed=2000)
+    if cls == "f32c_p16":
+        p, tol = vel_params(s["sigma"], prec=PREC_F32C, order=16, leaf=64), 1e-6
+    else:
+        p, tol = vel_params(s["sigma"], prec=PREC_F64, order=30, leaf=64, ws=2), 1e-12
+    dt, overlap, pc = 1e-4, 0.8, (1e-8, 1e-8)
+    remesh = dict(sigma=s["sigma"], overlap=overlap, population_control=pc)
+    with BlobSystem(s["x"], s["y"], s["w"]) as sysm, Server(slots=2) as srv:
+        sid = srv.add(sysm)
+        res = srv.wait(srv.submit(sid, p, dt, remesh=remesh))
+        assert res["status"] == 0
+        srv.remove(sid)
+        xg, yg, wg = sysm.get()
+    xr, yr, wr = _served_step_cpu(O, s, dt, overlap, pc)
+    # lattice nodes are exact; a node whose weight sits at the threshold
+    # may be kept on one side only, so compare on the union of nodes
+    key = lambda x, y: np.round(x / (s["sigma"] * overlap)).astype(np.int64) * (1 << 32) + \
+        np.round(y / (s["sigma"] * overlap)).astype(np.int64)
+    kg, kr = key(xg, yg), key(xr, yr)
+    allk = np.union1d(kg, kr)
+    a, b = np.zeros(allk.size), np.zeros(allk.size)
+    a[np.searchsorted(allk, kg)] = wg
+    b[np.searchsorted(allk, kr)] = wr
+    e = O.rel_l2(a, b)
+    print("config5 %s nodes %d/%d rel_l2(w)=%.3e" % (cls, xg.size, xr.size, e), flush=True)
+    assert e <= tol
+    assert abs(xg.size - xr.size) <= max(2, xr.size // 1000)
