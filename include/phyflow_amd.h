@@ -242,6 +242,14 @@ typedef struct {
     double total_ms;    /* wall time of the call */
 } phf_solve_info;
 
+/* The factorisation behind phf_dense_solve on its own: float32 LU with
+ * partial pivoting of the column-major n x n A, in LAPACK getrf form (the
+ * factor the reference takes through scipy.linalg.lu_factor,
+ * __panelsClass.py:961): LU holds U and the unit-lower L, ipiv[k] (0-based)
+ * the row exchanged with row k at step k, info = k + 1 for the first zero
+ * pivot U[k][k] (0 if none). */
+int phf_lu_factor_f32(const float *A, int64_t n, float *LU, int32_t *ipiv, int32_t *info);
+
 int phf_dense_solve(const double *B, int64_t rows, int64_t n, const double *c, double *x,
                     phf_solve_info *info);
 
@@ -322,6 +330,7 @@ int phf_system_get_velocity(phf_system *s, double *vx, double *vy);
 typedef struct phf_panels_s phf_panels;
 #define PHF_PANELS_FIXED 0
 #define PHF_PANELS_PRESCRIBED 1
+#define PHF_PANELS_MOVING 2  /* square system A s = RHS, rebuilt by phf_panels_update */
 
 int phf_panels_create(const double *xs, const double *ys, const double *xe, const double *ye,
                       int64_t m, int32_t problem, double gTotal, phf_panels **out);
@@ -332,6 +341,22 @@ int phf_panels_info(phf_panels *p, int64_t *m, double *xcp, double *ycp, double 
  * cached factors: strengths for RHS = -(v . t). */
 int phf_panels_solve(phf_panels *p, const double *vx, const double *vy, double *strength,
                      int32_t *iterations);
+
+/* Device times (ms) of one rebuild of a panel body. */
+typedef struct {
+    double influence_ms; /* influence matrix (+ Kelvin row) */
+    double normal_ms;    /* float64 copies, B^T B for 'prescribed' */
+    double lu_ms;        /* float32 LU with partial pivoting (matrix cores) */
+    double inverse_ms;   /* float32 inverse from the factors */
+    double total_ms;     /* all of the above + grouped-velocity factors */
+} phf_panels_timing;
+
+/* Panels.updateBody (__panelsClass.py:530-638): new start/end points of
+ * the same m panels (a moving or deforming body).  Geometry, collocation
+ * points, the influence matrix (__assembleInfluenceMatrix :872-961) and the
+ * factors are rebuilt on the device; timing may be NULL. */
+int phf_panels_update(phf_panels *p, const double *xs, const double *ys, const double *xe,
+                      const double *ye, phf_panels_timing *timing);
 
 typedef struct {
     double dt;

@@ -49,6 +49,14 @@ class SolveInfo(ctypes.Structure):
                 ("total_ms", _dbl)]
 
 
+class PanelsTiming(ctypes.Structure):
+    _fields_ = [("influence_ms", _dbl), ("normal_ms", _dbl), ("lu_ms", _dbl),
+                ("inverse_ms", _dbl), ("total_ms", _dbl)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class StepParams(ctypes.Structure):
     _fields_ = [("dt", _dbl), ("vinf_x", _dbl), ("vinf_y", _dbl), ("panel_update", _i32),
                 ("remesh", _i32), ("remesh_sigma", _dbl), ("remesh_overlap", _dbl),
@@ -150,11 +158,14 @@ _SIGS = {
     "phf_blobset_free": ([_ptr], None),
     "phf_panel_influence": ([_ptr] * 8 + [_i64, _ptr], ctypes.c_int),
     "phf_panel_induced_velocity": ([_ptr, _i64, _ptr, _ptr, _i64] + [_ptr] * 8, ctypes.c_int),
+    "phf_lu_factor_f32": ([_ptr, _i64, _ptr, _ptr, ctypes.POINTER(_i32)], ctypes.c_int),
     "phf_dense_solve": ([_ptr, _i64, _i64, _ptr, _ptr, ctypes.POINTER(SolveInfo)], ctypes.c_int),
     "phf_panels_create": ([_ptr, _ptr, _ptr, _ptr, _i64, _i32, _dbl, ctypes.POINTER(_ptr)],
                           ctypes.c_int),
     "phf_panels_destroy": ([_ptr], ctypes.c_int),
     "phf_panels_info": ([_ptr, ctypes.POINTER(_i64), _ptr, _ptr, _ptr], ctypes.c_int),
+    "phf_panels_update": ([_ptr, _ptr, _ptr, _ptr, _ptr, ctypes.POINTER(PanelsTiming)],
+                          ctypes.c_int),
     "phf_panels_solve": ([_ptr, _ptr, _ptr, _ptr, ctypes.POINTER(_i32)], ctypes.c_int),
     "phf_system_step_rk2": ([_ptr, ctypes.POINTER(VelParams), _ptr, ctypes.POINTER(StepParams),
                              ctypes.POINTER(StepInfo)], ctypes.c_int),

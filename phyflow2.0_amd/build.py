@@ -44,13 +44,17 @@ def common_flags():
     ]
 
 
+# per-file additions: the LU panel loop is unrolled in full (32 columns)
+FILE_FLAGS = {"phf_panel.hip": ["-mllvm", "-pragma-unroll-threshold=200000"]}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [hipcc()] + common_flags()
+    cmd = [hipcc()] + common_flags() + FILE_FLAGS.get(os.path.basename(src), [])
     if src.endswith(".cpp"):
         cmd += ["-x", "hip"]
     cmd += ["-c", src, "-o", obj]

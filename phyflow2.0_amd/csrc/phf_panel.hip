@@ -166,17 +166,48 @@ __global__ void to_colmajor(const double *__restrict__ B, int64_t rows, int64_t 
 }
 
 // N = B^T B (n x n, column-major, F64 and F32 copies); Bc column-major rows x n.
-__global__ void normal_matrix(const double *__restrict__ Bc, int64_t rows, int64_t n,
-                              double *__restrict__ N64, float *__restrict__ N32)
+// One workgroup per 32 x 32 tile of N: the 32 columns of B on each side are
+// staged in LDS 32 rows at a time and each thread adds 2 x 2 outputs, every
+// output summed over the rows in order (one FMA chain).
+constexpr int NM_T = 32;
+__global__ __launch_bounds__(256) void normal_matrix(const double *__restrict__ Bc, int64_t rows,
+                                                     int64_t n, double *__restrict__ N64,
+                                                     float *__restrict__ N32)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n * n) return;
-    const int64_t j = t / n, i = t - (t / n) * n;
-    const double *ci = Bc + i * rows, *cj = Bc + j * rows;
-    double s = 0.0;
-    for (int64_t r = 0; r < rows; ++r) s = __builtin_fma(ci[r], cj[r], s);
-    N64[t] = s;
-    N32[t] = (float)s;
+    __shared__ double Ti[NM_T][NM_T + 1];  // [column of the i side][row]
+    __shared__ double Tj[NM_T][NM_T + 1];
+    const int64_t i0 = (int64_t)blockIdx.x * NM_T, j0 = (int64_t)blockIdx.y * NM_T;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int64_t r0 = 0; r0 < rows; r0 += NM_T) {
+        for (int e = threadIdx.x; e < NM_T * NM_T; e += 256) {
+            const int c = e / NM_T, k = e % NM_T;
+            const int64_t r = r0 + k;
+            Ti[c][k] = (i0 + c < n && r < rows) ? Bc[(i0 + c) * rows + r] : 0.0;
+            Tj[c][k] = (j0 + c < n && r < rows) ? Bc[(j0 + c) * rows + r] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < NM_T; ++k) {
+            const double a0 = Ti[tx][k], a1 = Ti[tx + 16][k];
+            const double b0 = Tj[ty][k], b1 = Tj[ty + 16][k];
+            acc[0][0] = __builtin_fma(a0, b0, acc[0][0]);
+            acc[1][0] = __builtin_fma(a1, b0, acc[1][0]);
+            acc[0][1] = __builtin_fma(a0, b1, acc[0][1]);
+            acc[1][1] = __builtin_fma(a1, b1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int64_t i = i0 + tx + 16 * a, j = j0 + ty + 16 * b;
+            if (i < n && j < n) {
+                N64[j * n + i] = acc[a][b];
+                N32[j * n + i] = (float)acc[a][b];
+            }
+        }
 }
 
 // --------------------------------------------------------------- LU -------
@@ -326,6 +357,311 @@ __global__ __launch_bounds__(64) void lu_update_mfma(float *__restrict__ A, int 
     }
 }
 
+// -------------------------------------------- blocked LU, n <= 1024 ------
+// Right-looking LU with partial pivoting in panels of NB columns, one
+// launch per panel step, the next panel factored inside the same launch:
+//   lu_first     one workgroup factors panel 0;
+//   lu_step(k)   one workgroup per NB-column block j of the matrix:
+//                  j <  k   the row swaps of panel k on its columns (L part)
+//                  j == k   nothing (factored by the launch before)
+//                  j >  k   the swaps, U12 = L11^-1 A12, A22 -= L21 U12 on
+//                           the matrix cores, and for j == k + 1 the
+//                           factorisation of panel k + 1.
+// A panel is factored with its rows in registers (row k1 + t in thread t):
+// per column a wave-shuffle arg-max, one barrier for the cross-wave pivot,
+// one for the exchange of the pivot row through LDS (double-buffered by
+// column parity, so the next column needs no third barrier).  Each row
+// carries the index of the row it came from; the rows that moved give the
+// panel's composite permutation (at most 2 NB rows), which the other blocks
+// apply with one load and one store per element.
+// ipiv follows LAPACK getrf: row k was exchanged with row ipiv[k] at step k.
+constexpr int LU_TMAX = 1024;  // n <= LU_TMAX: one row per thread
+
+struct LuSwaps {
+    int32_t cnt, pad;
+    int2 mv[2 * NB];  // (row, row its content comes from)
+};
+
+// Workgroup barrier that orders LDS accesses only: outstanding global loads
+// stay in flight across it (a __syncthreads also waits for those).
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int T>
+struct LuShared {
+    // per column parity: each wave's largest |entry|, its row and that
+    // row's entries of columns c .. c + NB - 1
+    float rv[2][T / 64];
+    int ri[2][T / 64];
+    float4 cand[2][T / 64][NB / 4];
+    int piv[NB];
+    int first_zero, cnt;
+    float Lt[NB][NB + 1];  // L11, [column][row]
+    float Ut[NB][NB + 1];  // U12, [column][row]
+};
+
+// Largest value of the wave (v >= -1), in every lane: DPP shifts within
+// rows of 16 lanes, then row broadcasts to lane 63, then a read of lane 63.
+__device__ __forceinline__ float wave_max(float v)
+{
+    // dpp controls: row_shr:n = 0x110 + n, row_bcast:15 = 0x142, row_bcast:31 = 0x143
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                            0x111, 0xf, 0xf, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                            0x112, 0xf, 0xf, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                            0x114, 0xf, 0xf, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                            0x118, 0xf, 0xf, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                            0x142, 0xa, 0xf, false)));
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v),
+                                                            0x143, 0xc, 0xf, false)));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Panel columns k1 .. k1 + w of rows k1 .. n - 1, one row per thread in
+// registers, held rotated: a[j] is the entry of column c + j at step c, so
+// every register index is fixed (the loop is unrolled in full, so the
+// rotation costs no instructions).  Rows are exchanged by label only: each
+// thread keeps its content and carries the row index it currently
+// occupies.  Column c of a content is final after step c and is stored at
+// the content's own (starting) row; once the panel is done, the contents
+// that ended in another row are moved there.  One LDS barrier per column:
+// each wave publishes its largest entry with that entry's row before it
+// (the column stores stay in flight).
+template <int T>
+__device__ __forceinline__ void lu_factor_panel(float *__restrict__ A, int n, int k1,
+                                                int32_t *__restrict__ ipiv,
+                                                int32_t *__restrict__ info,
+                                                LuSwaps *__restrict__ sw, LuShared<T> &sh)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int w = min(NB, n - k1);
+    const int r0 = k1 + tid;  // this thread's content: its starting row
+    const bool live = r0 < n;
+    int r = r0;               // the row it occupies now
+    float a[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) a[j] = (live && j < w) ? A[(int64_t)(k1 + j) * n + r0] : 0.0f;
+    if (tid == 0) {
+        sh.first_zero = -1;
+        sh.cnt = 0;
+    }
+    // columns c >= w (last panel) run on zero entries and change nothing;
+    // the loop stays rolled (its body stays in the instruction cache)
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+        const int rc = k1 + c;
+        const int buf = c & 1;
+        // the wave's largest |a[0]| over rows >= rc and the first row with it
+        const bool cand = live && r >= rc;
+        const float mine = cand ? fabsf(a[0]) : -1.0f;
+        const float v = wave_max(mine);
+        // the smallest occupied row among the lanes holding the maximum
+        int best_r = cand && mine == v ? r : 0x7fffffff;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) best_r = min(best_r, __shfl_xor(best_r, o));
+        if (r == best_r) {
+            sh.rv[buf][wv] = v;
+            sh.ri[buf][wv] = r;
+#pragma unroll
+            for (int j = 0; j < NB / 4; ++j)
+                sh.cand[buf][wv][j] = make_float4(a[4 * j], a[4 * j + 1], a[4 * j + 2], a[4 * j + 3]);
+        } else if (lane == 0 && best_r == 0x7fffffff) {
+            sh.rv[buf][wv] = -1.0f;
+            sh.ri[buf][wv] = 0x7fffffff;
+        }
+        lds_barrier();
+        float pv = sh.rv[buf][0];
+        int p = sh.ri[buf][0], pw = 0;
+#pragma unroll
+        for (int q = 1; q < T / 64; ++q) {
+            const float vq = sh.rv[buf][q];
+            const int iq = sh.ri[buf][q];
+            if (vq > pv || (vq == pv && iq < p)) {
+                pv = vq;
+                p = iq;
+                pw = q;
+            }
+        }
+        if (tid == 0) {
+            sh.piv[c] = p;
+            if (pv == 0.0f && sh.first_zero < 0 && c < w) sh.first_zero = rc;
+        }
+        // the exchange of rows rc and p: labels only
+        if (r == p)
+            r = rc;
+        else if (r == rc)
+            r = p;
+        const float4 *pr = sh.cand[buf][pw];
+        float prow[NB];
+#pragma unroll
+        for (int j = 0; j < NB / 4; ++j) {
+            const float4 q4 = pr[j];
+            prow[4 * j] = q4.x;
+            prow[4 * j + 1] = q4.y;
+            prow[4 * j + 2] = q4.z;
+            prow[4 * j + 3] = q4.w;
+        }
+        const float d = prow[0];
+        if (d != 0.0f && r > rc && live) {
+            const float l = a[0] * __builtin_amdgcn_rcpf(d);
+            a[0] = l;
+#pragma unroll
+            for (int j = 1; j < NB; ++j) a[j] = __builtin_fmaf(-l, prow[j], a[j]);
+        }
+        if (live && c < w) A[(int64_t)(k1 + c) * n + r0] = a[0];
+#pragma unroll
+        for (int j = 0; j + 1 < NB; ++j) a[j] = a[j + 1];
+        a[NB - 1] = 0.0f;
+    }
+    // contents that ended in another row move there (all stores done first)
+    const bool moved = live && r != r0;
+    __syncthreads();
+    float t[NB];
+    if (moved) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) t[c] = c < w ? A[(int64_t)(k1 + c) * n + r0] : 0.0f;
+        const int q = atomicAdd(&sh.cnt, 1);
+        sw->mv[q] = make_int2(r, r0);
+    }
+    __syncthreads();
+    if (moved) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c < w) A[(int64_t)(k1 + c) * n + r] = t[c];
+    }
+    if (tid < w) ipiv[k1 + tid] = sh.piv[tid];
+    if (tid == 0) {
+        sw->cnt = sh.cnt;
+        if (sh.first_zero >= 0 && info[0] == 0) info[0] = sh.first_zero + 1;
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void lu_first(float *__restrict__ A, int n,
+                                              int32_t *__restrict__ ipiv,
+                                              int32_t *__restrict__ info,
+                                              LuSwaps *__restrict__ sw)
+{
+    __shared__ LuShared<T> sh;
+    if (threadIdx.x == 0) info[0] = 0;
+    __syncthreads();
+    lu_factor_panel<T>(A, n, 0, ipiv, info, sw, sh);
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void lu_step(float *__restrict__ A, int n, int k,
+                                             int32_t *__restrict__ ipiv,
+                                             int32_t *__restrict__ info,
+                                             LuSwaps *__restrict__ sw)
+{
+    __shared__ LuShared<T> sh;
+    const int j = blockIdx.x;
+    if (j == k) return;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int k0 = k * NB, w = min(NB, n - k0);
+    const int c0 = j * NB, wj = min(NB, n - c0);
+    // 1. the composite row permutation of panel k on columns c0 .. c0 + wj
+    {
+        const LuSwaps &S = sw[k];
+        const int cnt = S.cnt;
+        constexpr int PER = (2 * NB * NB + T - 1) / T;
+        float v[PER];
+#pragma unroll
+        for (int it = 0; it < PER; ++it) {
+            const int idx = tid + T * it, q = idx / NB, cc = idx % NB;
+            v[it] = (q < cnt && cc < wj) ? A[(int64_t)(c0 + cc) * n + S.mv[q].y] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < PER; ++it) {
+            const int idx = tid + T * it, q = idx / NB, cc = idx % NB;
+            if (q < cnt && cc < wj) A[(int64_t)(c0 + cc) * n + S.mv[q].x] = v[it];
+        }
+    }
+    if (j < k) return;
+    __syncthreads();
+    // 2. U12 = L11^-1 A12 (L11 unit lower, w x w), one column per lane of
+    // wave 0
+    for (int e = tid; e < NB * NB; e += T) {
+        const int c = e / NB, r = e % NB;
+        sh.Lt[c][r] = (c < w && r < w) ? A[(int64_t)(k0 + c) * n + k0 + r] : 0.0f;
+        sh.Ut[c][r] = (c < wj && r < w) ? A[(int64_t)(c0 + c) * n + k0 + r] : 0.0f;
+    }
+    __syncthreads();
+    if (tid < wj) {
+        // column tid of U12 stays in LDS (its row stride NB + 1 keeps the
+        // 32 lanes on distinct banks); L11 reads are broadcasts
+        float *u = sh.Ut[tid];
+        for (int c = 0; c < w; ++c) {
+            const float uc = u[c];
+#pragma unroll 4
+            for (int r = c + 1; r < w; ++r) u[r] = __builtin_fmaf(-sh.Lt[c][r], uc, u[r]);
+        }
+        for (int r = 0; r < w; ++r) A[(int64_t)(c0 + tid) * n + k0 + r] = u[r];
+    }
+    __syncthreads();
+    // 3. A22 -= L21 U12, 16 x 16 tiles on v_mfma_f32_16x16x4f32 (A operand
+    // lane l holds A[l%16][l/16], B operand B[l/16][l%16], result
+    // D[4 (l/16) + v][l%16])
+    // A wave holds at most LU_TILES tiles (m <= T: T covers the next
+    // panel's rows), and every tile's operands are loaded before the first
+    // product, so the loads of all tiles are in flight together.
+    const int r0 = k0 + w, m = n - r0;
+    if (m > 0) {
+        constexpr int LU_TILES = 8;
+        const int li = lane & 15, lk = lane >> 4;
+        const int tiles_r = (m + 15) / 16, tiles_c = (wj + 15) / 16;
+        const int ntile = tiles_r * tiles_c;
+        float av[LU_TILES][NB / 4];
+        floatx4 acc[LU_TILES];
+#pragma unroll
+        for (int q = 0; q < LU_TILES; ++q) {
+            const int t = wv + q * (T / 64);
+            const int tr = t % tiles_r, tc = t / tiles_r;
+            const int row = r0 + 16 * tr, cl = 16 * tc;  // cl: column inside the block
+            const bool on = t < ntile;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = row + 4 * lk + v, jj = cl + li;
+                acc[q][v] = (on && i < n && jj < wj) ? A[(int64_t)(c0 + jj) * n + i] : 0.0f;
+            }
+#pragma unroll
+            for (int kk = 0; kk < NB / 4; ++kk) {
+                const int i = row + li, kc = 4 * kk + lk;
+                av[q][kk] = (on && i < n && kc < w) ? A[(int64_t)(k0 + kc) * n + i] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < LU_TILES; ++q) {
+            const int t = wv + q * (T / 64);
+            if (t >= ntile) break;  // uniform across the wave
+            const int tr = t % tiles_r, tc = t / tiles_r;
+            const int row = r0 + 16 * tr, cl = 16 * tc;
+#pragma unroll
+            for (int kk = 0; kk < NB / 4; ++kk) {
+                const int kc = 4 * kk + lk;
+                const float b = (kc < w && cl + li < wj) ? sh.Ut[cl + li][kc] : 0.0f;
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[q][kk], b, acc[q], 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = row + 4 * lk + v, jj = cl + li;
+                if (i < n && jj < wj) A[(int64_t)(c0 + jj) * n + i] = acc[q][v];
+            }
+        }
+    }
+    // 4. lookahead: the next panel is this block, now up to date
+    if (j == k + 1) {
+        __syncthreads();
+        lu_factor_panel<T>(A, n, c0, ipiv, info, sw + j, sh);
+    }
+}
+
 // ------------------------------------------------- approximate inverse ----
 // X = (P L U)^-1 in float32, column k from the right-hand side e_k (one
 // workgroup per column; blocked substitution in LDS: a wave solves each
@@ -403,6 +739,154 @@ __global__ __launch_bounds__(SOLVE_T) void lu_inverse_kernel(const float *__rest
         __syncthreads();
     }
     for (int i = tid; i < n; i += SOLVE_T) X[(int64_t)i * n + col] = y[i];
+}
+
+// The same inverse for n <= 1024, 16 columns of X per workgroup of 16 waves.
+// Column c of P E is the unit vector at the position the row swaps take
+// row j0 + c to (one lane per column follows it through ipiv); then
+// Y = L^-1 P E and X = U^-1 Y are solved in blocks of 16 rows kept in LDS.
+// The update of a block by every block before it (after it, for U) is a
+// 16 x 16 x K product on v_mfma_f32_16x16x4f32 with K split over the 16
+// waves, whose partial tiles are added in wave order by 256 threads, one
+// per tile entry; the 16 x 16 triangle is then solved by 16 lanes, one per
+// column.  The factor entries of the next block are loaded while the
+// current one is finished (the barriers order LDS only, so those loads
+// stay in flight).
+constexpr int INV_C = 16;
+constexpr int INV_NMAX = 1024;
+constexpr int INV_W = 16;                          // waves per workgroup
+constexpr int INV_SMAX = INV_NMAX / 4 / INV_W;     // k-steps of 4 per wave and block
+
+__global__ __launch_bounds__(64 * INV_W) void lu_inverse_mfma(const float *__restrict__ LU,
+                                                              int n,
+                                                              const int32_t *__restrict__ ipiv,
+                                                              float *__restrict__ X)
+{
+    extern __shared__ float y[];  // y[r * INV_C + c], rows padded to 16
+    __shared__ floatx4 part[INV_W][64];
+    __shared__ float dg[16][17];  // the diagonal 16 x 16 block of the factors
+    __shared__ int pos[INV_C];
+    constexpr int T = 64 * INV_W;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int j0 = blockIdx.x * INV_C;
+    const int nb = (n + 15) / 16, npad = 16 * nb;
+    if (tid < INV_C) {
+        int q = j0 + tid;
+        if (q < n)
+            for (int k = 0; k < n; ++k) {
+                const int p = ipiv[k];
+                if (q == k)
+                    q = p;
+                else if (q == p)
+                    q = k;
+            }
+        pos[tid] = q;
+    }
+    __syncthreads();
+    for (int e = tid; e < npad * INV_C; e += T) {
+        const int r = e / INV_C, c = e % INV_C;
+        y[e] = (j0 + c < n && r == pos[c]) ? 1.0f : 0.0f;
+    }
+    float av[INV_SMAX];
+    float dgv = 0.0f;
+    // factor entries of the block at rows rb for k in [ka, kb): this wave's
+    // share of the products, and one entry of the diagonal block
+    auto prefetch = [&](int rb, int ka, int kb) {
+        const int steps = (kb - ka) / 4, per = (steps + INV_W - 1) / INV_W, s0 = wv * per;
+        const int i = rb + li;
+#pragma unroll
+        for (int u = 0; u < INV_SMAX; ++u) {
+            const int st = s0 + u, k = ka + 4 * st + lk;
+            const bool ok = u < per && st < steps;
+            av[u] = (ok && i < n && k < n) ? LU[(int64_t)k * n + i] : 0.0f;
+        }
+        if (tid < 256) {
+            const int dr = tid >> 4, dc = tid & 15;
+            dgv = (rb + dr < n && rb + dc < n) ? LU[(int64_t)(rb + dc) * n + rb + dr] : 0.0f;
+        }
+    };
+    // this wave's partial tile, then (after a barrier) y[block] -= sum of
+    // the partials in wave order
+    auto products = [&](int ka, int kb) {
+        const int steps = (kb - ka) / 4, per = (steps + INV_W - 1) / INV_W, s0 = wv * per;
+        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int u = 0; u < INV_SMAX; ++u) {
+            const int st = s0 + u, k = ka + 4 * st + lk;
+            if (u < per && st < steps)  // uniform across the wave
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], y[k * INV_C + li], acc, 0, 0,
+                                                           0);
+        }
+        part[wv][lane] = acc;
+    };
+    auto subtract = [&](int rb, float dcur) {
+        if (tid < 256) {
+            const int r = tid >> 4, c = tid & 15;
+            const int l2 = (r >> 2) * 16 + c, v = r & 3;
+            float t = 0.0f;
+#pragma unroll
+            for (int q = 0; q < INV_W; ++q) t += part[q][l2][v];
+            y[(rb + r) * INV_C + c] -= t;
+            dg[r][c] = dcur;
+        }
+    };
+    prefetch(0, 0, 0);
+    __syncthreads();
+    // forward: L unit lower
+    for (int b = 0; b < nb; ++b) {
+        const int rb = 16 * b;
+        const float dcur = dgv;  // this block's diagonal entry (dgv is refilled next)
+        products(0, rb);
+        if (b + 1 < nb)
+            prefetch(rb + 16, 0, rb + 16);
+        lds_barrier();
+        subtract(rb, dcur);
+        lds_barrier();
+        if (tid < INV_C) {
+            float col[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) col[r] = y[(rb + r) * INV_C + tid];
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+#pragma unroll
+                for (int rr = 0; rr < 16; ++rr)
+                    if (rr > r) col[rr] = __builtin_fmaf(-dg[rr][r], col[r], col[rr]);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) y[(rb + r) * INV_C + tid] = col[r];
+        }
+        lds_barrier();
+    }
+    // backward: U upper, non-unit
+    prefetch(npad - 16, npad, npad);
+    for (int b = nb - 1; b >= 0; --b) {
+        const int rb = 16 * b;
+        const float dcur = dgv;
+        products(rb + 16, npad);
+        if (b > 0) prefetch(rb - 16, rb, npad);
+        lds_barrier();
+        subtract(rb, dcur);
+        lds_barrier();
+        if (tid < INV_C) {
+            float col[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) col[r] = y[(rb + r) * INV_C + tid];
+#pragma unroll
+            for (int r = 15; r >= 0; --r) {
+                if (rb + r < n) col[r] = col[r] / dg[r][r];
+#pragma unroll
+                for (int rr = 0; rr < 16; ++rr)
+                    if (rr < r) col[rr] = __builtin_fmaf(-dg[rr][r], col[r], col[rr]);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) y[(rb + r) * INV_C + tid] = col[r];
+        }
+        lds_barrier();
+    }
+    for (int e = tid; e < n * INV_C; e += T) {
+        const int r = e / INV_C, c = e % INV_C;
+        if (j0 + c < n) X[(int64_t)r * n + j0 + c] = y[e];
+    }
 }
 
 // ------------------------------------------------- refinement kernels -----
@@ -535,8 +1019,70 @@ __global__ __launch_bounds__(256) void refine_update(double *__restrict__ x,
 }
 
 // ------------------------------------------------------------- host ------
-int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info)
+size_t lu_work_bytes(int n) { return sizeof(LuSwaps) * (size_t)std::max(1, (n + NB - 1) / NB); }
+
+// Workgroup size for a panel of `rows` live rows: one row per thread, as
+// few waves as cover them (the panel loop is bound by the waves' vector
+// instruction issue, so idle waves would cost as much as busy ones).
+static int lu_threads(int rows)
 {
+    int t = 64;
+    while (t < rows) t *= 2;
+    return t;
+}
+
+template <int T>
+static void lu_launch_first(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info,
+                            LuSwaps *sw)
+{
+    lu_first<T><<<1, T, 0, L->stream>>>(A32, n, ipiv, info, sw);
+}
+
+template <int T>
+static void lu_launch_step(Lane *L, float *A32, int n, int k, int nsteps, int32_t *ipiv,
+                           int32_t *info, LuSwaps *sw)
+{
+    lu_step<T><<<(unsigned)nsteps, T, 0, L->stream>>>(A32, n, k, ipiv, info, sw);
+}
+
+int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, void *work)
+{
+    if (n <= 0) return PHF_OK;
+    if (n <= LU_TMAX) {
+        const int nsteps = (n + NB - 1) / NB;
+        LuSwaps *sw = reinterpret_cast<LuSwaps *>(work);
+        if (!sw) PHF_ALLOC(sw, L->arena, LuSwaps, (size_t)nsteps);
+        for (int k = -1; k < nsteps; ++k) {
+            // launch k + 1 factors panel k + 1 (rows from (k + 1) NB on);
+            // k = -1 is the first panel alone
+            const int t = lu_threads(std::max(1, n - (k + 1) * NB));
+            switch (t) {
+            case 64:
+                k < 0 ? lu_launch_first<64>(L, A32, n, ipiv, info, sw)
+                      : lu_launch_step<64>(L, A32, n, k, nsteps, ipiv, info, sw);
+                break;
+            case 128:
+                k < 0 ? lu_launch_first<128>(L, A32, n, ipiv, info, sw)
+                      : lu_launch_step<128>(L, A32, n, k, nsteps, ipiv, info, sw);
+                break;
+            case 256:
+                k < 0 ? lu_launch_first<256>(L, A32, n, ipiv, info, sw)
+                      : lu_launch_step<256>(L, A32, n, k, nsteps, ipiv, info, sw);
+                break;
+            case 512:
+                k < 0 ? lu_launch_first<512>(L, A32, n, ipiv, info, sw)
+                      : lu_launch_step<512>(L, A32, n, k, nsteps, ipiv, info, sw);
+                break;
+            default:
+                k < 0 ? lu_launch_first<1024>(L, A32, n, ipiv, info, sw)
+                      : lu_launch_step<1024>(L, A32, n, k, nsteps, ipiv, info, sw);
+                break;
+            }
+        }
+        PHF_HIP(hipGetLastError());
+        return PHF_OK;
+    }
+    // larger systems: one kernel per panel stage
     PHF_HIP(hipMemsetAsync(info, 0, sizeof(int32_t), L->stream));
     const size_t lds_max = Context::get().props().sharedMemPerBlock;
     for (int k0 = 0; k0 < n; k0 += NB) {
@@ -555,27 +1101,43 @@ int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info)
     return PHF_OK;
 }
 
-int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f)
+int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
+                        hipEvent_t *marks)
 {
-    f->rows = rows;
-    f->n = n;
     const bool normal = rows > n;
-    PHF_ALLOC(f->Bc, L->arena, double, (size_t)(rows * n));
-    PHF_ALLOC(f->LU, L->arena, float, (size_t)(n * n));
-    PHF_ALLOC(f->ipiv, L->arena, int32_t, (size_t)n);
-    PHF_ALLOC(f->info, L->arena, int32_t, 1);
+    if (!(f->LU && f->rows == rows && f->n == n)) {
+        f->rows = rows;
+        f->n = n;
+        PHF_ALLOC(f->Bc, L->arena, double, (size_t)(rows * n));
+        PHF_ALLOC(f->LU, L->arena, float, (size_t)(n * n));
+        PHF_ALLOC(f->ipiv, L->arena, int32_t, (size_t)n);
+        PHF_ALLOC(f->info, L->arena, int32_t, 1);
+        PHF_ALLOC(f->Xinv, L->arena, float, (size_t)(n * n));
+        if (normal) PHF_ALLOC(f->N64, L->arena, double, (size_t)(n * n));
+        if (n <= LU_TMAX) {
+            char *w;
+            PHF_ALLOC(w, L->arena, char, lu_work_bytes((int)n));
+            f->lu_work = w;
+        }
+    }
     to_colmajor<<<div_up(rows * n, 256), 256, 0, L->stream>>>(B, rows, n, f->Bc,
                                                               normal ? nullptr : f->LU);
     if (normal) {
-        double *N64;
-        PHF_ALLOC(N64, L->arena, double, (size_t)(n * n));
-        normal_matrix<<<div_up(n * n, 256), 256, 0, L->stream>>>(f->Bc, rows, n, N64, f->LU);
+        const dim3 grid(div_up(n, NM_T), div_up(n, NM_T));
+        normal_matrix<<<grid, 256, 0, L->stream>>>(f->Bc, rows, n, f->N64, f->LU);
     }
-    PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info));
-    PHF_ALLOC(f->Xinv, L->arena, float, (size_t)(n * n));
-    lu_inverse_kernel<<<(unsigned)n, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(
-        f->LU, (int)n, f->ipiv, f->Xinv);
+    if (marks) PHF_HIP(hipEventRecord(marks[0], L->stream));
+    PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->lu_work));
+    if (marks) PHF_HIP(hipEventRecord(marks[1], L->stream));
+    const size_t inv_lds = sizeof(float) * INV_C * 16 * div_up(n, 16);
+    if (n <= INV_NMAX && inv_lds + 8192 <= Context::get().props().sharedMemPerBlock)
+        lu_inverse_mfma<<<div_up(n, INV_C), 64 * INV_W, inv_lds, L->stream>>>(
+            f->LU, (int)n, f->ipiv, f->Xinv);
+    else
+        lu_inverse_kernel<<<(unsigned)n, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(
+            f->LU, (int)n, f->ipiv, f->Xinv);
     PHF_HIP(hipGetLastError());
+    if (marks) PHF_HIP(hipEventRecord(marks[2], L->stream));
     f->Br = B;
     int32_t hinfo = 0;
     PHF_HIP(hipMemcpyAsync(&hinfo, f->info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
@@ -846,13 +1408,17 @@ int panel_series_geom_device(Lane *L, int64_t m, const double *xs, const double 
                              const double *xe, const double *ye, const double *cosA,
                              const double *sinA, PanelSeriesGeom *out)
 {
-    *out = PanelSeriesGeom{};
-    if (m < 4 * PG_SIZE) return PHF_OK;  // the per-panel path serves such bodies
+    if (m < 4 * PG_SIZE) {  // the per-panel path serves such bodies
+        *out = PanelSeriesGeom{};
+        return PHF_OK;
+    }
     const int64_t ng = (m + PG_SIZE - 1) / PG_SIZE;
-    PanelG *grp;
-    double2 *geo;
-    PHF_ALLOC(grp, L->arena, PanelG, (size_t)ng);
-    PHF_ALLOC(geo, L->arena, double2, (size_t)m * PG_TERMS);
+    PanelG *grp = out->m == m ? static_cast<PanelG *>(out->grp) : nullptr;
+    double2 *geo = out->m == m ? out->geo : nullptr;
+    if (!grp || !geo) {
+        PHF_ALLOC(grp, L->arena, PanelG, (size_t)ng);
+        PHF_ALLOC(geo, L->arena, double2, (size_t)m * PG_TERMS);
+    }
     panel_group_geom<<<(unsigned)ng, 64, 0, L->stream>>>(m, xs, ys, xe, ye, cosA, sinA, grp, geo);
     PHF_HIP(hipGetLastError());
     out->grp = grp;
@@ -1073,7 +1639,7 @@ int phf_panel_influence(const double *xCP, const double *yCP, const double *xs,
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     double *d[8], *dA;
     for (int k = 0; k < 8; ++k) PHF_TRY(upload(L, in[k], m, &d[k]));
     PHF_ALLOC(dA, L->arena, double, (size_t)(m * m));
@@ -1109,7 +1675,7 @@ int phf_panel_induced_velocity(const double *gamma, int64_t m, const double *xE,
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     double *d[7], *dxE, *dyE, *dvx, *dvy;
     for (int k = 0; k < 7; ++k) PHF_TRY(upload(L, in[k], m, &d[k]));
     PHF_TRY(upload(L, xE, ne, &dxE));
@@ -1137,7 +1703,7 @@ int phf_dense_solve(const double *B, int64_t rows, int64_t n, const double *c, d
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;
-    PHF_TRY(L->arena.reset());
+    PHF_TRY(reset_arena(L));
     auto t0 = std::chrono::steady_clock::now();
     double *dB, *dc, *dx;
     PHF_TRY(upload(L, B, rows * n, &dB));
@@ -1158,6 +1724,33 @@ int phf_dense_solve(const double *B, int64_t rows, int64_t n, const double *c, d
         info->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
     return rc;
+}
+
+int phf_lu_factor_f32(const float *A, int64_t n, float *LU, int32_t *ipiv, int32_t *info)
+{
+    clear_error();
+    PHF_REQUIRE(n >= 1 && n <= 4096, "n out of range (%lld)", (long long)n);
+    PHF_REQUIRE(A && LU && ipiv, "NULL argument");
+    for (int64_t i = 0; i < n * n; ++i)
+        PHF_REQUIRE(std::isfinite(A[i]), "A[%lld] is not finite", (long long)i);
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(reset_arena(L));
+    float *dA;
+    int32_t *dp, *di;
+    PHF_ALLOC(dA, L->arena, float, (size_t)(n * n));
+    PHF_ALLOC(dp, L->arena, int32_t, (size_t)n);
+    PHF_ALLOC(di, L->arena, int32_t, 1);
+    PHF_HIP(hipMemcpyAsync(dA, A, sizeof(float) * n * n, hipMemcpyHostToDevice, L->stream));
+    PHF_TRY(lu_factor_f32(L, dA, (int)n, dp, di));
+    PHF_HIP(hipMemcpyAsync(LU, dA, sizeof(float) * n * n, hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int32_t) * n, hipMemcpyDeviceToHost, L->stream));
+    int32_t hinfo = 0;
+    PHF_HIP(hipMemcpyAsync(&hinfo, di, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    if (info) *info = hinfo;
+    return PHF_OK;
 }
 
 }  // extern "C"

@@ -13,7 +13,10 @@ struct SolveStats {
 };
 
 // In-place float32 LU with partial pivoting of the column-major n x n A32.
-int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info);
+// work: lu_work_bytes(n) of device memory (nullptr: taken from the arena).
+int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info,
+                  void *work = nullptr);
+size_t lu_work_bytes(int n);
 
 // Float32 LU factors of B (rows == n) or of B^T B (rows > n) and the
 // float32 inverse formed from them, with the float64 column-major copy of B
@@ -24,11 +27,17 @@ struct DenseFactors {
     double *Bc = nullptr;
     float *LU = nullptr;
     float *Xinv = nullptr;       // float32 (P L U)^-1, row-major
+    double *N64 = nullptr;       // B^T B (rows > n)
+    void *lu_work = nullptr;
     int32_t *ipiv = nullptr, *info = nullptr;
     int singular_at = 0;
 };
 
-int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f);
+// Factors of B; buffers already held by f for the same shape are reused.
+// marks (optional, 3 events): recorded after B^T B, after the LU and after
+// the inverse.
+int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
+                        hipEvent_t *marks = nullptr);
 // Refinement sweeps from x = 0, or from the x given (warm); sweeps go in
 // bursts between host checks, the first of first_burst sweeps.
 int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double *x,
@@ -48,6 +57,7 @@ struct PanelSeriesGeom {
     double2 *geo = nullptr;
     int64_t m = 0;
 };
+// Buffers already held by out for the same m are reused.
 int panel_series_geom_device(Lane *L, int64_t m, const double *xs, const double *ys,
                              const double *xe, const double *ye, const double *cosA,
                              const double *sinA, PanelSeriesGeom *out);

@@ -29,7 +29,7 @@ struct Panels {
     std::mutex mu;
     Lane lane;  // arena holds the persistent panel data (never reset)
     int64_t m = 0;
-    int problem = 0;  // 0 'fixed', 1 'prescribed'
+    int problem = 0;  // PHF_PANELS_FIXED, _PRESCRIBED or _MOVING
     double gTotal = 0.0;
     double *xs = nullptr, *ys = nullptr, *xe = nullptr, *ye = nullptr;
     double *cosA = nullptr, *sinA = nullptr, *L = nullptr, *xcp = nullptr, *ycp = nullptr;
@@ -128,7 +128,8 @@ static int panels_solve_device(Lane *L, Panels *P, const double *vx_cp, const do
     double *rhs;
     PHF_ALLOC(rhs, L->arena, double, (size_t)rows);
     slip_rhs_kernel<<<div_up(rows, 256), 256, 0, L->stream>>>(
-        vx_cp, vy_cp, P->m, vinf_x, vinf_y, P->cosA, P->sinA, P->problem, P->gTotal, rhs);
+        vx_cp, vy_cp, P->m, vinf_x, vinf_y, P->cosA, P->sinA,
+        P->problem == PHF_PANELS_PRESCRIBED ? 1 : 0, P->gTotal, rhs);
     PHF_HIP(hipGetLastError());
     // start from the last strengths (the slip condition changes little from
     // one stage to the next) with a first burst of one sweep more than the
@@ -199,6 +200,95 @@ static void moments_to(const double4 &m, double *circ, double *ix, double *iy, d
     *m2 = m.w;
 }
 
+// Panel geometry on the host (__panelsClass.py:1036-1058): L = |end - start|,
+// (cos, sin) = (end - start) / L, n = (-sin, cos), CP = mid - n L 0.1.
+struct PanelGeomHost {
+    const double *xs, *ys, *xe, *ye;
+    std::vector<double> cs, sn, Ln, xc, yc;
+};
+
+static int panel_geometry_host(const double *xs, const double *ys, const double *xe,
+                               const double *ye, int64_t m, PanelGeomHost *g)
+{
+    PHF_TRY(check_host_array(xs, m, "xPanelStart"));
+    PHF_TRY(check_host_array(ys, m, "yPanelStart"));
+    PHF_TRY(check_host_array(xe, m, "xPanelEnd"));
+    PHF_TRY(check_host_array(ye, m, "yPanelEnd"));
+    g->xs = xs;
+    g->ys = ys;
+    g->xe = xe;
+    g->ye = ye;
+    g->cs.resize(m);
+    g->sn.resize(m);
+    g->Ln.resize(m);
+    g->xc.resize(m);
+    g->yc.resize(m);
+    for (int64_t j = 0; j < m; ++j) {
+        const double lx = xe[j] - xs[j], ly = ye[j] - ys[j];
+        g->Ln[j] = std::sqrt(lx * lx + ly * ly);
+        PHF_REQUIRE(g->Ln[j] > 0.0, "panel %lld has zero length", (long long)j);
+        g->cs[j] = lx / g->Ln[j];
+        g->sn[j] = ly / g->Ln[j];
+        const double nx = -g->sn[j], ny = g->cs[j];
+        g->xc[j] = 0.5 * (xe[j] + xs[j]) - nx * g->Ln[j] * 0.1;
+        g->yc[j] = 0.5 * (ye[j] + ys[j]) - ny * g->Ln[j] * 0.1;
+    }
+    return PHF_OK;
+}
+
+__global__ void zero_doubles(double *__restrict__ p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// New geometry into the body's device buffers, then its influence matrix
+// (with the Kelvin row for 'prescribed'), the float32 factors and the
+// grouped-velocity factors, all rebuilt in place (__panelsClass.py:
+// updateBody :530-638 -> __assembleInfluenceMatrix :872-961).  timing
+// (optional): device times of the stages.
+static int panels_rebuild(Panels *P, const PanelGeomHost &g, phf_panels_timing *timing)
+{
+    Lane *L = &P->lane;
+    const int64_t m = P->m;
+    const double *host[9] = {g.xs, g.ys, g.xe, g.ye, g.cs.data(), g.sn.data(), g.Ln.data(),
+                             g.xc.data(), g.yc.data()};
+    double *dev[9] = {P->xs, P->ys, P->xe, P->ye, P->cosA, P->sinA, P->L, P->xcp, P->ycp};
+    for (int k = 0; k < 9; ++k)
+        PHF_HIP(hipMemcpyAsync(dev[k], host[k], sizeof(double) * m, hipMemcpyHostToDevice,
+                               L->stream));
+    P->hxcp = g.xc;
+    P->hycp = g.yc;
+    hipEvent_t *ev = L->ev;  // [0] start, [1] influence, [2..4] factor marks
+    PHF_HIP(hipEventRecord(ev[0], L->stream));
+    PHF_TRY(panel_influence_device(L, m, P->xcp, P->ycp, P->xs, P->ys, P->xe, P->ye, P->cosA,
+                                   P->sinA, P->A));
+    if (P->problem == PHF_PANELS_PRESCRIBED)  // Kelvin row: sum_j L_j s_j = -gTotal (:509-518)
+        PHF_HIP(hipMemcpyAsync(P->A + m * m, P->L, sizeof(double) * m, hipMemcpyDeviceToDevice,
+                               L->stream));
+    PHF_HIP(hipEventRecord(ev[1], L->stream));
+    const int64_t rows = P->problem == PHF_PANELS_PRESCRIBED ? m + 1 : m;
+    PHF_TRY(dense_factor_device(L, P->A, rows, m, &P->f, ev + 2));
+    PHF_TRY(panel_series_geom_device(L, m, P->xs, P->ys, P->xe, P->ye, P->cosA, P->sinA,
+                                     &P->sg));
+    PHF_HIP(hipEventRecord(ev[5], L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    if (timing) {
+        float a = 0, b = 0, c = 0, d = 0, e = 0;
+        (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+        (void)hipEventElapsedTime(&d, ev[3], ev[4]);
+        (void)hipEventElapsedTime(&e, ev[0], ev[5]);
+        timing->influence_ms = a;
+        timing->normal_ms = b;
+        timing->lu_ms = c;
+        timing->inverse_ms = d;
+        timing->total_ms = e;
+    }
+    return PHF_OK;
+}
+
 }  // namespace phf
 
 // ------------------------------------------------------------ C entry ----
@@ -215,26 +305,12 @@ int phf_panels_create(const double *xs, const double *ys, const double *xe, cons
     PHF_REQUIRE(out != nullptr, "out is NULL");
     *out = nullptr;
     PHF_REQUIRE(m >= 1 && m <= 4096, "panel count out of range (%lld)", (long long)m);
-    PHF_REQUIRE(problem == PHF_PANELS_FIXED || problem == PHF_PANELS_PRESCRIBED,
+    PHF_REQUIRE(problem == PHF_PANELS_FIXED || problem == PHF_PANELS_PRESCRIBED ||
+                    problem == PHF_PANELS_MOVING,
                 "unknown problem type %d", problem);
     PHF_REQUIRE(std::isfinite(gTotal), "gTotal must be finite");
-    PHF_TRY(check_host_array(xs, m, "xPanelStart"));
-    PHF_TRY(check_host_array(ys, m, "yPanelStart"));
-    PHF_TRY(check_host_array(xe, m, "xPanelEnd"));
-    PHF_TRY(check_host_array(ye, m, "yPanelEnd"));
-    // geometry (__panelsClass.py:1036-1058): L = |end-start|, (cos, sin) =
-    // (end-start)/L, n = (-sin, cos), CP = mid - n L 0.1
-    std::vector<double> cs(m), sn(m), Ln(m), xc(m), yc(m);
-    for (int64_t j = 0; j < m; ++j) {
-        const double lx = xe[j] - xs[j], ly = ye[j] - ys[j];
-        Ln[j] = std::sqrt(lx * lx + ly * ly);
-        PHF_REQUIRE(Ln[j] > 0.0, "panel %lld has zero length", (long long)j);
-        cs[j] = lx / Ln[j];
-        sn[j] = ly / Ln[j];
-        const double nx = -sn[j], ny = cs[j];
-        xc[j] = 0.5 * (xe[j] + xs[j]) - nx * Ln[j] * 0.1;
-        yc[j] = 0.5 * (ye[j] + ys[j]) - ny * Ln[j] * 0.1;
-    }
+    PanelGeomHost g;
+    PHF_TRY(panel_geometry_host(xs, ys, xe, ye, m, &g));
     PHF_TRY(Context::get().init());
     PHF_HIP(hipSetDevice(Context::get().device()));
     phf_panels_s *P = new phf_panels_s();
@@ -251,29 +327,15 @@ int phf_panels_create(const double *xs, const double *ys, const double *xe, cons
         P->m = m;
         P->problem = problem;
         P->gTotal = gTotal;
-        P->hxcp = xc;
-        P->hycp = yc;
-        PHF_TRY(upload(L, xs, m, &P->xs));
-        PHF_TRY(upload(L, ys, m, &P->ys));
-        PHF_TRY(upload(L, xe, m, &P->xe));
-        PHF_TRY(upload(L, ye, m, &P->ye));
-        PHF_TRY(upload(L, cs.data(), m, &P->cosA));
-        PHF_TRY(upload(L, sn.data(), m, &P->sinA));
-        PHF_TRY(upload(L, Ln.data(), m, &P->L));
-        PHF_TRY(upload(L, xc.data(), m, &P->xcp));
-        PHF_TRY(upload(L, yc.data(), m, &P->ycp));
+        double **bufs[9] = {&P->xs, &P->ys, &P->xe, &P->ye, &P->cosA, &P->sinA, &P->L, &P->xcp,
+                            &P->ycp};
+        for (double **b : bufs) PHF_ALLOC(*b, L->arena, double, (size_t)m);
         const int64_t rows = problem == PHF_PANELS_PRESCRIBED ? m + 1 : m;
         PHF_ALLOC(P->A, L->arena, double, (size_t)(rows * m));
         PHF_ALLOC(P->s, L->arena, double, (size_t)m);
-        PHF_TRY(panel_influence_device(L, m, P->xcp, P->ycp, P->xs, P->ys, P->xe, P->ye, P->cosA,
-                                       P->sinA, P->A));
-        if (problem == PHF_PANELS_PRESCRIBED)  // Kelvin row: sum_j L_j s_j = -gTotal (:509-518)
-            PHF_HIP(hipMemcpyAsync(P->A + m * m, P->L, sizeof(double) * m,
-                                   hipMemcpyDeviceToDevice, L->stream));
-        PHF_TRY(dense_factor_device(L, P->A, rows, m, &P->f));
-        PHF_TRY(panel_series_geom_device(L, m, P->xs, P->ys, P->xe, P->ye, P->cosA, P->sinA,
-                                         &P->sg));
-        PHF_HIP(hipMemsetAsync(P->s, 0, sizeof(double) * m, L->stream));
+        PHF_TRY(panels_rebuild(P, g, nullptr));
+        zero_doubles<<<div_up(m, 256), 256, 0, L->stream>>>(P->s, m);
+        PHF_HIP(hipGetLastError());
         PHF_HIP(hipStreamSynchronize(L->stream));
         return PHF_OK;
     }();
@@ -282,6 +344,23 @@ int phf_panels_create(const double *xs, const double *ys, const double *xe, cons
         return rc;
     }
     *out = P;
+    return PHF_OK;
+}
+
+int phf_panels_update(phf_panels *P, const double *xs, const double *ys, const double *xe,
+                      const double *ye, phf_panels_timing *timing)
+{
+    clear_error();
+    PHF_REQUIRE(P != nullptr, "panels is NULL");
+    PanelGeomHost g;
+    PHF_TRY(panel_geometry_host(xs, ys, xe, ye, P->m, &g));
+    std::lock_guard<std::mutex> lk(P->mu);
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    phf_panels_timing t{};
+    PHF_TRY(panels_rebuild(P, g, &t));
+    // strengths of the old geometry are no start for the new one
+    P->have_solution = false;
+    if (timing) *timing = t;
     return PHF_OK;
 }
 

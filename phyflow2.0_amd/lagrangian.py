@@ -15,32 +15,68 @@ import ctypes
 
 import numpy as np
 
-from ._native import StepInfo, StepParams, call, f64, lib, ptr
+from ._native import PanelsTiming, StepInfo, StepParams, call, f64, lib, ptr
 
-PROBLEM = {"fixed": 0, "prescribed": 1}
+PROBLEM = {"fixed": 0, "prescribed": 1, "moving": 2}
 
 __all__ = ["PanelBody", "step_rk2"]
 
 
+def _ends(xPanel, yPanel, xs, ys, xe, ye):
+    if xPanel is not None:
+        # end points of consecutive panels (reference geometry dicts,
+        # examples/panels/ex01_solveNoSlip_test.py:26-38)
+        xp, yp = f64(xPanel), f64(yPanel)
+        xs, ys, xe, ye = xp[:-1], yp[:-1], xp[1:], yp[1:]
+    return tuple(f64(v).copy() for v in (xs, ys, xe, ye))
+
+
 class PanelBody:
-    """Closed panel polygon(s) given by panel start/end points."""
+    """Closed panel polygon(s) given by panel start/end points.
+
+    problemType: 'prescribed' (least squares with the Kelvin row, the
+    reference default), 'fixed' (A s = RHS) or 'moving' (A s = RHS with A
+    rebuilt whenever the body moves, __panelsClass.py:785-798).  The given
+    coordinates are the body's local coordinates; updateBody(cmGlobal,
+    thetaLocal) places it (rotation about the local origin, then
+    translation, __panelsClass.py:530-638) and rebuilds the influence
+    matrix and its factors on the GPU."""
 
     def __init__(self, xPanel=None, yPanel=None, problemType="prescribed", gTotal=0.0, *,
                  xs=None, ys=None, xe=None, ye=None):
-        if xPanel is not None:
-            # end points of consecutive panels (reference geometry dicts,
-            # examples/panels/ex01_solveNoSlip_test.py:26-38)
-            xp, yp = f64(xPanel), f64(yPanel)
-            xs, ys, xe, ye = xp[:-1], yp[:-1], xp[1:], yp[1:]
-        xs, ys, xe, ye = (f64(v).copy() for v in (xs, ys, xe, ye))
+        xs, ys, xe, ye = _ends(xPanel, yPanel, xs, ys, xe, ye)
         if problemType not in PROBLEM:
-            raise ValueError("problemType must be 'fixed' or 'prescribed'")
+            raise ValueError("problemType must be 'fixed', 'prescribed' or 'moving'")
         h = ctypes.c_void_p()
         call("phf_panels_create", ptr(xs), ptr(ys), ptr(xe), ptr(ye), xs.size,
              PROBLEM[problemType], float(gTotal), ctypes.byref(h))
         self._h = h
         self.m = xs.size
         self.problemType = problemType
+        self._local = (xs, ys, xe, ye)
+        self.last_update = None
+
+    def update_geometry(self, xPanel=None, yPanel=None, *, xs=None, ys=None, xe=None, ye=None):
+        """New global start/end points of the same panels: geometry,
+        collocation points, influence matrix and factors are rebuilt.
+        Returns (and keeps in last_update) the device times of the rebuild."""
+        xs, ys, xe, ye = _ends(xPanel, yPanel, xs, ys, xe, ye)
+        if xs.size != self.m:
+            raise ValueError("the body has %d panels, got %d" % (self.m, xs.size))
+        t = PanelsTiming()
+        call("phf_panels_update", self._h, ptr(xs), ptr(ys), ptr(xe), ptr(ye), ctypes.byref(t))
+        self.last_update = t.as_dict()
+        return self.last_update
+
+    def updateBody(self, cmGlobal, thetaLocal):
+        """Panels.updateBody for this body: rotate the local coordinates by
+        thetaLocal (anti-clockwise) about the local origin, move them to
+        cmGlobal, and rebuild."""
+        c, s = np.cos(thetaLocal), np.sin(thetaLocal)
+        cx, cy = float(cmGlobal[0]), float(cmGlobal[1])
+        xs, ys, xe, ye = self._local
+        return self.update_geometry(xs=c * xs - s * ys + cx, ys=s * xs + c * ys + cy,
+                                    xe=c * xe - s * ye + cx, ye=s * xe + c * ye + cy)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -9,8 +9,11 @@ Panels.solve (src/python/panels/__panelsClass.py:715-822).
                     panelKernel=1)                                            :148
     solve(A, RHS, problemType='prescribed', L=None, gTotal=0.0)
         -> strengths, the linear algebra of Panels.solve: 'fixed' solves
-        A s = RHS (scipy lu_factor/lu_solve, :961/:782); 'prescribed' the
-        least-squares problem [A; L] s = [RHS; -gTotal] (numpy lstsq, :822).
+        A s = RHS (scipy lu_factor/lu_solve, :961/:782); 'moving' the same
+        square system, rebuilt with the body (numpy solve / gmres / bicgstab
+        to the solver tolerance, :785-798: here always the direct solve);
+        'prescribed' the least-squares problem [A; L] s = [RHS; -gTotal]
+        (numpy lstsq, :822).
     slip_rhs(vxExt, vyExt, cosAlpha, sinAlpha) -> RHS = -(v . t)              :771
 
 Only the constant-strength vortex panel kernel (panelKernel 1, 'csv', the
@@ -56,6 +59,24 @@ def slip_rhs(vxExt, vyExt, cosAlpha, sinAlpha):
     return (-f64(vxExt)) * f64(cosAlpha) + (-f64(vyExt)) * f64(sinAlpha)
 
 
+def lu_factor_f32(A):
+    """(LU, piv, info): float32 LU with partial pivoting of the square A on
+    the GPU in the form of scipy.linalg.lu_factor (LAPACK getrf, used by
+    Panels for the 'fixed' problem, __panelsClass.py:961): LU holds U and
+    the unit-lower L, piv[k] the row exchanged with row k at step k."""
+    A = np.asarray(A, dtype=np.float32)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError("A must be square")
+    n = A.shape[0]
+    Af = np.asfortranarray(A)
+    LU = np.empty((n, n), dtype=np.float32, order="F")
+    piv = np.empty(n, dtype=np.int32)
+    info = ctypes.c_int32()
+    call("phf_lu_factor_f32", Af.ctypes.data, n, LU.ctypes.data, piv.ctypes.data,
+         ctypes.byref(info))
+    return LU, piv, info.value
+
+
 def dense_solve(B, c, return_info=False):
     """x solving B x = c (square) or min ||B x - c|| (tall), float32 LU +
     float64 iterative refinement on the GPU."""
@@ -75,7 +96,7 @@ def dense_solve(B, c, return_info=False):
 def solve(A, RHS, problemType="prescribed", L=None, gTotal=0.0, return_info=False):
     A = np.ascontiguousarray(A, dtype=np.float64)
     RHS = f64(RHS)
-    if problemType == "fixed":
+    if problemType in ("fixed", "moving"):
         return dense_solve(A, RHS, return_info)
     if problemType == "prescribed":
         if L is None:
@@ -85,4 +106,5 @@ def solve(A, RHS, problemType="prescribed", L=None, gTotal=0.0, return_info=Fals
         B = np.vstack([A, rows])
         c = np.concatenate([RHS, -g])
         return dense_solve(B, c, return_info)
-    raise ValueError("problemType %r not provided ('fixed' or 'prescribed')" % (problemType,))
+    raise ValueError("problemType %r not provided ('fixed', 'moving' or 'prescribed')"
+                     % (problemType,))

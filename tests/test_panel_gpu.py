@@ -63,6 +63,9 @@ def test_dense_solve_random(n, O):
     x, info = PS.dense_solve(A, b, return_info=True)
     ref = O.lu_solve(A, b)
     assert info["converged"], info
+    # float32 LU + inverse of this well-conditioned matrix contract the
+    # error by ~1e-5 per sweep: a factor that is wrong does not converge
+    assert info["iterations"] <= 8, info
     assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
 
 
@@ -87,3 +90,81 @@ def test_singular_matrix_raises():
     from phyflow_amd._native import PhfError
     with pytest.raises(PhfError):
         PS.dense_solve(np.zeros((4, 4)), np.ones(4))
+
+
+def _ellipse(m, a=0.5, b=0.25):
+    th = np.linspace(np.pi, -np.pi, m + 1)
+    return a * np.cos(th), b * np.sin(th)
+
+
+@pytest.mark.parametrize("problem", ["moving", "prescribed", "fixed"])
+def test_body_update_rebuilds_matrix_and_factors(problem, O):
+    """Panels.updateBody (__panelsClass.py:530-638): after rotating and
+    moving the body, its collocation points and influence matrix are the
+    oracle's for the moved panels and the strengths solve the moved system
+    ('moving'/'fixed': A s = RHS, __panelsClass.py:785-798; 'prescribed':
+    least squares with the Kelvin row) to 1e-12 of float64 LAPACK."""
+    from phyflow_amd.lagrangian import PanelBody
+    m = 384
+    xp, yp = _ellipse(m)
+    body = PanelBody(xp, yp, problem, 0.0)
+    for cm, th in (((0.3, -0.2), 0.4), ((-1.0, 0.5), -1.1)):
+        t = body.updateBody(cm, th)
+        assert t["lu_ms"] > 0 and t["total_ms"] >= t["lu_ms"], t
+        c, s_ = np.cos(th), np.sin(th)
+        g = W.panel_geometry(c * xp - s_ * yp + cm[0], s_ * xp + c * yp + cm[1])
+        xc, yc = body.collocation_points()
+        assert np.max(np.abs(xc - g["xcp"])) <= 1e-14 and np.max(np.abs(yc - g["ycp"])) <= 1e-14
+        R = O.panel_influence(g["xcp"], g["ycp"], g["xs"], g["ys"], g["xe"], g["ye"], g["cosA"],
+                              g["sinA"])
+        A = body.A
+        assert np.max(np.abs(A - R)) <= 1e-13 * np.max(np.abs(R))
+        vx, vy = np.full(m, 1.0), np.full(m, 0.3)
+        s = body.solve(vx, vy)
+        rhs = PS.slip_rhs(vx, vy, g["cosA"], g["sinA"])
+        if problem == "prescribed":
+            ref = np.linalg.lstsq(np.vstack([R, g["L"][None, :]]), np.concatenate([rhs, [0.0]]),
+                                  rcond=None)[0]
+        else:
+            ref = O.lu_solve(R, rhs)
+        assert np.max(np.abs(s - ref)) <= 1e-12 * np.max(np.abs(ref)), (problem, cm, th)
+    body.close()
+
+
+def test_moving_solve_in_panel_solver(O):
+    g = _cyl(200)
+    A = PS.influenceMatrix(g["xcp"], g["ycp"], g["xs"], g["ys"], g["xe"], g["ye"], g["cosA"],
+                           g["sinA"])
+    rhs = PS.slip_rhs(np.ones(200), np.zeros(200), g["cosA"], g["sinA"])
+    x = PS.solve(A, rhs, "moving")
+    ref = O.lu_solve(A, rhs)
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("n", [1, 7, 31, 32, 33, 100, 257, 512, 700])
+def test_lu_factor_f32_matches_lapack(n):
+    """The float32 factor itself against LAPACK's float32 getrf (scipy
+    lu_factor on the float32 matrix): the same pivot sequence and the same
+    factors to float32 rounding."""
+    import scipy.linalg as sl
+    rng = np.random.default_rng(100 + n)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    LU, piv, info = PS.lu_factor_f32(A)
+    LUr, pivr = sl.lu_factor(A)
+    assert info == 0
+    assert np.array_equal(piv, pivr), np.nonzero(piv != pivr)[0][:5]
+    # float32 rounding in another summation order, growing with n
+    assert np.linalg.norm(LU - LUr) <= 1e-5 * max(1.0, n / 256) * np.linalg.norm(LUr)
+    # and P A = L U in float64 arithmetic to float32 accuracy
+    L = np.tril(LU.astype(np.float64), -1) + np.eye(n)
+    U = np.triu(LU.astype(np.float64))
+    PA = A.astype(np.float64).copy()
+    for k, p in enumerate(piv):
+        PA[[k, p]] = PA[[p, k]]
+    assert np.linalg.norm(L @ U - PA) <= 1e-5 * np.linalg.norm(PA) * max(1.0, np.sqrt(n) / 4)
+
+
+def test_lu_factor_f32_zero_pivot():
+    A = np.ones((40, 40), dtype=np.float32)
+    LU, piv, info = PS.lu_factor_f32(A)
+    assert info == 2  # the second column has no nonzero pivot left
