@@ -281,6 +281,8 @@ typedef struct {
     double k;        /* core multiplier, k*sigma^2 (default 2) */
     double tol;      /* FMM tolerance (default 1e-6); 0: direct Oseen sum */
     double cutoff;   /* Oseen cutoff; <=0: 5*xopt (_induced.py:240) */
+    double leaf_side; /* FMM domain: 0 cost model, < 0 the bounding box, > 0 leaves at
+                         least leaf_side * cutoff wide (set by the autotuner) */
 } phf_vel_params;
 
 /* Work and timing of the last evaluation (stage ms are HIP-event times on

@@ -110,7 +110,7 @@ N_STAGES = len(STAGES)
 class VelParams(ctypes.Structure):
     _fields_ = [("method", _i32), ("prec", _i32), ("order", _i32), ("leaf", _i32),
                 ("ws", _i32), ("ndirect", _i32), ("sigma", _dbl), ("k", _dbl),
-                ("tol", _dbl), ("cutoff", _dbl)]
+                ("tol", _dbl), ("cutoff", _dbl), ("leaf_side", _dbl)]
 
 
 class EvalInfo(ctypes.Structure):

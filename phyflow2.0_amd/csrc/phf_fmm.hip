@@ -923,12 +923,12 @@ Lists make_lists(const Domain &dom, double c2, int ws)
 // for cutoffs large against the box) and the domain side from
 // choose_domain_side.
 Domain choose_domain(const double4 &hb, int L, double nb, double nt, double cutoff, int ws,
-                     double shape_log2, int p1, int chunk)
+                     double shape_log2, int p1, int chunk, double leaf_side)
 {
     const double c2 = cutoff * cutoff;
     for (;;) {
         const double side = choose_domain_side(hb.y - hb.x, hb.w - hb.z, L, nb, nt, cutoff, ws,
-                                               shape_log2, p1, chunk);
+                                               shape_log2, p1, chunk, leaf_side);
         const Domain d = make_domain(hb.x, hb.y, hb.z, hb.w, L, side);
         if (L == 0 || near_radius(near_set(d.s, c2, ws, near_window(d.s, c2, ws, L))) <= 6)
             return d;
@@ -1592,7 +1592,7 @@ static bool same_args(const FmmArgs &x, const FmmArgs &y)
     return x.xb == y.xb && x.yb == y.yb && x.wb == y.wb && x.nb == y.nb && x.xt == y.xt &&
            x.yt == y.yt && x.nt == y.nt && x.same == y.same && x.xopt == y.xopt &&
            x.cutoff == y.cutoff && x.p == y.p && x.ws == y.ws && x.prec == y.prec &&
-           x.Lhint == y.Lhint && x.qr == y.qr && x.qi == y.qi &&
+           x.Lhint == y.Lhint && x.leaf_side == y.leaf_side && x.qr == y.qr && x.qi == y.qi &&
            x.velocity_out == y.velocity_out;
 }
 
@@ -1779,7 +1779,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         PHF_HIP(hipMemcpyAsync(&hb, bb, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
         PHF_HIP(hipStreamSynchronize(L->stream));
         dom = choose_domain(hb, a.Lhint, (double)a.nb, (double)a.nt, a.cutoff, a.ws, shape_log2,
-                            a.p + 1, a.prec == PHF_PREC_F32C ? 16 : 64);
+                            a.p + 1, a.prec == PHF_PREC_F32C ? 16 : 64, a.leaf_side);
         dom.org = bb;
         if (graph) {
             // the recording computes the bounding box again in its own slot
@@ -1805,7 +1805,7 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
             info->domain_ok = [hb, a, L0, D0, shape_log2]() {
                 const Domain d = choose_domain(*hb, a.Lhint, (double)a.nb, (double)a.nt,
                                                a.cutoff, a.ws, shape_log2, a.p + 1,
-                                               a.prec == PHF_PREC_F32C ? 16 : 64);
+                                               a.prec == PHF_PREC_F32C ? 16 : 64, a.leaf_side);
                 return d.L == L0 && d.D == D0;
             };
         }

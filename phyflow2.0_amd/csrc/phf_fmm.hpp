@@ -15,6 +15,7 @@ struct FmmArgs {
     bool same;                   // targets are the sources
     double xopt, cutoff;         // Oseen smoother (fmm2d_py.cpp:61)
     int p, ws, prec, Lhint;      // order, well-separation, precision, leaf level
+    double leaf_side = 0.0;      // domain choice (choose_domain_side)
     double *qr, *qi;             // device outputs in target input order
     bool velocity_out = false;   // qr, qi receive (vx, vy) = (-qi, -qr)/(2 pi)
 };

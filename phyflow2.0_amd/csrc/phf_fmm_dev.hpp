@@ -115,15 +115,20 @@ __host__ __device__ inline double leaf_work(double ex, double ey, double D, int 
 
 // Domain side at leaf level L: the bounding-box side, or one enlarged so
 // that leaves are at least the Oseen cutoff wide (the near field is then
-// the 3 x 3 block of leaves) when leaf_work rates it cheaper.
+// the 3 x 3 block of leaves) when leaf_work rates it cheaper.  leaf_side
+// overrides the choice: < 0 the bounding box, > 0 leaves at least
+// leaf_side * cutoff wide (an autotuned setting); 0 the cost model.
 __host__ __device__ inline double choose_domain_side(double ex, double ey, int L, double nb,
                                                      double nt, double cutoff, int ws,
-                                                     double shape_log2, int p1, int chunk)
+                                                     double shape_log2, int p1, int chunk,
+                                                     double leaf_side = 0.0)
 {
 #pragma clang fp contract(off)
     const double ext = ex > ey ? ex : ey;
     const double D = domain_side(ext, 0.0);
-    if (L < 1) return D;
+    if (L < 1 || leaf_side < 0.0) return D;
+    if (leaf_side > 0.0)
+        return domain_side(ext, leaf_side * cutoff * ldexp(1.0, L) * (1.0 + 1e-12));
     const double Dc = domain_side(ext, cutoff * ldexp(1.0, L) * (1.0 + 1e-12));
     if (!(Dc > D)) return D;
     const double w0 = leaf_work(ex, ey, D, L, nb, nt, cutoff, ws, shape_log2, p1, chunk);

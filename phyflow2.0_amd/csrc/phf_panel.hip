@@ -424,8 +424,9 @@ __device__ __forceinline__ float wave_max(float v)
 
 // Panel columns k1 .. k1 + w of rows k1 .. n - 1, one row per thread in
 // registers, held rotated: a[j] is the entry of column c + j at step c, so
-// every register index is fixed (the loop is unrolled in full, so the
-// rotation costs no instructions).  Rows are exchanged by label only: each
+// every register index is fixed (the rotation is NB register moves per
+// column, cheaper than the instruction-cache misses of a fully unrolled
+// loop).  Rows are exchanged by label only: each
 // thread keeps its content and carries the row index it currently
 // occupies.  Column c of a content is final after step c and is stored at
 // the content's own (starting) row; once the panel is done, the contents

@@ -58,6 +58,8 @@ int check_vel_params(const phf_vel_params *p)
     PHF_REQUIRE(std::isfinite(p->tol) && p->tol >= 0, "tol must be finite and >= 0 (got %g)",
                 p->tol);
     PHF_REQUIRE(std::isfinite(p->cutoff), "cutoff must be finite");
+    PHF_REQUIRE(std::isfinite(p->leaf_side) && p->leaf_side <= 16.0,
+                "leaf_side must be finite and <= 16 (got %g)", p->leaf_side);
     PHF_REQUIRE(p->order <= 40, "order must be <= 40 (got %d)", p->order);
     PHF_REQUIRE(p->ws <= 3, "ws must be <= 3 (got %d)", p->ws);
     return PHF_OK;
@@ -101,6 +103,7 @@ int velocity_device(Lane *L, const double *x, const double *y, const double *w, 
         a.ws = p.ws > 0 ? p.ws : 1;
         a.prec = p.prec;
         a.Lhint = fmm_level(n, p.leaf, p.ndirect > 0 ? p.ndirect : 35);
+        a.leaf_side = p.leaf_side;
         a.qr = vx;
         a.qi = vy;
         a.velocity_out = true;

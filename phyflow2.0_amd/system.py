@@ -12,11 +12,14 @@ __all__ = ["BlobSystem", "vel_params", "METHOD_FMM", "METHOD_DIRECT", "PREC_F64"
 
 
 def vel_params(sigma, k=2.0, method=METHOD_FMM, prec=PREC_F64, order=0, leaf=0, ws=0,
-               ndirect=35, tol=1.0e-6, cutoff=None):
+               ndirect=35, tol=1.0e-6, cutoff=None, leaf_side=0.0):
     """Parameters of velocity() (_induced.py:34-36 defaults: k=2, Ndirect=35,
-    tol=1e-6, cutoff=None -> 5*xopt)."""
+    tol=1e-6, cutoff=None -> 5*xopt).  leaf_side selects the FMM domain:
+    0 the cost model, < 0 the bounding box, > 0 leaves at least leaf_side
+    cutoffs wide (phyflow_amd.autotune chooses it with leaf, ws and order)."""
     return VelParams(int(method), int(prec), int(order), int(leaf), int(ws), int(ndirect),
-                     float(sigma), float(k), float(tol), float(cutoff) if cutoff else 0.0)
+                     float(sigma), float(k), float(tol), float(cutoff) if cutoff else 0.0,
+                     float(leaf_side))
 
 
 class BlobSystem:
