@@ -31,6 +31,7 @@ from __graft_entry__ import load_package  # noqa: E402
 load_package()
 
 from phyflow_amd import _native  # noqa: E402
+from phyflow_amd import autotune  # noqa: E402
 from phyflow_amd import workloads as W  # noqa: E402
 from phyflow_amd.cpp.blobs.kernels import gauss  # noqa: E402
 from phyflow_amd.system import BlobSystem, vel_params  # noqa: E402
@@ -641,6 +642,17 @@ def main():
         e64 = rel_errors(sysm, p64, oref) if oref else (None, None)
         f64_same = {"prec": "f64", "order": args.order, "ms_per_step": 1e2 * w64,
                     "evals_per_s": args.n / (w64 / 10), "rel_l2": e64[0], "max_err": e64[1]}
+        # the measured settings table's choice for this size, precision and
+        # tolerance, timed and checked the same way
+        tp = autotune.tuned_params(s["sigma"], args.n, prec=prec, tol=1e-6)
+        tw, tlat, _, tinfo = timed_steps(sysm, tp, 5, 50, counts=False)
+        te = rel_errors(sysm, tp, oref) if oref else (None, None)
+        tuned = {"table": "phyflow_amd/autotune_table.json", "tol": 1e-6, "order": tp.order,
+                 "leaf": tp.leaf, "ws": tp.ws, "leaf_side": tp.leaf_side, "levels": tinfo["L"],
+                 "near_boxes": tinfo["nnear"], "ms_per_step": 1e3 * tw / 50,
+                 "p50_ms": float(np.percentile(tlat, 50) * 1e3),
+                 "p99_ms": float(np.percentile(tlat, 99) * 1e3),
+                 "evals_per_s": args.n / (tw / 50), "rel_l2": te[0], "max_err": te[1]}
         variants = []
         if args.variants:
             for pr in ("f32c", "f64"):
@@ -700,6 +712,7 @@ def main():
             "stage_ms": info["ms"], "eager_stage_ms": eager_ms, "device": dev["name"],
             "pcie_inclusive_evals_per_s": pcie_rate,
             "f64_same_config": f64_same,
+            "autotuned": tuned,
         },
         "roofline": {
             "kernel": "p2p_near_f32s" if prec == gauss.PREC_F32C else "p2p_near_f64s",

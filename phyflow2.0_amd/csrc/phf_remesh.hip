@@ -11,17 +11,26 @@
 //   duplicates summed, zero-weight touched cells kept, output in
 //   row-major (i, then j) order, x_new = (i h + h/2) + x0            (:200-218)
 //
-// GPU design (no float atomics, one sort of n keys instead of 16n):
-//   1. each blob's base cell (iL, jL) -> bounds of the touched range
-//   2. stable radix sort of blobs by base-cell key (row-major in range)
-//   3. a cell is touched iff one of its 16 base cells (i-a, j-b),
+// GPU design (no float atomics, one sort of n keys instead of 16n, no host
+// synchronisation before the output count is read):
+//   1. each blob's base cell (iL, jL) -> bounds and layout of the touched
+//      range, on the device
+//   2. stable radix sort of blobs by base-cell key (row-major in range);
+//      cell starts from the per-cell populations
+//   3. the 16 stencil values of every blob, laid out slot-major
+//   4. a cell is touched iff one of its 16 base cells (i-a, j-b),
 //      a, b in 0..3, holds a blob; exclusive scan of touched flags gives
 //      each output cell its row-major slot
-//   4. each touched cell sums its contributions slot-major (s = 0..15),
+//   5. each touched cell sums its contributions slot-major (s = 0..15),
 //      blobs of one base cell in ascending input index: exactly the order
 //      of a stable sort of the reference's COO entries (entry e = s*n + b),
 //      so the float64 sums are reproducible run to run and match the CPU
 //      restatement.
+// Kernels over cells run grid-stride over the count the device computed.
+// A range too large for the dense layout (far outliers) takes the sparse
+// path: occupied base cells and their neighbours as sorted lists (the
+// reference's scipy.sparse assembly needs no dense grid either), with the
+// same cells, order and sums.
 // Index and weight arithmetic runs in float64 with contraction disabled,
 // so cell indices and output coordinates are bit-exact to the reference
 // formulas; the cube |u|^3 is formed in double-double and rounded once (the
@@ -48,6 +57,36 @@ __global__ void m4kernel_1d_kernel(const double *__restrict__ x, int64_t n, doub
 
 // ---------------------------------------------------------- base cells --
 constexpr int RM_THREADS = 256;
+constexpr int RM_GRID = 2048;  // workgroups of the grid-stride cell kernels
+
+// Geometry of one remesh, computed on the device from the blobs' base cells.
+// The dense layout indexes base cells (i - imin) bj + (j - jmin) and touched
+// cells (i - imin) nj + (j - jmin), nj = bj + 3; it is used when the touched
+// range fits the capacity the buffers were sized for (ok), else the sparse
+// path runs from imin .. jmax.
+struct RmGeom {
+    long long imin, imax, jmin, jmax;  // base-cell bounds
+    long long bi, bj, nj;              // dense layout (0 when not ok)
+    int64_t nstart;                    // bi bj + 1 cell starts
+    int64_t ncell;                     // (bi + 3) nj touched-range cells
+    int64_t nscan;                     // ncell + 1
+    int64_t m;                         // output cells (set by the gather)
+    int32_t ok;
+    int32_t bits;                      // key bits of the base-cell index
+};
+
+// Touched-range capacity of the dense layout for n blobs.
+static int64_t dense_cells_cap(int64_t n)
+{
+    return std::min<int64_t>(64 * n + (int64_t(1) << 20), (int64_t(1) << 31) - 2);
+}
+
+static int bits_for(uint64_t count)
+{
+    int b = 1;
+    while (b < 32 && (uint64_t(1) << b) < count) ++b;
+    return b;
+}
 
 // per-block min/max of (iL, jL): longlong4 {imin, imax, jmin, jmax}
 __global__ __launch_bounds__(RM_THREADS) void base_bounds_partial(
@@ -77,8 +116,11 @@ __global__ __launch_bounds__(RM_THREADS) void base_bounds_partial(
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+// Reduces the partial bounds and lays out the dense range (capacity ccap
+// touched cells).
 __global__ __launch_bounds__(RM_THREADS) void base_bounds_final(const longlong4 *__restrict__ part,
-                                                                int np, longlong4 *__restrict__ out)
+                                                                int np, int64_t ccap,
+                                                                RmGeom *__restrict__ g)
 {
     __shared__ longlong4 sh[RM_THREADS];
     longlong4 acc = make_longlong4(LLONG_MAX, LLONG_MIN, LLONG_MAX, LLONG_MIN);
@@ -97,229 +139,552 @@ __global__ __launch_bounds__(RM_THREADS) void base_bounds_final(const longlong4 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *out = sh[0];
+    if (threadIdx.x != 0) return;
+    const longlong4 b = sh[0];
+    RmGeom r;
+    r.imin = b.x;
+    r.imax = b.y;
+    r.jmin = b.z;
+    r.jmax = b.w;
+    // spans in float64 first: far outliers can take them past 64 bits
+    const double si = (double)b.y - (double)b.x + 1.0, sj = (double)b.w - (double)b.z + 1.0;
+    r.ok = si >= 1.0 && sj >= 1.0 && (si + 3.0) * (sj + 3.0) <= (double)ccap;
+    r.bi = r.bj = r.nj = 0;
+    r.nstart = r.ncell = r.nscan = 0;
+    r.bits = 0;
+    r.m = -1;
+    if (r.ok) {
+        r.bi = b.y - b.x + 1;
+        r.bj = b.w - b.z + 1;
+        r.nj = r.bj + 3;
+        r.nstart = r.bi * r.bj + 1;
+        r.ncell = (r.bi + 3) * r.nj;
+        r.nscan = r.ncell + 1;
+        int k = 1;
+        while (k < 31 && (1ll << k) < r.bi * r.bj) ++k;
+        r.bits = k;
+    }
+    *g = r;
 }
 
-// key = (iL - imin) * nj + (jL - jmin), the blob's base cell in the range
+// v[0 .. *count) = 0 (grid-stride)
+__global__ __launch_bounds__(RM_THREADS) void zero_u32_dn(uint32_t *__restrict__ v,
+                                                          const int64_t *__restrict__ count)
+{
+    const int64_t c = *count;
+    for (int64_t k = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x; k < c;
+         k += (int64_t)gridDim.x * RM_THREADS)
+        v[k] = 0u;
+}
+
+// key = (iL - imin) bj + (jL - jmin), the blob's base cell in the dense
+// range, and the population of each base cell
 __global__ void base_keys(const double *__restrict__ x, const double *__restrict__ y, int64_t n,
-                          double x0, double y0, double h, long long imin, long long jmin,
-                          long long nj, uint32_t *__restrict__ keys, int32_t *__restrict__ vals)
+                          double x0, double y0, double h, const RmGeom *__restrict__ g,
+                          uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
+                          uint32_t *__restrict__ count)
 {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n) return;
-    const long long i = base_index(x[b], x0, h) - imin, j = base_index(y[b], y0, h) - jmin;
-    keys[b] = (uint32_t)(i * nj + j);
+    uint32_t key = 0u;
+    if (g->ok) {
+        const long long i = base_index(x[b], x0, h) - g->imin, j = base_index(y[b], y0, h) - g->jmin;
+        key = (uint32_t)(i * g->bj + j);
+        atomicAdd(count + key, 1u);
+    }
+    keys[b] = key;
     vals[b] = (int32_t)b;
 }
 
-// sorted blob data: (x/h, y/h, w)
-__global__ void gather_scaled(const double *__restrict__ x, const double *__restrict__ y,
-                              const double *__restrict__ w, const int32_t *__restrict__ perm,
-                              int64_t n, double h, double3 *__restrict__ out)
+// The 16 stencil values of each blob in base-cell order: V[s n + q] for
+// sorted blob q = perm^-1(b) and slot s = a + 4 b'.  M4' (KIND 0):
+// (M4'(ux) M4'(uy)) w with u = ((i + 0.5) + x0/h) - x/h, i = iL + a
+// (_regridding.py:170-184); Tutty (KIND 1): (F[a] G[b']) w with F, G the
+// weights about the node right of the base cell (_evolution.py:385-405).
+// Each value feeds exactly one output cell.
+template <int KIND>
+__global__ __launch_bounds__(RM_THREADS) void stencil_values(
+    const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ w,
+    const int32_t *__restrict__ perm, int64_t n, double x0, double y0, double h, double x0h,
+    double y0h, double c, double c2, double *__restrict__ V)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int32_t b = perm[k];
-    out[k] = make_double3(x[b] / h, y[b] / h, w[b]);
-}
-
-// Cells of the output range (ni x nj, range indices offset by imin, jmin;
-// base-cell range is (ni-3) x (nj-3)).  touched = any of the 16 base cells
-// (i-a, j-b) non-empty.
-__global__ void touched_flags(const int32_t *__restrict__ cstart, long long bi, long long bj,
-                              long long ni, long long nj, uint32_t *__restrict__ flag)
-{
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ni * nj) return;
-    long long i, j;
-    divmod_cells(c, nj, i, j);
-    uint32_t t = 0;
-    for (int s = 0; s < 16 && !t; ++s) {
-        const long long ii = i - (s & 3), jj = j - (s >> 2);
-        if (ii < 0 || jj < 0 || ii >= bi || jj >= bj) continue;
-        const int64_t k = ii * bj + jj;
-        t = cstart[k + 1] > cstart[k];
-    }
-    flag[c] = t;
-}
-
-__global__ void gather_sum(const int32_t *__restrict__ cstart, const double3 *__restrict__ sb,
-                           const uint32_t *__restrict__ flag, const uint32_t *__restrict__ slot,
-                           long long bi, long long bj, long long ni, long long nj,
-                           long long imin, long long jmin, double h, double x0, double y0,
-                           double x0h, double y0h, double c, double c2,
-                           double *__restrict__ xo, double *__restrict__ yo,
-                           double *__restrict__ wo, int64_t *__restrict__ io,
-                           int64_t *__restrict__ jo)
-{
-    const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cidx >= ni * nj || !flag[cidx]) return;
-    long long i, j;
-    divmod_cells(cidx, nj, i, j);
-    const long long I = i + imin, J = j + jmin;  // absolute cell indices
-    const double ui = ((double)I + 0.5) + x0h;
-    const double uj = ((double)J + 0.5) + y0h;
-    double acc = 0.0;
-    bool first = true;
-    for (int s = 0; s < 16; ++s) {
-        const long long ii = i - (s & 3), jj = j - (s >> 2);
-        if (ii < 0 || jj < 0 || ii >= bi || jj >= bj) continue;
-        const int64_t k = ii * bj + jj;
-        for (int32_t q = cstart[k]; q < cstart[k + 1]; ++q) {
-            const double3 p = sb[q];
-            const double v = (m4p_dev(ui - p.x, c, c2) * m4p_dev(uj - p.y, c, c2)) * p.z;
-            acc = first ? v : acc + v;
-            first = false;
+    const int64_t q = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x;
+    if (q >= n) return;
+    const int32_t b = perm[q];
+    const double xb = x[b], yb = y[b], wb = w[b];
+    const long long iL = base_index(xb, x0, h), jL = base_index(yb, y0, h);
+    double fx[4], fy[4];
+    if (KIND == 1) {
+        const double hh = 0.5 * h;
+        const double xc = ((double)(iL + 1) * h + hh) + x0;
+        const double yc = ((double)(jL + 1) * h + hh) + y0;
+        const double dx = (xb - xc) / h, dy = (yb - yc) / h;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            fx[a] = tutty_weight(a, dx, c2);
+            fy[a] = tutty_weight(a, dy, c2);
+        }
+    } else {
+        const double px = xb / h, py = yb / h;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            fx[a] = m4p_dev((((double)(iL + a) + 0.5) + x0h) - px, c, c2);
+            fy[a] = m4p_dev((((double)(jL + a) + 0.5) + y0h) - py, c, c2);
         }
     }
-    const uint32_t o = slot[cidx];
-    const double hh = 0.5 * h;
-    wo[o] = acc;
-    xo[o] = ((double)I * h + hh) + x0;
-    yo[o] = ((double)J * h + hh) + y0;
-    if (io) io[o] = I;
-    if (jo) jo[o] = J;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) V[(int64_t)s * n + q] = (fx[s & 3] * fy[s >> 2]) * wb;
 }
 
-// Tutty's diffusion-remesh (_evolution.py:325-445): same touched cells and
-// summation order as gather_sum, weights tutty_weight(a) x tutty_weight(b)
-// for slot (a, b) of the contributing blob; sb holds unscaled (x, y, w).
-__global__ void gather_sum_tutty(const int32_t *__restrict__ cstart, const double3 *__restrict__ sb,
-                                 const uint32_t *__restrict__ flag, const uint32_t *__restrict__ slot,
-                                 long long bi, long long bj, long long ni, long long nj,
-                                 long long imin, long long jmin, double h, double x0, double y0,
-                                 double c2, double *__restrict__ xo, double *__restrict__ yo,
-                                 double *__restrict__ wo, int64_t *__restrict__ io,
-                                 int64_t *__restrict__ jo)
+// Starts of the 16 base cells (i - a, j - b) of touched cell (i, j): st[a][e]
+// is the start of base cell (i - a, j - 3 + e), e = 0..4 (e = 4 closes
+// column j), 0 outside the range.
+__device__ __forceinline__ void base_starts(const int32_t *__restrict__ start, long long i,
+                                            long long j, long long bi, long long bj,
+                                            int32_t st[4][5], bool rowok[4])
 {
-    const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cidx >= ni * nj || !flag[cidx]) return;
-    long long i, j;
-    divmod_cells(cidx, nj, i, j);
-    const long long I = i + imin, J = j + jmin;
-    const double hh = 0.5 * h;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const long long ii = i - a;
+        rowok[a] = ii >= 0 && ii < bi;
+#pragma unroll
+        for (int e = 0; e < 5; ++e) {
+            const long long jj = j - 3 + e;
+            st[a][e] = (rowok[a] && jj >= 0 && jj <= bj) ? start[ii * bj + jj] : 0;
+        }
+    }
+}
+
+// touched flag of every cell of the dense range (grid-stride), and a
+// closing 0 so that the exclusive scan ends in the output count
+__global__ __launch_bounds__(RM_THREADS) void touched_cells(const int32_t *__restrict__ start,
+                                                            const RmGeom *__restrict__ g,
+                                                            uint32_t *__restrict__ flag)
+{
+    if (!g->ok) return;
+    const int64_t ncell = g->ncell;
+    const long long bi = g->bi, bj = g->bj, nj = g->nj;
+    for (int64_t cidx = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x; cidx <= ncell;
+         cidx += (int64_t)gridDim.x * RM_THREADS) {
+        if (cidx == ncell) {
+            flag[cidx] = 0u;
+            continue;
+        }
+        long long i, j;
+        divmod_cells(cidx, nj, i, j);
+        int32_t st[4][5];
+        bool rowok[4];
+        base_starts(start, i, j, bi, bj, st, rowok);
+        uint32_t t = 0u;
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const int a = s & 3, b = s >> 2;
+            const long long jj = j - b;
+            t |= (rowok[a] && jj >= 0 && jj < bj && st[a][4 - b] > st[a][3 - b]) ? 1u : 0u;
+        }
+        flag[cidx] = t;
+    }
+}
+
+// Sum of touched cell (i, j) slot-major (s = 0..15), blobs of one base cell
+// in ascending input index: the order of a stable sort of the reference's
+// COO entries (entry e = s n + b).
+__device__ __forceinline__ double cell_sum_dense(const int32_t st[4][5], const bool rowok[4],
+                                                 long long j, long long bj,
+                                                 const double *__restrict__ V, int64_t n)
+{
     double acc = 0.0;
     bool first = true;
+#pragma unroll
     for (int s = 0; s < 16; ++s) {
         const int a = s & 3, b = s >> 2;
-        const long long ii = i - a, jj = j - b;
-        if (ii < 0 || jj < 0 || ii >= bi || jj >= bj) continue;
-        const int64_t k = ii * bj + jj;
-        if (cstart[k + 1] <= cstart[k]) continue;
-        // node right of the base cell (ii + imin, jj + jmin)
-        const double xc = ((double)(ii + imin + 1) * h + hh) + x0;
-        const double yc = ((double)(jj + jmin + 1) * h + hh) + y0;
-        for (int32_t q = cstart[k]; q < cstart[k + 1]; ++q) {
-            const double3 p = sb[q];
-            const double v = (tutty_weight(a, (p.x - xc) / h, c2) *
-                              tutty_weight(b, (p.y - yc) / h, c2)) * p.z;
+        const long long jj = j - b;
+        if (!(rowok[a] && jj >= 0 && jj < bj)) continue;
+        const double *Vs = V + (int64_t)s * n;
+        for (int32_t q = st[a][3 - b]; q < st[a][4 - b]; ++q) {
+            const double v = Vs[q];
             acc = first ? v : acc + v;
             first = false;
         }
     }
-    const uint32_t o = slot[cidx];
-    wo[o] = acc;
-    xo[o] = ((double)I * h + hh) + x0;
-    yo[o] = ((double)J * h + hh) + y0;
-    if (io) io[o] = I;
-    if (jo) jo[o] = J;
+    return acc;
 }
 
-int remesh_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
-                  double sigma, double overlap, double x0, double y0, double c, RemeshOut *out,
-                  int kind)
+__global__ __launch_bounds__(RM_THREADS) void gather_cells(
+    const int32_t *__restrict__ start, const double *__restrict__ V, int64_t n,
+    const uint32_t *__restrict__ slot, RmGeom *__restrict__ g, double h, double x0, double y0,
+    double *__restrict__ xo, double *__restrict__ yo, double *__restrict__ wo,
+    int64_t *__restrict__ io, int64_t *__restrict__ jo)
 {
-    out->m = 0;
-    if (n == 0) return PHF_OK;
-    const double h = sigma * overlap;
-    // 1. bounds of base cells
-    const int nblk = (int)std::min<int64_t>(1024, (n + RM_THREADS * 8 - 1) / (RM_THREADS * 8));
-    longlong4 *part, *bnd;
-    PHF_ALLOC(part, L->arena, longlong4, (size_t)nblk);
-    PHF_ALLOC(bnd, L->arena, longlong4, 1);
-    base_bounds_partial<<<nblk, RM_THREADS, 0, L->stream>>>(x, y, n, x0, y0, h, part);
-    base_bounds_final<<<1, RM_THREADS, 0, L->stream>>>(part, nblk, bnd);
-    PHF_HIP(hipGetLastError());
-    longlong4 hb;
-    PHF_HIP(hipMemcpyAsync(&hb, bnd, sizeof(hb), hipMemcpyDeviceToHost, L->stream));
-    PHF_HIP(hipStreamSynchronize(L->stream));
-    const long long bi = hb.y - hb.x + 1, bj = hb.w - hb.z + 1;  // base-cell range
-    const long long ni = bi + 3, nj = bj + 3;                    // touched-cell range
-    if (!(bi > 0 && bj > 0) || (double)ni * (double)nj > 4.0e9 ||
-        (double)ni * (double)nj > 64.0 * (double)n + 1e6) {
-        set_error("remesh cell range %lld x %lld too large for %lld blobs", ni, nj, (long long)n);
-        return PHF_ERR_UNSUPPORTED;
+    if (!g->ok) return;
+    const int64_t ncell = g->ncell;
+    const long long bi = g->bi, bj = g->bj, nj = g->nj, imin = g->imin, jmin = g->jmin;
+    const int64_t t0 = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x;
+    if (t0 == 0) g->m = slot[ncell];
+    const double hh = 0.5 * h;
+    for (int64_t cidx = t0; cidx < ncell; cidx += (int64_t)gridDim.x * RM_THREADS) {
+        const uint32_t o = slot[cidx];
+        if (slot[cidx + 1] == o) continue;  // untouched
+        long long i, j;
+        divmod_cells(cidx, nj, i, j);
+        int32_t st[4][5];
+        bool rowok[4];
+        base_starts(start, i, j, bi, bj, st, rowok);
+        const long long I = i + imin, J = j + jmin;  // absolute cell indices
+        wo[o] = cell_sum_dense(st, rowok, j, bj, V, n);
+        xo[o] = ((double)I * h + hh) + x0;
+        yo[o] = ((double)J * h + hh) + y0;
+        if (io) io[o] = I;
+        if (jo) jo[o] = J;
     }
-    // 2. sort blobs by base cell
-    uint32_t *k0, *k1;
-    int32_t *v0, *v1;
-    PHF_ALLOC(k0, L->arena, uint32_t, (size_t)n);
-    PHF_ALLOC(k1, L->arena, uint32_t, (size_t)n);
-    PHF_ALLOC(v0, L->arena, int32_t, (size_t)n);
-    PHF_ALLOC(v1, L->arena, int32_t, (size_t)n);
-    base_keys<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, x0, y0, h, hb.x, hb.z, bj, k0, v0);
-    PHF_HIP(hipGetLastError());
-    int key_bits = 1;
-    while (key_bits < 32 && (1ull << key_bits) < (unsigned long long)(bi * bj)) ++key_bits;
-    uint32_t *ks;
-    int32_t *vs;
-    PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, n, key_bits, &ks, &vs));
-    int32_t *cstart;
-    PHF_ALLOC(cstart, L->arena, int32_t, (size_t)(bi * bj + 1));
-    PHF_TRY(leaf_starts_device(L, ks, n, bi * bj, cstart));
-    double3 *sb;
-    PHF_ALLOC(sb, L->arena, double3, (size_t)n);
-    // M4' works on x/h; Tutty on x itself
-    gather_scaled<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, w, vs, n, kind == 1 ? 1.0 : h, sb);
-    // 3. touched cells and their output slots
-    const int64_t ncell = ni * nj;
-    uint32_t *flag, *slot;
-    PHF_ALLOC(flag, L->arena, uint32_t, (size_t)ncell);
-    PHF_ALLOC(slot, L->arena, uint32_t, (size_t)ncell);
-    touched_flags<<<div_up(ncell, 256), 256, 0, L->stream>>>(cstart, bi, bj, ni, nj, flag);
-    PHF_HIP(hipGetLastError());
-    PHF_TRY(device_exclusive_scan(L, flag, ncell, slot));
-    uint32_t tail[2];
-    PHF_HIP(hipMemcpyAsync(&tail[0], slot + ncell - 1, 4, hipMemcpyDeviceToHost, L->stream));
-    PHF_HIP(hipMemcpyAsync(&tail[1], flag + ncell - 1, 4, hipMemcpyDeviceToHost, L->stream));
-    PHF_HIP(hipStreamSynchronize(L->stream));
-    const int64_t m = (int64_t)tail[0] + tail[1];
-    // 4. outputs
-    double *xo, *yo, *wo;
-    int64_t *io = nullptr, *jo = nullptr;
-    if (out->dst_x) {
-        PHF_REQUIRE(m <= out->dst_cap, "remesh output of %lld blobs exceeds capacity %lld",
-                    (long long)m, (long long)out->dst_cap);
-        xo = out->dst_x;
-        yo = out->dst_y;
-        wo = out->dst_w;
-    } else {
-        PHF_ALLOC(xo, L->arena, double, (size_t)std::max<int64_t>(m, 1));
-        PHF_ALLOC(yo, L->arena, double, (size_t)std::max<int64_t>(m, 1));
-        PHF_ALLOC(wo, L->arena, double, (size_t)std::max<int64_t>(m, 1));
+}
+
+// ------------------------------------------------------ sparse fallback --
+// Used when the blobs' bounding range is too large for the dense layout
+// (distant outliers, long thin wakes): only occupied base cells and their
+// 16 neighbours are kept, as sorted lists, which gives the same cells in
+// the same order and the same sums.
+
+__global__ void sparse_base_ij(const double *__restrict__ x, const double *__restrict__ y,
+                               int64_t n, double x0, double y0, double h, long long imin,
+                               long long jmin, uint32_t *__restrict__ bi_, uint32_t *__restrict__ bj_,
+                               int32_t *__restrict__ vals)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    bi_[b] = (uint32_t)(base_index(x[b], x0, h) - imin);
+    bj_[b] = (uint32_t)(base_index(y[b], y0, h) - jmin);
+    vals[b] = (int32_t)b;
+}
+
+// out[q] = in[idx[q]]
+__global__ void gather_u32(const uint32_t *__restrict__ in, const int32_t *__restrict__ idx,
+                           int64_t n, uint32_t *__restrict__ out)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = in[idx[q]];
+}
+
+// flag[q] = 1 where (a, b)[q] starts a run of equal pairs; flag[n] = 0
+__global__ void run_heads(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b, int64_t n,
+                          uint32_t *__restrict__ flag)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > n) return;
+    flag[q] = q == n ? 0u : (q == 0 || a[q] != a[q - 1] || b[q] != b[q - 1]) ? 1u : 0u;
+}
+
+// unique pairs as keys (a << 32 | b) with their first position
+__global__ void run_compact(const uint32_t *__restrict__ a, const uint32_t *__restrict__ b,
+                            int64_t n, const uint32_t *__restrict__ pos,
+                            uint64_t *__restrict__ ukey, int32_t *__restrict__ ustart)
+{
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > n) return;
+    if (q == n) {
+        ustart[pos[n]] = (int32_t)n;
+        return;
     }
-    if (out->want_indices) {
-        PHF_ALLOC(io, L->arena, int64_t, (size_t)std::max<int64_t>(m, 1));
-        PHF_ALLOC(jo, L->arena, int64_t, (size_t)std::max<int64_t>(m, 1));
+    if (pos[q + 1] == pos[q]) return;
+    ukey[pos[q]] = ((uint64_t)a[q] << 32) | b[q];
+    ustart[pos[q]] = (int32_t)q;
+}
+
+// the 16 touched cells (i + a, j + b) of each occupied base cell
+__global__ void stencil_cells(const uint64_t *__restrict__ ukey, int64_t U,
+                              uint32_t *__restrict__ ci, int32_t *__restrict__ cj)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 16 * U) return;
+    const int64_t u = e % U;
+    const int s = (int)(e / U);
+    ci[e] = (uint32_t)(ukey[u] >> 32) + (uint32_t)(s & 3);
+    cj[e] = (int32_t)((uint32_t)ukey[u] + (uint32_t)(s >> 2));
+}
+
+__device__ __forceinline__ int64_t find_key(const uint64_t *__restrict__ k, int64_t U, uint64_t v)
+{
+    int64_t lo = 0, hi = U;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (k[mid] < v)
+            lo = mid + 1;
+        else
+            hi = mid;
     }
+    return (lo < U && k[lo] == v) ? lo : -1;
+}
+
+__global__ __launch_bounds__(RM_THREADS) void gather_sparse(
+    const uint64_t *__restrict__ tkey, int64_t M, const uint64_t *__restrict__ ukey,
+    const int32_t *__restrict__ ustart, int64_t U, const double *__restrict__ V, int64_t n,
+    long long imin, long long jmin, double h, double x0, double y0, double *__restrict__ xo,
+    double *__restrict__ yo, double *__restrict__ wo, int64_t *__restrict__ io,
+    int64_t *__restrict__ jo)
+{
+    const int64_t t = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x;
+    if (t >= M) return;
+    const long long i = (long long)(tkey[t] >> 32), j = (long long)(uint32_t)tkey[t];
+    double acc = 0.0;
+    bool first = true;
+    for (int s = 0; s < 16; ++s) {
+        const long long ii = i - (s & 3), jj = j - (s >> 2);
+        if (ii < 0 || jj < 0) continue;
+        const int64_t u = find_key(ukey, U, ((uint64_t)ii << 32) | (uint64_t)jj);
+        if (u < 0) continue;
+        const double *Vs = V + (int64_t)s * n;
+        for (int32_t q = ustart[u]; q < ustart[u + 1]; ++q) {
+            const double v = Vs[q];
+            acc = first ? v : acc + v;
+            first = false;
+        }
+    }
+    const long long I = i + imin, J = j + jmin;
+    const double hh = 0.5 * h;
+    wo[t] = acc;
+    xo[t] = ((double)I * h + hh) + x0;
+    yo[t] = ((double)J * h + hh) + y0;
+    if (io) io[t] = I;
+    if (jo) jo[t] = J;
+}
+
+static int launch_stencil_values(Lane *L, int kind, const double *x, const double *y,
+                                 const double *w, const int32_t *perm, int64_t n, double x0,
+                                 double y0, double h, double c, double *V)
+{
     if (kind == 1) {
         const double r = c / h;  // c carries hNu = sqrt(nu dt)
-        gather_sum_tutty<<<div_up(ncell, 256), 256, 0, L->stream>>>(
-            cstart, sb, flag, slot, bi, bj, ni, nj, hb.x, hb.z, h, x0, y0, r * r, xo, yo, wo, io,
-            jo);
+        stencil_values<1><<<div_up(n, RM_THREADS), RM_THREADS, 0, L->stream>>>(
+            x, y, w, perm, n, x0, y0, h, 0.0, 0.0, 0.0, r * r, V);
     } else {
-        gather_sum<<<div_up(ncell, 256), 256, 0, L->stream>>>(
-            cstart, sb, flag, slot, bi, bj, ni, nj, hb.x, hb.z, h, x0, y0, x0 / h, y0 / h, c,
-            c * c, xo, yo, wo, io, jo);
+        stencil_values<0><<<div_up(n, RM_THREADS), RM_THREADS, 0, L->stream>>>(
+            x, y, w, perm, n, x0, y0, h, x0 / h, y0 / h, c, c * c, V);
     }
     PHF_HIP(hipGetLastError());
-    out->m = m;
+    return PHF_OK;
+}
+
+static int read_count(Lane *L, const uint32_t *d, int64_t *out)
+{
+    uint32_t v = 0;
+    PHF_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    *out = v;
+    return PHF_OK;
+}
+
+static int remesh_sparse(Lane *L, const double *x, const double *y, const double *w, int64_t n,
+                         double h, double x0, double y0, double c, int kind, const RmGeom &hg,
+                         RemeshOut *out)
+{
+    const double si = (double)hg.imax - (double)hg.imin + 1.0;
+    const double sj = (double)hg.jmax - (double)hg.jmin + 1.0;
+    if (!(si >= 1.0 && sj >= 1.0 && si + 3.0 < 2147483647.0 && sj + 3.0 < 2147483647.0)) {
+        set_error("remesh: the blobs span %.3g x %.3g cells (at most 2^31 - 4 per axis)", si, sj);
+        return PHF_ERR_UNSUPPORTED;
+    }
+    const uint32_t bi = (uint32_t)si, bj = (uint32_t)sj;
+    const size_t nn = (size_t)n;
+    // 1. blobs sorted by base cell (i, j): by j, then stably by i
+    uint32_t *ki, *kj, *t0, *t1, *t2;
+    int32_t *v0, *v1;
+    PHF_ALLOC(ki, L->arena, uint32_t, nn);
+    PHF_ALLOC(kj, L->arena, uint32_t, nn);
+    PHF_ALLOC(t0, L->arena, uint32_t, nn);
+    PHF_ALLOC(t1, L->arena, uint32_t, nn);
+    PHF_ALLOC(t2, L->arena, uint32_t, nn);
+    PHF_ALLOC(v0, L->arena, int32_t, nn);
+    PHF_ALLOC(v1, L->arena, int32_t, nn);
+    sparse_base_ij<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, x0, y0, h, hg.imin, hg.jmin,
+                                                          ki, kj, v0);
+    PHF_HIP(hipGetLastError());
+    PHF_HIP(hipMemcpyAsync(t0, kj, sizeof(uint32_t) * nn, hipMemcpyDeviceToDevice, L->stream));
+    uint32_t *ks;
+    int32_t *p1, *perm;
+    PHF_TRY(device_radix_sort(L, t0, v0, t1, v1, n, bits_for(bj), &ks, &p1));
+    uint32_t *ka = ks == t0 ? t1 : t0;  // the free key buffer
+    gather_u32<<<div_up(n, 256), 256, 0, L->stream>>>(ki, p1, n, ka);
+    PHF_TRY(device_radix_sort(L, ka, p1, ks, p1 == v0 ? v1 : v0, n, bits_for(bi), &ks, &perm));
+    uint32_t *is = ks, *js = t2;
+    gather_u32<<<div_up(n, 256), 256, 0, L->stream>>>(kj, perm, n, js);
+    PHF_HIP(hipGetLastError());
+    // 2. occupied base cells: keys (i << 32 | j) and blob ranges
+    uint32_t *flag, *pos, *sums;
+    PHF_ALLOC(flag, L->arena, uint32_t, nn + 1);
+    PHF_ALLOC(pos, L->arena, uint32_t, nn + 1);
+    PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(n + 1));
+    run_heads<<<div_up(n + 1, 256), 256, 0, L->stream>>>(is, js, n, flag);
+    PHF_TRY(device_exclusive_scan_ws(L, flag, n + 1, pos, sums));
+    int64_t U = 0;
+    PHF_TRY(read_count(L, pos + n, &U));
+    uint64_t *ukey;
+    int32_t *ustart;
+    PHF_ALLOC(ukey, L->arena, uint64_t, (size_t)U);
+    PHF_ALLOC(ustart, L->arena, int32_t, (size_t)U + 1);
+    run_compact<<<div_up(n + 1, 256), 256, 0, L->stream>>>(is, js, n, pos, ukey, ustart);
+    PHF_HIP(hipGetLastError());
+    // 3. touched cells: the 16 neighbours of each occupied base cell, sorted
+    //    row-major (by j, then stably by i) and made unique
+    const int64_t E = 16 * U;
+    const size_t ne = (size_t)E;
+    uint32_t *ci, *ck0, *ck1;
+    int32_t *cj, *cv0, *cv1;
+    PHF_ALLOC(ci, L->arena, uint32_t, ne);
+    PHF_ALLOC(cj, L->arena, int32_t, ne);
+    PHF_ALLOC(ck0, L->arena, uint32_t, ne);
+    PHF_ALLOC(ck1, L->arena, uint32_t, ne);
+    PHF_ALLOC(cv0, L->arena, int32_t, ne);
+    PHF_ALLOC(cv1, L->arena, int32_t, ne);
+    stencil_cells<<<div_up(E, 256), 256, 0, L->stream>>>(ukey, U, ci, cj);
+    PHF_HIP(hipGetLastError());
+    // sort 1: key j, value i
+    PHF_HIP(hipMemcpyAsync(ck0, cj, sizeof(uint32_t) * ne, hipMemcpyDeviceToDevice, L->stream));
+    PHF_HIP(hipMemcpyAsync(cv0, ci, sizeof(int32_t) * ne, hipMemcpyDeviceToDevice, L->stream));
+    uint32_t *sj_k;
+    int32_t *si_v;
+    PHF_TRY(device_radix_sort(L, ck0, cv0, ck1, cv1, E, bits_for((uint64_t)bj + 3), &sj_k, &si_v));
+    // sort 2: key i, value j
+    uint32_t *k2 = sj_k == ck0 ? ck1 : ck0;
+    int32_t *v2 = si_v == cv0 ? cv1 : cv0;
+    PHF_HIP(hipMemcpyAsync(k2, si_v, sizeof(uint32_t) * ne, hipMemcpyDeviceToDevice, L->stream));
+    PHF_HIP(hipMemcpyAsync(v2, sj_k, sizeof(int32_t) * ne, hipMemcpyDeviceToDevice, L->stream));
+    uint32_t *ti;
+    int32_t *tj;
+    PHF_TRY(device_radix_sort(L, k2, v2, sj_k, si_v, E, bits_for((uint64_t)bi + 3), &ti, &tj));
+    uint32_t *cflag, *cpos, *csums;
+    PHF_ALLOC(cflag, L->arena, uint32_t, ne + 1);
+    PHF_ALLOC(cpos, L->arena, uint32_t, ne + 1);
+    PHF_ALLOC(csums, L->arena, uint32_t, (size_t)scan_sums_size(E + 1));
+    run_heads<<<div_up(E + 1, 256), 256, 0, L->stream>>>(ti, reinterpret_cast<uint32_t *>(tj), E,
+                                                         cflag);
+    PHF_TRY(device_exclusive_scan_ws(L, cflag, E + 1, cpos, csums));
+    int64_t M = 0;
+    PHF_TRY(read_count(L, cpos + E, &M));
+    uint64_t *tkey;
+    int32_t *tstart;
+    PHF_ALLOC(tkey, L->arena, uint64_t, (size_t)std::max<int64_t>(M, 1));
+    PHF_ALLOC(tstart, L->arena, int32_t, (size_t)M + 1);
+    run_compact<<<div_up(E + 1, 256), 256, 0, L->stream>>>(ti, reinterpret_cast<uint32_t *>(tj),
+                                                           E, cpos, tkey, tstart);
+    PHF_HIP(hipGetLastError());
+    // 4. values and sums
+    double *V;
+    PHF_ALLOC(V, L->arena, double, 16 * nn);
+    PHF_TRY(launch_stencil_values(L, kind, x, y, w, perm, n, x0, y0, h, c, V));
+    double *xo, *yo, *wo;
+    int64_t *io = nullptr, *jo = nullptr;
+    const size_t mm = (size_t)std::max<int64_t>(M, 1);
+    PHF_ALLOC(xo, L->arena, double, mm);
+    PHF_ALLOC(yo, L->arena, double, mm);
+    PHF_ALLOC(wo, L->arena, double, mm);
+    if (out->want_indices) {
+        PHF_ALLOC(io, L->arena, int64_t, mm);
+        PHF_ALLOC(jo, L->arena, int64_t, mm);
+    }
+    gather_sparse<<<div_up(M, RM_THREADS), RM_THREADS, 0, L->stream>>>(
+        tkey, M, ukey, ustart, U, V, n, hg.imin, hg.jmin, h, x0, y0, xo, yo, wo, io, jo);
+    PHF_HIP(hipGetLastError());
+    out->m = M;
     out->x = xo;
     out->y = yo;
     out->w = wo;
     out->i = io;
     out->j = jo;
-    out->cells = ncell;
+    out->cells = M;
+    out->sparse = true;
+    return PHF_OK;
+}
+
+// Dense path: no host synchronisation until the output count is read at
+// the end (with whatever the caller queued before the call).  The touched
+// range is laid out on the device; if it exceeds the capacity the buffers
+// were sized for, the sparse path runs instead.
+int remesh_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
+                  double sigma, double overlap, double x0, double y0, double c, RemeshOut *out,
+                  int kind)
+{
+    out->m = 0;
+    out->sparse = false;
+    if (n == 0) return PHF_OK;
+    PHF_REQUIRE(n < (int64_t(1) << 31), "remesh of %lld blobs (at most 2^31 - 1)", (long long)n);
+    const double h = sigma * overlap;
+    const int64_t ccap = dense_cells_cap(n);
+    const size_t nn = (size_t)n;
+    // 1. bounds and layout of the base cells
+    const int nblk = (int)std::min<int64_t>(1024, (n + RM_THREADS * 8 - 1) / (RM_THREADS * 8));
+    longlong4 *part;
+    RmGeom *g;
+    PHF_ALLOC(part, L->arena, longlong4, (size_t)nblk);
+    PHF_ALLOC(g, L->arena, RmGeom, 1);
+    base_bounds_partial<<<nblk, RM_THREADS, 0, L->stream>>>(x, y, n, x0, y0, h, part);
+    base_bounds_final<<<1, RM_THREADS, 0, L->stream>>>(part, nblk, ccap, g);
+    PHF_HIP(hipGetLastError());
+    // 2. blobs sorted by base cell; cell starts from the cell populations
+    uint32_t *k0, *k1, *cnt, *sums;
+    int32_t *v0, *v1;
+    PHF_ALLOC(k0, L->arena, uint32_t, nn);
+    PHF_ALLOC(k1, L->arena, uint32_t, nn);
+    PHF_ALLOC(v0, L->arena, int32_t, nn);
+    PHF_ALLOC(v1, L->arena, int32_t, nn);
+    PHF_ALLOC(cnt, L->arena, uint32_t, (size_t)ccap + 1);
+    PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(ccap + 1));
+    zero_u32_dn<<<RM_GRID, RM_THREADS, 0, L->stream>>>(cnt, &g->nstart);
+    base_keys<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, x0, y0, h, g, k0, v0, cnt);
+    PHF_HIP(hipGetLastError());
+    uint32_t *ks;
+    int32_t *perm;
+    PHF_TRY(device_radix_sort_dbits(L, k0, v0, k1, v1, n, bits_for((uint64_t)ccap), &g->bits, &ks,
+                                    &perm));
+    uint32_t *ustart;
+    PHF_ALLOC(ustart, L->arena, uint32_t, (size_t)ccap + 1);
+    PHF_TRY(device_exclusive_scan_ws(L, cnt, ccap + 1, ustart, sums, &g->nstart));
+    const int32_t *start = reinterpret_cast<const int32_t *>(ustart);
+    // 3. stencil values of every blob
+    double *V;
+    PHF_ALLOC(V, L->arena, double, 16 * nn);
+    PHF_TRY(launch_stencil_values(L, kind, x, y, w, perm, n, x0, y0, h, c, V));
+    // 4. touched cells and their output slots
+    uint32_t *flag = cnt, *slot;  // the populations are no longer needed
+    PHF_ALLOC(slot, L->arena, uint32_t, (size_t)ccap + 1);
+    touched_cells<<<RM_GRID, RM_THREADS, 0, L->stream>>>(start, g, flag);
+    PHF_HIP(hipGetLastError());
+    PHF_TRY(device_exclusive_scan_ws(L, flag, ccap + 1, slot, sums, &g->nscan));
+    // 5. sums into outputs sized for the most cells n blobs can touch
+    const size_t cap = (size_t)std::min<int64_t>(16 * n, ccap);
+    double *xo, *yo, *wo;
+    int64_t *io = nullptr, *jo = nullptr;
+    PHF_ALLOC(xo, L->arena, double, cap);
+    PHF_ALLOC(yo, L->arena, double, cap);
+    PHF_ALLOC(wo, L->arena, double, cap);
+    if (out->want_indices) {
+        PHF_ALLOC(io, L->arena, int64_t, cap);
+        PHF_ALLOC(jo, L->arena, int64_t, cap);
+    }
+    gather_cells<<<RM_GRID, RM_THREADS, 0, L->stream>>>(start, V, n, slot, g, h, x0, y0, xo, yo,
+                                                        wo, io, jo);
+    PHF_HIP(hipGetLastError());
+    // the one synchronisation: the layout with the output count, and what
+    // the caller asked to read back
+    PHF_TRY(L->pin.ensure(sizeof(RmGeom) + out->read_bytes));
+    PHF_HIP(hipMemcpyAsync(L->pin.ptr, g, sizeof(RmGeom), hipMemcpyDeviceToHost, L->stream));
+    if (out->read_bytes)
+        PHF_HIP(hipMemcpyAsync(L->pin.ptr + sizeof(RmGeom), out->read_src, out->read_bytes,
+                               hipMemcpyDeviceToHost, L->stream));
+    PHF_TRY(L->pin.mark(L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    RmGeom hg;
+    std::memcpy(&hg, L->pin.ptr, sizeof(hg));
+    if (out->read_bytes) std::memcpy(out->read_dst, L->pin.ptr + sizeof(RmGeom), out->read_bytes);
+    if (!hg.ok) return remesh_sparse(L, x, y, w, n, h, x0, y0, c, kind, hg, out);
+    PHF_REQUIRE(hg.m >= 0 && hg.m <= (int64_t)cap, "remesh output count %lld out of range",
+                (long long)hg.m);
+    out->m = hg.m;
+    out->x = xo;
+    out->y = yo;
+    out->w = wo;
+    out->i = io;
+    out->j = jo;
+    out->cells = hg.ncell;
     return PHF_OK;
 }
 

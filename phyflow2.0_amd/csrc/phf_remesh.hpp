@@ -6,19 +6,23 @@
 namespace phf {
 
 struct RemeshOut {
-    // inputs: optional caller-provided destinations (device) and capacity
-    double *dst_x = nullptr, *dst_y = nullptr, *dst_w = nullptr;
-    int64_t dst_cap = 0;
-    bool want_indices = false;
-    // outputs (device; arena-owned unless dst_* given)
+    bool want_indices = false;  // input: also return the cell indices
+    // input: device bytes read back to the host with the output count
+    const void *read_src = nullptr;
+    size_t read_bytes = 0;
+    void *read_dst = nullptr;
+    // outputs (device, arena-owned)
     int64_t m = 0;
     double *x = nullptr, *y = nullptr, *w = nullptr;
     int64_t *i = nullptr, *j = nullptr;
-    int64_t cells = 0;  // cells scanned (touched-range size)
+    int64_t cells = 0;    // cells scanned (dense touched range, or the sparse list)
+    bool sparse = false;  // the range was too large for the dense layout
 };
 
 // kind 0: M4' with parameter c (Regrid); kind 1: Tutty's diffusion weights
-// with c = hNu = sqrt(nu dt) (diffusion_tutty).
+// with c = hNu = sqrt(nu dt) (diffusion_tutty).  One host synchronisation,
+// at the end (it also completes copies the caller queued before).  Not
+// capturable.
 int remesh_device(Lane *L, const double *x, const double *y, const double *w, int64_t n,
                   double sigma, double overlap, double x0, double y0, double c, RemeshOut *out,
                   int kind = 0);

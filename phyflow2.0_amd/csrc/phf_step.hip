@@ -478,12 +478,13 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
     double4 hm[2] = {};
     if (sp->remesh && n > 0) {
         PHF_TRY(moments_device(L, s->x, s->y, s->w, n, dm));
-        RemeshOut ro;
+        RemeshOut ro;  // the moments come back with the remesh's output count
+        ro.read_src = dm;
+        ro.read_bytes = sizeof(double4);
+        ro.read_dst = &hm[0];
         PHF_TRY(remesh_device(L, s->x, s->y, s->w, n, sp->remesh_sigma, sp->remesh_overlap,
                               sp->remesh_x0, sp->remesh_y0, sp->remesh_c, &ro,
                               sp->remesh_kernel));
-        PHF_HIP(hipMemcpyAsync(&hm[0], dm, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
-        PHF_HIP(hipStreamSynchronize(L->stream));
         PHF_TRY(sys_reserve(s, ro.m));
         PHF_HIP(hipMemcpyAsync(s->x, ro.x, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
         PHF_HIP(hipMemcpyAsync(s->y, ro.y, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));

@@ -177,10 +177,15 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s
     return before + inc - v;
 }
 
+// With dn set, the scan covers the first *dn entries (a count on the
+// device, at most the n the grid was sized for); workgroups past it return.
 __global__ __launch_bounds__(SC_THREADS) void scan_reduce(const uint32_t *__restrict__ in,
-                                                          int64_t n, uint32_t *__restrict__ sums)
+                                                          int64_t n, uint32_t *__restrict__ sums,
+                                                          const int64_t *__restrict__ dn)
 {
     __shared__ uint32_t shw[SC_THREADS / 64];
+    if (dn) n = *dn;
+    if ((int64_t)blockIdx.x * SC_CHUNK >= n && blockIdx.x > 0) return;
     const int64_t base = (int64_t)blockIdx.x * SC_CHUNK + (int64_t)threadIdx.x * SC_ITEMS;
     uint32_t s = 0;
 #pragma unroll
@@ -191,9 +196,14 @@ __global__ __launch_bounds__(SC_THREADS) void scan_reduce(const uint32_t *__rest
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(SC_THREADS) void scan_top(uint32_t *__restrict__ sums, int64_t nb)
+__global__ __launch_bounds__(SC_THREADS) void scan_top(uint32_t *__restrict__ sums, int64_t nb,
+                                                       const int64_t *__restrict__ dn)
 {
     __shared__ uint32_t shw[SC_THREADS / 64];
+    if (dn) {
+        nb = (*dn + SC_CHUNK - 1) / SC_CHUNK;
+        nb = nb > 0 ? nb : 1;
+    }
     uint32_t carry = 0;
     for (int64_t b0 = 0; b0 < nb; b0 += SC_THREADS) {
         int64_t i = b0 + threadIdx.x;
@@ -209,9 +219,12 @@ __global__ __launch_bounds__(SC_THREADS) void scan_top(uint32_t *__restrict__ su
 __global__ __launch_bounds__(SC_THREADS) void scan_apply(const uint32_t *__restrict__ in,
                                                          int64_t n,
                                                          const uint32_t *__restrict__ sums,
-                                                         uint32_t *__restrict__ out)
+                                                         uint32_t *__restrict__ out,
+                                                         const int64_t *__restrict__ dn)
 {
     __shared__ uint32_t shw[SC_THREADS / 64];
+    if (dn) n = *dn;
+    if ((int64_t)blockIdx.x * SC_CHUNK >= n) return;
     const int64_t base = (int64_t)blockIdx.x * SC_CHUNK + (int64_t)threadIdx.x * SC_ITEMS;
     uint32_t v[SC_ITEMS];
     uint32_t s = 0;
@@ -239,13 +252,13 @@ int device_exclusive_scan(Lane *L, const uint32_t *in, int64_t n, uint32_t *out)
 }
 
 int device_exclusive_scan_ws(Lane *L, const uint32_t *in, int64_t n, uint32_t *out,
-                             uint32_t *sums)
+                             uint32_t *sums, const int64_t *dn)
 {
     if (n <= 0) return PHF_OK;
     int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
-    scan_reduce<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums);
-    scan_top<<<1, SC_THREADS, 0, L->stream>>>(sums, nb);
-    scan_apply<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums, out);
+    scan_reduce<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums, dn);
+    scan_top<<<1, SC_THREADS, 0, L->stream>>>(sums, nb, dn);
+    scan_apply<<<(unsigned)nb, SC_THREADS, 0, L->stream>>>(in, n, sums, out, dn);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -325,11 +338,15 @@ __global__ __launch_bounds__(RS_THREADS) void morton_tiles(
 
 // Tile histograms of the digit (key >> shift) & mask: hist[d][tile]
 // (digit-major, nblocks tiles per row).
+// dbits (nullable, device): passes with shift >= *dbits do nothing here and
+// copy their tile in rs_scatter.
 __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint32_t *__restrict__ keys,
                                                       int64_t n, int shift, uint32_t mask,
-                                                      uint32_t *__restrict__ hist, int nblocks)
+                                                      uint32_t *__restrict__ hist, int nblocks,
+                                                      const int32_t *__restrict__ dbits)
 {
     __shared__ uint32_t h[256];
+    if (dbits && shift >= *dbits) return;
     const int tid = threadIdx.x;
     if (tid < 256) h[tid] = 0;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
@@ -353,9 +370,11 @@ __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint32_t *__restrict
 // tot[d] = all tiles.
 __global__ __launch_bounds__(RS_SCAN) void rs_rowscan(const uint32_t *__restrict__ hist,
                                                       int nblocks, uint32_t *__restrict__ offs,
-                                                      uint32_t *__restrict__ tot)
+                                                      uint32_t *__restrict__ tot, int shift,
+                                                      const int32_t *__restrict__ dbits)
 {
     __shared__ uint32_t shw[RS_SCAN / 64];
+    if (dbits && shift >= *dbits) return;
     const int64_t row = (int64_t)blockIdx.x * nblocks;
     uint32_t carry = 0;
     for (int b0 = 0; b0 < nblocks; b0 += RS_SCAN) {
@@ -381,7 +400,8 @@ template <class VT>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     const uint32_t *__restrict__ kin, const VT *__restrict__ vin, uint32_t *__restrict__ kout,
     VT *__restrict__ vout, int64_t n, int shift, int nbits, const uint32_t *__restrict__ hist,
-    const uint32_t *__restrict__ offs, const uint32_t *__restrict__ tot, int nblocks)
+    const uint32_t *__restrict__ offs, const uint32_t *__restrict__ tot, int nblocks,
+    const int32_t *__restrict__ dbits)
 {
     __shared__ uint32_t lrun[256], lstart[256], gbase[256];
     __shared__ uint32_t cnt[RS_WAVES][256];
@@ -393,6 +413,13 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(
     const uint32_t mask = (1u << nbits) - 1u;
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
     const int tile_n = (int)min((int64_t)RS_TILE, n - base);
+    if (dbits && shift >= *dbits) {  // nothing left to sort: keep the order
+        for (int j = tid; j < tile_n; j += RS_THREADS) {
+            kout[base + j] = kin[base + j];
+            vout[base + j] = vin[base + j];
+        }
+        return;
+    }
     uint32_t kr[RS_ROUNDS];
     VT vr[RS_ROUNDS];
 #pragma unroll
@@ -484,7 +511,7 @@ int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, in
 template <class VT>
 static int radix_sort_ws(Lane *L, uint32_t *keys, VT *vals, uint32_t *ktmp, VT *vtmp, int64_t n,
                          int key_bits, uint32_t *hist, uint32_t *offs, uint32_t **kres,
-                         VT **vres, bool hist0 = false)
+                         VT **vres, bool hist0 = false, const int32_t *dbits = nullptr)
 {
     *kres = keys;
     *vres = vals;
@@ -497,10 +524,12 @@ static int radix_sort_ws(Lane *L, uint32_t *keys, VT *vals, uint32_t *ktmp, VT *
         const int nbits = std::min(8, key_bits - shift);
         const uint32_t mask = (1u << nbits) - 1u;
         if (shift > 0 || !hist0)
-            rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks);
-        rs_rowscan<<<1u << nbits, RS_SCAN, 0, L->stream>>>(hist, nblocks, offs, tot);
+            rs_hist<<<nblocks, RS_THREADS, 0, L->stream>>>(ka, n, shift, mask, hist, nblocks,
+                                                           dbits);
+        rs_rowscan<<<1u << nbits, RS_SCAN, 0, L->stream>>>(hist, nblocks, offs, tot, shift,
+                                                           dbits);
         rs_scatter<VT><<<nblocks, RS_THREADS, 0, L->stream>>>(ka, va, kb, vb, n, shift, nbits,
-                                                              hist, offs, tot, nblocks);
+                                                              hist, offs, tot, nblocks, dbits);
         PHF_HIP(hipGetLastError());
         std::swap(ka, kb);
         std::swap(va, vb);
@@ -515,6 +544,20 @@ int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp,
                          uint32_t **kres, int32_t **vres)
 {
     return radix_sort_ws<int32_t>(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, kres, vres);
+}
+
+int device_radix_sort_dbits(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
+                            int64_t n, int key_bits, const int32_t *dbits, uint32_t **kres,
+                            int32_t **vres)
+{
+    *kres = keys;
+    *vres = vals;
+    if (n <= 1 || key_bits <= 0) return PHF_OK;
+    uint32_t *hist, *offs;
+    PHF_ALLOC(hist, L->arena, uint32_t, (size_t)radix_ws_size(n));
+    PHF_ALLOC(offs, L->arena, uint32_t, (size_t)radix_ws_size(n));
+    return radix_sort_ws<int32_t>(L, keys, vals, ktmp, vtmp, n, key_bits, hist, offs, kres, vres,
+                                  false, dbits);
 }
 
 // --------------------------------------------------------- leaf offsets --

@@ -43,8 +43,9 @@ int device_bbox_parts(Lane *L, const double *x1, const double *y1, int64_t n1, c
 int device_exclusive_scan(Lane *L, const uint32_t *in, int64_t n, uint32_t *out);
 // variants with caller-provided workspaces (no allocation: capturable)
 int64_t scan_sums_size(int64_t n);
+// dn (device, nullable): scan only the first *dn <= n entries
 int device_exclusive_scan_ws(Lane *L, const uint32_t *in, int64_t n, uint32_t *out,
-                             uint32_t *sums);
+                             uint32_t *sums, const int64_t *dn = nullptr);
 int64_t radix_ws_size(int64_t n);  // entries of hist and of offs
 int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
                          int64_t n, int key_bits, uint32_t *hist, uint32_t *offs,
@@ -55,6 +56,12 @@ int device_radix_sort_ws(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp,
 // result.
 int device_radix_sort(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
                       int64_t n, int key_bits, uint32_t **kres, int32_t **vres);
+// The same with the key width known only on the device: key_bits passes'
+// worth of launches, the passes at or above *dbits copying their tile
+// unchanged (the result lands in the same buffer either way).
+int device_radix_sort_dbits(Lane *L, uint32_t *keys, int32_t *vals, uint32_t *ktmp, int32_t *vtmp,
+                            int64_t n, int key_bits, const int32_t *dbits, uint32_t **kres,
+                            int32_t **vres);
 
 struct SortedPoints {
     int64_t n = 0;

@@ -155,3 +155,62 @@ def test_step_with_wee_remesh_moments():
     assert abs(info["impulse_y_after"] - info["impulse_y_before"]) <= 1e-13 * scale
     d2 = info["m2_after"] - info["m2_before"]
     assert abs(d2 - 4 * c * c * h * h * info["circ_before"]) <= 1e-12 * scale
+
+
+def _spread_case(kind):
+    """Blob sets whose touched-cell range exceeds the dense layout's capacity
+    (64 n + 2^20 cells), so the remesh takes its sparse path."""
+    rng = np.random.default_rng(21 if kind == "outliers" else 22)
+    if kind == "outliers":
+        # a dense cluster and a few blobs 10^3 .. 10^4 cells away on both axes
+        n = 20000
+        x, y = rng.uniform(-0.1, 0.1, (2, n))
+        far = np.array([[40.0, 40.0], [-25.0, 33.0], [31.5, -47.25], [40.0, 40.003],
+                        [-0.05, 60.0]])
+        x = np.concatenate([x, far[:, 0]])
+        y = np.concatenate([y, far[:, 1]])
+    else:
+        # a long thin diagonal wake: 50,000 blobs over 10^4 x 10^4 cells
+        n = 50000
+        t = np.sort(rng.uniform(0.0, 50.0, n))
+        x = t + rng.normal(0.0, 0.01, n)
+        y = 0.5 * t + rng.normal(0.0, 0.01, n)
+    w = rng.standard_normal(x.size)
+    return x, y, w
+
+
+@pytest.mark.parametrize("kind", ["outliers", "wake"])
+def test_regrid_sparse_range_matches_oracle(kind, O):
+    x, y, w = _spread_case(kind)
+    sigma, overlap = 0.004, 1.25
+    h = sigma * overlap
+    cells = ((np.ptp(x) / h + 4) * (np.ptp(y) / h + 4))
+    assert cells > 64 * x.size + 2 ** 20  # beyond the dense layout
+    got, ref = _cmp(x, y, w, sigma, overlap, -1.5 * h, -1.5 * h, 0.0, O)
+    assert got[0].size == ref[0].size > 0
+    _cmp(x, y, w, sigma, overlap, -1.5 * h, -1.5 * h, 0.3, O)  # Wee c
+
+
+def test_diffusion_tutty_sparse_range_matches_oracle(O):
+    from phyflow_amd.blobs.base.evolution import diffusion_tutty
+    x, y, w = _spread_case("outliers")
+    sigma, overlap, dt, nu = 0.01, 0.8, 0.05, 1e-4
+    h = sigma * overlap
+    got = diffusion_tutty(dt, nu, h, x, y, w, sigma, [-1.5 * h, 0], [-1.5 * h, 0], overlap,
+                          return_indices=True)
+    ref = O.regrid_tutty(x, y, w, sigma, overlap, -1.5 * h, -1.5 * h, np.sqrt(dt * nu))
+    for a, b in zip(got[3:], ref[3:]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.max(np.abs(got[2] - ref[2])) <= 1e-15 * np.max(np.abs(ref[2]))
+
+
+def test_regrid_dense_then_sparse_then_dense(O):
+    """Consecutive calls alternate the two layouts on one lane; each result
+    matches the oracle."""
+    rng = np.random.default_rng(23)
+    xd, yd = rng.uniform(0.0, 1.0, (2, 30000))
+    wd = rng.standard_normal(30000)
+    xs, ys, ws = _spread_case("outliers")
+    for x, y, w in ((xd, yd, wd), (xs, ys, ws), (xd[:1000], yd[:1000], wd[:1000])):
+        _cmp(x, y, w, 0.004, 1.25, -0.0075, -0.0075, 0.0, O)
