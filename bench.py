@@ -526,8 +526,27 @@ def bench_serve(args):
     for b in systems:
         b.close()
     lat = np.array([r["latency_ms"] for r in res])
+    queue = np.array([r["queue_ms"] for r in res])
+    run = lat - queue
+    nsys = np.array([r["info"]["n_before"] for r in res])
     cls = np.array(classes)
-    evals = float(np.sum([2 * r["info"]["n_before"] for r in res]))
+    evals = float(np.sum(2 * nsys))
+    # what the slowest 1% are: waiting for a slot, or long to run (class, size)
+    top = lat >= np.percentile(lat, 99)
+    big = nsys >= np.percentile(sizes, 75)
+    tail = {"p99_ms": float(np.percentile(lat, 99)),
+            "top1pct_count": int(top.sum()),
+            "top1pct_fp64_share": float(cls[top].mean()),
+            "top1pct_mean_queue_ms": float(queue[top].mean()),
+            "top1pct_mean_run_ms": float(run[top].mean()),
+            "top1pct_mean_n": float(nsys[top].mean()),
+            "run_ms_p50": float(np.percentile(run, 50)), "run_ms_p99": float(np.percentile(run, 99)),
+            "queue_ms_p50": float(np.percentile(queue, 50)),
+            "queue_ms_p99": float(np.percentile(queue, 99)),
+            "run_ms_p50_fp64_largest_quartile": float(np.percentile(run[cls & big], 50))
+            if (cls & big).any() else None,
+            "run_ms_p50_mixed_largest_quartile": float(np.percentile(run[~cls & big], 50))
+            if (~cls & big).any() else None}
     return {"metric": "served RK2 step requests: throughput and p50/p99 step latency",
             "value": len(res) / st["span_s"], "unit": "steps/s", "n_gpus": 1, "steps": len(res),
             "warmup": 2 * len(ids), "ms_per_step": float(np.mean(lat)),
@@ -543,6 +562,7 @@ def bench_serve(args):
                        "p99_ms_mixed": float(np.percentile(lat[~cls], 99)) if (~cls).any() else None,
                        "p50_ms_fp64": float(np.percentile(lat[cls], 50)) if cls.any() else None,
                        "p99_ms_fp64": float(np.percentile(lat[cls], 99)) if cls.any() else None,
+                       "tail": tail,
                        "completed": st["completed"], "failed": st["failed"],
                        "hbm_used_gb": st["hbm_used"] / 1e9}}
 
