@@ -39,6 +39,8 @@
 
 namespace phf {
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
 constexpr int kRMax = 6;                // largest near radius (boxes)
 constexpr int kRU = 2 * kRMax + 1;      // radius of interaction-list offsets
 constexpr int kMaxLevel = 11;
@@ -64,9 +66,15 @@ __device__ __forceinline__ bool near_pred(int dx, int dy, int ws, double s, doub
 
 __device__ __forceinline__ int near_radius(int ws, double s, double c2)
 {
-    // smallest R such that every near offset has max(|dx|,|dy|) <= R
-    const double r = sqrt(c2) / s;
-    int R = (int)ceil(r) + 1;
+    // smallest R such that every near offset has max(|dx|,|dy|) <= R: an
+    // offset with max(|dx|,|dy|) = a > ws is near only through a gap
+    // (a - 1) s below the cutoff, so R = ceil(cutoff / s) suffices once the
+    // predicate's own float64 test rejects the gap R s (monotone rounding
+    // then rejects every longer gap)
+#pragma clang fp contract(off)
+    int R = (int)ceil(sqrt(c2) / s);
+    R = R > 1 ? R : 1;
+    while (((double)R * s) * ((double)R * s) < c2) ++R;
     return R > ws ? R : ws;
 }
 
@@ -413,6 +421,180 @@ __global__ __launch_bounds__(64) void batch_m2l(
     }
 }
 
+// M2L with the binomial products on the matrix cores (float32, p + 1 <= 17).
+// One wave per target box.  The candidate offsets of the window are tested
+// 64 at a time, one per lane, and the interaction-list links compacted in
+// LDS; for each batch of up to 32 links the scaled multipoles u^k A_k form
+// the columns (real and imaginary parts apart) of a 17 x 64 matrix, and
+// C = Pascal x columns runs as 16 x 16 x 4 float32 MFMAs for rows and
+// terms 0..15, with term and row 16 on the vector ALU.  Each lane then
+// multiplies its entries of C by (-1)^m u^{m+1} and the 16 lanes of a row
+// sum their link columns.  Same links and operator as batch_m2l; the sum
+// over links runs in another order.
+constexpr int BM_LINKS = 32;
+
+constexpr int BM_ROWS = 17;  // p + 1 <= 17
+
+__global__ __launch_bounds__(64) void batch_m2l_mfma(
+    const BatchSys *__restrict__ sys, const int64_t *__restrict__ lvl, int S, int l,
+    const BatchDom *__restrict__ dom, const int32_t *__restrict__ start, int p1, int ws,
+    const float2 *__restrict__ M, float2 *__restrict__ B, const float2 *__restrict__ U,
+    const float *__restrict__ binom, int32_t *__restrict__ flag)
+{
+    __shared__ float Bs[BM_ROWS][2 * BM_LINKS + 4];
+    __shared__ float2 Us[BM_ROWS][BM_LINKS];  // u^{m+1} of each link, m = 0..p
+    __shared__ int32_t lsrc[64];
+    __shared__ int32_t loff[64];
+    const int lane = threadIdx.x;
+    const int64_t *row = lvl + (int64_t)l * (S + 1);
+    const int64_t nb = row[S] - row[0];
+    const int64_t slot = blockIdx.x;
+    if (slot >= nb) return;
+    const int64_t gb = row[0] + slot;
+    const int s = find_seg(row, S, gb);
+    const int L = sys[s].L;
+    if (l > L) return;
+    const uint32_t b = (uint32_t)(gb - row[s]);
+    const int bx = (int)compact_bits(b), by = (int)compact_bits(b >> 1);
+    const int par = (bx & 1) | ((by & 1) << 1);
+    const int n = 1 << l;
+    const double D = dom[s].D;
+    const double sl = D / ldexp(1.0, l), slp = 2.0 * sl;
+    const double c2 = sys[s].c2;
+    int Rp = near_radius(ws, slp, c2);
+    if (Rp > kRMax) {
+        if (lane == 0) atomicOr(flag, 1);
+        Rp = kRMax;
+    }
+    const int Rw = 2 * Rp + 1, side = 2 * Rw + 1, ncand = side * side;
+    const int W2 = 2 * kRU + 1;
+    const bool active = box_count(start + sys[s].leafbase, b, 2 * (L - l)) > 0;
+    // operator rows in registers: A operand of the 16 x 16 x 4 products
+    // (lane holds P[lane % 16][4 ks + lane / 16]), term 16 of rows 4 (lane / 16) + v
+    const int mr = lane & 15, kq = lane >> 4;
+    float pa[4], p16[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+        const int k = 4 * ks + kq;
+        pa[ks] = (mr < p1 && k < p1) ? binom[mr * p1 + k] : 0.0f;
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int m = 4 * kq + v;
+        p16[v] = (p1 == 17 && m < p1) ? binom[m * p1 + 16] : 0.0f;
+    }
+    // terms p + 1 .. 16 of the staged matrix stay zero
+    for (int i = lane; i < (BM_ROWS - p1) * (2 * BM_LINKS + 4); i += 64)
+        Bs[p1 + i / (2 * BM_LINKS + 4)][i % (2 * BM_LINKS + 4)] = 0.0f;
+    float pr[4] = {0, 0, 0, 0}, pi[4] = {0, 0, 0, 0};
+    float qr = 0.0f, qi = 0.0f;  // row 16
+    for (int c0 = 0; active && c0 < ncand; c0 += 64) {
+        // links of this chunk of candidate offsets, in offset order
+        const int t = c0 + lane;
+        bool ok = t < ncand;
+        int tx = 0, ty = 0;
+        uint32_t sb = 0;
+        if (ok) {
+            tx = t % side - Rw;
+            ty = t / side - Rw;
+            const int sx = bx + tx, sy = by + ty;
+            ok = sx >= 0 && sy >= 0 && sx < n && sy < n;
+            if (ok) {
+                const int px = floordiv2(tx + (par & 1)), py = floordiv2(ty + (par >> 1));
+                ok = near_pred(px, py, ws, slp, c2) && !near_pred(tx, ty, ws, sl, c2);
+            }
+            if (ok) {
+                sb = morton2((uint32_t)sx, (uint32_t)sy);
+                ok = box_count(start + sys[s].leafbase, sb, 2 * (L - l)) > 0;
+            }
+        }
+        const uint64_t mask = __ballot(ok);
+        const int K = __popcll(mask);
+        if (ok) {
+            const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+            lsrc[pos] = (int32_t)(row[s] + sb);
+            loff[pos] = (ty + kRU) * W2 + (tx + kRU);
+        }
+        __syncthreads();
+        for (int j0 = 0; j0 < K; j0 += BM_LINKS) {
+            const int Kb = K - j0 < BM_LINKS ? K - j0 : BM_LINKS;
+            // stage u^k A_k (columns 2j, 2j + 1) and u^{m+1} of each link
+            for (int i = lane; i < BM_LINKS * p1; i += 64) {
+                const int j = i / p1, k = i - j * p1;
+                float2 a = make_float2(0.0f, 0.0f), um = make_float2(0.0f, 0.0f);
+                if (j < Kb) {
+                    const float2 *Uo = U + (int64_t)loff[j0 + j] * (p1 + 1);
+                    a = cmul(M[(int64_t)lsrc[j0 + j] * p1 + k], Uo[k]);
+                    um = Uo[k + 1];
+                }
+                Bs[k][2 * j] = a.x;
+                Bs[k][2 * j + 1] = a.y;
+                Us[k][j] = um;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ct = 0; ct < 2 * BM_LINKS / 16; ++ct) {
+                const int c = 16 * ct + mr;  // this lane's column of the tile
+                floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[ks], Bs[4 * ks + kq][c], acc, 0,
+                                                               0, 0);
+                const int j = c >> 1;
+                const bool im = c & 1;
+                const float b16 = Bs[16][c];  // zero below order 16
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = 4 * kq + v;
+                    if (m < p1 && m < 16) {
+                        const float d = acc[v] + p16[v] * b16;
+                        const float2 u = Us[m][j];
+                        const float sg = (m & 1) ? -1.0f : 1.0f;
+                        // C[m][j] = d (real column) or i d (imaginary column)
+                        pr[v] += im ? -sg * d * u.y : sg * d * u.x;
+                        pi[v] += im ? sg * d * u.x : sg * d * u.y;
+                    }
+                }
+            }
+            if (p1 == 17) {  // row 16: column `lane`
+                float d = 0.0f;
+#pragma unroll
+                for (int k = 0; k < BM_ROWS; ++k) d += binom[16 * BM_ROWS + k] * Bs[k][lane];
+                const float2 u = Us[16][lane >> 1];
+                // (-1)^16 = 1
+                qr += (lane & 1) ? -d * u.y : d * u.x;
+                qi += (lane & 1) ? d * u.x : d * u.y;
+            }
+            __syncthreads();
+        }
+    }
+    // sum over the 16 link columns of each row group, and over all lanes for row 16
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            pr[v] += __shfl_xor(pr[v], o, 64);
+            pi[v] += __shfl_xor(pi[v], o, 64);
+        }
+    if (p1 == 17)
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            qr += __shfl_xor(qr, o, 64);
+            qi += __shfl_xor(qi, o, 64);
+        }
+    const float inv_r = (float)(1.0 / sl);
+    if (mr == 0)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = 4 * kq + v;
+            if (m < p1 && m < 16)
+                B[gb * p1 + m] = active ? make_float2(pr[v] * inv_r, pi[v] * inv_r)
+                                        : make_float2(0.0f, 0.0f);
+        }
+    if (p1 == 17 && lane == 0)
+        B[gb * p1 + 16] = active ? make_float2(qr * inv_r, qi * inv_r) : make_float2(0.0f, 0.0f);
+}
+
 // ------------------------------------------------------ P2P + L2P ---------
 // Near set of a leaf of system s: the offsets of the window |d| <= R that
 // pass the near predicate, enumerated from a table of the window |d| <= kRMax
@@ -617,11 +799,19 @@ int launch_bm2l(Lane *L, const BatchPlan &P, int l, int blocks, const void *M, v
 }
 
 template <class T>
-int dispatch_bm2l(Lane *L, const BatchPlan &P, int l, int blocks, const void *M, void *B,
+int dispatch_bm2l(Lane *L, const BatchPlan &P, int l, int64_t nb, const void *M, void *B,
                   const void *U, const void *bin, int32_t *flag, const BatchSys *sys,
                   const int64_t *lvl, const BatchDom *dom, const int32_t *start)
 {
     const int p1 = P.p + 1;
+    if (sizeof(T) == 4 && p1 <= BM_ROWS) {
+        batch_m2l_mfma<<<(unsigned)nb, 64, 0, L->stream>>>(
+            sys, lvl, P.S, l, dom, start, p1, P.ws, (const float2 *)M, (float2 *)B,
+            (const float2 *)U, (const float *)bin, flag);
+        PHF_HIP(hipGetLastError());
+        return PHF_OK;
+    }
+    const int blocks = (int)((nb + 64 / p1 - 1) / (64 / p1));
 #define PHF_BM2L(Q) \
     if (p1 <= Q) return launch_bm2l<T, Q>(L, P, l, blocks, M, B, U, bin, flag, sys, lvl, dom, start);
     PHF_BM2L(9)
@@ -878,9 +1068,8 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
         PHF_HIP(hipGetLastError());
         for (int l = 2; l <= P.Lmax; ++l) {
             const int64_t nb = P.lvl[(size_t)l * (S + 1) + S] - P.lvl[(size_t)l * (S + 1)];
-            const int G = 64 / p1;
             if (nb > 0)
-                PHF_TRY(dispatch_bm2l<T>(L, P, l, (int)((nb + G - 1) / G), M, B, Bf.U, Bf.bin,
+                PHF_TRY(dispatch_bm2l<T>(L, P, l, nb, M, B, Bf.U, Bf.bin,
                                          Bf.flag, Bf.sys, Bf.lvl, Bf.dom, Bf.start));
         }
         for (int l = 2; l < P.Lmax; ++l) {
