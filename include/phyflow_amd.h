@@ -250,6 +250,15 @@ typedef struct {
  * pivot U[k][k] (0 if none). */
 int phf_lu_factor_f32(const float *A, int64_t n, float *LU, int32_t *ipiv, int32_t *info);
 
+/* Float32 Cholesky A = L Lᵀ of the symmetric positive definite
+ * column-major n x n A (n <= 1024), the factorisation phf_dense_solve uses
+ * for the normal matrix BᵀB of a least-squares system (in place of the
+ * reference's numpy.linalg.lstsq, __panelsClass.py:822), returned in the
+ * getrf form of phf_lu_factor_f32 with identity pivots: LU holds the unit
+ * lower L D^-1 and the upper D Lᵀ, D = diag(L).  info = k + 1 for the first
+ * pivot <= 0 (not positive definite in float32; 0 if none). */
+int phf_chol_factor_f32(const float *A, int64_t n, float *LU, int32_t *info);
+
 int phf_dense_solve(const double *B, int64_t rows, int64_t n, const double *c, double *x,
                     phf_solve_info *info);
 

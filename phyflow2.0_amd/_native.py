@@ -159,6 +159,7 @@ _SIGS = {
     "phf_panel_influence": ([_ptr] * 8 + [_i64, _ptr], ctypes.c_int),
     "phf_panel_induced_velocity": ([_ptr, _i64, _ptr, _ptr, _i64] + [_ptr] * 8, ctypes.c_int),
     "phf_lu_factor_f32": ([_ptr, _i64, _ptr, _ptr, ctypes.POINTER(_i32)], ctypes.c_int),
+    "phf_chol_factor_f32": ([_ptr, _i64, _ptr, ctypes.POINTER(_i32)], ctypes.c_int),
     "phf_dense_solve": ([_ptr, _i64, _i64, _ptr, _ptr, ctypes.POINTER(SolveInfo)], ctypes.c_int),
     "phf_panels_create": ([_ptr, _ptr, _ptr, _ptr, _i64, _i32, _dbl, ctypes.POINTER(_ptr)],
                           ctypes.c_int),

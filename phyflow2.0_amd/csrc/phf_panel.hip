@@ -368,9 +368,10 @@ __global__ __launch_bounds__(64) void lu_update_mfma(float *__restrict__ A, int 
 //                           the matrix cores, and for j == k + 1 the
 //                           factorisation of panel k + 1.
 // A panel is factored with its rows in registers (row k1 + t in thread t):
-// per column a wave-shuffle arg-max, one barrier for the cross-wave pivot,
-// one for the exchange of the pivot row through LDS (double-buffered by
-// column parity, so the next column needs no third barrier).  Each row
+// per column a DPP maximum in each wave, its row by ballot, and one LDS
+// barrier after which a single 64-bit LDS maximum names the pivot row and
+// the wave whose slot holds its entries (double-buffered by column
+// parity, so the next column needs no second barrier).  Each row
 // carries the index of the row it came from; the rows that moved give the
 // panel's composite permutation (at most 2 NB rows), which the other blocks
 // apply with one load and one store per element.
@@ -391,12 +392,12 @@ __device__ __forceinline__ void lds_barrier()
 
 template <int T>
 struct LuShared {
-    // per column parity: each wave's largest |entry|, its row and that
-    // row's entries of columns c .. c + NB - 1
-    float rv[2][T / 64];
-    int ri[2][T / 64];
+    // per column parity: each wave's candidate row's entries of columns
+    // c .. c + NB - 1
     float4 cand[2][T / 64][NB / 4];
     int piv[NB];
+    // per column c mod 3: the largest (|entry|, -row, wave) over the waves
+    unsigned long long key[3];
     int first_zero, cnt;
     float Lt[NB][NB + 1];  // L11, [column][row]
     float Ut[NB][NB + 1];  // U12, [column][row]
@@ -450,7 +451,9 @@ __device__ __forceinline__ void lu_factor_panel(float *__restrict__ A, int n, in
     if (tid == 0) {
         sh.first_zero = -1;
         sh.cnt = 0;
+        sh.key[0] = sh.key[1] = sh.key[2] = 0ull;
     }
+    __syncthreads();
     // columns c >= w (last panel) run on zero entries and change nothing;
     // the loop stays rolled (its body stays in the instruction cache)
 #pragma unroll 1
@@ -461,31 +464,57 @@ __device__ __forceinline__ void lu_factor_panel(float *__restrict__ A, int n, in
         const bool cand = live && r >= rc;
         const float mine = cand ? fabsf(a[0]) : -1.0f;
         const float v = wave_max(mine);
-        // the smallest occupied row among the lanes holding the maximum
-        int best_r = cand && mine == v ? r : 0x7fffffff;
+        // the smallest occupied row among the lanes holding the maximum: a
+        // ballot names it when one lane holds the maximum (ties need the
+        // cross-lane minimum)
+        const uint64_t hold = __ballot(cand && mine == v);
+        int best_r = 0x7fffffff;
+        if (__popcll(hold) == 1) {
+            best_r = __builtin_amdgcn_readlane(r, (int)__builtin_ctzll(hold));
+        } else if (hold != 0ull) {
+            best_r = cand && mine == v ? r : 0x7fffffff;
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) best_r = min(best_r, __shfl_xor(best_r, o));
-        if (r == best_r) {
-            sh.rv[buf][wv] = v;
-            sh.ri[buf][wv] = r;
-#pragma unroll
-            for (int j = 0; j < NB / 4; ++j)
-                sh.cand[buf][wv][j] = make_float4(a[4 * j], a[4 * j + 1], a[4 * j + 2], a[4 * j + 3]);
-        } else if (lane == 0 && best_r == 0x7fffffff) {
-            sh.rv[buf][wv] = -1.0f;
-            sh.ri[buf][wv] = 0x7fffffff;
+            for (int o = 32; o > 0; o >>= 1) best_r = min(best_r, __shfl_xor(best_r, o));
         }
-        lds_barrier();
-        float pv = sh.rv[buf][0];
-        int p = sh.ri[buf][0], pw = 0;
+        float pv;
+        int p, pw;
+        if (T == 64) {
+            if (r == best_r) {
 #pragma unroll
-        for (int q = 1; q < T / 64; ++q) {
-            const float vq = sh.rv[buf][q];
-            const int iq = sh.ri[buf][q];
-            if (vq > pv || (vq == pv && iq < p)) {
-                pv = vq;
-                p = iq;
-                pw = q;
+                for (int j = 0; j < NB / 4; ++j)
+                    sh.cand[buf][0][j] =
+                        make_float4(a[4 * j], a[4 * j + 1], a[4 * j + 2], a[4 * j + 3]);
+            }
+            lds_barrier();
+            pv = best_r == 0x7fffffff ? -1.0f : v;
+            p = best_r;
+            pw = 0;
+        } else {
+            // the waves' candidates meet in one 64-bit LDS maximum: |value|
+            // bits, then the row inverted (a tie goes to the smaller row),
+            // then the wave; the candidate's row entries go to its wave's slot
+            if (r == best_r) {
+                const unsigned long long key =
+                    ((unsigned long long)__float_as_uint(v) << 32) |
+                    ((unsigned long long)(0xFFFFFFu - (uint32_t)r) << 8) | (unsigned long long)wv;
+                atomicMax(&sh.key[c % 3], key);
+#pragma unroll
+                for (int j = 0; j < NB / 4; ++j)
+                    sh.cand[buf][wv][j] =
+                        make_float4(a[4 * j], a[4 * j + 1], a[4 * j + 2], a[4 * j + 3]);
+            }
+            lds_barrier();
+            const unsigned long long key = sh.key[c % 3];
+            // column c + 2's slot: last read in column c - 1, before this barrier
+            if (tid == 0) sh.key[(c + 2) % 3] = 0ull;
+            if (key == 0ull) {
+                pv = -1.0f;
+                p = 0x7fffffff;
+                pw = 0;
+            } else {
+                pv = __uint_as_float((uint32_t)(key >> 32));
+                p = (int)(0xFFFFFFu - (uint32_t)((key >> 8) & 0xFFFFFFu));
+                pw = (int)(key & 0xFFu);
             }
         }
         if (tid == 0) {
@@ -539,6 +568,72 @@ __device__ __forceinline__ void lu_factor_panel(float *__restrict__ A, int n, in
     if (tid == 0) {
         sw->cnt = sh.cnt;
         if (sh.first_zero >= 0 && info[0] == 0) info[0] = sh.first_zero + 1;
+    }
+}
+
+// Rows r0 .. n - 1 of column block c0 .. c0 + wj: A -= P Q, P the w
+// columns k0 .. k0 + w of A on those rows, Q[kk][c] = Ut[c][kk] (w x wj) in
+// LDS.  16 x 16 tiles on v_mfma_f32_16x16x4f32 (A operand lane l holds
+// A[l%16][l/16], B operand B[l/16][l%16], result D[4 (l/16) + v][l%16]).
+// A wave holds at most LU_TILES tiles (n - r0 <= T: T covers the rows),
+// and every tile's operands are loaded before the first product, so the
+// loads of all tiles are in flight together.
+// rows (nullable): the results go to LDS instead, row-major with stride
+// NB + 1 from row r0 (for a panel factor that follows in the same launch).
+template <int T>
+__device__ __forceinline__ void lu_block_update(float *__restrict__ A, int n, int r0, int c0,
+                                                int wj, int k0, int w,
+                                                const float (*__restrict__ Ut)[NB + 1],
+                                                float *__restrict__ rows = nullptr)
+{
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int m = n - r0;
+    if (m <= 0) return;
+    constexpr int LU_TILES = 8;
+    const int li = lane & 15, lk = lane >> 4;
+    const int tiles_r = (m + 15) / 16, tiles_c = (wj + 15) / 16;
+    const int ntile = tiles_r * tiles_c;
+    float av[LU_TILES][NB / 4];
+    floatx4 acc[LU_TILES];
+#pragma unroll
+    for (int q = 0; q < LU_TILES; ++q) {
+        const int t = wv + q * (T / 64);
+        const int tr = t % tiles_r, tc = t / tiles_r;
+        const int row = r0 + 16 * tr, cl = 16 * tc;  // cl: column inside the block
+        const bool on = t < ntile;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = row + 4 * lk + v, jj = cl + li;
+            acc[q][v] = (on && i < n && jj < wj) ? A[(int64_t)(c0 + jj) * n + i] : 0.0f;
+        }
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+            const int i = row + li, kc = 4 * kk + lk;
+            av[q][kk] = (on && i < n && kc < w) ? A[(int64_t)(k0 + kc) * n + i] : 0.0f;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LU_TILES; ++q) {
+        const int t = wv + q * (T / 64);
+        if (t >= ntile) break;  // uniform across the wave
+        const int tr = t % tiles_r, tc = t / tiles_r;
+        const int row = r0 + 16 * tr, cl = 16 * tc;
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+            const int kc = 4 * kk + lk;
+            const float b = (kc < w && cl + li < wj) ? Ut[cl + li][kc] : 0.0f;
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[q][kk], b, acc[q], 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = row + 4 * lk + v, jj = cl + li;
+            if (i < n && jj < wj) {
+                if (rows)
+                    rows[(i - r0) * (NB + 1) + jj] = acc[q][v];
+                else
+                    A[(int64_t)(c0 + jj) * n + i] = acc[q][v];
+            }
+        }
     }
 }
 
@@ -606,61 +701,186 @@ __global__ __launch_bounds__(T) void lu_step(float *__restrict__ A, int n, int k
         for (int r = 0; r < w; ++r) A[(int64_t)(c0 + tid) * n + k0 + r] = u[r];
     }
     __syncthreads();
-    // 3. A22 -= L21 U12, 16 x 16 tiles on v_mfma_f32_16x16x4f32 (A operand
-    // lane l holds A[l%16][l/16], B operand B[l/16][l%16], result
-    // D[4 (l/16) + v][l%16])
-    // A wave holds at most LU_TILES tiles (m <= T: T covers the next
-    // panel's rows), and every tile's operands are loaded before the first
-    // product, so the loads of all tiles are in flight together.
-    const int r0 = k0 + w, m = n - r0;
-    if (m > 0) {
-        constexpr int LU_TILES = 8;
-        const int li = lane & 15, lk = lane >> 4;
-        const int tiles_r = (m + 15) / 16, tiles_c = (wj + 15) / 16;
-        const int ntile = tiles_r * tiles_c;
-        float av[LU_TILES][NB / 4];
-        floatx4 acc[LU_TILES];
-#pragma unroll
-        for (int q = 0; q < LU_TILES; ++q) {
-            const int t = wv + q * (T / 64);
-            const int tr = t % tiles_r, tc = t / tiles_r;
-            const int row = r0 + 16 * tr, cl = 16 * tc;  // cl: column inside the block
-            const bool on = t < ntile;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = row + 4 * lk + v, jj = cl + li;
-                acc[q][v] = (on && i < n && jj < wj) ? A[(int64_t)(c0 + jj) * n + i] : 0.0f;
-            }
-#pragma unroll
-            for (int kk = 0; kk < NB / 4; ++kk) {
-                const int i = row + li, kc = 4 * kk + lk;
-                av[q][kk] = (on && i < n && kc < w) ? A[(int64_t)(k0 + kc) * n + i] : 0.0f;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < LU_TILES; ++q) {
-            const int t = wv + q * (T / 64);
-            if (t >= ntile) break;  // uniform across the wave
-            const int tr = t % tiles_r, tc = t / tiles_r;
-            const int row = r0 + 16 * tr, cl = 16 * tc;
-#pragma unroll
-            for (int kk = 0; kk < NB / 4; ++kk) {
-                const int kc = 4 * kk + lk;
-                const float b = (kc < w && cl + li < wj) ? sh.Ut[cl + li][kc] : 0.0f;
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[q][kk], b, acc[q], 0, 0, 0);
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = row + 4 * lk + v, jj = cl + li;
-                if (i < n && jj < wj) A[(int64_t)(c0 + jj) * n + i] = acc[q][v];
-            }
-        }
-    }
+    // 3. A22 -= L21 U12 on the matrix cores
+    lu_block_update<T>(A, n, k0 + w, c0, wj, k0, w, sh.Ut);
     // 4. lookahead: the next panel is this block, now up to date
     if (j == k + 1) {
         __syncthreads();
         lu_factor_panel<T>(A, n, c0, ipiv, info, sw + j, sh);
     }
+}
+
+// ------------------------------------- blocked Cholesky, n <= 1024 --------
+// For the symmetric positive definite normal matrix BᵀB of the
+// least-squares ('prescribed') system: A = L Lᵀ, right-looking in panels of
+// NB columns with lookahead, one launch per panel (as the LU, without the
+// per-column pivot search):
+//   chol_first     factors panel 0;
+//   chol_step(k)   one workgroup per column block j > k: rows >= j NB of
+//                  the block -= L(:, panel k) L(block j rows, panel k)ᵀ on
+//                  the matrix cores; block k + 1 keeps its updated rows in
+//                  LDS and factors them as the next panel.
+// A panel factor: wave 0 eliminates the NB x NB diagonal block and the 64 -
+// NB rows below it in its lanes (row c's entries come by readlane, so no
+// barrier per column); the other waves' rows solve x L11ᵀ = a by forward
+// substitution against L11 in LDS, one row per thread.  The diagonal of L
+// goes to dg.  A pivot d <= 0 (not positive definite in float32) sets
+// info to its column + 1 and continues with 1.  Two elementwise kernels
+// then rewrite L in the getrf form of the LU (chol_to_lu_*).
+struct alignas(16) CholShared {
+    float Ld[NB][NB + 4];  // L11 [row][column] (16-byte rows)
+    float inv[NB];         // 1 / L11[c][c]
+    float Ut[NB][NB + 1];  // [c][kk] = L(j0 + c, k0 + kk)
+};
+
+// rows: the panel's rows in LDS (row-major, stride NB + 1), or nullptr to
+// read them from A.
+template <int T>
+__device__ __forceinline__ void chol_factor_panel(float *__restrict__ A, int n, int c0,
+                                                  int32_t *__restrict__ info,
+                                                  float *__restrict__ dg, CholShared &sh,
+                                                  const float *__restrict__ rows)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int w = min(NB, n - c0);
+    const int row = c0 + tid;
+    const bool live = row < n;
+    float a[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        a[j] = (live && j < w) ? (rows ? rows[tid * (NB + 1) + j] : A[(int64_t)(c0 + j) * n + row])
+                               : 0.0f;
+    if (wv == 0) {
+        // branch-free: lanes at or above row c run the update with a zero
+        // multiplier (a divergent branch would copy the row array at its
+        // merge); 1 / lcc by the hardware reciprocal square root (the
+        // factor is refined in float64 afterwards)
+        int bad = -1;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            if (c < w) {  // uniform
+                const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[c]), c));
+                const bool pos = d > 0.0f;
+                if (!pos && bad < 0) bad = c;
+                const float inv = pos ? __builtin_amdgcn_rsqf(d) : 1.0f;
+                const float lcc = pos ? d * inv : 1.0f;
+                // L(r, c) = a(r, c) / lcc; a(r, j) -= L(r, c) L(j, c) with
+                // L(j, c) = a(c, j) / lcc by symmetry
+                const bool below = lane > c;
+                const float l = below ? a[c] * inv : 0.0f, li = l * inv;
+                float rowc[NB];
+#pragma unroll
+                for (int j = c + 1; j < NB; ++j)
+                    rowc[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[j]), c));
+#pragma unroll
+                for (int j = c + 1; j < NB; ++j) a[j] = __builtin_fmaf(-li, rowc[j], a[j]);
+                a[c] = below ? l : (lane == c ? lcc : a[c]);
+                if (lane == 0) sh.inv[c] = inv;
+            }
+        }
+        if (lane < NB) {
+#pragma unroll
+            for (int q = 0; q < NB / 4; ++q)
+                *reinterpret_cast<float4 *>(&sh.Ld[lane][4 * q]) =
+                    make_float4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
+        }
+        if (lane == 0 && bad >= 0 && info[0] == 0) info[0] = c0 + bad + 1;
+    }
+    __syncthreads();
+    if (wv > 0 && live) {
+        // x L11ᵀ = a by forward substitution, L11 rows read 16 bytes at a time
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            float s = a[c];
+#pragma unroll
+            for (int q = 0; 4 * q < c; ++q) {
+                const float4 l4 = *reinterpret_cast<const float4 *>(&sh.Ld[c][4 * q]);
+                s = __builtin_fmaf(-a[4 * q], l4.x, s);
+                if (4 * q + 1 < c) s = __builtin_fmaf(-a[4 * q + 1], l4.y, s);
+                if (4 * q + 2 < c) s = __builtin_fmaf(-a[4 * q + 2], l4.z, s);
+                if (4 * q + 3 < c) s = __builtin_fmaf(-a[4 * q + 3], l4.w, s);
+            }
+            a[c] = s * sh.inv[c];
+        }
+    }
+    // the lower part of the panel (row >= column)
+    if (live) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j < w && tid >= j) A[(int64_t)(c0 + j) * n + row] = a[j];
+    }
+    if (tid < w) dg[c0 + tid] = sh.Ld[tid][tid];
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void chol_first(float *__restrict__ A, int n,
+                                                int32_t *__restrict__ info, float *__restrict__ dg)
+{
+    __shared__ CholShared sh;
+    if (threadIdx.x == 0) info[0] = 0;
+    __syncthreads();
+    chol_factor_panel<T>(A, n, 0, info, dg, sh, nullptr);
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void chol_step(float *__restrict__ A, int n, int k,
+                                               int32_t *__restrict__ info, float *__restrict__ dg)
+{
+    constexpr bool kRowsInLds = T <= 512;
+    __shared__ CholShared sh;
+    __shared__ float rows[kRowsInLds ? T * (NB + 1) : 1];
+    const int j = k + 1 + blockIdx.x;
+    const int tid = threadIdx.x;
+    const int k0 = k * NB, w = min(NB, n - k0);
+    const int c0 = j * NB, wj = min(NB, n - c0);
+    // Q = L(block j rows, panel k)ᵀ, read along the rows of each column
+    for (int e = tid; e < NB * NB; e += T) {
+        const int kk = e / NB, c = e % NB;
+        sh.Ut[c][kk] = (c < wj && kk < w) ? A[(int64_t)(k0 + kk) * n + c0 + c] : 0.0f;
+    }
+    __syncthreads();
+    const bool next = j == k + 1;
+    float *dst = next && kRowsInLds ? rows : nullptr;
+    lu_block_update<T>(A, n, c0, c0, wj, k0, w, sh.Ut, dst);
+    if (next) {
+        __syncthreads();
+        chol_factor_panel<T>(A, n, c0, info, dg, sh, dst);
+    }
+}
+
+// The factor in LAPACK getrf form, for the inverse and the refinement:
+// L Lᵀ = (L D⁻¹)(D Lᵀ), D = diag(L), pivots the identity.  First the upper
+// triangle U(i, j) = D(i) L(j, i), i < j (reads the lower triangle only) ...
+__global__ void chol_to_lu_upper(float *__restrict__ A, int n, const float *__restrict__ dg)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int j = (int)(e / n), i = (int)(e % n);  // column j, row i
+    if (i < j) A[e] = dg[i] * A[(int64_t)i * n + j];
+}
+
+// ... then the unit lower L(i, j) / D(j), the diagonal D(j)², the pivots
+__global__ void chol_to_lu_lower(float *__restrict__ A, int n, const float *__restrict__ dg,
+                                 int32_t *__restrict__ ipiv)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)n * n) return;
+    const int j = (int)(e / n), i = (int)(e % n);
+    if (i > j)
+        A[e] = A[e] / dg[j];
+    else if (i == j) {
+        A[e] = dg[j] * dg[j];
+        ipiv[j] = j;
+    }
+}
+
+template <int T>
+static void chol_launch(Lane *L, float *A, int n, int k, int nsteps, int32_t *info, float *dg)
+{
+    if (k < 0)
+        chol_first<T><<<1, T, 0, L->stream>>>(A, n, info, dg);
+    else if (nsteps - k - 1 > 0)
+        chol_step<T><<<nsteps - k - 1, T, 0, L->stream>>>(A, n, k, info, dg);
 }
 
 // ------------------------------------------------- approximate inverse ----
@@ -1102,6 +1322,48 @@ int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, void
     return PHF_OK;
 }
 
+int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, float *dg)
+{
+    if (n <= 0) return PHF_OK;
+    PHF_REQUIRE(n <= LU_TMAX, "Cholesky factorisation of n = %d (at most %d)", n, LU_TMAX);
+    const int nsteps = (n + NB - 1) / NB;
+    for (int k = -1; k + 1 < nsteps; ++k) {
+        // launch k + 1 factors panel k + 1 (rows from (k + 1) NB on)
+        switch (lu_threads(std::max(1, n - (k + 1) * NB))) {
+        case 64: chol_launch<64>(L, A32, n, k, nsteps, info, dg); break;
+        case 128: chol_launch<128>(L, A32, n, k, nsteps, info, dg); break;
+        case 256: chol_launch<256>(L, A32, n, k, nsteps, info, dg); break;
+        case 512: chol_launch<512>(L, A32, n, k, nsteps, info, dg); break;
+        default: chol_launch<1024>(L, A32, n, k, nsteps, info, dg); break;
+        }
+    }
+    const int64_t nn = (int64_t)n * n;
+    chol_to_lu_upper<<<div_up(nn, 256), 256, 0, L->stream>>>(A32, n, dg);
+    chol_to_lu_lower<<<div_up(nn, 256), 256, 0, L->stream>>>(A32, n, dg, ipiv);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
+__global__ void to_f32(const double *__restrict__ a, int64_t n, float *__restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+static int dense_inverse(Lane *L, DenseFactors *f)
+{
+    const int64_t n = f->n;
+    const size_t inv_lds = sizeof(float) * INV_C * 16 * div_up(n, 16);
+    if (n <= INV_NMAX && inv_lds + 8192 <= Context::get().props().sharedMemPerBlock)
+        lu_inverse_mfma<<<div_up(n, INV_C), 64 * INV_W, inv_lds, L->stream>>>(
+            f->LU, (int)n, f->ipiv, f->Xinv);
+    else
+        lu_inverse_kernel<<<(unsigned)n, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(
+            f->LU, (int)n, f->ipiv, f->Xinv);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
 int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
                         hipEvent_t *marks)
 {
@@ -1114,6 +1376,7 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
         PHF_ALLOC(f->ipiv, L->arena, int32_t, (size_t)n);
         PHF_ALLOC(f->info, L->arena, int32_t, 1);
         PHF_ALLOC(f->Xinv, L->arena, float, (size_t)(n * n));
+        PHF_ALLOC(f->dg, L->arena, float, (size_t)n);
         if (normal) PHF_ALLOC(f->N64, L->arena, double, (size_t)(n * n));
         if (n <= LU_TMAX) {
             char *w;
@@ -1128,21 +1391,31 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
         normal_matrix<<<grid, 256, 0, L->stream>>>(f->Bc, rows, n, f->N64, f->LU);
     }
     if (marks) PHF_HIP(hipEventRecord(marks[0], L->stream));
-    PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->lu_work));
-    if (marks) PHF_HIP(hipEventRecord(marks[1], L->stream));
-    const size_t inv_lds = sizeof(float) * INV_C * 16 * div_up(n, 16);
-    if (n <= INV_NMAX && inv_lds + 8192 <= Context::get().props().sharedMemPerBlock)
-        lu_inverse_mfma<<<div_up(n, INV_C), 64 * INV_W, inv_lds, L->stream>>>(
-            f->LU, (int)n, f->ipiv, f->Xinv);
+    // the normal matrix is symmetric positive definite: Cholesky (no pivot
+    // search); B itself: LU with partial pivoting
+    const bool chol = normal && n <= LU_TMAX;
+    if (chol)
+        PHF_TRY(chol_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->dg));
     else
-        lu_inverse_kernel<<<(unsigned)n, SOLVE_T, sizeof(float) * (size_t)n, L->stream>>>(
-            f->LU, (int)n, f->ipiv, f->Xinv);
-    PHF_HIP(hipGetLastError());
+        PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->lu_work));
+    if (marks) PHF_HIP(hipEventRecord(marks[1], L->stream));
+    PHF_TRY(dense_inverse(L, f));
     if (marks) PHF_HIP(hipEventRecord(marks[2], L->stream));
     f->Br = B;
+    f->cholesky = chol;
     int32_t hinfo = 0;
     PHF_HIP(hipMemcpyAsync(&hinfo, f->info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
+    if (chol && hinfo != 0) {
+        // not positive definite in float32: the pivoted LU of the same matrix
+        to_f32<<<div_up(n * n, 256), 256, 0, L->stream>>>(f->N64, n * n, f->LU);
+        PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->lu_work));
+        PHF_TRY(dense_inverse(L, f));
+        f->cholesky = false;
+        PHF_HIP(hipMemcpyAsync(&hinfo, f->info, sizeof(int32_t), hipMemcpyDeviceToHost,
+                               L->stream));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+    }
     f->singular_at = hinfo;
     if (hinfo != 0) {
         set_error("matrix is singular in float32 (zero pivot at column %d)", hinfo - 1);
@@ -1747,6 +2020,33 @@ int phf_lu_factor_f32(const float *A, int64_t n, float *LU, int32_t *ipiv, int32
     PHF_TRY(lu_factor_f32(L, dA, (int)n, dp, di));
     PHF_HIP(hipMemcpyAsync(LU, dA, sizeof(float) * n * n, hipMemcpyDeviceToHost, L->stream));
     PHF_HIP(hipMemcpyAsync(ipiv, dp, sizeof(int32_t) * n, hipMemcpyDeviceToHost, L->stream));
+    int32_t hinfo = 0;
+    PHF_HIP(hipMemcpyAsync(&hinfo, di, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    if (info) *info = hinfo;
+    return PHF_OK;
+}
+
+int phf_chol_factor_f32(const float *A, int64_t n, float *LU, int32_t *info)
+{
+    clear_error();
+    PHF_REQUIRE(n >= 1 && n <= 1024, "n out of range (%lld)", (long long)n);
+    PHF_REQUIRE(A && LU, "NULL argument");
+    for (int64_t i = 0; i < n * n; ++i)
+        PHF_REQUIRE(std::isfinite(A[i]), "A[%lld] is not finite", (long long)i);
+    LaneGuard g;
+    PHF_TRY(g.rc);
+    Lane *L = g.lane;
+    PHF_TRY(reset_arena(L));
+    float *dA, *dg;
+    int32_t *dp, *di;
+    PHF_ALLOC(dA, L->arena, float, (size_t)(n * n));
+    PHF_ALLOC(dg, L->arena, float, (size_t)n);
+    PHF_ALLOC(dp, L->arena, int32_t, (size_t)n);
+    PHF_ALLOC(di, L->arena, int32_t, 1);
+    PHF_HIP(hipMemcpyAsync(dA, A, sizeof(float) * n * n, hipMemcpyHostToDevice, L->stream));
+    PHF_TRY(chol_factor_f32(L, dA, (int)n, dp, di, dg));
+    PHF_HIP(hipMemcpyAsync(LU, dA, sizeof(float) * n * n, hipMemcpyDeviceToHost, L->stream));
     int32_t hinfo = 0;
     PHF_HIP(hipMemcpyAsync(&hinfo, di, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));

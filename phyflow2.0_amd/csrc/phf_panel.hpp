@@ -17,8 +17,14 @@ struct SolveStats {
 int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info,
                   void *work = nullptr);
 size_t lu_work_bytes(int n);
+// In-place float32 Cholesky of the symmetric positive definite column-major
+// n x n A32 (n <= 1024), returned in the LU form above (unit lower L D^-1,
+// upper D L^T, identity pivots); dg: n floats of device memory (the
+// diagonal of L).  info = column + 1 of the first pivot <= 0.
+int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, float *dg);
 
-// Float32 LU factors of B (rows == n) or of B^T B (rows > n) and the
+// Float32 LU factors of B (rows == n), or of B^T B (rows > n: Cholesky in
+// LU form, the pivoted LU if B^T B is not positive definite in float32), and the
 // float32 inverse formed from them, with the float64 column-major copy of B
 // for refinement residuals (arena-owned).
 struct DenseFactors {
@@ -29,6 +35,8 @@ struct DenseFactors {
     float *Xinv = nullptr;       // float32 (P L U)^-1, row-major
     double *N64 = nullptr;       // B^T B (rows > n)
     void *lu_work = nullptr;
+    float *dg = nullptr;         // diagonal of the Cholesky factor
+    bool cholesky = false;       // B^T B factored by Cholesky (else pivoted LU)
     int32_t *ipiv = nullptr, *info = nullptr;
     int singular_at = 0;
 };

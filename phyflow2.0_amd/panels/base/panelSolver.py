@@ -77,6 +77,24 @@ def lu_factor_f32(A):
     return LU, piv, info.value
 
 
+def chol_factor_f32(A):
+    """(LU, info): float32 Cholesky A = L L^T of the symmetric positive
+    definite A on the GPU, in the getrf form of lu_factor_f32 with identity
+    pivots (LU holds the unit lower L D^-1 and the upper D L^T, D = diag L);
+    the factorisation of the normal matrix of the 'prescribed' least-squares
+    system (the reference solves that one by lstsq, __panelsClass.py:822).
+    info = k + 1 for the first pivot <= 0."""
+    A = np.asarray(A, dtype=np.float32)
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError("A must be square")
+    n = A.shape[0]
+    Af = np.asfortranarray(A)
+    LU = np.empty((n, n), dtype=np.float32, order="F")
+    info = ctypes.c_int32()
+    call("phf_chol_factor_f32", Af.ctypes.data, n, LU.ctypes.data, ctypes.byref(info))
+    return LU, info.value
+
+
 def dense_solve(B, c, return_info=False):
     """x solving B x = c (square) or min ||B x - c|| (tall), float32 LU +
     float64 iterative refinement on the GPU."""

@@ -168,3 +168,49 @@ def test_lu_factor_f32_zero_pivot():
     A = np.ones((40, 40), dtype=np.float32)
     LU, piv, info = PS.lu_factor_f32(A)
     assert info == 2  # the second column has no nonzero pivot left
+
+
+@pytest.mark.parametrize("n", [1, 7, 31, 32, 33, 64, 65, 100, 257, 512, 700])
+def test_chol_factor_f32_matches_lapack(n):
+    """Float32 Cholesky of a symmetric positive definite matrix (the normal
+    matrix BᵀB of a tall system, as the 'prescribed' solve forms it) against
+    numpy's float64 Cholesky of the same float32 matrix, in the getrf form
+    with identity pivots: L D^-1 below the diagonal, D Lᵀ above."""
+    rng = np.random.default_rng(300 + n)
+    B = rng.standard_normal((n + 1 + n // 2, n))
+    A = (B.T @ B).astype(np.float32)
+    LU, info = PS.chol_factor_f32(A)
+    assert info == 0
+    Lu = np.tril(LU.astype(np.float64), -1) + np.eye(n)
+    U = np.triu(LU.astype(np.float64))
+    A64 = A.astype(np.float64)
+    # A = (L D^-1)(D Lᵀ) to float32 accuracy
+    assert np.linalg.norm(Lu @ U - A64) <= 1e-5 * np.linalg.norm(A64) * max(1.0, np.sqrt(n) / 4)
+    # the factor itself: L = (L D^-1) D with D = sqrt(diag(D Lᵀ))
+    d = np.sqrt(np.diag(U))
+    Lr = np.linalg.cholesky(A64)
+    kappa = np.linalg.cond(A64)
+    assert np.linalg.norm(Lu * d[None, :] - Lr) <= 1e-6 * kappa ** 0.5 * max(1.0, n / 64) * \
+        np.linalg.norm(Lr)
+    # the upper triangle is D Lᵀ of the same factor
+    Lf = Lu * d[None, :]
+    assert np.linalg.norm(U - np.diag(d) @ Lf.T) <= 1e-6 * np.linalg.norm(U)
+
+
+def test_chol_factor_f32_not_positive_definite():
+    A = np.eye(50, dtype=np.float32)
+    A[20, 20] = -1.0
+    LU, info = PS.chol_factor_f32(A)
+    assert info == 21
+
+
+def test_prescribed_solve_uses_cholesky_and_matches_lstsq(O):
+    """The least-squares panel solve (normal matrix factored by Cholesky,
+    float64 refinement) against the float64 lstsq solution."""
+    rng = np.random.default_rng(9)
+    m = 300
+    B = rng.standard_normal((m + 1, m))
+    c = rng.standard_normal(m + 1)
+    x = PS.dense_solve(B, c)
+    ref = np.linalg.lstsq(B, c, rcond=None)[0]
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
