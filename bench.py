@@ -541,18 +541,43 @@ def bench_serve(args):
     [1e4, 2e6], seed 0, uniform blobs, h = 1/sqrt(N), sigma = h/0.8).  Each
     request is one RK2 step (two velocity evaluations), an M4' remesh onto
     the lattice of spacing h and population control (thresholds 1e-8, the
-    Blobs defaults); 80% of requests ask for 1e-6 (F32C, order 16), 20% for
-    1e-12 (float64, order 30, ws 2).  Latency = submission -> completion."""
+    Blobs defaults); 80% of requests ask for 1e-6 (F32C), 20% for 1e-12
+    (float64), each with the FMM settings of the measured table for its
+    precision, tolerance and size (--fixed-params: order 16 / order 30 with
+    ws 2).  Each class's error against the float64 direct sum is checked on
+    the smallest and the largest simulation first.  Latency = submission ->
+    completion."""
     from phyflow_amd.serve import Server
     rng = np.random.default_rng(0)
     sizes = np.floor(np.exp(rng.uniform(math.log(1e4), math.log(2e6), args.sims))).astype(int)
-    systems, params, remesh = [], [], []
+    systems, params, remesh, data = [], [], [], []
     for i, n in enumerate(sizes):
         s = W.uniform_system(int(n), seed=2000 + i)
+        data.append(s)
         systems.append(BlobSystem(s["x"], s["y"], s["w"]))
-        params.append((vel_params(s["sigma"], prec=gauss.PREC_F32C, order=16, leaf=64),
-                       vel_params(s["sigma"], prec=gauss.PREC_F64, order=30, leaf=64, ws=2)))
+        if args.fixed_params:
+            params.append((vel_params(s["sigma"], prec=gauss.PREC_F32C, order=16, leaf=64),
+                           vel_params(s["sigma"], prec=gauss.PREC_F64, order=30, leaf=64, ws=2)))
+        else:
+            params.append((autotune.tuned_params(s["sigma"], int(n), prec=gauss.PREC_F32C, tol=1e-6),
+                           autotune.tuned_params(s["sigma"], int(n), prec=gauss.PREC_F64,
+                                                 tol=1e-12)))
         remesh.append(dict(sigma=s["sigma"], overlap=0.8, population_control=(1e-8, 1e-8)))
+    checks = []
+    for i in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        s = data[i]
+        sub = W.target_subset(s["x"].size, 1024, seed=i)
+        xopt, cutoff = W.oseen_params(s["sigma"])
+        qr, qi = gauss.fmm2d_gpu(s["x"], s["y"], s["w"], s["x"][sub], s["y"][sub], 35, xopt, cutoff,
+                                 0.0, prec=gauss.PREC_F64)
+        ref = np.concatenate([-qi / (2 * math.pi), -qr / (2 * math.pi)])
+        for cls, p in (("f32c", params[i][0]), ("f64", params[i][1])):
+            systems[i].velocity(p, fetch=False)
+            vx, vy = systems[i].last_velocity()
+            got = np.concatenate([vx[sub], vy[sub]])
+            checks.append({"n": int(sizes[i]), "class": cls, "order": p.order, "leaf": p.leaf,
+                           "ws": p.ws, "leaf_side": p.leaf_side,
+                           "rel_l2": float(np.linalg.norm(got - ref) / np.linalg.norm(ref))})
     dt = 1e-4
     with Server(slots=args.slots) as srv:
         ids = [srv.add(b) for b in systems]
@@ -622,6 +647,8 @@ def bench_serve(args):
                        "p50_ms_fp64": float(np.percentile(lat[cls], 50)) if cls.any() else None,
                        "p99_ms_fp64": float(np.percentile(lat[cls], 99)) if cls.any() else None,
                        "tail": tail,
+                       "params": "fixed" if args.fixed_params else "autotune table",
+                       "class_checks": checks,
                        "completed": st["completed"], "failed": st["failed"],
                        "hbm_used_gb": st["hbm_used"] / 1e9}}
 
@@ -659,6 +686,8 @@ def main():
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--rate", type=float, default=100.0)
     ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--fixed-params", action="store_true",
+                    help="serve: order 16 / order 30 ws 2 instead of the measured table")
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
