@@ -76,6 +76,7 @@ struct BatchRemeshPlan {
     int key_bits = 1;
     std::vector<BatchRm> rm;
     std::vector<int64_t> cb;  // cbase per system
+    std::vector<int32_t> bsys;  // system of each workgroup of cells
 };
 
 struct BatchRemeshBuffers {
@@ -83,6 +84,7 @@ struct BatchRemeshBuffers {
     BatchRm *rm = nullptr;
     BatchRmDyn *dyn = nullptr;
     int64_t *cb = nullptr, *mo = nullptr;  // mo: (offset, count) per system
+    int32_t *bsys = nullptr;               // system of each workgroup of cells
     double *xn = nullptr, *yn = nullptr;
     double3 *sb = nullptr;
     int32_t *cstart = nullptr;

@@ -164,14 +164,20 @@ __global__ void brm_gather_scaled(const int64_t *__restrict__ poff, int S,
     out[k] = make_double3(x[b] / h, y[b] / h, w[b]);
 }
 
-// touched = one of the 16 base cells (i-a, j-b), a, b in 0..3, holds a blob
-__global__ void brm_touched(const int64_t *__restrict__ cb, int S, const BatchRm *__restrict__ rm,
-                            const BatchRmDyn *__restrict__ dyn, const int32_t *__restrict__ cstart,
-                            int64_t C, uint32_t *__restrict__ flag)
+// touched = one of the 16 base cells (i-a, j-b), a, b in 0..3, holds a blob.
+// Cell kernels run BRM_BLOCK threads per workgroup; every system's cell range
+// is a whole number of workgroups, whose system bsys names.
+constexpr int BRM_BLOCK = 256;
+
+__global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restrict__ bsys,
+                                                         const BatchRm *__restrict__ rm,
+                                                         const BatchRmDyn *__restrict__ dyn,
+                                                         const int32_t *__restrict__ cstart,
+                                                         int64_t C, uint32_t *__restrict__ flag)
 {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = (int64_t)blockIdx.x * BRM_BLOCK + threadIdx.x;
     if (c >= C) return;
-    const int s = find_seg64(cb, S, c);
+    const int s = bsys[blockIdx.x];
     const BatchRmDyn d = dyn[s];
     const int64_t base = rm[s].cbase, loc = c - base;
     const long long nj = d.bj + 3;
@@ -189,16 +195,16 @@ __global__ void brm_touched(const int64_t *__restrict__ cb, int S, const BatchRm
     flag[c] = t;
 }
 
-__global__ void brm_gather_sum(const int64_t *__restrict__ cb, int S,
-                               const BatchRm *__restrict__ rm, const BatchRmDyn *__restrict__ dyn,
-                               const int32_t *__restrict__ cstart, const double3 *__restrict__ sb,
-                               const uint32_t *__restrict__ flag, const uint32_t *__restrict__ slot,
-                               int64_t C, double c, double c2, double *__restrict__ xo,
-                               double *__restrict__ yo, double *__restrict__ wo)
+__global__ __launch_bounds__(BRM_BLOCK) void brm_gather_sum(
+    const int32_t *__restrict__ bsys, const BatchRm *__restrict__ rm,
+    const BatchRmDyn *__restrict__ dyn, const int32_t *__restrict__ cstart,
+    const double3 *__restrict__ sb, const uint32_t *__restrict__ flag,
+    const uint32_t *__restrict__ slot, int64_t C, double c, double c2, double *__restrict__ xo,
+    double *__restrict__ yo, double *__restrict__ wo)
 {
-    const int64_t cidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t cidx = (int64_t)blockIdx.x * BRM_BLOCK + threadIdx.x;
     if (cidx >= C || !flag[cidx]) return;
-    const int s = find_seg64(cb, S, cidx);
+    const int s = bsys[blockIdx.x];
     const BatchRm r = rm[s];
     const BatchRmDyn d = dyn[s];
     const int64_t loc = cidx - r.cbase;
@@ -268,7 +274,9 @@ int batch_remesh_plan(const BatchPlan &P, const double *sigma, const phf_batch_r
         r.x0h = r.x0 / r.h;
         r.y0h = r.y0 / r.h;
         r.cbase = base;
-        r.ccap = batch_cell_capacity(P.sys[s].n, rp.cell_factor);
+        // whole workgroups of cells per system
+        r.ccap = (batch_cell_capacity(P.sys[s].n, rp.cell_factor) + BRM_BLOCK - 1) / BRM_BLOCK *
+                 BRM_BLOCK;
         base += r.ccap;
     }
     R->C = base;
@@ -278,6 +286,11 @@ int batch_remesh_plan(const BatchPlan &P, const double *sigma, const phf_batch_r
     while (R->key_bits < 32 && (int64_t(1) << R->key_bits) < R->C + 1) ++R->key_bits;
     R->cb.resize(P.S);
     for (int s = 0; s < P.S; ++s) R->cb[s] = R->rm[s].cbase;
+    R->bsys.resize((size_t)(R->C / BRM_BLOCK));
+    for (int s = 0; s < P.S; ++s)
+        for (int64_t b = R->rm[s].cbase / BRM_BLOCK; b < (R->rm[s].cbase + R->rm[s].ccap) / BRM_BLOCK;
+             ++b)
+            R->bsys[(size_t)b] = s;
     return PHF_OK;
 }
 
@@ -288,6 +301,7 @@ int batch_remesh_alloc(const BatchPlan &P, const BatchRemeshPlan &R, BatchRemesh
     PHF_ALLOC(Bf->rm, A, BatchRm, (size_t)P.S);
     PHF_ALLOC(Bf->dyn, A, BatchRmDyn, (size_t)P.S);
     PHF_ALLOC(Bf->cb, A, int64_t, (size_t)P.S);
+    PHF_ALLOC(Bf->bsys, A, int32_t, std::max<size_t>(R.bsys.size(), 1));
     PHF_ALLOC(Bf->xn, A, double, N);
     PHF_ALLOC(Bf->yn, A, double, N);
     PHF_ALLOC(Bf->sb, A, double3, N);
@@ -301,6 +315,9 @@ int batch_remesh_alloc(const BatchPlan &P, const BatchRemeshPlan &R, BatchRemesh
     PHF_ALLOC(Bf->mo, A, int64_t, (size_t)2 * P.S);
     PHF_HIP(hipMemcpy(Bf->rm, R.rm.data(), sizeof(BatchRm) * P.S, hipMemcpyHostToDevice));
     PHF_HIP(hipMemcpy(Bf->cb, R.cb.data(), sizeof(int64_t) * P.S, hipMemcpyHostToDevice));
+    if (!R.bsys.empty())
+        PHF_HIP(hipMemcpy(Bf->bsys, R.bsys.data(), sizeof(int32_t) * R.bsys.size(),
+                          hipMemcpyHostToDevice));
     return PHF_OK;
 }
 
@@ -339,12 +356,12 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
         brm_gather_scaled<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, xs, ys, w, vs,
                                                                  N, Bf.sb);
     if (C > 0) {
-        brm_touched<<<div_up(C, 256), 256, 0, L->stream>>>(Bf.cb, S, Bf.rm, Bf.dyn, Bf.cstart, C,
-                                                           Bf.flag);
+        brm_touched<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK, 0, L->stream>>>(
+            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, C, Bf.flag);
         PHF_HIP(hipGetLastError());
         PHF_TRY(device_exclusive_scan_ws(L, Bf.flag, C, Bf.slot, Bf.sums));
-        brm_gather_sum<<<div_up(C, 256), 256, 0, L->stream>>>(
-            Bf.cb, S, Bf.rm, Bf.dyn, Bf.cstart, Bf.sb, Bf.flag, Bf.slot, C, R.p.c, R.p.c * R.p.c,
+        brm_gather_sum<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK, 0, L->stream>>>(
+            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, Bf.sb, Bf.flag, Bf.slot, C, R.p.c, R.p.c * R.p.c,
             Bf.xo, Bf.yo, Bf.wo);
     }
     brm_counts<<<div_up(S, 64), 64, 0, L->stream>>>(Bf.rm, Bf.dyn, S, Bf.flag, Bf.slot, C, Bf.mo);
