@@ -849,29 +849,27 @@ __global__ __launch_bounds__(T) void chol_step(float *__restrict__ A, int n, int
 }
 
 // The factor in LAPACK getrf form, for the inverse and the refinement:
-// L Lᵀ = (L D⁻¹)(D Lᵀ), D = diag(L), pivots the identity.  First the upper
-// triangle U(i, j) = D(i) L(j, i), i < j (reads the lower triangle only) ...
-__global__ void chol_to_lu_upper(float *__restrict__ A, int n, const float *__restrict__ dg)
+// L Lᵀ = (L D⁻¹)(D Lᵀ), D = diag(L), pivots the identity.  One thread per
+// pair (i, j), i >= j, of the lower triangle: it reads L(i, j) and writes
+// both L(i, j) / D(j) below the diagonal and D(j) L(i, j) = U(j, i) above
+// it (D(j)² on the diagonal), so no entry is read after another thread
+// wrote it.
+__global__ void chol_to_lu(float *__restrict__ A, int n, const float *__restrict__ dg,
+                           int32_t *__restrict__ ipiv)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)n * n) return;
     const int j = (int)(e / n), i = (int)(e % n);  // column j, row i
-    if (i < j) A[e] = dg[i] * A[(int64_t)i * n + j];
-}
-
-// ... then the unit lower L(i, j) / D(j), the diagonal D(j)², the pivots
-__global__ void chol_to_lu_lower(float *__restrict__ A, int n, const float *__restrict__ dg,
-                                 int32_t *__restrict__ ipiv)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)n * n) return;
-    const int j = (int)(e / n), i = (int)(e % n);
-    if (i > j)
-        A[e] = A[e] / dg[j];
-    else if (i == j) {
-        A[e] = dg[j] * dg[j];
+    if (i < j) return;
+    const float d = dg[j];
+    if (i == j) {
+        A[e] = d * d;
         ipiv[j] = j;
+        return;
     }
+    const float l = A[e];
+    A[e] = l / d;
+    A[(int64_t)i * n + j] = d * l;
 }
 
 template <int T>
@@ -1338,8 +1336,7 @@ int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, fl
         }
     }
     const int64_t nn = (int64_t)n * n;
-    chol_to_lu_upper<<<div_up(nn, 256), 256, 0, L->stream>>>(A32, n, dg);
-    chol_to_lu_lower<<<div_up(nn, 256), 256, 0, L->stream>>>(A32, n, dg, ipiv);
+    chol_to_lu<<<div_up(nn, 256), 256, 0, L->stream>>>(A32, n, dg, ipiv);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
