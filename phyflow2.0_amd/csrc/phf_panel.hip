@@ -580,15 +580,25 @@ __device__ __forceinline__ void lu_factor_panel(float *__restrict__ A, int n, in
 // loads of all tiles are in flight together.
 // rows (nullable): the results go to LDS instead, row-major with stride
 // NB + 1 from row r0 (for a panel factor that follows in the same launch).
-template <int T>
+// mid() runs after the operand loads are issued and before the products
+// (it may fill Ut and wait at a barrier while those loads are in flight).
+struct NoMid {
+    __device__ void operator()() const {}
+};
+
+template <int T, class Mid = NoMid>
 __device__ __forceinline__ void lu_block_update(float *__restrict__ A, int n, int r0, int c0,
                                                 int wj, int k0, int w,
                                                 const float (*__restrict__ Ut)[NB + 1],
-                                                float *__restrict__ rows = nullptr)
+                                                float *__restrict__ rows = nullptr,
+                                                Mid mid = Mid())
 {
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int m = n - r0;
-    if (m <= 0) return;
+    if (m <= 0) {
+        mid();
+        return;
+    }
     constexpr int LU_TILES = 8;
     const int li = lane & 15, lk = lane >> 4;
     const int tiles_r = (m + 15) / 16, tiles_c = (wj + 15) / 16;
@@ -612,6 +622,7 @@ __device__ __forceinline__ void lu_block_update(float *__restrict__ A, int n, in
             av[q][kk] = (on && i < n && kc < w) ? A[(int64_t)(k0 + kc) * n + i] : 0.0f;
         }
     }
+    mid();
 #pragma unroll
     for (int q = 0; q < LU_TILES; ++q) {
         const int t = wv + q * (T / 64);
@@ -833,15 +844,25 @@ __global__ __launch_bounds__(T) void chol_step(float *__restrict__ A, int n, int
     const int tid = threadIdx.x;
     const int k0 = k * NB, w = min(NB, n - k0);
     const int c0 = j * NB, wj = min(NB, n - c0);
-    // Q = L(block j rows, panel k)ᵀ, read along the rows of each column
-    for (int e = tid; e < NB * NB; e += T) {
-        const int kk = e / NB, c = e % NB;
-        sh.Ut[c][kk] = (c < wj && kk < w) ? A[(int64_t)(k0 + kk) * n + c0 + c] : 0.0f;
+    // Q = L(block j rows, panel k)ᵀ, read along the rows of each column; its
+    // loads and the update's operand loads are in flight together
+    constexpr int QPER = (NB * NB + T - 1) / T;
+    float qv[QPER];
+#pragma unroll
+    for (int it = 0; it < QPER; ++it) {
+        const int e = tid + T * it, kk = e / NB, c = e % NB;
+        qv[it] = (e < NB * NB && c < wj && kk < w) ? A[(int64_t)(k0 + kk) * n + c0 + c] : 0.0f;
     }
-    __syncthreads();
     const bool next = j == k + 1;
     float *dst = next && kRowsInLds ? rows : nullptr;
-    lu_block_update<T>(A, n, c0, c0, wj, k0, w, sh.Ut, dst);
+    lu_block_update<T>(A, n, c0, c0, wj, k0, w, sh.Ut, dst, [&]() {
+#pragma unroll
+        for (int it = 0; it < QPER; ++it) {
+            const int e = tid + T * it;
+            if (e < NB * NB) sh.Ut[e % NB][e / NB] = qv[it];
+        }
+        lds_barrier();
+    });
     if (next) {
         __syncthreads();
         chol_factor_panel<T>(A, n, c0, info, dg, sh, dst);
