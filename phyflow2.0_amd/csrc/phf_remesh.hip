@@ -73,7 +73,14 @@ struct RmGeom {
     int64_t m;                         // output cells (set by the gather)
     int32_t ok;
     int32_t bits;                      // key bits of the base-cell index
+    int32_t crowded;                   // a base cell holds more than kCellSortMax blobs
+    int32_t pad;
 };
+
+// Base cells with at most this many blobs have their blobs put in input
+// order by one thread (insertion sort); a more crowded cell sends the call
+// to the radix sort instead.
+constexpr int kCellSortMax = 32;
 
 // Touched-range capacity of the dense layout for n blobs.
 static int64_t dense_cells_cap(int64_t n)
@@ -152,6 +159,8 @@ __global__ __launch_bounds__(RM_THREADS) void base_bounds_final(const longlong4 
     r.bi = r.bj = r.nj = 0;
     r.nstart = r.ncell = r.nscan = 0;
     r.bits = 0;
+    r.crowded = 0;
+    r.pad = 0;
     r.m = -1;
     if (r.ok) {
         r.bi = b.y - b.x + 1;
@@ -194,6 +203,46 @@ __global__ void base_keys(const double *__restrict__ x, const double *__restrict
     }
     keys[b] = key;
     vals[b] = (int32_t)b;
+}
+
+// Counting sort by base cell: blob b takes a place in its cell's range
+// [start[key], start[key + 1]) in arrival order (cursor: zeroed counts) ...
+__global__ void cell_scatter(const uint32_t *__restrict__ keys, int64_t n,
+                             const RmGeom *__restrict__ g, const uint32_t *__restrict__ start,
+                             uint32_t *__restrict__ cursor, int32_t *__restrict__ perm)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n || !g->ok) return;
+    const uint32_t key = keys[b];
+    perm[start[key] + atomicAdd(cursor + key, 1u)] = (int32_t)b;
+}
+
+// ... then each cell's blobs in ascending input index: the order of a
+// stable sort by base cell
+__global__ __launch_bounds__(RM_THREADS) void cell_order(const uint32_t *__restrict__ start,
+                                                         RmGeom *__restrict__ g,
+                                                         int32_t *__restrict__ perm)
+{
+    if (!g->ok) return;
+    const int64_t nb = g->nstart - 1;
+    for (int64_t k = (int64_t)blockIdx.x * RM_THREADS + threadIdx.x; k < nb;
+         k += (int64_t)gridDim.x * RM_THREADS) {
+        const uint32_t lo = start[k], hi = start[k + 1];
+        if (hi - lo < 2u) continue;
+        if (hi - lo > (uint32_t)kCellSortMax) {
+            g->crowded = 1;
+            continue;
+        }
+        for (uint32_t i = lo + 1; i < hi; ++i) {
+            const int32_t v = perm[i];
+            uint32_t j = i;
+            while (j > lo && perm[j - 1] > v) {
+                perm[j] = perm[j - 1];
+                --j;
+            }
+            perm[j] = v;
+        }
+    }
 }
 
 // The 16 stencil values of each blob in base-cell order: V[s n + q] for
@@ -619,40 +668,28 @@ int remesh_device(Lane *L, const double *x, const double *y, const double *w, in
     base_bounds_partial<<<nblk, RM_THREADS, 0, L->stream>>>(x, y, n, x0, y0, h, part);
     base_bounds_final<<<1, RM_THREADS, 0, L->stream>>>(part, nblk, ccap, g);
     PHF_HIP(hipGetLastError());
-    // 2. blobs sorted by base cell; cell starts from the cell populations
-    uint32_t *k0, *k1, *cnt, *sums;
-    int32_t *v0, *v1;
+    // 2. blobs ordered by base cell (counting sort; the radix sort when a
+    //    cell is crowded); cell starts from the cell populations
+    uint32_t *k0, *cnt, *sums, *ustart;
+    int32_t *v0;
     PHF_ALLOC(k0, L->arena, uint32_t, nn);
-    PHF_ALLOC(k1, L->arena, uint32_t, nn);
     PHF_ALLOC(v0, L->arena, int32_t, nn);
-    PHF_ALLOC(v1, L->arena, int32_t, nn);
     PHF_ALLOC(cnt, L->arena, uint32_t, (size_t)ccap + 1);
+    PHF_ALLOC(ustart, L->arena, uint32_t, (size_t)ccap + 1);
     PHF_ALLOC(sums, L->arena, uint32_t, (size_t)scan_sums_size(ccap + 1));
     zero_u32_dn<<<RM_GRID, RM_THREADS, 0, L->stream>>>(cnt, &g->nstart);
     base_keys<<<div_up(n, 256), 256, 0, L->stream>>>(x, y, n, x0, y0, h, g, k0, v0, cnt);
     PHF_HIP(hipGetLastError());
-    uint32_t *ks;
-    int32_t *perm;
-    PHF_TRY(device_radix_sort_dbits(L, k0, v0, k1, v1, n, bits_for((uint64_t)ccap), &g->bits, &ks,
-                                    &perm));
-    uint32_t *ustart;
-    PHF_ALLOC(ustart, L->arena, uint32_t, (size_t)ccap + 1);
     PHF_TRY(device_exclusive_scan_ws(L, cnt, ccap + 1, ustart, sums, &g->nstart));
     const int32_t *start = reinterpret_cast<const int32_t *>(ustart);
-    // 3. stencil values of every blob
-    double *V;
-    PHF_ALLOC(V, L->arena, double, 16 * nn);
-    PHF_TRY(launch_stencil_values(L, kind, x, y, w, perm, n, x0, y0, h, c, V));
-    // 4. touched cells and their output slots
-    uint32_t *flag = cnt, *slot;  // the populations are no longer needed
+    // the touched-cell scan below overwrites slot; until then it is the cursor
+    uint32_t *slot;
     PHF_ALLOC(slot, L->arena, uint32_t, (size_t)ccap + 1);
-    touched_cells<<<RM_GRID, RM_THREADS, 0, L->stream>>>(start, g, flag);
-    PHF_HIP(hipGetLastError());
-    PHF_TRY(device_exclusive_scan_ws(L, flag, ccap + 1, slot, sums, &g->nscan));
-    // 5. sums into outputs sized for the most cells n blobs can touch
+    // outputs sized for the most cells n blobs can touch
     const size_t cap = (size_t)std::min<int64_t>(16 * n, ccap);
-    double *xo, *yo, *wo;
+    double *V, *xo, *yo, *wo;
     int64_t *io = nullptr, *jo = nullptr;
+    PHF_ALLOC(V, L->arena, double, 16 * nn);
     PHF_ALLOC(xo, L->arena, double, cap);
     PHF_ALLOC(yo, L->arena, double, cap);
     PHF_ALLOC(wo, L->arena, double, cap);
@@ -660,21 +697,50 @@ int remesh_device(Lane *L, const double *x, const double *y, const double *w, in
         PHF_ALLOC(io, L->arena, int64_t, cap);
         PHF_ALLOC(jo, L->arena, int64_t, cap);
     }
-    gather_cells<<<RM_GRID, RM_THREADS, 0, L->stream>>>(start, V, n, slot, g, h, x0, y0, xo, yo,
-                                                        wo, io, jo);
-    PHF_HIP(hipGetLastError());
-    // the one synchronisation: the layout with the output count, and what
-    // the caller asked to read back
-    PHF_TRY(L->pin.ensure(sizeof(RmGeom) + out->read_bytes));
-    PHF_HIP(hipMemcpyAsync(L->pin.ptr, g, sizeof(RmGeom), hipMemcpyDeviceToHost, L->stream));
-    if (out->read_bytes)
-        PHF_HIP(hipMemcpyAsync(L->pin.ptr + sizeof(RmGeom), out->read_src, out->read_bytes,
-                               hipMemcpyDeviceToHost, L->stream));
-    PHF_TRY(L->pin.mark(L->stream));
-    PHF_HIP(hipStreamSynchronize(L->stream));
     RmGeom hg;
-    std::memcpy(&hg, L->pin.ptr, sizeof(hg));
-    if (out->read_bytes) std::memcpy(out->read_dst, L->pin.ptr + sizeof(RmGeom), out->read_bytes);
+    for (int radix = 0; radix < 2; ++radix) {
+        int32_t *perm = v0;
+        if (!radix) {
+            int32_t *pc;
+            PHF_ALLOC(pc, L->arena, int32_t, nn);
+            zero_u32_dn<<<RM_GRID, RM_THREADS, 0, L->stream>>>(slot, &g->nstart);
+            cell_scatter<<<div_up(n, 256), 256, 0, L->stream>>>(k0, n, g, ustart, slot, pc);
+            cell_order<<<RM_GRID, RM_THREADS, 0, L->stream>>>(ustart, g, pc);
+            PHF_HIP(hipGetLastError());
+            perm = pc;
+        } else {
+            uint32_t *k1, *ks;
+            int32_t *v1;
+            PHF_ALLOC(k1, L->arena, uint32_t, nn);
+            PHF_ALLOC(v1, L->arena, int32_t, nn);
+            PHF_TRY(device_radix_sort_dbits(L, k0, v0, k1, v1, n, bits_for((uint64_t)ccap),
+                                            &g->bits, &ks, &perm));
+        }
+        // 3. stencil values of every blob
+        PHF_TRY(launch_stencil_values(L, kind, x, y, w, perm, n, x0, y0, h, c, V));
+        // 4. touched cells and their output slots
+        uint32_t *flag = cnt;  // the populations are no longer needed
+        touched_cells<<<RM_GRID, RM_THREADS, 0, L->stream>>>(start, g, flag);
+        PHF_HIP(hipGetLastError());
+        PHF_TRY(device_exclusive_scan_ws(L, flag, ccap + 1, slot, sums, &g->nscan));
+        // 5. sums
+        gather_cells<<<RM_GRID, RM_THREADS, 0, L->stream>>>(start, V, n, slot, g, h, x0, y0, xo,
+                                                            yo, wo, io, jo);
+        PHF_HIP(hipGetLastError());
+        // the one synchronisation: the layout with the output count, and what
+        // the caller asked to read back
+        PHF_TRY(L->pin.ensure(sizeof(RmGeom) + out->read_bytes));
+        PHF_HIP(hipMemcpyAsync(L->pin.ptr, g, sizeof(RmGeom), hipMemcpyDeviceToHost, L->stream));
+        if (out->read_bytes)
+            PHF_HIP(hipMemcpyAsync(L->pin.ptr + sizeof(RmGeom), out->read_src, out->read_bytes,
+                                   hipMemcpyDeviceToHost, L->stream));
+        PHF_TRY(L->pin.mark(L->stream));
+        PHF_HIP(hipStreamSynchronize(L->stream));
+        std::memcpy(&hg, L->pin.ptr, sizeof(hg));
+        if (out->read_bytes)
+            std::memcpy(out->read_dst, L->pin.ptr + sizeof(RmGeom), out->read_bytes);
+        if (!(hg.ok && hg.crowded)) break;
+    }
     if (!hg.ok) return remesh_sparse(L, x, y, w, n, h, x0, y0, c, kind, hg, out);
     PHF_REQUIRE(hg.m >= 0 && hg.m <= (int64_t)cap, "remesh output count %lld out of range",
                 (long long)hg.m);

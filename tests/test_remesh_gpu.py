@@ -214,3 +214,23 @@ def test_regrid_dense_then_sparse_then_dense(O):
     xs, ys, ws = _spread_case("outliers")
     for x, y, w in ((xd, yd, wd), (xs, ys, ws), (xd[:1000], yd[:1000], wd[:1000])):
         _cmp(x, y, w, 0.004, 1.25, -0.0075, -0.0075, 0.0, O)
+
+
+def test_regrid_crowded_cell_matches_oracle(O):
+    """A base cell holding more blobs than the per-cell ordering handles (100
+    blobs inside one cell among spread ones): the remesh takes its radix-sort
+    path and still matches the oracle."""
+    rng = np.random.default_rng(31)
+    sigma, overlap = 0.01, 1.0
+    h = sigma * overlap
+    x = np.concatenate([rng.uniform(0.0, 1.0, 5000), 0.4 + h * rng.uniform(0.1, 0.9, 100)])
+    y = np.concatenate([rng.uniform(0.0, 1.0, 5000), 0.6 + h * rng.uniform(0.1, 0.9, 100)])
+    w = rng.standard_normal(x.size)
+    _cmp(x, y, w, sigma, overlap, -1.5 * h, -1.5 * h, 0.0, O)
+    # and with a few cells of 2..32 blobs (the per-cell ordering itself)
+    k = np.repeat(np.arange(40), np.arange(2, 42) % 31 + 2)
+    xs = 0.1 + (k % 7) * 3 * h + h * rng.uniform(0.1, 0.9, k.size)
+    ys = 0.2 + (k // 7) * 3 * h + h * rng.uniform(0.1, 0.9, k.size)
+    perm = rng.permutation(k.size)
+    _cmp(xs[perm], ys[perm], rng.standard_normal(k.size), sigma, overlap, -1.5 * h, -1.5 * h,
+         0.0, O)
