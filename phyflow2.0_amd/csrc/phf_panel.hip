@@ -575,9 +575,13 @@ __device__ __forceinline__ void lu_factor_panel(float *__restrict__ A, int n, in
 // columns k0 .. k0 + w of A on those rows, Q[kk][c] = Ut[c][kk] (w x wj) in
 // LDS.  16 x 16 tiles on v_mfma_f32_16x16x4f32 (A operand lane l holds
 // A[l%16][l/16], B operand B[l/16][l%16], result D[4 (l/16) + v][l%16]).
-// A wave holds at most LU_TILES tiles (n - r0 <= T: T covers the rows),
-// and every tile's operands are loaded before the first product, so the
-// loads of all tiles are in flight together.
+// A wave takes whole 16-row strips (at most LU_STRIPS; n - r0 <= T: T
+// covers the rows) and both 16-column tiles of each, so a strip's P
+// operand is loaded once for the two tiles.  A tile's result rows
+// 4 (l/16) .. + 3 of column l%16 are consecutive in the column-major A, so
+// the block is read and written 16 bytes per lane where A's columns are
+// 16-byte aligned.  Every strip's operands are loaded before the first
+// product, so the loads of all strips are in flight together.
 // rows (nullable): the results go to LDS instead, row-major with stride
 // NB + 1 from row r0 (for a panel factor that follows in the same launch).
 // mid() runs after the operand loads are issued and before the products
@@ -599,22 +603,33 @@ __device__ __forceinline__ void lu_block_update(float *__restrict__ A, int n, in
         mid();
         return;
     }
-    constexpr int LU_TILES = 8;
+    static_assert(NB == 32, "two 16-column tiles per block");
+    constexpr int LU_STRIPS = 4, WAVES = T / 64;
     const int li = lane & 15, lk = lane >> 4;
-    const int tiles_r = (m + 15) / 16, tiles_c = (wj + 15) / 16;
-    const int ntile = tiles_r * tiles_c;
-    float av[LU_TILES][NB / 4];
-    floatx4 acc[LU_TILES];
+    const int strips = (m + 15) / 16;
+    const bool vec = (n & 3) == 0 && (r0 & 3) == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+    float av[LU_STRIPS][NB / 4];
+    floatx4 acc[LU_STRIPS][2];
 #pragma unroll
-    for (int q = 0; q < LU_TILES; ++q) {
-        const int t = wv + q * (T / 64);
-        const int tr = t % tiles_r, tc = t / tiles_r;
-        const int row = r0 + 16 * tr, cl = 16 * tc;  // cl: column inside the block
-        const bool on = t < ntile;
+    for (int q = 0; q < LU_STRIPS; ++q) {
+        const int row = r0 + 16 * (wv + q * WAVES);
+        const bool on = wv + q * WAVES < strips;
+        const int i0 = row + 4 * lk;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = row + 4 * lk + v, jj = cl + li;
-            acc[q][v] = (on && i < n && jj < wj) ? A[(int64_t)(c0 + jj) * n + i] : 0.0f;
+        for (int tc = 0; tc < 2; ++tc) {
+            const int jj = 16 * tc + li;
+            const float *src = A + (int64_t)(c0 + jj) * n + i0;
+            const bool col = on && jj < wj;
+            if (col && vec && i0 + 3 < n) {
+                const float4 v4 = *reinterpret_cast<const float4 *>(src);
+                acc[q][tc][0] = v4.x;
+                acc[q][tc][1] = v4.y;
+                acc[q][tc][2] = v4.z;
+                acc[q][tc][3] = v4.w;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[q][tc][v] = (col && i0 + v < n) ? src[v] : 0.0f;
+            }
         }
 #pragma unroll
         for (int kk = 0; kk < NB / 4; ++kk) {
@@ -624,25 +639,36 @@ __device__ __forceinline__ void lu_block_update(float *__restrict__ A, int n, in
     }
     mid();
 #pragma unroll
-    for (int q = 0; q < LU_TILES; ++q) {
-        const int t = wv + q * (T / 64);
-        if (t >= ntile) break;  // uniform across the wave
-        const int tr = t % tiles_r, tc = t / tiles_r;
-        const int row = r0 + 16 * tr, cl = 16 * tc;
+    for (int q = 0; q < LU_STRIPS; ++q) {
+        if (wv + q * WAVES >= strips) break;  // uniform across the wave
+        const int row = r0 + 16 * (wv + q * WAVES);
+        const int i0 = row + 4 * lk;
 #pragma unroll
-        for (int kk = 0; kk < NB / 4; ++kk) {
-            const int kc = 4 * kk + lk;
-            const float b = (kc < w && cl + li < wj) ? Ut[cl + li][kc] : 0.0f;
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[q][kk], b, acc[q], 0, 0, 0);
-        }
+        for (int tc = 0; tc < 2; ++tc) {
+            if (16 * tc >= wj) break;  // uniform
+            const int jj = 16 * tc + li;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int i = row + 4 * lk + v, jj = cl + li;
-            if (i < n && jj < wj) {
-                if (rows)
-                    rows[(i - r0) * (NB + 1) + jj] = acc[q][v];
-                else
-                    A[(int64_t)(c0 + jj) * n + i] = acc[q][v];
+            for (int kk = 0; kk < NB / 4; ++kk) {
+                const int kc = 4 * kk + lk;
+                const float b = (kc < w && jj < wj) ? Ut[jj][kc] : 0.0f;
+                acc[q][tc] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[q][kk], b, acc[q][tc], 0, 0,
+                                                                  0);
+            }
+            if (jj >= wj) continue;
+            if (rows) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (i0 + v < n) rows[(i0 + v - r0) * (NB + 1) + jj] = acc[q][tc][v];
+            } else {
+                float *dst = A + (int64_t)(c0 + jj) * n + i0;
+                if (vec && i0 + 3 < n) {
+                    *reinterpret_cast<float4 *>(dst) =
+                        make_float4(acc[q][tc][0], acc[q][tc][1], acc[q][tc][2], acc[q][tc][3]);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (i0 + v < n) dst[v] = acc[q][tc][v];
+                }
             }
         }
     }
@@ -762,33 +788,44 @@ __device__ __forceinline__ void chol_factor_panel(float *__restrict__ A, int n, 
         a[j] = (live && j < w) ? (rows ? rows[tid * (NB + 1) + j] : A[(int64_t)(c0 + j) * n + row])
                                : 0.0f;
     if (wv == 0) {
+        // Columns w .. NB - 1 of a narrow last panel are the identity (a
+        // unit diagonal and zeros; their rows are past n), so every column
+        // is eliminated the same way, without a branch: a branch target
+        // inside the sequence makes the compiler wait for every memory
+        // operation in flight at each column.
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            if (j >= w && lane == j) a[j] = 1.0f;
         // branch-free: lanes at or above row c run the update with a zero
         // multiplier (a divergent branch would copy the row array at its
         // merge); 1 / lcc by the hardware reciprocal square root (the
-        // factor is refined in float64 afterwards)
+        // factor is refined in float64 afterwards), kept by lane c
         int bad = -1;
+        float invl = 1.0f;
 #pragma unroll
         for (int c = 0; c < NB; ++c) {
-            if (c < w) {  // uniform
-                const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[c]), c));
-                const bool pos = d > 0.0f;
-                if (!pos && bad < 0) bad = c;
-                const float inv = pos ? __builtin_amdgcn_rsqf(d) : 1.0f;
-                const float lcc = pos ? d * inv : 1.0f;
-                // L(r, c) = a(r, c) / lcc; a(r, j) -= L(r, c) L(j, c) with
-                // L(j, c) = a(c, j) / lcc by symmetry
-                const bool below = lane > c;
-                const float l = below ? a[c] * inv : 0.0f, li = l * inv;
-                float rowc[NB];
+            const float d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[c]), c));
+            const bool pos = d > 0.0f;
+            if (!pos && bad < 0) bad = c;
+            const float inv = pos ? __builtin_amdgcn_rsqf(d) : 1.0f;
+            const float lcc = pos ? d * inv : 1.0f;
+            // L(r, c) = a(r, c) / lcc; a(r, j) -= L(r, c) L(j, c) with
+            // L(j, c) = a(c, j) / lcc by symmetry
+            const bool below = lane > c;
+            const float l = below ? a[c] * inv : 0.0f, li = l * inv;
+            float rowc[NB];
 #pragma unroll
-                for (int j = c + 1; j < NB; ++j)
-                    rowc[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[j]), c));
+            for (int j = c + 1; j < NB; ++j)
+                rowc[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a[j]), c));
 #pragma unroll
-                for (int j = c + 1; j < NB; ++j) a[j] = __builtin_fmaf(-li, rowc[j], a[j]);
-                a[c] = below ? l : (lane == c ? lcc : a[c]);
-                if (lane == 0) sh.inv[c] = inv;
-            }
+            for (int j = c + 1; j < NB; ++j) a[j] = __builtin_fmaf(-li, rowc[j], a[j]);
+            a[c] = below ? l : (lane == c ? lcc : a[c]);
+            invl = lane == c ? inv : invl;
+            // column by column: interleaving columns chains each a[j]'s
+            // readlane and update across columns, with wait states between
+            __builtin_amdgcn_sched_barrier(0);
         }
+        if (lane < NB) sh.inv[lane] = invl;
         if (lane < NB) {
 #pragma unroll
             for (int q = 0; q < NB / 4; ++q)
