@@ -911,23 +911,29 @@ __global__ __launch_bounds__(T) void chol_step(float *__restrict__ A, int n, int
 // pair (i, j), i >= j, of the lower triangle: it reads L(i, j) and writes
 // both L(i, j) / D(j) below the diagonal and D(j) L(i, j) = U(j, i) above
 // it (D(j)² on the diagonal), so no entry is read after another thread
-// wrote it.
+// wrote it.  Lt (nullable): also the strictly lower part of L D⁻¹,
+// row-major ldt x ldt (ldt >= n), zeros elsewhere (for spd_inverse); the
+// threads then cover ldt x ldt.
 __global__ void chol_to_lu(float *__restrict__ A, int n, const float *__restrict__ dg,
-                           int32_t *__restrict__ ipiv)
+                           int32_t *__restrict__ ipiv, float *__restrict__ Lt, int ldt)
 {
+    const int m = Lt ? ldt : n;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)n * n) return;
-    const int j = (int)(e / n), i = (int)(e % n);  // column j, row i
-    if (i < j) return;
+    if (e >= (int64_t)m * m) return;
+    const int j = (int)(e / m), i = (int)(e % m);  // column j, row i
+    if (Lt && (i <= j || i >= n)) Lt[(int64_t)i * ldt + j] = 0.0f;
+    if (i < j || i >= n) return;
+    const int64_t ea = (int64_t)j * n + i;
     const float d = dg[j];
     if (i == j) {
-        A[e] = d * d;
+        A[ea] = d * d;
         ipiv[j] = j;
         return;
     }
-    const float l = A[e];
-    A[e] = l / d;
+    const float l = A[ea];
+    A[ea] = l / d;
     A[(int64_t)i * n + j] = d * l;
+    if (Lt) Lt[(int64_t)i * ldt + j] = l / d;
 }
 
 template <int T>
@@ -1166,6 +1172,170 @@ __global__ __launch_bounds__(64 * INV_W) void lu_inverse_mfma(const float *__res
     }
 }
 
+// ------------------------------ inverse of the Cholesky-factored matrix ---
+// The Cholesky factors in getrf form are N = L' D² L'ᵀ (L' unit lower, D²
+// the diagonal of U), so N⁻¹ = W'ᵀ D⁻² W' with W' = L'⁻¹.  W' is formed by
+// doubling rather than by n column solves: the 32 x 32 diagonal blocks are
+// inverted first (one wave each), then for h = 32, 64, ... each pair of
+// inverted h-blocks of [L11 0; L21 L22] is completed by
+// W21 = -W22 (L21 W11), two products on the matrix cores per level; X =
+// W'ᵀ D⁻² W' follows in 32 x 32 tiles over the whole matrix.  The work
+// arrays are npad x npad (npad = n rounded up to 32), and the rows and
+// columns past n are the identity in W', which adds nothing to X.  The
+// products read their operands 16 bytes per lane: lanes 16 q .. 16 q + 15
+// take k = k0 + 4 q + s at step s of a 16-wide chunk of the sum (an order
+// of the terms, the same for both operands), so A must be row-major and B
+// column-major.  The k ranges skip the blocks above the diagonal of W',
+// which are never written.
+constexpr int SI_B = 32;
+
+// Block b of W' = L'⁻¹ (a 32 x 32 unit lower block): lane j forms column j
+// by forward substitution against the block of L' (rows of Lt in LDS,
+// zeros on and above the diagonal), written column-major to Wc and
+// row-major to Wr, zeros above the diagonal included.
+__global__ __launch_bounds__(64) void spd_inv_diag(const float *__restrict__ Lt, int npad,
+                                                   float *__restrict__ Wc, float *__restrict__ Wr)
+{
+    __shared__ float4 Ls[SI_B][SI_B / 4];
+    const int lane = threadIdx.x, r0 = SI_B * blockIdx.x;
+    for (int e = lane; e < SI_B * SI_B / 4; e += 64) {
+        const int r = e / (SI_B / 4), q = e % (SI_B / 4);
+        Ls[r][q] = *reinterpret_cast<const float4 *>(Lt + (int64_t)(r0 + r) * npad + r0 + 4 * q);
+    }
+    __syncthreads();
+    if (lane >= SI_B) return;
+    const int j = lane;
+    float w[SI_B];  // entries above j come out 0: their sums have only zero terms
+#pragma unroll
+    for (int i = 0; i < SI_B; ++i) {
+        float s = i == j ? 1.0f : 0.0f;
+#pragma unroll
+        for (int q = 0; 4 * q < i; ++q) {
+            const float4 l4 = Ls[i][q];
+            s = __builtin_fmaf(-l4.x, w[4 * q], s);
+            if (4 * q + 1 < i) s = __builtin_fmaf(-l4.y, w[4 * q + 1], s);
+            if (4 * q + 2 < i) s = __builtin_fmaf(-l4.z, w[4 * q + 2], s);
+            if (4 * q + 3 < i) s = __builtin_fmaf(-l4.w, w[4 * q + 3], s);
+        }
+        w[i] = s;
+    }
+    float *wc = Wc + (int64_t)(r0 + j) * npad + r0;
+#pragma unroll
+    for (int q = 0; q < SI_B / 4; ++q)
+        *reinterpret_cast<float4 *>(wc + 4 * q) =
+            make_float4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
+#pragma unroll
+    for (int i = 0; i < SI_B; ++i) Wr[(int64_t)(r0 + i) * npad + r0 + j] = w[i];
+}
+
+// 16 x 16 tile of A B on the matrix cores over k in [ka, kb) (multiples of
+// 16): a points at row li of A (row-major), b at column li of B
+// (column-major), both at k = 0; the result lane holds D(4 q + v, li).
+__device__ __forceinline__ floatx4 spd_tile(const float *__restrict__ a, const float *__restrict__ b,
+                                            int ka, int kb, int q)
+{
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = ka; k < kb; k += 16) {
+        const float4 a4 = *reinterpret_cast<const float4 *>(a + k + 4 * q);
+        const float4 b4 = *reinterpret_cast<const float4 *>(b + k + 4 * q);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x, b4.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y, b4.y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.z, b4.z, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w, b4.w, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Level h of the doubling for the pair of blocks at r0 = 2 h blockIdx.y
+// (the second block has h2 = min(h, npad - r0 - h) rows); a workgroup
+// computes a 32 x 32 tile (ti, tj) of the h2 x h result, a wave 16 x 16 of
+// it.  STAGE 0: Wt = L21 W11, k from the tile's first column (W11 is lower
+// triangular), stored where W21 goes; STAGE 1: W21 = -W22 Wt, k up to the
+// tile's last row (W22 is lower triangular).
+template <int STAGE>
+__global__ __launch_bounds__(256) void spd_inv_level(const float *__restrict__ Lt,
+                                                     float *__restrict__ Wc,
+                                                     float *__restrict__ Wr,
+                                                     float *__restrict__ Wt, int npad, int h)
+{
+    const int r0 = 2 * h * blockIdx.y, h2 = min(h, npad - r0 - h);
+    const int tiles = h / SI_B, ti = blockIdx.x / tiles, tj = blockIdx.x % tiles;
+    if (h2 <= 0 || SI_B * ti >= h2) return;  // uniform across the workgroup
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, q = lane >> 4;
+    const int i0 = SI_B * ti + 16 * (wv & 1), j0 = SI_B * tj + 16 * (wv >> 1);
+    const int row = r0 + h + i0, col = r0 + j0;  // of this wave's tile in W'
+    floatx4 acc;
+    if (STAGE == 0) {
+        acc = spd_tile(Lt + (int64_t)(row + li) * npad + r0, Wc + (int64_t)(col + li) * npad + r0,
+                       SI_B * tj, h, q);
+        *reinterpret_cast<float4 *>(Wt + (int64_t)(col + li) * npad + row + 4 * q) =
+            make_float4(acc[0], acc[1], acc[2], acc[3]);
+    } else {
+        acc = spd_tile(Wr + (int64_t)(row + li) * npad + r0 + h,
+                       Wt + (int64_t)(col + li) * npad + r0 + h, 0, SI_B * (ti + 1), q);
+        *reinterpret_cast<float4 *>(Wc + (int64_t)(col + li) * npad + row + 4 * q) =
+            make_float4(-acc[0], -acc[1], -acc[2], -acc[3]);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Wr[(int64_t)(row + 4 * q + v) * npad + col + li] = -acc[v];
+    }
+}
+
+// X = W'ᵀ D⁻² W' (n x n, row-major), tile (blockIdx.x, blockIdx.y) with k
+// from the larger of its row and column offsets (W' is lower triangular).
+// D² is the diagonal of the Cholesky factor squared.  X is symmetric, so a
+// lane's D(i, j) for rows i = 4 q + v of column j goes to the row-major
+// place of (j, i): 16 bytes along row j.
+__global__ __launch_bounds__(256) void spd_inv_x(const float *__restrict__ Wc,
+                                                 const float *__restrict__ dg, int n, int npad,
+                                                 float *__restrict__ X)
+{
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, q = lane >> 4;
+    const int i0 = SI_B * blockIdx.x + 16 * (wv & 1), j0 = SI_B * blockIdx.y + 16 * (wv >> 1);
+    const float *a = Wc + (int64_t)(i0 + li) * npad, *b = Wc + (int64_t)(j0 + li) * npad;
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = SI_B * max(blockIdx.x, blockIdx.y); k < npad; k += 16) {
+        const int kq = k + 4 * q;
+        float4 a4 = *reinterpret_cast<const float4 *>(a + kq);
+        const float4 b4 = *reinterpret_cast<const float4 *>(b + kq);
+        float s[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const float d = kq + t < n ? dg[kq + t] : 1.0f;
+            s[t] = kq + t < n ? 1.0f / (d * d) : 0.0f;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x * s[0], b4.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y * s[1], b4.y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.z * s[2], b4.z, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w * s[3], b4.w, acc, 0, 0, 0);
+    }
+    const int j = j0 + li, i = i0 + 4 * q;
+    if (j >= n) return;
+    float *x = X + (int64_t)j * n + i;
+    if ((n & 3) == 0 && i + 3 < n) {
+        *reinterpret_cast<float4 *>(x) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (i + v < n) x[v] = acc[v];
+    }
+}
+
+// Xinv = N⁻¹ from the Cholesky factors of f (Lt, dg filled by the
+// factorisation).
+static int spd_inverse(Lane *L, DenseFactors *f)
+{
+    const int n = (int)f->n, npad = SI_B * (int)div_up(f->n, SI_B), nb = npad / SI_B;
+    spd_inv_diag<<<nb, 64, 0, L->stream>>>(f->Lt, npad, f->Wc, f->Wr);
+    for (int h = SI_B; h < npad; h *= 2) {
+        const dim3 grid((h / SI_B) * (h / SI_B), (unsigned)div_up(npad, 2 * h));
+        spd_inv_level<0><<<grid, 256, 0, L->stream>>>(f->Lt, f->Wc, f->Wr, f->Wt, npad, h);
+        spd_inv_level<1><<<grid, 256, 0, L->stream>>>(f->Lt, f->Wc, f->Wr, f->Wt, npad, h);
+    }
+    spd_inv_x<<<dim3(nb, nb), 256, 0, L->stream>>>(f->Wc, f->dg, n, npad, f->Xinv);
+    PHF_HIP(hipGetLastError());
+    return PHF_OK;
+}
+
 // ------------------------------------------------- refinement kernels -----
 // Convergence state of one refinement, kept on the device so that sweeps
 // run back to back; every kernel of a sweep returns at once when done.
@@ -1378,7 +1548,8 @@ int lu_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, void
     return PHF_OK;
 }
 
-int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, float *dg)
+int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, float *dg,
+                    float *Lt, int ldt)
 {
     if (n <= 0) return PHF_OK;
     PHF_REQUIRE(n <= LU_TMAX, "Cholesky factorisation of n = %d (at most %d)", n, LU_TMAX);
@@ -1393,8 +1564,10 @@ int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, fl
         default: chol_launch<1024>(L, A32, n, k, nsteps, info, dg); break;
         }
     }
-    const int64_t nn = (int64_t)n * n;
-    chol_to_lu<<<div_up(nn, 256), 256, 0, L->stream>>>(A32, n, dg, ipiv);
+    PHF_REQUIRE(!Lt || ldt >= n, "Cholesky: row stride %d of the unit lower copy below n = %d",
+                ldt, n);
+    const int m = Lt ? ldt : n;
+    chol_to_lu<<<div_up((int64_t)m * m, 256), 256, 0, L->stream>>>(A32, n, dg, ipiv, Lt, ldt);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }
@@ -1433,6 +1606,13 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
         PHF_ALLOC(f->Xinv, L->arena, float, (size_t)(n * n));
         PHF_ALLOC(f->dg, L->arena, float, (size_t)n);
         if (normal) PHF_ALLOC(f->N64, L->arena, double, (size_t)(n * n));
+        if (normal && n <= LU_TMAX) {
+            const size_t np = (size_t)(SI_B * div_up(n, SI_B));
+            PHF_ALLOC(f->Lt, L->arena, float, np * np);
+            PHF_ALLOC(f->Wc, L->arena, float, np * np);
+            PHF_ALLOC(f->Wr, L->arena, float, np * np);
+            PHF_ALLOC(f->Wt, L->arena, float, np * np);
+        }
         if (n <= LU_TMAX) {
             char *w;
             PHF_ALLOC(w, L->arena, char, lu_work_bytes((int)n));
@@ -1450,11 +1630,15 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
     // search); B itself: LU with partial pivoting
     const bool chol = normal && n <= LU_TMAX;
     if (chol)
-        PHF_TRY(chol_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->dg));
+        PHF_TRY(chol_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->dg, f->Lt,
+                                SI_B * (int)div_up(n, SI_B)));
     else
         PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->lu_work));
     if (marks) PHF_HIP(hipEventRecord(marks[1], L->stream));
-    PHF_TRY(dense_inverse(L, f));
+    if (chol)
+        PHF_TRY(spd_inverse(L, f));
+    else
+        PHF_TRY(dense_inverse(L, f));
     if (marks) PHF_HIP(hipEventRecord(marks[2], L->stream));
     f->Br = B;
     f->cholesky = chol;

@@ -20,8 +20,11 @@ size_t lu_work_bytes(int n);
 // In-place float32 Cholesky of the symmetric positive definite column-major
 // n x n A32 (n <= 1024), returned in the LU form above (unit lower L D^-1,
 // upper D L^T, identity pivots); dg: n floats of device memory (the
-// diagonal of L).  info = column + 1 of the first pivot <= 0.
-int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, float *dg);
+// diagonal of L).  info = column + 1 of the first pivot <= 0.  Lt
+// (optional): also the strictly lower part of the unit lower factor,
+// row-major with row stride ldt, zeros above it.
+int chol_factor_f32(Lane *L, float *A32, int n, int32_t *ipiv, int32_t *info, float *dg,
+                    float *Lt = nullptr, int ldt = 0);
 
 // Float32 LU factors of B (rows == n), or of B^T B (rows > n: Cholesky in
 // LU form, the pivoted LU if B^T B is not positive definite in float32), and the
@@ -36,6 +39,10 @@ struct DenseFactors {
     double *N64 = nullptr;       // B^T B (rows > n)
     void *lu_work = nullptr;
     float *dg = nullptr;         // diagonal of the Cholesky factor
+    // Cholesky path: the unit lower factor row-major (Lt), its inverse
+    // column- and row-major (Wc, Wr) and a product buffer (Wt), each
+    // npad x npad with npad = n rounded up to 32
+    float *Lt = nullptr, *Wc = nullptr, *Wr = nullptr, *Wt = nullptr;
     bool cholesky = false;       // B^T B factored by Cholesky (else pivoted LU)
     int32_t *ipiv = nullptr, *info = nullptr;
     int singular_at = 0;

@@ -214,3 +214,21 @@ def test_prescribed_solve_uses_cholesky_and_matches_lstsq(O):
     x = PS.dense_solve(B, c)
     ref = np.linalg.lstsq(B, c, rcond=None)[0]
     assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("m", [1, 20, 33, 100, 257, 512])
+def test_least_squares_solve_sizes(m):
+    """Least squares at sizes whose 32-row blocks are uneven.  The normal
+    matrix's float32 inverse comes from its Cholesky factors (diagonal
+    32 x 32 blocks of the unit lower factor inverted, then completed by
+    doubling, then W'ᵀ D⁻² W'): an inverse with a misplaced block leaves the
+    float64 refinement slow or divergent, so the sweep count is bounded as
+    well as the distance to lstsq."""
+    rng = np.random.default_rng(100 + m)
+    B = rng.standard_normal((m + m // 3 + 1, m))
+    c = rng.standard_normal(B.shape[0])
+    x, info = PS.dense_solve(B, c, return_info=True)
+    ref = np.linalg.lstsq(B, c, rcond=None)[0]
+    assert info["converged"], info
+    assert info["iterations"] <= 8, info
+    assert np.max(np.abs(x - ref)) <= 1e-12 * np.max(np.abs(ref))
