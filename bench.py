@@ -116,6 +116,7 @@ def host_cpu():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fmm_1m_p2p_hbm_traffic.json")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "fmm_1m_p2p_valu_issue.json")
 
 
 def measured_traffic(kernel, args):
@@ -133,6 +134,24 @@ def measured_traffic(kernel, args):
         return None
     return {"bytes_per_launch": d["bytes_per_launch"],
             "source": "profiles/fmm_1m_p2p_hbm_traffic.json"}
+
+
+def measured_issue(kernel, args):
+    """Fraction of the SIMDs' issue cycles the P2P kernel keeps its VALU
+    busy at the default workload, from the committed rocprofv3
+    SQ_INSTS_VALU / GRBM_GUI_ACTIVE summary; None when the summary is absent
+    or was taken on another kernel or workload."""
+    try:
+        with open(ISSUE_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("kernel") != kernel or (args.n, args.order, args.leaf, args.prec) != (
+            1048576, 16, 64, "f32c"):
+        return None
+    return {"valu_issue_frac": d["valu_issue_frac"],
+            "lane_instructions_per_pair": d["lane_instructions_per_pair"],
+            "source": "profiles/fmm_1m_p2p_valu_issue.json"}
 
 
 def cpu_baseline(s, budget_s=40.0):
@@ -860,6 +879,14 @@ def main():
                     "frac": m2l_flop / t_m2l / 1e12 / peak},
         },
     }
+    iss = measured_issue(out["roofline"]["kernel"], args)
+    if iss:
+        out["roofline"]["valu_issue_frac"] = iss["valu_issue_frac"]
+        out["roofline"]["lane_instructions_per_pair"] = iss["lane_instructions_per_pair"]
+        out["roofline"]["issue_source"] = iss["source"]
+        out["roofline"]["issue_note"] = (
+            "vector-issue bound: the peak counts packed FP32 (two lanes per slot), which "
+            "this build does not emit, and counts every slot as an FMA")
     tr = measured_traffic(out["roofline"]["kernel"], args)
     if tr:
         out["roofline"]["traffic"] = tr["bytes_per_launch"]
