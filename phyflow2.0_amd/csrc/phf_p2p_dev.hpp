@@ -150,24 +150,26 @@ __device__ __forceinline__ int p2p_stage_batch(const float4 *__restrict__ src,
     return stage_near_batch<P2P_STAGE>(src, sp, st, nnear, nf, xf, q, jq, core_end);
 }
 
-// Sum over the staged entries x in [a, b) with x = o (mod 16) for NT
+// Sum over the staged entries x in [a, b) with x = o (mod S) for NT
 // targets (xt[u], yt[u]) at once (each staged record read once for all of
 // them): per target, plain float32 partials over at most 16 pairs folded
-// into the TwoSum accumulators (the same order for any NT).
-template <int KIND, bool CORE, int NT>
+// into the TwoSum accumulators (the same order for any NT).  S = 16 (a
+// lane's share when 16 lanes serve 16 targets) or 32 (two lanes per target).
+template <int KIND, bool CORE, int NT, int S = 16>
 __device__ __forceinline__ void p2p_sweep(const float4 *__restrict__ st, int a, int b, int o,
                                           const float *xt, const float *yt, float c2s,
                                           float scale_f, float *ha, float *la, float *hb,
                                           float *lb)
 {
-    int x = a + ((o - a) & 15);
-    for (; x + 240 < b; x += 256) {
+    static_assert(S == 16 || S == 32, "stride");
+    int x = a + ((o - a) & (S - 1));
+    for (; x + 15 * S < b; x += 16 * S) {
         float aa[NT], ab[NT];
 #pragma unroll
         for (int u = 0; u < NT; ++u) aa[u] = ab[u] = 0.f;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            const float4 q = st[x + 16 * k];
+            const float4 q = st[x + S * k];
 #pragma unroll
             for (int u = 0; u < NT; ++u) {
                 const float dx = xt[u] - q.x, dy = yt[u] - q.y;
@@ -186,10 +188,10 @@ __device__ __forceinline__ void p2p_sweep(const float4 *__restrict__ st, int a, 
         float aa[NT], ab[NT];
 #pragma unroll
         for (int u = 0; u < NT; ++u) aa[u] = ab[u] = 0.f;
-        for (; x + 48 < b; x += 64) {
+        for (; x + 3 * S < b; x += 4 * S) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const float4 q = st[x + 16 * k];
+                const float4 q = st[x + S * k];
 #pragma unroll
                 for (int u = 0; u < NT; ++u) {
                     const float dx = xt[u] - q.x, dy = yt[u] - q.y;
@@ -199,7 +201,7 @@ __device__ __forceinline__ void p2p_sweep(const float4 *__restrict__ st, int a, 
                 }
             }
         }
-        for (; x < b; x += 16) {
+        for (; x < b; x += S) {
             const float4 q = st[x];
 #pragma unroll
             for (int u = 0; u < NT; ++u) {
@@ -228,7 +230,8 @@ constexpr int P2P_CHUNKS = 8;  // target chunks of 16 per round (128 targets)
 // staged batch of near sources is swept by all four waves for every chunk,
 // lane 16 g + t of wave w taking target t of the chunk and the staged
 // entries = 4 w + g (mod 16), so the waves carry equal work whatever the
-// leaf population.  Per chunk the four groups of a wave are combined by a
+// leaf population (a last chunk of at most 8 targets: lane 16 g + t takes
+// target t mod 8 and the entries = 4 w + g + 16 (t / 8) (mod 32)).  Per chunk the four groups of a wave are combined by a
 // butterfly and the wave's partial is merged into its own LDS slot
 // acc[(c * 4 + w) * 16 + t]; at the end of a round one thread per target
 // merges the four waves' slots in wave order and calls
@@ -246,6 +249,12 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
     const float c2s = pp.c2_f * (lam * lam);
     for (int32_t rc = t0; rc < t1; rc += 16 * P2P_CHUNKS) {
         const int nch = min(P2P_CHUNKS, (int)((t1 - rc + 15) >> 4));
+        // a last chunk of 1 .. 8 targets goes alone with two lanes per
+        // target (sources x = o + 16 (t >> 3) mod 32), so half its lanes
+        // are not spent on absent targets
+        const int rem = (int)min(t1 - rc, (int32_t)(16 * P2P_CHUNKS)) & 15;
+        const bool half = rem >= 1 && rem <= 8;
+        const int npair = half ? nch - 1 : nch;
         if (g == 0)
             for (int c = 0; c < nch; ++c)
                 acc[(c * 4 + wv) * 16 + t] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -255,7 +264,7 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
             const int used = p2p_stage_batch(src, sp, st, nnear, nf, lam, q, jq, core_end);
             __syncthreads();
             // chunks two at a time (each staged record read once for both)
-            for (int c = 0; c < nch; c += 2) {
+            for (int c = 0; c < npair; c += 2) {
                 float xt[2], yt[2], ha[2] = {}, la[2] = {}, hb[2] = {}, lb[2] = {};
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
@@ -264,7 +273,7 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
                     xt[u] = tp.x * lam;
                     yt[u] = tp.y * lam;
                 }
-                const int nu = min(2, nch - c);
+                const int nu = min(2, npair - c);
                 if (nu == 2) {
                     p2p_sweep<KIND, true, 2>(st, 0, core_end, o, xt, yt, c2s, pp.scale_f, ha, la,
                                              hb, lb);
@@ -285,6 +294,28 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
                         two_sum_merge(v.z, v.w, hb[u], lb[u]);
                         A = v;
                     }
+                }
+            }
+            if (half) {
+                const int c = nch - 1, th = t & 7;
+                const int32_t i = rc + 16 * c + th;
+                const float2 tp = i < t1 ? tgt(i) : make_float2(0.f, 0.f);
+                float xt[1] = {tp.x * lam}, yt[1] = {tp.y * lam};
+                float ha[1] = {}, la[1] = {}, hb[1] = {}, lb[1] = {};
+                const int oh = o + 16 * (t >> 3);
+                p2p_sweep<KIND, true, 1, 32>(st, 0, core_end, oh, xt, yt, c2s, pp.scale_f, ha, la,
+                                             hb, lb);
+                p2p_sweep<KIND, false, 1, 32>(st, core_end, used, oh, xt, yt, c2s, pp.scale_f, ha,
+                                              la, hb, lb);
+                two_sum_merge(ha[0], la[0], __shfl_xor(ha[0], 8), __shfl_xor(la[0], 8));
+                two_sum_merge(hb[0], lb[0], __shfl_xor(hb[0], 8), __shfl_xor(lb[0], 8));
+                p2p_merge_groups(ha[0], la[0], hb[0], lb[0]);
+                if (g == 0 && t < 8) {
+                    float4 &A = acc[(c * 4 + wv) * 16 + t];
+                    float4 v = A;
+                    two_sum_merge(v.x, v.y, ha[0], la[0]);
+                    two_sum_merge(v.z, v.w, hb[0], lb[0]);
+                    A = v;
                 }
             }
             __syncthreads();

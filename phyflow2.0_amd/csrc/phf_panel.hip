@@ -1235,6 +1235,7 @@ __device__ __forceinline__ floatx4 spd_tile(const float *__restrict__ a, const f
                                             int ka, int kb, int q)
 {
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
     for (int k = ka; k < kb; k += 16) {
         const float4 a4 = *reinterpret_cast<const float4 *>(a + k + 4 * q);
         const float4 b4 = *reinterpret_cast<const float4 *>(b + k + 4 * q);
@@ -1293,15 +1294,16 @@ __global__ __launch_bounds__(256) void spd_inv_x(const float *__restrict__ Wc,
     const int i0 = SI_B * blockIdx.x + 16 * (wv & 1), j0 = SI_B * blockIdx.y + 16 * (wv >> 1);
     const float *a = Wc + (int64_t)(i0 + li) * npad, *b = Wc + (int64_t)(j0 + li) * npad;
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
     for (int k = SI_B * max(blockIdx.x, blockIdx.y); k < npad; k += 16) {
         const int kq = k + 4 * q;
-        float4 a4 = *reinterpret_cast<const float4 *>(a + kq);
+        const float4 a4 = *reinterpret_cast<const float4 *>(a + kq);
         const float4 b4 = *reinterpret_cast<const float4 *>(b + kq);
-        float s[4];
+        float s[4];  // D⁻² by the hardware reciprocal (X preconditions a float64 refinement)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const float d = kq + t < n ? dg[kq + t] : 1.0f;
-            s[t] = kq + t < n ? 1.0f / (d * d) : 0.0f;
+            s[t] = kq + t < n ? __builtin_amdgcn_rcpf(d * d) : 0.0f;
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x * s[0], b4.x, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y * s[1], b4.y, acc, 0, 0, 0);
