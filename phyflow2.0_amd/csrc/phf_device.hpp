@@ -136,6 +136,18 @@ __device__ __forceinline__ float pair_f32_far(float dx, float dy, float w, const
 // relative at |r| = ln2/2); expm1(x) = 2^k expm1(r) + (2^k - 1), whose
 // second term is exact for k >= -53.  x is clamped at -40, where expm1
 // rounds to -1.
+// a b + c as the three-operand v_fma_f64.  With c a constant held in a
+// register (a Horner coefficient hoisted out of a loop) the compiler
+// otherwise copies c into the destination and uses the two-address
+// v_fmac_f64: one extra move per Horner step.  The same operation, so the
+// same result.
+__device__ __forceinline__ double fma3_f64(double a, double b, double c)
+{
+    double d;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
 __device__ __forceinline__ double expm1_neg_f64(double x)
 {
     x = __builtin_fmax(x, -40.0);
@@ -144,18 +156,18 @@ __device__ __forceinline__ double expm1_neg_f64(double x)
     r = __builtin_fma(-k, -0x1.718432a1b0e26p-35, r);
     // q = sum_{j=0..12} r^j / (j+1)!
     double q = 1.6059043836821613e-10;
-    q = __builtin_fma(q, r, 2.08767569878681e-09);
-    q = __builtin_fma(q, r, 2.505210838544172e-08);
-    q = __builtin_fma(q, r, 2.7557319223985888e-07);
-    q = __builtin_fma(q, r, 2.7557319223985893e-06);
-    q = __builtin_fma(q, r, 2.4801587301587302e-05);
-    q = __builtin_fma(q, r, 1.9841269841269841e-04);
-    q = __builtin_fma(q, r, 1.3888888888888889e-03);
-    q = __builtin_fma(q, r, 8.3333333333333332e-03);
-    q = __builtin_fma(q, r, 4.1666666666666664e-02);
-    q = __builtin_fma(q, r, 1.6666666666666666e-01);
-    q = __builtin_fma(q, r, 0.5);
-    q = __builtin_fma(q, r, 1.0);
+    q = fma3_f64(q, r, 2.08767569878681e-09);
+    q = fma3_f64(q, r, 2.505210838544172e-08);
+    q = fma3_f64(q, r, 2.7557319223985888e-07);
+    q = fma3_f64(q, r, 2.7557319223985893e-06);
+    q = fma3_f64(q, r, 2.4801587301587302e-05);
+    q = fma3_f64(q, r, 1.9841269841269841e-04);
+    q = fma3_f64(q, r, 1.3888888888888889e-03);
+    q = fma3_f64(q, r, 8.3333333333333332e-03);
+    q = fma3_f64(q, r, 4.1666666666666664e-02);
+    q = fma3_f64(q, r, 1.6666666666666666e-01);
+    q = fma3_f64(q, r, 0.5);
+    q = fma3_f64(q, r, 1.0);
     const double t = __builtin_amdgcn_ldexp(1.0, (int)k);
     return __builtin_fma(t, r * q, t - 1.0);
 }

@@ -354,13 +354,17 @@ __device__ __forceinline__ void p2p_sweep_f64(const double4 *__restrict__ st, in
             double P;
             if (KIND == PAIR_OSEEN || KIND == PAIR_OSEEN_NOCUT) {
                 // pair_f64's Oseen form; the smoothing factor is skipped
-                // (1 exactly) when no lane of the wave is inside the cutoff
-                const double r2 = __builtin_fma(dx, dx, dy * dy);
+                // (1 exactly) when no lane of the wave is inside the cutoff.
+                // The 1e-200 in r2 changes no pair at a distance (it is far
+                // below half an ulp of any r2 above 1e-184) and makes a
+                // blob's pair with itself finite, so its P dx is 0 as
+                // pair_f64's r2 > 0 test makes it, without that compare and
+                // two selects per pair.
+                const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, 1e-200));
                 double om = 1.0;
                 if (__ballot(r2 < pp.c2))
                     om = __builtin_fmin(-expm1_neg_f64(-pp.inv_ks * r2) * pp.scale, 1.0);
-                const double Pv = (q.z * rcp_f64(r2)) * om;
-                P = r2 > 0.0 ? Pv : 0.0;
+                P = (q.z * rcp_f64(r2)) * om;
             } else {
                 P = pair_f64<KIND>(dx, dy, q.z, pp);
             }
