@@ -338,15 +338,17 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
 // ---------------------------------------------------------------- float64
 constexpr int P2P_STAGE_F64 = 768;  // staged sources (double4) per batch
 
-// float64 pair sweep over staged entries x in [a, b) with x = o (mod 16)
+// float64 pair sweep over staged entries x in [a, b) with x = o (mod S)
 // for NT targets at once (each staged record read once for all of them),
-// absolute coordinates, pair_f64's operation order.
-template <int KIND, int NT>
+// absolute coordinates, pair_f64's operation order.  (Reading the next
+// entry ahead of the current pairs was measured slower: 1.62 against
+// 1.55 ms at 1M blobs.)
+template <int KIND, int NT, int S = 16>
 __device__ __forceinline__ void p2p_sweep_f64(const double4 *__restrict__ st, int a, int b, int o,
                                               const double *xt, const double *yt,
                                               const PairParams &pp, double *ax, double *ay)
 {
-    for (int x = a + ((o - a) & 15); x < b; x += 16) {
+    for (int x = a + ((o - a) & (S - 1)); x < b; x += S) {
         const double4 q = st[x];
 #pragma unroll
         for (int u = 0; u < NT; ++u) {
@@ -389,6 +391,10 @@ __device__ __forceinline__ void p2p_leaf_f64(const double4 *__restrict__ src,
     auto xf = [](double4 r, float, float) { return r; };
     for (int32_t rc = t0; rc < t1; rc += 16 * P2P_CHUNKS) {
         const int nch = min(P2P_CHUNKS, (int)((t1 - rc + 15) >> 4));
+        // a last chunk of 1 .. 8 targets: two lanes per target (as p2p_leaf)
+        const int rem = (int)min(t1 - rc, (int32_t)(16 * P2P_CHUNKS)) & 15;
+        const bool half = rem >= 1 && rem <= 8;
+        const int npair = half ? nch - 1 : nch;
         if (g == 0)
             for (int c = 0; c < nch; ++c) acc[(c * 4 + wv) * 16 + t] = make_double2(0.0, 0.0);
         int q = 0, jq = 0;
@@ -398,7 +404,7 @@ __device__ __forceinline__ void p2p_leaf_f64(const double4 *__restrict__ src,
                                                              core_end);
             __syncthreads();
             // chunks two at a time (each staged record read once for both)
-            for (int c = 0; c < nch; c += 2) {
+            for (int c = 0; c < npair; c += 2) {
                 double xt[2], yt[2], ax[2] = {}, ay[2] = {};
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
@@ -407,7 +413,7 @@ __device__ __forceinline__ void p2p_leaf_f64(const double4 *__restrict__ src,
                     xt[u] = tp.x;
                     yt[u] = tp.y;
                 }
-                const int nu = min(2, nch - c);
+                const int nu = min(2, npair - c);
                 if (nu == 2)
                     p2p_sweep_f64<KIND, 2>(st, 0, used, o, xt, yt, pp, ax, ay);
                 else
@@ -421,6 +427,23 @@ __device__ __forceinline__ void p2p_leaf_f64(const double4 *__restrict__ src,
                         double2 &A = acc[((c + u) * 4 + wv) * 16 + t];
                         A = make_double2(A.x + ax[u], A.y + ay[u]);
                     }
+                }
+            }
+            if (half) {
+                const int c = nch - 1, th = t & 7;
+                const int32_t i = rc + 16 * c + th;
+                const double2 tp = i < t1 ? tgt(i) : make_double2(0.0, 0.0);
+                double xt[1] = {tp.x}, yt[1] = {tp.y}, ax[1] = {}, ay[1] = {};
+                p2p_sweep_f64<KIND, 1, 32>(st, 0, used, o + 16 * (t >> 3), xt, yt, pp, ax, ay);
+                ax[0] += __shfl_xor(ax[0], 8);
+                ay[0] += __shfl_xor(ay[0], 8);
+                ax[0] += __shfl_xor(ax[0], 16);
+                ay[0] += __shfl_xor(ay[0], 16);
+                ax[0] += __shfl_xor(ax[0], 32);
+                ay[0] += __shfl_xor(ay[0], 32);
+                if (g == 0 && t < 8) {
+                    double2 &A = acc[(c * 4 + wv) * 16 + t];
+                    A = make_double2(A.x + ax[0], A.y + ay[0]);
                 }
             }
             __syncthreads();
