@@ -678,7 +678,7 @@ __device__ __forceinline__ void batch_p2p_leaf(
         vx[o] = -qi / two_pi;  // _induced.py:334-336
         vy[o] = -qr / two_pi;
     };
-    p2p_leaf<KIND>(src, sp, st, acc, nnear, nf, t0, t1, tpos, pp, lam, out);
+    p2p_leaf<KIND, false>(src, sp, st, acc, nnear, nf, t0, t1, tpos, pp, lam, out);
 }
 
 // One workgroup of 4 waves per leaf; workgroups are dealt round-robin over

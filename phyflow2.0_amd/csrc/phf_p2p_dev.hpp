@@ -230,14 +230,15 @@ constexpr int P2P_CHUNKS = 8;  // target chunks of 16 per round (128 targets)
 // staged batch of near sources is swept by all four waves for every chunk,
 // lane 16 g + t of wave w taking target t of the chunk and the staged
 // entries = 4 w + g (mod 16), so the waves carry equal work whatever the
-// leaf population (a last chunk of at most 8 targets: lane 16 g + t takes
-// target t mod 8 and the entries = 4 w + g + 16 (t / 8) (mod 32)).  Per chunk the four groups of a wave are combined by a
+// leaf population (HALF: a last chunk of at most 8 targets has lane 16 g + t
+// take target t mod 8 and the entries = 4 w + g + 16 (t / 8) (mod 32); off
+// in the batched kernel, where it measured slower).  Per chunk the four groups of a wave are combined by a
 // butterfly and the wave's partial is merged into its own LDS slot
 // acc[(c * 4 + w) * 16 + t]; at the end of a round one thread per target
 // merges the four waves' slots in wave order and calls
 // out(i, hi_a, lo_a, hi_b, lo_b): the sums are lam (hi + lo).  Every thread
 // of the workgroup must call this (it synchronises).
-template <int KIND, class NearFn, class TgtFn, class OutFn>
+template <int KIND, bool HALF = true, class NearFn, class TgtFn, class OutFn>
 __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
                                          const int32_t *__restrict__ sp, float4 *st, float4 *acc,
                                          int nnear, const NearFn &nf, int32_t t0, int32_t t1,
@@ -253,7 +254,7 @@ __device__ __forceinline__ void p2p_leaf(const float4 *__restrict__ src,
         // target (sources x = o + 16 (t >> 3) mod 32), so half its lanes
         // are not spent on absent targets
         const int rem = (int)min(t1 - rc, (int32_t)(16 * P2P_CHUNKS)) & 15;
-        const bool half = rem >= 1 && rem <= 8;
+        const bool half = HALF && rem >= 1 && rem <= 8;
         const int npair = half ? nch - 1 : nch;
         if (g == 0)
             for (int c = 0; c < nch; ++c)
