@@ -369,6 +369,17 @@ typedef struct {
 int phf_panels_update(phf_panels *p, const double *xs, const double *ys, const double *xe,
                       const double *ye, phf_panels_timing *timing);
 
+/* The same rebuild, enqueued on the body's stream without waiting for it
+ * (no counterpart in the reference, whose updateBody returns with the
+ * factors done).  The arrays are copied before the call returns.  The next
+ * call on the body completes the rebuild first and reports its errors (a
+ * singular matrix): phf_panels_solve, phf_panels_info, phf_panels_update,
+ * phf_panels_update_async, phf_system_step_rk2.  In the coupled step only
+ * the panel solve waits for the factors, so the first blob velocity
+ * evaluation runs beside the factorisation. */
+int phf_panels_update_async(phf_panels *p, const double *xs, const double *ys, const double *xe,
+                            const double *ye);
+
 typedef struct {
     double dt;
     double vinf_x, vinf_y;   /* free stream */

@@ -167,6 +167,7 @@ _SIGS = {
     "phf_panels_info": ([_ptr, ctypes.POINTER(_i64), _ptr, _ptr, _ptr], ctypes.c_int),
     "phf_panels_update": ([_ptr, _ptr, _ptr, _ptr, _ptr, ctypes.POINTER(PanelsTiming)],
                           ctypes.c_int),
+    "phf_panels_update_async": ([_ptr, _ptr, _ptr, _ptr, _ptr], ctypes.c_int),
     "phf_panels_solve": ([_ptr, _ptr, _ptr, _ptr, ctypes.POINTER(_i32)], ctypes.c_int),
     "phf_system_step_rk2": ([_ptr, ctypes.POINTER(VelParams), _ptr, ctypes.POINTER(StepParams),
                              ctypes.POINTER(StepInfo)], ctypes.c_int),

@@ -1594,8 +1594,8 @@ static int dense_inverse(Lane *L, DenseFactors *f)
     return PHF_OK;
 }
 
-int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
-                        hipEvent_t *marks)
+int dense_factor_enqueue(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
+                         hipEvent_t *marks)
 {
     const bool normal = rows > n;
     if (!(f->LU && f->rows == rows && f->n == n)) {
@@ -1644,10 +1644,13 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
     if (marks) PHF_HIP(hipEventRecord(marks[2], L->stream));
     f->Br = B;
     f->cholesky = chol;
-    int32_t hinfo = 0;
-    PHF_HIP(hipMemcpyAsync(&hinfo, f->info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
-    PHF_HIP(hipStreamSynchronize(L->stream));
-    if (chol && hinfo != 0) {
+    return PHF_OK;
+}
+
+int dense_factor_finish(Lane *L, DenseFactors *f, int32_t hinfo)
+{
+    const int64_t n = f->n;
+    if (f->cholesky && hinfo != 0) {
         // not positive definite in float32: the pivoted LU of the same matrix
         to_f32<<<div_up(n * n, 256), 256, 0, L->stream>>>(f->N64, n * n, f->LU);
         PHF_TRY(lu_factor_f32(L, f->LU, (int)n, f->ipiv, f->info, f->lu_work));
@@ -1663,6 +1666,16 @@ int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, Dense
         return PHF_ERR_INVALID;
     }
     return PHF_OK;
+}
+
+int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
+                        hipEvent_t *marks)
+{
+    PHF_TRY(dense_factor_enqueue(L, B, rows, n, f, marks));
+    int32_t hinfo = 0;
+    PHF_HIP(hipMemcpyAsync(&hinfo, f->info, sizeof(int32_t), hipMemcpyDeviceToHost, L->stream));
+    PHF_HIP(hipStreamSynchronize(L->stream));
+    return dense_factor_finish(L, f, hinfo);
 }
 
 int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double *x,

@@ -50,9 +50,16 @@ struct DenseFactors {
 
 // Factors of B; buffers already held by f for the same shape are reused.
 // marks (optional, 3 events): recorded after B^T B, after the LU and after
-// the inverse.
+// the inverse.  One host synchronisation (the factorisation's info).
 int dense_factor_device(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
                         hipEvent_t *marks = nullptr);
+// The same in two parts: enqueue everything on L's stream (no
+// synchronisation; f->info on the device holds the result), then, once that
+// work is complete and hinfo holds f->info, finish: the pivoted LU if the
+// Cholesky met a non-positive pivot, and the error for a singular matrix.
+int dense_factor_enqueue(Lane *L, const double *B, int64_t rows, int64_t n, DenseFactors *f,
+                         hipEvent_t *marks = nullptr);
+int dense_factor_finish(Lane *L, DenseFactors *f, int32_t hinfo);
 // Refinement sweeps from x = 0, or from the x given (warm); sweeps go in
 // bursts between host checks, the first of first_burst sweeps.
 int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double *x,

@@ -40,7 +40,18 @@ struct Panels {
     std::vector<double> hxcp, hycp;
     int last_iterations = 0;
     bool have_solution = false;  // s holds a converged solution
+    // A rebuild enqueued on the body's stream and not yet completed
+    // (phf_panels_update_async): ev_geom follows the copies of the new
+    // geometry, ev_done the whole rebuild, after which info_host holds the
+    // factorisation's info.  geom_pin stages the geometry (9 m doubles) so
+    // the copies do not read the caller's arrays after the call returns.
+    bool pending = false;
+    hipEvent_t ev_geom = nullptr, ev_done = nullptr;
+    double *geom_pin = nullptr;
+    int32_t *info_host = nullptr;
 };
+
+static int panels_rebuild_finish(Panels *P, phf_panels_timing *timing = nullptr);
 
 // ------------------------------------------------------------- kernels ----
 // RHS of the no-slip condition, -(v . t) (__panelsClass.py:771); for the
@@ -170,6 +181,10 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
     PHF_ALLOC(vy, L->arena, double, (size_t)nt);
     PHF_ALLOC(px, L->arena, double, (size_t)std::max<int64_t>(n, 1));
     PHF_ALLOC(py, L->arena, double, (size_t)std::max<int64_t>(n, 1));
+    // a rebuild still in flight on the body's stream: the collocation points
+    // are needed now, the factors only for the solve below, so the blob
+    // velocity evaluation runs beside the factorisation
+    if (P->pending) PHF_HIP(hipStreamWaitEvent(L->stream, P->ev_geom, 0));
     concat_points<<<div_up(nt, 256), 256, 0, L->stream>>>(X, Y, w, n, P->xcp, P->ycp, m, tx, ty,
                                                           tw);
     PHF_HIP(hipGetLastError());
@@ -177,6 +192,7 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
     // collocation points join the sources with zero circulation (exact
     // zeros in every sum), so targets and sources share one tree
     PHF_TRY(velocity_device(L, tx, ty, tw, nt, tx, ty, nt, true, vp, vx, vy, tm, used, &fi));
+    PHF_TRY(panels_rebuild_finish(P));
     if (solve_panels) {
         tm.begin(PHF_STAGE_PANEL);
         used[PHF_STAGE_PANEL] = true;
@@ -245,34 +261,61 @@ __global__ void zero_doubles(double *__restrict__ p, int64_t n)
 // New geometry into the body's device buffers, then its influence matrix
 // (with the Kelvin row for 'prescribed'), the float32 factors and the
 // grouped-velocity factors, all rebuilt in place (__panelsClass.py:
-// updateBody :530-638 -> __assembleInfluenceMatrix :872-961).  timing
-// (optional): device times of the stages.
-static int panels_rebuild(Panels *P, const PanelGeomHost &g, phf_panels_timing *timing)
+// updateBody :530-638 -> __assembleInfluenceMatrix :872-961), enqueued on
+// the body's stream without a host synchronisation.  Any earlier rebuild
+// must be finished (the staging buffer and the events are reused).
+static int panels_rebuild_enqueue(Panels *P, const PanelGeomHost &g)
 {
     Lane *L = &P->lane;
     const int64_t m = P->m;
     const double *host[9] = {g.xs, g.ys, g.xe, g.ye, g.cs.data(), g.sn.data(), g.Ln.data(),
                              g.xc.data(), g.yc.data()};
     double *dev[9] = {P->xs, P->ys, P->xe, P->ye, P->cosA, P->sinA, P->L, P->xcp, P->ycp};
-    for (int k = 0; k < 9; ++k)
-        PHF_HIP(hipMemcpyAsync(dev[k], host[k], sizeof(double) * m, hipMemcpyHostToDevice,
+    int rc = [&]() -> int {
+        for (int k = 0; k < 9; ++k) {
+            std::memcpy(P->geom_pin + k * m, host[k], sizeof(double) * m);
+            PHF_HIP(hipMemcpyAsync(dev[k], P->geom_pin + k * m, sizeof(double) * m,
+                                   hipMemcpyHostToDevice, L->stream));
+        }
+        PHF_HIP(hipEventRecord(P->ev_geom, L->stream));
+        P->hxcp = g.xc;
+        P->hycp = g.yc;
+        hipEvent_t *ev = L->ev;  // [0] start, [1] influence, [2..4] factor marks
+        PHF_HIP(hipEventRecord(ev[0], L->stream));
+        PHF_TRY(panel_influence_device(L, m, P->xcp, P->ycp, P->xs, P->ys, P->xe, P->ye,
+                                       P->cosA, P->sinA, P->A));
+        if (P->problem == PHF_PANELS_PRESCRIBED)  // Kelvin row: sum_j L_j s_j = -gTotal (:509-518)
+            PHF_HIP(hipMemcpyAsync(P->A + m * m, P->L, sizeof(double) * m,
+                                   hipMemcpyDeviceToDevice, L->stream));
+        PHF_HIP(hipEventRecord(ev[1], L->stream));
+        const int64_t rows = P->problem == PHF_PANELS_PRESCRIBED ? m + 1 : m;
+        PHF_TRY(dense_factor_enqueue(L, P->A, rows, m, &P->f, ev + 2));
+        PHF_TRY(panel_series_geom_device(L, m, P->xs, P->ys, P->xe, P->ye, P->cosA, P->sinA,
+                                         &P->sg));
+        PHF_HIP(hipEventRecord(ev[5], L->stream));
+        PHF_HIP(hipMemcpyAsync(P->info_host, P->f.info, sizeof(int32_t), hipMemcpyDeviceToHost,
                                L->stream));
-    P->hxcp = g.xc;
-    P->hycp = g.yc;
-    hipEvent_t *ev = L->ev;  // [0] start, [1] influence, [2..4] factor marks
-    PHF_HIP(hipEventRecord(ev[0], L->stream));
-    PHF_TRY(panel_influence_device(L, m, P->xcp, P->ycp, P->xs, P->ys, P->xe, P->ye, P->cosA,
-                                   P->sinA, P->A));
-    if (P->problem == PHF_PANELS_PRESCRIBED)  // Kelvin row: sum_j L_j s_j = -gTotal (:509-518)
-        PHF_HIP(hipMemcpyAsync(P->A + m * m, P->L, sizeof(double) * m, hipMemcpyDeviceToDevice,
-                               L->stream));
-    PHF_HIP(hipEventRecord(ev[1], L->stream));
-    const int64_t rows = P->problem == PHF_PANELS_PRESCRIBED ? m + 1 : m;
-    PHF_TRY(dense_factor_device(L, P->A, rows, m, &P->f, ev + 2));
-    PHF_TRY(panel_series_geom_device(L, m, P->xs, P->ys, P->xe, P->ye, P->cosA, P->sinA,
-                                     &P->sg));
-    PHF_HIP(hipEventRecord(ev[5], L->stream));
-    PHF_HIP(hipStreamSynchronize(L->stream));
+        PHF_HIP(hipEventRecord(P->ev_done, L->stream));
+        return PHF_OK;
+    }();
+    if (rc != PHF_OK) {
+        (void)hipStreamSynchronize(L->stream);  // nothing left in flight
+        return rc;
+    }
+    P->pending = true;
+    return PHF_OK;
+}
+
+// Complete an enqueued rebuild: wait for it, take the pivoted LU if the
+// Cholesky met a non-positive pivot, and report a singular matrix.  timing
+// (optional): device times of the stages.
+static int panels_rebuild_finish(Panels *P, phf_panels_timing *timing)
+{
+    if (!P->pending) return PHF_OK;
+    P->pending = false;
+    Lane *L = &P->lane;
+    hipEvent_t *ev = L->ev;
+    PHF_HIP(hipEventSynchronize(P->ev_done));
     if (timing) {
         float a = 0, b = 0, c = 0, d = 0, e = 0;
         (void)hipEventElapsedTime(&a, ev[0], ev[1]);
@@ -286,7 +329,15 @@ static int panels_rebuild(Panels *P, const PanelGeomHost &g, phf_panels_timing *
         timing->inverse_ms = d;
         timing->total_ms = e;
     }
-    return PHF_OK;
+    return dense_factor_finish(L, &P->f, *P->info_host);
+}
+
+// The rebuild for geometry g, complete when this returns.
+static int panels_rebuild(Panels *P, const PanelGeomHost &g, phf_panels_timing *timing)
+{
+    PHF_TRY(panels_rebuild_finish(P));  // an earlier one still in flight
+    PHF_TRY(panels_rebuild_enqueue(P, g));
+    return panels_rebuild_finish(P, timing);
 }
 
 }  // namespace phf
@@ -317,9 +368,15 @@ int phf_panels_create(const double *xs, const double *ys, const double *xe, cons
     hipError_t e = hipStreamCreateWithFlags(&P->lane.stream, hipStreamNonBlocking);
     for (int k = 0; e == hipSuccess && k < 2 * PHF_STAGE_COUNT; ++k)
         e = hipEventCreate(&P->lane.ev[k]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&P->ev_geom, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&P->ev_done, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&P->geom_pin), sizeof(double) * 9 * m);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&P->info_host), sizeof(int32_t));
     if (e != hipSuccess) {
-        set_error("stream/event creation failed: %s", hipGetErrorName(e));
-        delete P;
+        set_error("stream/event/pinned buffer creation failed: %s", hipGetErrorName(e));
+        phf_panels_destroy(P);
         return PHF_ERR_DEVICE;
     }
     int rc = [&]() -> int {
@@ -364,6 +421,21 @@ int phf_panels_update(phf_panels *P, const double *xs, const double *ys, const d
     return PHF_OK;
 }
 
+int phf_panels_update_async(phf_panels *P, const double *xs, const double *ys, const double *xe,
+                            const double *ye)
+{
+    clear_error();
+    PHF_REQUIRE(P != nullptr, "panels is NULL");
+    PanelGeomHost g;
+    PHF_TRY(panel_geometry_host(xs, ys, xe, ye, P->m, &g));
+    std::lock_guard<std::mutex> lk(P->mu);
+    PHF_HIP(hipSetDevice(Context::get().device()));
+    PHF_TRY(panels_rebuild_finish(P));  // the previous update's errors surface here
+    PHF_TRY(panels_rebuild_enqueue(P, g));
+    P->have_solution = false;
+    return PHF_OK;
+}
+
 int phf_panels_destroy(phf_panels *P)
 {
     clear_error();
@@ -376,6 +448,10 @@ int phf_panels_destroy(phf_panels *P)
         P->lane.pin.release();
         for (auto &ev : P->lane.ev)
             if (ev) (void)hipEventDestroy(ev);
+        if (P->ev_geom) (void)hipEventDestroy(P->ev_geom);
+        if (P->ev_done) (void)hipEventDestroy(P->ev_done);
+        if (P->geom_pin) (void)hipHostFree(P->geom_pin);
+        if (P->info_host) (void)hipHostFree(P->info_host);
         if (P->lane.stream) (void)hipStreamDestroy(P->lane.stream);
     }
     delete static_cast<phf_panels_s *>(P);
@@ -387,6 +463,7 @@ int phf_panels_info(phf_panels *P, int64_t *m, double *xcp, double *ycp, double 
     clear_error();
     PHF_REQUIRE(P != nullptr, "panels is NULL");
     std::lock_guard<std::mutex> lk(P->mu);
+    PHF_TRY(panels_rebuild_finish(P));
     if (m) *m = P->m;
     if (xcp) std::memcpy(xcp, P->hxcp.data(), sizeof(double) * P->m);
     if (ycp) std::memcpy(ycp, P->hycp.data(), sizeof(double) * P->m);
@@ -407,6 +484,7 @@ int phf_panels_solve(phf_panels *P, const double *vx, const double *vy, double *
     PHF_TRY(check_host_array(vy, P->m, "vyExternal"));
     PHF_REQUIRE(strength != nullptr, "strength is NULL");
     std::lock_guard<std::mutex> lk(P->mu);
+    PHF_TRY(panels_rebuild_finish(P));
     LaneGuard g;
     PHF_TRY(g.rc);
     Lane *L = g.lane;

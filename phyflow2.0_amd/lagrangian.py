@@ -56,27 +56,36 @@ class PanelBody:
         self._local = (xs, ys, xe, ye)
         self.last_update = None
 
-    def update_geometry(self, xPanel=None, yPanel=None, *, xs=None, ys=None, xe=None, ye=None):
+    def update_geometry(self, xPanel=None, yPanel=None, *, xs=None, ys=None, xe=None, ye=None,
+                        wait=True):
         """New global start/end points of the same panels: geometry,
         collocation points, influence matrix and factors are rebuilt.
-        Returns (and keeps in last_update) the device times of the rebuild."""
+        Returns (and keeps in last_update) the device times of the rebuild.
+        wait=False: the rebuild is only enqueued and None is returned; the
+        next use of the body (solve, step_rk2, collocation points, A, another
+        update) completes it and raises its errors, and a coupled step
+        evaluates the blob velocity while the factors are formed."""
         xs, ys, xe, ye = _ends(xPanel, yPanel, xs, ys, xe, ye)
         if xs.size != self.m:
             raise ValueError("the body has %d panels, got %d" % (self.m, xs.size))
+        if not wait:
+            call("phf_panels_update_async", self._h, ptr(xs), ptr(ys), ptr(xe), ptr(ye))
+            self.last_update = None
+            return None
         t = PanelsTiming()
         call("phf_panels_update", self._h, ptr(xs), ptr(ys), ptr(xe), ptr(ye), ctypes.byref(t))
         self.last_update = t.as_dict()
         return self.last_update
 
-    def updateBody(self, cmGlobal, thetaLocal):
+    def updateBody(self, cmGlobal, thetaLocal, wait=True):
         """Panels.updateBody for this body: rotate the local coordinates by
         thetaLocal (anti-clockwise) about the local origin, move them to
-        cmGlobal, and rebuild."""
+        cmGlobal, and rebuild (wait as in update_geometry)."""
         c, s = np.cos(thetaLocal), np.sin(thetaLocal)
         cx, cy = float(cmGlobal[0]), float(cmGlobal[1])
         xs, ys, xe, ye = self._local
         return self.update_geometry(xs=c * xs - s * ys + cx, ys=s * xs + c * ys + cy,
-                                    xe=c * xe - s * ye + cx, ye=s * xe + c * ye + cy)
+                                    xe=c * xe - s * ye + cx, ye=s * xe + c * ye + cy, wait=wait)
 
     def close(self):
         if getattr(self, "_h", None):

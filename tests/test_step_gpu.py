@@ -123,3 +123,42 @@ def test_remesh_conserves_invariants():
     assert abs(info["impulse_x_after"] - info["impulse_x_before"]) <= 1e-13 * scale
     assert abs(info["impulse_y_after"] - info["impulse_y_before"]) <= 1e-13 * scale
     assert info["n_after"] > 0
+
+
+def test_async_body_update_matches_sync():
+    """phf_panels_update_async enqueues the rebuild and the step waits only
+    where it needs the new body (collocation points on the device, the
+    factors before the panel solve), so the blob velocity evaluation runs
+    beside the factorisation: the steps must come out bit for bit as after
+    the synchronous update, and solve() and the collocation points after an
+    asynchronous update must see the new body."""
+    xp, yp = W.cylinder_panels(128, R=0.5)
+    a = W.annulus_blobs(20000, sigma=0.01, seed=5)
+    p = vel_params(0.01, prec=1, order=16, leaf=64)
+    rem = dict(sigma=0.01, overlap=1.25)
+    out = []
+    for wait in (True, False):
+        body = PanelBody(xp, yp, "prescribed", 0.0)
+        try:
+            with BlobSystem(a["x"], a["y"], a["w"]) as s:
+                for k in range(3):
+                    th = 0.05 * (k + 1)
+                    c, sn = math.cos(th), math.sin(th)
+                    body.update_geometry(c * xp - sn * yp + 0.01 * k, sn * xp + c * yp, wait=wait)
+                    info = step_rk2(s, p, 1e-3, vInf=(1.0, 0.0), panels=body, remesh=rem)
+                out.append((s.get(), info["solve_iterations"], info["circ_after"]))
+            # a solve and the collocation points right after an async update
+            body.update_geometry(xp + 0.2, yp, wait=wait)
+            xc, yc = body.collocation_points()
+            out.append((xc, yc, body.solve(np.ones(body.m), np.zeros(body.m))))
+        finally:
+            body.close()
+    (g_sync, it_sync, c_sync), pts_sync, (g_async, it_async, c_async), pts_async = out
+    for u, v in zip(g_sync, g_async):
+        assert np.array_equal(u, v)
+    assert it_sync == it_async and c_sync == c_async
+    for u, v in zip(pts_sync, pts_async):
+        assert np.array_equal(u, v)
+    # the collocation points are those of the moved body
+    xr, yr = W.cylinder_panels(128, R=0.5)
+    assert abs(float(np.mean(pts_async[0])) - (float(np.mean(0.5 * (xr[:-1] + xr[1:]))) + 0.2)) < 0.05
