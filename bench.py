@@ -235,10 +235,21 @@ def bench_cylinder(args):
     t = time.perf_counter()
     body = PanelBody(xp, yp, "prescribed", 0.0)
     t_build = time.perf_counter() - t
-    p = vel_params(0.004, prec=gauss.PREC_F32C, order=16, leaf=64)
+    p_fixed = vel_params(0.004, prec=gauss.PREC_F32C, order=16, leaf=64)
+    # the FMM settings measured for this blob system by the autotuner (the
+    # fastest leaf size, leaf side, near-set width and order whose error
+    # against the float64 direct sum on 1,024 seeded targets is <= 5e-7)
+    t = time.perf_counter()
+    tune = autotune.tune_system(dict(x=a["x"], y=a["y"], w=a["w"], sigma=0.004), "f32c", 1e-6)
+    t_tune = time.perf_counter() - t
+    ch = tune["choice"]
+    p = p_fixed if ch is None else vel_params(0.004, prec=gauss.PREC_F32C, order=ch["order"],
+                                              leaf=ch["leaf"], ws=ch["ws"],
+                                              leaf_side=ch["leaf_side"])
     remesh = dict(sigma=0.004, overlap=1.25)
 
-    def run(rebuild, wait=False):
+    def run(rebuild, wait=False, params=None):
+        params = p if params is None else params
         times, infos = [], []
         # one resident simulation stepped repeatedly (the first steps warm
         # the system's device workspace); every step is a full coupled RK2
@@ -252,7 +263,8 @@ def bench_cylinder(args):
                 t = time.perf_counter()
                 if rebuild:
                     body.update_geometry(xp, yp, wait=wait)
-                info = step_rk2(sysm, p, 0.001, vInf=(1.0, 0.0), panels=body, remesh=remesh)
+                info = step_rk2(sysm, params, 0.001, vInf=(1.0, 0.0), panels=body,
+                                remesh=remesh)
                 dt = time.perf_counter() - t
                 if k >= args.warmup:
                     times.append(dt)
@@ -262,6 +274,7 @@ def bench_cylinder(args):
     times, infos = run(False)
     times_rb, infos_rb = run(True)
     times_rbw, _ = run(True, wait=True)
+    times_fix, _ = run(True, params=p_fixed)
     # the rebuild's stage times (device events), from waited updates
     rb = [body.update_geometry(xp, yp) for _ in range(max(3, args.steps))]
     info = infos[-1]
@@ -286,6 +299,9 @@ def bench_cylinder(args):
                        "ms_per_step_factors_kept": 1e3 * float(np.median(times)),
                        "p99_ms_factors_kept": 1e3 * float(np.percentile(times, 99)),
                        "ms_per_step_update_waited": 1e3 * float(np.median(times_rbw)),
+                       "fmm_params": {"tuned": ch, "tune_seconds": t_tune,
+                                      "ms_per_step_fixed_p16_leaf64":
+                                          1e3 * float(np.median(times_fix))},
                        "update": "enqueued (phf_panels_update_async); the step waits for the "
                                  "factors only at its panel solve",
                        "panel_rebuild_ms": {k: float(np.median([r[k] for r in rb]))
