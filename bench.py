@@ -360,8 +360,8 @@ def mix_leg(args, count=256, steps=10):
     systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
     total = int(np.sum(sizes))
     errs = []
-    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C, order=16,
-                      leaf=64) as b:
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C,
+                      order=batch_order(systems), leaf=64) as b:
         for i, s in enumerate(systems):
             b.set_system(i, s["x"], s["y"], s["w"])
         b.set_remesh(0.8, dt=1e-3)
@@ -413,6 +413,7 @@ def mix_leg(args, count=256, steps=10):
             r.close()
     return {"workload": "%d systems, N log-uniform [1024, 262144] seed 0, total %d blobs"
                         % (count, total),
+            "batched_order": batch_order(systems),
             "batched_f32c_velocity_evals_per_s": batched_v,
             "batched_f32c_step_evals_per_s": total / (wall_s / steps),
             "batched_steps_timed": steps,
@@ -440,8 +441,8 @@ def bench_mix(args):
     sizes = W.uneven_sizes(count, 1024, 262144, seed=0)
     systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
     total = int(np.sum(sizes))
-    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C, order=16,
-                      leaf=64) as b:
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C,
+                      order=batch_order(systems), leaf=64) as b:
         for i, s in enumerate(systems):
             b.set_system(i, s["x"], s["y"], s["w"])
         b.set_remesh(overlap, dt=dt)
@@ -509,6 +510,7 @@ def bench_mix(args):
             "data": "synthetic",
             "config": {"workload": "%d systems, N log-uniform [1024, 262144] seed 0, total %d blobs"
                                    % (count, total),
+                       "order": batch_order(systems),
                        "graph_device_ms": dev_ms, "remeshed_blobs": remeshed,
                        "spilled_systems": spilled,
                        "velocity_only_evals_per_s": total / (wall_v / args.steps),
@@ -521,6 +523,13 @@ def bench_mix(args):
                        "remesh_bitexact_vs_single_system": bool(remesh_exact)}}
 
 
+def batch_order(systems, tol=1e-6):
+    """One expansion order for a batch (the batched FMM takes one): the
+    largest the measured table gives any of its systems at this tolerance."""
+    return max(autotune.tuned_params(s["sigma"], s["x"].size, gauss.PREC_F32C, tol).order
+               for s in systems)
+
+
 def bench_batched(args):
     """The batched half of the 256-system mix alone (no float64 one-at-a-time
     legs): the same systems as bench_mix, velocity-only graph replays, then
@@ -531,8 +540,8 @@ def bench_batched(args):
     sizes = W.uneven_sizes(count, 1024, 262144, seed=0)
     systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
     total = int(np.sum(sizes))
-    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C, order=16,
-                      leaf=64) as b:
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=gauss.PREC_F32C,
+                      order=batch_order(systems), leaf=64) as b:
         for i, s in enumerate(systems):
             b.set_system(i, s["x"], s["y"], s["w"])
         for _ in range(args.warmup):
@@ -571,6 +580,7 @@ def bench_batched(args):
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "%d systems, N log-uniform [1024, 262144] seed 0, total %d blobs"
                                    % (count, total),
+                       "order": batch_order(systems),
                        "graph_device_ms": info["ms"], "velocity_ms": 1e3 * wall_v / args.steps,
                        "velocity_graph_device_ms": vinfo["ms"],
                        "velocity_evals_per_s": total / (wall_v / args.steps),
