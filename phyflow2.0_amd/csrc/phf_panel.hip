@@ -1235,8 +1235,24 @@ __device__ __forceinline__ floatx4 spd_tile(const float *__restrict__ a, const f
                                             int ka, int kb, int q)
 {
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 4
-    for (int k = ka; k < kb; k += 16) {
+    int k = ka;
+    // four 16-wide chunks' operands in flight before their products
+    for (; k + 48 < kb; k += 64) {
+        float4 a4[4], b4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a4[u] = *reinterpret_cast<const float4 *>(a + k + 16 * u + 4 * q);
+            b4[u] = *reinterpret_cast<const float4 *>(b + k + 16 * u + 4 * q);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[u].x, b4[u].x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[u].y, b4[u].y, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[u].z, b4[u].z, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[u].w, b4[u].w, acc, 0, 0, 0);
+        }
+    }
+    for (; k < kb; k += 16) {
         const float4 a4 = *reinterpret_cast<const float4 *>(a + k + 4 * q);
         const float4 b4 = *reinterpret_cast<const float4 *>(b + k + 4 * q);
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x, b4.x, acc, 0, 0, 0);
@@ -1294,21 +1310,38 @@ __global__ __launch_bounds__(256) void spd_inv_x(const float *__restrict__ Wc,
     const int i0 = SI_B * blockIdx.x + 16 * (wv & 1), j0 = SI_B * blockIdx.y + 16 * (wv >> 1);
     const float *a = Wc + (int64_t)(i0 + li) * npad, *b = Wc + (int64_t)(j0 + li) * npad;
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 4
-    for (int k = SI_B * max(blockIdx.x, blockIdx.y); k < npad; k += 16) {
+    // D⁻² by the hardware reciprocal (X preconditions a float64 refinement);
+    // two 16-wide chunks' operands in flight before their products
+    auto chunk = [&](int k, float4 &a4, float4 &b4, float s[4]) {
         const int kq = k + 4 * q;
-        const float4 a4 = *reinterpret_cast<const float4 *>(a + kq);
-        const float4 b4 = *reinterpret_cast<const float4 *>(b + kq);
-        float s[4];  // D⁻² by the hardware reciprocal (X preconditions a float64 refinement)
+        a4 = *reinterpret_cast<const float4 *>(a + kq);
+        b4 = *reinterpret_cast<const float4 *>(b + kq);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const float d = kq + t < n ? dg[kq + t] : 1.0f;
             s[t] = kq + t < n ? __builtin_amdgcn_rcpf(d * d) : 0.0f;
         }
+    };
+    auto products = [&](const float4 &a4, const float4 &b4, const float s[4]) {
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x * s[0], b4.x, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y * s[1], b4.y, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.z * s[2], b4.z, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w * s[3], b4.w, acc, 0, 0, 0);
+    };
+    int k = SI_B * max(blockIdx.x, blockIdx.y);
+    for (; k + 16 < npad; k += 32) {
+        float4 a0, b0, a1, b1;
+        float s0[4], s1[4];
+        chunk(k, a0, b0, s0);
+        chunk(k + 16, a1, b1, s1);
+        products(a0, b0, s0);
+        products(a1, b1, s1);
+    }
+    for (; k < npad; k += 16) {
+        float4 a0, b0;
+        float s0[4];
+        chunk(k, a0, b0, s0);
+        products(a0, b0, s0);
     }
     const int j = j0 + li, i = i0 + 4 * q;
     if (j >= n) return;
@@ -1748,10 +1781,14 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
 // < 1e-14 (float64 path).  Nearer groups are summed panel by panel.
 // Targets are evaluated in Morton order (16-bit keys over their bounding
 // box) so that a wave's targets take the same branch for almost every
-// group; each target's sum runs over the groups in index order.
+// group.  Each target has PI_LPT lanes: lane s sums the groups s, s +
+// PI_LPT, ... in index order and the lanes' sums are added by a fixed
+// butterfly (one lane per target left the SIMDs latency-bound: each
+// group's series is a serial chain, and the VALU issued 29% of the time).
 constexpr int PG_SIZE = 16;    // panels per group
 constexpr int PG_TERMS = 30;   // series terms kept (float64; float32 uses 16)
 constexpr int PG_CHUNK = 16;   // groups staged per round (256 panels)
+constexpr int PI_LPT = 4;      // lanes per target in the grouped evaluation
 
 struct PanelG {
     double zx, zy;     // centre
@@ -1984,8 +2021,10 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
     __shared__ PanelK sp[PG_CHUNK * PG_SIZE];
     __shared__ PanelG sg[PG_CHUNK];
     __shared__ CT sc[PG_CHUNK * NT];
-    const int tid = threadIdx.x;
-    const int64_t k = (int64_t)blockIdx.x * PI_T + tid;
+    // PI_LPT lanes per target, each taking every PI_LPT-th group of a round:
+    // the groups' chains are short and serial, so more of them in flight
+    const int tid = threadIdx.x, sub = tid & (PI_LPT - 1);
+    const int64_t k = (int64_t)blockIdx.x * (PI_T / PI_LPT) + tid / PI_LPT;
     const int64_t i = k < ne ? order[k] : 0;
     const double x = k < ne ? xE[i] : 0.0, y = k < ne ? yE[i] : 0.0;
     double sx = 0.0, sy = 0.0;
@@ -2005,7 +2044,7 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
                 sc[e] = c;
         }
         __syncthreads();
-        for (int gg = 0; gg < ng; ++gg) {
+        for (int gg = sub; gg < ng; gg += PI_LPT) {
             const PanelG G = sg[gg];
             const double dx = x - G.zx, dy = y - G.zy;
             if (dx * dx + dy * dy > G.far2) {
@@ -2068,7 +2107,13 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
             }
         }
     }
-    if (k < ne) {
+    // a target's lanes add their sums (a fixed butterfly)
+#pragma unroll
+    for (int o = 1; o < PI_LPT; o <<= 1) {
+        sx += __shfl_xor(sx, o);
+        sy += __shfl_xor(sy, o);
+    }
+    if (k < ne && sub == 0) {
         vx[i] = sx;
         vy[i] = sy;
     }
@@ -2125,10 +2170,10 @@ int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, co
     PHF_HIP(hipGetLastError());
     PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, ne, 16, &ks, &vs));
     if (prec == PHF_PREC_F32C)
-        panel_induced_grouped<true><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(
+        panel_induced_grouped<true><<<div_up(ne, PI_T / PI_LPT), PI_T, 0, L->stream>>>(
             m, ng, pk, grp, coef, ne, vs, xE, yE, vx, vy);
     else
-        panel_induced_grouped<false><<<div_up(ne, PI_T), PI_T, 0, L->stream>>>(
+        panel_induced_grouped<false><<<div_up(ne, PI_T / PI_LPT), PI_T, 0, L->stream>>>(
             m, ng, pk, grp, coef, ne, vs, xE, yE, vx, vy);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
