@@ -1500,6 +1500,11 @@ static int fmm_enqueue(Lane *L, const FmmArgs &a, const Domain &dom_in, StageTim
         PHF_TRY(build_sorted_points(L, dom, a.xb, a.yb, a.nb, &S));
     if (a.same) {
         Tg = S;
+        if (info) {
+            info->sorted_pay = S.pay;
+            info->sorted_perm = S.pay ? nullptr : S.perm;
+            info->sorted_n = a.nb;
+        }
     } else {
         // the source build has completed the bounding box at dom.org
         Domain dt = dom;
@@ -1820,6 +1825,9 @@ int fmm_device(Lane *L, const FmmArgs &a, StageTimer &tm, bool *used, FmmInfo *i
         }
         PHF_TRY(rc);
         used[PHF_STAGE_TREE_FAR] = used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
+        info->sorted_pay = nullptr;
+        info->sorted_perm = nullptr;
+        info->sorted_n = 0;
     } else {
         used[PHF_STAGE_TREE] = used[PHF_STAGE_P2P] = true;
         PHF_TRY(fmm_enqueue(L, a, dom, tm, used, info, nullptr, &nnear));

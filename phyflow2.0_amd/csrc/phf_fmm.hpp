@@ -32,6 +32,14 @@ struct FmmInfo {
     // no_speculation
     std::function<bool()> domain_ok;
     unsigned long long *d_counts = nullptr;  // device, arena-owned
+    // an evaluation run eagerly whose targets were its sources: the points
+    // in the tree's Morton order (arena-owned, valid until the lane's arena
+    // is reset): sorted float32 records whose last component holds the
+    // input index bits (float32 path), or sorted position -> input index
+    // (float64 path); sorted_n points.  Null after a graph replay.
+    const float4 *sorted_pay = nullptr;
+    const int32_t *sorted_perm = nullptr;
+    int64_t sorted_n = 0;
 };
 
 int fmm_order_from_tol(double tol);

@@ -2011,8 +2011,8 @@ template <bool F32>
 __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
     int64_t M, int64_t NG, const PanelK *__restrict__ pk, const PanelG *__restrict__ grp,
     const double2 *__restrict__ coef, int64_t ne, const int32_t *__restrict__ order,
-    const double *__restrict__ xE, const double *__restrict__ yE, double *__restrict__ vx,
-    double *__restrict__ vy)
+    int ostride, const double *__restrict__ xE, const double *__restrict__ yE,
+    double *__restrict__ vx, double *__restrict__ vy)
 {
     constexpr int NT = F32 ? 16 : PG_TERMS;
     // series coefficients in the precision of the sum (float32: converted
@@ -2025,7 +2025,7 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
     // the groups' chains are short and serial, so more of them in flight
     const int tid = threadIdx.x, sub = tid & (PI_LPT - 1);
     const int64_t k = (int64_t)blockIdx.x * (PI_T / PI_LPT) + tid / PI_LPT;
-    const int64_t i = k < ne ? order[k] : 0;
+    const int64_t i = k < ne ? order[k * ostride] : 0;
     const double x = k < ne ? xE[i] : 0.0, y = k < ne ? yE[i] : 0.0;
     double sx = 0.0, sy = 0.0;
     for (int64_t g0 = 0; g0 < NG; g0 += PG_CHUNK) {
@@ -2122,7 +2122,8 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
 int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
                          const double *yE, const double *xs, const double *ys, const double *xe,
                          const double *ye, const double *cosA, const double *sinA, double *vx,
-                         double *vy, int prec, const PanelSeriesGeom *sg)
+                         double *vy, int prec, const PanelSeriesGeom *sg, const int32_t *order,
+                         int order_stride)
 {
     if (ne <= 0) return PHF_OK;
     PanelK *pk;
@@ -2156,25 +2157,31 @@ int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, co
         panel_group_coeffs<<<(unsigned)ng, 64, 0, L->stream>>>(m, gamma, xs, ys, xe, ye, cosA,
                                                                sinA, grp, coef);
     }
-    // targets in Morton order
-    double4 *bb;
-    PHF_ALLOC(bb, L->arena, double4, 1);
-    PHF_TRY(device_bbox(L, xE, yE, ne, nullptr, nullptr, 0, bb));
-    uint32_t *k0, *k1, *ks;
-    int32_t *v0, *v1, *vs;
-    PHF_ALLOC(k0, L->arena, uint32_t, (size_t)ne);
-    PHF_ALLOC(k1, L->arena, uint32_t, (size_t)ne);
-    PHF_ALLOC(v0, L->arena, int32_t, (size_t)ne);
-    PHF_ALLOC(v1, L->arena, int32_t, (size_t)ne);
-    target_keys16<<<div_up(ne, 256), 256, 0, L->stream>>>(xE, yE, ne, bb, k0, v0);
-    PHF_HIP(hipGetLastError());
-    PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, ne, 16, &ks, &vs));
+    // targets in Morton order: the caller's, or sorted here
+    const int32_t *vs = order;
+    int vstride = order_stride;
+    if (!vs) {
+        double4 *bb;
+        PHF_ALLOC(bb, L->arena, double4, 1);
+        PHF_TRY(device_bbox(L, xE, yE, ne, nullptr, nullptr, 0, bb));
+        uint32_t *k0, *k1, *ks;
+        int32_t *v0, *v1, *sv;
+        PHF_ALLOC(k0, L->arena, uint32_t, (size_t)ne);
+        PHF_ALLOC(k1, L->arena, uint32_t, (size_t)ne);
+        PHF_ALLOC(v0, L->arena, int32_t, (size_t)ne);
+        PHF_ALLOC(v1, L->arena, int32_t, (size_t)ne);
+        target_keys16<<<div_up(ne, 256), 256, 0, L->stream>>>(xE, yE, ne, bb, k0, v0);
+        PHF_HIP(hipGetLastError());
+        PHF_TRY(device_radix_sort(L, k0, v0, k1, v1, ne, 16, &ks, &sv));
+        vs = sv;
+        vstride = 1;
+    }
     if (prec == PHF_PREC_F32C)
         panel_induced_grouped<true><<<div_up(ne, PI_T / PI_LPT), PI_T, 0, L->stream>>>(
-            m, ng, pk, grp, coef, ne, vs, xE, yE, vx, vy);
+            m, ng, pk, grp, coef, ne, vs, vstride, xE, yE, vx, vy);
     else
         panel_induced_grouped<false><<<div_up(ne, PI_T / PI_LPT), PI_T, 0, L->stream>>>(
-            m, ng, pk, grp, coef, ne, vs, xE, yE, vx, vy);
+            m, ng, pk, grp, coef, ne, vs, vstride, xE, yE, vx, vy);
     PHF_HIP(hipGetLastError());
     return PHF_OK;
 }

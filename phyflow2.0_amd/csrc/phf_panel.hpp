@@ -87,12 +87,15 @@ int panel_series_geom_device(Lane *L, int64_t m, const double *xs, const double 
 // Velocity induced at (xE, yE) by the m vortex panels of strengths gamma.
 // sg (optional): the body's PanelSeriesGeom, so the grouped path forms its
 // series coefficients as strengths times factors instead of from the
-// geometry.
+// geometry.  order (optional): the targets in a spatially compact order,
+// order[k * order_stride] the input index of the k-th (the FMM tree's
+// Morton order of the same points), used instead of sorting them here.
 int panel_induced_device(Lane *L, int64_t m, const double *gamma, int64_t ne, const double *xE,
                          const double *yE, const double *xs, const double *ys, const double *xe,
                          const double *ye, const double *cosA, const double *sinA, double *vx,
                          double *vy, int prec = PHF_PREC_F64,
-                         const PanelSeriesGeom *sg = nullptr);
+                         const PanelSeriesGeom *sg = nullptr, const int32_t *order = nullptr,
+                         int order_stride = 1);
 int panel_influence_device(Lane *L, int64_t m, const double *xCP, const double *yCP,
                            const double *xs, const double *ys, const double *xe, const double *ye,
                            const double *cosA, const double *sinA, double *A);

@@ -179,8 +179,8 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
     PHF_ALLOC(tw, L->arena, double, (size_t)nt);
     PHF_ALLOC(vx, L->arena, double, (size_t)nt);
     PHF_ALLOC(vy, L->arena, double, (size_t)nt);
-    PHF_ALLOC(px, L->arena, double, (size_t)std::max<int64_t>(n, 1));
-    PHF_ALLOC(py, L->arena, double, (size_t)std::max<int64_t>(n, 1));
+    PHF_ALLOC(px, L->arena, double, (size_t)std::max<int64_t>(nt, 1));
+    PHF_ALLOC(py, L->arena, double, (size_t)std::max<int64_t>(nt, 1));
     // a rebuild still in flight on the body's stream: the collocation points
     // are needed now, the factors only for the solve below, so the blob
     // velocity evaluation runs beside the factorisation
@@ -199,8 +199,18 @@ static int stage_velocity(Lane *L, const double *X, const double *Y, const doubl
         PHF_TRY(panels_solve_device(L, P, vx + n, vy + n, vinf_x, vinf_y));
         tm.end(PHF_STAGE_PANEL);
     }
-    PHF_TRY(panel_induced_device(L, m, P->s, n, X, Y, P->xs, P->ys, P->xe, P->ye, P->cosA,
-                                 P->sinA, px, py, vp.prec, &P->sg));
+    // the panel velocity at the same points in the tree's Morton order when
+    // the evaluation left it (the collocation points' values go unused),
+    // else at the blobs, sorted by panel_induced_device itself
+    const int32_t *ord = fi.sorted_pay ? reinterpret_cast<const int32_t *>(fi.sorted_pay) + 3
+                                       : fi.sorted_perm;
+    if (ord && fi.sorted_n == nt)
+        PHF_TRY(panel_induced_device(L, m, P->s, nt, tx, ty, P->xs, P->ys, P->xe, P->ye,
+                                     P->cosA, P->sinA, px, py, vp.prec, &P->sg, ord,
+                                     fi.sorted_pay ? 4 : 1));
+    else
+        PHF_TRY(panel_induced_device(L, m, P->s, n, X, Y, P->xs, P->ys, P->xe, P->ye, P->cosA,
+                                     P->sinA, px, py, vp.prec, &P->sg));
     total_velocity_to<<<div_up(n, 256), 256, 0, L->stream>>>(vx, vy, px, py, n, vinf_x, vinf_y,
                                                              kx, ky);
     PHF_HIP(hipGetLastError());
