@@ -130,6 +130,28 @@ __global__ __launch_bounds__(64) void brm_bounds(const BatchSys *__restrict__ sy
     }
 }
 
+__global__ void brm_zero_i32(int32_t *__restrict__ a, int32_t *__restrict__ b, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        a[i] = 0;
+        b[i] = 0;
+    }
+}
+
+// The blob range of every occupied cell from the sorted keys: the first
+// blob of a run of equal keys writes its start, the last its end (an empty
+// cell keeps the range [0, 0)).
+__global__ void brm_cell_runs(const uint32_t *__restrict__ ks, int64_t N,
+                              int32_t *__restrict__ cstart, int32_t *__restrict__ cend)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const uint32_t k = ks[i];
+    if (i == 0 || ks[i - 1] != k) cstart[k] = (int32_t)i;
+    if (i == N - 1 || ks[i + 1] != k) cend[k] = (int32_t)(i + 1);
+}
+
 __global__ void brm_keys(const int64_t *__restrict__ poff, int S, const BatchRm *__restrict__ rm,
                          const BatchRmDyn *__restrict__ dyn, const double *__restrict__ x,
                          const double *__restrict__ y, int64_t N, uint32_t *__restrict__ keys,
@@ -173,6 +195,7 @@ __global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restri
                                                          const BatchRm *__restrict__ rm,
                                                          const BatchRmDyn *__restrict__ dyn,
                                                          const int32_t *__restrict__ cstart,
+                                                         const int32_t *__restrict__ cend,
                                                          int64_t C, uint32_t *__restrict__ flag)
 {
     const int64_t c = (int64_t)blockIdx.x * BRM_BLOCK + threadIdx.x;
@@ -189,7 +212,7 @@ __global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restri
             const long long ii = i - (q & 3), jj = j - (q >> 2);
             if (ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
             const int64_t k = base + ii * d.bj + jj;
-            t = cstart[k + 1] > cstart[k];
+            t = cend[k] > cstart[k];
         }
     }
     flag[c] = t;
@@ -198,7 +221,8 @@ __global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restri
 __global__ __launch_bounds__(BRM_BLOCK) void brm_gather_sum(
     const int32_t *__restrict__ bsys, const BatchRm *__restrict__ rm,
     const BatchRmDyn *__restrict__ dyn, const int32_t *__restrict__ cstart,
-    const double3 *__restrict__ sb, const uint32_t *__restrict__ flag,
+    const int32_t *__restrict__ cend, const double3 *__restrict__ sb,
+    const uint32_t *__restrict__ flag,
     const uint32_t *__restrict__ slot, int64_t C, double c, double c2, double *__restrict__ xo,
     double *__restrict__ yo, double *__restrict__ wo)
 {
@@ -214,13 +238,22 @@ __global__ __launch_bounds__(BRM_BLOCK) void brm_gather_sum(
     const long long I = i + d.imin, J = j + d.jmin;  // absolute cell indices
     const double ui = ((double)I + 0.5) + r.x0h;
     const double uj = ((double)J + 0.5) + r.y0h;
-    double acc = 0.0;
-    bool first = true;
+    // the 16 base cells' blob ranges first, so that their loads overlap
+    int32_t lo[16], hi[16];
+#pragma unroll
     for (int q = 0; q < 16; ++q) {
         const long long ii = i - (q & 3), jj = j - (q >> 2);
+        lo[q] = hi[q] = 0;
         if (ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
         const int64_t k = r.cbase + ii * d.bj + jj;
-        for (int32_t p = cstart[k]; p < cstart[k + 1]; ++p) {
+        lo[q] = cstart[k];
+        hi[q] = cend[k];
+    }
+    double acc = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        for (int32_t p = lo[q]; p < hi[q]; ++p) {
             const double3 b = sb[p];
             const double v = (m4p_dev(ui - b.x, c, c2) * m4p_dev(uj - b.y, c, c2)) * b.z;
             acc = first ? v : acc + v;
@@ -305,7 +338,8 @@ int batch_remesh_alloc(const BatchPlan &P, const BatchRemeshPlan &R, BatchRemesh
     PHF_ALLOC(Bf->xn, A, double, N);
     PHF_ALLOC(Bf->yn, A, double, N);
     PHF_ALLOC(Bf->sb, A, double3, N);
-    PHF_ALLOC(Bf->cstart, A, int32_t, C + 1);
+    PHF_ALLOC(Bf->cstart, A, int32_t, C);
+    PHF_ALLOC(Bf->cend, A, int32_t, C);
     PHF_ALLOC(Bf->flag, A, uint32_t, C);
     PHF_ALLOC(Bf->slot, A, uint32_t, C);
     PHF_ALLOC(Bf->sums, A, uint32_t, (size_t)scan_sums_size((int64_t)C));
@@ -343,6 +377,11 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
     PHF_HIP(hipGetLastError());
     uint32_t *ks = Bv.k0;
     int32_t *vs = Bv.v0;
+    // every cell's blob range: empty first (a fill kernel, not a memset: the
+    // sequence is captured in a graph), then the occupied cells' from the
+    // runs of the sorted keys
+    brm_zero_i32<<<(unsigned)std::min<int64_t>(div_up(C, 256), 4096), 256, 0, L->stream>>>(
+        Bf.cstart, Bf.cend, C);
     if (N > 0) {
         brm_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, Bf.dyn, xs, ys, N,
                                                         Bv.k0, Bv.v0);
@@ -350,18 +389,20 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
         // the velocity pass is done with the sort buffers and workspaces
         PHF_TRY(device_radix_sort_ws(L, Bv.k0, Bv.v0, Bv.k1, Bv.v1, N, R.key_bits, Bv.hist,
                                      Bv.offs, &ks, &vs));
+        brm_cell_runs<<<div_up(N, 256), 256, 0, L->stream>>>(ks, N, Bf.cstart, Bf.cend);
     }
-    PHF_TRY(leaf_starts_device(L, ks, N, C, Bf.cstart));
+    PHF_HIP(hipGetLastError());
     if (N > 0)
         brm_gather_scaled<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, xs, ys, w, vs,
                                                                  N, Bf.sb);
     if (C > 0) {
         brm_touched<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK, 0, L->stream>>>(
-            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, C, Bf.flag);
+            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, Bf.cend, C, Bf.flag);
         PHF_HIP(hipGetLastError());
         PHF_TRY(device_exclusive_scan_ws(L, Bf.flag, C, Bf.slot, Bf.sums));
         brm_gather_sum<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK, 0, L->stream>>>(
-            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, Bf.sb, Bf.flag, Bf.slot, C, R.p.c, R.p.c * R.p.c,
+            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, Bf.cend, Bf.sb, Bf.flag, Bf.slot, C, R.p.c,
+            R.p.c * R.p.c,
             Bf.xo, Bf.yo, Bf.wo);
     }
     brm_counts<<<div_up(S, 64), 64, 0, L->stream>>>(Bf.rm, Bf.dyn, S, Bf.flag, Bf.slot, C, Bf.mo);
