@@ -87,9 +87,9 @@ struct BatchRemeshBuffers {
     int32_t *bsys = nullptr;               // system of each workgroup of cells
     double *xn = nullptr, *yn = nullptr;
     double3 *sb = nullptr;
-    // blob range [cstart[k], cend[k]) of cell k in the sorted order (both 0
-    // for an empty cell)
-    int32_t *cstart = nullptr, *cend = nullptr;
+    // blob range [x, y) of cell k in the sorted order (both 0 for an empty
+    // cell)
+    int2 *crange = nullptr;
     uint32_t *flag = nullptr, *slot = nullptr, *sums = nullptr;
     double *xo = nullptr, *yo = nullptr, *wo = nullptr;
 };

@@ -130,26 +130,24 @@ __global__ __launch_bounds__(64) void brm_bounds(const BatchSys *__restrict__ sy
     }
 }
 
-__global__ void brm_zero_i32(int32_t *__restrict__ a, int32_t *__restrict__ b, int64_t n)
+__global__ void brm_zero_ranges(int2 *__restrict__ a, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        a[i] = 0;
-        b[i] = 0;
-    }
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = make_int2(0, 0);
 }
 
 // The blob range of every occupied cell from the sorted keys: the first
 // blob of a run of equal keys writes its start, the last its end (an empty
 // cell keeps the range [0, 0)).
 __global__ void brm_cell_runs(const uint32_t *__restrict__ ks, int64_t N,
-                              int32_t *__restrict__ cstart, int32_t *__restrict__ cend)
+                              int2 *__restrict__ crange)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const uint32_t k = ks[i];
-    if (i == 0 || ks[i - 1] != k) cstart[k] = (int32_t)i;
-    if (i == N - 1 || ks[i + 1] != k) cend[k] = (int32_t)(i + 1);
+    if (i == 0 || ks[i - 1] != k) crange[k].x = (int32_t)i;
+    if (i == N - 1 || ks[i + 1] != k) crange[k].y = (int32_t)(i + 1);
 }
 
 __global__ void brm_keys(const int64_t *__restrict__ poff, int S, const BatchRm *__restrict__ rm,
@@ -194,8 +192,7 @@ constexpr int BRM_BLOCK = 256;
 __global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restrict__ bsys,
                                                          const BatchRm *__restrict__ rm,
                                                          const BatchRmDyn *__restrict__ dyn,
-                                                         const int32_t *__restrict__ cstart,
-                                                         const int32_t *__restrict__ cend,
+                                                         const int2 *__restrict__ crange,
                                                          int64_t C, uint32_t *__restrict__ flag)
 {
     const int64_t c = (int64_t)blockIdx.x * BRM_BLOCK + threadIdx.x;
@@ -212,59 +209,111 @@ __global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restri
             const long long ii = i - (q & 3), jj = j - (q >> 2);
             if (ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
             const int64_t k = base + ii * d.bj + jj;
-            t = cend[k] > cstart[k];
+            const int2 rg = crange[k];
+            t = rg.y > rg.x;
         }
     }
     flag[c] = t;
 }
 
-__global__ __launch_bounds__(BRM_BLOCK) void brm_gather_sum(
+// Sums of the touched cells (i, j0 + t), t < len <= 4, of one row.  They
+// share their base cells: columns jj = j0 + 3 .. j0 - 3 of base rows
+// i .. i - 3 (all 28 blob ranges loaded first), and each blob's x weight
+// serves every cell of the row.  Per cell the contributions arrive in the
+// order of the one-cell sum -- jj descending, then base row descending,
+// then blobs ascending -- so the sums are the same.  acc starts at -0.0,
+// which the first contribution replaces exactly (-0.0 + v == v for every
+// v); a touched cell has at least one contribution.
+__device__ __forceinline__ void brm_gather_row4(
+    const BatchRm &r, const BatchRmDyn &d, long long i, long long j0, int len, int64_t c0,
+    const int2 *__restrict__ crange, const double3 *__restrict__ sb,
+    const uint32_t *__restrict__ flag, const uint32_t *__restrict__ slot, double c, double c2,
+    double *__restrict__ xo, double *__restrict__ yo, double *__restrict__ wo)
+{
+    bool tf[4], any = false;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        tf[t] = t < len && flag[c0 + t] != 0u;
+        any |= tf[t];
+    }
+    if (!any) return;
+    int2 rg[7][4];
+#pragma unroll
+    for (int cc = 0; cc < 7; ++cc) {
+        const long long jj = j0 + 3 - cc;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const long long ii = i - a;
+            rg[cc][a] = make_int2(0, 0);
+            // column jj feeds the cells t = jj - j0 .. jj - j0 + 3
+            if (3 - cc >= len || ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
+            rg[cc][a] = crange[r.cbase + ii * d.bj + jj];
+        }
+    }
+    const double ui = ((double)(i + d.imin) + 0.5) + r.x0h;
+    double uj[4], acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        uj[t] = ((double)(j0 + t + d.jmin) + 0.5) + r.y0h;
+        acc[t] = -0.0;
+    }
+#pragma unroll
+    for (int cc = 0; cc < 7; ++cc) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            for (int32_t p = rg[cc][a].x; p < rg[cc][a].y; ++p) {
+                const double3 b = sb[p];
+                const double fx = m4p_dev(ui - b.x, c, c2);
+#pragma unroll
+                for (int t = (3 - cc > 0 ? 3 - cc : 0); t <= (6 - cc < 3 ? 6 - cc : 3); ++t)
+                    acc[t] = acc[t] + (fx * m4p_dev(uj[t] - b.y, c, c2)) * b.z;
+            }
+        }
+    }
+    const double hh = 0.5 * r.h;
+    const double xc = ((double)(i + d.imin) * r.h + hh) + r.x0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        if (!tf[t]) continue;
+        const uint32_t o = slot[c0 + t];
+        wo[o] = acc[t];
+        xo[o] = xc;
+        yo[o] = ((double)(j0 + t + d.jmin) * r.h + hh) + r.y0;
+    }
+}
+
+// Four consecutive cells of a system's touched range per thread (BRM_BLOCK
+// cells per 64-lane workgroup); with nj >= 4 they span at most two rows.
+__global__ __launch_bounds__(BRM_BLOCK / 4) void brm_gather_sum(
     const int32_t *__restrict__ bsys, const BatchRm *__restrict__ rm,
-    const BatchRmDyn *__restrict__ dyn, const int32_t *__restrict__ cstart,
-    const int32_t *__restrict__ cend, const double3 *__restrict__ sb,
-    const uint32_t *__restrict__ flag,
+    const BatchRmDyn *__restrict__ dyn, const int2 *__restrict__ crange,
+    const double3 *__restrict__ sb, const uint32_t *__restrict__ flag,
     const uint32_t *__restrict__ slot, int64_t C, double c, double c2, double *__restrict__ xo,
     double *__restrict__ yo, double *__restrict__ wo)
 {
-    const int64_t cidx = (int64_t)blockIdx.x * BRM_BLOCK + threadIdx.x;
-    if (cidx >= C || !flag[cidx]) return;
+    const int64_t c0 = (int64_t)blockIdx.x * BRM_BLOCK + 4 * threadIdx.x;
+    if (c0 >= C) return;
     const int s = bsys[blockIdx.x];
     const BatchRm r = rm[s];
     const BatchRmDyn d = dyn[s];
-    const int64_t loc = cidx - r.cbase;
-    const long long nj = d.bj + 3;
+    if (!d.ok) return;
+    const int64_t loc = c0 - r.cbase;
+    const long long nj = d.bj + 3, ni = d.bi + 3;
+    if (loc >= ni * nj) return;
     long long i, j;
     divmod_cells(loc, nj, i, j);
-    const long long I = i + d.imin, J = j + d.jmin;  // absolute cell indices
-    const double ui = ((double)I + 0.5) + r.x0h;
-    const double uj = ((double)J + 0.5) + r.y0h;
-    // the 16 base cells' blob ranges first, so that their loads overlap
-    int32_t lo[16], hi[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const long long ii = i - (q & 3), jj = j - (q >> 2);
-        lo[q] = hi[q] = 0;
-        if (ii < 0 || jj < 0 || ii >= d.bi || jj >= d.bj) continue;
-        const int64_t k = r.cbase + ii * d.bj + jj;
-        lo[q] = cstart[k];
-        hi[q] = cend[k];
+    int len = (int)(nj - j < 4 ? nj - j : 4);
+    int64_t cs = c0;
+    // one copy of the row code for both segments
+#pragma nounroll
+    for (int seg = 0; seg < 2; ++seg) {
+        brm_gather_row4(r, d, i, j, len, cs, crange, sb, flag, slot, c, c2, xo, yo, wo);
+        if (cs + len == c0 + 4 || i + 1 >= ni) break;
+        cs += len;
+        len = (int)(c0 + 4 - cs);
+        ++i;
+        j = 0;
     }
-    double acc = 0.0;
-    bool first = true;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        for (int32_t p = lo[q]; p < hi[q]; ++p) {
-            const double3 b = sb[p];
-            const double v = (m4p_dev(ui - b.x, c, c2) * m4p_dev(uj - b.y, c, c2)) * b.z;
-            acc = first ? v : acc + v;
-            first = false;
-        }
-    }
-    const uint32_t o = slot[cidx];
-    const double hh = 0.5 * r.h;
-    wo[o] = acc;
-    xo[o] = ((double)I * r.h + hh) + r.x0;
-    yo[o] = ((double)J * r.h + hh) + r.y0;
 }
 
 // per system: packed output offset and count (-1: did not fit)
@@ -338,8 +387,7 @@ int batch_remesh_alloc(const BatchPlan &P, const BatchRemeshPlan &R, BatchRemesh
     PHF_ALLOC(Bf->xn, A, double, N);
     PHF_ALLOC(Bf->yn, A, double, N);
     PHF_ALLOC(Bf->sb, A, double3, N);
-    PHF_ALLOC(Bf->cstart, A, int32_t, C);
-    PHF_ALLOC(Bf->cend, A, int32_t, C);
+    PHF_ALLOC(Bf->crange, A, int2, C);
     PHF_ALLOC(Bf->flag, A, uint32_t, C);
     PHF_ALLOC(Bf->slot, A, uint32_t, C);
     PHF_ALLOC(Bf->sums, A, uint32_t, (size_t)scan_sums_size((int64_t)C));
@@ -380,8 +428,8 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
     // every cell's blob range: empty first (a fill kernel, not a memset: the
     // sequence is captured in a graph), then the occupied cells' from the
     // runs of the sorted keys
-    brm_zero_i32<<<(unsigned)std::min<int64_t>(div_up(C, 256), 4096), 256, 0, L->stream>>>(
-        Bf.cstart, Bf.cend, C);
+    brm_zero_ranges<<<(unsigned)std::min<int64_t>(div_up(C, 256), 4096), 256, 0, L->stream>>>(
+        Bf.crange, C);
     if (N > 0) {
         brm_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, Bf.dyn, xs, ys, N,
                                                         Bv.k0, Bv.v0);
@@ -389,7 +437,7 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
         // the velocity pass is done with the sort buffers and workspaces
         PHF_TRY(device_radix_sort_ws(L, Bv.k0, Bv.v0, Bv.k1, Bv.v1, N, R.key_bits, Bv.hist,
                                      Bv.offs, &ks, &vs));
-        brm_cell_runs<<<div_up(N, 256), 256, 0, L->stream>>>(ks, N, Bf.cstart, Bf.cend);
+        brm_cell_runs<<<div_up(N, 256), 256, 0, L->stream>>>(ks, N, Bf.crange);
     }
     PHF_HIP(hipGetLastError());
     if (N > 0)
@@ -397,11 +445,11 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
                                                                  N, Bf.sb);
     if (C > 0) {
         brm_touched<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK, 0, L->stream>>>(
-            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, Bf.cend, C, Bf.flag);
+            Bf.bsys, Bf.rm, Bf.dyn, Bf.crange, C, Bf.flag);
         PHF_HIP(hipGetLastError());
         PHF_TRY(device_exclusive_scan_ws(L, Bf.flag, C, Bf.slot, Bf.sums));
-        brm_gather_sum<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK, 0, L->stream>>>(
-            Bf.bsys, Bf.rm, Bf.dyn, Bf.cstart, Bf.cend, Bf.sb, Bf.flag, Bf.slot, C, R.p.c,
+        brm_gather_sum<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK / 4, 0, L->stream>>>(
+            Bf.bsys, Bf.rm, Bf.dyn, Bf.crange, Bf.sb, Bf.flag, Bf.slot, C, R.p.c,
             R.p.c * R.p.c,
             Bf.xo, Bf.yo, Bf.wo);
     }
