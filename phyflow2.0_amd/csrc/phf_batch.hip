@@ -59,6 +59,22 @@ __device__ __forceinline__ int find_seg(const int64_t *__restrict__ arr, int S, 
     return lo;
 }
 
+// find_seg where the active lanes of a wave mostly share one segment
+// (consecutive points, sorted keys, consecutive boxes): the first active
+// lane's segment is searched with wave-uniform loads and serves every lane
+// that lies inside it; the others search alone.
+__device__ __forceinline__ int find_seg_wave(const int64_t *__restrict__ arr, int S, int64_t g)
+{
+    const uint64_t u = (uint64_t)g;
+    const int64_t g0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                                      (int)(uint32_t)(u >> 32))
+                                  << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+    const int s0 = find_seg(arr, S, g0);
+    if (arr[s0] <= g && (s0 == S - 1 || g < arr[s0 + 1])) return s0;
+    return find_seg(arr, S, g);
+}
+
 __device__ __forceinline__ bool near_pred(int dx, int dy, int ws, double s, double c2)
 {
     return near_offset(dx, dy, ws, s, c2);
@@ -148,19 +164,25 @@ __global__ __launch_bounds__(64) void batch_domain(const BatchSys *__restrict__ 
     }
 }
 
+// Leaf keys of the points, and each point's (x, y, w) packed in one record
+// so that the pass in sorted order reads one 32-byte record per point
+// instead of three scattered values.
 __global__ void batch_keys(const BatchSys *__restrict__ sys, const int64_t *__restrict__ poff,
                            int S, const BatchDom *__restrict__ dom, const double *__restrict__ x,
-                           const double *__restrict__ y, int64_t N, uint32_t *__restrict__ keys,
-                           int32_t *__restrict__ vals)
+                           const double *__restrict__ y, const double *__restrict__ w, int64_t N,
+                           uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
+                           double4 *__restrict__ pk)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const int s = find_seg(poff, S, i);
+    const int s = find_seg_wave(poff, S, i);
     const BatchDom d = dom[s];
     const int L = sys[s].L;
     const double hi = (double)((1u << L) - 1u);
-    double fx = floor((x[i] - d.x0) * d.inv_s);
-    double fy = floor((y[i] - d.y0) * d.inv_s);
+    const double xi = x[i], yi = y[i];
+    pk[i] = make_double4(xi, yi, w[i], 0.0);
+    double fx = floor((xi - d.x0) * d.inv_s);
+    double fy = floor((yi - d.y0) * d.inv_s);
     fx = fx >= 0.0 ? fx : 0.0;
     fy = fy >= 0.0 ? fy : 0.0;
     fx = fx > hi ? hi : fx;
@@ -174,24 +196,23 @@ __global__ void batch_keys(const BatchSys *__restrict__ sys, const int64_t *__re
 template <class T>
 __global__ void batch_gather(const BatchSys *__restrict__ sys, const int64_t *__restrict__ loff,
                              int S, const BatchDom *__restrict__ dom,
-                             const double *__restrict__ x, const double *__restrict__ y,
-                             const double *__restrict__ w, const uint32_t *__restrict__ skeys,
+                             const double4 *__restrict__ pk, const uint32_t *__restrict__ skeys,
                              const int32_t *__restrict__ perm, const int32_t *__restrict__ start,
                              const int32_t *__restrict__ sp, int64_t N, void *__restrict__ out)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
-    const int32_t j = perm[k];
+    const double4 q = pk[perm[k]];
     if (sizeof(T) == 4) {
         const int64_t g = skeys[k];
-        const int s = find_seg(loff, S, g);
+        const int s = find_seg_wave(loff, S, g);
         const BatchDom d = dom[s];
         double cx, cy;
         leaf_center((uint32_t)(g - sys[s].leafbase), d.x0, d.y0, d.s, cx, cy);
         reinterpret_cast<float4 *>(out)[sp[g] + (k - start[g])] =
-            make_float4((float)(x[j] - cx), (float)(y[j] - cy), (float)w[j], 0.0f);
+            make_float4((float)(q.x - cx), (float)(q.y - cy), (float)q.z, 0.0f);
     } else {
-        reinterpret_cast<double4 *>(out)[k] = make_double4(x[j], y[j], w[j], 0.0);
+        reinterpret_cast<double4 *>(out)[k] = make_double4(q.x, q.y, q.z, 0.0);
     }
 }
 
@@ -270,7 +291,7 @@ __global__ void batch_m2m(const BatchSys *__restrict__ sys, const int64_t *__res
     if (t >= nb * p1) return;
     const int64_t gb = row[0] + t / p1;
     const int m = (int)(t % p1);
-    const int s = find_seg(row, S, gb);
+    const int s = find_seg_wave(row, S, gb);
     const int L = sys[s].L;
     if (l >= L) return;  // level l is not a parent level of system s
     const uint32_t b = (uint32_t)(gb - row[s]);
@@ -306,7 +327,7 @@ __global__ void batch_l2l(const BatchSys *__restrict__ sys, const int64_t *__res
     if (t >= nb * p1) return;
     const int64_t gb = row[0] + t / p1;
     const int m = (int)(t % p1);
-    const int s = find_seg(row, S, gb);
+    const int s = find_seg_wave(row, S, gb);
     const int L = sys[s].L;
     if (l < 2 || l + 1 > L) return;
     const uint32_t b = (uint32_t)(gb - row[s]);
@@ -451,7 +472,7 @@ __global__ __launch_bounds__(64) void batch_m2l_mfma(
     const int64_t slot = blockIdx.x;
     if (slot >= nb) return;
     const int64_t gb = row[0] + slot;
-    const int s = find_seg(row, S, gb);
+    const int s = find_seg_wave(row, S, gb);
     const int L = sys[s].L;
     if (l > L) return;
     const uint32_t b = (uint32_t)(gb - row[s]);
@@ -914,6 +935,7 @@ int batch_alloc(Lane *L, const BatchPlan &P, BatchBuffers *Bf)
     PHF_ALLOC(Bf->k1, L->arena, uint32_t, N);
     PHF_ALLOC(Bf->v0, L->arena, int32_t, N);
     PHF_ALLOC(Bf->v1, L->arena, int32_t, N);
+    PHF_ALLOC(Bf->pk, L->arena, double4, N);
     PHF_ALLOC(Bf->start, L->arena, int32_t, (size_t)P.nleaf + 1);
     if (f32) {
         const size_t nl = (size_t)P.nleaf + 1;
@@ -1033,8 +1055,8 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
                                           reinterpret_cast<const double4 *>(Bf.cpart), P.ws,
                                           P.p + 1, P.prec == PHF_PREC_F32C ? 16 : 64, Bf.dom);
     if (N > 0)
-        batch_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.poff, S, Bf.dom, x, y, N,
-                                                          Bf.k0, Bf.v0);
+        batch_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.poff, S, Bf.dom, x, y, w, N,
+                                                          Bf.k0, Bf.v0, Bf.pk);
     PHF_HIP(hipGetLastError());
     int key_bits = 1;
     while (key_bits < 32 && (int64_t(1) << key_bits) < P.nleaf) ++key_bits;
@@ -1050,8 +1072,8 @@ static int batch_enqueue_typed(Lane *L, const BatchPlan &P, BatchBuffers &Bf, co
         PHF_TRY(zero_source_pads(L, Bf.start, sp, P.nleaf, reinterpret_cast<float4 *>(Bf.src)));
     }
     if (N > 0)
-        batch_gather<T><<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.loff, S, Bf.dom, x, y,
-                                                               w, ks, vs, Bf.start, sp, N, Bf.src);
+        batch_gather<T><<<div_up(N, 256), 256, 0, L->stream>>>(Bf.sys, Bf.loff, S, Bf.dom, Bf.pk,
+                                                               ks, vs, Bf.start, sp, N, Bf.src);
     PHF_HIP(hipGetLastError());
     V *M = reinterpret_cast<V *>(Bf.M), *B = reinterpret_cast<V *>(Bf.B);
     const VC *m2m = reinterpret_cast<const VC *>(Bf.m2m);

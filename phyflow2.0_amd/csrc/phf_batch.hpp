@@ -47,6 +47,7 @@ struct BatchBuffers {
     int64_t *poff = nullptr, *loff = nullptr, *lvl = nullptr;
     uint32_t *k0 = nullptr, *k1 = nullptr;
     int32_t *v0 = nullptr, *v1 = nullptr;
+    double4 *pk = nullptr;  // (x, y, w, 0) per point in input order
     int32_t *start = nullptr;
     char *src = nullptr, *M = nullptr, *B = nullptr;
     char *m2m = nullptr, *l2l = nullptr, *bin = nullptr, *U = nullptr;

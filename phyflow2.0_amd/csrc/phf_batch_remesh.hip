@@ -150,38 +150,47 @@ __global__ void brm_cell_runs(const uint32_t *__restrict__ ks, int64_t N,
     if (i == N - 1 || ks[i + 1] != k) crange[k].y = (int32_t)(i + 1);
 }
 
+// Cell keys of the blobs, and each blob's (x/h, y/h, w) packed in one
+// record so that the pass in sorted order reads one 32-byte record per
+// blob instead of three scattered values.
 __global__ void brm_keys(const int64_t *__restrict__ poff, int S, const BatchRm *__restrict__ rm,
                          const BatchRmDyn *__restrict__ dyn, const double *__restrict__ x,
-                         const double *__restrict__ y, int64_t N, uint32_t *__restrict__ keys,
-                         int32_t *__restrict__ vals)
+                         const double *__restrict__ y, const double *__restrict__ w, int64_t N,
+                         uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
+                         double4 *__restrict__ pk)
 {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= N) return;
-    const int s = find_seg64(poff, S, b);
+    // the wave's first blob's system serves every blob of the wave inside it
+    const uint64_t u = (uint64_t)b;
+    const int64_t b0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                                      (int)(uint32_t)(u >> 32))
+                                  << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u));
+    int s = find_seg64(poff, S, b0);
+    if (!(poff[s] <= b && (s == S - 1 || b < poff[s + 1]))) s = find_seg64(poff, S, b);
     const BatchRm r = rm[s];
     const BatchRmDyn d = dyn[s];
+    const double xb = x[b], yb = y[b];
     uint32_t key = (uint32_t)r.cbase;  // a flagged system's blobs share one key
     if (d.ok) {
-        const long long i = base_index(x[b], r.x0, r.h) - d.imin;
-        const long long j = base_index(y[b], r.y0, r.h) - d.jmin;
+        const long long i = base_index(xb, r.x0, r.h) - d.imin;
+        const long long j = base_index(yb, r.y0, r.h) - d.jmin;
         key = (uint32_t)(r.cbase + i * d.bj + j);
     }
     keys[b] = key;
     vals[b] = (int32_t)b;
+    pk[b] = make_double4(xb / r.h, yb / r.h, w[b], 0.0);
 }
 
 // sorted blob data: (x/h, y/h, w)
-__global__ void brm_gather_scaled(const int64_t *__restrict__ poff, int S,
-                                  const BatchRm *__restrict__ rm, const double *__restrict__ x,
-                                  const double *__restrict__ y, const double *__restrict__ w,
-                                  const int32_t *__restrict__ perm, int64_t N,
-                                  double3 *__restrict__ out)
+__global__ void brm_gather_scaled(const double4 *__restrict__ pk, const int32_t *__restrict__ perm,
+                                  int64_t N, double3 *__restrict__ out)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
-    const int32_t b = perm[k];
-    const double h = rm[find_seg64(poff, S, b)].h;
-    out[k] = make_double3(x[b] / h, y[b] / h, w[b]);
+    const double4 q = pk[perm[k]];
+    out[k] = make_double3(q.x, q.y, q.z);
 }
 
 // touched = one of the 16 base cells (i-a, j-b), a, b in 0..3, holds a blob.
@@ -282,6 +291,16 @@ __device__ __forceinline__ void brm_gather_row4(
     }
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD
+// b mod 8), each with its own L2.  This maps them so that XCD x takes a
+// contiguous run of cell blocks: neighbouring rows of cells, which read the
+// same blobs, then meet in the same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
+{
+    const unsigned q = G / 8u, r = G % 8u, x = b % 8u, k = b / 8u;
+    return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + k;
+}
+
 // Four consecutive cells of a system's touched range per thread (BRM_BLOCK
 // cells per 64-lane workgroup); with nj >= 4 they span at most two rows.
 __global__ __launch_bounds__(BRM_BLOCK / 4) void brm_gather_sum(
@@ -291,9 +310,10 @@ __global__ __launch_bounds__(BRM_BLOCK / 4) void brm_gather_sum(
     const uint32_t *__restrict__ slot, int64_t C, double c, double c2, double *__restrict__ xo,
     double *__restrict__ yo, double *__restrict__ wo)
 {
-    const int64_t c0 = (int64_t)blockIdx.x * BRM_BLOCK + 4 * threadIdx.x;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t c0 = (int64_t)blk * BRM_BLOCK + 4 * threadIdx.x;
     if (c0 >= C) return;
-    const int s = bsys[blockIdx.x];
+    const int s = bsys[blk];
     const BatchRm r = rm[s];
     const BatchRmDyn d = dyn[s];
     if (!d.ok) return;
@@ -431,8 +451,8 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
     brm_zero_ranges<<<(unsigned)std::min<int64_t>(div_up(C, 256), 4096), 256, 0, L->stream>>>(
         Bf.crange, C);
     if (N > 0) {
-        brm_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, Bf.dyn, xs, ys, N,
-                                                        Bv.k0, Bv.v0);
+        brm_keys<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, Bf.dyn, xs, ys, w, N,
+                                                        Bv.k0, Bv.v0, Bv.pk);
         PHF_HIP(hipGetLastError());
         // the velocity pass is done with the sort buffers and workspaces
         PHF_TRY(device_radix_sort_ws(L, Bv.k0, Bv.v0, Bv.k1, Bv.v1, N, R.key_bits, Bv.hist,
@@ -441,8 +461,7 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
     }
     PHF_HIP(hipGetLastError());
     if (N > 0)
-        brm_gather_scaled<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.poff, S, Bf.rm, xs, ys, w, vs,
-                                                                 N, Bf.sb);
+        brm_gather_scaled<<<div_up(N, 256), 256, 0, L->stream>>>(Bv.pk, vs, N, Bf.sb);
     if (C > 0) {
         brm_touched<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK, 0, L->stream>>>(
             Bf.bsys, Bf.rm, Bf.dyn, Bf.crange, C, Bf.flag);
