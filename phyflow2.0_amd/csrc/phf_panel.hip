@@ -1787,7 +1787,7 @@ int dense_solve_device(Lane *L, const double *B, int64_t rows, int64_t n, const 
 // group's series is a serial chain, and the VALU issued 29% of the time).
 constexpr int PG_SIZE = 16;    // panels per group
 constexpr int PG_TERMS = 30;   // series terms kept (float64; float32 uses 16)
-constexpr int PG_CHUNK = 16;   // groups staged per round (256 panels)
+constexpr int PG_CHUNK = 64;   // group headers and series staged per round
 constexpr int PI_LPT = 4;      // lanes per target in the grouped evaluation
 
 struct PanelG {
@@ -2018,7 +2018,6 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
     // series coefficients in the precision of the sum (float32: converted
     // once per chunk, not per term and target)
     using CT = typename std::conditional<F32, float2, double2>::type;
-    __shared__ PanelK sp[PG_CHUNK * PG_SIZE];
     __shared__ PanelG sg[PG_CHUNK];
     __shared__ CT sc[PG_CHUNK * NT];
     // PI_LPT lanes per target, each taking every PI_LPT-th group of a round:
@@ -2030,10 +2029,7 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
     double sx = 0.0, sy = 0.0;
     for (int64_t g0 = 0; g0 < NG; g0 += PG_CHUNK) {
         const int ng = (int)min((int64_t)PG_CHUNK, NG - g0);
-        const int64_t p0 = g0 * PG_SIZE;
-        const int npan = (int)min((int64_t)PG_CHUNK * PG_SIZE, M - p0);
         __syncthreads();
-        for (int e = tid; e < npan; e += PI_T) sp[e] = pk[p0 + e];
         if (tid < ng) sg[tid] = grp[g0 + tid];
         for (int e = tid; e < ng * NT; e += PI_T) {
             const int gg = e / NT, kk = e - gg * NT;
@@ -2081,9 +2077,10 @@ __global__ __launch_bounds__(PI_T) void panel_induced_grouped(
                 sx += wr * G.inv_R;
                 sy -= wi * G.inv_R;
             } else {
-                const int q0 = G.first - (int32_t)p0;
-                for (int q = q0; q < q0 + G.count; ++q) {
-                    const PanelK p = sp[q];
+                // a near group's panels come from memory (L1/L2): only the
+                // few targets beside the body read them
+                for (int q = G.first; q < G.first + G.count; ++q) {
+                    const PanelK p = pk[q];
                     const double ex = x - p.xs, ey = y - p.ys;
                     const double xP = p.c * ex + p.s * ey;
                     const double yP = -p.s * ex + p.c * ey;
