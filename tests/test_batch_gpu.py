@@ -125,6 +125,48 @@ def test_batch_step_remesh_matches_single(cell_factor, O):
         assert total == info["remeshed"]
 
 
+def test_batch_step_remesh_thin_and_crowded_systems():
+    """Lattice shapes the batched gather walks in rows of four cells: rows
+    of 4..11 cells (every remainder of the row length mod 4, so a thread's
+    cells wrap into the next row), a one-cell-high line of blobs, a
+    one-cell-wide column, and a cell holding 40 blobs.  Each system is
+    remeshed by the batch (none spills) and is bit-identical to the
+    single-system Regrid of the same convected positions."""
+    rng = np.random.default_rng(12)
+    sigma, overlap = 0.01, 0.8
+    h = sigma * overlap
+    systems = []
+    for bj in range(1, 9):  # touched rows of bj + 3 = 4..11 cells
+        n = 60
+        x = (rng.integers(0, 12, n) + rng.random(n)) * h
+        y = (rng.integers(0, bj, n) + rng.random(n)) * h
+        systems.append((x, y))
+    n = 200  # a line along x, one cell high; then a column along y
+    systems.append(((np.arange(n) + rng.random(n)) * h, np.full(n, 0.3 * h)))
+    systems.append((np.full(n, 0.6 * h), (np.arange(n) + rng.random(n)) * h))
+    xc = np.concatenate([np.full(40, 5.5 * h) + rng.random(40) * 0.4 * h,
+                         (rng.integers(0, 12, 60) + rng.random(60)) * h])
+    yc = np.concatenate([np.full(40, 3.5 * h) + rng.random(40) * 0.4 * h,
+                         (rng.integers(0, 12, 60) + rng.random(60)) * h])
+    systems.append((xc, yc))
+    systems = [dict(x=x, y=y, w=rng.standard_normal(x.size) * 1e-4, sigma=sigma)
+               for x, y in systems]
+    sizes = [s["x"].size for s in systems]
+    dt, vinf = 1e-3, (0.0, 0.0)
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C) as b:
+        for i, s in enumerate(systems):
+            b.set_system(i, s["x"], s["y"], s["w"])
+        b.set_remesh(overlap, dt=dt, vInf=vinf)
+        info = b.step(use_graph=False)
+        assert info["flags"] == 0 and info["spilled"] == 0
+        for i, s in enumerate(systems):
+            v = b.get_velocity(i)
+            got = b.get_remeshed(i)
+            ref, _ = _single_step(s, v, dt, vinf, overlap)
+            for a, r in zip(got, ref[:3]):
+                assert np.array_equal(a, r), i
+
+
 def test_batch_step_graph_replays():
     sizes, systems = _mix(6, seed=11)
     with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C) as b:
