@@ -325,8 +325,9 @@ def bench_cylinder(args):
                                           "ms": rem_ms, "achieved": rem_bytes / (rem_ms * 1e-3) / 1e9,
                                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                           "frac": rem_bytes / (rem_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                          "note": "remesh stage of the step (host-timed stage, "
-                                                  "includes its synchronisations)"}}}
+                                          "note": "remesh stage of the step (events on the step's "
+                                                  "stream; includes the stream's wait while the "
+                                                  "host reads the output count)"}}}
 
 
 def mix_leg_child(timeout_s=400):

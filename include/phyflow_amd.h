@@ -400,6 +400,8 @@ typedef struct {
     double impulse_x_before, impulse_x_after, impulse_y_before, impulse_y_after;
     double m2_before, m2_after;
     int32_t solve_iterations, pad;
+    /* convection and remesh: elapsed time between events on the step's
+     * stream; total: host wall time of the call */
     double ms_convection, ms_remesh, ms_total;
 } phf_step_info;
 

@@ -138,6 +138,8 @@ struct Lane {
     FmmGraphCache *fmm_graphs = nullptr;
     hipStream_t stream2 = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // start, end of convection, end of a time step (its stage times)
+    hipEvent_t step_ev[3] = {};
 };
 
 // Waits for all work enqueued on the lane's streams (its stream and, once

@@ -120,6 +120,19 @@ __global__ void euler_stage(const double *__restrict__ x, const double *__restri
     y1[i] = y[i] + dt * ky[i];
 }
 
+// the remeshed blobs into the system's arrays: one launch for the three
+__global__ void copy3_f64(const double *__restrict__ ax, const double *__restrict__ ay,
+                          const double *__restrict__ aw, int64_t n, double *__restrict__ bx,
+                          double *__restrict__ by, double *__restrict__ bw)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        bx[i] = ax[i];
+        by[i] = ay[i];
+        bw[i] = aw[i];
+    }
+}
+
 __global__ void heun_final(double *__restrict__ x, double *__restrict__ y,
                            const double *__restrict__ k1x, const double *__restrict__ k1y,
                            const double *__restrict__ k2x, const double *__restrict__ k2y,
@@ -535,6 +548,11 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
     Lane *L = &s->lane;
     PHF_TRY(reset_arena(L));
     auto t0 = std::chrono::steady_clock::now();
+    // stage times from events on the step's stream: no synchronisation
+    // between convection and remesh just to time them
+    for (auto &ev : L->step_ev)
+        if (!ev) PHF_HIP(hipEventCreate(&ev));
+    PHF_HIP(hipEventRecord(L->step_ev[0], L->stream));
     StageTimer tm(L, false);
     bool used[PHF_STAGE_COUNT] = {};
     const int64_t n = s->n;
@@ -559,8 +577,7 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
         PHF_HIP(hipGetLastError());
         if (P) inf.solve_iterations = P->last_iterations;
     }
-    PHF_HIP(hipStreamSynchronize(L->stream));
-    auto t1 = std::chrono::steady_clock::now();
+    PHF_HIP(hipEventRecord(L->step_ev[1], L->stream));
     double4 *dm;
     PHF_ALLOC(dm, L->arena, double4, 2);
     double4 hm[2] = {};
@@ -574,9 +591,11 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
                               sp->remesh_x0, sp->remesh_y0, sp->remesh_c, &ro,
                               sp->remesh_kernel));
         PHF_TRY(sys_reserve(s, ro.m));
-        PHF_HIP(hipMemcpyAsync(s->x, ro.x, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
-        PHF_HIP(hipMemcpyAsync(s->y, ro.y, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
-        PHF_HIP(hipMemcpyAsync(s->w, ro.w, sizeof(double) * ro.m, hipMemcpyDeviceToDevice, L->stream));
+        if (ro.m > 0) {
+            copy3_f64<<<(unsigned)std::min<int64_t>(div_up(ro.m, 256), 4096), 256, 0, L->stream>>>(
+                ro.x, ro.y, ro.w, ro.m, s->x, s->y, s->w);
+            PHF_HIP(hipGetLastError());
+        }
         s->n = ro.m;
         if (sp->pc_enabled) {
             // Blobs.evolve: redistribute, then populationControl (__blobsClass.py:771-781)
@@ -599,6 +618,7 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
         PHF_HIP(hipMemcpyAsync(&hm[0], dm, sizeof(double4), hipMemcpyDeviceToHost, L->stream));
         hm[1] = hm[0];
     }
+    PHF_HIP(hipEventRecord(L->step_ev[2], L->stream));
     PHF_HIP(hipStreamSynchronize(L->stream));
     s->has_velocity = false;
     auto t2 = std::chrono::steady_clock::now();
@@ -607,8 +627,11 @@ int phf_system_step_rk2(phf_system *h, const phf_vel_params *vp, phf_panels *P,
                &inf.m2_before);
     moments_to(hm[1], &inf.circ_after, &inf.impulse_x_after, &inf.impulse_y_after,
                &inf.m2_after);
-    inf.ms_convection = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    inf.ms_remesh = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    float ms_conv = 0.f, ms_rem = 0.f;
+    PHF_HIP(hipEventElapsedTime(&ms_conv, L->step_ev[0], L->step_ev[1]));
+    PHF_HIP(hipEventElapsedTime(&ms_rem, L->step_ev[1], L->step_ev[2]));
+    inf.ms_convection = ms_conv;
+    inf.ms_remesh = ms_rem;
     inf.ms_total = std::chrono::duration<double, std::milli>(t2 - t0).count();
     if (info) *info = inf;
     return PHF_OK;

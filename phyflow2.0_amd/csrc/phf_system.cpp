@@ -171,6 +171,8 @@ int phf_system_destroy(phf_system *h)
         s->lane.pin.release();
         for (auto &ev : s->lane.ev)
             if (ev) (void)hipEventDestroy(ev);
+        for (auto &ev : s->lane.step_ev)
+            if (ev) (void)hipEventDestroy(ev);
         if (s->lane.stream) (void)hipStreamDestroy(s->lane.stream);
     }
     delete s;
