@@ -233,6 +233,7 @@ __global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restri
 // then blobs ascending -- so the sums are the same.  acc starts at -0.0,
 // which the first contribution replaces exactly (-0.0 + v == v for every
 // v); a touched cell has at least one contribution.
+template <bool WEE>
 __device__ __forceinline__ void brm_gather_row4(
     const BatchRm &r, const BatchRmDyn &d, long long i, long long j0, int len, int64_t c0,
     const int2 *__restrict__ crange, const double3 *__restrict__ sb,
@@ -272,10 +273,12 @@ __device__ __forceinline__ void brm_gather_row4(
         for (int a = 0; a < 4; ++a) {
             for (int32_t p = rg[cc][a].x; p < rg[cc][a].y; ++p) {
                 const double3 b = sb[p];
-                const double fx = m4p_dev(ui - b.x, c, c2);
+                const double fx = WEE ? m4p_dev(ui - b.x, c, c2) : m4p_dev0(ui - b.x);
 #pragma unroll
-                for (int t = (3 - cc > 0 ? 3 - cc : 0); t <= (6 - cc < 3 ? 6 - cc : 3); ++t)
-                    acc[t] = acc[t] + (fx * m4p_dev(uj[t] - b.y, c, c2)) * b.z;
+                for (int t = (3 - cc > 0 ? 3 - cc : 0); t <= (6 - cc < 3 ? 6 - cc : 3); ++t) {
+                    const double fy = WEE ? m4p_dev(uj[t] - b.y, c, c2) : m4p_dev0(uj[t] - b.y);
+                    acc[t] = acc[t] + (fx * fy) * b.z;
+                }
             }
         }
     }
@@ -303,6 +306,8 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
 
 // Four consecutive cells of a system's touched range per thread (BRM_BLOCK
 // cells per 64-lane workgroup); with nj >= 4 they span at most two rows.
+// WEE: Wee's diffusion term (c != 0) in the weights.
+template <bool WEE>
 __global__ __launch_bounds__(BRM_BLOCK / 4) void brm_gather_sum(
     const int32_t *__restrict__ bsys, const BatchRm *__restrict__ rm,
     const BatchRmDyn *__restrict__ dyn, const int2 *__restrict__ crange,
@@ -327,7 +332,7 @@ __global__ __launch_bounds__(BRM_BLOCK / 4) void brm_gather_sum(
     // one copy of the row code for both segments
 #pragma nounroll
     for (int seg = 0; seg < 2; ++seg) {
-        brm_gather_row4(r, d, i, j, len, cs, crange, sb, flag, slot, c, c2, xo, yo, wo);
+        brm_gather_row4<WEE>(r, d, i, j, len, cs, crange, sb, flag, slot, c, c2, xo, yo, wo);
         if (cs + len == c0 + 4 || i + 1 >= ni) break;
         cs += len;
         len = (int)(c0 + 4 - cs);
@@ -467,10 +472,15 @@ int batch_remesh_enqueue(Lane *L, const BatchPlan &P, BatchBuffers &Bv, const Ba
             Bf.bsys, Bf.rm, Bf.dyn, Bf.crange, C, Bf.flag);
         PHF_HIP(hipGetLastError());
         PHF_TRY(device_exclusive_scan_ws(L, Bf.flag, C, Bf.slot, Bf.sums));
-        brm_gather_sum<<<(unsigned)(C / BRM_BLOCK), BRM_BLOCK / 4, 0, L->stream>>>(
-            Bf.bsys, Bf.rm, Bf.dyn, Bf.crange, Bf.sb, Bf.flag, Bf.slot, C, R.p.c,
-            R.p.c * R.p.c,
-            Bf.xo, Bf.yo, Bf.wo);
+        const unsigned G = (unsigned)(C / BRM_BLOCK);
+        if (R.p.c != 0.0)
+            brm_gather_sum<true><<<G, BRM_BLOCK / 4, 0, L->stream>>>(
+                Bf.bsys, Bf.rm, Bf.dyn, Bf.crange, Bf.sb, Bf.flag, Bf.slot, C, R.p.c,
+                R.p.c * R.p.c, Bf.xo, Bf.yo, Bf.wo);
+        else
+            brm_gather_sum<false><<<G, BRM_BLOCK / 4, 0, L->stream>>>(
+                Bf.bsys, Bf.rm, Bf.dyn, Bf.crange, Bf.sb, Bf.flag, Bf.slot, C, 0.0, 0.0, Bf.xo,
+                Bf.yo, Bf.wo);
     }
     brm_counts<<<div_up(S, 64), 64, 0, L->stream>>>(Bf.rm, Bf.dyn, S, Bf.flag, Bf.slot, C, Bf.mo);
     PHF_HIP(hipGetLastError());

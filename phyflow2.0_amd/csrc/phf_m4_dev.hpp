@@ -42,6 +42,31 @@ __host__ __device__ __forceinline__ double m4p_dev(double x, double c, double c2
     return 0.0;
 }
 
+// m4p_dev with c = 0 (plain M4', no diffusion term): the c2 products are
+// then +-0, and subtracting +-0 leaves the first part unchanged, since that
+// part is never -0 (it is < 0 on 1 < |x| < 2, and on |x| < 1 a sum that
+// cancels exactly rounds to +0).  Same values as m4p_dev(x, 0, 0), with
+// about a third fewer operations.
+__host__ __device__ __forceinline__ double m4p_dev0(double x)
+{
+#pragma clang fp contract(off)
+    const double ax = fabs(x);
+    if ((ax > 1.0) && (ax < 2.0)) {
+        const double t = 2.0 - ax;
+        const double t2 = t * t;
+        return 0.5 * t2 * (1.0 - ax);
+    } else if ((ax >= 0.0) && (ax < 1.0)) {
+#ifdef __HIP_DEVICE_COMPILE__
+        const double a3 = cube_rn(ax);
+#else
+        const double a3 = std::pow(ax, 3.0);
+#endif
+        const double x2 = x * x;
+        return 1.0 - 2.5 * x2 + 1.5 * a3;
+    }
+    return 0.0;
+}
+
 // iL = int64(ceil(((x - x0) - 2h + h/2)/h)) - 1  (_regridding.py:157-158)
 __device__ __forceinline__ int64_t base_index(double x, double x0, double h)
 {

@@ -125,13 +125,15 @@ def test_batch_step_remesh_matches_single(cell_factor, O):
         assert total == info["remeshed"]
 
 
-def test_batch_step_remesh_thin_and_crowded_systems():
+@pytest.mark.parametrize("c", [0.0, 0.3])
+def test_batch_step_remesh_thin_and_crowded_systems(c):
     """Lattice shapes the batched gather walks in rows of four cells: rows
     of 4..11 cells (every remainder of the row length mod 4, so a thread's
     cells wrap into the next row), a one-cell-high line of blobs, a
     one-cell-wide column, and a cell holding 40 blobs.  Each system is
     remeshed by the batch (none spills) and is bit-identical to the
-    single-system Regrid of the same convected positions."""
+    single-system Regrid of the same convected positions, with plain M4'
+    weights (c = 0) and with Wee's diffusion term (c = 0.3)."""
     rng = np.random.default_rng(12)
     sigma, overlap = 0.01, 0.8
     h = sigma * overlap
@@ -156,13 +158,13 @@ def test_batch_step_remesh_thin_and_crowded_systems():
     with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C) as b:
         for i, s in enumerate(systems):
             b.set_system(i, s["x"], s["y"], s["w"])
-        b.set_remesh(overlap, dt=dt, vInf=vinf)
+        b.set_remesh(overlap, dt=dt, vInf=vinf, c=c)
         info = b.step(use_graph=False)
         assert info["flags"] == 0 and info["spilled"] == 0
         for i, s in enumerate(systems):
             v = b.get_velocity(i)
             got = b.get_remeshed(i)
-            ref, _ = _single_step(s, v, dt, vinf, overlap)
+            ref, _ = _single_step(s, v, dt, vinf, overlap, c=c)
             for a, r in zip(got, ref[:3]):
                 assert np.array_equal(a, r), i
 
