@@ -928,7 +928,7 @@ def main():
     if variants:
         out["variants"] = variants
     if not args.no_mix:
-        out["config4_mix"] = mix_leg_child()
+        out["mix_256_systems"] = mix_leg_child()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)

@@ -1,21 +1,20 @@
-"""The benchmark configurations at their stated sizes, against the CPU
-oracle (not against another GPU path).
+"""The benchmark workloads at their full sizes, against the CPU oracle (not
+against another GPU path).
 
-  config 2  1,048,576 uniform blobs (seed 0, h = 1/1024, sigma = h/0.8),
-            FMM at orders 8, 16 and 24 with leaf 64, float64 and F32C:
-            velocity at a seeded 4,096-target subset (seed 1) against the
-            oracle's float64 direct Oseen sum (direct.cpp:1409-1485).
-  config 3  512-panel cylinder + 262,144 annulus blobs: one coupled RK2
-            step (F32C FMM order 16) against the CPU restatement of the step
-            (oracle blob and panel velocities, float64 least squares) at a
-            4,096-blob subset, and the float64 invariants of the step with
-            the M4' remesh.
-  config 4  256 uneven systems batched (F32C, one graph): eight of them
-            against the oracle direct sum, remesh bit-exact against the
-            oracle remesh of the same convected positions.
-  config 5  one served request of each accuracy class (F32C order 16;
-            float64 order 30, ws 2) on a 10,000-blob simulation: RK2 + M4'
-            remesh + population control against the CPU restatement.
+  1,048,576 uniform blobs (seed 0, h = 1/1024, sigma = h/0.8): FMM at
+      orders 8, 16 and 24 with leaf 64, float64 and F32C: velocity at a
+      seeded 4,096-target subset (seed 1) against the oracle's float64
+      direct Oseen sum (direct.cpp:1409-1485).
+  512-panel cylinder + 262,144 annulus blobs: one coupled RK2 step (F32C
+      FMM order 16) against the CPU restatement of the step (oracle blob and
+      panel velocities, float64 least squares) at a 4,096-blob subset, and
+      the float64 invariants of the step with the M4' remesh.
+  256 uneven systems batched (F32C, one graph): eight of them against the
+      oracle direct sum, remesh bit-exact against the oracle remesh of the
+      same convected positions.
+  one served request of each accuracy class (F32C order 16; float64 order
+      30, ws 2) on a 10,000-blob simulation: RK2 + M4' remesh + population
+      control against the CPU restatement.
 
 Tolerances: the mixed path 1e-6 relative L2 at order >= 16; float64 per
 order (the truncation error of the expansion: 1e-4 at p = 8, 1e-7 at
@@ -41,9 +40,9 @@ def _velocity_oracle(O, s, xt, yt):
     return -qi / TWO_PI, -qr / TWO_PI
 
 
-# ------------------------------------------------------------- config 2 ---
+# ------------------------------------------------- 1,048,576 uniform blobs ---
 @pytest.fixture(scope="module")
-def config2(O):
+def fmm_1m(O):
     s = W.uniform_system(1048576, seed=0, h=1.0 / 1024)
     sub = W.target_subset(s["x"].size, 4096, seed=0)  # drawn from stream seed + 1
     ref = _velocity_oracle(O, s, s["x"][sub], s["y"][sub])
@@ -59,16 +58,16 @@ def config2(O):
     (PREC_F64, 16, 1e-7, 1e-6),
     (PREC_F64, 24, 1e-10, 1e-9),
 ])
-def test_config2_fmm_1m_blobs_vs_oracle(config2, O, prec, order, l2, mx):
-    s, sub, ref, sysm = config2
+def test_fmm_1m_uniform_blobs_vs_oracle(fmm_1m, O, prec, order, l2, mx):
+    s, sub, ref, sysm = fmm_1m
     vx, vy = sysm.velocity(vel_params(s["sigma"], prec=prec, order=order, leaf=64))
     got = (vx[sub], vy[sub])
     e2, em = O.rel_l2(got, ref), O.max_rel(got, ref)
-    print("config2 prec=%d p=%d rel_l2=%.3e max=%.3e" % (prec, order, e2, em), flush=True)
+    print("fmm_1m prec=%d p=%d rel_l2=%.3e max=%.3e" % (prec, order, e2, em), flush=True)
     assert e2 <= l2 and em <= mx
 
 
-# ------------------------------------------------------------- config 3 ---
+# -------------------------------------- cylinder, 512 panels + 262,144 blobs ---
 def _panel_strengths_cpu(O, a, g, sigma, vinf):
     """'prescribed' solve (__panelsClass.py:800-822): least squares of
     [A; L] s = [-(v . t); 0] with the slip velocity of blobs + free stream
@@ -87,7 +86,7 @@ def _stage_cpu(O, src, sigma, g, sp, vinf, xt, yt):
     return vx + px + vinf[0], vy + py + vinf[1]
 
 
-def test_config3_cylinder_step_full_size(O):
+def test_cylinder_512_panels_262k_blobs_step(O):
     sigma, dt, vinf = 0.004, 1e-3, (1.0, 0.0)
     xp, yp = W.cylinder_panels(512, R=0.5)
     g = W.panel_geometry(xp, yp)
@@ -108,7 +107,7 @@ def test_config3_cylinder_step_full_size(O):
     ref = (xr - a["x"][sub] - vinf[0] * dt, yr - a["y"][sub] - vinf[1] * dt)
     got = (xg[sub] - a["x"][sub] - vinf[0] * dt, yg[sub] - a["y"][sub] - vinf[1] * dt)
     e = O.rel_l2(got, ref)
-    print("config3 displacement rel_l2=%.3e" % e, flush=True)
+    print("cylinder step displacement rel_l2=%.3e" % e, flush=True)
     assert e <= 1e-6
     # the same step with the M4' remesh onto h = 0.005: float64 invariants
     with BlobSystem(a["x"], a["y"], a["w"]) as s:
@@ -120,8 +119,8 @@ def test_config3_cylinder_step_full_size(O):
     body.close()
 
 
-# ------------------------------------------------------------- config 4 ---
-def test_config4_mix_256_systems(O):
+# ---------------------------------------------------- 256 uneven systems ---
+def test_batch_256_uneven_systems(O):
     from phyflow_amd.batch import BatchSystems
     sizes = W.uneven_sizes(256, 1024, 262144, seed=0)
     systems = [W.uniform_system(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
@@ -147,7 +146,7 @@ def test_config4_mix_256_systems(O):
             assert O.rel_l2(wr, o[2]) <= 1e-15, i
 
 
-# ------------------------------------------------------------- config 5 ---
+# ------------------------------------------------------- served requests ---
 def _served_step_cpu(O, s, dt, overlap, pc):
     k1 = _velocity_oracle(O, s, s["x"], s["y"])
     moved = dict(x=s["x"] + dt * k1[0], y=s["y"] + dt * k1[1], w=s["w"], sigma=s["sigma"])
@@ -161,7 +160,7 @@ def _served_step_cpu(O, s, dt, overlap, pc):
 
 
 @pytest.mark.parametrize("cls", ["f32c_p16", "f64_p30_ws2"])
-def test_config5_served_request_vs_cpu(O, cls):
+def test_served_request_vs_cpu(O, cls):
     from phyflow_amd.serve import Server
     s = W.uniform_system(10000, seed=2000)
     if cls == "f32c_p16":
@@ -187,6 +186,6 @@ def test_config5_served_request_vs_cpu(O, cls):
     a[np.searchsorted(allk, kg)] = wg
     b[np.searchsorted(allk, kr)] = wr
     e = O.rel_l2(a, b)
-    print("config5 %s nodes %d/%d rel_l2(w)=%.3e" % (cls, xg.size, xr.size, e), flush=True)
+    print("served %s nodes %d/%d rel_l2(w)=%.3e" % (cls, xg.size, xr.size, e), flush=True)
     assert e <= tol
     assert abs(xg.size - xr.size) <= max(2, xr.size // 1000)
