@@ -1377,6 +1377,11 @@ static int spd_inverse(Lane *L, DenseFactors *f)
 struct RefineState {
     double last_dx;
     int32_t done, iterations, converged, pad;
+    // one sweep's max |d| and max |x| as the bit patterns of non-negative
+    // doubles (their unsigned order is the numeric order, NaN above
+    // infinity), and the workgroups of the sweep that have finished
+    unsigned long long md_bits, mx_bits;
+    uint32_t finished, pad2;
 };
 
 __device__ __forceinline__ double wave_sum_f64(double v)
@@ -1396,6 +1401,9 @@ __global__ void refine_init(double *__restrict__ x, int64_t n, int warm,
         st->done = 0;
         st->iterations = 0;
         st->converged = 0;
+        st->md_bits = 0ull;
+        st->mx_bits = 0ull;
+        st->finished = 0u;
     }
 }
 
@@ -1434,70 +1442,76 @@ __global__ __launch_bounds__(256) void tapply_cols(const double *__restrict__ Bc
     if (lane == 0) g[j] = s;
 }
 
-// d = X v  (X float32 row-major n x n, products and sums in float64)
-__global__ __launch_bounds__(256) void xapply_rows(const float *__restrict__ X, int64_t n,
-                                                   const double *__restrict__ v,
-                                                   double *__restrict__ d,
-                                                   const RefineState *__restrict__ st)
-{
-    if (st->done) return;
-    const int lane = threadIdx.x & 63;
-    const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
-    const float *row = X + i * n;
-    double s = 0.0;
-    for (int64_t j = lane; j < n; j += 64) s = __builtin_fma((double)row[j], v[j], s);
-    s = wave_sum_f64(s);
-    if (lane == 0) d[i] = s;
-}
-
-// x += d, then the stopping rule: the correction has reached float64
-// resolution (|d| <= 4 eps |x|), or it stopped shrinking (after the third
-// sweep, |d| >= |d_prev| / 2: converged if |d| <= 1e-10 |x|), or a
-// non-finite value, or max_it sweeps.  One block.
-__global__ __launch_bounds__(256) void refine_update(double *__restrict__ x,
-                                                     const double *__restrict__ d, int64_t n,
+// One sweep's correction and update: d = X v (X float32 row-major n x n,
+// products and sums in float64, one wave per row), x += d, then the
+// stopping rule: the correction has reached float64 resolution (|d| <= 4
+// eps |x|), or it stopped shrinking (after the third sweep, |d| >=
+// |d_prev| / 2: converged if |d| <= 1e-10 |x|), or a non-finite value, or
+// max_it sweeps.  Each workgroup folds its max |d| and max |x| into the
+// state; the last one to finish applies the rule and clears the folds for
+// the next sweep (no workgroup waits for another).
+__global__ __launch_bounds__(256) void xapply_update(const float *__restrict__ X, int64_t n,
+                                                     const double *__restrict__ v,
+                                                     double *__restrict__ x,
                                                      RefineState *__restrict__ st, int max_it)
 {
     if (st->done) return;
-    __shared__ double sd[256], sx[256];
-    double md = 0.0, mx = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) {
-        const double v = x[i] + d[i];
-        x[i] = v;
-        md = fmax(md, fabs(d[i]));
-        mx = fmax(mx, fabs(v));
+    __shared__ unsigned long long smd[4], smx[4];
+    __shared__ int last;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * 4 + wv;
+    unsigned long long bd = 0ull, bx = 0ull;
+    if (i < n) {
+        const float *row = X + i * n;
+        double s = 0.0;
+        for (int64_t j = lane; j < n; j += 64) s = __builtin_fma((double)row[j], v[j], s);
+        s = wave_sum_f64(s);
+        const double xn = x[i] + s;
+        if (lane == 0) x[i] = xn;
+        bd = (unsigned long long)__double_as_longlong(fabs(s));
+        bx = (unsigned long long)__double_as_longlong(fabs(xn));
     }
-    sd[threadIdx.x] = md;
-    sx[threadIdx.x] = mx;
+    if (lane == 0) {
+        smd[wv] = bd;
+        smx[wv] = bx;
+    }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            sd[threadIdx.x] = fmax(sd[threadIdx.x], sd[threadIdx.x + s]);
-            sx[threadIdx.x] = fmax(sx[threadIdx.x], sx[threadIdx.x + s]);
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        const double dmax = sd[0], xmax = sx[0];
-        const int it = st->iterations;
-        const double eps = 2.220446049250313e-16;
-        int stop = 0, conv = 0;
-        if (!isfinite(dmax) || !isfinite(xmax)) {
-            stop = 1;
-        } else if (dmax <= 4.0 * eps * xmax) {
-            stop = conv = 1;
-        } else if (it > 2 && dmax >= 0.5 * st->last_dx) {
-            stop = 1;
-            conv = dmax <= 1e-10 * xmax;
+        unsigned long long md = smd[0], mx = smx[0];
+        for (int k = 1; k < 4; ++k) {
+            md = smd[k] > md ? smd[k] : md;
+            mx = smx[k] > mx ? smx[k] : mx;
         }
-        st->iterations = it + 1;
-        st->last_dx = dmax;
-        if (stop || it + 1 >= max_it) {
-            st->done = 1;
-            st->converged = conv;
-        }
+        atomicMax(&st->md_bits, md);
+        atomicMax(&st->mx_bits, mx);
+        __threadfence();
+        last = atomicAdd(&st->finished, 1u) == gridDim.x - 1u;
     }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    __threadfence();
+    const double dmax = __longlong_as_double((long long)atomicMax(&st->md_bits, 0ull));
+    const double xmax = __longlong_as_double((long long)atomicMax(&st->mx_bits, 0ull));
+    const int it = st->iterations;
+    const double eps = 2.220446049250313e-16;
+    int stop = 0, conv = 0;
+    if (!isfinite(dmax) || !isfinite(xmax)) {
+        stop = 1;
+    } else if (dmax <= 4.0 * eps * xmax) {
+        stop = conv = 1;
+    } else if (it > 2 && dmax >= 0.5 * st->last_dx) {
+        stop = 1;
+        conv = dmax <= 1e-10 * xmax;
+    }
+    st->iterations = it + 1;
+    st->last_dx = dmax;
+    if (stop || it + 1 >= max_it) {
+        st->done = 1;
+        st->converged = conv;
+    }
+    st->md_bits = 0ull;
+    st->mx_bits = 0ull;
+    st->finished = 0u;
 }
 
 // ------------------------------------------------------------- host ------
@@ -1716,11 +1730,10 @@ int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double 
 {
     const int64_t rows = f.rows, n = f.n;
     const bool normal = rows > n;
-    double *r, *g, *d;
+    double *r, *g;
     RefineState *dst;
     PHF_ALLOC(r, L->arena, double, (size_t)rows);
     PHF_ALLOC(g, L->arena, double, (size_t)n);
-    PHF_ALLOC(d, L->arena, double, (size_t)n);
     PHF_ALLOC(dst, L->arena, RefineState, 1);
     refine_init<<<div_up(std::max<int64_t>(n, 1), 256), 256, 0, L->stream>>>(x, n, warm ? 1 : 0,
                                                                           dst);
@@ -1735,8 +1748,7 @@ int dense_refine_device(Lane *L, const DenseFactors &f, const double *c, double 
                 tapply_cols<<<div_up(n, 4), 256, 0, L->stream>>>(f.Bc, rows, n, r, g, dst);
                 rhs = g;
             }
-            xapply_rows<<<div_up(n, 4), 256, 0, L->stream>>>(f.Xinv, n, rhs, d, dst);
-            refine_update<<<1, 256, 0, L->stream>>>(x, d, n, dst, max_it);
+            xapply_update<<<div_up(n, 4), 256, 0, L->stream>>>(f.Xinv, n, rhs, x, dst, max_it);
         }
         PHF_HIP(hipGetLastError());
         PHF_HIP(hipMemcpyAsync(&hs, dst, sizeof(hs), hipMemcpyDeviceToHost, L->stream));
