@@ -169,6 +169,44 @@ def test_batch_step_remesh_thin_and_crowded_systems(c):
                 assert np.array_equal(a, r), i
 
 
+def test_many_tiny_systems_velocity_and_remesh(O):
+    """120 systems of 0..90 blobs, so most waves of the batch's point kernels
+    straddle systems and take the per-lane system search, and the leaf and
+    box kernels mix systems of different depths: every system's velocity
+    agrees with the float64 direct sum (<= 1e-6) and its remesh is
+    bit-identical to the single-system Regrid of the same convected
+    positions."""
+    rng = np.random.default_rng(21)
+    sizes = rng.integers(0, 91, 120)
+    sizes[[0, 7, 63]] = 0
+    systems = []
+    for i, n in enumerate(sizes):
+        sigma = 0.002 * (1.0 + rng.random())
+        x, y = rng.random((2, n)) * 0.05
+        systems.append(dict(x=x, y=y, w=rng.standard_normal(n) * 1e-4, sigma=sigma))
+    # a coarse lattice (h = 4 sigma) keeps each system's touched cells
+    # within its reserve, so the batch remeshes every system itself
+    dt, vinf, overlap = 1e-3, (0.1, 0.0), 4.0
+    with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C, order=16,
+                      leaf=64) as b:
+        for i, s in enumerate(systems):
+            b.set_system(i, s["x"], s["y"], s["w"])
+        b.set_remesh(overlap, dt=dt, vInf=vinf)
+        info = b.step(use_graph=False)
+        assert info["flags"] == 0 and info["spilled"] == 0
+        for i, s in enumerate(systems):
+            v = b.get_velocity(i)
+            if s["x"].size:
+                xopt, cutoff = W.oseen_params(s["sigma"])
+                qr, qi = O.direct_oseen(s["x"], s["y"], s["w"], s["x"], s["y"], xopt, cutoff)
+                d = (-qi / (2 * math.pi), -qr / (2 * math.pi))
+                assert O.rel_l2(v, d) <= 1e-6, i
+            got = b.get_remeshed(i)
+            ref, _ = _single_step(s, v, dt, vinf, overlap)
+            for a, r in zip(got, ref[:3]):
+                assert np.array_equal(a, r), i
+
+
 def test_batch_step_graph_replays():
     sizes, systems = _mix(6, seed=11)
     with BatchSystems(sizes, [s["sigma"] for s in systems], prec=PREC_F32C) as b:
