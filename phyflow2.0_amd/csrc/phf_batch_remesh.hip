@@ -193,6 +193,16 @@ __global__ void brm_gather_scaled(const double4 *__restrict__ pk, const int32_t 
     out[k] = make_double3(q.x, q.y, q.z);
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD
+// b mod 8), each with its own L2.  This maps them so that XCD x takes a
+// contiguous run of cell blocks: neighbouring rows of cells, which read the
+// same blobs, then meet in the same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
+{
+    const unsigned q = G / 8u, r = G % 8u, x = b % 8u, k = b / 8u;
+    return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + k;
+}
+
 // touched = one of the 16 base cells (i-a, j-b), a, b in 0..3, holds a blob.
 // Cell kernels run BRM_BLOCK threads per workgroup; every system's cell range
 // is a whole number of workgroups, whose system bsys names.
@@ -204,9 +214,10 @@ __global__ __launch_bounds__(BRM_BLOCK) void brm_touched(const int32_t *__restri
                                                          const int2 *__restrict__ crange,
                                                          int64_t C, uint32_t *__restrict__ flag)
 {
-    const int64_t c = (int64_t)blockIdx.x * BRM_BLOCK + threadIdx.x;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t c = (int64_t)blk * BRM_BLOCK + threadIdx.x;
     if (c >= C) return;
-    const int s = bsys[blockIdx.x];
+    const int s = bsys[blk];
     const BatchRmDyn d = dyn[s];
     const int64_t base = rm[s].cbase, loc = c - base;
     const long long nj = d.bj + 3;
@@ -294,15 +305,6 @@ __device__ __forceinline__ void brm_gather_row4(
     }
 }
 
-// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD
-// b mod 8), each with its own L2.  This maps them so that XCD x takes a
-// contiguous run of cell blocks: neighbouring rows of cells, which read the
-// same blobs, then meet in the same L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
-{
-    const unsigned q = G / 8u, r = G % 8u, x = b % 8u, k = b / 8u;
-    return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + k;
-}
 
 // Four consecutive cells of a system's touched range per thread (BRM_BLOCK
 // cells per 64-lane workgroup); with nj >= 4 they span at most two rows.
