@@ -263,13 +263,27 @@ __global__ __launch_bounds__(64) void batch_p2m(const BatchSys *__restrict__ sys
                 q = cmul(q, z);
             }
             __syncthreads();
+            // term k summed by 64 / W lanes (k, k + W, ...), each over every
+            // (64 / W)-th point, then combined by shuffles: a chain of
+            // cnt W / 64 adds per lane instead of cnt
             const int cnt = min(64, s1 - c0);
-            if (lane < p1)
-                for (int jj = 0; jj < cnt; ++jj) {
-                    const V t = tab[jj * p1 + lane];
-                    acc.x += t.x;
-                    acc.y += t.y;
+            const int W = p1 <= 16 ? 16 : (p1 <= 32 ? 32 : 64);
+            const int k = lane & (W - 1), R = 64 / W;
+            V part = cmk<T>(0, 0);
+            if (k < p1)
+                for (int jj = lane / W; jj < cnt; jj += R) {
+                    const V t = tab[jj * p1 + k];
+                    part.x += t.x;
+                    part.y += t.y;
                 }
+            for (int o = W; o < 64; o <<= 1) {
+                part.x += __shfl_xor(part.x, o);
+                part.y += __shfl_xor(part.y, o);
+            }
+            if (lane < p1) {
+                acc.x += part.x;
+                acc.y += part.y;
+            }
             __syncthreads();
         }
     }
